@@ -1,0 +1,201 @@
+"""Force-evaluation benchmark of the ionic rock-salt workload on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 64 000 ions, 40^3 rock-salt
+lattice, a = 2.82 A, charges +1/-1, Gaussian jitter 0.1 A; LJ eps = 0.01 eV, sigma = 2.8 A,
+cutoff 9 A; smooth PME with C = 14.399645 eV A/e^2, real cutoff 9 A, erfc(alpha rc) = 1e-5,
+mesh spacing 1.0 A (120^3), B-spline order 5; FP32 arithmetic, FP64 energy / virial sums.
+
+One step = one force evaluation with energy and full virial: list coverage check (with the
+predicated rebuild), LJ + real-space Coulomb pair kernel, spread, FFT + influence + inverse
+FFT, gather. Inputs are resident in HBM before timing starts. Prints ONE JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "particledynamics.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+DT_FS = 1.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-error", action="store_true", help="skip the FP64 Ewald force-error measurement")
+    p.add_argument("--lanes", type=int, default=0)
+    p.add_argument("--skin", type=float, default=1.0)
+    return p.parse_args()
+
+
+def pair_flops_per_visit(npe, npf, energy=True):
+    """FP32 operations the pair kernel spends per visited row entry (each pair is visited
+    from both ends): periodic differences and r^2 (8), LJ force (9), Coulomb force with the
+    force polynomial (8 + 2 npf), force accumulation (6); with energy: LJ energy (4), Coulomb
+    energy with its polynomial (4 + 2 npe), energy sum (1), six virial products (15)."""
+    f = 8 + 9 + 8 + 2 * npf + 6
+    if energy:
+        f += 4 + 4 + 2 * npe + 1 + 15
+    return f
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import pdx
+    from pdx import systems
+
+    torch.cuda.set_device(0)
+    sysd = systems.rock_salt(40, 2.82, 0.1, seed=0)
+    N, box = sysd["N"], sysd["box"]
+    C = 14.399645
+    spec = pdx.PME(coulomb_prefactor=C, real_cutoff=9.0, real_space_tolerance=1e-5, mesh_spacing=1.0, order=5)
+    lj = pdx.LennardJones(epsilon=0.01, sigma=2.8, cutoff=9.0)
+    eng = pdx.ForceEngine(N, box, lj, electrostatics=spec, charges=sysd["q"], skin=args.skin, lanes=args.lanes)
+    X, Y, Z, Q = (torch.from_numpy(sysd[k]).cuda() for k in ("x", "y", "z", "q"))
+    fx, fy, fz, E = (torch.empty(N, device="cuda") for _ in range(4))
+    V = torch.empty(6, N, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def step():
+        eng.evaluate(X, Y, Z, Q, fx, fy, fz, E, V, energy=True)
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    st = eng.nl.status()
+
+    # ---- timed region: exactly K steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = 1e3 * dt / args.steps
+    evals = args.steps / dt
+
+    # ---- per-kernel durations with events on the launch stream (same work, split by phase)
+    ev = {k: [] for k in ("update", "pairs", "spread", "solve", "gather")}
+    nl, pf, sol = eng.nl, eng.pairs, eng.solver
+    order = nl._arrays.particle_ids_sorted
+    reps = max(10, min(args.steps, 50))
+    for _ in range(reps):
+        marks = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        marks[0].record(stream)
+        nl.update(X, Y, Z, Q)
+        marks[1].record(stream)
+        pf.compute(nl, fx, fy, fz, E, V, eng.pair_totals, energy=True)
+        marks[2].record(stream)
+        sol.spread(X, Y, Z, Q, order=order)
+        marks[3].record(stream)
+        sol.solve(energy=True)
+        marks[4].record(stream)
+        sol.mesh_gather(X, Y, Z, Q, fx, fy, fz, E, eng.Vglobal, energy=True, order=order)
+        marks[5].record(stream)
+        torch.cuda.synchronize()
+        for k, (a, b) in zip(ev, zip(marks[:-1], marks[1:])):
+            ev[k].append(a.elapsed_time(b))
+    phase_ms = {k: float(np.mean(v)) for k, v in ev.items()}
+
+    # forced rebuild cost (decide + full rebuild)
+    rb = []
+    for _ in range(10):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        nl.rebuild(X, Y, Z, Q)
+        b.record(stream)
+        torch.cuda.synchronize()
+        rb.append(a.elapsed_time(b))
+    rebuild_ms = float(np.median(rb))
+    interval = st.interval
+    md_ms = ms + rebuild_ms / max(interval, 1)
+    ns_day = (1e3 / md_ms) * DT_FS * 86400 / 1e6
+
+    # ---- roofline of the dominant kernel (the pair kernel: FP32 VALU)
+    counts = nl.counts().to(torch.int64)
+    visits = int(counts.sum().item())
+    info = pf.info()
+    fpv = pair_flops_per_visit(info["n_energy_terms"], info["n_force_terms"], True)
+    pair_flop = visits * fpv
+    achieved = pair_flop / (phase_ms["pairs"] * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "k_pair<LJ,coulomb,energy>", "note": "FP32 VALU peak; flops counted per row visit",
+                "flop_per_launch": pair_flop, "row_visits": visits, "flop_per_visit": fpv}
+
+    tot = eng.totals()
+    out = {"metric": "force evals/s + ns/day at matched force err: learned-corrected vs plain PME",
+           "value": round(evals, 2), "unit": "force evals/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "rock-salt 64000 ions, LJ + SPME 120^3 P=5, rc 9 A, tol 1e-5, plain PME",
+                      "N": N, "mesh": list(sol.mesh), "order": sol.order, "alpha": sol.alpha, "skin": args.skin,
+                      "cap": nl.cap, "parallelism": "single GPU"},
+           "ns_per_day": round(ns_day, 2), "ns_per_day_note": f"dt=1 fs, one rebuild every {interval} steps",
+           "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()}, "rebuild_ms": round(rebuild_ms, 4),
+           "energy_eV": tot["energy"], "roofline": roofline}
+
+    if not args.no_error:
+        # Coulomb force error of plain PME against FP64 Ewald on 256 sampled ions
+        e2 = pdx.ForceEngine(N, box, None, electrostatics=spec, charges=sysd["q"], skin=args.skin)
+        gx, gy, gz = (torch.empty(N, device="cuda") for _ in range(3))
+        e2.evaluate(X, Y, Z, Q, gx, gy, gz, None, None, energy=False)
+        rng = np.random.default_rng(1)
+        ids = np.sort(rng.choice(N, 256, replace=False)).astype(np.int32)
+        xs = [t.double() for t in (X, Y, Z, Q)]
+        Fref, _ = pdx.ewald_fp64(*xs, box, C, 0.25, digits=12.0, sample=torch.from_numpy(ids).cuda(),
+                                 want_energy=False)
+        Fg = torch.stack([gx, gy, gz], 1).double()[torch.from_numpy(ids).long().cuda()]
+        err = math.sqrt(((Fg - Fref) ** 2).sum().item() / (Fref ** 2).sum().item())
+        out["force_rel_rms_vs_ewald"] = err
+        out["force_error_sample"] = "Coulomb forces of 256 ions vs FP64 Ewald (alpha 0.25, 12 digits)"
+
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(sysd, C)
+    print(json.dumps(out))
+
+
+def cpu_baseline(sysd, C):
+    """The FP64 CPU oracle (Ewald + LJ direct sums) on a bounded sample of ions, extrapolated
+    linearly in the sample size to one full force evaluation (t(S) = a + b S)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import oracle
+    x, y, z, q = (sysd[k].astype(np.float64) for k in ("x", "y", "z", "q"))
+    box = sysd["box"]
+    N = len(x)
+    ts = {}
+    rng = np.random.default_rng(2)
+    for S in (16, 48):
+        ids = np.sort(rng.choice(N, S, replace=False))
+        t0 = time.perf_counter()
+        oracle.pair_direct(x, y, z, box, kind="lj", sigma=2.8, epsilon=0.01, cutoff=9.0, ids=ids)
+        oracle.ewald_sampled(x, y, z, q, box, C_=C, alpha=0.15, digits=12.0, ids=ids)
+        ts[S] = time.perf_counter() - t0
+    b = (ts[48] - ts[16]) / 32
+    a = ts[16] - 16 * b
+    t_full = a + b * N
+    return {"value": round(1.0 / t_full, 6), "unit": "force evals/s", "cores": oracle.num_threads(),
+            "kind": "port",
+            "sample": f"FP64 Ewald (alpha 0.15, 12 digits) + LJ direct sums on 16 and 48 of {N} ions "
+                      f"({ts[16]:.1f} s, {ts[48]:.1f} s), extrapolated to all ions: {t_full:.1f} s per evaluation"}
+
+
+if __name__ == "__main__":
+    main()

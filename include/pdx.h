@@ -1,0 +1,262 @@
+/*
+ * pdx.h — C ABI of libpdx, the MI355X (gfx950) force hot path of ParticleDynamics.jl.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and returns an
+ * int status (PDX_OK = 0). Device arrays are raw device pointers owned by the caller
+ * ("It holds no references to particle arrays", src/electrostatics/Solver.jl:67-74);
+ * the library owns only its workspaces, allocated at *_create and never on a warm call.
+ * Enqueue functions never synchronise; pdx_nl_status / pdx_*_totals are the only
+ * device-to-host reads (src/neighbors/DeviceNeighborCUDA.jl:639-651).
+ *
+ * Layouts follow the reference bit for bit where other consumers read them:
+ *   - ELL rows slot-major, 1-based ids: slot t of particle i at neighbors_flat[t*N + i]
+ *     (src/neighbors/NeighborTypes.jl:28-31);
+ *   - cell_offsets 1-based, last entry N+1; particle_ids_sorted 1-based
+ *     (src/neighbors/NeighborAllocation.jl:133-150);
+ *   - per-particle virial V is N x 6 column-major (xx, yy, zz, xy, xz, yz)
+ *     (src/nonbonded/LJELLEngineKernels.jl:75-77);
+ *   - mesh grids column-major, x fastest (src/electrostatics/MeshKernels.jl:4-5),
+ *     half spectrum (K1/2+1, K2, K3) (src/electrostatics/Solver.jl:48-49).
+ * Coordinates are centred, in [-L/2, L/2) (src/geometry/PeriodicDisplacement.jl:21-25).
+ */
+#ifndef PDX_H
+#define PDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (host-side mirrors of the reference's exceptions) ---- */
+#define PDX_OK              0
+#define PDX_ERR_ARGUMENT    1  /* ArgumentError (e.g. NeighborAllocation.jl:5-26, Spec.jl:60-101) */
+#define PDX_ERR_DIMENSION   2  /* DimensionMismatch */
+#define PDX_ERR_CAPACITY    3  /* NeighborCapacityError (NeighborTypes.jl:41-50) */
+#define PDX_ERR_COORDINATE  4  /* non-finite / uncentred coordinates (DeviceNeighborCUDA.jl:641) */
+#define PDX_ERR_DEVICE      5  /* HIP runtime failure */
+#define PDX_ERR_STATE       6  /* no valid list (DeviceNeighborCUDA.jl:645) */
+
+/* ---- device state words of a neighbor list, 0-based (DN_* of DeviceNeighborCUDA.jl:4-33, minus 1) ---- */
+#define PDX_DN_REQUEST             0
+#define PDX_DN_VALID               1
+#define PDX_DN_ERROR               2
+#define PDX_DN_STEP                3
+#define PDX_DN_LAST_BUILD          4
+#define PDX_DN_INTERVAL            5
+#define PDX_DN_CHECKS              6
+#define PDX_DN_BUILDS              7
+#define PDX_DN_REQUIRED            8
+#define PDX_DN_REASON              9
+#define PDX_DN_ATTEMPTS           10
+#define PDX_DN_DISPLACEMENT_BUILDS 11
+#define PDX_DN_AGE_BUILDS         12
+#define PDX_DN_INVALID_BUILDS     13
+#define PDX_DN_SORT_BUILDS        14
+#define PDX_DN_OVERLAP_BUILDS     15
+#define PDX_DN_LAST_CANDIDATES    16
+#define PDX_DN_TOTAL_CANDIDATES   17
+#define PDX_DN_LAST_ACCEPTED      18
+#define PDX_DN_TOTAL_ACCEPTED     19
+#define PDX_DN_MAX_ROW            20
+#define PDX_DN_DIAGNOSTICS        21
+#define PDX_DN_SECTION            22
+#define PDX_DN_NSTATE             23
+
+#define PDX_DN_REASON_INVALID      1
+#define PDX_DN_REASON_DISPLACEMENT 2
+#define PDX_DN_REASON_AGE          4
+#define PDX_DN_CAPACITY_ERROR      1
+#define PDX_DN_COORDINATE_ERROR    2
+
+/* ---- neighbor-list flags ---- */
+#define PDX_NL_ALLPAIRS     1  /* rows from all pairs, one image (DenseNeighborCUDA.jl:128-151) */
+#define PDX_NL_DIAGNOSTICS  2  /* count candidates / accepted pairs (DeviceNeighborCUDA.jl:17-22) */
+
+/* ---- pair models (src/engine/PairModels.jl:13-14, 45-54) ---- */
+#define PDX_PAIR_NONE  0
+#define PDX_PAIR_LJ    1  /* f/r = 24e(2s^12-s^6)/r^2, u = 4e(s^12-s^6) - u_c */
+#define PDX_PAIR_WCA   2  /* same force, u + e, rc^2 = 1.2599211 sigma^2 */
+#define PDX_PAIR_SOFT  3  /* harmonic repulsion, rc = sigma */
+
+/* ---- evaluation flags ---- */
+#define PDX_ENERGY  1  /* also per-particle energy / virial and FP64 totals */
+#define PDX_ADD     2  /* add into fx, fy, fz, Epot, V instead of overwriting */
+
+/* A hipStream_t passed as an opaque pointer (NULL = the default stream). */
+typedef void *pdx_stream_t;
+typedef struct pdx_nl_s *pdx_nl_t;
+typedef struct pdx_pair_s *pdx_pair_t;
+typedef struct pdx_mesh_s *pdx_mesh_t;
+
+/* Library version (major*10000 + minor*100 + patch) and the last error message of this thread. */
+int pdx_version(void);
+const char *pdx_last_error(void);
+
+/* ======================================================================= */
+/* Neighbor lists: counting-sort cell list + slot-major ELL rows           */
+/* ======================================================================= */
+
+/* Device pointers into a list's storage (read-only for callers). */
+typedef struct {
+  int32_t *neighbors_flat;      /* N*cap, slot-major, 1-based ids */
+  int32_t *counts;              /* N (row lengths; kept past cap to report overflow) */
+  int32_t *cell_offsets;        /* ncell+1, 1-based, last N+1 */
+  int32_t *particle_ids_sorted; /* N, 1-based, cell order (stable: ascending id within a cell) */
+  int32_t *cell_of_particle;    /* N, 0-based cell id (cz*ny+cy)*nx+cx */
+  float *rref_x, *rref_y, *rref_z; /* reference positions of the last build */
+  float *packed;                /* N float4 (x, y, z, q) in storage order, written by pdx_nl_update/_rebuild */
+  int64_t *state;               /* PDX_DN_NSTATE words */
+} pdx_nl_arrays;
+
+/* Allocate a device list for N particles in a centred box (DeviceNeighborWorkspace,
+ * DeviceNeighborCUDA.jl:64-82; _alloc_neighbor_matrix, NeighborAllocation.jl:28-83).
+ * cell = cutoff + skin; grid floor(L/cell) >= 3 per dimension unless PDX_NL_ALLPAIRS,
+ * which requires cutoff + skin <= L/2. interval: initial adaptive rebuild interval, 1..100. */
+int pdx_nl_create(int32_t n, const double box[3], double cutoff, double skin, int32_t cap,
+                  int32_t interval, int32_t flags, pdx_nl_t *out);
+int pdx_nl_destroy(pdx_nl_t nl);
+/* Grid, capacity and geometry: out[0..7] = N, cap, nx, ny, nz, ncell, allpairs, diagnostics. */
+int pdx_nl_info(pdx_nl_t nl, int32_t out[8]);
+int pdx_nl_arrays_get(pdx_nl_t nl, pdx_nl_arrays *out);
+
+/* Coverage check + predicated rebuild on the device (update_device_neighbors!,
+ * DeviceNeighborCUDA.jl:421-427): max |dr|^2 since the last build > skin^2/4, or the
+ * build age >= the adaptive interval, requests a rebuild; every rebuild kernel is
+ * enqueued and returns at once when none is requested (graph-capturable). Also packs
+ * (x, y, z, q) into arrays.packed; q may be NULL (w = 0). */
+int pdx_nl_update(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
+                  pdx_stream_t stream);
+/* Unconditional rebuild (build_neighbors_dense!, NeighborAllocation.jl:198-214). */
+int pdx_nl_rebuild(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
+                   pdx_stream_t stream);
+/* Advance the device step counter (advance_device_neighbor_step!, DeviceNeighborCUDA.jl:404-407). */
+int pdx_nl_advance(pdx_nl_t nl, pdx_stream_t stream);
+/* Clear a latched capacity error and request the failed rebuild again (rearm_capacity_rebuild!, :622). */
+int pdx_nl_rearm(pdx_nl_t nl, pdx_stream_t stream);
+/* Replace the row storage by one of capacity cap (resize_capacity!, :628-635). Host boundary. */
+int pdx_nl_resize_capacity(pdx_nl_t nl, int32_t cap);
+/* Change the box (set_device_box!, :89-116); the next check rebuilds. Host boundary. */
+int pdx_nl_set_box(pdx_nl_t nl, const double box[3], pdx_stream_t stream);
+/* Boundary readback of the state words (device_neighbor_status, :639-651). Synchronises
+ * `stream`. Returns PDX_ERR_CAPACITY / PDX_ERR_COORDINATE / PDX_ERR_STATE when check != 0
+ * and the list is unusable; state is filled in every case. */
+int pdx_nl_status(pdx_nl_t nl, int64_t state[PDX_DN_NSTATE], int32_t check, pdx_stream_t stream);
+
+/* ======================================================================= */
+/* Pair forces over ELL rows (LJ / WCA / soft, optional real-space Ewald)  */
+/* ======================================================================= */
+
+typedef struct {
+  int32_t model;          /* PDX_PAIR_* */
+  int32_t shift;          /* LJ potential-shift modifier (PairModels.jl:20-25) */
+  double epsilon, sigma, cutoff;   /* cutoff: LJ only; WCA/SOFT fix their own */
+  /* real-space Ewald Coulomb, C q_i q_j (1/r + P_e(r^2)) (PairModels.jl:90-110, PairFunctor.jl:127-156) */
+  int32_t coulomb;        /* 0 off, 1 Gaussian (Ewald) split */
+  int32_t coulomb_shift;  /* potential-shift modifier on the Coulomb energy (Spec.jl:25-29) */
+  double coulomb_C;       /* prefactor / relative permittivity */
+  double coulomb_cutoff;  /* real-space cutoff */
+  double alpha;           /* Ewald splitting parameter */
+  int32_t lanes;          /* lanes per particle: 0 = automatic, else 1,2,4,8,16,32,64 */
+} pdx_pair_spec;
+
+/* Build a pair model (UniformLJ/UniformWCA/ChargedPairs constructors, PairModels.jl:16-18,
+ * 84-89; coulomb_polynomial, PairFunctor.jl:65-84: our own Chebyshev fit of the Ewald
+ * long-range correction in r^2, FP32 Horner on device). */
+int pdx_pair_create(const pdx_pair_spec *spec, pdx_pair_t *out);
+int pdx_pair_destroy(pdx_pair_t p);
+/* Polynomial sizes and fit error: out[0]=n_pe, out[1]=n_pf, out[2]=max rel err (energy), out[3]=(force). */
+int pdx_pair_info(pdx_pair_t p, double out[4]);
+
+/* Forces (+ per-particle half energies and half virials with PDX_ENERGY) over the rows of nl
+ * (_lj3_ell_kernel_virial!, LJELLEngineKernels.jl:44-79; _pair_cooperative_kernel!,
+ * PairTraversal.jl:66-125). Positions are read from arrays.packed (written by pdx_nl_update).
+ * Epot, V may be NULL. totals (device, 7 doubles: E, Wxx, Wyy, Wzz, Wxy, Wxz, Wyz), may be
+ * NULL, receives FP64 sums of the per-particle values (written, or added with PDX_ADD).
+ * With Coulomb, Epot includes the self energy -C alpha/sqrt(pi) q_i^2. No-op while the list
+ * is invalid (_device_if_valid!, DeviceNeighborCUDA.jl:157-162). */
+int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, float *fz, float *Epot,
+                    float *V, double *totals, int32_t flags, pdx_stream_t stream);
+
+/* ======================================================================= */
+/* Smooth PME mesh engine (B-spline spread, 3D FFT, influence, gather)     */
+/* ======================================================================= */
+
+#define PDX_INFLUENCE_SPME 1
+#define PDX_NET_CHARGE_ERROR 0
+#define PDX_NET_CHARGE_BACKGROUND 1
+
+typedef struct {
+  double coulomb_prefactor;      /* carries the E*L/Q^2 unit */
+  double relative_permittivity;  /* C = prefactor / eps_r */
+  double real_cutoff;
+  double alpha;                  /* Gaussian split; <= 0: from real_space_tolerance */
+  double real_space_tolerance;   /* erfc(alpha rc) = tol (alpha_from_tolerance, Spec.jl:154-163) */
+  int32_t order;                 /* B-spline order P, 3..12 */
+  int32_t mesh[3];               /* K1, K2, K3; mesh[0] <= 0: from mesh_spacing (resolve_mesh, Spec.jl:141-146) */
+  double mesh_spacing;
+  int32_t influence;             /* PDX_INFLUENCE_SPME */
+  int32_t net_charge;            /* PDX_NET_CHARGE_* */
+  double net_charge_tolerance;
+  int32_t real_space_modifier;   /* 0 none, 1 potential shift */
+} pdx_mesh_spec;
+
+/* Allocate the mesh workspace and compute the FP64 influence function for the box
+ * (ElectrostaticsSolver, Solver.jl:129-158; compute_influence!, Influence.jl:190-227). */
+int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const double box[3], pdx_mesh_t *out);
+int pdx_mesh_destroy(pdx_mesh_t m);
+/* Resolved parameters: out[0..7] = K1, K2, K3, P, alpha, C, self_coefficient, bg (per unit q). */
+int pdx_mesh_info(pdx_mesh_t m, double out[8]);
+/* Host charges: total charge, sum q^2, net-charge policy (set_solver_charges!, Solver.jl:166-185).
+ * Also fixes the fixed-point scale of the spread from sum |q|. */
+int pdx_mesh_set_charges(pdx_mesh_t m, const float *q_host, int32_t n);
+/* New box, same mesh: influence function recomputed (refresh_geometry!, Solver.jl:193-209). */
+int pdx_mesh_set_box(pdx_mesh_t m, const double box[3]);
+/* Spread q into the int64 fixed-point grid (_spread_lines_kernel!, MeshKernels.jl:247-276),
+ * deterministic: integer atomics commute. order: 1-based traversal order (e.g.
+ * particle_ids_sorted) or NULL for storage order. The grid is consumed (and cleared) by
+ * pdx_mesh_solve. */
+int pdx_mesh_spread(pdx_mesh_t m, const float *x, const float *y, const float *z, const float *q,
+                    const int32_t *order, pdx_stream_t stream);
+/* Forward R2C 3D FFT, multiply by G, FP64 mesh energy and virial (with PDX_ENERGY), inverse
+ * C2R FFT into the potential grid (mesh_transform!, Solver.jl:305-341; _solve_kernel!,
+ * MeshKernels.jl:57-81). */
+int pdx_mesh_solve(pdx_mesh_t m, int32_t flags, pdx_stream_t stream);
+/* Forces F_i = -q_i grad phi (and E_i += q_i phi/2 + bg q_i with PDX_ENERGY), added into
+ * fx.. (or written with set != 0) (_gather_ad_lines_kernel!, MeshKernels.jl:313-333).
+ * Vglobal (6 floats, may be NULL) receives the reciprocal virial + background term
+ * (_store_global_virial_kernel!, Solver.jl:382-393). */
+int pdx_mesh_gather(pdx_mesh_t m, const float *x, const float *y, const float *z, const float *q,
+                    const int32_t *order, float *fx, float *fy, float *fz, float *Epot,
+                    float *Vglobal, int32_t flags, int32_t set, pdx_stream_t stream);
+/* Boundary readback: out[0] mesh energy, out[1..6] reciprocal virial (with the background
+ * term on the diagonal), out[7] self energy, out[8] background energy (FP64). */
+int pdx_mesh_totals(pdx_mesh_t m, double out[9], pdx_stream_t stream);
+/* Debug/validation views: copy the potential grid (K1*K2*K3 floats) or the half spectrum
+ * of the last solve to device memory `dst`. which: 0 fixed-point charge grid as double,
+ * 1 potential grid (float). */
+int pdx_mesh_copy_grid(pdx_mesh_t m, int32_t which, void *dst, pdx_stream_t stream);
+/* Stand-alone batched 3D FFT of a real grid (K1,K2,K3) (validation of the transform;
+ * the cuFFT wrapper tests, test/test_electrostatics.jl:260-279). dir=-1: R2C real->half
+ * spectrum (float2); dir=+1: C2R (unnormalised). */
+int pdx_mesh_fft(pdx_mesh_t m, int32_t dir, const void *in, void *out, pdx_stream_t stream);
+
+/* ======================================================================= */
+/* FP64 Ewald summation on the device (the converged reference force)      */
+/* ======================================================================= */
+
+/* Direct FP64 Ewald sum for an orthorhombic box, tin-foil, optional background, following
+ * test/ewald_reference.jl:50-126 (real-space images to rcut = sqrt(digits ln 10)/alpha,
+ * reciprocal vectors to kmax = 2 alpha sqrt(digits ln 10)). Forces are evaluated for the
+ * `nsample` particles in `sample` (0-based ids, device) into F (3*nsample doubles, xyz
+ * interleaved); energy and virial (out: E, Wxx, Wyy, Wzz, Wxy, Wxz, Wyz on the host) are
+ * computed over all particles when want_energy != 0. x, y, z, q are device FP64 arrays. */
+int pdx_ewald_fp64(int32_t n, const double *x, const double *y, const double *z, const double *q,
+                   const double box[3], double C, double alpha, double digits, int32_t background,
+                   int32_t nsample, const int32_t *sample, double *F, int32_t want_energy,
+                   double out[7], pdx_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PDX_H */

@@ -1,0 +1,146 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle.c) — TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg, as the
+checker. See oracle.c's header for the reference file each function restates and for the
+checks that pin it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    d, i = C.c_double, C.c_int
+    lib.or_neighbor_rows.argtypes = [i, i, P(d), P(d), P(d), P(d), P(d), i, P(i), P(i), P(d)]
+    lib.or_pair_direct.argtypes = [i, P(d), P(d), P(d), P(d), i, d, d, d, i, P(i), P(d)]
+    lib.or_erfc.argtypes = [d]
+    lib.or_erfc.restype = d
+    lib.or_ewald.argtypes = [i, P(d), P(d), P(d), d, d, d, i, i, P(i), P(i), P(d), P(d), P(d), P(d)]
+    lib.or_ewald_sampled.argtypes = [i, P(d), P(d), P(d), P(d), P(d), d, d, d, i, P(i), P(d)]
+    lib.or_bspline.argtypes = [i, d, P(d), P(d)]
+    lib.or_bspline.restype = i
+    lib.or_cell_lj.argtypes = [i, P(d), P(d), P(d), P(d), d, d, d, P(d), P(d), P(d), P(d)]
+    lib.or_cell_lj.restype = C.c_long
+    lib.or_num_threads.restype = i
+    return lib
+
+
+_lib = _load()
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def num_threads():
+    return _lib.or_num_threads()
+
+
+def erfc(x):
+    return _lib.or_erfc(float(x))
+
+
+def neighbor_rows(coords, box, radii, cap=None):
+    """Rows (lists of 0-based ids, ascending) by brute force (test/neighbor_reference.jl:6-16).
+    coords: tuple of D arrays; radii: per particle. Returns (rows, counts, margin)."""
+    D = len(coords)
+    n = len(coords[0])
+    cs = [_d(c) for c in coords] + [_d(np.zeros(1))] * (3 - D)
+    bx = _d(np.asarray(list(box) + [1.0] * (3 - D)))
+    rr = _d(radii)
+    cap = n if cap is None else cap
+    counts = np.zeros(n, dtype=np.int32)
+    rows = np.zeros(n * max(cap, 1), dtype=np.int32)
+    margin = C.c_double()
+    _lib.or_neighbor_rows(n, D, cs[0][1], cs[1][1], cs[2][1], bx[1], rr[1], cap,
+                          counts.ctypes.data_as(C.POINTER(C.c_int)), rows.ctypes.data_as(C.POINTER(C.c_int)),
+                          C.byref(margin))
+    out = [list(rows[i * cap:i * cap + min(counts[i], cap)]) for i in range(n)]
+    return out, counts, margin.value
+
+
+def pair_direct(x, y, z, box, kind="lj", sigma=1.0, epsilon=1.0, cutoff=2.5, ids=None):
+    """FP64 direct pair sums at the sampled ids. Returns dict F (n,3), energy (n), virial (n,6), counts."""
+    k = {"lj": 1, "wca": 2, "soft": 3}[kind]
+    n = len(x)
+    ids = np.arange(n) if ids is None else np.asarray(ids)
+    xa, ya, za, bx = _d(x), _d(y), _d(z), _d(box)
+    ia = _i(ids)
+    out = np.zeros((len(ids), 12))
+    _lib.or_pair_direct(n, xa[1], ya[1], za[1], bx[1], k, float(sigma), float(epsilon), float(cutoff), len(ids),
+                        ia[1], out.ctypes.data_as(C.POINTER(C.c_double)))
+    return dict(F=out[:, 0:3], energy=out[:, 3], virial=out[:, 4:10], pair_scale=out[:, 10], counts=out[:, 11])
+
+
+def ewald(R, q, H, C_=1.0, alpha=None, digits=16.0, background=False, exceptions=()):
+    """FP64 Ewald (test/ewald_reference.jl:50-126). R: (n,3); H: 3x3 cell (columns = box vectors).
+    Returns (E, F (n,3), W 3x3)."""
+    R = np.asarray(R, dtype=np.float64)
+    n = R.shape[0]
+    Ra = _d(R.reshape(-1))
+    qa = _d(q)
+    Ha = _d(np.asarray(H, dtype=np.float64).T.reshape(-1))  # column-major
+    ne = len(exceptions)
+    ei = _i([e[0] for e in exceptions] or [0])
+    ej = _i([e[1] for e in exceptions] or [0])
+    es = _d([(e[2] if len(e) == 3 else 0.0) for e in exceptions] or [0.0])
+    E = C.c_double()
+    F = np.zeros(3 * n)
+    W = np.zeros(9)
+    _lib.or_ewald(n, Ra[1], qa[1], Ha[1], float(C_), float(alpha) if alpha else 0.0, float(digits),
+                  1 if background else 0, ne, ei[1], ej[1], es[1], C.byref(E),
+                  F.ctypes.data_as(C.POINTER(C.c_double)), W.ctypes.data_as(C.POINTER(C.c_double)))
+    return E.value, F.reshape(n, 3), W.reshape(3, 3).T
+
+
+def ewald_sampled(x, y, z, q, box, C_=1.0, alpha=0.35, digits=14.0, ids=None):
+    """FP64 Ewald forces on the sampled ids of an orthorhombic box. Returns F (ns, 3)."""
+    n = len(x)
+    ids = np.arange(n) if ids is None else np.asarray(ids)
+    xa, ya, za, qa, bx = _d(x), _d(y), _d(z), _d(q), _d(box)
+    ia = _i(ids)
+    F = np.zeros(3 * len(ids))
+    _lib.or_ewald_sampled(n, xa[1], ya[1], za[1], qa[1], bx[1], float(C_), float(alpha), float(digits), len(ids),
+                          ia[1], F.ctypes.data_as(C.POINTER(C.c_double)))
+    return F.reshape(-1, 3)
+
+
+def bspline(P, u):
+    w = np.zeros(P)
+    dw = np.zeros(P)
+    k0 = _lib.or_bspline(int(P), float(u), w.ctypes.data_as(C.POINTER(C.c_double)),
+                         dw.ctypes.data_as(C.POINTER(C.c_double)))
+    return k0, w, dw
+
+
+def cell_lj(x, y, z, box, sigma=1.0, epsilon=1.0, cutoff=2.5):
+    """Same-algorithm CPU LJ evaluation (cell list, FP64) for the CPU baseline timing."""
+    n = len(x)
+    xa, ya, za, bx = _d(x), _d(y), _d(z), _d(box)
+    fx, fy, fz = np.zeros(n), np.zeros(n), np.zeros(n)
+    e = C.c_double()
+    P = C.POINTER(C.c_double)
+    pairs = _lib.or_cell_lj(n, xa[1], ya[1], za[1], bx[1], sigma, epsilon, cutoff, fx.ctypes.data_as(P),
+                            fy.ctypes.data_as(P), fz.ctypes.data_as(P), C.byref(e))
+    return np.stack([fx, fy, fz], 1), e.value, pairs

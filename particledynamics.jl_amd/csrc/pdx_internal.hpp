@@ -1,0 +1,215 @@
+// Internal declarations shared by the libpdx translation units: error plumbing,
+// device helpers (periodic geometry, wave64 reductions) and the handle structs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "pdx.h"
+
+namespace pdx {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define PDX_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return ::pdx::fail(PDX_ERR_DEVICE, std::string(#call " failed: ") +          \
+                                             hipGetErrorString(e_));               \
+  } while (0)
+
+#define PDX_LAUNCH_CHECK()                                                         \
+  do {                                                                             \
+    hipError_t e_ = hipGetLastError();                                             \
+    if (e_ != hipSuccess)                                                          \
+      return ::pdx::fail(PDX_ERR_DEVICE, std::string("kernel launch failed: ") +   \
+                                             hipGetErrorString(e_));               \
+  } while (0)
+
+inline hipStream_t as_stream(pdx_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Owned device buffer (allocated at create time only).
+template <typename T> struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return PDX_OK;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      n = 0;
+      return fail(PDX_ERR_DEVICE, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                                      " bytes failed: " + hipGetErrorString(e));
+    }
+    return PDX_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+constexpr int WAVE = 64;
+
+// ---------------------------------------------------------------- device helpers
+
+// Minimum image for coordinates in one periodic image of [-L/2, L/2)
+// (mic_fast, src/neighbors/NeighborTypes.jl:65-69): branch-free select.
+__device__ __forceinline__ float mic_fast(float dx, float halfL, float L) {
+  dx -= (dx > halfL) ? L : 0.0f;
+  dx += (dx < -halfL) ? L : 0.0f;
+  return dx;
+}
+
+// Compensated periodic difference (periodic_delta, src/geometry/PeriodicDisplacement.jl:4-14):
+// the rounding error of xi - xj (TwoDiff) is carried past the exact (Sterbenz) subtraction
+// of L, so short separations across the periodic seam keep their low bits.
+__device__ __forceinline__ float periodic_delta(float xi, float xj, float halfL, float L) {
+  float d = xi - xj;
+  if (fabsf(d) > halfL) {
+    float bv = xi - d;
+    float res = (xi - (d + bv)) + (bv - xj);
+    return (d - copysignf(L, d)) + res;
+  }
+  return d;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
+  return v;
+}
+__device__ __forceinline__ float wave_sum_width(float v, int width) {
+  for (int off = width >> 1; off >= 1; off >>= 1) v += __shfl_xor(v, off, width);
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, WAVE));
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, WAVE));
+  return v;
+}
+
+// Deterministic grid-wide FP64 sum of NV values per thread, in one launch: every block
+// reduces its values in a fixed order and stores them in `partial[block*NV + c]`; the
+// last block to finish (atomic ticket) sums the partials in block order and writes (or
+// adds into) out[c], then resets the ticket for the next launch. Must be called by every
+// thread of every block (blockDim.x a multiple of 64, at most 1024).
+template <int NV>
+__device__ void grid_sum_finalize(const double (&v)[NV], double *partial, unsigned *ticket,
+                                  double *out, bool add) {
+  __shared__ double sh[NV][16];
+  __shared__ bool last;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = threadIdx.x / WAVE;
+  const int nwave = blockDim.x / WAVE;
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    double s = wave_sum(v[c]);
+    if (lane == 0) sh[c][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+    for (int w = 0; w < nwave; ++w) s += sh[threadIdx.x][w];
+    partial[(size_t)blockIdx.x * NV + threadIdx.x] = s;
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // Fixed-order sum over blocks: thread t takes blocks t, t+blockDim, ... then a fixed tree.
+  double acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) acc[c] = 0.0;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < NV; ++c) acc[c] += __builtin_nontemporal_load(&partial[(size_t)b * NV + c]);
+  }
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    double s = wave_sum(acc[c]);
+    if (lane == 0) sh[c][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+    for (int w = 0; w < nwave; ++w) s += sh[threadIdx.x][w];
+    if (out) out[threadIdx.x] = add ? out[threadIdx.x] + s : s;
+  }
+  if (threadIdx.x == 0) *ticket = 0u;
+}
+
+}  // namespace pdx
+
+// ---------------------------------------------------------------- handles
+
+struct pdx_nl_s {
+  int32_t N = 0, cap = 0, nx = 1, ny = 1, nz = 1, ncell = 1;
+  int32_t interval = 20;
+  bool allpairs = false, diagnostics = false;
+  double box[3] = {0, 0, 0};
+  double cutoff = 0, skin = 0;
+  float rl2 = 0;  // (cutoff + skin)^2 in FP32, as the reference's rl2
+  pdx::DevBuf<int64_t> state;
+  pdx::DevBuf<float> geom;  // device box lengths (3)
+  pdx::DevBuf<int32_t> neighbors_flat, counts, cell_offsets, cell_counts, particle_ids_sorted,
+      cell_of_particle, cell_rank;
+  pdx::DevBuf<float> rref;      // 3*N: x | y | z
+  pdx::DevBuf<float4> packed;   // N, storage order (x, y, z, q)
+  pdx::DevBuf<float4> sorted;   // N, cell order (x, y, z, id bits)
+  pdx::DevBuf<float> disp_partial;
+  pdx::DevBuf<int32_t> cap_partial;
+  pdx::DevBuf<int64_t> stat_partial;  // 2 per block: candidates, accepted
+  pdx::DevBuf<int32_t> scan_block;    // per-1024-cell block totals
+  pdx::DevBuf<int64_t> candidates;    // N (diagnostics)
+  pdx::DevBuf<unsigned> tickets;      // last-block tickets (decide, commit)
+};
+
+struct pdx_pair_s {
+  pdx_pair_spec spec{};
+  float eps = 0, sig2 = 0, rc2 = 0, eshift = 0, wca_shift = 0;
+  float cC = 0, crc2 = 0, csx = 0, ceshift = 0, halfself = 0;
+  int npe = 0, npf = 0;
+  float pe[24] = {0}, pf[24] = {0};
+  double fit_err_e = 0, fit_err_f = 0;
+  int lanes = 4;
+  pdx::DevBuf<double> partial;
+  pdx::DevBuf<unsigned> ticket;
+};
+
+struct pdx_mesh_s {
+  pdx_mesh_spec spec{};
+  int32_t N = 0;
+  int K1 = 0, K2 = 0, K3 = 0, K1h = 0, P = 0;
+  double box[3] = {0, 0, 0};
+  double C = 0, alpha = 0, selfc = 0, bgcoef = 0;
+  double qtotal = 0, qsq = 0, qabs = 0, ubg = 0;
+  float bg = 0;
+  double fixed_scale = 0, fixed_inv = 0;  // 2^S and 2^-S of the fixed-point spread
+  pdx::DevBuf<unsigned long long> grid64;  // K1*K2*K3 fixed-point charges
+  pdx::DevBuf<float2> spec_c;              // K1h*K2*K3 half spectrum
+  pdx::DevBuf<float> pot;                  // K1*K2*K3 potential
+  pdx::DevBuf<float> G;                    // K1h*K2*K3 influence function (FP32)
+  pdx::DevBuf<float2> tw;                  // twiddles exp(-2 pi i m / K) for K1 | K2 | K3
+  pdx::DevBuf<double> acc;                 // 8: mesh energy + 6 virial + spare
+  pdx::DevBuf<double> partial;
+  pdx::DevBuf<unsigned> ticket;
+  int nmode_blocks = 0;
+};

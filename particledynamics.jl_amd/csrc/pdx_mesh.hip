@@ -1,0 +1,924 @@
+// Smooth PME mesh engine on gfx950: B-spline charge spreading into an int64 fixed-point
+// grid, a mixed-radix (2,3,4,5,7) Stockham 3D FFT in LDS, the influence-function
+// convolution with FP64 energy / virial reductions fused into the z transform, and the
+// analytic-differentiation force gather.
+//
+// Pass structure of one solve (every pass reads and writes the mesh once):
+//   k_fft_x_r2c   int64 fixed point -> float, R2C along x, clears the charge grid
+//   k_fft_y       complex FFT along y (forward)
+//   k_fft_z       forward z FFT, x G (+ FP64 energy/virial), inverse z FFT   [fused solve]
+//   k_fft_y       inverse along y
+//   k_fft_x_c2r   C2R along x into the potential grid (unnormalised, as cuFFT's brfft)
+// Lines of a pass are staged in LDS as [element*T + line] for T adjacent lines, so global
+// loads and stores of a tile are contiguous runs of T complex values.
+#include "pdx_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+using namespace pdx;
+
+namespace {
+
+constexpr int TPB = 256;
+constexpr int MAXSTAGE = 16;
+
+struct FFTPlan {
+  int K;
+  int nstage;
+  int radix[MAXSTAGE];
+};
+
+bool make_plan(int K, FFTPlan &pl) {
+  pl.K = K;
+  pl.nstage = 0;
+  int n = K;
+  const int rs[5] = {4, 2, 3, 5, 7};
+  for (int r : rs) {
+    while (n % r == 0) {
+      if (pl.nstage >= MAXSTAGE) return false;
+      pl.radix[pl.nstage++] = r;
+      n /= r;
+    }
+  }
+  return n == 1;
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by sign*i
+__device__ __forceinline__ float2 cmuli(float2 a, float sign) { return make_float2(-sign * a.y, sign * a.x); }
+
+// In-place radix-R DFT of u[0..R) with exponent sign `sg` (-1 forward, +1 inverse).
+template <int R> __device__ __forceinline__ void dft(float2 (&u)[7], float sg);
+
+template <> __device__ __forceinline__ void dft<2>(float2 (&u)[7], float) {
+  const float2 a = u[0], b = u[1];
+  u[0] = cadd(a, b);
+  u[1] = csub(a, b);
+}
+template <> __device__ __forceinline__ void dft<3>(float2 (&u)[7], float sg) {
+  const float h = 0.86602540378443864676f;
+  const float2 t = cadd(u[1], u[2]);
+  const float2 d = cmuli(csub(u[1], u[2]), sg * h);
+  const float2 m = make_float2(u[0].x - 0.5f * t.x, u[0].y - 0.5f * t.y);
+  u[0] = cadd(u[0], t);
+  u[1] = cadd(m, d);
+  u[2] = csub(m, d);
+}
+template <> __device__ __forceinline__ void dft<4>(float2 (&u)[7], float sg) {
+  const float2 t0 = cadd(u[0], u[2]), t1 = csub(u[0], u[2]);
+  const float2 t2 = cadd(u[1], u[3]), t3 = cmuli(csub(u[1], u[3]), sg);
+  u[0] = cadd(t0, t2);
+  u[2] = csub(t0, t2);
+  u[1] = cadd(t1, t3);
+  u[3] = csub(t1, t3);
+}
+template <> __device__ __forceinline__ void dft<5>(float2 (&u)[7], float sg) {
+  const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+  const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+  const float2 a1 = cadd(u[1], u[4]), b1 = csub(u[1], u[4]);
+  const float2 a2 = cadd(u[2], u[3]), b2 = csub(u[2], u[3]);
+  const float2 u0 = u[0];
+  const float2 r1 = make_float2(u0.x + c1 * a1.x + c2 * a2.x, u0.y + c1 * a1.y + c2 * a2.y);
+  const float2 r2 = make_float2(u0.x + c2 * a1.x + c1 * a2.x, u0.y + c2 * a1.y + c1 * a2.y);
+  const float2 i1 = cmuli(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y), sg);
+  const float2 i2 = cmuli(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y), sg);
+  u[0] = cadd(u0, cadd(a1, a2));
+  u[1] = cadd(r1, i1);
+  u[4] = csub(r1, i1);
+  u[2] = cadd(r2, i2);
+  u[3] = csub(r2, i2);
+}
+template <> __device__ __forceinline__ void dft<7>(float2 (&u)[7], float sg) {
+  // direct sum with cos/sin(2 pi k/7)
+  const float c[7] = {1.0f, 0.62348980185873353053f, -0.22252093395631440429f, -0.90096886790241912624f,
+                      -0.90096886790241912624f, -0.22252093395631440429f, 0.62348980185873353053f};
+  const float s[7] = {0.0f, 0.78183148246802980871f, 0.97492791218182360702f, 0.43388373911755812048f,
+                      -0.43388373911755812048f, -0.97492791218182360702f, -0.78183148246802980871f};
+  float2 v[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int r = 0; r < 7; ++r) {
+      const int m = (q * r) % 7;
+      const float2 w = make_float2(c[m], sg * s[m]);
+      acc = cadd(acc, cmul(u[r], w));
+    }
+    v[q] = acc;
+  }
+#pragma unroll
+  for (int q = 0; q < 7; ++q) u[q] = v[q];
+}
+
+template <int R>
+__device__ __forceinline__ void stage(const float2 *__restrict__ in, float2 *__restrict__ out, int K, int T,
+                                      int p, const float2 *__restrict__ tw, float sg) {
+  const int KR = K / R;
+  const int tstride = K / (p * R);
+  const int nb = T * KR;
+  for (int idx = threadIdx.x; idx < nb; idx += blockDim.x) {
+    const int l = idx % T;
+    const int i = idx / T;
+    const int k = i % p;
+    float2 u[7];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      float2 v = in[(i + r * KR) * T + l];
+      if (r > 0 && k > 0) {
+        float2 w = tw[r * k * tstride];
+        w.y *= -sg;  // the table holds exp(-2 pi i m/K): kept forward, conjugated inverse
+        v = cmul(v, w);
+      }
+      u[r] = v;
+    }
+    dft<R>(u, sg);
+    const int j = (i / p) * p * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) out[(j + r * p) * T + l] = u[r];
+  }
+}
+
+// Full transform of T lines of length K held in buf0 ([e*T + l]); returns the buffer with
+// the result (buf0 or buf1). sg = -1 forward, +1 inverse (unnormalised).
+__device__ float2 *lds_fft(float2 *buf0, float2 *buf1, const FFTPlan &pl, int T, const float2 *__restrict__ tw,
+                           float sg) {
+  float2 *a = buf0, *b = buf1;
+  int p = 1;
+  for (int s = 0; s < pl.nstage; ++s) {
+    const int R = pl.radix[s];
+    switch (R) {
+      case 4: stage<4>(a, b, pl.K, T, p, tw, sg); break;
+      case 2: stage<2>(a, b, pl.K, T, p, tw, sg); break;
+      case 3: stage<3>(a, b, pl.K, T, p, tw, sg); break;
+      case 5: stage<5>(a, b, pl.K, T, p, tw, sg); break;
+      default: stage<7>(a, b, pl.K, T, p, tw, sg); break;
+    }
+    __syncthreads();
+    float2 *t = a; a = b; b = t;
+    p *= R;
+  }
+  return a;
+}
+
+// ---- x pass, forward: fixed-point (or float) real lines -> half spectrum
+template <bool FIXED>
+__global__ __launch_bounds__(TPB) void k_fft_x_r2c(const void *src, unsigned long long *clear, float2 *spec,
+                                                    int K1, int K1h, int nlines, int T, FFTPlan pl,
+                                                    const float2 *__restrict__ tw, double inv_scale) {
+  extern __shared__ float2 lds[];
+  float2 *b0 = lds, *b1 = lds + (size_t)T * K1;
+  const int line0 = blockIdx.x * T;
+  const int n = T * K1;
+  for (int e = threadIdx.x; e < n; e += TPB) {
+    const int l = e / K1, n1 = e % K1;
+    const int line = line0 + l;
+    float v = 0.f;
+    if (line < nlines) {
+      const size_t g = (size_t)line * K1 + n1;
+      if (FIXED) {
+        const long long q = (long long)((const unsigned long long *)src)[g];
+        v = (float)((double)q * inv_scale);
+        clear[g] = 0ull;
+      } else {
+        v = ((const float *)src)[g];
+      }
+    }
+    b0[n1 * T + l] = make_float2(v, 0.f);
+  }
+  __syncthreads();
+  float2 *r = lds_fft(b0, b1, pl, T, tw, -1.0f);
+  const int nh = T * K1h;
+  for (int e = threadIdx.x; e < nh; e += TPB) {
+    const int l = e / K1h, m1 = e % K1h;
+    const int line = line0 + l;
+    if (line < nlines) spec[(size_t)line * K1h + m1] = r[m1 * T + l];
+  }
+}
+
+// ---- x pass, inverse: half spectrum -> real lines (Hermitian completion in LDS)
+__global__ __launch_bounds__(TPB) void k_fft_x_c2r(const float2 *spec, float *out, int K1, int K1h, int nlines,
+                                                   int T, FFTPlan pl, const float2 *__restrict__ tw) {
+  extern __shared__ float2 lds[];
+  float2 *b0 = lds, *b1 = lds + (size_t)T * K1;
+  const int line0 = blockIdx.x * T;
+  const int n = T * K1;
+  for (int e = threadIdx.x; e < n; e += TPB) {
+    const int l = e / K1, m = e % K1;
+    const int line = line0 + l;
+    float2 v = make_float2(0.f, 0.f);
+    if (line < nlines) {
+      if (m < K1h) {
+        v = spec[(size_t)line * K1h + m];
+      } else {
+        const float2 c = spec[(size_t)line * K1h + (K1 - m)];
+        v = make_float2(c.x, -c.y);
+      }
+    }
+    b0[m * T + l] = v;
+  }
+  __syncthreads();
+  float2 *r = lds_fft(b0, b1, pl, T, tw, +1.0f);
+  for (int e = threadIdx.x; e < n; e += TPB) {
+    const int l = e / K1, n1 = e % K1;
+    const int line = line0 + l;
+    if (line < nlines) out[(size_t)line * K1 + n1] = r[n1 * T + l].x;
+  }
+}
+
+// ---- y pass: tiles of T adjacent m1 at fixed m3; elements m2 at stride K1h
+__global__ __launch_bounds__(TPB) void k_fft_y(float2 *spec, int K1h, int K2, int T, FFTPlan pl,
+                                               const float2 *__restrict__ tw, float sg) {
+  extern __shared__ float2 lds[];
+  float2 *b0 = lds, *b1 = lds + (size_t)T * K2;
+  const int tiles = (K1h + T - 1) / T;
+  const int m10 = (blockIdx.x % tiles) * T;
+  const int m3 = blockIdx.x / tiles;
+  const size_t base = (size_t)m3 * K2 * K1h;
+  const int n = T * K2;
+  for (int e = threadIdx.x; e < n; e += TPB) {
+    const int l = e % T, m2 = e / T;
+    const int m1 = m10 + l;
+    b0[m2 * T + l] = m1 < K1h ? spec[base + (size_t)m2 * K1h + m1] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2 *r = lds_fft(b0, b1, pl, T, tw, sg);
+  for (int e = threadIdx.x; e < n; e += TPB) {
+    const int l = e % T, m2 = e / T;
+    const int m1 = m10 + l;
+    if (m1 < K1h) spec[base + (size_t)m2 * K1h + m1] = r[m2 * T + l];
+  }
+}
+
+struct SolveArgs {
+  const float *G;
+  int K1, K2, K3, K1h;
+  double twopi_L[3];   // 2 pi / L_d
+  double inv4a2;       // 1/(4 alpha^2): Vc_ab = G (delta_ab + c k_a k_b), c = -2/k^2 - 2 inv4a2
+  double *partial;
+  unsigned *ticket;
+  double *acc;
+};
+
+// ---- z pass: MODE 0 forward only, 1 inverse only, 2 forward + solve + inverse
+template <int MODE, bool ENERGY>
+__global__ __launch_bounds__(TPB) void k_fft_z(float2 *spec, int T, FFTPlan pl, const float2 *__restrict__ tw,
+                                               SolveArgs s) {
+  extern __shared__ float2 lds[];
+  const int K1h = s.K1h, K2 = s.K2, K3 = s.K3;
+  float2 *b0 = lds, *b1 = lds + (size_t)T * K3;
+  const int tiles = (K1h + T - 1) / T;
+  const int m10 = (blockIdx.x % tiles) * T;
+  const int m2 = blockIdx.x / tiles;
+  const size_t plane = (size_t)K1h * K2;
+  const size_t base = (size_t)m2 * K1h;
+  const int n = T * K3;
+  for (int e = threadIdx.x; e < n; e += TPB) {
+    const int l = e % T, m3 = e / T;
+    const int m1 = m10 + l;
+    b0[m3 * T + l] = m1 < K1h ? spec[base + (size_t)m3 * plane + m1] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2 *r = b0;
+  if (MODE != 1) r = lds_fft(b0, b1, pl, T, tw, -1.0f);
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (MODE == 2) {
+    const int s2 = m2 <= K2 / 2 ? m2 : m2 - K2;
+    const bool ny2 = (K2 % 2 == 0) && (m2 == K2 / 2);
+    for (int e = threadIdx.x; e < n; e += TPB) {
+      const int l = e % T, m3 = e / T;
+      const int m1 = m10 + l;
+      if (m1 >= K1h) continue;
+      const size_t idx = base + (size_t)m3 * plane + m1;
+      const float Gm = s.G[idx];
+      float2 z = r[m3 * T + l];
+      if (ENERGY && Gm != 0.0f) {
+        const double w = (m1 == 0 || ((s.K1 % 2 == 0) && m1 == s.K1 / 2)) ? 0.5 : 1.0;
+        const double a = w * ((double)z.x * z.x + (double)z.y * z.y) * (double)Gm;
+        const int s3 = m3 <= K3 / 2 ? m3 : m3 - K3;
+        const bool ny1 = (s.K1 % 2 == 0) && (m1 == s.K1 / 2);
+        const bool ny3 = (K3 % 2 == 0) && (m3 == K3 / 2);
+        const double kx = s.twopi_L[0] * m1, ky = s.twopi_L[1] * s2, kz = s.twopi_L[2] * s3;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        const double c = -2.0 / k2 - 2.0 * s.inv4a2;
+        acc[0] += a;
+        acc[1] += a * (1.0 + c * kx * kx);
+        acc[2] += a * (1.0 + c * ky * ky);
+        acc[3] += a * (1.0 + c * kz * kz);
+        acc[4] += (ny1 || ny2) ? 0.0 : a * c * kx * ky;
+        acc[5] += (ny1 || ny3) ? 0.0 : a * c * kx * kz;
+        acc[6] += (ny2 || ny3) ? 0.0 : a * c * ky * kz;
+      }
+      r[m3 * T + l] = cscale(z, Gm);
+    }
+    __syncthreads();
+    float2 *other = (r == b0) ? b1 : b0;
+    r = lds_fft(r, other, pl, T, tw, +1.0f);
+  } else if (MODE == 1) {
+    r = lds_fft(b0, b1, pl, T, tw, +1.0f);
+  }
+  for (int e = threadIdx.x; e < n; e += TPB) {
+    const int l = e % T, m3 = e / T;
+    const int m1 = m10 + l;
+    if (m1 < K1h) spec[base + (size_t)m3 * plane + m1] = r[m3 * T + l];
+  }
+  if (MODE == 2 && ENERGY) {
+    // mesh energy = 1/2 sum_full |b|^2 G = sum_half w |b|^2 G (w = 1/2 on the m1 = 0 and
+    // Nyquist planes); virial W_ab = sum_half w |b|^2 Vc_ab
+    grid_sum_finalize<7>(acc, s.partial, s.ticket, s.acc, false);
+  }
+}
+
+// ---- B-spline weights (window_stencil, src/electrostatics/Windows.jl:209-223): order-P
+// cardinal B-spline weights w[j] of nodes k0 + j (ascending) and derivatives dw[j].
+template <int P>
+__device__ __forceinline__ int bspline(float u, float (&w)[P], float (&dw)[P]) {
+  const float fl = floorf(u);
+  const float t = u - fl;
+  float a[P];  // a[j] = M_n(t + j), built up from M_2
+  a[0] = t;
+  a[1] = 1.0f - t;
+#pragma unroll
+  for (int j = 2; j < P; ++j) a[j] = 0.f;
+#pragma unroll
+  for (int n = 2; n < P - 1; ++n) {  // M_n -> M_{n+1}
+    const float inv = 1.0f / (float)n;
+#pragma unroll
+    for (int j = n; j >= 0; --j) {
+      const float left = j < n ? a[j] : 0.f;
+      const float right = j >= 1 ? a[j - 1] : 0.f;
+      a[j] = ((t + j) * left + ((float)(n + 1) - t - j) * right) * inv;
+    }
+  }
+  // a = M_{P-1}(t + j), j = 0..P-2; final step to M_P and derivatives
+  const float inv = 1.0f / (float)(P - 1);
+  float full[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const float left = j < P - 1 ? a[j] : 0.f;
+    const float right = j >= 1 ? a[j - 1] : 0.f;
+    full[j] = ((t + j) * left + ((float)P - t - j) * right) * inv;
+  }
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    const int j = P - 1 - i;  // node k0 + i carries M_P(t + j)
+    w[i] = full[j];
+    const float left = j <= P - 2 ? a[j] : 0.f;
+    const float right = j >= 1 ? a[j - 1] : 0.f;
+    dw[i] = left - right;
+  }
+  return (int)fl - (P - 1);
+}
+
+// Fractional grid coordinate in [0, K) (_grid_coordinate, MeshKernels.jl:9-15).
+__device__ __forceinline__ float grid_coordinate(float x, float L, int K) {
+  float s = x / L;
+  s -= floorf(s);
+  float u = s * (float)K;
+  if (!isfinite(u)) return 0.f;
+  return u >= (float)K ? u - (float)K : u;
+}
+__device__ __forceinline__ int wrap_node(int n, int K) { return n < 0 ? n + K : (n >= K ? n - K : n); }
+
+template <int P>
+__device__ __forceinline__ float sel(const float (&v)[P], int j) {
+  float r = v[0];
+#pragma unroll
+  for (int k = 1; k < P; ++k) r = (j == k) ? v[k] : r;
+  return r;
+}
+
+constexpr int lanes_for(int P) { return P * P <= 16 ? 16 : (P * P <= 32 ? 32 : 64); }
+
+// ---- spread: LPP lanes per particle, lane l owns the (x, y) line (l % P, l / P) of the
+// particle's P x P x P stencil and walks its P z nodes with 64-bit integer atomics (the
+// sum is exact and independent of the order of arrival).
+template <int P>
+__global__ __launch_bounds__(TPB) void k_spread(int N, const float *__restrict__ x, const float *__restrict__ y,
+                                                const float *__restrict__ z, const float *__restrict__ q,
+                                                const int32_t *__restrict__ order, unsigned long long *grid,
+                                                int K1, int K2, int K3, float Lx, float Ly, float Lz,
+                                                float scale) {
+  constexpr int LPP = lanes_for(P);
+  const int tid = blockIdx.x * TPB + threadIdx.x;
+  const int t = tid / LPP;
+  const int lane = tid % LPP;
+  if (t >= N) return;
+  const int i = order ? order[t] - 1 : t;
+  const float qi = q[i];
+  if (qi == 0.0f) return;
+  float wx[P], wy[P], wz[P], d[P];
+  const int k1 = bspline<P>(grid_coordinate(x[i], Lx, K1), wx, d);
+  const int k2 = bspline<P>(grid_coordinate(y[i], Ly, K2), wy, d);
+  const int k3 = bspline<P>(grid_coordinate(z[i], Lz, K3), wz, d);
+  const size_t KK = (size_t)K1 * K2;
+  for (int l = lane; l < P * P; l += LPP) {
+    const int j1 = l % P, j2 = l / P;
+    const int n1 = wrap_node(k1 + j1, K1), n2 = wrap_node(k2 + j2, K2);
+    const float q12 = qi * sel<P>(wx, j1) * sel<P>(wy, j2) * scale;
+    const size_t b = (size_t)n1 + (size_t)K1 * n2;
+#pragma unroll
+    for (int j3 = 0; j3 < P; ++j3) {
+      const int n3 = wrap_node(k3 + j3, K3);
+      const long long v = __float2ll_rn(q12 * wz[j3]);
+      atomicAdd(&grid[b + KK * n3], (unsigned long long)v);
+    }
+  }
+}
+
+// ---- gather: same lane layout; line partial sums reduced over the particle's lanes.
+template <int P, bool ENERGY>
+__global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__ x, const float *__restrict__ y,
+                                                const float *__restrict__ z, const float *__restrict__ q,
+                                                const int32_t *__restrict__ order, const float *__restrict__ pot,
+                                                int K1, int K2, int K3, float Lx, float Ly, float Lz,
+                                                float *fx, float *fy, float *fz, float *Epot, float bg, int set,
+                                                float *Vglobal, const double *acc, double ubg) {
+  constexpr int LPP = lanes_for(P);
+  const int tid = blockIdx.x * TPB + threadIdx.x;
+  if (Vglobal && tid < 6) Vglobal[tid] = (float)(acc[1 + tid] + (tid < 3 ? ubg : 0.0));
+  const int t = tid / LPP;
+  const int lane = tid % LPP;
+  const bool active = t < N;
+  const int i = active ? (order ? order[t] - 1 : t) : 0;
+  const float qi = active ? q[i] : 0.0f;
+  float phi = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+  if (active && qi != 0.0f) {
+    float wx[P], wy[P], wz[P], dx[P], dy[P], dz[P];
+    const int k1 = bspline<P>(grid_coordinate(x[i], Lx, K1), wx, dx);
+    const int k2 = bspline<P>(grid_coordinate(y[i], Ly, K2), wy, dy);
+    const int k3 = bspline<P>(grid_coordinate(z[i], Lz, K3), wz, dz);
+    const size_t KK = (size_t)K1 * K2;
+    for (int l = lane; l < P * P; l += LPP) {
+      const int j1 = l % P, j2 = l / P;
+      const int n1 = wrap_node(k1 + j1, K1), n2 = wrap_node(k2 + j2, K2);
+      const size_t b = (size_t)n1 + (size_t)K1 * n2;
+      float s0 = 0.f, sz = 0.f;
+#pragma unroll
+      for (int j3 = 0; j3 < P; ++j3) {
+        const float c = pot[b + KK * wrap_node(k3 + j3, K3)];
+        s0 = fmaf(wz[j3], c, s0);
+        sz = fmaf(dz[j3], c, sz);
+      }
+      const float ax = sel<P>(wx, j1), ay = sel<P>(wy, j2);
+      const float w12 = ax * ay;
+      phi = fmaf(w12, s0, phi);
+      gz = fmaf(w12, sz, gz);
+      gx = fmaf(sel<P>(dx, j1) * ay, s0, gx);
+      gy = fmaf(ax * sel<P>(dy, j2), s0, gy);
+    }
+  }
+  gx = wave_sum_width(gx, LPP);
+  gy = wave_sum_width(gy, LPP);
+  gz = wave_sum_width(gz, LPP);
+  if (ENERGY) phi = wave_sum_width(phi, LPP);
+  if (!active || lane != 0) return;
+  const float ffx = -(qi * gx * ((float)K1 / Lx));
+  const float ffy = -(qi * gy * ((float)K2 / Ly));
+  const float ffz = -(qi * gz * ((float)K3 / Lz));
+  if (set) {
+    fx[i] = ffx; fy[i] = ffy; fz[i] = ffz;
+    if (ENERGY && Epot) Epot[i] = 0.5f * qi * phi + bg * qi;
+  } else if (qi != 0.0f) {
+    fx[i] += ffx; fy[i] += ffy; fz[i] += ffz;
+    if (ENERGY && Epot) Epot[i] += 0.5f * qi * phi + bg * qi;
+  }
+}
+
+__global__ void k_fixed_to_double(const unsigned long long *g, double *out, size_t n, double inv) {
+  const size_t k = (size_t)blockIdx.x * TPB + threadIdx.x;
+  if (k < n) out[k] = (double)(long long)g[k] * inv;
+}
+
+// ---------------------------------------------------------------- host setup
+double host_erfc(double x) { return std::erfc(x); }
+
+int fft_friendly(int n) {
+  int m = std::max(n, 1);
+  while (true) {
+    int r = m;
+    for (int p : {2, 3, 5, 7})
+      while (r % p == 0) r /= p;
+    if (r == 1) return m;
+    ++m;
+  }
+}
+
+// M_P at the integers 0..P-1 (FP64), then the SPME deconvolution 1/|sum_j M_P(j) e^{2 pi i m j/K}|^2
+// with the odd-order Nyquist zero replaced by the neighbour average (window_deconvolution,
+// src/electrostatics/Windows.jl:283-308).
+std::vector<double> deconvolution(int P, int K) {
+  std::vector<double> a = {0.0, 1.0};
+  while ((int)a.size() < P) {
+    const int n = (int)a.size();
+    std::vector<double> b(n + 1);
+    for (int j = 0; j <= n; ++j) {
+      const double left = j < n ? a[j] : 0.0, right = j >= 1 ? a[j - 1] : 0.0;
+      b[j] = ((0.0 + j) * left + (n + 1 - 0.0 - j) * right) / n;
+    }
+    a = b;
+  }
+  std::vector<double> D(K);
+  for (int m = 0; m < K; ++m) {
+    double re = 0, im = 0;
+    for (int j = 0; j < P; ++j) {
+      re += a[j] * std::cos(2 * M_PI * m * j / K);
+      im += a[j] * std::sin(2 * M_PI * m * j / K);
+    }
+    const double den = re * re + im * im;
+    D[m] = den > 1e-12 ? 1.0 / den : NAN;
+  }
+  for (int m = 0; m < K; ++m)
+    if (std::isnan(D[m])) D[m] = 0.5 * (D[(m - 1 + K) % K] + D[(m + 1) % K]);
+  return D;
+}
+
+int lines_per_tile(int K) {
+  int T = 32;
+  // two ping-pong buffers within 60 KiB, leaving room for the reductions' static LDS
+  while (T > 1 && (size_t)2 * T * K * sizeof(float2) > 61440) T >>= 1;
+  return T;
+}
+
+int compute_influence(pdx_mesh_t m) {
+  const int K1 = m->K1, K2 = m->K2, K3 = m->K3, K1h = m->K1h;
+  const double L1 = m->box[0], L2 = m->box[1], L3 = m->box[2];
+  const double CoverV = m->C / (L1 * L2 * L3);
+  const std::vector<double> D1 = deconvolution(m->P, K1), D2 = deconvolution(m->P, K2), D3 = deconvolution(m->P, K3);
+  std::vector<float> G((size_t)K1h * K2 * K3);
+  const double a2 = m->alpha * m->alpha;
+  for (int m3 = 0; m3 < K3; ++m3)
+    for (int m2 = 0; m2 < K2; ++m2)
+      for (int m1 = 0; m1 < K1h; ++m1) {
+        const size_t idx = (size_t)m1 + (size_t)K1h * (m2 + (size_t)K2 * m3);
+        const int s2 = m2 <= K2 / 2 ? m2 : m2 - K2, s3 = m3 <= K3 / 2 ? m3 : m3 - K3;
+        if (m1 == 0 && s2 == 0 && s3 == 0) { G[idx] = 0.f; continue; }
+        const double kx = 2 * M_PI * m1 / L1, ky = 2 * M_PI * s2 / L2, kz = 2 * M_PI * s3 / L3;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        const double g = 4.0 * M_PI * std::exp(-k2 / (4.0 * a2)) / k2;  // spectral_g (Splits.jl:220-224)
+        G[idx] = (float)(CoverV * g * D1[m1] * D2[m2] * D3[m3]);
+      }
+  PDX_HIP(hipMemcpy(m->G.p, G.data(), sizeof(float) * G.size(), hipMemcpyHostToDevice));
+  return PDX_OK;
+}
+
+int upload_twiddles(pdx_mesh_t m) {
+  std::vector<float2> tw(m->K1 + m->K2 + m->K3);
+  size_t o = 0;
+  for (int K : {m->K1, m->K2, m->K3})
+    for (int k = 0; k < K; ++k, ++o) tw[o] = make_float2((float)std::cos(2 * M_PI * k / K), (float)-std::sin(2 * M_PI * k / K));
+  PDX_HIP(hipMemcpy(m->tw.p, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
+  return PDX_OK;
+}
+
+void refresh_background(pdx_mesh_t m) {
+  const double V = m->box[0] * m->box[1] * m->box[2];
+  const bool bgon = m->spec.net_charge == PDX_NET_CHARGE_BACKGROUND;
+  m->bg = bgon ? (float)(-m->C * m->bgcoef * m->qtotal / V) : 0.0f;
+  m->ubg = bgon ? -m->C * m->bgcoef * m->qtotal * m->qtotal / V : 0.0;
+}
+
+struct Plans {
+  FFTPlan p1, p2, p3;
+  int T1, T2, T3;
+};
+
+Plans plans_of(pdx_mesh_t m) {
+  Plans pl;
+  make_plan(m->K1, pl.p1);
+  make_plan(m->K2, pl.p2);
+  make_plan(m->K3, pl.p3);
+  pl.T1 = lines_per_tile(m->K1);
+  pl.T2 = lines_per_tile(m->K2);
+  pl.T3 = lines_per_tile(m->K3);
+  return pl;
+}
+
+SolveArgs solve_args(pdx_mesh_t m) {
+  SolveArgs s{};
+  s.G = m->G.p;
+  s.K1 = m->K1; s.K2 = m->K2; s.K3 = m->K3; s.K1h = m->K1h;
+  for (int d = 0; d < 3; ++d) s.twopi_L[d] = 2 * M_PI / m->box[d];
+  s.inv4a2 = 1.0 / (4.0 * m->alpha * m->alpha);
+  s.partial = m->partial.p;
+  s.ticket = m->ticket.p;
+  s.acc = m->acc.p;
+  return s;
+}
+
+int enqueue_forward_xy(pdx_mesh_t m, const Plans &pl, const void *src, bool fixed, hipStream_t st) {
+  const float2 *tw1 = m->tw.p, *tw2 = m->tw.p + m->K1;
+  const int nlines = m->K2 * m->K3;
+  const size_t sh1 = (size_t)2 * pl.T1 * m->K1 * sizeof(float2);
+  const int gx = (nlines + pl.T1 - 1) / pl.T1;
+  if (fixed)
+    hipLaunchKernelGGL(k_fft_x_r2c<true>, dim3(gx), dim3(TPB), sh1, st, src, m->grid64.p, m->spec_c.p, m->K1,
+                       m->K1h, nlines, pl.T1, pl.p1, tw1, m->fixed_inv);
+  else
+    hipLaunchKernelGGL(k_fft_x_r2c<false>, dim3(gx), dim3(TPB), sh1, st, src, nullptr, m->spec_c.p, m->K1, m->K1h,
+                       nlines, pl.T1, pl.p1, tw1, 1.0);
+  const int gy = ((m->K1h + pl.T2 - 1) / pl.T2) * m->K3;
+  hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(TPB), (size_t)2 * pl.T2 * m->K2 * sizeof(float2), st, m->spec_c.p,
+                     m->K1h, m->K2, pl.T2, pl.p2, tw2, -1.0f);
+  PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
+int enqueue_inverse_yx(pdx_mesh_t m, const Plans &pl, float *dst, hipStream_t st) {
+  const float2 *tw1 = m->tw.p, *tw2 = m->tw.p + m->K1;
+  const int gy = ((m->K1h + pl.T2 - 1) / pl.T2) * m->K3;
+  hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(TPB), (size_t)2 * pl.T2 * m->K2 * sizeof(float2), st, m->spec_c.p,
+                     m->K1h, m->K2, pl.T2, pl.p2, tw2, +1.0f);
+  const int nlines = m->K2 * m->K3;
+  hipLaunchKernelGGL(k_fft_x_c2r, dim3((nlines + pl.T1 - 1) / pl.T1), dim3(TPB),
+                     (size_t)2 * pl.T1 * m->K1 * sizeof(float2), st, m->spec_c.p, dst, m->K1, m->K1h, nlines, pl.T1,
+                     pl.p1, tw1);
+  PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
+template <int MODE, bool ENERGY>
+void launch_z(pdx_mesh_t m, const Plans &pl, hipStream_t st) {
+  const float2 *tw3 = m->tw.p + m->K1 + m->K2;
+  const int gz = ((m->K1h + pl.T3 - 1) / pl.T3) * m->K2;
+  hipLaunchKernelGGL((k_fft_z<MODE, ENERGY>), dim3(gz), dim3(TPB), (size_t)2 * pl.T3 * m->K3 * sizeof(float2), st,
+                     m->spec_c.p, pl.T3, pl.p3, tw3, solve_args(m));
+}
+
+#define PDX_SPREAD_CASE(PP)                                                                                    \
+  case PP:                                                                                                     \
+    hipLaunchKernelGGL(k_spread<PP>, dim3(((int64_t)N * lanes_for(PP) + TPB - 1) / TPB), dim3(TPB), 0, st, N, \
+                       x, y, z, q, order, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale);         \
+    break;
+
+#define PDX_GATHER_CASE(PP)                                                                                     \
+  case PP:                                                                                                      \
+    if (energy)                                                                                                 \
+      hipLaunchKernelGGL((k_gather<PP, true>), dim3(blocks_of(PP)), dim3(TPB), 0, st, N, x, y, z, q, order,     \
+                         m->pot.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], fx, fy, fz, Epot, m->bg, set,         \
+                         Vglobal, m->acc.p, m->ubg);                                                           \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_gather<PP, false>), dim3(blocks_of(PP)), dim3(TPB), 0, st, N, x, y, z, q, order,    \
+                         m->pot.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], fx, fy, fz, Epot, m->bg, set,         \
+                         Vglobal, m->acc.p, m->ubg);                                                           \
+    break;
+
+}  // namespace
+
+extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const double box[3], pdx_mesh_t *out) {
+  if (!spec || !box || !out) return fail(PDX_ERR_ARGUMENT, "null argument");
+  *out = nullptr;
+  const pdx_mesh_spec &sp = *spec;
+  if (!(std::isfinite(sp.coulomb_prefactor) && sp.coulomb_prefactor > 0))
+    return fail(PDX_ERR_ARGUMENT, "coulomb_prefactor must be finite and positive (it carries the caller's E*L/Q^2 unit).");
+  if (!(std::isfinite(sp.relative_permittivity) && sp.relative_permittivity > 0))
+    return fail(PDX_ERR_ARGUMENT, "relative_permittivity must be finite and positive.");
+  if (!(std::isfinite(sp.real_cutoff) && sp.real_cutoff > 0))
+    return fail(PDX_ERR_ARGUMENT, "real_cutoff must be finite and positive.");
+  if (sp.influence != PDX_INFLUENCE_SPME) return fail(PDX_ERR_ARGUMENT, "Only the SPME influence function is implemented.");
+  if (sp.order < 3 || sp.order > 12)
+    return fail(PDX_ERR_ARGUMENT, "B-spline assignment order must be between 3 and 12.");
+  if (sp.order > 8) return fail(PDX_ERR_ARGUMENT, "Assignment orders above 8 are not compiled in this build.");
+  if (sp.net_charge != PDX_NET_CHARGE_ERROR && sp.net_charge != PDX_NET_CHARGE_BACKGROUND)
+    return fail(PDX_ERR_ARGUMENT, "net_charge must be :error or :background.");
+  if (sp.real_space_modifier != 0 && sp.real_space_modifier != 1)
+    return fail(PDX_ERR_ARGUMENT, "real_space_modifier must be :none or :potential_shift.");
+  if (n <= 0) return fail(PDX_ERR_ARGUMENT, "particle count must be positive");
+  for (int d = 0; d < 3; ++d)
+    if (!(std::isfinite(box[d]) && box[d] > 0)) return fail(PDX_ERR_ARGUMENT, "box lengths must be finite and positive");
+  const double mn = std::min(box[0], std::min(box[1], box[2]));
+  if (!(2 * sp.real_cutoff < mn))
+    return fail(PDX_ERR_ARGUMENT, "real_cutoff must be smaller than half the shortest box length (minimum-image real space).");
+  double alpha = sp.alpha;
+  if (!(alpha > 0)) {
+    const double tol = sp.real_space_tolerance;
+    if (!(tol > 0 && tol < 1)) return fail(PDX_ERR_ARGUMENT, "real_space_tolerance must lie in (0, 1).");
+    double lo = 0.0, hi = 10.0;  // alpha_from_tolerance (Spec.jl:154-163): bisection of erfc(x) = tol
+    for (int it = 0; it < 200; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      (host_erfc(mid) - tol > 0) ? lo = mid : hi = mid;
+    }
+    alpha = 0.5 * (lo + hi) / sp.real_cutoff;
+  }
+  if (!(std::isfinite(alpha) && alpha > 0)) return fail(PDX_ERR_ARGUMENT, "Ewald alpha must be finite and positive.");
+  int K[3];
+  for (int d = 0; d < 3; ++d) {
+    if (sp.mesh[0] > 0) {
+      K[d] = sp.mesh[d];
+    } else {
+      if (!(std::isfinite(sp.mesh_spacing) && sp.mesh_spacing > 0))
+        return fail(PDX_ERR_ARGUMENT, "Provide either `mesh=(K1,K2,K3)` or `mesh_spacing`.");
+      K[d] = std::max(fft_friendly((int)std::ceil(box[d] / sp.mesh_spacing - 1e-9)), sp.order);
+    }
+    if (K[d] < sp.order) return fail(PDX_ERR_ARGUMENT, "Every mesh dimension must be at least the assignment order.");
+    FFTPlan pl;
+    if (!make_plan(K[d], pl))
+      return fail(PDX_ERR_ARGUMENT, "Mesh dimensions must have prime factors in {2, 3, 5, 7} (fft_friendly).");
+    if (K[d] > 4096) return fail(PDX_ERR_ARGUMENT, "Mesh dimensions above 4096 are not supported.");
+  }
+  const int64_t nnodes = (int64_t)K[0] * K[1] * K[2];
+  if (nnodes >= INT32_MAX) return fail(PDX_ERR_ARGUMENT, "Mesh is too large for 32-bit grid indexing.");
+  auto *m = new pdx_mesh_s();
+  m->spec = sp;
+  m->N = n;
+  m->K1 = K[0]; m->K2 = K[1]; m->K3 = K[2]; m->K1h = K[0] / 2 + 1;
+  m->P = sp.order;
+  for (int d = 0; d < 3; ++d) m->box[d] = box[d];
+  m->alpha = alpha;
+  m->C = sp.coulomb_prefactor / sp.relative_permittivity;
+  m->selfc = 2.0 * alpha / std::sqrt(M_PI);      // self_coefficient (Splits.jl:67)
+  m->bgcoef = M_PI / (2.0 * alpha * alpha);       // background_coefficient (Splits.jl:85)
+  const size_t nmodes = (size_t)m->K1h * m->K2 * m->K3;
+  const Plans pl = plans_of(m);
+  m->nmode_blocks = ((m->K1h + pl.T3 - 1) / pl.T3) * m->K2;
+  int rc;
+  if ((rc = m->grid64.alloc(nnodes)) || (rc = m->spec_c.alloc(nmodes)) || (rc = m->pot.alloc(nnodes)) ||
+      (rc = m->G.alloc(nmodes)) || (rc = m->tw.alloc(K[0] + K[1] + K[2])) || (rc = m->acc.alloc(8)) ||
+      (rc = m->partial.alloc((size_t)7 * m->nmode_blocks)) || (rc = m->ticket.alloc(1))) {
+    delete m;
+    return rc;
+  }
+  if (hipMemset(m->grid64.p, 0, sizeof(unsigned long long) * nnodes) != hipSuccess ||
+      hipMemset(m->acc.p, 0, sizeof(double) * 8) != hipSuccess ||
+      hipMemset(m->ticket.p, 0, sizeof(unsigned)) != hipSuccess ||
+      hipMemset(m->pot.p, 0, sizeof(float) * nnodes) != hipSuccess) {
+    delete m;
+    return fail(PDX_ERR_DEVICE, "mesh initialisation failed");
+  }
+  if ((rc = compute_influence(m)) || (rc = upload_twiddles(m))) {
+    delete m;
+    return rc;
+  }
+  m->fixed_scale = std::ldexp(1.0, 40);
+  m->fixed_inv = std::ldexp(1.0, -40);
+  refresh_background(m);
+  if (hipDeviceSynchronize() != hipSuccess) {
+    delete m;
+    return fail(PDX_ERR_DEVICE, "mesh initialisation failed");
+  }
+  *out = m;
+  return PDX_OK;
+}
+
+extern "C" int pdx_mesh_destroy(pdx_mesh_t m) {
+  delete m;
+  return PDX_OK;
+}
+
+extern "C" int pdx_mesh_info(pdx_mesh_t m, double out[8]) {
+  if (!m || !out) return fail(PDX_ERR_ARGUMENT, "null argument");
+  out[0] = m->K1; out[1] = m->K2; out[2] = m->K3; out[3] = m->P;
+  out[4] = m->alpha; out[5] = m->C; out[6] = m->selfc; out[7] = m->bg;
+  return PDX_OK;
+}
+
+extern "C" int pdx_mesh_set_charges(pdx_mesh_t m, const float *q, int32_t n) {
+  if (!m || (!q && n > 0)) return fail(PDX_ERR_ARGUMENT, "null argument");
+  if (n != m->N) return fail(PDX_ERR_DIMENSION, "charge count differs from the mesh particle count");
+  double Q = 0, qabs = 0, qsq = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!std::isfinite(q[i])) return fail(PDX_ERR_ARGUMENT, "Charges must be finite.");
+    Q += q[i];
+    qabs += std::fabs((double)q[i]);
+    qsq += (double)q[i] * q[i];
+  }
+  if (m->spec.net_charge == PDX_NET_CHARGE_ERROR && std::fabs(Q) > m->spec.net_charge_tolerance * std::max(qabs, 1.0))
+    return fail(PDX_ERR_ARGUMENT,
+                "The system carries net charge. Neutralise it, or select net_charge=:background to add an explicit "
+                "uniform neutralising background.");
+  m->qtotal = Q;
+  m->qabs = qabs;
+  m->qsq = qsq;
+  // fixed-point scale 2^S: every node sum is bounded by sum |q| (weights sum to 1 per
+  // particle), so |node| * 2^S < 2^62 cannot overflow; S <= 40 keeps 2^-40 resolution.
+  int S = (int)std::floor(62.0 - std::log2(std::max(qabs, 1.0)));
+  S = std::min(S, 40);
+  m->fixed_scale = std::ldexp(1.0, S);
+  m->fixed_inv = std::ldexp(1.0, -S);
+  refresh_background(m);
+  return PDX_OK;
+}
+
+extern "C" int pdx_mesh_set_box(pdx_mesh_t m, const double box[3]) {
+  if (!m || !box) return fail(PDX_ERR_ARGUMENT, "null argument");
+  for (int d = 0; d < 3; ++d)
+    if (!(std::isfinite(box[d]) && box[d] > 0)) return fail(PDX_ERR_ARGUMENT, "box lengths must be finite and positive");
+  const double mn = std::min(box[0], std::min(box[1], box[2]));
+  if (!(2 * m->spec.real_cutoff < mn)) return fail(PDX_ERR_ARGUMENT, "real_cutoff must stay below half the shortest box length.");
+  PDX_HIP(hipDeviceSynchronize());
+  for (int d = 0; d < 3; ++d) m->box[d] = box[d];
+  int rc = compute_influence(m);
+  if (rc) return rc;
+  refresh_background(m);
+  return PDX_OK;
+}
+
+extern "C" int pdx_mesh_spread(pdx_mesh_t m, const float *x, const float *y, const float *z, const float *q,
+                               const int32_t *order, pdx_stream_t stream) {
+  if (!m || !x || !y || !z || !q) return fail(PDX_ERR_ARGUMENT, "null argument");
+  const int N = m->N;
+  const float L[3] = {(float)m->box[0], (float)m->box[1], (float)m->box[2]};
+  const float scale = (float)m->fixed_scale;
+  hipStream_t st = as_stream(stream);
+  switch (m->P) {
+    PDX_SPREAD_CASE(3)
+    PDX_SPREAD_CASE(4)
+    PDX_SPREAD_CASE(5)
+    PDX_SPREAD_CASE(6)
+    PDX_SPREAD_CASE(7)
+    PDX_SPREAD_CASE(8)
+    default: return fail(PDX_ERR_ARGUMENT, "unsupported order");
+  }
+  PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
+extern "C" int pdx_mesh_solve(pdx_mesh_t m, int32_t flags, pdx_stream_t stream) {
+  if (!m) return fail(PDX_ERR_ARGUMENT, "null mesh");
+  hipStream_t st = as_stream(stream);
+  const Plans pl = plans_of(m);
+  int rc = enqueue_forward_xy(m, pl, m->grid64.p, true, st);
+  if (rc) return rc;
+  if (flags & PDX_ENERGY) launch_z<2, true>(m, pl, st);
+  else launch_z<2, false>(m, pl, st);
+  PDX_LAUNCH_CHECK();
+  return enqueue_inverse_yx(m, pl, m->pot.p, st);
+}
+
+extern "C" int pdx_mesh_gather(pdx_mesh_t m, const float *x, const float *y, const float *z, const float *q,
+                               const int32_t *order, float *fx, float *fy, float *fz, float *Epot, float *Vglobal,
+                               int32_t flags, int32_t set, pdx_stream_t stream) {
+  if (!m || !x || !y || !z || !q || !fx || !fy || !fz) return fail(PDX_ERR_ARGUMENT, "null argument");
+  const int N = m->N;
+  const float L[3] = {(float)m->box[0], (float)m->box[1], (float)m->box[2]};
+  const bool energy = (flags & PDX_ENERGY) != 0;
+  if (!energy) Vglobal = nullptr;
+  hipStream_t st = as_stream(stream);
+  auto blocks_of = [&](int P) { return (int)(((int64_t)N * lanes_for(P) + TPB - 1) / TPB); };
+  switch (m->P) {
+    PDX_GATHER_CASE(3)
+    PDX_GATHER_CASE(4)
+    PDX_GATHER_CASE(5)
+    PDX_GATHER_CASE(6)
+    PDX_GATHER_CASE(7)
+    PDX_GATHER_CASE(8)
+    default: return fail(PDX_ERR_ARGUMENT, "unsupported order");
+  }
+  PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
+extern "C" int pdx_mesh_totals(pdx_mesh_t m, double out[9], pdx_stream_t stream) {
+  if (!m || !out) return fail(PDX_ERR_ARGUMENT, "null argument");
+  PDX_HIP(hipStreamSynchronize(as_stream(stream)));
+  double acc[8];
+  PDX_HIP(hipMemcpy(acc, m->acc.p, sizeof(acc), hipMemcpyDeviceToHost));
+  out[0] = acc[0];
+  for (int c = 0; c < 6; ++c) out[1 + c] = acc[1 + c] + (c < 3 ? m->ubg : 0.0);
+  out[7] = -0.5 * m->C * m->selfc * m->qsq;  // self_energy (Solver.jl:421-423)
+  out[8] = m->ubg;                           // background_energy (Solver.jl:427-429)
+  return PDX_OK;
+}
+
+extern "C" int pdx_mesh_copy_grid(pdx_mesh_t m, int32_t which, void *dst, pdx_stream_t stream) {
+  if (!m || !dst) return fail(PDX_ERR_ARGUMENT, "null argument");
+  const size_t nn = (size_t)m->K1 * m->K2 * m->K3;
+  hipStream_t st = as_stream(stream);
+  if (which == 0) {
+    hipLaunchKernelGGL(k_fixed_to_double, dim3((nn + TPB - 1) / TPB), dim3(TPB), 0, st, m->grid64.p, (double *)dst, nn,
+                       m->fixed_inv);
+    PDX_LAUNCH_CHECK();
+  } else if (which == 1) {
+    PDX_HIP(hipMemcpyAsync(dst, m->pot.p, sizeof(float) * nn, hipMemcpyDeviceToDevice, st));
+  } else {
+    return fail(PDX_ERR_ARGUMENT, "which must be 0 or 1");
+  }
+  return PDX_OK;
+}
+
+extern "C" int pdx_mesh_fft(pdx_mesh_t m, int32_t dir, const void *in, void *out, pdx_stream_t stream) {
+  if (!m || !in || !out) return fail(PDX_ERR_ARGUMENT, "null argument");
+  hipStream_t st = as_stream(stream);
+  const Plans pl = plans_of(m);
+  const size_t nmodes = (size_t)m->K1h * m->K2 * m->K3;
+  if (dir == -1) {
+    int rc = enqueue_forward_xy(m, pl, in, false, st);
+    if (rc) return rc;
+    launch_z<0, false>(m, pl, st);
+    PDX_LAUNCH_CHECK();
+    PDX_HIP(hipMemcpyAsync(out, m->spec_c.p, sizeof(float2) * nmodes, hipMemcpyDeviceToDevice, st));
+    return PDX_OK;
+  }
+  if (dir == 1) {
+    PDX_HIP(hipMemcpyAsync(m->spec_c.p, in, sizeof(float2) * nmodes, hipMemcpyDeviceToDevice, st));
+    launch_z<1, false>(m, pl, st);
+    PDX_LAUNCH_CHECK();
+    return enqueue_inverse_yx(m, pl, (float *)out, st);
+  }
+  return fail(PDX_ERR_ARGUMENT, "dir must be -1 (R2C) or +1 (C2R)");
+}

@@ -1,0 +1,683 @@
+// Cell-list binning and slot-major ELL row construction on gfx950.
+//
+// Pipeline of one predicated rebuild (all kernels are enqueued every time; each returns at
+// once unless the device state requests a rebuild, so the sequence can be captured in a
+// hipGraph — the "predicated reference path" of src/neighbors/DeviceNeighborCUDA.jl:413-420):
+//   k_decide       coverage check: max |dr|^2 vs skin^2/4, adaptive interval, packing of
+//                  (x, y, z, q), zeroing of the cell histogram (one launch, last-block decide)
+//   k_bin          cell id + histogram; the atomic's return value is the particle's slot
+//   k_scan_local / k_scan_blocks / k_scan_finish   exclusive scan -> 1-based cell_offsets
+//   k_scatter      particle_ids_sorted (1-based)
+//   k_sort_cells   ascending id inside each cell (rows and traversal become reproducible),
+//                  and the cell-ordered packed positions read by the row kernel
+//   k_rows         27-cell stencil, r^2 <= rlist^2, slot-major rows t*N + i, 1-based ids
+//   k_commit       max row length vs cap, state words (last-block commit)
+//   k_copy_refs    reference positions of the successful build
+#include "pdx_internal.hpp"
+
+#include <cmath>
+
+using namespace pdx;
+
+namespace {
+
+constexpr int TPB = 256;
+constexpr int SCAN_BLOCK = 1024;  // cells per block of the local scan (256 threads x 4)
+
+__device__ __forceinline__ bool rebuild_requested(const int64_t *s) {
+  return s[PDX_DN_REQUEST] != 0 && s[PDX_DN_ERROR] == 0;
+}
+
+// ------------------------------------------------------------------ decide
+// Coverage check (_device_displacement_kernel! + _device_decide_kernel!,
+// DeviceNeighborCUDA.jl:169-236) fused into one launch with a last-block decision.
+__global__ __launch_bounds__(TPB) void k_decide(int N, const float *__restrict__ x,
+                                                 const float *__restrict__ y,
+                                                 const float *__restrict__ z,
+                                                 const float *__restrict__ q,
+                                                 const float *__restrict__ rref, float4 *packed,
+                                                 int32_t *cell_counts, int ncell, int allpairs,
+                                                 const float *geom, int64_t *state,
+                                                 float *partial, unsigned *ticket,
+                                                 float threshold, int force) {
+  const int64_t err = state[PDX_DN_ERROR];
+  if (err != 0) return;  // latched: every block leaves (uniform; the ticket is not touched)
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  const float L[3] = {geom[0], geom[1], geom[2]};
+  const bool valid = state[PDX_DN_VALID] != 0;
+  float value = 0.0f;
+  if (i < N) {
+    const float c[3] = {x[i], y[i], z[i]};
+    // z, then y, then x: the nested muladd order of the reference (threshold ties included)
+    for (int d = 2; d >= 0; --d) {
+      const float h = 0.5f * L[d];
+      if (!(isfinite(c[d]) && -h <= c[d] && c[d] <= h)) {
+        value = INFINITY;
+      } else if (valid && value != INFINITY) {
+        float dx = mic_fast(c[d] - rref[(size_t)d * N + i], h, L[d]);
+        value = fmaf(dx, dx, value);
+      }
+    }
+    packed[i] = make_float4(c[0], c[1], c[2], q ? q[i] : 0.0f);
+  }
+  // histogram storage is scratch of the next rebuild: clear it here (one launch fewer)
+  if (!allpairs)
+    for (int c = i; c < ncell; c += gridDim.x * TPB) cell_counts[c] = 0;
+
+  __shared__ float smax[TPB / WAVE];
+  __shared__ bool last;
+  float m = wave_max(value);
+  if ((threadIdx.x & (WAVE - 1)) == 0) smax[threadIdx.x / WAVE] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float b = smax[0];
+    for (int w = 1; w < TPB / WAVE; ++w) b = fmaxf(b, smax[w]);
+    partial[blockIdx.x] = b;
+    __threadfence();
+    last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  float v = 0.0f;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += TPB) v = fmaxf(v, __builtin_nontemporal_load(&partial[b]));
+  v = wave_max(v);
+  if ((threadIdx.x & (WAVE - 1)) == 0) smax[threadIdx.x / WAVE] = v;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  float largest = smax[0];
+  for (int w = 1; w < TPB / WAVE; ++w) largest = fmaxf(largest, smax[w]);
+  *ticket = 0u;
+  state[PDX_DN_CHECKS] += 1;
+  state[PDX_DN_REQUEST] = 0;
+  const bool diag = state[PDX_DN_DIAGNOSTICS] != 0;
+  if (diag) state[PDX_DN_REASON] = 0;
+  if (!isfinite(largest)) {
+    state[PDX_DN_ERROR] = PDX_DN_COORDINATE_ERROR;
+    state[PDX_DN_VALID] = 0;
+    return;
+  }
+  if (force || !valid) {
+    state[PDX_DN_REQUEST] = 1;
+    state[PDX_DN_VALID] = 0;
+    if (diag) state[PDX_DN_REASON] = PDX_DN_REASON_INVALID;
+    return;
+  }
+  const int64_t age = state[PDX_DN_STEP] - state[PDX_DN_LAST_BUILD];
+  const bool disp = largest > threshold;
+  const bool old = age >= state[PDX_DN_INTERVAL];
+  if (disp || old) {
+    state[PDX_DN_REQUEST] = 1;
+    state[PDX_DN_VALID] = 0;
+    if (diag) state[PDX_DN_REASON] = (disp ? PDX_DN_REASON_DISPLACEMENT : 0) | (old ? PDX_DN_REASON_AGE : 0);
+    // adaptive interval: x0.9 on displacement, x1.1 when the list aged out early, in [5, 100]
+    if (disp) {
+      float t = rintf(0.9f * (float)state[PDX_DN_INTERVAL]);
+      state[PDX_DN_INTERVAL] = t < 5.0f ? 5 : (int64_t)t;
+    } else if (largest < 0.1f * threshold) {
+      float t = rintf(1.1f * (float)state[PDX_DN_INTERVAL]);
+      state[PDX_DN_INTERVAL] = t > 100.0f ? 100 : (int64_t)t;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ binning
+// Cell id from the current box, fractional binning floor(x n/L) (_kernel_cell_ids3g!,
+// DeviceNeighborCUDA.jl:464-481) + histogram (_kernel_cell_histogram!, DenseNeighborCUDA.jl:50).
+__global__ __launch_bounds__(TPB) void k_bin(int N, const float4 *__restrict__ packed,
+                                              const float *geom, int nx, int ny, int nz,
+                                              int32_t *cell_of_particle, int32_t *cell_counts,
+                                              int32_t *cell_rank, const int64_t *state) {
+  if (!rebuild_requested(state)) return;
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i >= N) return;
+  const float4 p = packed[i];
+  const float Lx = geom[0], Ly = geom[1], Lz = geom[2];
+  float xx = p.x + 0.5f * Lx; xx -= floorf(xx / Lx) * Lx;
+  float yy = p.y + 0.5f * Ly; yy -= floorf(yy / Ly) * Ly;
+  float zz = p.z + 0.5f * Lz; zz -= floorf(zz / Lz) * Lz;
+  int cx = (int)floorf(xx * ((float)nx / Lx)); cx = min(max(cx, 0), nx - 1);
+  int cy = (int)floorf(yy * ((float)ny / Ly)); cy = min(max(cy, 0), ny - 1);
+  int cz = (int)floorf(zz * ((float)nz / Lz)); cz = min(max(cz, 0), nz - 1);
+  const int c = (cz * ny + cy) * nx + cx;
+  cell_of_particle[i] = c;
+  cell_rank[i] = atomicAdd(&cell_counts[c], 1);
+}
+
+// Exclusive scan of cell_counts, SCAN_BLOCK cells per block (block-local part).
+__global__ __launch_bounds__(TPB) void k_scan_local(int ncell, const int32_t *counts,
+                                                     int32_t *offsets, int32_t *block_totals,
+                                                     const int64_t *state) {
+  if (!rebuild_requested(state)) return;
+  __shared__ int32_t sh[TPB];
+  const int base = blockIdx.x * SCAN_BLOCK + threadIdx.x * 4;
+  int32_t v[4];
+  int32_t run = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = (base + k < ncell) ? counts[base + k] : 0;
+    run += v[k];
+  }
+  sh[threadIdx.x] = run;
+  __syncthreads();
+  for (int off = 1; off < TPB; off <<= 1) {
+    int32_t a = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += a;
+    __syncthreads();
+  }
+  int32_t excl = sh[threadIdx.x] - run;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < ncell) offsets[base + k] = excl;
+    excl += v[k];
+  }
+  if (threadIdx.x == TPB - 1) block_totals[blockIdx.x] = sh[TPB - 1];
+}
+
+// Exclusive scan of the block totals (one block, sequential over chunks of 1024).
+__global__ __launch_bounds__(1024) void k_scan_blocks(int nblocks, int32_t *block_totals,
+                                                       const int64_t *state) {
+  if (!rebuild_requested(state)) return;
+  __shared__ int32_t sh[1024];
+  int32_t carry = 0;
+  for (int base = 0; base < nblocks; base += 1024) {
+    const int k = base + threadIdx.x;
+    const int32_t v = k < nblocks ? block_totals[k] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      int32_t a = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += a;
+      __syncthreads();
+    }
+    if (k < nblocks) block_totals[k] = carry + sh[threadIdx.x] - v;
+    const int32_t tot = sh[1023];
+    __syncthreads();
+    carry += tot;
+  }
+}
+
+// 1-based offsets with last entry N+1 (_device_offsets_kernel!, DeviceNeighborCUDA.jl:280-289).
+__global__ __launch_bounds__(TPB) void k_scan_finish(int ncell, int N, int32_t *offsets,
+                                                      const int32_t *block_totals,
+                                                      const int64_t *state) {
+  if (!rebuild_requested(state)) return;
+  const int c = blockIdx.x * TPB + threadIdx.x;
+  if (c < ncell) offsets[c] += block_totals[c / SCAN_BLOCK] + 1;
+  if (c == ncell) offsets[ncell] = N + 1;
+}
+
+__global__ __launch_bounds__(TPB) void k_scatter(int N, const int32_t *cell_of_particle,
+                                                  const int32_t *cell_rank,
+                                                  const int32_t *offsets, int32_t *ids_sorted,
+                                                  const int64_t *state) {
+  if (!rebuild_requested(state)) return;
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i >= N) return;
+  const int c = cell_of_particle[i];
+  ids_sorted[offsets[c] - 1 + cell_rank[i]] = i + 1;
+}
+
+// Stable order inside each cell (ascending id; cells hold tens of particles) and the
+// cell-ordered packed copy (x, y, z, id) the row kernel streams through.
+__global__ __launch_bounds__(TPB) void k_sort_cells(int ncell, const int32_t *offsets,
+                                                     int32_t *ids_sorted,
+                                                     const float4 *__restrict__ packed,
+                                                     float4 *sorted, const int64_t *state) {
+  if (!rebuild_requested(state)) return;
+  const int c = blockIdx.x * TPB + threadIdx.x;
+  if (c >= ncell) return;
+  const int s = offsets[c] - 1, e = offsets[c + 1] - 1;
+  for (int a = s + 1; a < e; ++a) {
+    const int32_t key = ids_sorted[a];
+    int b = a - 1;
+    while (b >= s && ids_sorted[b] > key) {
+      ids_sorted[b + 1] = ids_sorted[b];
+      --b;
+    }
+    ids_sorted[b + 1] = key;
+  }
+  for (int k = s; k < e; ++k) {
+    const int i = ids_sorted[k] - 1;
+    const float4 p = packed[i];
+    sorted[k] = make_float4(p.x, p.y, p.z, __int_as_float(i));
+  }
+}
+
+// ------------------------------------------------------------------ rows
+// One thread per particle in cell order (threads of a wave share their home cell, so the
+// candidate loads are broadcast), 27-cell stencil, mic_fast differences and the reference's
+// r^2 = muladd(dx,dx,muladd(dy,dy,dz^2)) <= rl^2 test (_kernel_neighbors3!,
+// DenseNeighborCUDA.jl:153-206). Row slots are written slot-major at t*N + i with 1-based
+// ids; the count keeps growing past cap so the commit can report the required capacity.
+__global__ __launch_bounds__(TPB) void k_rows(int N, int cap, int nx, int ny, int nz,
+                                               const float4 *__restrict__ sorted,
+                                               const int32_t *__restrict__ offsets,
+                                               const int32_t *__restrict__ cell_of_particle,
+                                               const float *geom, float rl2, int32_t *nbr,
+                                               int32_t *counts, int64_t *candidates,
+                                               const int64_t *state) {
+  if (!rebuild_requested(state)) return;
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= N) return;
+  const float Lx = geom[0], Ly = geom[1], Lz = geom[2];
+  const float hx = 0.5f * Lx, hy = 0.5f * Ly, hz = 0.5f * Lz;
+  const float4 pi = sorted[t];
+  const int i = __float_as_int(pi.w);
+  const int c0 = cell_of_particle[i];
+  const int cx = c0 % nx, cy = (c0 / nx) % ny, cz = c0 / (nx * ny);
+  int found = 0;
+  int64_t tested = 0;
+  for (int oz = -1; oz <= 1; ++oz) {
+    int cz2 = cz + oz; cz2 -= (cz2 >= nz) ? nz : 0; cz2 += (cz2 < 0) ? nz : 0;
+    for (int oy = -1; oy <= 1; ++oy) {
+      int cy2 = cy + oy; cy2 -= (cy2 >= ny) ? ny : 0; cy2 += (cy2 < 0) ? ny : 0;
+      for (int ox = -1; ox <= 1; ++ox) {
+        int cx2 = cx + ox; cx2 -= (cx2 >= nx) ? nx : 0; cx2 += (cx2 < 0) ? nx : 0;
+        const int c = (cz2 * ny + cy2) * nx + cx2;
+        const int s = offsets[c] - 1, e = offsets[c + 1] - 1;
+        for (int k = s; k < e; ++k) {
+          const float4 pj = sorted[k];
+          const int j = __float_as_int(pj.w);
+          if (j == i) continue;
+          ++tested;
+          const float dx = mic_fast(pj.x - pi.x, hx, Lx);
+          const float dy = mic_fast(pj.y - pi.y, hy, Ly);
+          const float dz = mic_fast(pj.z - pi.z, hz, Lz);
+          const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          if (r2 <= rl2) {
+            if (found < cap) nbr[(size_t)found * N + i] = j + 1;
+            ++found;
+          }
+        }
+      }
+    }
+  }
+  counts[i] = found;
+  if (candidates) candidates[i] = tested;
+}
+
+// All-pairs rows for small boxes, one image (_kernel_neighbors_allpairs!, DenseNeighborCUDA.jl:128-151).
+__global__ __launch_bounds__(TPB) void k_rows_allpairs(int N, int cap,
+                                                        const float4 *__restrict__ packed,
+                                                        const float *geom, float rl2,
+                                                        int32_t *nbr, int32_t *counts,
+                                                        int64_t *candidates,
+                                                        const int64_t *state) {
+  if (!rebuild_requested(state)) return;
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i >= N) return;
+  const float Lx = geom[0], Ly = geom[1], Lz = geom[2];
+  const float4 pi = packed[i];
+  int found = 0;
+  for (int j = 0; j < N; ++j) {
+    if (j == i) continue;
+    const float4 pj = packed[j];
+    // dimension order of the reference's loop: r2 = muladd(dd, dd, r2) for d = 1..D
+    float r2 = 0.0f;
+    float dd = mic_fast(pj.x - pi.x, 0.5f * Lx, Lx); r2 = fmaf(dd, dd, r2);
+    dd = mic_fast(pj.y - pi.y, 0.5f * Ly, Ly); r2 = fmaf(dd, dd, r2);
+    dd = mic_fast(pj.z - pi.z, 0.5f * Lz, Lz); r2 = fmaf(dd, dd, r2);
+    if (r2 <= rl2) {
+      if (found < cap) nbr[(size_t)found * N + i] = j + 1;
+      ++found;
+    }
+  }
+  counts[i] = found;
+  if (candidates) candidates[i] = N - 1;
+}
+
+// ------------------------------------------------------------------ commit
+// Largest row vs cap, build counters (_device_capacity_partial_kernel! + _device_commit_kernel!,
+// DeviceNeighborCUDA.jl:291-388), one launch with a last-block commit.
+__global__ __launch_bounds__(TPB) void k_commit(int N, int cap, const int32_t *counts,
+                                                 const int64_t *candidates, int32_t *partial,
+                                                 int64_t *stat_partial, unsigned *ticket,
+                                                 int64_t *state) {
+  if (!rebuild_requested(state)) return;
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  const int v = i < N ? counts[i] : 0;
+  const int64_t cand = (candidates && i < N) ? candidates[i] : 0;
+  __shared__ int smax[TPB / WAVE];
+  __shared__ long long ssum[2][TPB / WAVE];
+  __shared__ bool last;
+  int m = wave_max(v);
+  long long sc = cand, sa = v;
+  for (int off = 32; off >= 1; off >>= 1) {
+    sc += __shfl_xor(sc, off, WAVE);
+    sa += __shfl_xor(sa, off, WAVE);
+  }
+  const int w = threadIdx.x / WAVE;
+  if ((threadIdx.x & (WAVE - 1)) == 0) { smax[w] = m; ssum[0][w] = sc; ssum[1][w] = sa; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int b = smax[0]; long long c0 = ssum[0][0], a0 = ssum[1][0];
+    for (int k = 1; k < TPB / WAVE; ++k) { b = max(b, smax[k]); c0 += ssum[0][k]; a0 += ssum[1][k]; }
+    partial[blockIdx.x] = b;
+    stat_partial[2 * blockIdx.x] = c0;
+    stat_partial[2 * blockIdx.x + 1] = a0;
+    __threadfence();
+    last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last || threadIdx.x != 0) return;
+  __threadfence();
+  int largest = 0;
+  long long ctot = 0, atot = 0;
+  for (unsigned b = 0; b < gridDim.x; ++b) {
+    largest = max(largest, __builtin_nontemporal_load(&partial[b]));
+    ctot += __builtin_nontemporal_load(&stat_partial[2 * b]);
+    atot += __builtin_nontemporal_load(&stat_partial[2 * b + 1]);
+  }
+  *ticket = 0u;
+  state[PDX_DN_REQUIRED] = largest;
+  const bool diag = state[PDX_DN_DIAGNOSTICS] != 0;
+  if (diag) {
+    state[PDX_DN_ATTEMPTS] += 1;
+    state[PDX_DN_LAST_CANDIDATES] = ctot;
+    state[PDX_DN_TOTAL_CANDIDATES] += ctot;
+    state[PDX_DN_LAST_ACCEPTED] = atot;
+    state[PDX_DN_TOTAL_ACCEPTED] += atot;
+    if (largest > state[PDX_DN_MAX_ROW]) state[PDX_DN_MAX_ROW] = largest;
+  }
+  if (largest > cap) {
+    state[PDX_DN_ERROR] = PDX_DN_CAPACITY_ERROR;
+    state[PDX_DN_VALID] = 0;
+    return;
+  }
+  state[PDX_DN_VALID] = 1;
+  state[PDX_DN_LAST_BUILD] = state[PDX_DN_STEP];
+  state[PDX_DN_BUILDS] += 1;
+  if (diag) {
+    const int64_t reason = state[PDX_DN_REASON];
+    if (reason & PDX_DN_REASON_INVALID) state[PDX_DN_INVALID_BUILDS] += 1;
+    else if (reason & PDX_DN_REASON_DISPLACEMENT) state[PDX_DN_DISPLACEMENT_BUILDS] += 1;
+    else state[PDX_DN_AGE_BUILDS] += 1;
+    if ((reason & (PDX_DN_REASON_DISPLACEMENT | PDX_DN_REASON_AGE)) ==
+        (PDX_DN_REASON_DISPLACEMENT | PDX_DN_REASON_AGE))
+      state[PDX_DN_OVERLAP_BUILDS] += 1;
+  }
+}
+
+__global__ __launch_bounds__(TPB) void k_copy_refs(int N, const float4 *__restrict__ packed,
+                                                    float *rref, const int64_t *state) {
+  if (!rebuild_requested(state)) return;
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i >= N) return;
+  const float4 p = packed[i];
+  rref[i] = p.x;
+  rref[(size_t)N + i] = p.y;
+  rref[(size_t)2 * N + i] = p.z;
+}
+
+__global__ void k_advance(int64_t *state) {
+  if (state[PDX_DN_VALID] != 0 && state[PDX_DN_ERROR] == 0) state[PDX_DN_STEP] += 1;
+}
+
+__global__ void k_rearm(int64_t *state) {
+  state[PDX_DN_ERROR] = 0;
+  state[PDX_DN_REQUEST] = 1;
+}
+
+__global__ void k_invalidate(int64_t *state) { state[PDX_DN_VALID] = 0; }
+
+inline int blocks_for(int64_t n, int tpb) { return (int)((n + tpb - 1) / tpb); }
+
+int grid_for_box(const double box[3], double rl, int32_t out[3]) {
+  for (int d = 0; d < 3; ++d) {
+    if (!(std::isfinite(box[d]) && box[d] > 0))
+      return fail(PDX_ERR_ARGUMENT, "Neighbor box lengths must be finite and positive.");
+    // floor(L / cell) in the list precision, as _choose_grid (NeighborTypes.jl:52-63)
+    const float n = std::floor((float)box[d] / (float)rl);
+    out[d] = n < 1 ? 1 : (int32_t)n;
+  }
+  return PDX_OK;
+}
+
+int enqueue_rebuild(pdx_nl_t nl, hipStream_t st) {
+  const int N = nl->N;
+  const int nb = blocks_for(N, TPB);
+  int64_t *s = nl->state.p;
+  const float *geom = nl->geom.p;
+  if (!nl->allpairs) {
+    const int sblocks = blocks_for(nl->ncell, SCAN_BLOCK);
+    hipLaunchKernelGGL(k_bin, dim3(nb), dim3(TPB), 0, st, N, nl->packed.p, geom, nl->nx, nl->ny, nl->nz,
+                       nl->cell_of_particle.p, nl->cell_counts.p, nl->cell_rank.p, s);
+    hipLaunchKernelGGL(k_scan_local, dim3(sblocks), dim3(TPB), 0, st, nl->ncell, nl->cell_counts.p,
+                       nl->cell_offsets.p, nl->scan_block.p, s);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, sblocks, nl->scan_block.p, s);
+    hipLaunchKernelGGL(k_scan_finish, dim3(blocks_for(nl->ncell + 1, TPB)), dim3(TPB), 0, st, nl->ncell, N,
+                       nl->cell_offsets.p, nl->scan_block.p, s);
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(TPB), 0, st, N, nl->cell_of_particle.p, nl->cell_rank.p,
+                       nl->cell_offsets.p, nl->particle_ids_sorted.p, s);
+    hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(nl->ncell, TPB)), dim3(TPB), 0, st, nl->ncell,
+                       nl->cell_offsets.p, nl->particle_ids_sorted.p, nl->packed.p, nl->sorted.p, s);
+    hipLaunchKernelGGL(k_rows, dim3(nb), dim3(TPB), 0, st, N, nl->cap, nl->nx, nl->ny, nl->nz, nl->sorted.p,
+                       nl->cell_offsets.p, nl->cell_of_particle.p, geom, nl->rl2, nl->neighbors_flat.p,
+                       nl->counts.p, nl->diagnostics ? nl->candidates.p : nullptr, s);
+  } else {
+    hipLaunchKernelGGL(k_rows_allpairs, dim3(nb), dim3(TPB), 0, st, N, nl->cap, nl->packed.p, geom,
+                       nl->rl2, nl->neighbors_flat.p, nl->counts.p,
+                       nl->diagnostics ? nl->candidates.p : nullptr, s);
+  }
+  hipLaunchKernelGGL(k_commit, dim3(nb), dim3(TPB), 0, st, N, nl->cap, nl->counts.p,
+                     nl->diagnostics ? nl->candidates.p : nullptr, nl->cap_partial.p, nl->stat_partial.p,
+                     nl->tickets.p + 1, s);
+  hipLaunchKernelGGL(k_copy_refs, dim3(nb), dim3(TPB), 0, st, N, nl->packed.p, nl->rref.p, s);
+  PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
+int enqueue_update(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
+                   hipStream_t st, int force) {
+  if (!nl) return fail(PDX_ERR_ARGUMENT, "null neighbor list");
+  if (!x || !y || !z) return fail(PDX_ERR_ARGUMENT, "null coordinate array");
+  const int N = nl->N;
+  const float s = (float)nl->skin;
+  const float threshold = (0.25f * s) * s;
+  hipLaunchKernelGGL(k_decide, dim3(blocks_for(N, TPB)), dim3(TPB), 0, st, N, x, y, z, q, nl->rref.p,
+                     nl->packed.p, nl->cell_counts.p, nl->ncell, (int)nl->allpairs, nl->geom.p,
+                     nl->state.p, nl->disp_partial.p, nl->tickets.p, threshold, force);
+  PDX_LAUNCH_CHECK();
+  return enqueue_rebuild(nl, st);
+}
+
+}  // namespace
+
+extern "C" int pdx_nl_create(int32_t n, const double box[3], double cutoff, double skin, int32_t cap,
+                             int32_t interval, int32_t flags, pdx_nl_t *out) {
+  if (!out || !box) return fail(PDX_ERR_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (!(n > 0 && n < INT32_MAX)) return fail(PDX_ERR_ARGUMENT, "Neighbor builders require 0 < N < typemax(Int32).");
+  if (cap <= 0) return fail(PDX_ERR_ARGUMENT, "Neighbor capacity must be positive.");
+  if (!(std::isfinite(cutoff) && cutoff >= 0)) return fail(PDX_ERR_ARGUMENT, "Neighbor cutoff must be finite and nonnegative.");
+  if (!(std::isfinite(skin) && skin >= 0)) return fail(PDX_ERR_ARGUMENT, "Neighbor skin must be finite and nonnegative.");
+  if (interval < 1 || interval > 100) return fail(PDX_ERR_ARGUMENT, "Rebuild interval must be between 1 and 100");
+  if ((int64_t)n * cap > INT32_MAX)
+    return fail(PDX_ERR_ARGUMENT, "Neighbor table of N particles x cap exceeds the Int32 index range; lower `cap`.");
+  auto *nl = new pdx_nl_s();
+  nl->N = n;
+  nl->cap = cap;
+  nl->interval = interval;
+  nl->cutoff = cutoff;
+  nl->skin = skin;
+  nl->allpairs = (flags & PDX_NL_ALLPAIRS) != 0;
+  nl->diagnostics = (flags & PDX_NL_DIAGNOSTICS) != 0;
+  for (int d = 0; d < 3; ++d) nl->box[d] = box[d];
+  const float rlf = std::max(1e-6f, (float)cutoff + (float)skin);
+  nl->rl2 = rlf * rlf;
+  int32_t g[3];
+  int rc = grid_for_box(box, rlf, g);
+  if (rc) { delete nl; return rc; }
+  if (nl->allpairs) {
+    const double mn = std::min(box[0], std::min(box[1], box[2]));
+    if (!(2.0 * rlf <= mn)) {
+      delete nl;
+      return fail(PDX_ERR_ARGUMENT, "All-pairs neighbor rows need cutoff+skin <= L/2 (minimum image).");
+    }
+    g[0] = g[1] = g[2] = 1;
+  } else if (g[0] < 3 || g[1] < 3 || g[2] < 3) {
+    delete nl;
+    return fail(PDX_ERR_ARGUMENT,
+                "Dense cell-list neighbor builder requires at least 3 cells per periodic dimension to avoid "
+                "duplicate neighbor-cell visits. Increase the box size, reduce cutoff+skin, or use all-pairs rows.");
+  }
+  nl->nx = g[0]; nl->ny = g[1]; nl->nz = g[2];
+  nl->ncell = g[0] * g[1] * g[2];
+  const int nb = blocks_for(n, TPB);
+  const int sblocks = blocks_for(nl->ncell, SCAN_BLOCK);
+  if ((rc = nl->state.alloc(PDX_DN_NSTATE)) || (rc = nl->geom.alloc(3)) ||
+      (rc = nl->neighbors_flat.alloc((size_t)n * cap)) || (rc = nl->counts.alloc(n)) ||
+      (rc = nl->cell_offsets.alloc(nl->ncell + 1)) || (rc = nl->cell_counts.alloc(nl->ncell)) ||
+      (rc = nl->particle_ids_sorted.alloc(n)) || (rc = nl->cell_of_particle.alloc(n)) ||
+      (rc = nl->cell_rank.alloc(n)) || (rc = nl->rref.alloc((size_t)3 * n)) || (rc = nl->packed.alloc(n)) ||
+      (rc = nl->sorted.alloc(n)) || (rc = nl->disp_partial.alloc(nb)) || (rc = nl->cap_partial.alloc(nb)) ||
+      (rc = nl->stat_partial.alloc((size_t)2 * nb)) || (rc = nl->scan_block.alloc(sblocks)) ||
+      (rc = nl->candidates.alloc(nl->diagnostics ? n : 1)) || (rc = nl->tickets.alloc(4))) {
+    delete nl;
+    return rc;
+  }
+  int64_t st[PDX_DN_NSTATE] = {0};
+  st[PDX_DN_INTERVAL] = interval;
+  st[PDX_DN_DIAGNOSTICS] = nl->diagnostics ? 1 : 0;
+  const float geomf[3] = {(float)box[0], (float)box[1], (float)box[2]};
+  std::vector<int32_t> ones(nl->ncell + 1, 1);
+  // identity traversal order until the first build (mesh kernels may read it before then)
+  std::vector<int32_t> ident(n);
+  for (int32_t k = 0; k < n; ++k) ident[k] = k + 1;
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMemcpy(nl->state.p, st, sizeof(st), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(nl->geom.p, geomf, sizeof(geomf), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(nl->neighbors_flat.p, 0xff, sizeof(int32_t) * (size_t)n * cap);  // -1 padding
+  if (e == hipSuccess) e = hipMemset(nl->counts.p, 0, sizeof(int32_t) * n);
+  if (e == hipSuccess) e = hipMemset(nl->cell_counts.p, 0, sizeof(int32_t) * nl->ncell);
+  if (e == hipSuccess) e = hipMemcpy(nl->cell_offsets.p, ones.data(), sizeof(int32_t) * (nl->ncell + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(nl->tickets.p, 0, sizeof(unsigned) * 4);
+  if (e == hipSuccess) e = hipMemcpy(nl->particle_ids_sorted.p, ident.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(nl->cell_of_particle.p, 0, sizeof(int32_t) * n);
+  if (e == hipSuccess) e = hipMemset(nl->packed.p, 0, sizeof(float4) * n);
+  if (e == hipSuccess) e = hipMemset(nl->rref.p, 0, sizeof(float) * 3 * (size_t)n);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    delete nl;
+    return fail(PDX_ERR_DEVICE, std::string("neighbor list initialisation failed: ") + hipGetErrorString(e));
+  }
+  *out = nl;
+  return PDX_OK;
+}
+
+extern "C" int pdx_nl_destroy(pdx_nl_t nl) {
+  delete nl;
+  return PDX_OK;
+}
+
+extern "C" int pdx_nl_info(pdx_nl_t nl, int32_t out[8]) {
+  if (!nl || !out) return fail(PDX_ERR_ARGUMENT, "null argument");
+  out[0] = nl->N; out[1] = nl->cap; out[2] = nl->nx; out[3] = nl->ny; out[4] = nl->nz;
+  out[5] = nl->ncell; out[6] = nl->allpairs; out[7] = nl->diagnostics;
+  return PDX_OK;
+}
+
+extern "C" int pdx_nl_arrays_get(pdx_nl_t nl, pdx_nl_arrays *out) {
+  if (!nl || !out) return fail(PDX_ERR_ARGUMENT, "null argument");
+  out->neighbors_flat = nl->neighbors_flat.p;
+  out->counts = nl->counts.p;
+  out->cell_offsets = nl->cell_offsets.p;
+  out->particle_ids_sorted = nl->particle_ids_sorted.p;
+  out->cell_of_particle = nl->cell_of_particle.p;
+  out->rref_x = nl->rref.p;
+  out->rref_y = nl->rref.p + nl->N;
+  out->rref_z = nl->rref.p + 2 * (size_t)nl->N;
+  out->packed = reinterpret_cast<float *>(nl->packed.p);
+  out->state = nl->state.p;
+  return PDX_OK;
+}
+
+extern "C" int pdx_nl_update(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
+                             pdx_stream_t stream) {
+  return enqueue_update(nl, x, y, z, q, as_stream(stream), 0);
+}
+
+extern "C" int pdx_nl_rebuild(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
+                              pdx_stream_t stream) {
+  return enqueue_update(nl, x, y, z, q, as_stream(stream), 1);
+}
+
+extern "C" int pdx_nl_advance(pdx_nl_t nl, pdx_stream_t stream) {
+  if (!nl) return fail(PDX_ERR_ARGUMENT, "null neighbor list");
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, as_stream(stream), nl->state.p);
+  PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
+extern "C" int pdx_nl_rearm(pdx_nl_t nl, pdx_stream_t stream) {
+  if (!nl) return fail(PDX_ERR_ARGUMENT, "null neighbor list");
+  hipLaunchKernelGGL(k_rearm, dim3(1), dim3(1), 0, as_stream(stream), nl->state.p);
+  PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
+extern "C" int pdx_nl_resize_capacity(pdx_nl_t nl, int32_t cap) {
+  if (!nl) return fail(PDX_ERR_ARGUMENT, "null neighbor list");
+  if (cap <= 0) return fail(PDX_ERR_ARGUMENT, "Invalid ELL capacity");
+  if ((int64_t)nl->N * cap > INT32_MAX) return fail(PDX_ERR_ARGUMENT, "Neighbor table exceeds the Int32 index range");
+  PDX_HIP(hipDeviceSynchronize());
+  int rc = nl->neighbors_flat.alloc((size_t)nl->N * cap);
+  if (rc) return rc;
+  PDX_HIP(hipMemset(nl->neighbors_flat.p, 0xff, sizeof(int32_t) * (size_t)nl->N * cap));
+  nl->cap = cap;
+  hipLaunchKernelGGL(k_invalidate, dim3(1), dim3(1), 0, 0, nl->state.p);
+  PDX_HIP(hipDeviceSynchronize());
+  return PDX_OK;
+}
+
+extern "C" int pdx_nl_set_box(pdx_nl_t nl, const double box[3], pdx_stream_t stream) {
+  if (!nl || !box) return fail(PDX_ERR_ARGUMENT, "null argument");
+  const float rlf = std::sqrt(nl->rl2);
+  int32_t g[3];
+  int rc = grid_for_box(box, rlf, g);
+  if (rc) return rc;
+  if (nl->allpairs) {
+    const double mn = std::min(box[0], std::min(box[1], box[2]));
+    if (!(2.0 * rlf <= mn)) return fail(PDX_ERR_ARGUMENT, "All-pairs neighbor rows need cutoff+skin <= L/2.");
+  } else {
+    if (g[0] < 3 || g[1] < 3 || g[2] < 3)
+      return fail(PDX_ERR_ARGUMENT, "The box became too small for the device cell grid (3 cells per dimension).");
+    if (g[0] != nl->nx || g[1] != nl->ny || g[2] != nl->nz) {
+      PDX_HIP(hipStreamSynchronize(as_stream(stream)));
+      const int ncell = g[0] * g[1] * g[2];
+      if ((rc = nl->cell_offsets.alloc(ncell + 1)) || (rc = nl->cell_counts.alloc(ncell)) ||
+          (rc = nl->scan_block.alloc(blocks_for(ncell, SCAN_BLOCK))))
+        return rc;
+      std::vector<int32_t> ones(ncell + 1, 1);
+      PDX_HIP(hipMemcpy(nl->cell_offsets.p, ones.data(), sizeof(int32_t) * (ncell + 1), hipMemcpyHostToDevice));
+      PDX_HIP(hipMemset(nl->cell_counts.p, 0, sizeof(int32_t) * ncell));
+      nl->nx = g[0]; nl->ny = g[1]; nl->nz = g[2]; nl->ncell = ncell;
+    }
+  }
+  for (int d = 0; d < 3; ++d) nl->box[d] = box[d];
+  const float geomf[3] = {(float)box[0], (float)box[1], (float)box[2]};
+  PDX_HIP(hipMemcpyAsync(nl->geom.p, geomf, sizeof(geomf), hipMemcpyHostToDevice, as_stream(stream)));
+  hipLaunchKernelGGL(k_invalidate, dim3(1), dim3(1), 0, as_stream(stream), nl->state.p);
+  PDX_HIP(hipStreamSynchronize(as_stream(stream)));
+  return PDX_OK;
+}
+
+extern "C" int pdx_nl_status(pdx_nl_t nl, int64_t state[PDX_DN_NSTATE], int32_t check, pdx_stream_t stream) {
+  if (!nl || !state) return fail(PDX_ERR_ARGUMENT, "null argument");
+  PDX_HIP(hipStreamSynchronize(as_stream(stream)));
+  PDX_HIP(hipMemcpy(state, nl->state.p, sizeof(int64_t) * PDX_DN_NSTATE, hipMemcpyDeviceToHost));
+  if (!check) return PDX_OK;
+  if (state[PDX_DN_ERROR] == PDX_DN_CAPACITY_ERROR)
+    return fail(PDX_ERR_CAPACITY, "Neighbor capacity " + std::to_string(nl->cap) + " is insufficient; at least " +
+                                      std::to_string(state[PDX_DN_REQUIRED]) +
+                                      " entries per particle are required. Rebuild with a larger cap.");
+  if (state[PDX_DN_ERROR] == PDX_DN_COORDINATE_ERROR)
+    return fail(PDX_ERR_COORDINATE,
+                "Device ELL requires finite, centered coordinates; execution stopped before rebuilding/forces");
+  if (state[PDX_DN_ERROR] != 0) return fail(PDX_ERR_STATE, "Device list stopped with an unknown error code");
+  if (state[PDX_DN_VALID] == 0) return fail(PDX_ERR_STATE, "Device ELL has no valid list");
+  return PDX_OK;
+}

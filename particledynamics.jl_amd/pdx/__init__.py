@@ -1,0 +1,18 @@
+"""pdx — MI355X-native force hot path of ParticleDynamics.jl (host side over libpdx's C ABI).
+
+Importing this package loads ``particledynamics.jl_amd/lib/libpdx.so``; it fails loudly
+when the library is missing. Device arrays are torch tensors on ``cuda`` (HIP).
+"""
+from ._lib import (ArgumentError, DeviceError, DimensionMismatch, NeighborCapacityError, check, last_error,
+                   lib, LIB_PATH)
+from .neighbors import DeviceNeighborWorkspace, CellList, build_neighbors_dense
+from .forces import LennardJones, WCA, SoftRepulsive, RealSpaceCoulomb, PairForce
+from .electrostatics import (ElectrostaticsSpec, PME, ElectrostaticsSolver, alpha_from_tolerance, fft_friendly,
+                             ewald_fp64)
+from .engine import ForceEngine, default_capacity
+from . import systems
+
+__all__ = ["ArgumentError", "DeviceError", "DimensionMismatch", "NeighborCapacityError", "check", "last_error",
+           "lib", "LIB_PATH", "DeviceNeighborWorkspace", "CellList", "build_neighbors_dense", "LennardJones", "WCA",
+           "SoftRepulsive", "RealSpaceCoulomb", "PairForce", "ElectrostaticsSpec", "PME", "ElectrostaticsSolver",
+           "alpha_from_tolerance", "fft_friendly", "ewald_fp64", "ForceEngine", "default_capacity", "systems"]

@@ -1,0 +1,239 @@
+"""Smooth PME on libpdx: specification, solver and the FP64 Ewald reference.
+
+``ElectrostaticsSpec`` and ``PME`` mirror src/electrostatics/Spec.jl:31-102 and the workflow
+descriptor src/workflow/Electrostatics.jl:45-63; ``ElectrostaticsSolver`` mirrors
+src/electrostatics/Solver.jl:75-158 (``compute_mesh``, ``mesh_transform``, ``mesh_gather``,
+``set_solver_charges``, ``refresh_geometry``, ``self_energy``, ``background_energy``).
+The solver holds no references to particle arrays: they are passed on every call.
+"""
+from __future__ import annotations
+
+import ctypes
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+from . import _lib as L
+
+
+def fft_friendly(n: int) -> int:
+    """Smallest integer >= n whose prime factors are in {2, 3, 5, 7} (Spec.jl:122-134)."""
+    m = max(int(n), 1)
+    while True:
+        r = m
+        for p in (2, 3, 5, 7):
+            while r % p == 0:
+                r //= p
+        if r == 1:
+            return m
+        m += 1
+
+
+def alpha_from_tolerance(cutoff: float, tol: float) -> float:
+    """alpha with erfc(alpha r_c) = tol, by bisection (Spec.jl:154-163)."""
+    if not (0 < tol < 1):
+        raise L.ArgumentError("real_space_tolerance must lie in (0, 1).")
+    lo, hi = 0.0, 10.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if math.erfc(mid) - tol > 0:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi) / cutoff
+
+
+@dataclass
+class ElectrostaticsSpec:
+    coulomb_prefactor: float
+    real_cutoff: float
+    alpha: float | None = None
+    real_space_tolerance: float | None = None
+    order: int = 4
+    mesh: tuple | None = None
+    mesh_spacing: float | None = None
+    relative_permittivity: float = 1.0
+    net_charge: str = "error"
+    real_space_modifier: str = "none"
+    net_charge_tolerance: float = 1e-8
+    influence: str = "spme"
+
+    def __post_init__(self):
+        if self.mesh is None and self.mesh_spacing is None:
+            raise L.ArgumentError("Provide either `mesh=(K1,K2,K3)` or `mesh_spacing`.")
+        if self.alpha is None and self.real_space_tolerance is None:
+            raise L.ArgumentError("Provide `alpha` or `real_space_tolerance`.")
+        if self.net_charge not in ("error", "background"):
+            raise L.ArgumentError("net_charge must be :error or :background")
+        if self.real_space_modifier not in ("none", "potential_shift"):
+            raise L.ArgumentError("real_space_modifier must be :none or :potential_shift")
+        if self.influence != "spme":
+            raise L.ArgumentError("Only the SPME influence function is implemented.")
+
+    @property
+    def C(self):
+        return self.coulomb_prefactor / self.relative_permittivity
+
+    def resolved_alpha(self):
+        return float(self.alpha) if self.alpha is not None else alpha_from_tolerance(self.real_cutoff,
+                                                                                     self.real_space_tolerance)
+
+    def c_spec(self):
+        s = L.pdx_mesh_spec()
+        s.coulomb_prefactor = float(self.coulomb_prefactor)
+        s.relative_permittivity = float(self.relative_permittivity)
+        s.real_cutoff = float(self.real_cutoff)
+        s.alpha = float(self.alpha) if self.alpha is not None else 0.0
+        s.real_space_tolerance = float(self.real_space_tolerance or 0.0)
+        s.order = int(self.order)
+        if self.mesh is not None:
+            for d in range(3):
+                s.mesh[d] = int(self.mesh[d])
+        else:
+            s.mesh[0] = 0
+        s.mesh_spacing = float(self.mesh_spacing or 0.0)
+        s.influence = L.PDX_INFLUENCE_SPME
+        s.net_charge = 1 if self.net_charge == "background" else 0
+        s.net_charge_tolerance = float(self.net_charge_tolerance)
+        s.real_space_modifier = 1 if self.real_space_modifier == "potential_shift" else 0
+        return s
+
+
+def PME(*, coulomb_prefactor, real_cutoff, alpha=None, real_space_tolerance=None, mesh=None, mesh_spacing=None,
+        order=4, relative_permittivity=1.0, net_charge="error", real_space_modifier="none"):
+    """Workflow descriptor of smooth PME (src/workflow/Electrostatics.jl:45-63)."""
+    return ElectrostaticsSpec(coulomb_prefactor=coulomb_prefactor, real_cutoff=real_cutoff, alpha=alpha,
+                              real_space_tolerance=real_space_tolerance, order=order, mesh=mesh,
+                              mesh_spacing=mesh_spacing, relative_permittivity=relative_permittivity,
+                              net_charge=net_charge, real_space_modifier=real_space_modifier)
+
+
+class ElectrostaticsSolver:
+    def __init__(self, spec: ElectrostaticsSpec, box, charges):
+        """``charges``: host sequence in storage order (set_solver_charges!, Solver.jl:166-185)."""
+        import numpy as np
+        self.spec = spec
+        q = np.ascontiguousarray(np.asarray(charges, dtype=np.float32))
+        self.N = int(q.size)
+        h = C.c_void_p()
+        cs = spec.c_spec()
+        L.check(L.lib.pdx_mesh_create(C.byref(cs), self.N, L.dbl3(box), C.byref(h)))
+        self._h = h
+        self.box = tuple(float(b) for b in box)
+        self.set_charges(q)
+        info = (C.c_double * 8)()
+        L.check(L.lib.pdx_mesh_info(self._h, info))
+        self.mesh = (int(info[0]), int(info[1]), int(info[2]))
+        self.order = int(info[3])
+        self.alpha = info[4]
+        self.C = info[5]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            L.lib.pdx_mesh_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_charges(self, charges):
+        import numpy as np
+        q = np.ascontiguousarray(np.asarray(charges, dtype=np.float32))
+        L.check(L.lib.pdx_mesh_set_charges(self._h, q.ctypes.data_as(C.POINTER(C.c_float)), int(q.size)))
+        self.qtotal = float(q.astype(np.float64).sum())
+        self.qsq = float((q.astype(np.float64) ** 2).sum())
+
+    def refresh_geometry(self, box):
+        L.check(L.lib.pdx_mesh_set_box(self._h, L.dbl3(box)))
+        self.box = tuple(float(b) for b in box)
+
+    def real_space(self):
+        """The matching short-range term for a ``PairForce``."""
+        from .forces import RealSpaceCoulomb
+        return RealSpaceCoulomb(C=self.C, cutoff=self.spec.real_cutoff, alpha=self.alpha,
+                                shift=self.spec.real_space_modifier == "potential_shift")
+
+    # ---- enqueue
+    def spread(self, x, y, z, q, order=None, stream=None):
+        L.check(L.lib.pdx_mesh_spread(self._h, L.ptr(x), L.ptr(y), L.ptr(z), L.ptr(q), L.ptr(order),
+                                      L.stream_handle(stream)))
+
+    def solve(self, energy=True, stream=None):
+        L.check(L.lib.pdx_mesh_solve(self._h, L.PDX_ENERGY if energy else 0, L.stream_handle(stream)))
+
+    def mesh_transform(self, x, y, z, q, energy=True, order=None, stream=None):
+        self.spread(x, y, z, q, order, stream)
+        self.solve(energy, stream)
+
+    def mesh_gather(self, x, y, z, q, fx, fy, fz, Epot=None, Vglobal=None, energy=True, order=None, set=False,
+                    stream=None):
+        L.check(L.lib.pdx_mesh_gather(self._h, L.ptr(x), L.ptr(y), L.ptr(z), L.ptr(q), L.ptr(order), L.ptr(fx),
+                                      L.ptr(fy), L.ptr(fz), L.ptr(Epot), L.ptr(Vglobal),
+                                      L.PDX_ENERGY if energy else 0, 1 if set else 0, L.stream_handle(stream)))
+
+    def compute_mesh(self, x, y, z, q, fx, fy, fz, Epot=None, Vglobal=None, energy=True, order=None, set=False,
+                     stream=None):
+        self.mesh_transform(x, y, z, q, energy, order, stream)
+        self.mesh_gather(x, y, z, q, fx, fy, fz, Epot, Vglobal, energy, order, set, stream)
+
+    # ---- boundary readbacks
+    def totals(self, stream=None):
+        out = (C.c_double * 9)()
+        L.check(L.lib.pdx_mesh_totals(self._h, out, L.stream_handle(stream)))
+        return dict(mesh_energy=out[0], virial=list(out[1:7]), self_energy=out[7], background_energy=out[8])
+
+    def self_energy(self):
+        return -0.5 * self.C * (2.0 * self.alpha / math.sqrt(math.pi)) * self.qsq
+
+    def background_energy(self):
+        if self.spec.net_charge != "background":
+            return 0.0
+        V = self.box[0] * self.box[1] * self.box[2]
+        return -self.C * (math.pi / (2 * self.alpha ** 2)) * self.qtotal ** 2 / V
+
+    def fixed_grid(self, stream=None):
+        import torch
+        K1, K2, K3 = self.mesh
+        out = torch.empty(K3, K2, K1, dtype=torch.float64, device="cuda")
+        L.check(L.lib.pdx_mesh_copy_grid(self._h, 0, L.ptr(out), L.stream_handle(stream)))
+        return out
+
+    def potential_grid(self, stream=None):
+        import torch
+        K1, K2, K3 = self.mesh
+        out = torch.empty(K3, K2, K1, dtype=torch.float32, device="cuda")
+        L.check(L.lib.pdx_mesh_copy_grid(self._h, 1, L.ptr(out), L.stream_handle(stream)))
+        return out
+
+    def fft_r2c(self, grid, stream=None):
+        """R2C of a float grid shaped (K3, K2, K1) -> complex64 (K3, K2, K1//2+1)."""
+        import torch
+        K1, K2, K3 = self.mesh
+        out = torch.empty(K3, K2, K1 // 2 + 1, dtype=torch.complex64, device="cuda")
+        L.check(L.lib.pdx_mesh_fft(self._h, -1, L.ptr(grid.contiguous()), L.ptr(out), L.stream_handle(stream)))
+        return out
+
+    def fft_c2r(self, spec, stream=None):
+        import torch
+        K1, K2, K3 = self.mesh
+        out = torch.empty(K3, K2, K1, dtype=torch.float32, device="cuda")
+        L.check(L.lib.pdx_mesh_fft(self._h, 1, L.ptr(spec.contiguous()), L.ptr(out), L.stream_handle(stream)))
+        return out
+
+
+def ewald_fp64(x, y, z, q, box, C, alpha, digits=14.0, background=False, sample=None, want_energy=True,
+               stream=None):
+    """FP64 Ewald reference on the device (``pdx_ewald_fp64``). x..q: float64 device tensors;
+    sample: int32 device tensor of 0-based ids. Returns (F[nsample,3] tensor or None, totals dict)."""
+    import torch
+    n = x.numel()
+    ns = 0 if sample is None else sample.numel()
+    F = torch.zeros(max(ns, 1), 3, dtype=torch.float64, device="cuda")
+    out = (ctypes.c_double * 7)()
+    L.check(L.lib.pdx_ewald_fp64(n, L.ptr(x), L.ptr(y), L.ptr(z), L.ptr(q), L.dbl3(box), float(C), float(alpha),
+                                 float(digits), 1 if background else 0, ns, L.ptr(sample), L.ptr(F),
+                                 1 if want_energy else 0, out, L.stream_handle(stream)))
+    tot = dict(energy=out[0], virial=list(out[1:7])) if want_energy else None
+    return (F[:ns] if ns else None), tot

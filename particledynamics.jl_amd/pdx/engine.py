@@ -1,0 +1,65 @@
+"""One force evaluation on the device: list check + predicated rebuild, pair forces (LJ/WCA
+and real-space Coulomb), and the PME mesh — the work of ``evaluate_forces_into_f!``
+(src/simulation/ForceEvaluation.jl:40-76) on the reference's ELL path (``Run.enqueue_force!``
++ ``Run.enqueue_ell_mesh!``, src/engine/Run.jl:179-238)."""
+from __future__ import annotations
+
+import math
+
+from . import _lib as L
+from .electrostatics import ElectrostaticsSolver
+from .forces import PairForce
+from .neighbors import DeviceNeighborWorkspace
+
+
+def default_capacity(N, box, rlist):
+    """ceil(1.5 (4 pi/3) rlist^3 rho + 64) (src/engine/StateEngine.jl:463)."""
+    rho = N / (box[0] * box[1] * box[2])
+    return int(math.ceil(1.5 * (4.0 * math.pi / 3.0) * rlist ** 3 * rho + 64))
+
+
+class ForceEngine:
+    def __init__(self, N, box, pair_model=None, electrostatics=None, charges=None, skin=0.4, cap=None,
+                 interval=20, lanes=0, diagnostics=False):
+        import torch
+        self.N = int(N)
+        self.box = tuple(float(b) for b in box)
+        self.solver = ElectrostaticsSolver(electrostatics, box, charges) if electrostatics is not None else None
+        rc = 0.0
+        if pair_model is not None:
+            rc = max(rc, float(pair_model.cutoff))
+        if self.solver is not None:
+            rc = max(rc, float(electrostatics.real_cutoff))
+        self.cutoff = rc
+        self.skin = float(skin)
+        cap = cap if cap is not None else default_capacity(N, box, rc + skin)
+        self.nl = DeviceNeighborWorkspace(N, box, rc, skin, cap, interval=interval, diagnostics=diagnostics)
+        self.pairs = PairForce(pair_model, coulomb=self.solver.real_space() if self.solver else None, lanes=lanes)
+        self.pair_totals = torch.zeros(7, dtype=torch.float64, device="cuda")
+        self.Vglobal = torch.zeros(6, dtype=torch.float32, device="cuda")
+
+    def evaluate(self, x, y, z, q, fx, fy, fz, Epot=None, V=None, energy=True, stream=None):
+        """Enqueue one force evaluation (no host synchronisation)."""
+        self.nl.update(x, y, z, q, stream=stream)
+        self.pairs.compute(self.nl, fx, fy, fz, Epot, V, self.pair_totals if energy else None, energy=energy,
+                           add=False, stream=stream)
+        if self.solver is not None:
+            order = self.nl._arrays.particle_ids_sorted
+            self.solver.compute_mesh(x, y, z, q, fx, fy, fz, Epot, self.Vglobal if energy else None, energy,
+                                     order=order, set=False, stream=stream)
+
+    def totals(self, stream=None):
+        """FP64 energy and virial of the last energy evaluation (boundary readback)."""
+        self.nl.status(check=True, stream=stream)
+        p = self.pair_totals.cpu().tolist()
+        E = p[0]
+        W = p[1:7]
+        out = dict(pair_energy=p[0], pair_virial=p[1:7])
+        if self.solver is not None:
+            m = self.solver.totals(stream=stream)
+            E += m["mesh_energy"] + m["background_energy"]
+            W = [a + b for a, b in zip(W, m["virial"])]
+            out.update(mesh_energy=m["mesh_energy"], mesh_virial=m["virial"], self_energy=m["self_energy"],
+                       background_energy=m["background_energy"])
+        out.update(energy=E, virial=W)
+        return out

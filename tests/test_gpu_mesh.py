@@ -1,0 +1,158 @@
+"""SPME mesh engine: FFT conventions (test/test_electrostatics.jl:260-279), spreading
+identities (:695-703), accuracy against the FP64 Ewald oracle (:299-340, :553-564), the
+rock-salt Madelung energy at the full 64 000-ion size, and the device FP64 Ewald kernel
+against the CPU oracle."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import pdx
+from conftest import dev
+from pdx import systems
+
+pytestmark = pytest.mark.gpu
+MADELUNG_NACL = 1.747564594633182
+L_ES = (14.0, 15.0, 16.0)
+
+
+def _solver(mesh, P=6, box=L_ES, N=8, C=1.0, alpha=0.9, rc=4.0, q=None):
+    spec = pdx.PME(coulomb_prefactor=C, real_cutoff=rc, alpha=alpha, mesh=mesh, order=P)
+    return pdx.ElectrostaticsSolver(spec, box, np.zeros(N) if q is None else q)
+
+
+def test_fft_normalisation(cuda):
+    import torch
+    s = _solver((8, 6, 5), P=4, box=(9.0, 9.0, 9.0), rc=2.0)
+    g = torch.zeros(5, 6, 8, device="cuda")
+    g[0, 0, 0] = 1.0
+    S = s.fft_r2c(g)
+    assert S.shape == (5, 6, 5)
+    assert torch.allclose(S, torch.ones_like(S), atol=1e-6)  # impulse -> flat spectrum
+    S = s.fft_r2c(torch.full((5, 6, 8), 2.0, device="cuda"))
+    assert abs(S[0, 0, 0].real.item() - 480.0) < 1e-3 and S.abs().flatten()[1:].max().item() < 1e-4
+    i = np.arange(8)[None, None, :]
+    j = np.arange(6)[None, :, None]
+    k = np.arange(5)[:, None, None]
+    g = np.cos(2 * np.pi * (1 * i / 8 + 2 * j / 6 + 1 * k / 5)).astype(np.float32)
+    S = s.fft_r2c(dev(g))
+    assert abs(S[1, 2, 1].real.item() - 120.0) < 1e-3  # single mode: M/2
+    back = s.fft_c2r(S).cpu().numpy()
+    assert np.abs(back / 240 - g).max() < 1e-5  # backward is unnormalised
+
+
+@pytest.mark.parametrize("mesh", [(20, 22, 24), (48, 48, 54), (120, 120, 120), (60, 64, 63)])
+def test_fft_matches_torch(cuda, mesh):
+    import torch
+    s = _solver(mesh, P=5, box=(30.0, 30.0, 30.0), rc=4.0)
+    K1, K2, K3 = mesh
+    g = torch.randn(K3, K2, K1, device="cuda")
+    S = s.fft_r2c(g)
+    ref = torch.fft.rfftn(g.double(), dim=(0, 1, 2))
+    err = (S.to(torch.complex128) - ref).abs().max().item() / ref.abs().max().item()
+    print(f"[fft] {mesh} r2c err={err:.2e}", flush=True)
+    assert err < 2e-6
+    back = s.fft_c2r(S)
+    err2 = (back.double() / (K1 * K2 * K3) - g.double()).abs().max().item()
+    assert err2 < 2e-5
+
+
+def test_spread_conserves_and_is_deterministic(cuda):
+    import torch
+    rng = np.random.default_rng(5)
+    N = 2000
+    R, q = systems.random_cloud(N, L_ES, seed=5)
+    s = _solver((20, 22, 24), P=5, N=N, q=q)
+    X, Y, Z, Q = dev(R[:, 0]), dev(R[:, 1]), dev(R[:, 2]), dev(q)
+    s.spread(X, Y, Z, Q)
+    g1 = s.fixed_grid().clone()
+    s.solve(energy=False)  # consumes and clears the charge grid
+    order = torch.arange(N, 0, -1, dtype=torch.int32, device="cuda")  # reversed traversal
+    s.spread(X, Y, Z, Q, order=order)
+    g2 = s.fixed_grid().clone()
+    s.solve(energy=False)
+    assert torch.equal(g1, g2)  # integer atomics: bitwise independent of arrival order
+    assert abs(g1.sum().item() - float(np.float32(q).astype(np.float64).sum())) < 1e-6
+
+
+def _es_eval(R, q, L, spec, skin_frac=0.075):
+    """LJ-free Coulomb evaluation: real space on the rows + mesh, energy and virial."""
+    import torch
+    N = len(q)
+    eng = pdx.ForceEngine(N, L, None, electrostatics=spec, charges=q, skin=skin_frac * spec.real_cutoff,
+                          cap=max(64, pdx.default_capacity(N, L, spec.real_cutoff * 1.1)))
+    X, Y, Z = dev(R[:, 0]), dev(R[:, 1]), dev(R[:, 2])
+    Q = dev(q)
+    fx, fy, fz, E = (torch.empty(N, device="cuda") for _ in range(4))
+    eng.evaluate(X, Y, Z, Q, fx, fy, fz, E, None, energy=True)
+    tot = eng.totals()
+    F = torch.stack([fx, fy, fz], 1).double().cpu().numpy()
+    W = np.array(tot["virial"])
+    Wm = np.array([[W[0], W[3], W[4]], [W[3], W[1], W[5]], [W[4], W[5], W[2]]])
+    return tot["energy"], F, Wm, E.double().cpu().numpy()
+
+
+def _relrms(a, b):
+    return math.sqrt(((a - b) ** 2).sum() / (b ** 2).sum())
+
+
+def test_spme_against_ewald(cuda):
+    """120 random charges, L=(14,15,16), C=1.7, rc=4, alpha=0.9, (48,48,54), P=6 in FP32:
+    force rel. RMS and energy < 1e-4 (test/test_electrostatics.jl:553-564)."""
+    R, q = systems.random_cloud(120, L_ES, seed=3)
+    R32 = R.astype(np.float32).astype(np.float64)
+    q32 = q.astype(np.float32).astype(np.float64)
+    Eref, Fref, Wref = oracle.ewald(R32, q32, np.diag(L_ES), C_=1.7)
+    spec = pdx.PME(coulomb_prefactor=1.7, real_cutoff=4.0, alpha=0.9, mesh=(48, 48, 54), order=6)
+    E, F, W, Ep = _es_eval(R, q, L_ES, spec)
+    print(f"[spme] relrms={_relrms(F, Fref):.3e} dE={abs(E - Eref) / abs(Eref):.3e} "
+          f"dW={np.abs(W - Wref).max() / abs(np.trace(Wref)):.3e}", flush=True)
+    assert _relrms(F, Fref) < 1e-4
+    assert abs(E - Eref) / abs(Eref) < 1e-4
+    assert np.abs(W - Wref).max() / abs(np.trace(Wref)) < 1e-3
+    assert abs(Ep.sum() - E) / abs(E) < 1e-5  # per-particle energies sum to the FP64 total
+
+
+def test_spme_refined_against_ewald(cuda):
+    R, q = systems.random_cloud(120, L_ES, seed=3)
+    R32 = R.astype(np.float32).astype(np.float64)
+    q32 = q.astype(np.float32).astype(np.float64)
+    Eref, Fref, _ = oracle.ewald(R32, q32, np.diag(L_ES), C_=1.7)
+    spec = pdx.PME(coulomb_prefactor=1.7, real_cutoff=4.0, alpha=0.9, mesh=(64, 64, 72), order=8)
+    E, F, W, _ = _es_eval(R, q, L_ES, spec)
+    print(f"[spme refined] relrms={_relrms(F, Fref):.3e} dE={abs(E - Eref) / abs(Eref):.3e}", flush=True)
+    assert _relrms(F, Fref) < 2e-5
+    assert abs(E - Eref) / abs(Eref) < 2e-5
+
+
+def test_madelung_rock_salt_full_size(cuda):
+    """64 000-ion rock salt, unjittered: E = -(N/2) M C / a, forces vanish, W = E/3 per diagonal."""
+    s = systems.rock_salt(40, 2.82, 0.0)
+    C = 14.399645
+    spec = pdx.PME(coulomb_prefactor=C, real_cutoff=9.0, real_space_tolerance=1e-5, mesh_spacing=1.0, order=5)
+    R = np.stack([s["x"], s["y"], s["z"]], 1).astype(np.float64)
+    E, F, W, _ = _es_eval(R, s["q"], s["box"], spec)
+    Emad = -(s["N"] / 2) * MADELUNG_NACL * C / 2.82
+    print(f"[madelung] E={E:.6f} ref={Emad:.6f} rel={abs(E - Emad) / abs(Emad):.3e} "
+          f"maxF={np.abs(F).max():.3e}", flush=True)
+    assert abs(E - Emad) / abs(Emad) < 1e-4
+    assert np.abs(F).max() < 1e-3 * abs(C / 2.82 ** 2)
+    assert abs(np.trace(W) - E) / abs(E) < 1e-3
+
+
+def test_device_ewald_matches_oracle(cuda):
+    import torch
+    R, q = systems.random_cloud(120, L_ES, seed=3)
+    Eref, Fref, Wref = oracle.ewald(R, q, np.diag(L_ES), C_=1.7, alpha=0.45, digits=14)
+    xs = [torch.tensor(R[:, d], dtype=torch.float64, device="cuda") for d in range(3)]
+    qd = torch.tensor(q, dtype=torch.float64, device="cuda")
+    ids = torch.arange(120, dtype=torch.int32, device="cuda")
+    F, tot = pdx.ewald_fp64(*xs, qd, L_ES, 1.7, 0.45, digits=14, sample=ids)
+    F = F.cpu().numpy()
+    assert np.abs(F - Fref).max() < 1e-10 * np.abs(Fref).max()
+    assert tot["energy"] == pytest.approx(Eref, rel=1e-11)
+    W = tot["virial"]
+    assert np.allclose([W[0], W[1], W[2], W[3], W[4], W[5]],
+                       [Wref[0, 0], Wref[1, 1], Wref[2, 2], Wref[0, 1], Wref[0, 2], Wref[1, 2]],
+                       rtol=1e-9, atol=1e-10 * abs(np.trace(Wref)))

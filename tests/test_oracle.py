@@ -1,0 +1,209 @@
+"""Pins the FP64 CPU oracle (oracle/oracle.c) against known answers, closed forms and
+invariants, chosen so that a plausible mistake (a dropped term, a sign, an index, a wrong
+image) fails one of them. CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+
+MADELUNG_NACL = 1.747564594633182
+
+
+# ---------------------------------------------------------------- neighbor rows
+def test_rows_known_answers():
+    # test/test_neighbor_reference.jl:1-6
+    rows, _, _ = oracle.neighbor_rows((np.array([0.0, 9.5, 5.0]), np.zeros(3)), (10.0, 10.0), np.ones(3))
+    assert rows == [[1], [0], []]
+    rows, _, _ = oracle.neighbor_rows((np.array([0.0, 1.0]), np.zeros(2), np.zeros(2)), (10.0, 10.0, 10.0),
+                                      np.array([0.5, 1.5]))
+    assert rows == [[], [0]]
+    rows, _, _ = oracle.neighbor_rows((np.array([0.0]), np.array([0.0])), (10.0, 10.0), np.array([1.0]))
+    assert rows == [[]]
+
+
+def test_rows_symmetric_and_counted():
+    rng = np.random.default_rng(3)
+    n, L, r = 300, 12.0, 2.2
+    c = [rng.random(n) * L - L / 2 for _ in range(3)]
+    rows, counts, margin = oracle.neighbor_rows(tuple(c), (L, L, L), np.full(n, r))
+    for i, row in enumerate(rows):
+        assert len(row) == len(set(row)) == counts[i]
+        for j in row:
+            assert i in rows[j]
+    # independent recount with numpy minimum image
+    P = np.stack(c, 1)
+    d = P[None, :, :] - P[:, None, :]
+    d -= L * np.round(d / L)
+    r2 = (d ** 2).sum(-1)
+    np.fill_diagonal(r2, np.inf)
+    assert (counts == (r2 <= r * r).sum(1)).all()
+    assert margin > 0
+
+
+# ---------------------------------------------------------------- pair sums
+def _two(rx, kind="lj", eps=1.0, sig=1.0, rc=2.5):
+    return oracle.pair_direct(np.array([0.0, rx]), np.zeros(2), np.zeros(2), (10.0, 10.0, 10.0), kind=kind,
+                              sigma=sig, epsilon=eps, cutoff=rc, ids=[0, 1])
+
+
+def test_pair_closed_forms():
+    # test/test_force_precision_gpu.jl:7-20
+    r = _two(1.0)
+    assert np.array_equal(r["F"], [[-24.0, 0, 0], [24.0, 0, 0]])
+    assert np.array_equal(r["energy"], [0.0, 0.0])
+    assert np.array_equal(r["virial"].sum(0), [24.0, 0, 0, 0, 0, 0])
+    r = _two(2.0)
+    assert np.array_equal(r["F"][:, 0], [93 / 512, -93 / 512])
+    assert np.array_equal(r["energy"], [-63 / 2048, -63 / 2048])
+    assert r["virial"].sum(0)[0] == -93 / 256
+
+
+@pytest.mark.parametrize("rfac", [0.95, 1.5])
+def test_pair_analytic_lj(rfac):
+    # test/test_analytic_pair_forces.jl: eps 2.5, sigma 1.2
+    eps, sig = 2.5, 1.2
+    r = rfac * sig
+    out = _two(r, eps=eps, sig=sig, rc=3.0)
+    fx = 24 * eps * (2 * (sig / r) ** 12 - (sig / r) ** 6) * (-r / (r * r))
+    assert math.isclose(out["F"][0, 0], fx, rel_tol=1e-13)
+    assert math.isclose(out["energy"].sum(), 4 * eps * ((sig / r) ** 12 - (sig / r) ** 6), rel_tol=1e-13)
+    assert out["F"][:, 0].sum() == pytest.approx(0.0, abs=1e-12)
+
+
+def test_pair_analytic_wca_soft():
+    eps, sig = 1.7, 1.0
+    r = 0.9 * sig
+    out = _two(r, kind="wca", eps=eps, sig=sig)
+    lj = 4 * eps * ((sig / r) ** 12 - (sig / r) ** 6)
+    assert math.isclose(out["energy"].sum(), lj + eps, rel_tol=1e-13)
+    out = _two(1.01 * 2 ** (1 / 6) * sig, kind="wca", eps=eps, sig=sig)
+    assert np.all(out["F"] == 0) and np.all(out["energy"] == 0)
+    eps, sig = 8.0, 1.5
+    r = 0.8 * sig
+    out = _two(r, kind="soft", eps=eps, sig=sig)
+    assert math.isclose(out["F"][0, 0], (eps / sig) * (1 - r / sig) * (-1.0), rel_tol=1e-13)
+    assert math.isclose(out["energy"].sum(), 0.5 * eps * (1 - r / sig) ** 2, rel_tol=1e-13)
+
+
+def test_pair_gradient_and_newton():
+    rng = np.random.default_rng(5)
+    n, L = 60, 6.0
+    x, y, z = (rng.random(n) * L - L / 2 for _ in range(3))
+    out = oracle.pair_direct(x, y, z, (L, L, L), kind="wca")
+    assert np.abs(out["F"].sum(0)).max() < 1e-9 * np.abs(out["F"]).max()
+    # F = -dE/dx for a smooth potential (soft repulsion), central differences
+    i, h = 7, 1e-6
+    e = lambda xx: oracle.pair_direct(xx, y, z, (L, L, L), kind="soft", sigma=1.3)["energy"].sum()
+    f = oracle.pair_direct(x, y, z, (L, L, L), kind="soft", sigma=1.3)["F"][i, 0]
+    xp, xm = x.copy(), x.copy()
+    xp[i] += h
+    xm[i] -= h
+    assert abs(-(e(xp) - e(xm)) / (2 * h) - f) < 1e-6 * max(1.0, abs(f))
+
+
+# ---------------------------------------------------------------- erfc, B-spline
+def test_erfc_matches_libm():
+    for x in np.linspace(0.0, 6.0, 61):
+        # the erf series loses a few ulp as 1 - erf(x) shrinks toward x = 2
+        assert oracle.erfc(x) == pytest.approx(math.erfc(x), rel=1e-13, abs=1e-300)
+
+
+@pytest.mark.parametrize("P", [3, 4, 5, 6, 8])
+def test_bspline_identities(P):
+    rng = np.random.default_rng(P)
+    for u in 40 * rng.random(10):
+        k0, w, dw = oracle.bspline(P, u)
+        assert k0 == math.floor(u) - P + 1
+        assert abs(w.sum() - 1) < 1e-14 and abs(dw.sum()) < 1e-13
+        _, wp, _ = oracle.bspline(P, u + 1e-7)
+        _, wm, _ = oracle.bspline(P, u - 1e-7)
+        if math.floor(u + 1e-7) == math.floor(u - 1e-7):
+            assert np.abs((wp - wm) / 2e-7 - dw).max() < 1e-6
+    # cubic B-spline closed form M_4 at t = 0.5: node weights 1/48, 23/48, 23/48, 1/48
+    _, w, _ = oracle.bspline(4, 10.5)
+    assert np.allclose(w, [1 / 48, 23 / 48, 23 / 48, 1 / 48], atol=1e-15)
+
+
+# ---------------------------------------------------------------- Ewald
+def _nacl(n=4, a=1.0):
+    g = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    q = np.where(g.sum(1) % 2 == 0, 1.0, -1.0)
+    return (g + 0.25) * a, q, n * a
+
+
+def test_ewald_madelung():
+    R, q, L = _nacl()
+    E, F, W = oracle.ewald(R, q, np.diag([L] * 3))
+    assert E == pytest.approx(-len(q) / 2 * MADELUNG_NACL, rel=1e-12)
+    assert np.abs(F).max() < 1e-12
+    assert np.trace(W) == pytest.approx(E, rel=1e-12)
+    assert np.allclose(np.diag(W), E / 3, rtol=1e-12)
+
+
+def _cloud(n, L, seed):
+    rng = np.random.default_rng(seed)
+    R = rng.random((n, 3)) * np.asarray(L) - np.asarray(L) / 2
+    q = rng.standard_normal(n)
+    return R, q - q.mean()
+
+
+def test_ewald_alpha_independent():
+    R, q = _cloud(40, (7.0, 7.5, 8.0), 2)
+    H = np.diag([7.0, 7.5, 8.0])
+    E1, F1, W1 = oracle.ewald(R, q, H, C_=1.3, alpha=0.6)
+    E2, F2, W2 = oracle.ewald(R, q, H, C_=1.3, alpha=1.1)
+    assert E1 == pytest.approx(E2, rel=1e-12)
+    assert np.abs(F1 - F2).max() < 1e-11 * np.abs(F1).max()
+    assert np.abs(W1 - W2).max() < 1e-11 * np.abs(np.trace(W1))
+
+
+def test_ewald_gradient_and_strain():
+    Ls = (7.0, 7.5, 8.0)
+    R, q = _cloud(30, Ls, 9)
+    H = np.diag(Ls)
+    E0, F0, W0 = oracle.ewald(R, q, H, alpha=0.8)
+    h = 1e-5
+    for i, d in ((3, 0), (17, 1), (25, 2)):
+        Rp, Rm = R.copy(), R.copy()
+        Rp[i, d] += h
+        Rm[i, d] -= h
+        dE = (oracle.ewald(Rp, q, H, alpha=0.8)[0] - oracle.ewald(Rm, q, H, alpha=0.8)[0]) / (2 * h)
+        assert abs(-dE - F0[i, d]) < 1e-7 * np.abs(F0).max()
+    for a in range(3):
+        eps = 1e-6
+        Es = []
+        for e in (eps, -eps):
+            s = np.ones(3)
+            s[a] += e
+            Es.append(oracle.ewald(R * s, q, np.diag(np.asarray(Ls) * s), alpha=0.8)[0])
+        assert -(Es[0] - Es[1]) / (2 * eps) == pytest.approx(W0[a, a], rel=1e-6, abs=1e-6 * abs(np.trace(W0)))
+
+
+def test_ewald_background_and_exceptions():
+    Ls = (7.0, 7.5, 8.0)
+    R, q = _cloud(30, Ls, 4)
+    qn = q + 0.05
+    H = np.diag(Ls)
+    a = 0.9
+    E0, _, W0 = oracle.ewald(R, qn, H, alpha=a)
+    Eb, _, Wb = oracle.ewald(R, qn, H, alpha=a, background=True)
+    Ebg = -math.pi * qn.sum() ** 2 / (2 * a * a * np.prod(Ls))
+    assert Eb - E0 == pytest.approx(Ebg, rel=1e-12)
+    assert np.allclose(np.diag(Wb - W0), Ebg, rtol=1e-10)
+    # an excluded pair removes exactly C q_i q_j / r at the minimum image
+    Ex, Fx, _ = oracle.ewald(R, q, H, alpha=a, exceptions=[(2, 5, 0.0)])
+    E, F, _ = oracle.ewald(R, q, H, alpha=a)
+    d = R[2] - R[5]
+    d -= np.asarray(Ls) * np.round(d / np.asarray(Ls))
+    assert Ex - E == pytest.approx(-q[2] * q[5] / np.linalg.norm(d), rel=1e-12)
+
+
+def test_ewald_sampled_equals_full():
+    Ls = (7.0, 7.5, 8.0)
+    R, q = _cloud(40, Ls, 11)
+    E, F, W = oracle.ewald(R, q, np.diag(Ls), C_=1.4, alpha=0.7, digits=14)
+    ids = [0, 5, 39]
+    Fs = oracle.ewald_sampled(R[:, 0], R[:, 1], R[:, 2], q, Ls, C_=1.4, alpha=0.7, digits=14, ids=ids)
+    assert np.abs(Fs - F[ids]).max() < 1e-11 * np.abs(F).max()
