@@ -157,7 +157,8 @@ typedef struct {
   double coulomb_C;       /* prefactor / relative permittivity */
   double coulomb_cutoff;  /* real-space cutoff */
   double alpha;           /* Ewald splitting parameter */
-  int32_t lanes;          /* lanes per particle: 0 = automatic, else 1,2,4,8,16,32,64 */
+  int32_t lanes;          /* traversal: 0 automatic; 1..64 lanes of a wave per particle; -2,-4,-8,-16:
+                             blocks of 64 particles whose rows are split over that many waves */
 } pdx_pair_spec;
 
 /* Build a pair model (UniformLJ/UniformWCA/ChargedPairs constructors, PairModels.jl:16-18,
@@ -214,10 +215,15 @@ int pdx_mesh_set_charges(pdx_mesh_t m, const float *q_host, int32_t n);
 int pdx_mesh_set_box(pdx_mesh_t m, const double box[3]);
 /* Spread q into the int64 fixed-point grid (_spread_lines_kernel!, MeshKernels.jl:247-276),
  * deterministic: integer atomics commute. order: 1-based traversal order (e.g.
- * particle_ids_sorted) or NULL for storage order. The grid is consumed (and cleared) by
- * pdx_mesh_solve. */
+ * particle_ids_sorted) or NULL for storage order. The grid is cleared first. */
 int pdx_mesh_spread(pdx_mesh_t m, const float *x, const float *y, const float *z, const float *q,
                     const int32_t *order, pdx_stream_t stream);
+/* Spread from a neighbor list's cells, owner computes: each workgroup accumulates one 16^3
+ * block of the grid in LDS (64-bit fixed point, exact) from the particles of the cells that
+ * reach it, and stores the block; no global atomics. Positions and charges are the list's
+ * packed (x, y, z, q) of the last pdx_nl_update on the same step; the list must cover the
+ * mesh's particles and box. Same grid as pdx_mesh_spread (bitwise). */
+int pdx_mesh_spread_cells(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t stream);
 /* Forward R2C 3D FFT, multiply by G, FP64 mesh energy and virial (with PDX_ENERGY), inverse
  * C2R FFT into the potential grid (mesh_transform!, Solver.jl:305-341; _solve_kernel!,
  * MeshKernels.jl:57-81). */

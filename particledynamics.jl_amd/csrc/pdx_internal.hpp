@@ -163,6 +163,7 @@ __device__ void grid_sum_finalize(const double (&v)[NV], double *partial, unsign
 struct pdx_nl_s {
   int32_t N = 0, cap = 0, nx = 1, ny = 1, nz = 1, ncell = 1;
   int32_t interval = 20;
+  int32_t row_lanes = 1;  // lanes per particle of the row build
   bool allpairs = false, diagnostics = false;
   double box[3] = {0, 0, 0};
   double cutoff = 0, skin = 0;
@@ -212,4 +213,5 @@ struct pdx_mesh_s {
   pdx::DevBuf<double> partial;
   pdx::DevBuf<unsigned> ticket;
   int nmode_blocks = 0;
+  int T1 = 0, T2 = 0, T3 = 0;  // FFT tile widths, fixed at creation
 };

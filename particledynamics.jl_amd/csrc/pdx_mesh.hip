@@ -1,10 +1,10 @@
 // Smooth PME mesh engine on gfx950: B-spline charge spreading into an int64 fixed-point
-// grid, a mixed-radix (2,3,4,5,7) Stockham 3D FFT in LDS, the influence-function
+// grid, a mixed-radix (2,3,4,5,7,11,13) Stockham 3D FFT in LDS, the influence-function
 // convolution with FP64 energy / virial reductions fused into the z transform, and the
 // analytic-differentiation force gather.
 //
 // Pass structure of one solve (every pass reads and writes the mesh once):
-//   k_fft_x_r2c   int64 fixed point -> float, R2C along x, clears the charge grid
+//   k_fft_x_r2c   int64 fixed point -> float, R2C along x
 //   k_fft_y       complex FFT along y (forward)
 //   k_fft_z       forward z FFT, x G (+ FP64 energy/virial), inverse z FFT   [fused solve]
 //   k_fft_y       inverse along y
@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 using namespace pdx;
 
@@ -23,23 +24,39 @@ namespace {
 constexpr int TPB = 256;
 constexpr int MAXSTAGE = 16;
 
+// Mixed-radix Stockham plan of one axis: stage s has radix R_s, p_s = product of the
+// radices before it, the magic multiplier ceil(2^32/p_s) (exact quotient of i/p_s for
+// i p_s < 2^32) and the twiddle stride K/(p_s R_s).
 struct FFTPlan {
   int K;
   int nstage;
   int radix[MAXSTAGE];
+  int p[MAXSTAGE];
+  unsigned magic[MAXSTAGE];
+  int tstride[MAXSTAGE];
+  int big;  // a radix of 7, 11 or 13 (compiled into separate, register-heavier kernels)
 };
 
 bool make_plan(int K, FFTPlan &pl) {
   pl.K = K;
   pl.nstage = 0;
   int n = K;
-  const int rs[5] = {4, 2, 3, 5, 7};
+  const int rs[7] = {4, 2, 3, 5, 7, 11, 13};
   for (int r : rs) {
     while (n % r == 0) {
       if (pl.nstage >= MAXSTAGE) return false;
       pl.radix[pl.nstage++] = r;
       n /= r;
     }
+  }
+  int p = 1;
+  pl.big = 0;
+  for (int s = 0; s < pl.nstage; ++s) {
+    if (pl.radix[s] >= 7) pl.big = 1;
+    pl.p[s] = p;
+    pl.magic[s] = p > 1 ? (unsigned)((((unsigned long long)1 << 32) + p - 1) / p) : 0u;
+    pl.tstride[s] = K / (p * pl.radix[s]);
+    p *= pl.radix[s];
   }
   return n == 1;
 }
@@ -54,14 +71,15 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 __device__ __forceinline__ float2 cmuli(float2 a, float sign) { return make_float2(-sign * a.y, sign * a.x); }
 
 // In-place radix-R DFT of u[0..R) with exponent sign `sg` (-1 forward, +1 inverse).
-template <int R> __device__ __forceinline__ void dft(float2 (&u)[7], float sg);
+constexpr int MAXR = 13;
+template <int R> __device__ __forceinline__ void dft(float2 (&u)[MAXR], float sg);
 
-template <> __device__ __forceinline__ void dft<2>(float2 (&u)[7], float) {
+template <> __device__ __forceinline__ void dft<2>(float2 (&u)[MAXR], float) {
   const float2 a = u[0], b = u[1];
   u[0] = cadd(a, b);
   u[1] = csub(a, b);
 }
-template <> __device__ __forceinline__ void dft<3>(float2 (&u)[7], float sg) {
+template <> __device__ __forceinline__ void dft<3>(float2 (&u)[MAXR], float sg) {
   const float h = 0.86602540378443864676f;
   const float2 t = cadd(u[1], u[2]);
   const float2 d = cmuli(csub(u[1], u[2]), sg * h);
@@ -70,7 +88,7 @@ template <> __device__ __forceinline__ void dft<3>(float2 (&u)[7], float sg) {
   u[1] = cadd(m, d);
   u[2] = csub(m, d);
 }
-template <> __device__ __forceinline__ void dft<4>(float2 (&u)[7], float sg) {
+template <> __device__ __forceinline__ void dft<4>(float2 (&u)[MAXR], float sg) {
   const float2 t0 = cadd(u[0], u[2]), t1 = csub(u[0], u[2]);
   const float2 t2 = cadd(u[1], u[3]), t3 = cmuli(csub(u[1], u[3]), sg);
   u[0] = cadd(t0, t2);
@@ -78,7 +96,7 @@ template <> __device__ __forceinline__ void dft<4>(float2 (&u)[7], float sg) {
   u[1] = cadd(t1, t3);
   u[3] = csub(t1, t3);
 }
-template <> __device__ __forceinline__ void dft<5>(float2 (&u)[7], float sg) {
+template <> __device__ __forceinline__ void dft<5>(float2 (&u)[MAXR], float sg) {
   const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
   const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
   const float2 a1 = cadd(u[1], u[4]), b1 = csub(u[1], u[4]);
@@ -94,162 +112,230 @@ template <> __device__ __forceinline__ void dft<5>(float2 (&u)[7], float sg) {
   u[2] = cadd(r2, i2);
   u[3] = csub(r2, i2);
 }
-template <> __device__ __forceinline__ void dft<7>(float2 (&u)[7], float sg) {
-  // direct sum with cos/sin(2 pi k/7)
-  const float c[7] = {1.0f, 0.62348980185873353053f, -0.22252093395631440429f, -0.90096886790241912624f,
-                      -0.90096886790241912624f, -0.22252093395631440429f, 0.62348980185873353053f};
-  const float s[7] = {0.0f, 0.78183148246802980871f, 0.97492791218182360702f, 0.43388373911755812048f,
-                      -0.43388373911755812048f, -0.97492791218182360702f, -0.78183148246802980871f};
-  float2 v[7];
+// Odd primes 7, 11, 13: direct O(R^2) sum with the roots cos/sin(2 pi k/R) from a table.
+template <int R>
+__device__ __forceinline__ void dft_direct(float2 (&u)[MAXR], float sg, const float *c, const float *s) {
+  float2 v[R];
 #pragma unroll
-  for (int q = 0; q < 7; ++q) {
+  for (int q = 0; q < R; ++q) {
     float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-    for (int r = 0; r < 7; ++r) {
-      const int m = (q * r) % 7;
-      const float2 w = make_float2(c[m], sg * s[m]);
-      acc = cadd(acc, cmul(u[r], w));
+    for (int r = 0; r < R; ++r) {
+      const int m = (q * r) % R;
+      acc = cadd(acc, cmul(u[r], make_float2(c[m], sg * s[m])));
     }
     v[q] = acc;
   }
 #pragma unroll
-  for (int q = 0; q < 7; ++q) u[q] = v[q];
+  for (int q = 0; q < R; ++q) u[q] = v[q];
+}
+template <> __device__ __forceinline__ void dft<7>(float2 (&u)[MAXR], float sg) {
+  const float c[7] = {1.00000000000000000000f, 0.62348980185873359439f, -0.22252093395631433737f, -0.90096886790241903498f, -0.90096886790241914600f, -0.22252093395631458717f, 0.62348980185873337234f};
+  const float s[7] = {0.00000000000000000000f, 0.78183148246802980363f, 0.97492791218182361934f, 0.43388373911755823142f, -0.43388373911755800938f, -0.97492791218182361934f, -0.78183148246802991466f};
+  dft_direct<7>(u, sg, c, s);
+}
+template <> __device__ __forceinline__ void dft<11>(float2 (&u)[MAXR], float sg) {
+  const float c[11] = {1.00000000000000000000f, 0.84125353283118120551f, 0.41541501300188643508f, -0.14231483827328500480f, -0.65486073394528498959f, -0.95949297361449736865f, -0.95949297361449747967f, -0.65486073394528521163f, -0.14231483827328522684f, 0.41541501300188604651f, 0.84125353283118120551f};
+  const float s[11] = {0.00000000000000000000f, 0.54064081745559755543f, 0.90963199535451833011f, 0.98982144188093279524f, 0.75574957435425826890f, 0.28173255684142967104f, -0.28173255684142939348f, -0.75574957435425815788f, -0.98982144188093268422f, -0.90963199535451855215f, -0.54064081745559744441f};
+  dft_direct<11>(u, sg, c, s);
+}
+template <> __device__ __forceinline__ void dft<13>(float2 (&u)[MAXR], float sg) {
+  const float c[13] = {1.00000000000000000000f, 0.88545602565320991051f, 0.56806474673115592289f, 0.12053668025532300601f, -0.35460488704253545489f, -0.74851074817110119231f, -0.97094181742605201180f, -0.97094181742605212282f, -0.74851074817110130333f, -0.35460488704253589898f, 0.12053668025532320029f, 0.56806474673115481266f, 0.88545602565321002153f};
+  const float s[13] = {0.00000000000000000000f, 0.46472317204376850652f, 0.82298386589365635224f, 0.99270887409805397272f, 0.93501624268541483342f, 0.66312265824079519305f, 0.23931566428755768339f, -0.23931566428755743359f, -0.66312265824079497101f, -0.93501624268541472240f, -0.99270887409805397272f, -0.82298386589365701838f, -0.46472317204376839550f};
+  dft_direct<13>(u, sg, c, s);
 }
 
-template <int R>
-__device__ __forceinline__ void stage(const float2 *__restrict__ in, float2 *__restrict__ out, int K, int T,
-                                      int p, const float2 *__restrict__ tw, float sg) {
+template <int T> constexpr int ilog2() { return T <= 1 ? 0 : 1 + ilog2<T / 2>(); }
+
+// One Stockham stage over T lines stored [element*T + line]; twiddles exp(-2 pi i m/K) from
+// LDS (conjugated for the inverse, sg = +1).
+template <int R, int T>
+__device__ __forceinline__ void stage(const float2 *__restrict__ in, float2 *__restrict__ out, int K, int p,
+                                      unsigned magic, int tstride, const float2 *__restrict__ tw, float sg) {
+  constexpr int TL = ilog2<T>();
   const int KR = K / R;
-  const int tstride = K / (p * R);
-  const int nb = T * KR;
+  const int nb = KR << TL;
   for (int idx = threadIdx.x; idx < nb; idx += blockDim.x) {
-    const int l = idx % T;
-    const int i = idx / T;
-    const int k = i % p;
-    float2 u[7];
+    const int l = idx & (T - 1);
+    const int i = idx >> TL;
+    const int q = p == 1 ? i : (int)__umulhi((unsigned)i, magic);
+    const int k = i - q * p;
+    float2 u[MAXR];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       float2 v = in[(i + r * KR) * T + l];
       if (r > 0 && k > 0) {
         float2 w = tw[r * k * tstride];
-        w.y *= -sg;  // the table holds exp(-2 pi i m/K): kept forward, conjugated inverse
+        w.y *= -sg;
         v = cmul(v, w);
       }
       u[r] = v;
     }
     dft<R>(u, sg);
-    const int j = (i / p) * p * R + k;
+    const int j = q * p * R + k;
 #pragma unroll
     for (int r = 0; r < R; ++r) out[(j + r * p) * T + l] = u[r];
   }
 }
 
-// Full transform of T lines of length K held in buf0 ([e*T + l]); returns the buffer with
-// the result (buf0 or buf1). sg = -1 forward, +1 inverse (unnormalised).
-__device__ float2 *lds_fft(float2 *buf0, float2 *buf1, const FFTPlan &pl, int T, const float2 *__restrict__ tw,
-                           float sg) {
+// Full transform of T lines held in buf0; returns the buffer with the result.
+// sg = -1 forward, +1 inverse (unnormalised).
+template <int T, bool BIG>
+__device__ float2 *lds_fft(float2 *buf0, float2 *buf1, const FFTPlan &pl, const float2 *__restrict__ tw, float sg) {
   float2 *a = buf0, *b = buf1;
-  int p = 1;
   for (int s = 0; s < pl.nstage; ++s) {
-    const int R = pl.radix[s];
-    switch (R) {
-      case 4: stage<4>(a, b, pl.K, T, p, tw, sg); break;
-      case 2: stage<2>(a, b, pl.K, T, p, tw, sg); break;
-      case 3: stage<3>(a, b, pl.K, T, p, tw, sg); break;
-      case 5: stage<5>(a, b, pl.K, T, p, tw, sg); break;
-      default: stage<7>(a, b, pl.K, T, p, tw, sg); break;
+    const int p = pl.p[s];
+    const unsigned mg = pl.magic[s];
+    const int ts = pl.tstride[s];
+    switch (pl.radix[s]) {
+      case 4: stage<4, T>(a, b, pl.K, p, mg, ts, tw, sg); break;
+      case 2: stage<2, T>(a, b, pl.K, p, mg, ts, tw, sg); break;
+      case 3: stage<3, T>(a, b, pl.K, p, mg, ts, tw, sg); break;
+      case 5: stage<5, T>(a, b, pl.K, p, mg, ts, tw, sg); break;
+      default:
+        if (BIG) {
+          switch (pl.radix[s]) {
+            case 7: stage<7, T>(a, b, pl.K, p, mg, ts, tw, sg); break;
+            case 11: stage<11, T>(a, b, pl.K, p, mg, ts, tw, sg); break;
+            default: stage<13, T>(a, b, pl.K, p, mg, ts, tw, sg); break;
+          }
+        }
+        break;
     }
     __syncthreads();
     float2 *t = a; a = b; b = t;
-    p *= R;
   }
   return a;
 }
 
-// ---- x pass, forward: fixed-point (or float) real lines -> half spectrum
-template <bool FIXED>
-__global__ __launch_bounds__(TPB) void k_fft_x_r2c(const void *src, unsigned long long *clear, float2 *spec,
-                                                    int K1, int K1h, int nlines, int T, FFTPlan pl,
-                                                    const float2 *__restrict__ tw, double inv_scale) {
-  extern __shared__ float2 lds[];
-  float2 *b0 = lds, *b1 = lds + (size_t)T * K1;
-  const int line0 = blockIdx.x * T;
-  const int n = T * K1;
-  for (int e = threadIdx.x; e < n; e += TPB) {
-    const int l = e / K1, n1 = e % K1;
-    const int line = line0 + l;
-    float v = 0.f;
-    if (line < nlines) {
-      const size_t g = (size_t)line * K1 + n1;
-      if (FIXED) {
-        const long long q = (long long)((const unsigned long long *)src)[g];
-        v = (float)((double)q * inv_scale);
-        clear[g] = 0ull;
-      } else {
-        v = ((const float *)src)[g];
-      }
+__device__ __forceinline__ void load_twiddles(float2 *twl, const float2 *__restrict__ tw, int K) {
+  for (int k = threadIdx.x; k < K; k += blockDim.x) twl[k] = tw[k];
+}
+
+// Element e -> (global load, LDS store) for e in [0, n), CH loads in flight per thread: all
+// loads of a chunk are issued before the first LDS store waits on them.
+template <int CH, typename V, typename LOAD, typename STORE>
+__device__ __forceinline__ void staged(int n, LOAD load, STORE store) {
+  for (int base = 0; base < n; base += CH * TPB) {
+    V v[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int e = base + c * TPB + (int)threadIdx.x;
+      if (e < n) v[c] = load(e);
     }
-    b0[n1 * T + l] = make_float2(v, 0.f);
-  }
-  __syncthreads();
-  float2 *r = lds_fft(b0, b1, pl, T, tw, -1.0f);
-  const int nh = T * K1h;
-  for (int e = threadIdx.x; e < nh; e += TPB) {
-    const int l = e / K1h, m1 = e % K1h;
-    const int line = line0 + l;
-    if (line < nlines) spec[(size_t)line * K1h + m1] = r[m1 * T + l];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int e = base + c * TPB + (int)threadIdx.x;
+      if (e < n) store(e, v[c]);
+    }
   }
 }
 
-// ---- x pass, inverse: half spectrum -> real lines (Hermitian completion in LDS)
-__global__ __launch_bounds__(TPB) void k_fft_x_c2r(const float2 *spec, float *out, int K1, int K1h, int nlines,
-                                                   int T, FFTPlan pl, const float2 *__restrict__ tw) {
+// ---- x pass, forward: 2T real lines per workgroup, packed two at a time as the real and
+// imaginary parts of T complex lines (z = a + i b); after the transform
+// A[m] = (Z[m] + conj Z[K-m])/2, B[m] = (Z[m] - conj Z[K-m])/(2i). The fixed-point charge
+// grid is converted to float as it is read.
+template <int T, bool FIXED, bool BIG>
+__global__ __launch_bounds__(TPB) void k_fft_x_r2c(const void *src, unsigned long long *clear, float2 *spec,
+                                                    int K1, int K1h, int nlines, FFTPlan pl,
+                                                    const float2 *__restrict__ tw, double inv_scale) {
   extern __shared__ float2 lds[];
-  float2 *b0 = lds, *b1 = lds + (size_t)T * K1;
-  const int line0 = blockIdx.x * T;
-  const int n = T * K1;
-  for (int e = threadIdx.x; e < n; e += TPB) {
-    const int l = e / K1, m = e % K1;
-    const int line = line0 + l;
-    float2 v = make_float2(0.f, 0.f);
-    if (line < nlines) {
-      if (m < K1h) {
-        v = spec[(size_t)line * K1h + m];
-      } else {
-        const float2 c = spec[(size_t)line * K1h + (K1 - m)];
-        v = make_float2(c.x, -c.y);
-      }
-    }
-    b0[m * T + l] = v;
+  float2 *twl = lds, *b0 = lds + K1, *b1 = b0 + (size_t)T * K1;
+  load_twiddles(twl, tw, K1);
+  const int line0 = blockIdx.x * 2 * T;
+  const int nl = min(2 * T, nlines - line0);  // real lines of this tile, contiguous in memory
+  const size_t g0 = (size_t)line0 * K1;
+  const int n = nl * K1;
+  if (nl < 2 * T)  // a partial last tile: zero the unused line slots
+    for (int e = threadIdx.x; e < T * K1; e += TPB) b0[e] = make_float2(0.f, 0.f);
+  __syncthreads();
+  auto put = [&](int e, float v) {
+    const int lr = e / K1, n1 = e - lr * K1;
+    reinterpret_cast<float *>(&b0[n1 * T + (lr >> 1)])[lr & 1] = v;
+  };
+  if (FIXED) {
+    const unsigned long long *g = (const unsigned long long *)src + g0;
+    staged<8, float>(n, [&](int e) { return (float)((double)(long long)g[e] * inv_scale); }, put);
+  } else {
+    const float *g = (const float *)src + g0;
+    staged<8, float>(n, [&](int e) { return g[e]; }, put);
   }
   __syncthreads();
-  float2 *r = lds_fft(b0, b1, pl, T, tw, +1.0f);
+  float2 *r = lds_fft<T, BIG>(b0, b1, pl, twl, -1.0f);
+  const int nh = 2 * T * K1h;
+  for (int e = threadIdx.x; e < nh; e += TPB) {
+    const int lr = e / K1h, m = e - lr * K1h;
+    const int line = line0 + lr;
+    if (line >= nlines) continue;
+    const int c = lr >> 1;
+    const float2 z = r[m * T + c];
+    const float2 w = r[(m == 0 ? 0 : K1 - m) * T + c];
+    const float2 o = (lr & 1) ? make_float2(0.5f * (z.y + w.y), 0.5f * (w.x - z.x))
+                              : make_float2(0.5f * (z.x + w.x), 0.5f * (z.y - w.y));
+    spec[(size_t)line * K1h + m] = o;
+  }
+}
+
+// ---- x pass, inverse: two half-spectrum lines per complex line (Z = A + i B after each is
+// completed by Hermitian symmetry; the imaginary parts of the DC and Nyquist terms are
+// dropped, as a C2R transform does); real part -> line a, imaginary part -> line b.
+template <int T, bool BIG>
+__global__ __launch_bounds__(TPB) void k_fft_x_c2r(const float2 *spec, float *out, int K1, int K1h, int nlines,
+                                                   FFTPlan pl, const float2 *__restrict__ tw) {
+  extern __shared__ float2 lds[];
+  float2 *twl = lds, *b0 = lds + K1, *b1 = b0 + (size_t)T * K1;
+  load_twiddles(twl, tw, K1);
+  const int line0 = blockIdx.x * 2 * T;
+  // stage the tile's half-spectrum lines (contiguous) in the ping-pong buffer b1 first
+  const int nl = min(2 * T, nlines - line0);
+  const float2 *g = spec + (size_t)line0 * K1h;
+  staged<8, float2>(nl * K1h, [&](int e) { return g[e]; }, [&](int e, float2 v) { b1[e] = v; });
+  __syncthreads();
+  const int n = T * K1;
   for (int e = threadIdx.x; e < n; e += TPB) {
-    const int l = e / K1, n1 = e % K1;
-    const int line = line0 + l;
-    if (line < nlines) out[(size_t)line * K1 + n1] = r[n1 * T + l].x;
+    const int c = e / K1, m = e - c * K1;
+    const int la = 2 * c, lb = la + 1;
+    const bool conj = m >= K1h;
+    const int mm = conj ? K1 - m : m;
+    float2 A = la < nl ? b1[la * K1h + mm] : make_float2(0.f, 0.f);
+    float2 B = lb < nl ? b1[lb * K1h + mm] : make_float2(0.f, 0.f);
+    if (mm == 0 || 2 * mm == K1) { A.y = 0.f; B.y = 0.f; }
+    if (conj) { A.y = -A.y; B.y = -B.y; }
+    b0[m * T + c] = make_float2(A.x - B.y, A.y + B.x);
+  }
+  __syncthreads();
+  float2 *r = lds_fft<T, BIG>(b0, b1, pl, twl, +1.0f);
+  const int nr = 2 * T * K1;
+  for (int e = threadIdx.x; e < nr; e += TPB) {
+    const int lr = e / K1, n1 = e - lr * K1;
+    const int line = line0 + lr;
+    if (line >= nlines) continue;
+    const float2 z = r[n1 * T + (lr >> 1)];
+    out[(size_t)line * K1 + n1] = (lr & 1) ? z.y : z.x;
   }
 }
 
 // ---- y pass: tiles of T adjacent m1 at fixed m3; elements m2 at stride K1h
-__global__ __launch_bounds__(TPB) void k_fft_y(float2 *spec, int K1h, int K2, int T, FFTPlan pl,
+template <int T, bool BIG>
+__global__ __launch_bounds__(TPB) void k_fft_y(float2 *spec, int K1h, int K2, FFTPlan pl,
                                                const float2 *__restrict__ tw, float sg) {
   extern __shared__ float2 lds[];
-  float2 *b0 = lds, *b1 = lds + (size_t)T * K2;
+  float2 *twl = lds, *b0 = lds + K2, *b1 = b0 + (size_t)T * K2;
+  load_twiddles(twl, tw, K2);
   const int tiles = (K1h + T - 1) / T;
   const int m10 = (blockIdx.x % tiles) * T;
   const int m3 = blockIdx.x / tiles;
   const size_t base = (size_t)m3 * K2 * K1h;
+  constexpr int TL = ilog2<T>();
   const int n = T * K2;
-  for (int e = threadIdx.x; e < n; e += TPB) {
-    const int l = e % T, m2 = e / T;
-    const int m1 = m10 + l;
-    b0[m2 * T + l] = m1 < K1h ? spec[base + (size_t)m2 * K1h + m1] : make_float2(0.f, 0.f);
-  }
+  staged<8, float2>(n, [&](int e) {
+    const int l = e & (T - 1), m2 = e >> TL;
+    return m10 + l < K1h ? spec[base + (size_t)m2 * K1h + m10 + l] : make_float2(0.f, 0.f);
+  }, [&](int e, float2 v) { b0[e] = v; });
   __syncthreads();
-  float2 *r = lds_fft(b0, b1, pl, T, tw, sg);
+  float2 *r = lds_fft<T, BIG>(b0, b1, pl, twl, sg);
   for (int e = threadIdx.x; e < n; e += TPB) {
-    const int l = e % T, m2 = e / T;
+    const int l = e & (T - 1), m2 = e >> TL;
     const int m1 = m10 + l;
     if (m1 < K1h) spec[base + (size_t)m2 * K1h + m1] = r[m2 * T + l];
   }
@@ -266,32 +352,33 @@ struct SolveArgs {
 };
 
 // ---- z pass: MODE 0 forward only, 1 inverse only, 2 forward + solve + inverse
-template <int MODE, bool ENERGY>
-__global__ __launch_bounds__(TPB) void k_fft_z(float2 *spec, int T, FFTPlan pl, const float2 *__restrict__ tw,
+template <int T, int MODE, bool ENERGY, bool BIG>
+__global__ __launch_bounds__(TPB) void k_fft_z(float2 *spec, FFTPlan pl, const float2 *__restrict__ tw,
                                                SolveArgs s) {
   extern __shared__ float2 lds[];
   const int K1h = s.K1h, K2 = s.K2, K3 = s.K3;
-  float2 *b0 = lds, *b1 = lds + (size_t)T * K3;
+  float2 *twl = lds, *b0 = lds + K3, *b1 = b0 + (size_t)T * K3;
+  load_twiddles(twl, tw, K3);
   const int tiles = (K1h + T - 1) / T;
   const int m10 = (blockIdx.x % tiles) * T;
   const int m2 = blockIdx.x / tiles;
   const size_t plane = (size_t)K1h * K2;
   const size_t base = (size_t)m2 * K1h;
+  constexpr int TL = ilog2<T>();
   const int n = T * K3;
-  for (int e = threadIdx.x; e < n; e += TPB) {
-    const int l = e % T, m3 = e / T;
-    const int m1 = m10 + l;
-    b0[m3 * T + l] = m1 < K1h ? spec[base + (size_t)m3 * plane + m1] : make_float2(0.f, 0.f);
-  }
+  staged<8, float2>(n, [&](int e) {
+    const int l = e & (T - 1), m3 = e >> TL;
+    return m10 + l < K1h ? spec[base + (size_t)m3 * plane + m10 + l] : make_float2(0.f, 0.f);
+  }, [&](int e, float2 v) { b0[e] = v; });
   __syncthreads();
   float2 *r = b0;
-  if (MODE != 1) r = lds_fft(b0, b1, pl, T, tw, -1.0f);
+  if (MODE != 1) r = lds_fft<T, BIG>(b0, b1, pl, twl, -1.0f);
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   if (MODE == 2) {
     const int s2 = m2 <= K2 / 2 ? m2 : m2 - K2;
     const bool ny2 = (K2 % 2 == 0) && (m2 == K2 / 2);
     for (int e = threadIdx.x; e < n; e += TPB) {
-      const int l = e % T, m3 = e / T;
+      const int l = e & (T - 1), m3 = e >> TL;
       const int m1 = m10 + l;
       if (m1 >= K1h) continue;
       const size_t idx = base + (size_t)m3 * plane + m1;
@@ -318,12 +405,12 @@ __global__ __launch_bounds__(TPB) void k_fft_z(float2 *spec, int T, FFTPlan pl, 
     }
     __syncthreads();
     float2 *other = (r == b0) ? b1 : b0;
-    r = lds_fft(r, other, pl, T, tw, +1.0f);
+    r = lds_fft<T, BIG>(r, other, pl, twl, +1.0f);
   } else if (MODE == 1) {
-    r = lds_fft(b0, b1, pl, T, tw, +1.0f);
+    r = lds_fft<T, BIG>(b0, b1, pl, twl, +1.0f);
   }
   for (int e = threadIdx.x; e < n; e += TPB) {
-    const int l = e % T, m3 = e / T;
+    const int l = e & (T - 1), m3 = e >> TL;
     const int m1 = m10 + l;
     if (m1 < K1h) spec[base + (size_t)m3 * plane + m1] = r[m3 * T + l];
   }
@@ -428,6 +515,143 @@ __global__ __launch_bounds__(TPB) void k_spread(int N, const float *__restrict__
       const long long v = __float2ll_rn(q12 * wz[j3]);
       atomicAdd(&grid[b + KK * n3], (unsigned long long)v);
     }
+  }
+}
+
+__global__ void k_zero64(unsigned long long *g, size_t n) {
+  const size_t k = (size_t)blockIdx.x * TPB + threadIdx.x;
+  if (k < n) g[k] = 0ull;
+}
+
+// ---- spread, owner computes: one workgroup per TILE^3 block of grid nodes. It scans the
+// cells of the neighbour list that can hold a particle whose P^3 stencil reaches the block
+// (a particle at u covers nodes floor(u)-P+1 .. floor(u); cells are widened by the list's
+// skin/2, the most any particle moved since it was binned), accumulates q w_x w_y w_z in
+// 64-bit fixed point with LDS integer atomics (exact, order-independent) and stores the
+// block with plain writes: no global atomics, and no clearing (every node is written).
+constexpr int STILE = 16;
+constexpr int SLIST = 1536;  // particles reaching one block staged in LDS (a few hundred in practice)
+template <int P>
+__global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__ packed,
+                                                      const int32_t *__restrict__ offsets,
+                                                      const int32_t *__restrict__ ids,
+                                                      const int64_t *__restrict__ state, int nx, int ny,
+                                                      int nz, float margin, unsigned long long *grid, int K1,
+                                                      int K2, int K3, float Lx, float Ly, float Lz, float scale) {
+  if (state[PDX_DN_ERROR] != 0) return;  // no usable cells / positions: the grid keeps its last value
+  constexpr int LPP = lanes_for(P);
+  __shared__ unsigned long long acc[STILE * STILE * STILE];
+  __shared__ float4 list[SLIST];
+  __shared__ int cstart[64], cpre[65];
+  __shared__ int nlist;
+  const int tx = (K1 + STILE - 1) / STILE, ty = (K2 + STILE - 1) / STILE;
+  const int b = blockIdx.x;
+  const int a1 = (b % tx) * STILE, a2 = ((b / tx) % ty) * STILE, a3 = (b / (tx * ty)) * STILE;
+  const int e1 = min(STILE, K1 - a1), e2 = min(STILE, K2 - a2), e3 = min(STILE, K3 - a3);
+  for (int k = threadIdx.x; k < STILE * STILE * STILE; k += TPB) acc[k] = 0ull;
+  // candidate cells per dimension, in the list's frame (x + L/2) / L
+  const int K[3] = {K1, K2, K3}, A[3] = {a1, a2, a3}, E[3] = {e1, e2, e3}, NC[3] = {nx, ny, nz};
+  const float Ls[3] = {Lx, Ly, Lz};
+  int c0[3], cn[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float f0 = (float)A[d] / (float)K[d] + 0.5f - margin / Ls[d] - 1e-5f;
+    const float f1 = (float)(A[d] + E[d] + P - 1) / (float)K[d] + 0.5f + margin / Ls[d] + 1e-5f;
+    const int lo = (int)floorf(f0 * NC[d]);
+    c0[d] = lo;
+    cn[d] = min((int)floorf(f1 * NC[d]) - lo + 1, NC[d]);
+  }
+  const int ncand = cn[0] * cn[1] * cn[2];
+  // (the cell range is at most 64 cells: a block plus its halo spans < 4 cells per dimension
+  // whenever the cells are wider than (STILE + P + skin) / 3 mesh spacings; otherwise the
+  // remaining cells are taken in further rounds of 64)
+  if (threadIdx.x == 0) nlist = 0;
+  for (int round = 0; round * 64 < ncand; ++round) {
+    const int nc = min(64, ncand - round * 64);
+    if (threadIdx.x < nc) {
+      const int cc = round * 64 + threadIdx.x;
+      int cx = c0[0] + cc % cn[0], cy = c0[1] + (cc / cn[0]) % cn[1], cz = c0[2] + cc / (cn[0] * cn[1]);
+      cx = ((cx % nx) + nx) % nx;
+      cy = ((cy % ny) + ny) % ny;
+      cz = ((cz % nz) + nz) % nz;
+      const int c = (cz * ny + cy) * nx + cx;
+      cstart[threadIdx.x] = offsets[c] - 1;
+      cpre[threadIdx.x + 1] = offsets[c + 1] - offsets[c];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      cpre[0] = 0;
+      for (int k = 1; k <= nc; ++k) cpre[k] += cpre[k - 1];
+    }
+    __syncthreads();
+    const int total = cpre[nc];
+    // flat candidate index -> cell (linear search over <= 64 prefix entries) -> particle
+    for (int f = threadIdx.x; f < total; f += TPB) {
+      int c = 0;
+      while (cpre[c + 1] <= f) ++c;
+      const float4 p = packed[ids[cstart[c] + (f - cpre[c])] - 1];
+      if (p.w == 0.0f) continue;
+      const float u1 = grid_coordinate(p.x, Lx, K1), u2 = grid_coordinate(p.y, Ly, K2),
+                  u3 = grid_coordinate(p.z, Lz, K3);
+      int o1 = (int)floorf(u1) - a1; o1 += o1 < 0 ? K1 : 0;
+      int o2 = (int)floorf(u2) - a2; o2 += o2 < 0 ? K2 : 0;
+      int o3 = (int)floorf(u3) - a3; o3 += o3 < 0 ? K3 : 0;
+      if (o1 > e1 + P - 2 || o2 > e2 + P - 2 || o3 > e3 + P - 2) continue;
+      const int slot = atomicAdd(&nlist, 1);
+      if (slot < SLIST) {
+        list[slot] = make_float4(u1, u2, u3, p.w);
+      } else {  // list full: this thread spreads the particle itself
+        float wx[P], wy[P], wz[P], d[P];
+        bspline<P>(u1, wx, d);
+        bspline<P>(u2, wy, d);
+        bspline<P>(u3, wz, d);
+        for (int j2 = 0; j2 < P; ++j2) {
+          int l2 = o2 - (P - 1) + j2; l2 += l2 < 0 ? K2 : 0; l2 -= l2 >= K2 ? K2 : 0;
+          if (l2 >= e2) continue;
+          for (int j1 = 0; j1 < P; ++j1) {
+            int l1 = o1 - (P - 1) + j1; l1 += l1 < 0 ? K1 : 0; l1 -= l1 >= K1 ? K1 : 0;
+            if (l1 >= e1) continue;
+            const float q12 = p.w * wx[j1] * wy[j2] * scale;
+            for (int j3 = 0; j3 < P; ++j3) {
+              int l3 = o3 - (P - 1) + j3; l3 += l3 < 0 ? K3 : 0; l3 -= l3 >= K3 ? K3 : 0;
+              if (l3 >= e3) continue;
+              atomicAdd(&acc[(l3 * STILE + l2) * STILE + l1], (unsigned long long)__float2ll_rn(q12 * wz[j3]));
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // staged particles: LPP lanes each, lane = one (x, y) line of the stencil walking P z nodes;
+  // the products and their order are those of k_spread: ((q w_x) w_y) 2^S, then times w_z
+  const int nl = min(nlist, SLIST);
+  const int grp = threadIdx.x / LPP, lane = threadIdx.x % LPP;
+  for (int t = grp; t < nl; t += TPB / LPP) {
+    const float4 u = list[t];
+    float wx[P], wy[P], wz[P], d[P];
+    const int k1 = bspline<P>(u.x, wx, d);
+    const int k2 = bspline<P>(u.y, wy, d);
+    const int k3 = bspline<P>(u.z, wz, d);
+    for (int l = lane; l < P * P; l += LPP) {
+      const int j1 = l % P, j2 = l / P;
+      int l1 = wrap_node(k1 + j1, K1) - a1; l1 += l1 < 0 ? K1 : 0;
+      int l2 = wrap_node(k2 + j2, K2) - a2; l2 += l2 < 0 ? K2 : 0;
+      if (l1 >= e1 || l2 >= e2) continue;
+      const float q12 = u.w * sel<P>(wx, j1) * sel<P>(wy, j2) * scale;
+#pragma unroll
+      for (int j3 = 0; j3 < P; ++j3) {
+        int l3 = wrap_node(k3 + j3, K3) - a3; l3 += l3 < 0 ? K3 : 0;
+        if (l3 >= e3) continue;
+        atomicAdd(&acc[(l3 * STILE + l2) * STILE + l1], (unsigned long long)__float2ll_rn(q12 * wz[j3]));
+      }
+    }
+  }
+  __syncthreads();
+  const size_t KK = (size_t)K1 * K2;
+  for (int k = threadIdx.x; k < STILE * STILE * STILE; k += TPB) {
+    const int l1 = k % STILE, l2 = (k / STILE) % STILE, l3 = k / (STILE * STILE);
+    if (l1 < e1 && l2 < e2 && l3 < e3) grid[(size_t)(a3 + l3) * KK + (size_t)(a2 + l2) * K1 + (a1 + l1)] = acc[k];
   }
 }
 
@@ -538,10 +762,14 @@ std::vector<double> deconvolution(int P, int K) {
   return D;
 }
 
-int lines_per_tile(int K) {
-  int T = 32;
-  // two ping-pong buffers within 60 KiB, leaving room for the reductions' static LDS
-  while (T > 1 && (size_t)2 * T * K * sizeof(float2) > 61440) T >>= 1;
+// Lines per workgroup tile (a power of two, 4..32): overridable per pass for tuning through
+// PDX_FFT_TX / PDX_FFT_TY / PDX_FFT_TZ; kept within a 60 KiB LDS budget for the twiddles and
+// the two ping-pong buffers (the reductions' static LDS comes on top).
+int lines_per_tile(int K, int def, const char *env) {
+  int T = def;
+  if (const char *v = std::getenv(env)) T = std::atoi(v);
+  if (T != 4 && T != 8 && T != 16 && T != 32) T = def;
+  while (T > 4 && ((size_t)2 * T * K + K + 2 * T) * sizeof(float2) > 61440) T >>= 1;
   return T;
 }
 
@@ -585,7 +813,7 @@ void refresh_background(pdx_mesh_t m) {
 
 struct Plans {
   FFTPlan p1, p2, p3;
-  int T1, T2, T3;
+  int T1, T2, T3;  // complex lines per tile (the x passes pack two real lines per complex line)
 };
 
 Plans plans_of(pdx_mesh_t m) {
@@ -593,9 +821,14 @@ Plans plans_of(pdx_mesh_t m) {
   make_plan(m->K1, pl.p1);
   make_plan(m->K2, pl.p2);
   make_plan(m->K3, pl.p3);
-  pl.T1 = lines_per_tile(m->K1);
-  pl.T2 = lines_per_tile(m->K2);
-  pl.T3 = lines_per_tile(m->K3);
+  if (m->T1 == 0) {
+    m->T1 = lines_per_tile(m->K1, 8, "PDX_FFT_TX");
+    m->T2 = lines_per_tile(m->K2, 16, "PDX_FFT_TY");
+    m->T3 = lines_per_tile(m->K3, 16, "PDX_FFT_TZ");
+  }
+  pl.T1 = m->T1;
+  pl.T2 = m->T2;
+  pl.T3 = m->T3;
   return pl;
 }
 
@@ -611,20 +844,45 @@ SolveArgs solve_args(pdx_mesh_t m) {
   return s;
 }
 
+// twiddles (K) + two ping-pong buffers (T K each) + 2T spare slots: the inverse x pass stages
+// 2T half-spectrum lines (2T (K/2 + 1) values) in the second buffer
+size_t fft_lds(int T, int K) { return ((size_t)2 * T * K + K + 2 * T) * sizeof(float2); }
+
+#define PDX_T_SWITCH(T, CALL) \
+  switch (T) {                \
+    case 4: CALL(4); break;   \
+    case 8: CALL(8); break;   \
+    case 16: CALL(16); break; \
+    default: CALL(32); break; \
+  }
+
 int enqueue_forward_xy(pdx_mesh_t m, const Plans &pl, const void *src, bool fixed, hipStream_t st) {
   const float2 *tw1 = m->tw.p, *tw2 = m->tw.p + m->K1;
   const int nlines = m->K2 * m->K3;
-  const size_t sh1 = (size_t)2 * pl.T1 * m->K1 * sizeof(float2);
-  const int gx = (nlines + pl.T1 - 1) / pl.T1;
-  if (fixed)
-    hipLaunchKernelGGL(k_fft_x_r2c<true>, dim3(gx), dim3(TPB), sh1, st, src, m->grid64.p, m->spec_c.p, m->K1,
-                       m->K1h, nlines, pl.T1, pl.p1, tw1, m->fixed_inv);
-  else
-    hipLaunchKernelGGL(k_fft_x_r2c<false>, dim3(gx), dim3(TPB), sh1, st, src, nullptr, m->spec_c.p, m->K1, m->K1h,
-                       nlines, pl.T1, pl.p1, tw1, 1.0);
+  const int gx = (nlines + 2 * pl.T1 - 1) / (2 * pl.T1);
+  const size_t sh1 = fft_lds(pl.T1, m->K1);
+#define PDX_XF2(TT, BB)                                                                                       \
+  if (fixed)                                                                                                  \
+    hipLaunchKernelGGL((k_fft_x_r2c<TT, true, BB>), dim3(gx), dim3(TPB), sh1, st, src, m->grid64.p,            \
+                       m->spec_c.p, m->K1, m->K1h, nlines, pl.p1, tw1, m->fixed_inv);                          \
+  else                                                                                                        \
+    hipLaunchKernelGGL((k_fft_x_r2c<TT, false, BB>), dim3(gx), dim3(TPB), sh1, st, src, nullptr, m->spec_c.p,  \
+                       m->K1, m->K1h, nlines, pl.p1, tw1, 1.0);
+#define PDX_XF(TT) \
+  if (pl.p1.big) { PDX_XF2(TT, true) } else { PDX_XF2(TT, false) }
+  PDX_T_SWITCH(pl.T1, PDX_XF)
+#undef PDX_XF
+#undef PDX_XF2
   const int gy = ((m->K1h + pl.T2 - 1) / pl.T2) * m->K3;
-  hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(TPB), (size_t)2 * pl.T2 * m->K2 * sizeof(float2), st, m->spec_c.p,
-                     m->K1h, m->K2, pl.T2, pl.p2, tw2, -1.0f);
+#define PDX_YF(TT)                                                                                         \
+  if (pl.p2.big)                                                                                             \
+    hipLaunchKernelGGL((k_fft_y<TT, true>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, m->spec_c.p, m->K1h,  \
+                       m->K2, pl.p2, tw2, -1.0f);                                                            \
+  else                                                                                                       \
+    hipLaunchKernelGGL((k_fft_y<TT, false>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, m->spec_c.p, m->K1h, \
+                       m->K2, pl.p2, tw2, -1.0f);
+  PDX_T_SWITCH(pl.T2, PDX_YF)
+#undef PDX_YF
   PDX_LAUNCH_CHECK();
   return PDX_OK;
 }
@@ -632,12 +890,26 @@ int enqueue_forward_xy(pdx_mesh_t m, const Plans &pl, const void *src, bool fixe
 int enqueue_inverse_yx(pdx_mesh_t m, const Plans &pl, float *dst, hipStream_t st) {
   const float2 *tw1 = m->tw.p, *tw2 = m->tw.p + m->K1;
   const int gy = ((m->K1h + pl.T2 - 1) / pl.T2) * m->K3;
-  hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(TPB), (size_t)2 * pl.T2 * m->K2 * sizeof(float2), st, m->spec_c.p,
-                     m->K1h, m->K2, pl.T2, pl.p2, tw2, +1.0f);
+#define PDX_YI(TT)                                                                                         \
+  if (pl.p2.big)                                                                                             \
+    hipLaunchKernelGGL((k_fft_y<TT, true>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, m->spec_c.p, m->K1h,  \
+                       m->K2, pl.p2, tw2, +1.0f);                                                            \
+  else                                                                                                       \
+    hipLaunchKernelGGL((k_fft_y<TT, false>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, m->spec_c.p, m->K1h, \
+                       m->K2, pl.p2, tw2, +1.0f);
+  PDX_T_SWITCH(pl.T2, PDX_YI)
+#undef PDX_YI
   const int nlines = m->K2 * m->K3;
-  hipLaunchKernelGGL(k_fft_x_c2r, dim3((nlines + pl.T1 - 1) / pl.T1), dim3(TPB),
-                     (size_t)2 * pl.T1 * m->K1 * sizeof(float2), st, m->spec_c.p, dst, m->K1, m->K1h, nlines, pl.T1,
-                     pl.p1, tw1);
+  const int gx = (nlines + 2 * pl.T1 - 1) / (2 * pl.T1);
+#define PDX_XI(TT)                                                                                            \
+  if (pl.p1.big)                                                                                                \
+    hipLaunchKernelGGL((k_fft_x_c2r<TT, true>), dim3(gx), dim3(TPB), fft_lds(TT, m->K1), st, m->spec_c.p, dst,    \
+                       m->K1, m->K1h, nlines, pl.p1, tw1);                                                      \
+  else                                                                                                          \
+    hipLaunchKernelGGL((k_fft_x_c2r<TT, false>), dim3(gx), dim3(TPB), fft_lds(TT, m->K1), st, m->spec_c.p, dst,   \
+                       m->K1, m->K1h, nlines, pl.p1, tw1);
+  PDX_T_SWITCH(pl.T1, PDX_XI)
+#undef PDX_XI
   PDX_LAUNCH_CHECK();
   return PDX_OK;
 }
@@ -646,8 +918,15 @@ template <int MODE, bool ENERGY>
 void launch_z(pdx_mesh_t m, const Plans &pl, hipStream_t st) {
   const float2 *tw3 = m->tw.p + m->K1 + m->K2;
   const int gz = ((m->K1h + pl.T3 - 1) / pl.T3) * m->K2;
-  hipLaunchKernelGGL((k_fft_z<MODE, ENERGY>), dim3(gz), dim3(TPB), (size_t)2 * pl.T3 * m->K3 * sizeof(float2), st,
-                     m->spec_c.p, pl.T3, pl.p3, tw3, solve_args(m));
+#define PDX_ZZ(TT)                                                                                              \
+  if (pl.p3.big)                                                                                                  \
+    hipLaunchKernelGGL((k_fft_z<TT, MODE, ENERGY, true>), dim3(gz), dim3(TPB), fft_lds(TT, m->K3), st,              \
+                       m->spec_c.p, pl.p3, tw3, solve_args(m));                                                  \
+  else                                                                                                            \
+    hipLaunchKernelGGL((k_fft_z<TT, MODE, ENERGY, false>), dim3(gz), dim3(TPB), fft_lds(TT, m->K3), st,             \
+                       m->spec_c.p, pl.p3, tw3, solve_args(m));
+  PDX_T_SWITCH(pl.T3, PDX_ZZ)
+#undef PDX_ZZ
 }
 
 #define PDX_SPREAD_CASE(PP)                                                                                    \
@@ -718,8 +997,9 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
     if (K[d] < sp.order) return fail(PDX_ERR_ARGUMENT, "Every mesh dimension must be at least the assignment order.");
     FFTPlan pl;
     if (!make_plan(K[d], pl))
-      return fail(PDX_ERR_ARGUMENT, "Mesh dimensions must have prime factors in {2, 3, 5, 7} (fft_friendly).");
-    if (K[d] > 4096) return fail(PDX_ERR_ARGUMENT, "Mesh dimensions above 4096 are not supported.");
+      return fail(PDX_ERR_ARGUMENT, "Mesh dimensions must have prime factors in {2, 3, 5, 7, 11, 13}.");
+    // one LDS tile holds 4 lines of K complex values twice plus the twiddles
+    if (K[d] > 760) return fail(PDX_ERR_ARGUMENT, "Mesh dimensions above 760 are not supported.");
   }
   const int64_t nnodes = (int64_t)K[0] * K[1] * K[2];
   if (nnodes >= INT32_MAX) return fail(PDX_ERR_ARGUMENT, "Mesh is too large for 32-bit grid indexing.");
@@ -825,6 +1105,8 @@ extern "C" int pdx_mesh_spread(pdx_mesh_t m, const float *x, const float *y, con
   const float L[3] = {(float)m->box[0], (float)m->box[1], (float)m->box[2]};
   const float scale = (float)m->fixed_scale;
   hipStream_t st = as_stream(stream);
+  const size_t nn = (size_t)m->K1 * m->K2 * m->K3;
+  hipLaunchKernelGGL(k_zero64, dim3((nn + TPB - 1) / TPB), dim3(TPB), 0, st, m->grid64.p, nn);
   switch (m->P) {
     PDX_SPREAD_CASE(3)
     PDX_SPREAD_CASE(4)
@@ -834,6 +1116,38 @@ extern "C" int pdx_mesh_spread(pdx_mesh_t m, const float *x, const float *y, con
     PDX_SPREAD_CASE(8)
     default: return fail(PDX_ERR_ARGUMENT, "unsupported order");
   }
+  PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
+extern "C" int pdx_mesh_spread_cells(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t stream) {
+  if (!m || !nl) return fail(PDX_ERR_ARGUMENT, "null argument");
+  if (nl->N != m->N) return fail(PDX_ERR_DIMENSION, "neighbor list and mesh particle counts differ");
+  if (nl->allpairs) return fail(PDX_ERR_ARGUMENT, "all-pairs lists have no cells; use pdx_mesh_spread");
+  for (int d = 0; d < 3; ++d)
+    if (std::fabs(nl->box[d] - m->box[d]) > 1e-12 * m->box[d])
+      return fail(PDX_ERR_ARGUMENT, "neighbor list and mesh boxes differ");
+  const float L[3] = {(float)m->box[0], (float)m->box[1], (float)m->box[2]};
+  const float scale = (float)m->fixed_scale;
+  const int tiles = ((m->K1 + STILE - 1) / STILE) * ((m->K2 + STILE - 1) / STILE) * ((m->K3 + STILE - 1) / STILE);
+  const float margin = 0.5f * (float)nl->skin;
+  hipStream_t st = as_stream(stream);
+#define PDX_TSPREAD(PP)                                                                                        \
+  case PP:                                                                                                     \
+    hipLaunchKernelGGL(k_spread_tiles<PP>, dim3(tiles), dim3(TPB), 0, st, nl->packed.p, nl->cell_offsets.p,     \
+                       nl->particle_ids_sorted.p, nl->state.p, nl->nx, nl->ny, nl->nz, margin, m->grid64.p, m->K1, \
+                       m->K2, m->K3, L[0], L[1], L[2], scale);                                                 \
+    break;
+  switch (m->P) {
+    PDX_TSPREAD(3)
+    PDX_TSPREAD(4)
+    PDX_TSPREAD(5)
+    PDX_TSPREAD(6)
+    PDX_TSPREAD(7)
+    PDX_TSPREAD(8)
+    default: return fail(PDX_ERR_ARGUMENT, "unsupported order");
+  }
+#undef PDX_TSPREAD
   PDX_LAUNCH_CHECK();
   return PDX_OK;
 }

@@ -16,6 +16,7 @@
 #include "pdx_internal.hpp"
 
 #include <cmath>
+#include <cstdlib>
 
 using namespace pdx;
 
@@ -220,26 +221,43 @@ __global__ __launch_bounds__(TPB) void k_scatter(int N, const int32_t *cell_of_p
   ids_sorted[offsets[c] - 1 + cell_rank[i]] = i + 1;
 }
 
-// Stable order inside each cell (ascending id; cells hold tens of particles) and the
-// cell-ordered packed copy (x, y, z, id) the row kernel streams through.
-__global__ __launch_bounds__(TPB) void k_sort_cells(int ncell, const int32_t *offsets,
-                                                     int32_t *ids_sorted,
-                                                     const float4 *__restrict__ packed,
-                                                     float4 *sorted, const int64_t *state) {
+// Stable order inside each cell (ascending id) and the cell-ordered packed copy (x, y, z, id)
+// the row kernel streams through. One wave per cell: the cell's ids are staged in LDS and each
+// lands at its rank (ids are distinct); cells beyond the LDS stage are sorted by one lane.
+constexpr int SORT_STAGE = 2048;
+__global__ __launch_bounds__(WAVE) void k_sort_cells(int ncell, const int32_t *offsets, int32_t *ids_sorted,
+                                                     const float4 *__restrict__ packed, float4 *sorted,
+                                                     const int64_t *state) {
   if (!rebuild_requested(state)) return;
-  const int c = blockIdx.x * TPB + threadIdx.x;
-  if (c >= ncell) return;
-  const int s = offsets[c] - 1, e = offsets[c + 1] - 1;
-  for (int a = s + 1; a < e; ++a) {
-    const int32_t key = ids_sorted[a];
-    int b = a - 1;
-    while (b >= s && ids_sorted[b] > key) {
-      ids_sorted[b + 1] = ids_sorted[b];
-      --b;
+  __shared__ int32_t sh[SORT_STAGE];
+  const int c = blockIdx.x;
+  const int s = offsets[c] - 1, e = offsets[c + 1] - 1, n = e - s;
+  if (n <= SORT_STAGE) {
+    for (int k = threadIdx.x; k < n; k += WAVE) sh[k] = ids_sorted[s + k];
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += WAVE) {
+      const int32_t key = sh[k];
+      int rank = 0;
+      for (int m = 0; m < n; ++m) rank += sh[m] < key;
+      ids_sorted[s + rank] = key;
+      const float4 p = packed[key - 1];
+      sorted[s + rank] = make_float4(p.x, p.y, p.z, __int_as_float(key - 1));
     }
-    ids_sorted[b + 1] = key;
+    return;
   }
-  for (int k = s; k < e; ++k) {
+  if (threadIdx.x == 0) {
+    for (int a = s + 1; a < e; ++a) {
+      const int32_t key = ids_sorted[a];
+      int b = a - 1;
+      while (b >= s && ids_sorted[b] > key) {
+        ids_sorted[b + 1] = ids_sorted[b];
+        --b;
+      }
+      ids_sorted[b + 1] = key;
+    }
+  }
+  __syncthreads();
+  for (int k = s + threadIdx.x; k < e; k += WAVE) {
     const int i = ids_sorted[k] - 1;
     const float4 p = packed[i];
     sorted[k] = make_float4(p.x, p.y, p.z, __int_as_float(i));
@@ -247,11 +265,15 @@ __global__ __launch_bounds__(TPB) void k_sort_cells(int ncell, const int32_t *of
 }
 
 // ------------------------------------------------------------------ rows
-// One thread per particle in cell order (threads of a wave share their home cell, so the
-// candidate loads are broadcast), 27-cell stencil, mic_fast differences and the reference's
-// r^2 = muladd(dx,dx,muladd(dy,dy,dz^2)) <= rl^2 test (_kernel_neighbors3!,
-// DenseNeighborCUDA.jl:153-206). Row slots are written slot-major at t*N + i with 1-based
-// ids; the count keeps growing past cap so the commit can report the required capacity.
+// G lanes per particle, particles in cell order (the groups of a wave share their home cell,
+// so candidate loads are broadcast). The group walks the 27-cell stencil G candidates at a
+// time; accepted neighbours are placed with a wave64 ballot + popcount in candidate order, so
+// every row is identical whatever G is. Distance test as the reference: mic_fast differences,
+// r^2 = muladd(dx,dx,muladd(dy,dy,dz^2)) <= rl^2 (_kernel_neighbors3!, DenseNeighborCUDA.jl:153-206;
+// cooperative placement as _cooperative_rows3_kernel!, src/engine/Builder.jl:39-129). Slots are
+// written slot-major at t*N + i with 1-based ids; the count keeps growing past cap so the
+// commit can report the required capacity.
+template <int G>
 __global__ __launch_bounds__(TPB) void k_rows(int N, int cap, int nx, int ny, int nz,
                                                const float4 *__restrict__ sorted,
                                                const int32_t *__restrict__ offsets,
@@ -260,8 +282,13 @@ __global__ __launch_bounds__(TPB) void k_rows(int N, int cap, int nx, int ny, in
                                                int32_t *counts, int64_t *candidates,
                                                const int64_t *state) {
   if (!rebuild_requested(state)) return;
-  const int t = blockIdx.x * TPB + threadIdx.x;
-  if (t >= N) return;
+  const int gid = blockIdx.x * TPB + threadIdx.x;
+  const int t = gid / G;
+  const int lg = gid % G;
+  const int wl = threadIdx.x & (WAVE - 1);
+  const unsigned long long gmask = (G == WAVE) ? ~0ull : (((1ull << G) - 1ull) << (wl - lg));
+  const unsigned long long below = (1ull << wl) - 1ull;
+  if (t >= N) return;  // whole groups leave together (G divides the block)
   const float Lx = geom[0], Ly = geom[1], Lz = geom[2];
   const float hx = 0.5f * Lx, hy = 0.5f * Ly, hz = 0.5f * Lz;
   const float4 pi = sorted[t];
@@ -278,25 +305,43 @@ __global__ __launch_bounds__(TPB) void k_rows(int N, int cap, int nx, int ny, in
         int cx2 = cx + ox; cx2 -= (cx2 >= nx) ? nx : 0; cx2 += (cx2 < 0) ? nx : 0;
         const int c = (cz2 * ny + cy2) * nx + cx2;
         const int s = offsets[c] - 1, e = offsets[c + 1] - 1;
-        for (int k = s; k < e; ++k) {
-          const float4 pj = sorted[k];
-          const int j = __float_as_int(pj.w);
-          if (j == i) continue;
-          ++tested;
-          const float dx = mic_fast(pj.x - pi.x, hx, Lx);
-          const float dy = mic_fast(pj.y - pi.y, hy, Ly);
-          const float dz = mic_fast(pj.z - pi.z, hz, Lz);
-          const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          if (r2 <= rl2) {
-            if (found < cap) nbr[(size_t)found * N + i] = j + 1;
-            ++found;
+        for (int k0 = s; k0 < e; k0 += G) {
+          const int k = k0 + lg;
+          bool acc = false, cand = false;
+          int j = 0;
+          if (k < e) {
+            const float4 pj = sorted[k];
+            j = __float_as_int(pj.w);
+            cand = j != i;
+            const float dx = mic_fast(pj.x - pi.x, hx, Lx);
+            const float dy = mic_fast(pj.y - pi.y, hy, Ly);
+            const float dz = mic_fast(pj.z - pi.z, hz, Lz);
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            acc = cand && (r2 <= rl2);
+          }
+          if (G == 1) {
+            tested += cand;
+            if (acc) {
+              if (found < cap) nbr[(size_t)found * N + i] = j + 1;
+              ++found;
+            }
+          } else {
+            const unsigned long long am = __ballot(acc) & gmask;
+            tested += __popcll(__ballot(cand) & gmask);
+            if (acc) {
+              const int slot = found + __popcll(am & below);
+              if (slot < cap) nbr[(size_t)slot * N + i] = j + 1;
+            }
+            found += __popcll(am);
           }
         }
       }
     }
   }
-  counts[i] = found;
-  if (candidates) candidates[i] = tested;
+  if (lg == 0) {
+    counts[i] = found;
+    if (candidates) candidates[i] = tested;
+  }
 }
 
 // All-pairs rows for small boxes, one image (_kernel_neighbors_allpairs!, DenseNeighborCUDA.jl:128-151).
@@ -452,11 +497,23 @@ int enqueue_rebuild(pdx_nl_t nl, hipStream_t st) {
                        nl->cell_offsets.p, nl->scan_block.p, s);
     hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(TPB), 0, st, N, nl->cell_of_particle.p, nl->cell_rank.p,
                        nl->cell_offsets.p, nl->particle_ids_sorted.p, s);
-    hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(nl->ncell, TPB)), dim3(TPB), 0, st, nl->ncell,
-                       nl->cell_offsets.p, nl->particle_ids_sorted.p, nl->packed.p, nl->sorted.p, s);
-    hipLaunchKernelGGL(k_rows, dim3(nb), dim3(TPB), 0, st, N, nl->cap, nl->nx, nl->ny, nl->nz, nl->sorted.p,
-                       nl->cell_offsets.p, nl->cell_of_particle.p, geom, nl->rl2, nl->neighbors_flat.p,
-                       nl->counts.p, nl->diagnostics ? nl->candidates.p : nullptr, s);
+    hipLaunchKernelGGL(k_sort_cells, dim3(nl->ncell), dim3(WAVE), 0, st, nl->ncell, nl->cell_offsets.p,
+                       nl->particle_ids_sorted.p, nl->packed.p, nl->sorted.p, s);
+    const int G = nl->row_lanes;
+    const int rb = (int)(((int64_t)N * G + TPB - 1) / TPB);
+#define PDX_ROWS(GG)                                                                                       \
+  hipLaunchKernelGGL(k_rows<GG>, dim3(rb), dim3(TPB), 0, st, N, nl->cap, nl->nx, nl->ny, nl->nz, nl->sorted.p, \
+                     nl->cell_offsets.p, nl->cell_of_particle.p, geom, nl->rl2, nl->neighbors_flat.p,           \
+                     nl->counts.p, nl->diagnostics ? nl->candidates.p : nullptr, s);
+    switch (G) {
+      case 1: PDX_ROWS(1) break;
+      case 2: PDX_ROWS(2) break;
+      case 4: PDX_ROWS(4) break;
+      case 8: PDX_ROWS(8) break;
+      case 16: PDX_ROWS(16) break;
+      default: PDX_ROWS(32) break;
+    }
+#undef PDX_ROWS
   } else {
     hipLaunchKernelGGL(k_rows_allpairs, dim3(nb), dim3(TPB), 0, st, N, nl->cap, nl->packed.p, geom,
                        nl->rl2, nl->neighbors_flat.p, nl->counts.p,
@@ -526,6 +583,17 @@ extern "C" int pdx_nl_create(int32_t n, const double box[3], double cutoff, doub
   }
   nl->nx = g[0]; nl->ny = g[1]; nl->nz = g[2];
   nl->ncell = g[0] * g[1] * g[2];
+  // lanes per particle of the row build: fill ~2^19 resident threads (256 CUs x 2048), at
+  // most 32; PDX_ROW_LANES overrides (tuning)
+  {
+    int G = 1;
+    while ((int64_t)n * G < 524288 && G < 32) G <<= 1;
+    if (const char *v = std::getenv("PDX_ROW_LANES")) {
+      const int e = std::atoi(v);
+      if (e == 1 || e == 2 || e == 4 || e == 8 || e == 16 || e == 32) G = e;
+    }
+    nl->row_lanes = G;
+  }
   const int nb = blocks_for(n, TPB);
   const int sblocks = blocks_for(nl->ncell, SCAN_BLOCK);
   if ((rc = nl->state.alloc(PDX_DN_NSTATE)) || (rc = nl->geom.alloc(3)) ||

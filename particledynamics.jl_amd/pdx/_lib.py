@@ -74,6 +74,10 @@ class pdx_mesh_spec(C.Structure):
 
 
 def _load():
+    # torch first: it bundles the HIP runtime (same soname as ROCm's). Loading libpdx first
+    # would map a second runtime into the process; with torch loaded, libpdx binds to torch's,
+    # so torch streams and allocations are valid handles for the library.
+    import torch  # noqa: F401
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"libpdx is not built: {LIB_PATH} is missing (run __graft_entry__.build())")
     lib = C.CDLL(LIB_PATH)
@@ -103,6 +107,7 @@ def _load():
         "pdx_mesh_set_charges": ([vp, P(C.c_float), i32], C.c_int),
         "pdx_mesh_set_box": ([vp, P(dbl)], C.c_int),
         "pdx_mesh_spread": ([vp, vp, vp, vp, vp, vp, vp], C.c_int),
+        "pdx_mesh_spread_cells": ([vp, vp, vp], C.c_int),
         "pdx_mesh_solve": ([vp, i32, vp], C.c_int),
         "pdx_mesh_gather": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp], C.c_int),
         "pdx_mesh_totals": ([vp, P(dbl), vp], C.c_int),

@@ -20,7 +20,7 @@ def default_capacity(N, box, rlist):
 
 class ForceEngine:
     def __init__(self, N, box, pair_model=None, electrostatics=None, charges=None, skin=0.4, cap=None,
-                 interval=20, lanes=0, diagnostics=False):
+                 interval=20, lanes=0, diagnostics=False, spread="cells"):
         import torch
         self.N = int(N)
         self.box = tuple(float(b) for b in box)
@@ -37,16 +37,55 @@ class ForceEngine:
         self.pairs = PairForce(pair_model, coulomb=self.solver.real_space() if self.solver else None, lanes=lanes)
         self.pair_totals = torch.zeros(7, dtype=torch.float64, device="cuda")
         self.Vglobal = torch.zeros(6, dtype=torch.float32, device="cuda")
+        # the mesh branch runs on its own stream (default priority), forked after the list
+        # check and joined before the gather adds the mesh forces to the pair forces
+        self.mesh_stream = torch.cuda.Stream() if self.solver is not None else None
+        if spread not in ("cells", "atomic"):
+            raise L.ArgumentError("spread must be 'cells' or 'atomic'")
+        self.spread_mode = spread
+        self._graph = None
 
-    def evaluate(self, x, y, z, q, fx, fy, fz, Epot=None, V=None, energy=True, stream=None):
+    def evaluate(self, x, y, z, q, fx, fy, fz, Epot=None, V=None, energy=True, stream=None, overlap=True):
         """Enqueue one force evaluation (no host synchronisation)."""
-        self.nl.update(x, y, z, q, stream=stream)
+        import torch
+        main = stream if stream is not None else torch.cuda.current_stream()
+        self.nl.update(x, y, z, q, stream=main)
+        if self.solver is None:
+            self.pairs.compute(self.nl, fx, fy, fz, Epot, V, self.pair_totals if energy else None, energy=energy,
+                               add=False, stream=main)
+            return
+        order = self.nl._arrays.particle_ids_sorted
+        side = self.mesh_stream if overlap else main
+        if overlap:
+            side.wait_stream(main)
+        if self.spread_mode == "cells":
+            self.solver.spread_cells(self.nl, stream=side)
+        else:
+            self.solver.spread(x, y, z, q, order, stream=side)
+        self.solver.solve(energy, stream=side)
         self.pairs.compute(self.nl, fx, fy, fz, Epot, V, self.pair_totals if energy else None, energy=energy,
-                           add=False, stream=stream)
-        if self.solver is not None:
-            order = self.nl._arrays.particle_ids_sorted
-            self.solver.compute_mesh(x, y, z, q, fx, fy, fz, Epot, self.Vglobal if energy else None, energy,
-                                     order=order, set=False, stream=stream)
+                           add=False, stream=main)
+        if overlap:
+            main.wait_stream(side)
+        self.solver.mesh_gather(x, y, z, q, fx, fy, fz, Epot, self.Vglobal if energy else None, energy,
+                                order=order, set=False, stream=main)
+
+    def capture(self, x, y, z, q, fx, fy, fz, Epot=None, V=None, energy=True, overlap=True):
+        """Capture one evaluation into a hipGraph (torch.cuda.CUDAGraph) over these buffers;
+        ``replay()`` then re-runs it with one launch. Every enqueue is capture-safe: no
+        allocation, synchronisation or memset inside."""
+        import torch
+        self.evaluate(x, y, z, q, fx, fy, fz, Epot, V, energy, overlap=overlap)  # warm: workspaces sized
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.evaluate(x, y, z, q, fx, fy, fz, Epot, V, energy, overlap=overlap)
+        torch.cuda.synchronize()
+        self._graph = g
+        return g
+
+    def replay(self):
+        self._graph.replay()
 
     def totals(self, stream=None):
         """FP64 energy and virial of the last energy evaluation (boundary readback)."""

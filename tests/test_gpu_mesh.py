@@ -42,7 +42,7 @@ def test_fft_normalisation(cuda):
     assert np.abs(back / 240 - g).max() < 1e-5  # backward is unnormalised
 
 
-@pytest.mark.parametrize("mesh", [(20, 22, 24), (48, 48, 54), (120, 120, 120), (60, 64, 63)])
+@pytest.mark.parametrize("mesh", [(20, 22, 24), (48, 48, 54), (120, 120, 120), (60, 64, 63), (26, 33, 28)])
 def test_fft_matches_torch(cuda, mesh):
     import torch
     s = _solver(mesh, P=5, box=(30.0, 30.0, 30.0), rc=4.0)
@@ -73,7 +73,33 @@ def test_spread_conserves_and_is_deterministic(cuda):
     g2 = s.fixed_grid().clone()
     s.solve(energy=False)
     assert torch.equal(g1, g2)  # integer atomics: bitwise independent of arrival order
-    assert abs(g1.sum().item() - float(np.float32(q).astype(np.float64).sum())) < 1e-6
+    # charge conservation at the resolution of the FP32 weight products q w_x w_y w_z
+    qf = np.float32(q).astype(np.float64)
+    assert abs(g1.sum().item() - qf.sum()) < 1e-6 * np.abs(qf).sum()
+
+
+@pytest.mark.parametrize("P,mesh", [(5, (20, 22, 24)), (4, (33, 16, 40)), (6, (48, 48, 54))])
+def test_spread_cells_equals_atomic_spread(cuda, P, mesh):
+    """Owner-computes spread from the list's cells gives bitwise the atomic spread's grid,
+    also after the particles moved (within the skin) since the cells were built."""
+    import torch
+    N = 3000
+    R, q = systems.random_cloud(N, L_ES, seed=P)
+    s = _solver(mesh, P=P, N=N, q=q)
+    X, Y, Z, Q = dev(R[:, 0]), dev(R[:, 1]), dev(R[:, 2]), dev(q)
+    nl = pdx.DeviceNeighborWorkspace(N, L_ES, 4.0, 0.3, 512)
+    nl.update(X, Y, Z, Q)
+    nl.status()
+    for move in (0.0, 0.07):
+        Xm = X + move
+        Xm = Xm - L_ES[0] * torch.floor(Xm / L_ES[0] + 0.5)
+        nl.update(Xm, Y, Z, Q)
+        assert nl.status().builds == 1  # moved less than skin/2: same cells
+        s.spread(Xm, Y, Z, Q)
+        ga = s.fixed_grid().clone()
+        s.spread_cells(nl)
+        gb = s.fixed_grid().clone()
+        assert torch.equal(ga, gb), f"move {move}"
 
 
 def _es_eval(R, q, L, spec, skin_frac=0.075):

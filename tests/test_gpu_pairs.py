@@ -101,13 +101,25 @@ def test_lanes_agree(cuda):
     s = systems.fcc_lattice(4000, 0.8442, 0.05, seed=4)
     cap = pdx.default_capacity(4000, s["box"], 2.9)
     base = None
-    for G in (1, 4, 16, 64):
+    for G in (1, 4, 16, 64, -2, -4, -8, -16):  # wave lanes, and the split layout over waves
         F, E, V, tot = _eval(s["x"], s["y"], s["z"], s["box"], pdx.LennardJones(), cap=cap, lanes=G)
         if base is None:
             base = (F, tot)
         else:
             assert _relrms(F, base[0]) < 1e-6
             assert abs(tot[0] - base[1][0]) < 1e-6 * abs(base[1][0])
+
+
+def test_split_layout_coulomb_matches_lanes(cuda):
+    """The two traversal layouts give the same forces and FP64 totals for a charged system."""
+    s = systems.rock_salt(12, 2.82, 0.1, seed=5)
+    coul = pdx.RealSpaceCoulomb(C=14.399645, cutoff=8.0, alpha=0.39)
+    lj = pdx.LennardJones(0.01, 2.8, 8.0)
+    cap = pdx.default_capacity(s["N"], s["box"], 8.5)
+    F1, E1, V1, t1 = _eval(s["x"], s["y"], s["z"], s["box"], lj, cap=cap, lanes=2, q=s["q"], coulomb=coul, skin=0.5)
+    F2, E2, V2, t2 = _eval(s["x"], s["y"], s["z"], s["box"], lj, cap=cap, lanes=-8, q=s["q"], coulomb=coul, skin=0.5)
+    assert _relrms(F2, F1) < 1e-6
+    assert np.allclose(t1, t2, rtol=1e-6, atol=1e-6 * abs(t1[0]))
 
 
 def test_real_space_coulomb(cuda):
