@@ -248,6 +248,45 @@ int pdx_mesh_copy_grid(pdx_mesh_t m, int32_t which, void *dst, pdx_stream_t stre
 int pdx_mesh_fft(pdx_mesh_t m, int32_t dir, const void *in, void *out, pdx_stream_t stream);
 
 /* ======================================================================= */
+/* Learned correction of coarse-mesh PME forces (BF16 MFMA MLP)            */
+/* ======================================================================= */
+/* No counterpart in the reference: the correction and its training are new. */
+
+/* 80 features per particle from the charge grid of the mesh's last spread (call after
+ * pdx_mesh_spread / _spread_cells on the same step): q_i Q(n) at the 64 nodes
+ * n = floor(u) - 3 + {0..3}^3 an order-4 B-spline at the particle's grid coordinate u covers
+ * (x fastest), q_i (u - floor(u)) per dimension (3), q_i, q_i^2 and 11 zeros. feats: device
+ * N x 80 FP32, row-major. */
+int pdx_mesh_features(pdx_mesh_t m, const float *x, const float *y, const float *z, const float *q,
+                      float *feats, pdx_stream_t stream);
+
+typedef struct pdx_mlp_s *pdx_mlp_t;
+/* Number of parameters of the 80-64-64-3 SiLU network, packed as W1 (80x64, row = input), b1,
+ * W2 (64x64), b2, W3 (64x3), b3. */
+int pdx_mlp_nparams(void);
+/* Network with seeded He initialisation (FP32 master weights, BF16 copies for the MFMA tiles). */
+int pdx_mlp_create(int32_t seed, pdx_mlp_t *out);
+int pdx_mlp_destroy(pdx_mlp_t m);
+/* Host parameter vectors (packed as above); setting resets the optimiser state. */
+int pdx_mlp_set_params(pdx_mlp_t m, const float *params, int32_t n);
+int pdx_mlp_get_params(pdx_mlp_t m, float *params, int32_t n);
+/* Feature standardisation x' = (x - mu) isd (host arrays of 80) and output scale: the network
+ * predicts residual / out_scale. */
+int pdx_mlp_set_norm(pdx_mlp_t m, const float *mu, const float *isd, float out_scale);
+/* Inference on N feature rows (device): out (N x 3, may be NULL) = out_scale MLP(x'); when fx,
+ * fy, fz are given the correction is added to them. */
+int pdx_mlp_forward(pdx_mlp_t m, const float *feats, int32_t n, float *out, float *fx, float *fy, float *fz,
+                    pdx_stream_t stream);
+/* Gradient of loss = mean (MLP(x') - t / out_scale)^2 over the batch rows[0..n) (device ids;
+ * NULL: rows 0..n-1) of feats / targets (N x 3): grad (device, nparams) and loss (device
+ * double), both may be NULL (internal buffers). Per-wave partials are reduced in a fixed order. */
+int pdx_mlp_gradient(pdx_mlp_t m, const float *feats, const float *targets, const int32_t *rows, int32_t n,
+                     float *grad, double *loss, pdx_stream_t stream);
+/* One Adam step (beta 0.9 / 0.999) on that gradient. */
+int pdx_mlp_train_step(pdx_mlp_t m, const float *feats, const float *targets, const int32_t *rows, int32_t n,
+                       float lr, double *loss, pdx_stream_t stream);
+
+/* ======================================================================= */
 /* FP64 Ewald summation on the device (the converged reference force)      */
 /* ======================================================================= */
 

@@ -714,6 +714,44 @@ __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__
   }
 }
 
+// Learned-correction inputs from the charge grid of the last spread (80 per particle):
+// q_i Q(n) at the 64 nodes n = floor(u) - 3 + {0..3}^3 that an order-4 B-spline at u covers
+// (x fastest), q_i (u - floor(u)) per dimension, q_i, q_i^2, then zeros up to the MFMA width.
+constexpr int NFEAT = 80;
+__global__ __launch_bounds__(TPB) void k_features(int N, const float *__restrict__ x, const float *__restrict__ y,
+                                                  const float *__restrict__ z, const float *__restrict__ q,
+                                                  const unsigned long long *__restrict__ grid, int K1, int K2,
+                                                  int K3, float Lx, float Ly, float Lz, double inv_scale,
+                                                  float *feats) {
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i >= N) return;
+  const float qi = q[i];
+  const float u1 = grid_coordinate(x[i], Lx, K1), u2 = grid_coordinate(y[i], Ly, K2),
+              u3 = grid_coordinate(z[i], Lz, K3);
+  const float f1 = floorf(u1), f2 = floorf(u2), f3 = floorf(u3);
+  const int b1 = (int)f1 - 3, b2 = (int)f2 - 3, b3 = (int)f3 - 3;
+  float *f = feats + (size_t)i * NFEAT;
+  const size_t KK = (size_t)K1 * K2;
+  int k = 0;
+  for (int d3 = 0; d3 < 4; ++d3) {
+    const int n3 = wrap_node(b3 + d3, K3);
+    for (int d2 = 0; d2 < 4; ++d2) {
+      const int n2 = wrap_node(b2 + d2, K2);
+      for (int d1 = 0; d1 < 4; ++d1) {
+        const int n1 = wrap_node(b1 + d1, K1);
+        const double v = (double)(long long)grid[(size_t)n3 * KK + (size_t)n2 * K1 + n1] * inv_scale;
+        f[k++] = qi * (float)v;
+      }
+    }
+  }
+  f[64] = qi * (u1 - f1);
+  f[65] = qi * (u2 - f2);
+  f[66] = qi * (u3 - f3);
+  f[67] = qi;
+  f[68] = qi * qi;
+  for (int c = 69; c < NFEAT; ++c) f[c] = 0.f;
+}
+
 __global__ void k_fixed_to_double(const unsigned long long *g, double *out, size_t n, double inv) {
   const size_t k = (size_t)blockIdx.x * TPB + threadIdx.x;
   if (k < n) out[k] = (double)(long long)g[k] * inv;
@@ -1196,6 +1234,16 @@ extern "C" int pdx_mesh_totals(pdx_mesh_t m, double out[9], pdx_stream_t stream)
   for (int c = 0; c < 6; ++c) out[1 + c] = acc[1 + c] + (c < 3 ? m->ubg : 0.0);
   out[7] = -0.5 * m->C * m->selfc * m->qsq;  // self_energy (Solver.jl:421-423)
   out[8] = m->ubg;                           // background_energy (Solver.jl:427-429)
+  return PDX_OK;
+}
+
+extern "C" int pdx_mesh_features(pdx_mesh_t m, const float *x, const float *y, const float *z, const float *q,
+                                 float *feats, pdx_stream_t stream) {
+  if (!m || !x || !y || !z || !q || !feats) return fail(PDX_ERR_ARGUMENT, "null argument");
+  const float L[3] = {(float)m->box[0], (float)m->box[1], (float)m->box[2]};
+  hipLaunchKernelGGL(k_features, dim3((m->N + TPB - 1) / TPB), dim3(TPB), 0, as_stream(stream), m->N, x, y, z, q,
+                     m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], m->fixed_inv, feats);
+  PDX_LAUNCH_CHECK();
   return PDX_OK;
 }
 

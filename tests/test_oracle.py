@@ -207,3 +207,33 @@ def test_ewald_sampled_equals_full():
     ids = [0, 5, 39]
     Fs = oracle.ewald_sampled(R[:, 0], R[:, 1], R[:, 2], q, Ls, C_=1.4, alpha=0.7, digits=14, ids=ids)
     assert np.abs(Fs - F[ids]).max() < 1e-11 * np.abs(F).max()
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """The FP64 MLP checker's backward equals central differences of its loss."""
+    import mlp
+    rng = np.random.default_rng(3)
+    p = rng.standard_normal(mlp.NPARAM) * 0.3
+    x = rng.standard_normal((20, mlp.NIN))
+    t = rng.standard_normal((20, 3))
+    mu, isd = rng.standard_normal(mlp.NIN) * 0.1, 1.0 + rng.random(mlp.NIN)
+    loss, g = mlp.loss_and_grad(p, x, t, mu, isd, 2.0, emulate=False)
+    h = 1e-6
+    ob1 = mlp.NIN * mlp.NH  # first bias of each layer
+    for k in list(range(0, mlp.NPARAM, 397)) + [mlp.NPARAM - 1, ob1, ob1 + mlp.NH + mlp.NH * mlp.NH]:
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (mlp.loss_and_grad(pp, x, t, mu, isd, 2.0, emulate=False)[0] -
+              mlp.loss_and_grad(pm, x, t, mu, isd, 2.0, emulate=False)[0]) / (2 * h)
+        assert abs(fd - g[k]) < 1e-7 * max(1.0, abs(g[k])), k
+
+
+def test_mlp_oracle_bf16_rounding():
+    import mlp
+    # BF16 keeps 7 fraction bits: 1 + 2^-8 is a tie between 1 and 1 + 2^-7 (to the even 1);
+    # 1 + 3 2^-8 ties between 1 + 2^-7 and 1 + 2^-6 (to the even 1 + 2^-6)
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 2 ** -9, -3.14159, 0.0, 1.0 + 3 * 2 ** -8])
+    r = mlp.bf16(v)
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 and r[5] == 1.0 + 2 ** -6
+    assert abs(r[3] + 3.140625) < 1e-12
