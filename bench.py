@@ -34,10 +34,30 @@ def parse():
     p.add_argument("--lanes", type=int, default=0)
     p.add_argument("--skin", type=float, default=1.0)
     p.add_argument("--spread", choices=("cells", "atomic"), default="cells")
+    p.add_argument("--no-learned", action="store_true", help="skip the learned-correction comparison")
+    p.add_argument("--no-replicas", action="store_true", help="skip the batched-replica run")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="run only a plain 256 MiB device copy (five launches): the reference reading for the "
+                        "FETCH_SIZE / WRITE_SIZE counters")
     p.add_argument("--mode", choices=("graph", "eager", "serial"), default="graph",
                    help="graph: hipGraph replay of the overlapped evaluation; eager: overlapped "
                         "streams without a graph; serial: one stream")
     return p.parse_args()
+
+
+def pair_traffic():
+    """HBM bytes per pair-kernel launch on this workload, as recorded by the counter summary
+    in profiles/ (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the reading of a
+    plain copy of known size); None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_pair_rocksalt64k_hbm_traffic.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return int(line.split()[-1])
+    except OSError:
+        pass
+    return None
 
 
 def pair_flops_per_visit(npe, npf, energy=True):
@@ -64,6 +84,16 @@ def main():
     from pdx import systems
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        n = 64 << 20  # 64 Mi floats = 256 MiB read + 256 MiB written per launch
+        src = torch.rand(n, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read_per_launch": 4 * n, "copy_bytes_written_per_launch": 4 * n,
+                          "launches": 5}))
+        return
     sysd = systems.rock_salt(40, 2.82, 0.1, seed=0)
     N, box = sysd["N"], sysd["box"]
     C = 14.399645
@@ -126,6 +156,16 @@ def main():
             ev[k].append(a.elapsed_time(b))
     phase_ms = {k: float(np.mean(v)) for k, v in ev.items()}
 
+    # ---- the pair kernel alone: `reps` back-to-back launches bracketed by HIP events on the
+    # stream they are launched on (the last `reps` pair dispatches of the run)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        pf.compute(nl, fx, fy, fz, E, V, eng.pair_totals, energy=True)
+    b.record(stream)
+    torch.cuda.synchronize()
+    pair_ms = a.elapsed_time(b) / reps
+
     # forced rebuild cost (decide + full rebuild)
     rb = []
     for _ in range(10):
@@ -146,11 +186,15 @@ def main():
     info = pf.info()
     fpv = pair_flops_per_visit(info["n_energy_terms"], info["n_force_terms"], True)
     pair_flop = visits * fpv
-    achieved = pair_flop / (phase_ms["pairs"] * 1e-3) / 1e12
+    achieved = pair_flop / (pair_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "k_pair<LJ,coulomb,energy>", "note": "FP32 VALU peak; flops counted per row visit",
-                "flop_per_launch": pair_flop, "row_visits": visits, "flop_per_visit": fpv}
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": pair_traffic(),
+                "kernel": "k_pair<LJ,coulomb,energy>", "duration_us": round(1e3 * pair_ms, 2), "launches_timed": reps,
+                "note": "FP32 VALU peak; flops counted per row visit; duration = mean over back-to-back "
+                        "launches of the kernel alone (HIP events on its stream); traffic = HBM bytes per launch "
+                        "from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a 256 MiB copy",
+                "flop_per_launch": pair_flop, "row_visits": visits, "flop_per_visit": fpv,
+                "algorithmic_bytes_per_launch": visits * 4 + N * 60}
 
     tot = eng.totals()
     out = {"metric": "force evals/s + ns/day at matched force err: learned-corrected vs plain PME",
@@ -179,9 +223,140 @@ def main():
         out["force_rel_rms_vs_ewald"] = err
         out["force_error_sample"] = "Coulomb forces of 256 ions vs FP64 Ewald (alpha 0.25, 12 digits)"
 
+    if not args.no_learned:
+        out["learned"] = learned_leg(args.steps, args.warmup, args.skin)
+    if not args.no_replicas:
+        out["replicas"] = replica_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sysd, C)
     print(json.dumps(out))
+
+
+def coulomb_error(X, Y, Z, Q, box, spec, C, skin, correction=None, nsample=256):
+    """Relative RMS Coulomb-force error of one mesh setting (optionally with the learned
+    correction) against FP64 Ewald (alpha 0.25, 12 digits) on sampled ions."""
+    import numpy as np
+    import torch
+
+    import pdx
+    N = X.numel()
+    eng = pdx.ForceEngine(N, box, None, electrostatics=spec, charges=Q.cpu().numpy(), skin=skin,
+                          correction=correction)
+    g = [torch.empty(N, device="cuda") for _ in range(3)]
+    eng.evaluate(X, Y, Z, Q, *g, None, None, energy=False)
+    ids = torch.from_numpy(np.sort(np.random.default_rng(1).choice(N, nsample, replace=False)).astype(np.int32)).cuda()
+    Fref, _ = pdx.ewald_fp64(*(t.double() for t in (X, Y, Z, Q)), box, C, 0.25, digits=12.0, sample=ids,
+                             want_energy=False)
+    Fg = torch.stack(g, 1).double()[ids.long()]
+    return math.sqrt(((Fg - Fref) ** 2).sum().item() / (Fref ** 2).sum().item())
+
+
+def learned_leg(steps, warmup, skin):
+    """Coarse-mesh PME plus the learned correction against plain PME on the ionic workload
+    with 5% of the sites vacated in neutral pairs (60 800 ions): the network (80-64-64-3,
+    BF16 MFMA) is trained on the device on residuals the method generates (coarse 2.0 A / P=4
+    against fine 0.5 A / P=6 PME, three training configurations), then each variant's full
+    force evaluation (LJ + real space + mesh [+ correction], energy and virial) is timed as a
+    graph replay and its Coulomb-force error measured against FP64 Ewald on a held-out
+    configuration."""
+    import numpy as np
+    import torch
+
+    import pdx
+    from pdx import learned, systems
+    C = 14.399645
+    mk = lambda h, P: pdx.PME(coulomb_prefactor=C, real_cutoff=9.0, real_space_tolerance=1e-5, mesh_spacing=h,
+                              order=P)
+    plain, coarse, fine = mk(1.0, 5), mk(2.0, 4), mk(0.5, 6)
+    lj = pdx.LennardJones(epsilon=0.01, sigma=2.8, cutoff=9.0)
+    train_sys = [systems.rock_salt(40, 2.82, 0.1, seed=s, vacancy_fraction=0.05) for s in (11, 12, 13)]
+    t0 = time.perf_counter()
+    F, T = learned.residual_dataset(train_sys, coarse, fine, skin)
+    mlp = learned.MLP(seed=0)
+    mu, isd, sc = learned.fit_standardisation(F, T)
+    mlp.set_norm(mu, isd, sc)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    nsteps, batch = 4000, 8192
+    hist = learned.train(mlp, F, T, steps=nsteps, batch=batch, lr=2e-3)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+
+    sysd = systems.rock_salt(40, 2.82, 0.1, seed=10, vacancy_fraction=0.05)
+    N, box = sysd["N"], sysd["box"]
+    X, Y, Z, Q = (torch.from_numpy(sysd[k]).cuda() for k in ("x", "y", "z", "q"))
+    res = {"N": N, "train": {"configs": len(train_sys), "samples": int(F.shape[0]), "steps": nsteps,
+                             "batch": batch, "data_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2),
+                             "final_loss": hist[-1] if hist else None}}
+    for name, spec, corr in (("plain", plain, None), ("coarse", coarse, None), ("corrected", coarse, mlp)):
+        eng = pdx.ForceEngine(N, box, lj, electrostatics=spec, charges=sysd["q"], skin=skin, correction=corr)
+        fx, fy, fz, E = (torch.empty(N, device="cuda") for _ in range(4))
+        V = torch.empty(6, N, device="cuda")
+        eng.capture(X, Y, Z, Q, fx, fy, fz, E, V, energy=True)
+        for _ in range(max(warmup, 1)):
+            eng.replay()
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(steps):
+            eng.replay()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - a) / steps
+        err = coulomb_error(X, Y, Z, Q, box, spec, C, skin, correction=corr)
+        res[name] = {"mesh": list(eng.solver.mesh), "order": eng.solver.order, "ms_per_eval": round(1e3 * dt, 4),
+                     "evals_per_s": round(1.0 / dt, 2), "force_rel_rms_vs_ewald": err}
+        del eng
+    res["speedup_corrected_vs_plain"] = round(res["plain"]["ms_per_eval"] / res["corrected"]["ms_per_eval"], 3)
+    res["error_ratio_corrected_vs_plain"] = round(res["corrected"]["force_rel_rms_vs_ewald"] /
+                                                  res["plain"]["force_rel_rms_vs_ewald"], 3)
+    res["matched"] = res["corrected"]["force_rel_rms_vs_ewald"] <= res["plain"]["force_rel_rms_vs_ewald"]
+    return res
+
+
+def replica_leg(nrep=512, nsteps=1000, seed=0):
+    """512 independent LJ / WCA boxes (n log-uniform in [256, 8192], density uniform in
+    [0.70, 0.90], model drawn per box), each advanced 1000 velocity-Verlet steps (dt 0.005)
+    in one launch captured as a hipGraph, after the launch shape is autotuned on 10 steps."""
+    import numpy as np
+    import torch
+
+    import pdx
+    from pdx import replicas, systems
+    b = systems.replica_batch(nrep, seed=seed)
+    rb = pdx.ReplicaBatch(b["specs"])
+    st = replicas.device_state(b)
+    rb.forces(st, energy=False)
+    best, times = replicas.autotune(rb, st, steps=10)
+    work = replicas.clone_state(st)
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        rb.run(work, 1)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    work = replicas.clone_state(st)
+    out = torch.zeros(nrep, 4, dtype=torch.float64, device="cuda")
+    e0 = rb.forces(replicas.clone_state(st)).cpu().numpy()
+    with torch.cuda.graph(g):
+        rb.run(work, nsteps, dt=0.005, energy=True, out=out)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    z = torch.cuda.Event(enable_timing=True)
+    a.record()
+    g.replay()
+    z.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(z)
+    e1 = out.cpu().numpy()
+    ns = np.array([sp["n"] for sp in b["specs"]])
+    drift = np.abs((e1[:, 0] + e1[:, 1]) - (e0[:, 0] + e0[:, 1])) / ns
+    return {"replicas": nrep, "atoms": int(ns.sum()), "steps": nsteps, "ms": round(ms, 3),
+            "atom_steps_per_s": round(float(ns.sum()) * nsteps / (ms * 1e-3), 1),
+            "replica_steps_per_s": round(nrep * nsteps / (ms * 1e-3), 1),
+            "models": {"lj": int(sum(sp["model"] == "lj" for sp in b["specs"])),
+                       "wca": int(sum(sp["model"] == "wca" for sp in b["specs"]))},
+            "launch": {"block": best[0], "cells_per_cutoff": best[1], "autotune_ms_10_steps": times},
+            "energy_drift_per_atom_max": float(drift.max()), "energy_drift_per_atom_median": float(np.median(drift))}
 
 
 def cpu_baseline(sysd, C):

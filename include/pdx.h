@@ -287,6 +287,51 @@ int pdx_mlp_train_step(pdx_mlp_t m, const float *feats, const float *targets, co
                        float lr, double *loss, pdx_stream_t stream);
 
 /* ======================================================================= */
+/* Batched replica boxes: velocity Verlet, one persistent launch           */
+/* ======================================================================= */
+/* No single counterpart in the reference: each replica is what one Run with UniformLJ /
+ * UniformWCA and the NVE kick-drift / kick kernels advances (src/engine/Run.jl:247-294,
+ * src/simulation/NVEFusion.jl:3-24, src/SimulationCore.jl:581-591); the batch runs them all
+ * in one launch, one workgroup per replica for every step. */
+
+#define PDX_VV_FORCES_VALID 4  /* fx, fy, fz already hold the forces of the current x */
+
+typedef struct {
+  int32_t model;    /* PDX_PAIR_LJ or PDX_PAIR_WCA */
+  int32_t shift;    /* LJ: energy shifted to zero at the cutoff */
+  int32_t n;        /* atoms, 1..8192 */
+  int32_t reserved;
+  double epsilon, sigma;
+  double cutoff;    /* LJ cutoff; WCA uses 2^(1/6) sigma (rc^2 = 1.2599211 sigma^2) */
+  double box[3];    /* orthorhombic box, each side >= 2 rc */
+} pdx_replica_spec;
+
+typedef struct pdx_rep_s *pdx_rep_t;
+/* Replica set: atoms of replica k occupy [offset_k, offset_k + n_k) of the concatenated SoA
+ * arrays (offsets = prefix sums of n). block: workgroup size 256 / 512 / 1024 (0: 512);
+ * stencil: cells per cutoff, 1 (27 cells of side >= rc) or 2 (125 cells of side >= rc/2)
+ * (0: 1); boxes too small for the stencil use all pairs. */
+int pdx_replicas_create(int32_t nrep, const pdx_replica_spec *specs, int32_t block, int32_t stencil,
+                        pdx_rep_t *out);
+int pdx_replicas_destroy(pdx_rep_t r);
+/* Launch shape for later runs (the autotuner's knob). */
+int pdx_replicas_set_launch(pdx_rep_t r, int32_t block, int32_t stencil);
+/* out: total atoms, replicas, largest replica, LDS bytes per workgroup, block, stencil,
+ * replicas on all pairs, largest cell count. */
+int pdx_replicas_info(pdx_rep_t r, int64_t out[8]);
+/* nrep + 1 offsets (host). */
+int pdx_replicas_offsets(pdx_rep_t r, int64_t *offsets);
+/* Advance every replica nsteps velocity-Verlet steps (v += c f; x += dt v, wrapped to
+ * [-L/2, L/2); forces; v += c f; c = dt / (2 mass)). Without PDX_VV_FORCES_VALID the forces
+ * of the current x are computed first (nsteps = 0: forces only). With PDX_ENERGY, out
+ * (device, 4 doubles per replica) receives the final potential energy, kinetic energy,
+ * virial trace sum r.F over pairs, and n. Positions, velocities and forces are updated in
+ * place (device, FP32). Within a replica the result depends only on that replica's data. */
+int pdx_replicas_run(pdx_rep_t r, float *x, float *y, float *z, float *vx, float *vy, float *vz, float *fx,
+                     float *fy, float *fz, double mass, double dt, int32_t nsteps, int32_t flags, double *out,
+                     pdx_stream_t stream);
+
+/* ======================================================================= */
 /* FP64 Ewald summation on the device (the converged reference force)      */
 /* ======================================================================= */
 

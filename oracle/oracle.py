@@ -144,3 +144,37 @@ def cell_lj(x, y, z, box, sigma=1.0, epsilon=1.0, cutoff=2.5):
     pairs = _lib.or_cell_lj(n, xa[1], ya[1], za[1], bx[1], sigma, epsilon, cutoff, fx.ctypes.data_as(P),
                             fy.ctypes.data_as(P), fz.ctypes.data_as(P), C.byref(e))
     return np.stack([fx, fy, fz], 1), e.value, pairs
+
+
+def vv_steps(x, y, z, vx, vy, vz, box, kind="lj", sigma=1.0, epsilon=1.0, cutoff=2.5, dt=0.005, mass=1.0,
+             nsteps=1, shift=False):
+    """FP64 velocity Verlet of one box with the direct pair forces: per step v += c F,
+    x += dt v (wrapped to [-L/2, L/2)), F(x), v += c F, c = dt / (2 mass)
+    (_nve_kick_drift3_kernel!, src/simulation/NVEFusion.jl:3-24; _nve_kick3_kernel!,
+    src/SimulationCore.jl:581-591). Returns x, v (n, 3), F (n, 3), potential energy (LJ
+    shifted to zero at the cutoff when ``shift``), kinetic energy, sum r.F over pairs.
+    Pinned in tests/test_oracle.py by a dimer against a converged ODE solution, time
+    reversibility and energy conservation."""
+    L = np.asarray(box, dtype=np.float64)
+    X = np.stack([np.asarray(a, dtype=np.float64) for a in (x, y, z)], 1)
+    V = np.stack([np.asarray(a, dtype=np.float64) for a in (vx, vy, vz)], 1)
+    c = 0.5 * dt / mass
+
+    def forces(X):
+        d = pair_direct(X[:, 0], X[:, 1], X[:, 2], L, kind=kind, sigma=sigma, epsilon=epsilon, cutoff=cutoff)
+        return d
+
+    d = forces(X)
+    for _ in range(nsteps):
+        V = V + c * d["F"]
+        X = X + dt * V
+        X = X - L * (X >= 0.5 * L) + L * (X < -0.5 * L)
+        d = forces(X)
+        V = V + c * d["F"]
+    pe = d["energy"].sum()
+    if kind == "lj" and shift:
+        s6 = (sigma / cutoff) ** 6
+        pe -= 0.5 * 4 * epsilon * (s6 * s6 - s6) * d["counts"].sum()
+    ke = 0.5 * mass * (V ** 2).sum()
+    w = d["virial"][:, :3].sum()
+    return X, V, d["F"], pe, ke, w

@@ -57,32 +57,55 @@ struct MLPDev {
 };
 
 // Weights into LDS (BF16) for the block.
-struct WLds {
+struct __align__(16) WLds {
   short w1[NIN * NH], w2[NH * NH], w3[NH * NOP];
   float b1[NH], b2[NH], b3[NOP], mu[NIN], isd[NIN];
 };
 
+// 16-byte copies: the three weight blocks are multiples of 8 BF16 values and 16-byte aligned.
+__device__ __forceinline__ void copy16(short *dst, const short *src, int count) {
+  const uint4 *s = reinterpret_cast<const uint4 *>(src);
+  uint4 *d = reinterpret_cast<uint4 *>(dst);
+  for (int k = threadIdx.x; k < count / 8; k += blockDim.x) d[k] = s[k];
+}
+
 __device__ void load_weights(WLds &L, const MLPDev &m) {
-  for (int k = threadIdx.x; k < NIN * NH; k += blockDim.x) L.w1[k] = m.w1[k];
-  for (int k = threadIdx.x; k < NH * NH; k += blockDim.x) L.w2[k] = m.w2[k];
-  for (int k = threadIdx.x; k < NH * NOP; k += blockDim.x) L.w3[k] = m.w3[k];
+  copy16(L.w1, m.w1, NIN * NH);
+  copy16(L.w2, m.w2, NH * NH);
+  copy16(L.w3, m.w3, NH * NOP);
   for (int k = threadIdx.x; k < NH; k += blockDim.x) { L.b1[k] = m.b1[k]; L.b2[k] = m.b2[k]; }
   for (int k = threadIdx.x; k < NOP; k += blockDim.x) L.b3[k] = m.b3[k];
   for (int k = threadIdx.x; k < NIN; k += blockDim.x) { L.mu[k] = m.mu[k]; L.isd[k] = m.isd[k]; }
 }
 
 // Standardised input tile (16 particles x NIN) into the wave's LDS scratch as BF16 [i][k] (ld NIN).
+// Each lane loads its float4 pieces of the tile first (all loads in flight at once), then
+// standardises and converts them.
+constexpr int NV4 = NIN / 4;              // float4 per feature row
+constexpr int XPER = 16 * NV4 / WAVE;     // float4 per lane per tile
+static_assert((16 * NV4) % WAVE == 0, "tile loads must divide evenly over the wave");
 __device__ __forceinline__ void stage_inputs(short *xs, const float *__restrict__ feats, const int32_t *rows,
                                              int base, int n, const WLds &L, int lane) {
-  for (int e = lane; e < 16 * NIN; e += WAVE) {
-    const int i = e / NIN, k = e % NIN;
+  float4 v[XPER];
+#pragma unroll
+  for (int u = 0; u < XPER; ++u) {
+    const int e = lane + u * WAVE, i = e / NV4, k4 = e % NV4;
     const int p = base + i;
-    float v = 0.f;
+    v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (p < n) {
       const int r = rows ? rows[p] : p;
-      v = (feats[(size_t)r * NIN + k] - L.mu[k]) * L.isd[k];
+      v[u] = *reinterpret_cast<const float4 *>(feats + (size_t)r * NIN + 4 * k4);
     }
-    xs[i * NIN + k] = f2bf(v);
+  }
+#pragma unroll
+  for (int u = 0; u < XPER; ++u) {
+    const int e = lane + u * WAVE, i = e / NV4, k = 4 * (e % NV4);
+    const bool in = base + i < n;
+    short *d = xs + i * NIN + k;
+    d[0] = f2bf(in ? (v[u].x - L.mu[k + 0]) * L.isd[k + 0] : 0.f);
+    d[1] = f2bf(in ? (v[u].y - L.mu[k + 1]) * L.isd[k + 1] : 0.f);
+    d[2] = f2bf(in ? (v[u].z - L.mu[k + 2]) * L.isd[k + 2] : 0.f);
+    d[3] = f2bf(in ? (v[u].w - L.mu[k + 3]) * L.isd[k + 3] : 0.f);
   }
 }
 
@@ -511,8 +534,9 @@ extern "C" int pdx_mlp_forward(pdx_mlp_t m, const float *feats, int32_t n, float
   if (!m || !feats) return fail(PDX_ERR_ARGUMENT, "null argument");
   if (n <= 0) return PDX_OK;
   if ((fx || fy || fz) && !(fx && fy && fz)) return fail(PDX_ERR_ARGUMENT, "give all three force arrays or none");
+  // one tile per wave (the weights are staged into LDS once per block of four waves)
   const int ntile = (n + 15) / 16;
-  const int blocks = std::min((ntile + 3) / 4, 2048);
+  const int blocks = std::max(1, std::min((ntile + 3) / 4, 2048));
   hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(TPB), 0, as_stream(stream), dev_of(m), feats, n, out, fx, fy,
                      fz);
   PDX_LAUNCH_CHECK();

@@ -529,9 +529,9 @@ __global__ void k_zero64(unsigned long long *g, size_t n) {
 // skin/2, the most any particle moved since it was binned), accumulates q w_x w_y w_z in
 // 64-bit fixed point with LDS integer atomics (exact, order-independent) and stores the
 // block with plain writes: no global atomics, and no clearing (every node is written).
-constexpr int STILE = 16;
+// Tile edge: 16 nodes, or 8 on meshes too small to give 256 tiles of 16 (one per CU).
 constexpr int SLIST = 1536;  // particles reaching one block staged in LDS (a few hundred in practice)
-template <int P>
+template <int P, int STILE>
 __global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__ packed,
                                                       const int32_t *__restrict__ offsets,
                                                       const int32_t *__restrict__ ids,
@@ -718,38 +718,35 @@ __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__
 // q_i Q(n) at the 64 nodes n = floor(u) - 3 + {0..3}^3 that an order-4 B-spline at u covers
 // (x fastest), q_i (u - floor(u)) per dimension, q_i, q_i^2, then zeros up to the MFMA width.
 constexpr int NFEAT = 80;
+// One wave per particle: lane l reads support node (l & 3, (l >> 2) & 3, l >> 4), so a row of
+// 80 features is written by contiguous lanes.
 __global__ __launch_bounds__(TPB) void k_features(int N, const float *__restrict__ x, const float *__restrict__ y,
                                                   const float *__restrict__ z, const float *__restrict__ q,
                                                   const unsigned long long *__restrict__ grid, int K1, int K2,
                                                   int K3, float Lx, float Ly, float Lz, double inv_scale,
                                                   float *feats) {
-  const int i = blockIdx.x * TPB + threadIdx.x;
+  const int i = blockIdx.x * (TPB / WAVE) + threadIdx.x / WAVE;
+  const int lane = threadIdx.x % WAVE;
   if (i >= N) return;
   const float qi = q[i];
   const float u1 = grid_coordinate(x[i], Lx, K1), u2 = grid_coordinate(y[i], Ly, K2),
               u3 = grid_coordinate(z[i], Lz, K3);
   const float f1 = floorf(u1), f2 = floorf(u2), f3 = floorf(u3);
-  const int b1 = (int)f1 - 3, b2 = (int)f2 - 3, b3 = (int)f3 - 3;
+  const int n1 = wrap_node((int)f1 - 3 + (lane & 3), K1);
+  const int n2 = wrap_node((int)f2 - 3 + ((lane >> 2) & 3), K2);
+  const int n3 = wrap_node((int)f3 - 3 + (lane >> 4), K3);
   float *f = feats + (size_t)i * NFEAT;
-  const size_t KK = (size_t)K1 * K2;
-  int k = 0;
-  for (int d3 = 0; d3 < 4; ++d3) {
-    const int n3 = wrap_node(b3 + d3, K3);
-    for (int d2 = 0; d2 < 4; ++d2) {
-      const int n2 = wrap_node(b2 + d2, K2);
-      for (int d1 = 0; d1 < 4; ++d1) {
-        const int n1 = wrap_node(b1 + d1, K1);
-        const double v = (double)(long long)grid[(size_t)n3 * KK + (size_t)n2 * K1 + n1] * inv_scale;
-        f[k++] = qi * (float)v;
-      }
-    }
+  const double v = (double)(long long)grid[(size_t)n3 * K1 * K2 + (size_t)n2 * K1 + n1] * inv_scale;
+  f[lane] = qi * (float)v;
+  if (lane < NFEAT - 64) {
+    float e = 0.f;
+    e = lane == 0 ? qi * (u1 - f1) : e;
+    e = lane == 1 ? qi * (u2 - f2) : e;
+    e = lane == 2 ? qi * (u3 - f3) : e;
+    e = lane == 3 ? qi : e;
+    e = lane == 4 ? qi * qi : e;
+    f[64 + lane] = e;
   }
-  f[64] = qi * (u1 - f1);
-  f[65] = qi * (u2 - f2);
-  f[66] = qi * (u3 - f3);
-  f[67] = qi;
-  f[68] = qi * qi;
-  for (int c = 69; c < NFEAT; ++c) f[c] = 0.f;
 }
 
 __global__ void k_fixed_to_double(const unsigned long long *g, double *out, size_t n, double inv) {
@@ -1167,14 +1164,21 @@ extern "C" int pdx_mesh_spread_cells(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t str
       return fail(PDX_ERR_ARGUMENT, "neighbor list and mesh boxes differ");
   const float L[3] = {(float)m->box[0], (float)m->box[1], (float)m->box[2]};
   const float scale = (float)m->fixed_scale;
-  const int tiles = ((m->K1 + STILE - 1) / STILE) * ((m->K2 + STILE - 1) / STILE) * ((m->K3 + STILE - 1) / STILE);
+  auto ntiles = [&](int T) { return ((m->K1 + T - 1) / T) * ((m->K2 + T - 1) / T) * ((m->K3 + T - 1) / T); };
+  const bool small = ntiles(16) < 256;
+  const int tiles = ntiles(small ? 8 : 16);
   const float margin = 0.5f * (float)nl->skin;
   hipStream_t st = as_stream(stream);
 #define PDX_TSPREAD(PP)                                                                                        \
   case PP:                                                                                                     \
-    hipLaunchKernelGGL(k_spread_tiles<PP>, dim3(tiles), dim3(TPB), 0, st, nl->packed.p, nl->cell_offsets.p,     \
-                       nl->particle_ids_sorted.p, nl->state.p, nl->nx, nl->ny, nl->nz, margin, m->grid64.p, m->K1, \
-                       m->K2, m->K3, L[0], L[1], L[2], scale);                                                 \
+    if (small)                                                                                                 \
+      hipLaunchKernelGGL((k_spread_tiles<PP, 8>), dim3(tiles), dim3(TPB), 0, st, nl->packed.p,                  \
+                         nl->cell_offsets.p, nl->particle_ids_sorted.p, nl->state.p, nl->nx, nl->ny, nl->nz,     \
+                         margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale);                    \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_spread_tiles<PP, 16>), dim3(tiles), dim3(TPB), 0, st, nl->packed.p,                 \
+                         nl->cell_offsets.p, nl->particle_ids_sorted.p, nl->state.p, nl->nx, nl->ny, nl->nz,     \
+                         margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale);                    \
     break;
   switch (m->P) {
     PDX_TSPREAD(3)
@@ -1240,8 +1244,9 @@ extern "C" int pdx_mesh_totals(pdx_mesh_t m, double out[9], pdx_stream_t stream)
 extern "C" int pdx_mesh_features(pdx_mesh_t m, const float *x, const float *y, const float *z, const float *q,
                                  float *feats, pdx_stream_t stream) {
   if (!m || !x || !y || !z || !q || !feats) return fail(PDX_ERR_ARGUMENT, "null argument");
+  if (m->K1 < 4 || m->K2 < 4 || m->K3 < 4) return fail(PDX_ERR_ARGUMENT, "features need at least 4 mesh nodes a side");
   const float L[3] = {(float)m->box[0], (float)m->box[1], (float)m->box[2]};
-  hipLaunchKernelGGL(k_features, dim3((m->N + TPB - 1) / TPB), dim3(TPB), 0, as_stream(stream), m->N, x, y, z, q,
+  hipLaunchKernelGGL(k_features, dim3((m->N + TPB / WAVE - 1) / (TPB / WAVE)), dim3(TPB), 0, as_stream(stream), m->N, x, y, z, q,
                      m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], m->fixed_inv, feats);
   PDX_LAUNCH_CHECK();
   return PDX_OK;

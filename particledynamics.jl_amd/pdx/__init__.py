@@ -10,9 +10,10 @@ from .forces import LennardJones, WCA, SoftRepulsive, RealSpaceCoulomb, PairForc
 from .electrostatics import (ElectrostaticsSpec, PME, ElectrostaticsSolver, alpha_from_tolerance, fft_friendly,
                              ewald_fp64)
 from .engine import ForceEngine, default_capacity
+from .replicas import ReplicaBatch
 from . import systems
 
 __all__ = ["ArgumentError", "DeviceError", "DimensionMismatch", "NeighborCapacityError", "check", "last_error",
            "lib", "LIB_PATH", "DeviceNeighborWorkspace", "CellList", "build_neighbors_dense", "LennardJones", "WCA",
            "SoftRepulsive", "RealSpaceCoulomb", "PairForce", "ElectrostaticsSpec", "PME", "ElectrostaticsSolver",
-           "alpha_from_tolerance", "fft_friendly", "ewald_fp64", "ForceEngine", "default_capacity", "systems"]
+           "alpha_from_tolerance", "fft_friendly", "ewald_fp64", "ForceEngine", "default_capacity", "ReplicaBatch", "systems"]

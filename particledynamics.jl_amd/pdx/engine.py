@@ -20,7 +20,7 @@ def default_capacity(N, box, rlist):
 
 class ForceEngine:
     def __init__(self, N, box, pair_model=None, electrostatics=None, charges=None, skin=0.4, cap=None,
-                 interval=20, lanes=0, diagnostics=False, spread="cells"):
+                 interval=20, lanes=0, diagnostics=False, spread="cells", correction=None):
         import torch
         self.N = int(N)
         self.box = tuple(float(b) for b in box)
@@ -43,6 +43,15 @@ class ForceEngine:
         if spread not in ("cells", "atomic"):
             raise L.ArgumentError("spread must be 'cells' or 'atomic'")
         self.spread_mode = spread
+        # learned force correction of a coarse mesh (an MLP of the charge-grid features): the
+        # features are taken on the mesh stream after the spread, the network's output is
+        # added to the forces after the gather; energies and virials are not corrected
+        self.correction = correction
+        if correction is not None:
+            if self.solver is None:
+                raise L.ArgumentError("a learned correction needs a mesh")
+            from .learned import NFEAT
+            self.feats = torch.empty(self.N, NFEAT, dtype=torch.float32, device="cuda")
         self._graph = None
 
     def evaluate(self, x, y, z, q, fx, fy, fz, Epot=None, V=None, energy=True, stream=None, overlap=True):
@@ -62,6 +71,9 @@ class ForceEngine:
             self.solver.spread_cells(self.nl, stream=side)
         else:
             self.solver.spread(x, y, z, q, order, stream=side)
+        if self.correction is not None:
+            from .learned import mesh_features
+            mesh_features(self.solver, x, y, z, q, self.feats, stream=side)
         self.solver.solve(energy, stream=side)
         self.pairs.compute(self.nl, fx, fy, fz, Epot, V, self.pair_totals if energy else None, energy=energy,
                            add=False, stream=main)
@@ -69,6 +81,8 @@ class ForceEngine:
             main.wait_stream(side)
         self.solver.mesh_gather(x, y, z, q, fx, fy, fz, Epot, self.Vglobal if energy else None, energy,
                                 order=order, set=False, stream=main)
+        if self.correction is not None:
+            self.correction.forward(self.feats, None, fx, fy, fz, stream=main)
 
     def capture(self, x, y, z, q, fx, fy, fz, Epot=None, V=None, energy=True, overlap=True):
         """Capture one evaluation into a hipGraph (torch.cuda.CUDAGraph) over these buffers;

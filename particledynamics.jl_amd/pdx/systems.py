@@ -65,3 +65,48 @@ def maxwell_velocities(N, temperature, mass=1.0, seed=0):
     v = rng.normal(0.0, np.sqrt(temperature / mass), size=(N, 3))
     v -= v.mean(0)
     return v.astype(np.float32)
+
+
+def replica_batch(nrep=512, seed=0, n_range=(256, 8192), density_range=(0.70, 0.90), temperature=0.72,
+                  models=("wca", "lj"), jitter=0.05):
+    """Seeded batch of independent LJ / WCA boxes (the batched-replica config): per replica,
+    n log-uniform in ``n_range``, density uniform in ``density_range``, the model drawn
+    uniformly from ``models`` (LJ: eps = sigma = 1, rc = 2.5, energy shifted; WCA: eps =
+    sigma = 1), atoms on a seeded random subset of the sites of the smallest FCC lattice
+    that holds them, Gaussian jitter, Maxwell velocities at ``temperature`` with zero net
+    momentum. Returns specs (for ``ReplicaBatch``), concatenated FP32 x, y, z, vx, vy, vz
+    and the offsets."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(n_range[0]), np.log(n_range[1])
+    specs, xs, vs = [], [], []
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    for k in range(nrep):
+        n = int(min(n_range[1], max(n_range[0], round(np.exp(rng.uniform(lo, hi))))))
+        rho = float(rng.uniform(*density_range))
+        model = models[int(rng.integers(len(models)))]
+        Lb = (n / rho) ** (1.0 / 3.0)
+        m = 1
+        while 4 * m ** 3 < n:
+            m += 1
+        g = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1).reshape(-1, 3)
+        sites = ((g[:, None, :] + basis[None, :, :]).reshape(-1, 3) + 0.25) * (Lb / m) - Lb / 2
+        sites = sites[np.sort(rng.choice(len(sites), n, replace=False))]
+        pos = _wrap(sites + rng.normal(0.0, jitter, size=sites.shape), Lb)
+        v = rng.normal(0.0, np.sqrt(temperature), size=(n, 3))
+        v -= v.mean(0)
+        spec = {"model": model, "n": n, "box": (Lb, Lb, Lb), "epsilon": 1.0, "sigma": 1.0, "density": rho}
+        if model == "lj":
+            spec.update(cutoff=2.5, shift=True)
+        specs.append(spec)
+        xs.append(pos)
+        vs.append(v)
+    X = np.concatenate(xs).astype(np.float32)
+    V = np.concatenate(vs).astype(np.float32)
+    offsets = np.concatenate([[0], np.cumsum([s["n"] for s in specs])]).astype(np.int64)
+    # FP32 rounding may carry a coordinate onto +L/2: fold it back into [-L/2, L/2)
+    for k, s in enumerate(specs):
+        Lb = np.float32(s["box"][0])
+        c = X[offsets[k]:offsets[k + 1]]
+        c[c >= Lb / 2] -= Lb
+    return dict(specs=specs, x=X[:, 0].copy(), y=X[:, 1].copy(), z=X[:, 2].copy(), vx=V[:, 0].copy(),
+                vy=V[:, 1].copy(), vz=V[:, 2].copy(), offsets=offsets, N=int(offsets[-1]))

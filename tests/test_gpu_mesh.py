@@ -78,7 +78,7 @@ def test_spread_conserves_and_is_deterministic(cuda):
     assert abs(g1.sum().item() - qf.sum()) < 1e-6 * np.abs(qf).sum()
 
 
-@pytest.mark.parametrize("P,mesh", [(5, (20, 22, 24)), (4, (33, 16, 40)), (6, (48, 48, 54))])
+@pytest.mark.parametrize("P,mesh", [(5, (20, 22, 24)), (4, (33, 16, 40)), (6, (48, 48, 54)), (5, (112, 128, 128))])
 def test_spread_cells_equals_atomic_spread(cuda, P, mesh):
     """Owner-computes spread from the list's cells gives bitwise the atomic spread's grid,
     also after the particles moved (within the skin) since the cells were built."""

@@ -237,3 +237,47 @@ def test_mlp_oracle_bf16_rounding():
     r = mlp.bf16(v)
     assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 and r[5] == 1.0 + 2 ** -6
     assert abs(r[3] + 3.140625) < 1e-12
+
+
+def test_vv_oracle_dimer_matches_converged_ode():
+    """Velocity Verlet of an LJ dimer from rest against a converged ODE solution of the
+    relative coordinate (r'' = 2 F(r) / m): the error is O(dt^2)."""
+    from scipy.integrate import solve_ivp
+    r0, dt, nsteps, m = 1.3, 0.002, 500, 1.0
+    L = (10.0, 10.0, 10.0)
+    X, V, F, pe, ke, w = oracle.vv_steps([-r0 / 2, r0 / 2], [0.0, 0.0], [0.0, 0.0], [0.0, 0.0], [0.0, 0.0],
+                                         [0.0, 0.0], L, kind="lj", cutoff=2.5, dt=dt, mass=m, nsteps=nsteps)
+
+    def rhs(t, s):
+        r = s[0]
+        f = 24.0 * (2.0 * r ** -12 - r ** -6) / r  # force on the atom at +r/2 along +x
+        return [s[1], 2.0 * f / m]
+
+    sol = solve_ivp(rhs, (0.0, dt * nsteps), [r0, 0.0], rtol=1e-12, atol=1e-14)
+    r_vv = X[1, 0] - X[0, 0]
+    assert abs(r_vv - sol.y[0, -1]) < 2e-5
+    assert abs((V[1, 0] - V[0, 0]) - sol.y[1, -1]) < 2e-4
+    # the error is second order: halving dt shrinks it about four times
+    X2, *_ = oracle.vv_steps([-r0 / 2, r0 / 2], [0.0, 0.0], [0.0, 0.0], [0.0, 0.0], [0.0, 0.0], [0.0, 0.0], L,
+                             kind="lj", cutoff=2.5, dt=dt / 2, mass=m, nsteps=2 * nsteps)
+    e1 = abs(r_vv - sol.y[0, -1])
+    e2 = abs((X2[1, 0] - X2[0, 0]) - sol.y[0, -1])
+    assert 3.0 < e1 / e2 < 5.0
+
+
+def test_vv_oracle_reversible_and_conserving():
+    rng = np.random.default_rng(5)
+    n, L = 64, 5.0
+    g = np.stack(np.meshgrid(*[np.arange(4)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    X0 = (g + 0.5) * (L / 4) - L / 2 + rng.normal(0, 0.03, (n, 3))
+    V0 = rng.normal(0, 0.8, (n, 3))
+    V0 -= V0.mean(0)
+    box = (L, L, L)
+    X1, V1, F1, pe1, ke1, _ = oracle.vv_steps(*X0.T, *V0.T, box, kind="wca", dt=0.004, nsteps=60)
+    X2, V2, *_ = oracle.vv_steps(*X1.T, *(-V1).T, box, kind="wca", dt=0.004, nsteps=60)
+    d = X2 - X0
+    d -= L * np.round(d / L)
+    assert np.abs(d).max() < 1e-9
+    assert np.abs(V2 + V0).max() < 1e-9
+    _, _, _, pe0, ke0, _ = oracle.vv_steps(*X0.T, *V0.T, box, kind="wca", dt=0.004, nsteps=0)
+    assert abs((pe1 + ke1) - (pe0 + ke0)) < 2e-3 * (ke0 + abs(pe0))
