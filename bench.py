@@ -36,6 +36,7 @@ def parse():
     p.add_argument("--spread", choices=("cells", "atomic"), default="cells")
     p.add_argument("--no-learned", action="store_true", help="skip the learned-correction comparison")
     p.add_argument("--no-replicas", action="store_true", help="skip the batched-replica run")
+    p.add_argument("--no-precision", action="store_true", help="skip the bead-spring accumulation study")
     p.add_argument("--copy-calibration", action="store_true",
                    help="run only a plain 256 MiB device copy (five launches): the reference reading for the "
                         "FETCH_SIZE / WRITE_SIZE counters")
@@ -227,6 +228,8 @@ def main():
         out["learned"] = learned_leg(args.steps, args.warmup, args.skin)
     if not args.no_replicas:
         out["replicas"] = replica_leg()
+    if not args.no_precision:
+        out["precision"] = precision_leg(args.steps, args.warmup)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sysd, C)
     print(json.dumps(out))
@@ -357,6 +360,60 @@ def replica_leg(nrep=512, nsteps=1000, seed=0):
                        "wca": int(sum(sp["model"] == "wca" for sp in b["specs"]))},
             "launch": {"block": best[0], "cells_per_cutoff": best[1], "autotune_ms_10_steps": times},
             "energy_drift_per_atom_max": float(drift.max()), "energy_drift_per_atom_median": float(np.median(drift))}
+
+
+def precision_leg(steps, warmup, nchains=10486, length=100):
+    """Kremer-Grest melt of 1 048 600 beads (10 486 random-walk chains of 100, bond 0.97,
+    density 0.85, pushed off with capped WCA + FENE descent): one force evaluation (list
+    check + WCA over all pairs + FENE, energy and virial) per mode of the per-particle sums
+    (FP32, 64-bit fixed point, FP64), each timed as a graph replay, and its error against
+    the FP64 oracle on all beads."""
+    import numpy as np
+    import torch
+
+    import pdx
+    from pdx import melt, systems
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    sysd = systems.kg_chains(nchains, length=length, density=0.85, seed=0)
+    t0 = time.perf_counter()
+    X, Y, Z, bonds = melt.push_off(sysd, iterations=300)
+    t_push = time.perf_counter() - t0
+    N, box = sysd["N"], sysd["box"]
+    t0 = time.perf_counter()
+    ref = oracle.kg_forces(sysd["x"], sysd["y"], sysd["z"], box, sysd["bonds"])
+    t_ref = time.perf_counter() - t0
+    Fn = np.linalg.norm(ref["F"])
+    res = {"N": N, "chains": nchains, "bonds": int(len(sysd["bonds"])), "box": box[0],
+           "push_off_s": round(t_push, 2), "fp64_oracle_s": round(t_ref, 2), "wca_pairs": ref["pairs"]}
+    for mode in ("fp32", "fixed64", "fp64"):
+        eng = pdx.ForceEngine(N, box, pdx.WCA(epsilon=1.0, sigma=1.0), skin=0.4, accumulation=mode, bonds=bonds)
+        f = [torch.empty(N, device="cuda") for _ in range(3)]
+        E = torch.empty(N, device="cuda")
+        V = torch.empty(6, N, device="cuda")
+        eng.capture(X, Y, Z, None, *f, E, V, energy=True)
+        for _ in range(max(warmup, 1)):
+            eng.replay()
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(steps):
+            eng.replay()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - a) / steps
+        eng.evaluate(X, Y, Z, None, *f, E, V, energy=True)
+        tot = eng.totals()
+        F = torch.stack(f, 1).double().cpu().numpy()
+        d = F - ref["F"]
+        W = np.array(tot["pair_virial"])
+        visits = int(eng.nl.counts().to(torch.int64).sum().item())
+        res[mode] = {"ms_per_eval": round(1e3 * dt, 4), "evals_per_s": round(1.0 / dt, 2),
+                     "force_rel_rms": float(np.linalg.norm(d) / Fn), "force_max_abs_err": float(np.abs(d).max()),
+                     "energy_rel_err": float(abs(tot["pair_energy"] - ref["energy"]) / abs(ref["energy"])),
+                     "virial_rel_err": float(np.linalg.norm(W - ref["W"]) / np.linalg.norm(ref["W"])),
+                     "row_visits": visits}
+        bonds.status()
+        del eng
+    return res
 
 
 def cpu_baseline(sysd, C):

@@ -183,7 +183,9 @@ int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, float *fz, 
 /* Smooth PME mesh engine (B-spline spread, 3D FFT, influence, gather)     */
 /* ======================================================================= */
 
-#define PDX_INFLUENCE_SPME 1
+#define PDX_INFLUENCE_SPME   1  /* smooth PME: G = (C/V) g(k) D1 D2 D3, analytic differentiation */
+#define PDX_INFLUENCE_P3M_AD 2  /* optimal influence for analytic differentiation (alias sums) */
+#define PDX_INFLUENCE_P3M_IK 3  /* optimal influence for ik differentiation, three field transforms */
 #define PDX_NET_CHARGE_ERROR 0
 #define PDX_NET_CHARGE_BACKGROUND 1
 
@@ -196,10 +198,11 @@ typedef struct {
   int32_t order;                 /* B-spline order P, 3..12 */
   int32_t mesh[3];               /* K1, K2, K3; mesh[0] <= 0: from mesh_spacing (resolve_mesh, Spec.jl:141-146) */
   double mesh_spacing;
-  int32_t influence;             /* PDX_INFLUENCE_SPME */
+  int32_t influence;             /* PDX_INFLUENCE_* (INFLUENCE_KINDS, Spec.jl:5-24) */
   int32_t net_charge;            /* PDX_NET_CHARGE_* */
   double net_charge_tolerance;
   int32_t real_space_modifier;   /* 0 none, 1 potential shift */
+  int32_t alias_order;           /* P3M alias sums |n_d| <= alias_order, 0..6 (Spec.jl:80) */
 } pdx_mesh_spec;
 
 /* Allocate the mesh workspace and compute the FP64 influence function for the box
@@ -246,6 +249,42 @@ int pdx_mesh_copy_grid(pdx_mesh_t m, int32_t which, void *dst, pdx_stream_t stre
  * the cuFFT wrapper tests, test/test_electrostatics.jl:260-279). dir=-1: R2C real->half
  * spectrum (float2); dir=+1: C2R (unnormalised). */
 int pdx_mesh_fft(pdx_mesh_t m, int32_t dir, const void *in, void *out, pdx_stream_t stream);
+
+/* Electrostatic exceptions (normalize_exceptions, src/electrostatics/Spec.jl:200-224): pairs
+ * (i, j) whose Coulomb interaction is scaled by s >= 0 (0: excluded). The real-space sums gain
+ * C q_i q_j (s - 1) / r at the minimum image, without cutoff (the mesh carries the rest of the
+ * pair), and the pair's energy carries no potential shift (_coulomb_real_kernel!,
+ * src/electrostatics/RealSpace.jl:83-99). Host arrays, 0-based ids; a pair listed twice must
+ * carry the same scale; s == 1 is dropped; nexc = 0 clears. Needs the real-space Coulomb term
+ * and the lanes layout (lanes >= 0). */
+int pdx_pair_set_exceptions(pdx_pair_t p, int32_t n, int32_t nexc, const int32_t *i, const int32_t *j,
+                            const double *scale);
+
+/* ---- accumulation of the per-particle sums (force, energy, virial) ---- */
+/* The precision study of a bead-spring melt: pair arithmetic stays FP32, the sums over a
+ * particle's row (and bonds) run in FP32, in 64-bit fixed point (2^32 scale: exact and
+ * independent of the summation order) or in FP64; per-particle outputs are FP32 and the
+ * totals FP64 in every mode. Not available with real-space Coulomb. */
+#define PDX_ACC_FP32    0
+#define PDX_ACC_FIXED64 1
+#define PDX_ACC_FP64    2
+int pdx_pair_set_accumulation(pdx_pair_t p, int32_t mode);
+
+/* FENE bonds (_fene3_EV!, src/bonded/FENEBondCUDA.jl): u = -k R0^2/2 ln(1 - r^2/R0^2),
+ * f/r = -k / (1 - r^2/R0^2), added to the bonded particles' pair sums (bonded pairs also
+ * keep their pair interaction, the Kremer-Grest convention, src/SimulationState.jl:152-155).
+ * bi, bj: host, nbond pairs of 0-based ids. */
+typedef struct pdx_bonds_s *pdx_bonds_t;
+int pdx_bonds_create(int32_t n, int32_t nbond, const int32_t *bi, const int32_t *bj, double k, double R0,
+                     pdx_bonds_t *out);
+int pdx_bonds_destroy(pdx_bonds_t b);
+/* Attach (or detach with NULL) bonds to a pair model: pdx_pair_forces then adds them. */
+int pdx_pair_set_bonds(pdx_pair_t p, pdx_bonds_t b);
+/* Domain check of the last evaluations (the reference's DomainError,
+ * FENEBondCUDA.jl:14-39): first_invalid receives the lowest particle id with a bond at
+ * r >= R0, or -1; with reset the device word is cleared. Synchronises the stream. Returns
+ * PDX_ERR_COORDINATE when a bond was out of range. */
+int pdx_bonds_status(pdx_bonds_t b, int32_t *first_invalid, int32_t reset, pdx_stream_t stream);
 
 /* ======================================================================= */
 /* Learned correction of coarse-mesh PME forces (BF16 MFMA MLP)            */

@@ -14,6 +14,8 @@
  *                     sample only (minimum image + the images rcut reaches), for large N
  *   or_bspline        src/electrostatics/Windows.jl:188-223  (B-spline weights, FP64)
  *   or_cell_pairs     cell-list restatement used only to time a same-algorithm CPU baseline
+ *   or_kg_forces      Kremer-Grest melt: WCA over all pairs (bonded pairs included) by cells
+ *                     plus FENE bonds (_fene3_EV!, src/bonded/FENEBondCUDA.jl), FP64
  *
  * Pinning (what each function is checked against, in tests/test_oracle.py):
  *   rows    — the three known answers of test/test_neighbor_reference.jl:1-6; symmetry
@@ -27,6 +29,8 @@
  *             W_aa = -dE/d eps_aa by strain differences, background closed form.
  *   erfc    — libm erfc to 1e-13 relative on [0, 6].
  *   bspline — partition of unity, sum dw = 0, derivative by differences, M_4 closed form.
+ *   kg      — WCA part equals or_pair_direct; FENE two-bead closed form; F = -dE/dx and the
+ *             virial W_aa = -dE/d eps_aa by central differences.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -477,4 +481,89 @@ int or_num_threads(void) {
 #else
   return 1;
 #endif
+}
+
+/* ---------------------------------------------------------------- Kremer-Grest melt */
+/* WCA (rc^2 = 1.2599211 sigma^2, u + eps) between every pair within the cutoff, bonded
+ * pairs included (exclude_bonded = false, src/SimulationState.jl:152-155), found with a cell
+ * list of width >= rc, plus FENE bonds u = -k R0^2/2 ln(1 - r^2/R0^2), f/r = -k/(1 - r^2/R0^2)
+ * (_fene3_EV!, src/bonded/FENEBondCUDA.jl). Bonds are pairs (bi[b], bj[b]), 0-based. Writes
+ * F (3n, xyz interleaved), per-particle half energies E (n), total energy, and the virial
+ * sum over pairs and bonds of r_a F_b (6: xx yy zz xy xz yz). Returns -1 - (first bond
+ * index) when a bond has r >= R0, else the number of WCA pairs (each counted once). */
+long or_kg_forces(int n, const double *x, const double *y, const double *z, const double *box, double sigma,
+                  double epsilon, int nbond, const int *bi, const int *bj, double k, double R0, double *F,
+                  double *E, double *energy, double *W) {
+  const double rc = pow(2.0, 1.0 / 6.0) * sigma, rc2 = 1.2599211 * sigma * sigma, R02 = R0 * R0;
+  int nc[3];
+  for (int d = 0; d < 3; ++d) { nc[d] = (int)floor(box[d] / rc); if (nc[d] < 3) nc[d] = 3; }
+  const int ncell = nc[0] * nc[1] * nc[2];
+  int *head = (int *)malloc(sizeof(int) * ncell), *next = (int *)malloc(sizeof(int) * n);
+  int *cell = (int *)malloc(sizeof(int) * n);
+  for (int c = 0; c < ncell; ++c) head[c] = -1;
+  for (int i = 0; i < n; ++i) {
+    const double p[3] = {x[i], y[i], z[i]};
+    int ci[3];
+    for (int d = 0; d < 3; ++d) {
+      double s = p[d] / box[d] + 0.5;
+      s -= floor(s);
+      ci[d] = (int)(s * nc[d]);
+      if (ci[d] >= nc[d]) ci[d] = nc[d] - 1;
+    }
+    cell[i] = (ci[2] * nc[1] + ci[1]) * nc[0] + ci[0];
+  }
+  for (int i = n - 1; i >= 0; --i) { next[i] = head[cell[i]]; head[cell[i]] = i; }
+  long pairs = 0;
+  double etot = 0.0, w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0;
+#pragma omp parallel for schedule(static) reduction(+ : pairs, etot, w0, w1, w2, w3, w4, w5)
+  for (int i = 0; i < n; ++i) {
+    const int c = cell[i];
+    const int cx = c % nc[0], cy = (c / nc[0]) % nc[1], cz = c / (nc[0] * nc[1]);
+    double ax = 0, ay = 0, az = 0, e = 0;
+    for (int oz = -1; oz <= 1; ++oz)
+      for (int oy = -1; oy <= 1; ++oy)
+        for (int ox = -1; ox <= 1; ++ox) {
+          const int c2 = (((cz + oz + nc[2]) % nc[2]) * nc[1] + (cy + oy + nc[1]) % nc[1]) * nc[0] +
+                         (cx + ox + nc[0]) % nc[0];
+          for (int j = head[c2]; j >= 0; j = next[j]) {
+            if (j == i) continue;
+            double dx = x[i] - x[j]; dx -= box[0] * jl_round(dx / box[0]);
+            double dy = y[i] - y[j]; dy -= box[1] * jl_round(dy / box[1]);
+            double dz = z[i] - z[j]; dz -= box[2] * jl_round(dz / box[2]);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (!(r2 < rc2)) continue;
+            const double s2 = sigma * sigma / r2, s6 = s2 * s2 * s2;
+            const double a = 24.0 * epsilon * s6 * (2.0 * s6 - 1.0) / r2;
+            ax += a * dx; ay += a * dy; az += a * dz;
+            e += 0.5 * (4.0 * epsilon * s6 * (s6 - 1.0) + epsilon);
+            w0 += 0.5 * a * dx * dx; w1 += 0.5 * a * dy * dy; w2 += 0.5 * a * dz * dz;
+            w3 += 0.5 * a * dx * dy; w4 += 0.5 * a * dx * dz; w5 += 0.5 * a * dy * dz;
+            if (j > i) ++pairs;
+          }
+        }
+    F[3 * i] = ax; F[3 * i + 1] = ay; F[3 * i + 2] = az;
+    E[i] = e;
+    etot += e;
+  }
+  long bad = -1;
+  for (int b = 0; b < nbond; ++b) {
+    const int i = bi[b], j = bj[b];
+    double dx = x[i] - x[j]; dx -= box[0] * jl_round(dx / box[0]);
+    double dy = y[i] - y[j]; dy -= box[1] * jl_round(dy / box[1]);
+    double dz = z[i] - z[j]; dz -= box[2] * jl_round(dz / box[2]);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < R02)) { if (bad < 0) bad = b; continue; }
+    const double a = -k / ((R02 - r2) / R02);
+    const double u = -0.5 * k * R02 * (r2 < 0.5 * R02 ? log1p(-r2 / R02) : log((R02 - r2) / R02));
+    F[3 * i] += a * dx; F[3 * i + 1] += a * dy; F[3 * i + 2] += a * dz;
+    F[3 * j] -= a * dx; F[3 * j + 1] -= a * dy; F[3 * j + 2] -= a * dz;
+    E[i] += 0.5 * u; E[j] += 0.5 * u;
+    etot += u;
+    w0 += a * dx * dx; w1 += a * dy * dy; w2 += a * dz * dz;
+    w3 += a * dx * dy; w4 += a * dx * dz; w5 += a * dy * dz;
+  }
+  *energy = etot;
+  W[0] = w0; W[1] = w1; W[2] = w2; W[3] = w3; W[4] = w4; W[5] = w5;
+  free(head); free(next); free(cell);
+  return bad >= 0 ? -1 - bad : pairs;
 }

@@ -36,6 +36,8 @@ def _load():
     lib.or_bspline.restype = i
     lib.or_cell_lj.argtypes = [i, P(d), P(d), P(d), P(d), d, d, d, P(d), P(d), P(d), P(d)]
     lib.or_cell_lj.restype = C.c_long
+    lib.or_kg_forces.argtypes = [i, P(d), P(d), P(d), P(d), d, d, i, P(i), P(i), d, d, P(d), P(d), P(d), P(d)]
+    lib.or_kg_forces.restype = C.c_long
     lib.or_num_threads.restype = i
     return lib
 
@@ -178,3 +180,24 @@ def vv_steps(x, y, z, vx, vy, vz, box, kind="lj", sigma=1.0, epsilon=1.0, cutoff
     ke = 0.5 * mass * (V ** 2).sum()
     w = d["virial"][:, :3].sum()
     return X, V, d["F"], pe, ke, w
+
+
+def kg_forces(x, y, z, box, bonds, sigma=1.0, epsilon=1.0, k=30.0, R0=1.5):
+    """FP64 Kremer-Grest forces: WCA over all pairs (bonded ones included) plus FENE bonds.
+    bonds: (nb, 2) int array, 0-based. Returns dict F (n,3), E (n), energy, W (6), pairs;
+    raises ValueError naming the first bond with r >= R0."""
+    n = len(x)
+    xa, ya, za, bx = _d(x), _d(y), _d(z), _d(box)
+    b = np.ascontiguousarray(bonds, dtype=np.int32).reshape(-1, 2)
+    bi, bj = _i(b[:, 0].copy()), _i(b[:, 1].copy())
+    F = np.zeros(3 * n)
+    E = np.zeros(n)
+    W = np.zeros(6)
+    e = C.c_double()
+    P = C.POINTER(C.c_double)
+    r = _lib.or_kg_forces(n, xa[1], ya[1], za[1], bx[1], float(sigma), float(epsilon), len(b), bi[1], bj[1],
+                          float(k), float(R0), F.ctypes.data_as(P), E.ctypes.data_as(P), C.byref(e),
+                          W.ctypes.data_as(P))
+    if r < 0:
+        raise ValueError(f"FENE bond {-1 - r} has r >= R0")
+    return dict(F=F.reshape(n, 3), E=E, energy=e.value, W=W, pairs=int(r))

@@ -191,8 +191,21 @@ struct pdx_pair_s {
   float pe[24] = {0}, pf[24] = {0};
   double fit_err_e = 0, fit_err_f = 0;
   int lanes = 4;
+  int accumulate = PDX_ACC_FP32;      // force / energy / virial accumulator of the row sums
+  struct pdx_bonds_s *bonds = nullptr;  // FENE bonds folded into the same per-particle sums
+  int32_t exc_n = 0, nexc = 0;          // Coulomb exceptions (CSR, both directions)
+  pdx::DevBuf<int32_t> exc_off, exc_j;
+  pdx::DevBuf<float> exc_s;
   pdx::DevBuf<double> partial;
   pdx::DevBuf<unsigned> ticket;
+};
+
+struct pdx_bonds_s {
+  int32_t N = 0, nbond = 0;
+  double k = 0, R0 = 0;
+  pdx::DevBuf<int32_t> offsets;   // N + 1: bonds of particle i at partners[offsets[i] .. offsets[i+1])
+  pdx::DevBuf<int32_t> partners;  // 2 nbond, 0-based (each bond listed from both ends)
+  pdx::DevBuf<int32_t> invalid;   // lowest particle id with a bond at r >= R0 (INT32_MAX: none)
 };
 
 struct pdx_mesh_s {
@@ -212,6 +225,13 @@ struct pdx_mesh_s {
   pdx::DevBuf<double> acc;                 // 8: mesh energy + 6 virial + spare
   pdx::DevBuf<double> partial;
   pdx::DevBuf<unsigned> ticket;
+  // P3M: virial coefficients Vc (6 x half spectrum, SoA) of the optimal influence function;
+  // ik: the z-field spectrum, a scratch spectrum and the three real field grids
+  pdx::DevBuf<float> Vc;
+  pdx::DevBuf<float2> spec_ez, spec_tmp;
+  pdx::DevBuf<float> ex, ey, ez;
+  pdx::DevBuf<double> Utab;                // alias tables (2 nmax + 1) x K per dimension
+  bool pot_valid = false;                  // ik: the potential grid of the last solve exists
   int nmode_blocks = 0;
   int T1 = 0, T2 = 0, T3 = 0;  // FFT tile widths, fixed at creation
 };

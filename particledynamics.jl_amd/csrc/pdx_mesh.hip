@@ -317,8 +317,9 @@ __global__ __launch_bounds__(TPB) void k_fft_x_c2r(const float2 *spec, float *ou
 
 // ---- y pass: tiles of T adjacent m1 at fixed m3; elements m2 at stride K1h
 template <int T, bool BIG>
-__global__ __launch_bounds__(TPB) void k_fft_y(float2 *spec, int K1h, int K2, FFTPlan pl,
-                                               const float2 *__restrict__ tw, float sg) {
+__global__ __launch_bounds__(TPB) void k_fft_y(const float2 *src, float2 *spec, int K1h, int K2, FFTPlan pl,
+                                               const float2 *__restrict__ tw, float sg, int ikdim, float kscale,
+                                               int K1) {
   extern __shared__ float2 lds[];
   float2 *twl = lds, *b0 = lds + K2, *b1 = b0 + (size_t)T * K2;
   load_twiddles(twl, tw, K2);
@@ -330,9 +331,22 @@ __global__ __launch_bounds__(TPB) void k_fft_y(float2 *spec, int K1h, int K2, FF
   const int n = T * K2;
   staged<8, float2>(n, [&](int e) {
     const int l = e & (T - 1), m2 = e >> TL;
-    return m10 + l < K1h ? spec[base + (size_t)m2 * K1h + m10 + l] : make_float2(0.f, 0.f);
+    return m10 + l < K1h ? src[base + (size_t)m2 * K1h + m10 + l] : make_float2(0.f, 0.f);
   }, [&](int e, float2 v) { b0[e] = v; });
   __syncthreads();
+  if (ikdim) {  // field component: times -i k_d, the Nyquist component of k_d zeroed
+    for (int e = threadIdx.x; e < n; e += TPB) {
+      const int l = e & (T - 1), m2 = e >> TL;
+      const int m1 = m10 + l;
+      int sgn;
+      if (ikdim == 1) sgn = ((K1 % 2 == 0) && m1 == K1 / 2) ? 0 : m1;
+      else sgn = ((K2 % 2 == 0) && m2 == K2 / 2) ? 0 : (m2 <= K2 / 2 ? m2 : m2 - K2);
+      const float kd = kscale * (float)sgn;
+      const float2 z = b0[e];
+      b0[e] = make_float2(z.y * kd, -z.x * kd);
+    }
+    __syncthreads();
+  }
   float2 *r = lds_fft<T, BIG>(b0, b1, pl, twl, sg);
   for (int e = threadIdx.x; e < n; e += TPB) {
     const int l = e & (T - 1), m2 = e >> TL;
@@ -343,6 +357,9 @@ __global__ __launch_bounds__(TPB) void k_fft_y(float2 *spec, int K1h, int K2, FF
 
 struct SolveArgs {
   const float *G;
+  const float *Vc;     // virial coefficients (6 x nmodes, SoA) or null: the Gaussian closed form
+  float2 *ez;          // ik: receives G b (before the inverse z transform) or null
+  size_t nmodes;
   int K1, K2, K3, K1h;
   double twopi_L[3];   // 2 pi / L_d
   double inv4a2;       // 1/(4 alpha^2): Vc_ab = G (delta_ab + c k_a k_b), c = -2/k^2 - 2 inv4a2
@@ -372,7 +389,7 @@ __global__ __launch_bounds__(TPB) void k_fft_z(float2 *spec, FFTPlan pl, const f
   }, [&](int e, float2 v) { b0[e] = v; });
   __syncthreads();
   float2 *r = b0;
-  if (MODE != 1) r = lds_fft<T, BIG>(b0, b1, pl, twl, -1.0f);
+  if (MODE == 0 || MODE == 2) r = lds_fft<T, BIG>(b0, b1, pl, twl, -1.0f);
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   if (MODE == 2) {
     const int s2 = m2 <= K2 / 2 ? m2 : m2 - K2;
@@ -390,23 +407,41 @@ __global__ __launch_bounds__(TPB) void k_fft_z(float2 *spec, FFTPlan pl, const f
         const int s3 = m3 <= K3 / 2 ? m3 : m3 - K3;
         const bool ny1 = (s.K1 % 2 == 0) && (m1 == s.K1 / 2);
         const bool ny3 = (K3 % 2 == 0) && (m3 == K3 / 2);
-        const double kx = s.twopi_L[0] * m1, ky = s.twopi_L[1] * s2, kz = s.twopi_L[2] * s3;
-        const double k2 = kx * kx + ky * ky + kz * kz;
-        const double c = -2.0 / k2 - 2.0 * s.inv4a2;
         acc[0] += a;
-        acc[1] += a * (1.0 + c * kx * kx);
-        acc[2] += a * (1.0 + c * ky * ky);
-        acc[3] += a * (1.0 + c * kz * kz);
-        acc[4] += (ny1 || ny2) ? 0.0 : a * c * kx * ky;
-        acc[5] += (ny1 || ny3) ? 0.0 : a * c * kx * kz;
-        acc[6] += (ny2 || ny3) ? 0.0 : a * c * ky * kz;
+        if (s.Vc) {  // tabulated -dG/d eps_ab (the optimal influence functions)
+          const double b2 = w * ((double)z.x * z.x + (double)z.y * z.y);
+#pragma unroll
+          for (int c = 0; c < 6; ++c) acc[1 + c] += b2 * (double)s.Vc[(size_t)c * s.nmodes + idx];
+        } else {
+          const double kx = s.twopi_L[0] * m1, ky = s.twopi_L[1] * s2, kz = s.twopi_L[2] * s3;
+          const double k2 = kx * kx + ky * ky + kz * kz;
+          const double c = -2.0 / k2 - 2.0 * s.inv4a2;
+          acc[1] += a * (1.0 + c * kx * kx);
+          acc[2] += a * (1.0 + c * ky * ky);
+          acc[3] += a * (1.0 + c * kz * kz);
+          acc[4] += (ny1 || ny2) ? 0.0 : a * c * kx * ky;
+          acc[5] += (ny1 || ny3) ? 0.0 : a * c * kx * kz;
+          acc[6] += (ny2 || ny3) ? 0.0 : a * c * ky * kz;
+        }
       }
-      r[m3 * T + l] = cscale(z, Gm);
+      const float2 gz = cscale(z, Gm);
+      if (s.ez) s.ez[idx] = gz;
+      r[m3 * T + l] = gz;
     }
     __syncthreads();
     float2 *other = (r == b0) ? b1 : b0;
     r = lds_fft<T, BIG>(r, other, pl, twl, +1.0f);
   } else if (MODE == 1) {
+    r = lds_fft<T, BIG>(b0, b1, pl, twl, +1.0f);
+  } else if (MODE == 3) {  // ik z field: times -i k3 (Nyquist zeroed), then the inverse z transform
+    for (int e = threadIdx.x; e < n; e += TPB) {
+      const int m3 = e >> TL;
+      const int s3 = ((K3 % 2 == 0) && m3 == K3 / 2) ? 0 : (m3 <= K3 / 2 ? m3 : m3 - K3);
+      const float kd = (float)(s.twopi_L[2] * s3);
+      const float2 z = b0[e];
+      b0[e] = make_float2(z.y * kd, -z.x * kd);
+    }
+    __syncthreads();
     r = lds_fft<T, BIG>(b0, b1, pl, twl, +1.0f);
   }
   for (int e = threadIdx.x; e < n; e += TPB) {
@@ -714,6 +749,157 @@ __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__
   }
 }
 
+// ---- ik gather (_gather_ik_kernel!, src/electrostatics/MeshKernels.jl:108-147): F_i = q_i
+// sum_n W(u_i - n) E(n) from the three field grids; with ENERGY, E_i += 1/2 q_i phi_i + bg q_i.
+// Same lane layout as k_gather.
+template <int P, bool ENERGY>
+__global__ __launch_bounds__(TPB) void k_gather_ik(int N, const float *__restrict__ x, const float *__restrict__ y,
+                                                   const float *__restrict__ z, const float *__restrict__ q,
+                                                   const int32_t *__restrict__ order, const float *__restrict__ ex,
+                                                   const float *__restrict__ ey, const float *__restrict__ ez,
+                                                   const float *__restrict__ pot, int K1, int K2, int K3, float Lx,
+                                                   float Ly, float Lz, float *fx, float *fy, float *fz, float *Epot,
+                                                   float bg, int set, float *Vglobal, const double *acc,
+                                                   double ubg) {
+  constexpr int LPP = lanes_for(P);
+  const int tid = blockIdx.x * TPB + threadIdx.x;
+  if (Vglobal && tid < 6) Vglobal[tid] = (float)(acc[1 + tid] + (tid < 3 ? ubg : 0.0));
+  const int t = tid / LPP;
+  const int lane = tid % LPP;
+  const bool active = t < N;
+  const int i = active ? (order ? order[t] - 1 : t) : 0;
+  const float qi = active ? q[i] : 0.0f;
+  float phi = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+  if (active && qi != 0.0f) {
+    float wx[P], wy[P], wz[P], d[P];
+    const int k1 = bspline<P>(grid_coordinate(x[i], Lx, K1), wx, d);
+    const int k2 = bspline<P>(grid_coordinate(y[i], Ly, K2), wy, d);
+    const int k3 = bspline<P>(grid_coordinate(z[i], Lz, K3), wz, d);
+    const size_t KK = (size_t)K1 * K2;
+    for (int l = lane; l < P * P; l += LPP) {
+      const int j1 = l % P, j2 = l / P;
+      const int n1 = wrap_node(k1 + j1, K1), n2 = wrap_node(k2 + j2, K2);
+      const size_t b = (size_t)n1 + (size_t)K1 * n2;
+      float ax = 0.f, ay = 0.f, az = 0.f, a0 = 0.f;
+#pragma unroll
+      for (int j3 = 0; j3 < P; ++j3) {
+        const size_t o = b + KK * wrap_node(k3 + j3, K3);
+        ax = fmaf(wz[j3], ex[o], ax);
+        ay = fmaf(wz[j3], ey[o], ay);
+        az = fmaf(wz[j3], ez[o], az);
+        if (ENERGY) a0 = fmaf(wz[j3], pot[o], a0);
+      }
+      const float w12 = sel<P>(wx, j1) * sel<P>(wy, j2);
+      sx = fmaf(w12, ax, sx);
+      sy = fmaf(w12, ay, sy);
+      sz = fmaf(w12, az, sz);
+      if (ENERGY) phi = fmaf(w12, a0, phi);
+    }
+  }
+  sx = wave_sum_width(sx, LPP);
+  sy = wave_sum_width(sy, LPP);
+  sz = wave_sum_width(sz, LPP);
+  if (ENERGY) phi = wave_sum_width(phi, LPP);
+  if (!active || lane != 0) return;
+  const float ffx = qi * sx, ffy = qi * sy, ffz = qi * sz;
+  if (set) {
+    fx[i] = ffx; fy[i] = ffy; fz[i] = ffz;
+    if (ENERGY && Epot) Epot[i] = 0.5f * qi * phi + bg * qi;
+  } else if (qi != 0.0f) {
+    fx[i] += ffx; fy[i] += ffy; fz[i] += ffz;
+    if (ENERGY && Epot) Epot[i] += 0.5f * qi * phi + bg * qi;
+  }
+}
+
+// ---- optimal influence functions (Hockney-Eastwood; Ballenegger, Cerda & Holm 2012) for
+// the B-spline window with alias sums |n_d| <= nmax, in FP64 on the device, one thread per
+// half-spectrum mode (_influence_alias_kernel!, src/electrostatics/Influence.jl:77-181):
+//   AD: G = (C/V) sum U^2 g k_n^2 / (sum U^2 . sum U^2 k_n^2)
+//   ik: G = (C/V) sum U^2 g (D.k_n) / (|D|^2 (sum U^2)^2), D = k with Nyquist components zeroed
+// with g(k) = 4 pi exp(-k^2/(4 alpha^2)) / k^2, and the virial coefficients Vc_ab = -dG/d eps_ab
+// at fixed fractional coordinates (Nyquist shears zeroed). U tables: U_d[m (2 nmax + 1) + n + nmax].
+__global__ void k_influence_alias(float *G, float *Vc, size_t nmodes, const double *__restrict__ U1t,
+                                  const double *__restrict__ U2t, const double *__restrict__ U3t, int K1, int K2,
+                                  int K3, double L1, double L2, double L3, double CoverV, double alpha, int nmax,
+                                  int ik) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nmodes) return;
+  const int K1h = K1 / 2 + 1;
+  const int m1 = (int)(idx % K1h);
+  const size_t rest = idx / K1h;
+  const int m2 = (int)(rest % K2), m3 = (int)(rest / K2);
+  const int s1 = m1, s2 = m2 <= K2 / 2 ? m2 : m2 - K2, s3 = m3 <= K3 / 2 ? m3 : m3 - K3;
+  const bool ny1 = (K1 % 2 == 0) && m1 == K1 / 2, ny2 = (K2 % 2 == 0) && m2 == K2 / 2,
+             ny3 = (K3 % 2 == 0) && m3 == K3 / 2;
+  const double tp = 2.0 * M_PI;
+  const double kx = tp * s1 / L1, ky = tp * s2 / L2, kz = tp * s3 / L3;
+  const double dx = ny1 ? 0.0 : kx, dy = ny2 ? 0.0 : ky, dz = ny3 ? 0.0 : kz;
+  const double d2 = dx * dx + dy * dy + dz * dz;
+  const bool zero = (s1 == 0) && (s2 == 0) && (s3 == 0);
+  if (zero || (ik && d2 == 0.0)) {
+    G[idx] = 0.f;
+    for (int c = 0; c < 6; ++c) Vc[(size_t)c * nmodes + idx] = 0.f;
+    return;
+  }
+  const int W = 2 * nmax + 1;
+  const double i4a2 = 1.0 / (4.0 * alpha * alpha);
+  double A = 0, B = 0, Cc = 0, a[6] = {0, 0, 0, 0, 0, 0}, c[6] = {0, 0, 0, 0, 0, 0};
+  for (int n1 = -nmax; n1 <= nmax; ++n1) {
+    const double U1 = U1t[(size_t)m1 * W + n1 + nmax];
+    const double q1 = tp * (s1 + (double)K1 * n1) / L1;
+    for (int n2 = -nmax; n2 <= nmax; ++n2) {
+      const double U2 = U2t[(size_t)m2 * W + n2 + nmax];
+      const double q2 = tp * (s2 + (double)K2 * n2) / L2;
+      for (int n3 = -nmax; n3 <= nmax; ++n3) {
+        const double U3 = U3t[(size_t)m3 * W + n3 + nmax];
+        const double q3 = tp * (s3 + (double)K3 * n3) / L3;
+        const double u = U1 * U2 * U3, U2tot = u * u;
+        B += U2tot;
+        const double qq = q1 * q1 + q2 * q2 + q3 * q3;
+        if (qq == 0.0) continue;
+        const double g = 4.0 * M_PI * exp(-qq * i4a2) / qq;
+        const double gpk = g * (-2.0 / qq - 2.0 * i4a2);  // g'(k)/k
+        if (ik) {
+          const double dq = dx * q1 + dy * q2 + dz * q3;
+          A += U2tot * g * dq;
+          const double h = U2tot * gpk * dq, e = U2tot * g;
+          a[0] += h * q1 * q1 + e * 2.0 * dx * q1;
+          a[1] += h * q2 * q2 + e * 2.0 * dy * q2;
+          a[2] += h * q3 * q3 + e * 2.0 * dz * q3;
+          a[3] += h * q1 * q2 + e * (dx * q2 + q1 * dy);
+          a[4] += h * q1 * q3 + e * (dx * q3 + q1 * dz);
+          a[5] += h * q2 * q3 + e * (dy * q3 + q2 * dz);
+        } else {
+          A += U2tot * g * qq;
+          Cc += U2tot * qq;
+          const double h = U2tot * (gpk * qq + 2.0 * g), h2 = 2.0 * U2tot;
+          a[0] += h * q1 * q1; a[1] += h * q2 * q2; a[2] += h * q3 * q3;
+          a[3] += h * q1 * q2; a[4] += h * q1 * q3; a[5] += h * q2 * q3;
+          c[0] += h2 * q1 * q1; c[1] += h2 * q2 * q2; c[2] += h2 * q3 * q3;
+          c[3] += h2 * q1 * q2; c[4] += h2 * q1 * q3; c[5] += h2 * q2 * q3;
+        }
+      }
+    }
+  }
+  double Gm, v[6];
+  if (ik) {
+    const double den = d2 * B * B;
+    Gm = CoverV * A / den;
+    const double f = CoverV / den, r = CoverV * A / (B * B) * 2.0 / (d2 * d2);
+    const double dd[6] = {dx * dx, dy * dy, dz * dz, dx * dy, dx * dz, dy * dz};
+    for (int k = 0; k < 6; ++k) v[k] = (k < 3 ? Gm : 0.0) + f * a[k] - r * dd[k];
+  } else {
+    Gm = CoverV * A / (B * Cc);
+    const double f = CoverV / (B * Cc), r = CoverV * A / (B * Cc * Cc);
+    for (int k = 0; k < 6; ++k) v[k] = (k < 3 ? Gm : 0.0) + f * a[k] - r * c[k];
+  }
+  if (ny1 || ny2) v[3] = 0.0;
+  if (ny1 || ny3) v[4] = 0.0;
+  if (ny2 || ny3) v[5] = 0.0;
+  G[idx] = (float)Gm;
+  for (int k = 0; k < 6; ++k) Vc[(size_t)k * nmodes + idx] = (float)v[k];
+}
+
 // Learned-correction inputs from the charge grid of the last spread (80 per particle):
 // q_i Q(n) at the 64 nodes n = floor(u) - 3 + {0..3}^3 that an order-4 B-spline at u covers
 // (x fastest), q_i (u - floor(u)) per dimension, q_i, q_i^2, then zeros up to the MFMA width.
@@ -808,10 +994,42 @@ int lines_per_tile(int K, int def, const char *env) {
   return T;
 }
 
+// B-spline window transform with alias images: U[m][n] = sinc(pi (s(m) + K n) / K)^P
+// (window_alias_table, src/electrostatics/Windows.jl:391-399).
+std::vector<double> alias_table(int P, int K, int nmax) {
+  const int W = 2 * nmax + 1;
+  std::vector<double> U((size_t)K * W);
+  for (int m = 0; m < K; ++m) {
+    const int sm = m <= K / 2 ? m : m - K;
+    for (int n = -nmax; n <= nmax; ++n) {
+      const double x = M_PI * (sm + (double)K * n) / K;
+      const double sn = std::fabs(x) < 1e-12 ? 1.0 : std::sin(x) / x;
+      U[(size_t)m * W + n + nmax] = std::pow(sn, P);
+    }
+  }
+  return U;
+}
+
 int compute_influence(pdx_mesh_t m) {
   const int K1 = m->K1, K2 = m->K2, K3 = m->K3, K1h = m->K1h;
   const double L1 = m->box[0], L2 = m->box[1], L3 = m->box[2];
   const double CoverV = m->C / (L1 * L2 * L3);
+  if (m->spec.influence != PDX_INFLUENCE_SPME) {
+    const int nmax = m->spec.alias_order, W = 2 * nmax + 1;
+    std::vector<double> U;
+    for (int K : {K1, K2, K3}) {
+      const std::vector<double> u = alias_table(m->P, K, nmax);
+      U.insert(U.end(), u.begin(), u.end());
+    }
+    PDX_HIP(hipMemcpy(m->Utab.p, U.data(), sizeof(double) * U.size(), hipMemcpyHostToDevice));
+    const size_t nmodes = (size_t)K1h * K2 * K3;
+    hipLaunchKernelGGL(k_influence_alias, dim3((unsigned)((nmodes + 127) / 128)), dim3(128), 0, 0, m->G.p, m->Vc.p,
+                       nmodes, m->Utab.p, m->Utab.p + (size_t)K1 * W, m->Utab.p + (size_t)(K1 + K2) * W, K1, K2,
+                       K3, L1, L2, L3, CoverV, m->alpha, nmax, m->spec.influence == PDX_INFLUENCE_P3M_IK ? 1 : 0);
+    PDX_LAUNCH_CHECK();
+    PDX_HIP(hipDeviceSynchronize());
+    return PDX_OK;
+  }
   const std::vector<double> D1 = deconvolution(m->P, K1), D2 = deconvolution(m->P, K2), D3 = deconvolution(m->P, K3);
   std::vector<float> G((size_t)K1h * K2 * K3);
   const double a2 = m->alpha * m->alpha;
@@ -870,6 +1088,9 @@ Plans plans_of(pdx_mesh_t m) {
 SolveArgs solve_args(pdx_mesh_t m) {
   SolveArgs s{};
   s.G = m->G.p;
+  s.Vc = m->Vc.p;
+  s.ez = nullptr;
+  s.nmodes = (size_t)m->K1h * m->K2 * m->K3;
   s.K1 = m->K1; s.K2 = m->K2; s.K3 = m->K3; s.K1h = m->K1h;
   for (int d = 0; d < 3; ++d) s.twopi_L[d] = 2 * M_PI / m->box[d];
   s.inv4a2 = 1.0 / (4.0 * m->alpha * m->alpha);
@@ -911,37 +1132,41 @@ int enqueue_forward_xy(pdx_mesh_t m, const Plans &pl, const void *src, bool fixe
   const int gy = ((m->K1h + pl.T2 - 1) / pl.T2) * m->K3;
 #define PDX_YF(TT)                                                                                         \
   if (pl.p2.big)                                                                                             \
-    hipLaunchKernelGGL((k_fft_y<TT, true>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, m->spec_c.p, m->K1h,  \
-                       m->K2, pl.p2, tw2, -1.0f);                                                            \
+    hipLaunchKernelGGL((k_fft_y<TT, true>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, m->spec_c.p,          \
+                       m->spec_c.p, m->K1h, m->K2, pl.p2, tw2, -1.0f, 0, 0.0f, m->K1);                        \
   else                                                                                                       \
-    hipLaunchKernelGGL((k_fft_y<TT, false>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, m->spec_c.p, m->K1h, \
-                       m->K2, pl.p2, tw2, -1.0f);
+    hipLaunchKernelGGL((k_fft_y<TT, false>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, m->spec_c.p,         \
+                       m->spec_c.p, m->K1h, m->K2, pl.p2, tw2, -1.0f, 0, 0.0f, m->K1);
   PDX_T_SWITCH(pl.T2, PDX_YF)
 #undef PDX_YF
   PDX_LAUNCH_CHECK();
   return PDX_OK;
 }
 
-int enqueue_inverse_yx(pdx_mesh_t m, const Plans &pl, float *dst, hipStream_t st) {
+// Inverse y pass from `src` into `spec` (in place when equal; times -i k_d first when ikdim is 1
+// or 2), then the inverse x pass of `spec` into the real grid `dst`.
+int enqueue_inverse_yx(pdx_mesh_t m, const Plans &pl, const float2 *src, float2 *spec, float *dst, int ikdim,
+                       hipStream_t st) {
   const float2 *tw1 = m->tw.p, *tw2 = m->tw.p + m->K1;
   const int gy = ((m->K1h + pl.T2 - 1) / pl.T2) * m->K3;
+  const float kscale = ikdim == 1 ? (float)(2 * M_PI / m->box[0]) : (float)(2 * M_PI / m->box[1]);
 #define PDX_YI(TT)                                                                                         \
   if (pl.p2.big)                                                                                             \
-    hipLaunchKernelGGL((k_fft_y<TT, true>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, m->spec_c.p, m->K1h,  \
-                       m->K2, pl.p2, tw2, +1.0f);                                                            \
+    hipLaunchKernelGGL((k_fft_y<TT, true>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, src, spec, m->K1h,    \
+                       m->K2, pl.p2, tw2, +1.0f, ikdim, kscale, m->K1);                                      \
   else                                                                                                       \
-    hipLaunchKernelGGL((k_fft_y<TT, false>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, m->spec_c.p, m->K1h, \
-                       m->K2, pl.p2, tw2, +1.0f);
+    hipLaunchKernelGGL((k_fft_y<TT, false>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, src, spec, m->K1h,   \
+                       m->K2, pl.p2, tw2, +1.0f, ikdim, kscale, m->K1);
   PDX_T_SWITCH(pl.T2, PDX_YI)
 #undef PDX_YI
   const int nlines = m->K2 * m->K3;
   const int gx = (nlines + 2 * pl.T1 - 1) / (2 * pl.T1);
 #define PDX_XI(TT)                                                                                            \
   if (pl.p1.big)                                                                                                \
-    hipLaunchKernelGGL((k_fft_x_c2r<TT, true>), dim3(gx), dim3(TPB), fft_lds(TT, m->K1), st, m->spec_c.p, dst,    \
+    hipLaunchKernelGGL((k_fft_x_c2r<TT, true>), dim3(gx), dim3(TPB), fft_lds(TT, m->K1), st, spec, dst,           \
                        m->K1, m->K1h, nlines, pl.p1, tw1);                                                      \
   else                                                                                                          \
-    hipLaunchKernelGGL((k_fft_x_c2r<TT, false>), dim3(gx), dim3(TPB), fft_lds(TT, m->K1), st, m->spec_c.p, dst,   \
+    hipLaunchKernelGGL((k_fft_x_c2r<TT, false>), dim3(gx), dim3(TPB), fft_lds(TT, m->K1), st, spec, dst,          \
                        m->K1, m->K1h, nlines, pl.p1, tw1);
   PDX_T_SWITCH(pl.T1, PDX_XI)
 #undef PDX_XI
@@ -950,16 +1175,19 @@ int enqueue_inverse_yx(pdx_mesh_t m, const Plans &pl, float *dst, hipStream_t st
 }
 
 template <int MODE, bool ENERGY>
-void launch_z(pdx_mesh_t m, const Plans &pl, hipStream_t st) {
+void launch_z(pdx_mesh_t m, const Plans &pl, hipStream_t st, float2 *spec = nullptr, float2 *ez = nullptr) {
   const float2 *tw3 = m->tw.p + m->K1 + m->K2;
   const int gz = ((m->K1h + pl.T3 - 1) / pl.T3) * m->K2;
+  if (!spec) spec = m->spec_c.p;
+  SolveArgs sa = solve_args(m);
+  sa.ez = ez;
 #define PDX_ZZ(TT)                                                                                              \
   if (pl.p3.big)                                                                                                  \
     hipLaunchKernelGGL((k_fft_z<TT, MODE, ENERGY, true>), dim3(gz), dim3(TPB), fft_lds(TT, m->K3), st,              \
-                       m->spec_c.p, pl.p3, tw3, solve_args(m));                                                  \
+                       spec, pl.p3, tw3, sa);                                                                    \
   else                                                                                                            \
     hipLaunchKernelGGL((k_fft_z<TT, MODE, ENERGY, false>), dim3(gz), dim3(TPB), fft_lds(TT, m->K3), st,             \
-                       m->spec_c.p, pl.p3, tw3, solve_args(m));
+                       spec, pl.p3, tw3, sa);
   PDX_T_SWITCH(pl.T3, PDX_ZZ)
 #undef PDX_ZZ
 }
@@ -982,6 +1210,18 @@ void launch_z(pdx_mesh_t m, const Plans &pl, hipStream_t st) {
                          Vglobal, m->acc.p, m->ubg);                                                           \
     break;
 
+#define PDX_GATHER_IK_CASE(PP)                                                                                  \
+  case PP:                                                                                                      \
+    if (energy)                                                                                                 \
+      hipLaunchKernelGGL((k_gather_ik<PP, true>), dim3(blocks_of(PP)), dim3(TPB), 0, st, N, x, y, z, q, order,  \
+                         m->ex.p, m->ey.p, m->ez.p, m->pot.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], fx, fy,    \
+                         fz, Epot, m->bg, set, Vglobal, m->acc.p, m->ubg);                                      \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_gather_ik<PP, false>), dim3(blocks_of(PP)), dim3(TPB), 0, st, N, x, y, z, q, order, \
+                         m->ex.p, m->ey.p, m->ez.p, m->pot.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], fx, fy,    \
+                         fz, Epot, m->bg, set, Vglobal, m->acc.p, m->ubg);                                      \
+    break;
+
 }  // namespace
 
 extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const double box[3], pdx_mesh_t *out) {
@@ -994,7 +1234,10 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
     return fail(PDX_ERR_ARGUMENT, "relative_permittivity must be finite and positive.");
   if (!(std::isfinite(sp.real_cutoff) && sp.real_cutoff > 0))
     return fail(PDX_ERR_ARGUMENT, "real_cutoff must be finite and positive.");
-  if (sp.influence != PDX_INFLUENCE_SPME) return fail(PDX_ERR_ARGUMENT, "Only the SPME influence function is implemented.");
+  if (sp.influence != PDX_INFLUENCE_SPME && sp.influence != PDX_INFLUENCE_P3M_AD && sp.influence != PDX_INFLUENCE_P3M_IK)
+    return fail(PDX_ERR_ARGUMENT, "Unknown influence kind.");
+  if (sp.influence != PDX_INFLUENCE_SPME && (sp.alias_order < 0 || sp.alias_order > 6))
+    return fail(PDX_ERR_ARGUMENT, "alias_order must be between 0 and 6.");
   if (sp.order < 3 || sp.order > 12)
     return fail(PDX_ERR_ARGUMENT, "B-spline assignment order must be between 3 and 12.");
   if (sp.order > 8) return fail(PDX_ERR_ARGUMENT, "Assignment orders above 8 are not compiled in this build.");
@@ -1055,6 +1298,18 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
   if ((rc = m->grid64.alloc(nnodes)) || (rc = m->spec_c.alloc(nmodes)) || (rc = m->pot.alloc(nnodes)) ||
       (rc = m->G.alloc(nmodes)) || (rc = m->tw.alloc(K[0] + K[1] + K[2])) || (rc = m->acc.alloc(8)) ||
       (rc = m->partial.alloc((size_t)7 * m->nmode_blocks)) || (rc = m->ticket.alloc(1))) {
+    delete m;
+    return rc;
+  }
+  if (sp.influence != PDX_INFLUENCE_SPME &&
+      ((rc = m->Vc.alloc(6 * nmodes)) ||
+       (rc = m->Utab.alloc((size_t)(K[0] + K[1] + K[2]) * (2 * sp.alias_order + 1))))) {
+    delete m;
+    return rc;
+  }
+  if (sp.influence == PDX_INFLUENCE_P3M_IK &&
+      ((rc = m->spec_ez.alloc(nmodes)) || (rc = m->spec_tmp.alloc(nmodes)) || (rc = m->ex.alloc(nnodes)) ||
+       (rc = m->ey.alloc(nnodes)) || (rc = m->ez.alloc(nnodes)))) {
     delete m;
     return rc;
   }
@@ -1200,10 +1455,26 @@ extern "C" int pdx_mesh_solve(pdx_mesh_t m, int32_t flags, pdx_stream_t stream) 
   const Plans pl = plans_of(m);
   int rc = enqueue_forward_xy(m, pl, m->grid64.p, true, st);
   if (rc) return rc;
-  if (flags & PDX_ENERGY) launch_z<2, true>(m, pl, st);
-  else launch_z<2, false>(m, pl, st);
+  const bool energy = (flags & PDX_ENERGY) != 0;
+  if (m->spec.influence != PDX_INFLUENCE_P3M_IK) {
+    if (energy) launch_z<2, true>(m, pl, st);
+    else launch_z<2, false>(m, pl, st);
+    PDX_LAUNCH_CHECK();
+    return enqueue_inverse_yx(m, pl, m->spec_c.p, m->spec_c.p, m->pot.p, 0, st);
+  }
+  // ik: G b is kept for the z field; E_d = IFFT(-i k_d G b) for d = z, x, y, and the potential
+  // (three field transforms, plus the potential when energies are wanted)
+  if (energy) launch_z<2, true>(m, pl, st, nullptr, m->spec_ez.p);
+  else launch_z<2, false>(m, pl, st, nullptr, m->spec_ez.p);
+  launch_z<3, false>(m, pl, st, m->spec_ez.p);
   PDX_LAUNCH_CHECK();
-  return enqueue_inverse_yx(m, pl, m->pot.p, st);
+  if ((rc = enqueue_inverse_yx(m, pl, m->spec_ez.p, m->spec_ez.p, m->ez.p, 0, st)) ||
+      (rc = enqueue_inverse_yx(m, pl, m->spec_c.p, m->spec_tmp.p, m->ex.p, 1, st)) ||
+      (rc = enqueue_inverse_yx(m, pl, m->spec_c.p, m->spec_tmp.p, m->ey.p, 2, st)))
+    return rc;
+  m->pot_valid = energy;
+  if (energy) return enqueue_inverse_yx(m, pl, m->spec_c.p, m->spec_c.p, m->pot.p, 0, st);
+  return PDX_OK;
 }
 
 extern "C" int pdx_mesh_gather(pdx_mesh_t m, const float *x, const float *y, const float *z, const float *q,
@@ -1216,6 +1487,21 @@ extern "C" int pdx_mesh_gather(pdx_mesh_t m, const float *x, const float *y, con
   if (!energy) Vglobal = nullptr;
   hipStream_t st = as_stream(stream);
   auto blocks_of = [&](int P) { return (int)(((int64_t)N * lanes_for(P) + TPB - 1) / TPB); };
+  if (m->spec.influence == PDX_INFLUENCE_P3M_IK) {
+    if (energy && !m->pot_valid)
+      return fail(PDX_ERR_ARGUMENT, "ik gather with energies needs a solve with PDX_ENERGY first");
+    switch (m->P) {
+      PDX_GATHER_IK_CASE(3)
+      PDX_GATHER_IK_CASE(4)
+      PDX_GATHER_IK_CASE(5)
+      PDX_GATHER_IK_CASE(6)
+      PDX_GATHER_IK_CASE(7)
+      PDX_GATHER_IK_CASE(8)
+      default: return fail(PDX_ERR_ARGUMENT, "unsupported order");
+    }
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+  }
   switch (m->P) {
     PDX_GATHER_CASE(3)
     PDX_GATHER_CASE(4)
@@ -1285,7 +1571,7 @@ extern "C" int pdx_mesh_fft(pdx_mesh_t m, int32_t dir, const void *in, void *out
     PDX_HIP(hipMemcpyAsync(m->spec_c.p, in, sizeof(float2) * nmodes, hipMemcpyDeviceToDevice, st));
     launch_z<1, false>(m, pl, st);
     PDX_LAUNCH_CHECK();
-    return enqueue_inverse_yx(m, pl, (float *)out, st);
+    return enqueue_inverse_yx(m, pl, m->spec_c.p, m->spec_c.p, (float *)out, 0, st);
   }
   return fail(PDX_ERR_ARGUMENT, "dir must be -1 (R2C) or +1 (C2R)");
 }

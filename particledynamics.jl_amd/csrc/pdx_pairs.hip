@@ -35,6 +35,13 @@ struct PairArgs {
   float cC, crc2, csx, ceshift, halfself;
   float pe[MAXC];
   float pf[MAXC];
+  // Coulomb exceptions (k_pair with COUL): CSR over particles, scale per entry
+  const int32_t *exo, *exj;
+  const float *exs;
+  // FENE bonds (k_pair_acc only)
+  const int32_t *boff, *bpart;
+  int32_t *binvalid;
+  float bk, bR02;
 };
 
 // Horner with a compile-time length: the coefficients are uniform kernel arguments held in
@@ -182,6 +189,28 @@ __global__ __launch_bounds__(TPB) void k_pair(const PairArgs a) {
         }
       }
     }
+    if (COUL && a.exo && qi != 0.0f) {
+      // exceptions: + C q_i q_j (s - 1) / r at the minimum image, no cutoff, no potential shift
+      for (int p = a.exo[i] + lane; p < a.exo[i + 1]; p += G) {
+        const float4 pj = a.packed[a.exj[p]];
+        const float sc = a.cC * qi * pj.w * (a.exs[p] - 1.0f);
+        const float dx = periodic_delta(pi.x, pj.x, hx, Lx);
+        const float dy = periodic_delta(pi.y, pj.y, hy, Ly);
+        const float dz = periodic_delta(pi.z, pj.z, hz, Lz);
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float invr = __builtin_amdgcn_rsqf(r2);
+        const float f = sc * invr * invr * invr;
+        ax = fmaf(f, dx, ax);
+        ay = fmaf(f, dy, ay);
+        az = fmaf(f, dz, az);
+        if (ENERGY) {
+          e += sc * invr + (r2 < a.crc2 ? a.cC * qi * pj.w * a.ceshift : 0.0f);
+          const float fxij = f * dx, fyij = f * dy, fzij = f * dz;
+          v[0] = fmaf(dx, fxij, v[0]); v[1] = fmaf(dy, fyij, v[1]); v[2] = fmaf(dz, fzij, v[2]);
+          v[3] = fmaf(dx, fyij, v[3]); v[4] = fmaf(dx, fzij, v[4]); v[5] = fmaf(dy, fzij, v[5]);
+        }
+      }
+    }
   }
   if (G > 1) {
     ax = wave_sum_width(ax, G);
@@ -267,6 +296,184 @@ __global__ __launch_bounds__(64 * S) void k_pair_split(const PairArgs a) {
     for (int c = 0; c < 6; ++c) v[c] = sum[4 + c];
   }
   pair_epilogue<COUL, ENERGY>(a, active && w == 0, i, qi, sum[0], sum[1], sum[2], ENERGY ? sum[3] : 0.f, v);
+}
+
+// ---- per-particle sums in a chosen precision (the bead-spring precision study). The pair
+// arithmetic is the FP32 arithmetic of k_pair; only the accumulation of the row sums changes:
+// FP32 (fma, as k_pair), 64-bit fixed point at 2^32 (exact, so independent of the order and of
+// the lane split), or FP64 (exact products, rounded sums).
+template <int ACC> struct Sum;
+template <> struct Sum<PDX_ACC_FP32> {
+  float v = 0.f;
+  __device__ __forceinline__ void add_prod(float a, float b) { v = fmaf(a, b, v); }
+  __device__ __forceinline__ void add(float a) { v += a; }
+  __device__ __forceinline__ double value() const { return (double)v; }
+  template <int G> __device__ __forceinline__ void combine() {
+#pragma unroll
+    for (int o = G >> 1; o >= 1; o >>= 1) v += __shfl_xor(v, o, G);
+  }
+};
+template <> struct Sum<PDX_ACC_FIXED64> {
+  long long v = 0;
+  __device__ __forceinline__ void add_prod(float a, float b) { v += __float2ll_rn(a * b * 4294967296.0f); }
+  __device__ __forceinline__ void add(float a) { v += __float2ll_rn(a * 4294967296.0f); }
+  __device__ __forceinline__ double value() const { return (double)v * 2.3283064365386962890625e-10; }
+  template <int G> __device__ __forceinline__ void combine() {
+#pragma unroll
+    for (int o = G >> 1; o >= 1; o >>= 1) v += __shfl_xor(v, o, G);
+  }
+};
+template <> struct Sum<PDX_ACC_FP64> {
+  double v = 0.0;
+  __device__ __forceinline__ void add_prod(float a, float b) { v += (double)a * (double)b; }
+  __device__ __forceinline__ void add(float a) { v += (double)a; }
+  __device__ __forceinline__ double value() const { return v; }
+  template <int G> __device__ __forceinline__ void combine() {
+#pragma unroll
+    for (int o = G >> 1; o >= 1; o >>= 1) v += __shfl_xor(v, o, G);
+  }
+};
+
+template <int MODEL, bool ENERGY, int G, int ACC>
+__global__ __launch_bounds__(TPB) void k_pair_acc(const PairArgs a, int bonded) {
+  const int64_t *s = a.state;
+  const bool valid = s[PDX_DN_VALID] != 0 && s[PDX_DN_ERROR] == 0;  // uniform
+  if (!valid) return;
+  const int gid = blockIdx.x * TPB + threadIdx.x;
+  const int i = gid / G;
+  const int lane = gid % G;
+  const int N = a.N;
+  const bool active = i < N;
+  const float Lx = a.geom[0], Ly = a.geom[1], Lz = a.geom[2];
+  const float hx = 0.5f * Lx, hy = 0.5f * Ly, hz = 0.5f * Lz;
+  Sum<ACC> ax, ay, az, e, v[6];
+  const float none[1] = {0.f};
+  if (active) {
+    const float4 pi = a.packed[i];
+    const int n = a.counts[i];
+    const int32_t *row = a.nbr + i;
+    constexpr int U = 4;
+    for (int t0 = lane; t0 < n; t0 += U * G) {
+      int jj[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int t = t0 + u * G;
+        jj[u] = t < n ? row[(size_t)t * N] - 1 : i;
+      }
+      float4 pj[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) pj[u] = a.packed[jj[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const float dx = periodic_delta(pi.x, pj[u].x, hx, Lx);
+        const float dy = periodic_delta(pi.y, pj[u].y, hy, Ly);
+        const float dz = periodic_delta(pi.z, pj[u].z, hz, Lz);
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        float f, uu;
+        pair_terms<MODEL, false, ENERGY, 1>(a, none, none, 0.f, 0.f, r2, f, uu);
+        ax.add_prod(f, dx);
+        ay.add_prod(f, dy);
+        az.add_prod(f, dz);
+        if (ENERGY) {
+          e.add(uu);
+          const float fxij = f * dx, fyij = f * dy, fzij = f * dz;
+          v[0].add_prod(dx, fxij); v[1].add_prod(dy, fyij); v[2].add_prod(dz, fzij);
+          v[3].add_prod(dx, fyij); v[4].add_prod(dx, fzij); v[5].add_prod(dy, fzij);
+        }
+      }
+    }
+    if (bonded) {
+      const int b0 = a.boff[i], nb = a.boff[i + 1] - b0;
+      const float R02 = a.bR02;
+      for (int t = lane; t < nb; t += G) {
+        const float4 pj = a.packed[a.bpart[b0 + t]];
+        const float dx = periodic_delta(pi.x, pj.x, hx, Lx);
+        const float dy = periodic_delta(pi.y, pj.y, hy, Ly);
+        const float dz = periodic_delta(pi.z, pj.z, hz, Lz);
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        if (!(r2 < R02)) {  // out of the FENE domain (or not finite): latch, add nothing
+          atomicMin(a.binvalid, i);
+          continue;
+        }
+        // subtraction before division keeps near-limit extensions; log1p for short bonds
+        // (_fene_denom, _fene_log, src/bonded/FENEBondCUDA.jl)
+        const float denom = (R02 - r2) / R02;
+        const float f = -a.bk / denom;
+        ax.add_prod(f, dx);
+        ay.add_prod(f, dy);
+        az.add_prod(f, dz);
+        if (ENERGY) {
+          const float lg = r2 < 0.5f * R02 ? log1pf(-r2 / R02) : logf(denom);
+          e.add(-0.5f * a.bk * R02 * lg);
+          const float fxij = f * dx, fyij = f * dy, fzij = f * dz;
+          v[0].add_prod(dx, fxij); v[1].add_prod(dy, fyij); v[2].add_prod(dz, fzij);
+          v[3].add_prod(dx, fyij); v[4].add_prod(dx, fzij); v[5].add_prod(dy, fzij);
+        }
+      }
+    }
+  }
+  if (G > 1) {
+    ax.template combine<G>();
+    ay.template combine<G>();
+    az.template combine<G>();
+    if (ENERGY) {
+      e.template combine<G>();
+#pragma unroll
+      for (int c = 0; c < 6; ++c) v[c].template combine<G>();
+    }
+  }
+  // every pair and bond was visited from both ends: half energies and half virials
+  const bool owner = active && lane == 0;
+  double ei = 0.0, w[6] = {0, 0, 0, 0, 0, 0};
+  if (owner) {
+    const float fx = (float)ax.value(), fy = (float)ay.value(), fz = (float)az.value();
+    if (a.add) {
+      a.fx[i] += fx; a.fy[i] += fy; a.fz[i] += fz;
+    } else {
+      a.fx[i] = fx; a.fy[i] = fy; a.fz[i] = fz;
+    }
+    if (ENERGY) {
+      ei = 0.5 * e.value();
+#pragma unroll
+      for (int c = 0; c < 6; ++c) w[c] = 0.5 * v[c].value();
+      if (a.Epot) a.Epot[i] = a.add ? a.Epot[i] + (float)ei : (float)ei;
+      if (a.V) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          float *p = a.V + (size_t)c * a.N + i;
+          *p = a.add ? *p + (float)w[c] : (float)w[c];
+        }
+      }
+    }
+  }
+  if (ENERGY && a.totals) {
+    const double d[7] = {ei, w[0], w[1], w[2], w[3], w[4], w[5]};
+    grid_sum_finalize<7>(d, a.partial, a.ticket, a.totals, a.add != 0);
+  }
+}
+
+template <int MODEL, bool ENERGY, int ACC>
+void launch_acc_g(int G, dim3 grid, hipStream_t st, const PairArgs &a, int bonded) {
+  switch (G) {
+    case 1: hipLaunchKernelGGL((k_pair_acc<MODEL, ENERGY, 1, ACC>), grid, dim3(TPB), 0, st, a, bonded); break;
+    case 2: hipLaunchKernelGGL((k_pair_acc<MODEL, ENERGY, 2, ACC>), grid, dim3(TPB), 0, st, a, bonded); break;
+    case 4: hipLaunchKernelGGL((k_pair_acc<MODEL, ENERGY, 4, ACC>), grid, dim3(TPB), 0, st, a, bonded); break;
+    default: hipLaunchKernelGGL((k_pair_acc<MODEL, ENERGY, 8, ACC>), grid, dim3(TPB), 0, st, a, bonded); break;
+  }
+}
+
+template <int MODEL>
+void launch_acc(int acc, bool energy, int G, dim3 grid, hipStream_t st, const PairArgs &a, int bonded) {
+  if (acc == PDX_ACC_FIXED64) {
+    if (energy) launch_acc_g<MODEL, true, PDX_ACC_FIXED64>(G, grid, st, a, bonded);
+    else launch_acc_g<MODEL, false, PDX_ACC_FIXED64>(G, grid, st, a, bonded);
+  } else if (acc == PDX_ACC_FP64) {
+    if (energy) launch_acc_g<MODEL, true, PDX_ACC_FP64>(G, grid, st, a, bonded);
+    else launch_acc_g<MODEL, false, PDX_ACC_FP64>(G, grid, st, a, bonded);
+  } else {
+    if (energy) launch_acc_g<MODEL, true, PDX_ACC_FP32>(G, grid, st, a, bonded);
+    else launch_acc_g<MODEL, false, PDX_ACC_FP32>(G, grid, st, a, bonded);
+  }
 }
 
 template <int MODEL, bool COUL, bool ENERGY, int NP>
@@ -496,6 +703,14 @@ extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, 
   const int N = nl->N;
   // lanes > 0: G lanes of a wave per particle; lanes < 0: the split layout with -lanes waves
   // per block of 64 particles; 0: automatic
+  const bool acc_path = p->accumulate != PDX_ACC_FP32 || p->bonds != nullptr;
+  if (acc_path) {
+    if (p->spec.coulomb) return fail(PDX_ERR_ARGUMENT, "accumulation modes and bonds are not available with Coulomb");
+    if (p->spec.model != PDX_PAIR_LJ && p->spec.model != PDX_PAIR_WCA && p->spec.model != PDX_PAIR_SOFT)
+      return fail(PDX_ERR_ARGUMENT, "accumulation modes and bonds need the LJ, WCA or soft model");
+    if (p->lanes < 0) return fail(PDX_ERR_ARGUMENT, "accumulation modes and bonds use the lanes layout (lanes >= 0)");
+    if (p->bonds && p->bonds->N != nl->N) return fail(PDX_ERR_DIMENSION, "bonds and neighbor list particle counts differ");
+  }
   int G = p->lanes;
   if (G == 0) {
     // about 2^17 threads: every CU busy while a wave's row slots stay in few cache lines
@@ -504,6 +719,7 @@ extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, 
     G = 1;
     while (G * 2 <= want && G < 64) G <<= 1;
   }
+  if (acc_path) G = std::min(G, 8);
   const int blocks = G < 0 ? (N + WAVE - 1) / WAVE : (int)(((int64_t)N * G + TPB - 1) / TPB);
   if (energy && totals) {
     if ((size_t)blocks * 7 > p->partial.n) {
@@ -531,6 +747,28 @@ extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, 
   const bool coul = p->spec.coulomb != 0;
   const int np = std::max(p->npe, p->npf);
   hipStream_t st = as_stream(stream);
+  if (p->nexc > 0) {
+    if (p->exc_n != N) return fail(PDX_ERR_DIMENSION, "exceptions and neighbor list particle counts differ");
+    if (G < 0) return fail(PDX_ERR_ARGUMENT, "exceptions use the lanes layout (lanes >= 0)");
+    a.exo = p->exc_off.p;
+    a.exj = p->exc_j.p;
+    a.exs = p->exc_s.p;
+  }
+  if (acc_path) {
+    const int bonded = p->bonds != nullptr;
+    if (bonded) {
+      a.boff = p->bonds->offsets.p;
+      a.bpart = p->bonds->partners.p;
+      a.binvalid = p->bonds->invalid.p;
+      a.bk = (float)p->bonds->k;
+      a.bR02 = (float)(p->bonds->R0 * p->bonds->R0);
+    }
+    if (p->spec.model == PDX_PAIR_LJ) launch_acc<PDX_PAIR_LJ>(p->accumulate, energy, G, dim3(blocks), st, a, bonded);
+    else if (p->spec.model == PDX_PAIR_WCA) launch_acc<PDX_PAIR_WCA>(p->accumulate, energy, G, dim3(blocks), st, a, bonded);
+    else launch_acc<PDX_PAIR_SOFT>(p->accumulate, energy, G, dim3(blocks), st, a, bonded);
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+  }
   switch (p->spec.model) {
     case PDX_PAIR_LJ: launch_model<PDX_PAIR_LJ>(coul, energy, np, G, dim3(blocks), st, a); break;
     case PDX_PAIR_WCA: launch_model<PDX_PAIR_WCA>(coul, energy, np, G, dim3(blocks), st, a); break;
@@ -538,5 +776,156 @@ extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, 
     default: launch_model<PDX_PAIR_NONE>(coul, energy, np, G, dim3(blocks), st, a); break;
   }
   PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
+extern "C" int pdx_pair_set_accumulation(pdx_pair_t p, int32_t mode) {
+  if (!p) return fail(PDX_ERR_ARGUMENT, "null pair model");
+  if (mode != PDX_ACC_FP32 && mode != PDX_ACC_FIXED64 && mode != PDX_ACC_FP64)
+    return fail(PDX_ERR_ARGUMENT, "accumulation mode must be PDX_ACC_FP32, PDX_ACC_FIXED64 or PDX_ACC_FP64");
+  p->accumulate = mode;
+  return PDX_OK;
+}
+
+extern "C" int pdx_bonds_create(int32_t n, int32_t nbond, const int32_t *bi, const int32_t *bj, double k, double R0,
+                                pdx_bonds_t *out) {
+  if (!out) return fail(PDX_ERR_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (n < 1 || nbond < 0) return fail(PDX_ERR_ARGUMENT, "particle and bond counts must be positive");
+  if (nbond > 0 && (!bi || !bj)) return fail(PDX_ERR_ARGUMENT, "null bond arrays");
+  if (!(std::isfinite(k) && k >= 0)) return fail(PDX_ERR_ARGUMENT, "FENE k must be finite and nonnegative");
+  if (!(std::isfinite(R0) && R0 > 0 && std::isfinite(R0 * R0)))
+    return fail(PDX_ERR_ARGUMENT, "FENE R0 and its square must be finite and positive");
+  std::vector<int32_t> cnt(n + 1, 0);
+  for (int b = 0; b < nbond; ++b) {
+    if (bi[b] < 0 || bi[b] >= n || bj[b] < 0 || bj[b] >= n || bi[b] == bj[b])
+      return fail(PDX_ERR_ARGUMENT, "bond " + std::to_string(b) + " names an invalid particle pair");
+    ++cnt[bi[b] + 1];
+    ++cnt[bj[b] + 1];
+  }
+  for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+  std::vector<int32_t> part(2 * (size_t)nbond), fill(cnt.begin(), cnt.end() - 1);
+  for (int b = 0; b < nbond; ++b) {
+    part[fill[bi[b]]++] = bj[b];
+    part[fill[bj[b]]++] = bi[b];
+  }
+  for (int i = 0; i < n; ++i) std::sort(part.begin() + cnt[i], part.begin() + cnt[i + 1]);  // fixed order
+  auto *b = new pdx_bonds_s();
+  b->N = n;
+  b->nbond = nbond;
+  b->k = k;
+  b->R0 = R0;
+  int rc;
+  if ((rc = b->offsets.alloc(n + 1)) || (rc = b->partners.alloc(std::max<size_t>(1, part.size()))) ||
+      (rc = b->invalid.alloc(1))) {
+    delete b;
+    return rc;
+  }
+  const int32_t none = INT32_MAX;
+  if (hipMemcpy(b->offsets.p, cnt.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+      (nbond > 0 && hipMemcpy(b->partners.p, part.data(), sizeof(int32_t) * part.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+      hipMemcpy(b->invalid.p, &none, sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+    delete b;
+    return fail(PDX_ERR_DEVICE, "bond table upload failed");
+  }
+  *out = b;
+  return PDX_OK;
+}
+
+extern "C" int pdx_bonds_destroy(pdx_bonds_t b) {
+  delete b;
+  return PDX_OK;
+}
+
+extern "C" int pdx_pair_set_bonds(pdx_pair_t p, pdx_bonds_t b) {
+  if (!p) return fail(PDX_ERR_ARGUMENT, "null pair model");
+  p->bonds = b;
+  return PDX_OK;
+}
+
+extern "C" int pdx_bonds_status(pdx_bonds_t b, int32_t *first_invalid, int32_t reset, pdx_stream_t stream) {
+  if (!b || !first_invalid) return fail(PDX_ERR_ARGUMENT, "null argument");
+  hipStream_t st = as_stream(stream);
+  int32_t v = INT32_MAX;
+  PDX_HIP(hipMemcpyAsync(&v, b->invalid.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PDX_HIP(hipStreamSynchronize(st));
+  if (reset && v != INT32_MAX) {
+    const int32_t none = INT32_MAX;
+    PDX_HIP(hipMemcpyAsync(b->invalid.p, &none, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    PDX_HIP(hipStreamSynchronize(st));
+  }
+  *first_invalid = v == INT32_MAX ? -1 : v;
+  if (v != INT32_MAX)
+    return fail(PDX_ERR_COORDINATE, "FENE bond incident on particle " + std::to_string(v + 1) +
+                                        " has nonfinite length or r >= R0=" + std::to_string(b->R0) +
+                                        ". Reduce the timestep or correct the configuration; forces are not clamped.");
+  return PDX_OK;
+}
+
+extern "C" int pdx_pair_set_exceptions(pdx_pair_t p, int32_t n, int32_t nexc, const int32_t *ei, const int32_t *ej,
+                                       const double *scale) {
+  if (!p) return fail(PDX_ERR_ARGUMENT, "null pair model");
+  if (nexc < 0 || n < 1) return fail(PDX_ERR_ARGUMENT, "particle and exception counts must be positive");
+  if (nexc > 0 && (!ei || !ej || !scale)) return fail(PDX_ERR_ARGUMENT, "null exception arrays");
+  if (nexc > 0 && !p->spec.coulomb) return fail(PDX_ERR_ARGUMENT, "exceptions need the real-space Coulomb term");
+  // validate and deduplicate (normalize_exceptions, Spec.jl:200-224); key (min, max)
+  std::vector<std::pair<std::pair<int32_t, int32_t>, double>> items;
+  items.reserve(nexc);
+  for (int k = 0; k < nexc; ++k) {
+    const int32_t a = ei[k], b = ej[k];
+    const double s = scale[k];
+    if (a < 0 || a >= n || b < 0 || b >= n)
+      return fail(PDX_ERR_ARGUMENT, "Exception pair (" + std::to_string(a + 1) + ", " + std::to_string(b + 1) +
+                                        ") is out of range for N=" + std::to_string(n) + ".");
+    if (a == b)
+      return fail(PDX_ERR_ARGUMENT, "Exception pair (" + std::to_string(a + 1) + ", " + std::to_string(b + 1) +
+                                        ") refers to a single particle.");
+    if (!(std::isfinite(s) && s >= 0))
+      return fail(PDX_ERR_ARGUMENT, "Exception scale must be finite and nonnegative.");
+    items.push_back({{std::min(a, b), std::max(a, b)}, s});
+  }
+  std::sort(items.begin(), items.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+  std::vector<std::pair<std::pair<int32_t, int32_t>, double>> uniq;
+  for (const auto &it : items) {
+    if (!uniq.empty() && uniq.back().first == it.first) {
+      if (uniq.back().second != it.second)
+        return fail(PDX_ERR_ARGUMENT, "Exception pair (" + std::to_string(it.first.first + 1) + ", " +
+                                          std::to_string(it.first.second + 1) + ") is listed with conflicting scales.");
+      continue;
+    }
+    uniq.push_back(it);
+  }
+  std::vector<int32_t> off(n + 1, 0);
+  size_t m = 0;
+  for (const auto &it : uniq) {
+    if (it.second == 1.0) continue;
+    ++off[it.first.first + 1];
+    ++off[it.first.second + 1];
+    m += 2;
+  }
+  for (int i = 0; i < n; ++i) off[i + 1] += off[i];
+  std::vector<int32_t> jj(std::max<size_t>(m, 1)), fill(off.begin(), off.end() - 1);
+  std::vector<float> ss(std::max<size_t>(m, 1));
+  for (const auto &it : uniq) {  // sorted by (i, j): each particle's list is in id order
+    if (it.second == 1.0) continue;
+    const int32_t a = it.first.first, b = it.first.second;
+    jj[fill[a]] = b; ss[fill[a]++] = (float)it.second;
+    jj[fill[b]] = a; ss[fill[b]++] = (float)it.second;
+  }
+  for (int i = 0; i < n; ++i) {  // partners of i in id order (the b-side entries arrive out of order)
+    std::vector<std::pair<int32_t, float>> row;
+    for (int t = off[i]; t < off[i + 1]; ++t) row.push_back({jj[t], ss[t]});
+    std::sort(row.begin(), row.end());
+    for (int t = off[i]; t < off[i + 1]; ++t) { jj[t] = row[t - off[i]].first; ss[t] = row[t - off[i]].second; }
+  }
+  p->nexc = 0;
+  p->exc_n = n;
+  if (m == 0) return PDX_OK;
+  int rc;
+  if ((rc = p->exc_off.alloc(n + 1)) || (rc = p->exc_j.alloc(m)) || (rc = p->exc_s.alloc(m))) return rc;
+  PDX_HIP(hipMemcpy(p->exc_off.p, off.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
+  PDX_HIP(hipMemcpy(p->exc_j.p, jj.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
+  PDX_HIP(hipMemcpy(p->exc_s.p, ss.data(), sizeof(float) * m, hipMemcpyHostToDevice));
+  p->nexc = (int32_t)(m / 2);
   return PDX_OK;
 }

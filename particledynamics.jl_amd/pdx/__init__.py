@@ -6,8 +6,8 @@ when the library is missing. Device arrays are torch tensors on ``cuda`` (HIP).
 from ._lib import (ArgumentError, DeviceError, DimensionMismatch, NeighborCapacityError, check, last_error,
                    lib, LIB_PATH)
 from .neighbors import DeviceNeighborWorkspace, CellList, build_neighbors_dense
-from .forces import LennardJones, WCA, SoftRepulsive, RealSpaceCoulomb, PairForce
-from .electrostatics import (ElectrostaticsSpec, PME, ElectrostaticsSolver, alpha_from_tolerance, fft_friendly,
+from .forces import LennardJones, WCA, SoftRepulsive, RealSpaceCoulomb, PairForce, FENEBonds, DomainError
+from .electrostatics import (ElectrostaticsSpec, PME, PPPM, ElectrostaticsSolver, alpha_from_tolerance, fft_friendly,
                              ewald_fp64)
 from .engine import ForceEngine, default_capacity
 from .replicas import ReplicaBatch
@@ -15,5 +15,5 @@ from . import systems
 
 __all__ = ["ArgumentError", "DeviceError", "DimensionMismatch", "NeighborCapacityError", "check", "last_error",
            "lib", "LIB_PATH", "DeviceNeighborWorkspace", "CellList", "build_neighbors_dense", "LennardJones", "WCA",
-           "SoftRepulsive", "RealSpaceCoulomb", "PairForce", "ElectrostaticsSpec", "PME", "ElectrostaticsSolver",
+           "SoftRepulsive", "RealSpaceCoulomb", "PairForce", "FENEBonds", "DomainError", "ElectrostaticsSpec", "PME", "PPPM", "ElectrostaticsSolver",
            "alpha_from_tolerance", "fft_friendly", "ewald_fp64", "ForceEngine", "default_capacity", "ReplicaBatch", "systems"]

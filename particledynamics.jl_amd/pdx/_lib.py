@@ -27,6 +27,8 @@ PDX_PAIR_NONE, PDX_PAIR_LJ, PDX_PAIR_WCA, PDX_PAIR_SOFT = 0, 1, 2, 3
 PDX_ENERGY = 1
 PDX_ADD = 2
 PDX_INFLUENCE_SPME = 1
+PDX_INFLUENCE_P3M_AD = 2
+PDX_INFLUENCE_P3M_IK = 3
 
 
 class ArgumentError(ValueError):
@@ -70,7 +72,7 @@ class pdx_mesh_spec(C.Structure):
                 ("real_cutoff", C.c_double), ("alpha", C.c_double), ("real_space_tolerance", C.c_double),
                 ("order", C.c_int32), ("mesh", C.c_int32 * 3), ("mesh_spacing", C.c_double),
                 ("influence", C.c_int32), ("net_charge", C.c_int32), ("net_charge_tolerance", C.c_double),
-                ("real_space_modifier", C.c_int32)]
+                ("real_space_modifier", C.c_int32), ("alias_order", C.c_int32)]
 
 
 def _load():

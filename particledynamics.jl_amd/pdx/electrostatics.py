@@ -57,6 +57,7 @@ class ElectrostaticsSpec:
     real_space_modifier: str = "none"
     net_charge_tolerance: float = 1e-8
     influence: str = "spme"
+    alias_order: int = 2
 
     def __post_init__(self):
         if self.mesh is None and self.mesh_spacing is None:
@@ -67,8 +68,10 @@ class ElectrostaticsSpec:
             raise L.ArgumentError("net_charge must be :error or :background")
         if self.real_space_modifier not in ("none", "potential_shift"):
             raise L.ArgumentError("real_space_modifier must be :none or :potential_shift")
-        if self.influence != "spme":
-            raise L.ArgumentError("Only the SPME influence function is implemented.")
+        if self.influence not in ("spme", "p3m_ad", "p3m_ik"):
+            raise L.ArgumentError(f"Unknown influence kind {self.influence}.")
+        if not 0 <= int(self.alias_order) <= 6:
+            raise L.ArgumentError("alias_order must be between 0 and 6.")
 
     @property
     def C(self):
@@ -92,11 +95,27 @@ class ElectrostaticsSpec:
         else:
             s.mesh[0] = 0
         s.mesh_spacing = float(self.mesh_spacing or 0.0)
-        s.influence = L.PDX_INFLUENCE_SPME
+        s.influence = {"spme": L.PDX_INFLUENCE_SPME, "p3m_ad": L.PDX_INFLUENCE_P3M_AD,
+                       "p3m_ik": L.PDX_INFLUENCE_P3M_IK}[self.influence]
+        s.alias_order = int(self.alias_order)
         s.net_charge = 1 if self.net_charge == "background" else 0
         s.net_charge_tolerance = float(self.net_charge_tolerance)
         s.real_space_modifier = 1 if self.real_space_modifier == "potential_shift" else 0
         return s
+
+
+def PPPM(*, coulomb_prefactor, real_cutoff, alpha=None, real_space_tolerance=None, mesh=None, mesh_spacing=None,
+        order=5, differentiation="ad", alias_order=2, relative_permittivity=1.0, net_charge="error",
+        real_space_modifier="none"):
+    """Workflow descriptor of P3M with the Hockney-Eastwood optimal influence function for
+    analytic ("ad") or ik ("ik") differentiation (PPPM, src/workflow/Electrostatics.jl:64-100)."""
+    if differentiation not in ("ad", "ik"):
+        raise L.ArgumentError("differentiation must be :ad or :ik")
+    return ElectrostaticsSpec(coulomb_prefactor=coulomb_prefactor, real_cutoff=real_cutoff, alpha=alpha,
+                              real_space_tolerance=real_space_tolerance, order=order, mesh=mesh,
+                              mesh_spacing=mesh_spacing, relative_permittivity=relative_permittivity,
+                              net_charge=net_charge, real_space_modifier=real_space_modifier,
+                              influence="p3m_" + differentiation, alias_order=alias_order)
 
 
 def PME(*, coulomb_prefactor, real_cutoff, alpha=None, real_space_tolerance=None, mesh=None, mesh_spacing=None,

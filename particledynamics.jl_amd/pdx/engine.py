@@ -20,7 +20,8 @@ def default_capacity(N, box, rlist):
 
 class ForceEngine:
     def __init__(self, N, box, pair_model=None, electrostatics=None, charges=None, skin=0.4, cap=None,
-                 interval=20, lanes=0, diagnostics=False, spread="cells", correction=None):
+                 interval=20, lanes=0, diagnostics=False, spread="cells", correction=None, accumulation="fp32",
+                 bonds=None, exceptions=()):
         import torch
         self.N = int(N)
         self.box = tuple(float(b) for b in box)
@@ -34,7 +35,12 @@ class ForceEngine:
         self.skin = float(skin)
         cap = cap if cap is not None else default_capacity(N, box, rc + skin)
         self.nl = DeviceNeighborWorkspace(N, box, rc, skin, cap, interval=interval, diagnostics=diagnostics)
-        self.pairs = PairForce(pair_model, coulomb=self.solver.real_space() if self.solver else None, lanes=lanes)
+        self.pairs = PairForce(pair_model, coulomb=self.solver.real_space() if self.solver else None, lanes=lanes,
+                               accumulation=accumulation, bonds=bonds)
+        if exceptions:
+            if self.solver is None:
+                raise L.ArgumentError("electrostatic exceptions need a mesh")
+            self.pairs.set_exceptions(self.N, exceptions)
         self.pair_totals = torch.zeros(7, dtype=torch.float64, device="cuda")
         self.Vglobal = torch.zeros(6, dtype=torch.float32, device="cuda")
         # the mesh branch runs on its own stream (default priority), forked after the list

@@ -55,8 +55,71 @@ class RealSpaceCoulomb:
     shift: bool = False
 
 
+ACCUMULATION = {"fp32": 0, "fixed64": 1, "fp64": 2}
+
+
+def _sig(lib):
+    vp, i32, f64 = C.c_void_p, C.c_int32, C.c_double
+    P = C.POINTER
+    for name, args in {
+        "pdx_pair_set_accumulation": [vp, i32],
+        "pdx_bonds_create": [i32, i32, P(C.c_int32), P(C.c_int32), f64, f64, P(vp)],
+        "pdx_bonds_destroy": [vp],
+        "pdx_pair_set_bonds": [vp, vp],
+        "pdx_bonds_status": [vp, P(C.c_int32), i32, vp],
+        "pdx_pair_set_exceptions": [vp, i32, i32, P(C.c_int32), P(C.c_int32), P(C.c_double)],
+    }.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int
+
+
+_sig(L.lib)
+
+
+class DomainError(ValueError):
+    """A FENE bond reached r >= R0 (the reference's ``DomainError``)."""
+
+
+class FENEBonds:
+    """FENE bonds u = -k R0^2/2 ln(1 - r^2/R0^2) (``FENEParams``, src/Definitions.jl:93-101;
+    kernels in src/bonded/FENEBondCUDA.jl). ``pairs``: (nb, 2) 0-based particle ids."""
+
+    def __init__(self, n, pairs, k=30.0, R0=1.5):
+        import numpy as np
+        b = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        bi, bj = np.ascontiguousarray(b[:, 0]), np.ascontiguousarray(b[:, 1])
+        h = C.c_void_p()
+        P = C.POINTER(C.c_int32)
+        L.check(L.lib.pdx_bonds_create(int(n), len(b), bi.ctypes.data_as(P), bj.ctypes.data_as(P), float(k),
+                                       float(R0), C.byref(h)))
+        self._h = h
+        self.n, self.nbond, self.k, self.R0 = int(n), len(b), float(k), float(R0)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            L.lib.pdx_bonds_destroy(h)
+            self._h = None
+
+    def status(self, reset=True, stream=None):
+        """Raise ``DomainError`` naming the first particle whose bond left the FENE domain
+        in the evaluations since the last reset; returns -1 otherwise."""
+        v = C.c_int32()
+        rc = L.lib.pdx_bonds_status(self._h, C.byref(v), 1 if reset else 0, L.stream_handle(stream))
+        if rc == L.PDX_ERR_COORDINATE:
+            raise DomainError(L.last_error())
+        L.check(rc)
+        return v.value
+
+
 class PairForce:
-    def __init__(self, model=None, coulomb: RealSpaceCoulomb | None = None, lanes: int = 0):
+    def __init__(self, model=None, coulomb: RealSpaceCoulomb | None = None, lanes: int = 0,
+                 accumulation: str = "fp32", bonds: FENEBonds | None = None):
         spec = L.pdx_pair_spec()
         if model is None:
             spec.model = L.PDX_PAIR_NONE
@@ -78,6 +141,40 @@ class PairForce:
         self._h = h
         self.model = model
         self.coulomb = coulomb
+        self.bonds = None
+        self.set_accumulation(accumulation)
+        if bonds is not None:
+            self.set_bonds(bonds)
+
+    def set_accumulation(self, mode: str):
+        """Precision of the per-particle sums: "fp32", "fixed64" (64-bit fixed point, exact)
+        or "fp64"; the pair arithmetic itself stays FP32."""
+        if mode not in ACCUMULATION:
+            raise L.ArgumentError("accumulation must be 'fp32', 'fixed64' or 'fp64'")
+        L.check(L.lib.pdx_pair_set_accumulation(self._h, ACCUMULATION[mode]))
+        self.accumulation = mode
+
+    def set_exceptions(self, n, exceptions):
+        """Electrostatic exceptions as (i, j) or (i, j, scale) with 1-based particle ids, the
+        reference's convention (normalize_exceptions, src/electrostatics/Spec.jl:200-224)."""
+        import numpy as np
+        ii, jj, ss = [], [], []
+        for item in exceptions:
+            if len(item) not in (2, 3):
+                raise L.ArgumentError("Each electrostatic exception must be (i, j) or (i, j, scale).")
+            ii.append(int(item[0]) - 1)
+            jj.append(int(item[1]) - 1)
+            ss.append(float(item[2]) if len(item) == 3 else 0.0)
+        a = np.ascontiguousarray(ii, dtype=np.int32)
+        b = np.ascontiguousarray(jj, dtype=np.int32)
+        c = np.ascontiguousarray(ss, dtype=np.float64)
+        L.check(L.lib.pdx_pair_set_exceptions(self._h, int(n), len(a), a.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              b.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              c.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def set_bonds(self, bonds: FENEBonds | None):
+        L.check(L.lib.pdx_pair_set_bonds(self._h, bonds.handle if bonds is not None else None))
+        self.bonds = bonds  # kept alive while attached
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -110,3 +110,35 @@ def replica_batch(nrep=512, seed=0, n_range=(256, 8192), density_range=(0.70, 0.
         c[c >= Lb / 2] -= Lb
     return dict(specs=specs, x=X[:, 0].copy(), y=X[:, 1].copy(), z=X[:, 2].copy(), vx=V[:, 0].copy(),
                 vy=V[:, 1].copy(), vz=V[:, 2].copy(), offsets=offsets, N=int(offsets[-1]))
+
+
+def kg_chains(nchains, length=100, density=0.85, bond=0.97, seed=0, max_cos=-0.66):
+    """Random-walk bead-spring chains (the Kremer-Grest melt's starting point): chain starts
+    uniform in the box, steps of length ``bond`` in uniform random directions, redrawn while
+    the angle to the previous step has cos < ``max_cos`` (so beads i and i+2 stay > 0.8
+    apart), box side (N / density)^(1/3). Beads of a chain are consecutive ids. Returns x, y,
+    z (FP32, centred), the bonds (nb, 2) and the box."""
+    rng = np.random.default_rng(seed)
+    N = nchains * length
+    L = (N / density) ** (1.0 / 3.0)
+    pos = np.empty((nchains, length, 3))
+    pos[:, 0] = rng.random((nchains, 3)) * L - L / 2
+    prev = None
+    for s in range(1, length):
+        v = rng.normal(size=(nchains, 3))
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        if prev is not None:
+            bad = (v * prev).sum(1) < max_cos
+            while bad.any():
+                w = rng.normal(size=(int(bad.sum()), 3))
+                v[bad] = w / np.linalg.norm(w, axis=1, keepdims=True)
+                bad = (v * prev).sum(1) < max_cos
+        pos[:, s] = pos[:, s - 1] + bond * v
+        prev = v
+    P = _wrap(pos.reshape(-1, 3), L).astype(np.float32)
+    for d in range(3):
+        c = P[:, d]
+        c[c >= np.float32(L / 2)] -= np.float32(L)
+    ids = np.arange(N).reshape(nchains, length)
+    bonds = np.stack([ids[:, :-1].reshape(-1), ids[:, 1:].reshape(-1)], 1).astype(np.int32)
+    return dict(x=P[:, 0].copy(), y=P[:, 1].copy(), z=P[:, 2].copy(), bonds=bonds, box=(L, L, L), N=N)

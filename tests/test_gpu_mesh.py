@@ -152,6 +152,82 @@ def test_spme_refined_against_ewald(cuda):
     assert abs(E - Eref) / abs(Eref) < 2e-5
 
 
+@pytest.mark.parametrize("diff", ["ad", "ik"])
+def test_p3m_against_ewald(cuda, diff):
+    """P3M with the optimal influence function for AD / ik differentiation on the reference's
+    120-charge case (test/test_electrostatics.jl:299-340), FP32 bounds."""
+    R, q = systems.random_cloud(120, L_ES, seed=3)
+    R32 = R.astype(np.float32).astype(np.float64)
+    q32 = q.astype(np.float32).astype(np.float64)
+    Eref, Fref, Wref = oracle.ewald(R32, q32, np.diag(L_ES), C_=1.7)
+    spec = pdx.PPPM(coulomb_prefactor=1.7, real_cutoff=4.0, alpha=0.9, mesh=(48, 48, 54), order=6,
+                    differentiation=diff)
+    E, F, W, Ep = _es_eval(R, q, L_ES, spec)
+    print(f"[p3m-{diff}] relrms={_relrms(F, Fref):.3e} dE={abs(E - Eref) / abs(Eref):.3e} "
+          f"dW={np.abs(W - Wref).max() / abs(np.trace(Wref)):.3e}", flush=True)
+    assert _relrms(F, Fref) < 1e-4
+    assert abs(E - Eref) / abs(Eref) < 1e-4
+    assert np.abs(W - Wref).max() / abs(np.trace(Wref)) < 1e-3
+    assert abs(Ep.sum() - E) / abs(E) < 1e-5
+
+
+def test_optimal_influence_on_a_coarse_mesh(cuda):
+    """On a coarse mesh the ik-optimal influence function gives a far lower force error than
+    SPME with the same B-spline assignment, and the AD-optimal one about the same (SPME and
+    P3M-AD are known to be close; Ballenegger, Cerda & Holm 2012)."""
+    R, q = systems.random_cloud(120, L_ES, seed=3)
+    R32 = R.astype(np.float32).astype(np.float64)
+    q32 = q.astype(np.float32).astype(np.float64)
+    _, Fref, _ = oracle.ewald(R32, q32, np.diag(L_ES), C_=1.7)
+    errs = {}
+    for name, spec in (
+            ("spme", pdx.PME(coulomb_prefactor=1.7, real_cutoff=4.0, alpha=0.9, mesh=(20, 20, 24), order=4)),
+            ("ad", pdx.PPPM(coulomb_prefactor=1.7, real_cutoff=4.0, alpha=0.9, mesh=(20, 20, 24), order=4)),
+            ("ik", pdx.PPPM(coulomb_prefactor=1.7, real_cutoff=4.0, alpha=0.9, mesh=(20, 20, 24), order=4,
+                            differentiation="ik"))):
+        _, F, _, _ = _es_eval(R, q, L_ES, spec)
+        errs[name] = _relrms(F, Fref)
+    print(f"[influence] coarse-mesh force errors {errs}", flush=True)
+    assert errs["ik"] < 0.5 * errs["spme"]
+    assert errs["ad"] < 1.25 * errs["spme"]
+
+
+def test_exceptions_against_ewald(cuda):
+    """Excluded and scaled pairs, one straddling the periodic boundary
+    (test/test_electrostatics.jl:380-393), FP32 bounds; normalisation errors."""
+    R, q = systems.random_cloud(100, L_ES, seed=13)
+    R[0] = (6.8, 0.0, 0.0)
+    R[1] = (-6.6, 0.3, 0.1)
+    exc = [(1, 2, 0.0), (5, 9, 0.5), (10, 11, 0.0), (10, 12, 0.8333)]
+    R32 = R.astype(np.float32).astype(np.float64)
+    q32 = q.astype(np.float32).astype(np.float64)
+    Er, Fr, Wr = oracle.ewald(R32, q32, np.diag(L_ES), C_=1.3,
+                              exceptions=[(i - 1, j - 1, s) for i, j, s in exc])
+    spec = pdx.PME(coulomb_prefactor=1.3, real_cutoff=4.0, alpha=0.9, mesh=(48, 48, 54), order=6)
+    import torch
+    N = len(q)
+    eng = pdx.ForceEngine(N, L_ES, None, electrostatics=spec, charges=q, skin=0.3, exceptions=exc,
+                          cap=max(64, pdx.default_capacity(N, L_ES, 4.4)))
+    X, Y, Z, Q = dev(R[:, 0]), dev(R[:, 1]), dev(R[:, 2]), dev(q)
+    fx, fy, fz, E = (torch.empty(N, device="cuda") for _ in range(4))
+    eng.evaluate(X, Y, Z, Q, fx, fy, fz, E, None, energy=True)
+    tot = eng.totals()
+    F = torch.stack([fx, fy, fz], 1).double().cpu().numpy()
+    W = np.array(tot["virial"])
+    Wm = np.array([[W[0], W[3], W[4]], [W[3], W[1], W[5]], [W[4], W[5], W[2]]])
+    print(f"[exceptions] relrms={_relrms(F, Fr):.3e} dE={abs(tot['energy'] - Er) / abs(Er):.3e} "
+          f"dW={np.abs(Wm - Wr).max() / abs(np.trace(Wr)):.3e}", flush=True)
+    assert _relrms(F, Fr) < 1e-4
+    assert abs(tot["energy"] - Er) / abs(Er) < 1e-4
+    assert np.abs(Wm - Wr).max() / abs(np.trace(Wr)) < 1e-3
+    p = pdx.PairForce(None, coulomb=pdx.RealSpaceCoulomb(C=1.0, cutoff=4.0, alpha=0.9))
+    with pytest.raises(pdx.ArgumentError, match="conflicting"):
+        p.set_exceptions(10, [(1, 2, 0.0), (2, 1, 0.5)])
+    with pytest.raises(pdx.ArgumentError, match="single particle"):
+        p.set_exceptions(10, [(1, 1, 0.0)])
+    p.set_exceptions(10, [(1, 2, 1.0)])  # scale 1 is an ordinary pair: nothing kept
+
+
 def test_madelung_rock_salt_full_size(cuda):
     """64 000-ion rock salt, unjittered: E = -(N/2) M C / a, forces vanish, W = E/3 per diagonal."""
     s = systems.rock_salt(40, 2.82, 0.0)

@@ -281,3 +281,69 @@ def test_vv_oracle_reversible_and_conserving():
     assert np.abs(V2 + V0).max() < 1e-9
     _, _, _, pe0, ke0, _ = oracle.vv_steps(*X0.T, *V0.T, box, kind="wca", dt=0.004, nsteps=0)
     assert abs((pe1 + ke1) - (pe0 + ke0)) < 2e-3 * (ke0 + abs(pe0))
+
+
+def _kg_small(seed=0, nchain=6, length=20, L=9.0):
+    """Small bead-spring melt (random-walk chains, bond 0.97), no bead pair closer than 0.8."""
+    rng = np.random.default_rng(seed)
+    pts = []
+    bonds = []
+    for c in range(nchain):
+        while True:
+            p = rng.random(3) * L - L / 2
+            if all(np.linalg.norm((p - q) - L * np.round((p - q) / L)) > 0.8 for q in pts):
+                break
+        pts.append(p)
+        for s in range(1, length):
+            for _ in range(1000):
+                v = rng.normal(size=3)
+                q = pts[-1] + 0.97 * v / np.linalg.norm(v)
+                q = q - L * np.floor(q / L + 0.5)
+                if all(np.linalg.norm((q - w) - L * np.round((q - w) / L)) > 0.8 for w in pts[:-1]):
+                    break
+            bonds.append((len(pts) - 1, len(pts)))
+            pts.append(q)
+    return np.array(pts), np.array(bonds), (L, L, L)
+
+
+def test_kg_oracle_pair_part_equals_direct_sums():
+    R, bonds, box = _kg_small()
+    d = oracle.kg_forces(*R.T, box, bonds[:0])
+    ref = oracle.pair_direct(*R.T, box, kind="wca")
+    assert np.abs(d["F"] - ref["F"]).max() < 1e-9 * max(1.0, np.abs(ref["F"]).max())
+    assert abs(d["energy"] - ref["energy"].sum()) < 1e-9 * max(1.0, abs(ref["energy"].sum()))
+    assert np.allclose(d["W"], ref["virial"].sum(0), rtol=1e-9, atol=1e-9)
+
+
+def test_kg_oracle_fene_closed_form():
+    k, R0, r = 30.0, 1.5, 1.1
+    d = oracle.kg_forces([0.0, r], [0.0, 0.0], [0.0, 0.0], (10.0, 10.0, 10.0), np.array([[0, 1]]), epsilon=0.0)
+    f = -k * r / (1 - r * r / (R0 * R0))  # force on bead 0 along +x (toward bead 1)
+    assert abs(d["F"][0, 0] + f) < 1e-12 * abs(f) and abs(d["F"][1, 0] - f) < 1e-12 * abs(f)
+    u = -0.5 * k * R0 * R0 * math.log(1 - r * r / (R0 * R0))
+    assert abs(d["energy"] - u) < 1e-12 * u
+    assert abs(d["W"][0] - (-f) * (-r)) < 1e-12 * abs(f * r)
+    with pytest.raises(ValueError):
+        oracle.kg_forces([0.0, 1.6], [0.0, 0.0], [0.0, 0.0], (10.0, 10.0, 10.0), np.array([[0, 1]]))
+
+
+def test_kg_oracle_gradient_and_strain():
+    R, bonds, box = _kg_small(seed=2, nchain=4, length=15, L=7.0)
+    d = oracle.kg_forces(*R.T, box, bonds)
+    h = 1e-6
+    for i in (0, 7, 31, 59):
+        for a in range(3):
+            Rp, Rm = R.copy(), R.copy()
+            Rp[i, a] += h
+            Rm[i, a] -= h
+            fd = -(oracle.kg_forces(*Rp.T, box, bonds)["energy"] - oracle.kg_forces(*Rm.T, box, bonds)["energy"]) / (2 * h)
+            assert abs(fd - d["F"][i, a]) < 1e-5 * max(1.0, abs(d["F"][i, a]))
+    for a in range(3):
+        s = np.ones(3)
+        s[a] += h
+        bp = tuple(np.array(box) * s)
+        ep = oracle.kg_forces(*(R * s).T, bp, bonds)["energy"]
+        s[a] -= 2 * h
+        bm = tuple(np.array(box) * s)
+        em = oracle.kg_forces(*(R * s).T, bm, bonds)["energy"]
+        assert abs(-(ep - em) / (2 * h) - d["W"][a]) < 1e-5 * max(1.0, abs(d["W"][a]))
