@@ -34,6 +34,8 @@ def parse():
     p.add_argument("--lanes", type=int, default=0)
     p.add_argument("--skin", type=float, default=1.0)
     p.add_argument("--spread", choices=("cells", "atomic"), default="cells")
+    p.add_argument("--traversal", choices=("rows", "cells"), default="rows",
+                   help="pair traversal: the list's ELL rows, or its cell grid without rows")
     p.add_argument("--no-learned", action="store_true", help="skip the learned-correction comparison")
     p.add_argument("--no-replicas", action="store_true", help="skip the batched-replica run")
     p.add_argument("--no-precision", action="store_true", help="skip the bead-spring accumulation study")
@@ -101,7 +103,7 @@ def main():
     spec = pdx.PME(coulomb_prefactor=C, real_cutoff=9.0, real_space_tolerance=1e-5, mesh_spacing=1.0, order=5)
     lj = pdx.LennardJones(epsilon=0.01, sigma=2.8, cutoff=9.0)
     eng = pdx.ForceEngine(N, box, lj, electrostatics=spec, charges=sysd["q"], skin=args.skin, lanes=args.lanes,
-                          spread=args.spread)
+                          spread=args.spread, traversal=args.traversal)
     X, Y, Z, Q = (torch.from_numpy(sysd[k]).cuda() for k in ("x", "y", "z", "q"))
     fx, fy, fz, E = (torch.empty(N, device="cuda") for _ in range(4))
     V = torch.empty(6, N, device="cuda")
@@ -204,7 +206,8 @@ def main():
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "rock-salt 64000 ions, LJ + SPME 120^3 P=5, rc 9 A, tol 1e-5, plain PME",
                       "N": N, "mesh": list(sol.mesh), "order": sol.order, "alpha": sol.alpha, "skin": args.skin,
-                      "cap": nl.cap, "lanes": args.lanes, "mode": args.mode, "spread": args.spread, "parallelism": "single GPU"},
+                      "cap": nl.cap, "lanes": args.lanes, "mode": args.mode, "spread": args.spread,
+                      "traversal": args.traversal, "parallelism": "single GPU"},
            "ns_per_day": round(ns_day, 2), "ns_per_day_note": f"dt=1 fs, one rebuild every {interval} steps",
            "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()}, "rebuild_ms": round(rebuild_ms, 4),
            "energy_eV": tot["energy"], "roofline": roofline}
@@ -387,7 +390,8 @@ def precision_leg(steps, warmup, nchains=10486, length=100):
     res = {"N": N, "chains": nchains, "bonds": int(len(sysd["bonds"])), "box": box[0],
            "push_off_s": round(t_push, 2), "fp64_oracle_s": round(t_ref, 2), "wca_pairs": ref["pairs"]}
     for mode in ("fp32", "fixed64", "fp64"):
-        eng = pdx.ForceEngine(N, box, pdx.WCA(epsilon=1.0, sigma=1.0), skin=0.4, accumulation=mode, bonds=bonds)
+        eng = pdx.ForceEngine(N, box, pdx.WCA(epsilon=1.0, sigma=1.0), skin=0.4, accumulation=mode, bonds=bonds,
+                              cap=2 * pdx.default_capacity(N, box, 2 ** (1 / 6) + 0.4))
         f = [torch.empty(N, device="cuda") for _ in range(3)]
         E = torch.empty(N, device="cuda")
         V = torch.empty(6, N, device="cuda")

@@ -260,6 +260,18 @@ int pdx_mesh_fft(pdx_mesh_t m, int32_t dir, const void *in, void *out, pdx_strea
 int pdx_pair_set_exceptions(pdx_pair_t p, int32_t n, int32_t nexc, const int32_t *i, const int32_t *j,
                             const double *scale);
 
+/* ---- traversal of the pair kernel ---- */
+/* ROWS: the ELL rows of the list (the reference's path). CELLS: no rows are read; one
+ * wave64 per particle walks the 27 cells of the list's grid (width >= cutoff + skin, so the
+ * cells cover every pair within the cutoff until the list asks for a rebuild), ballots the
+ * candidates within the cutoff into an LDS queue and evaluates them 64 at a time. Forces,
+ * energies and virials are those of the rows path up to the order of the FP32 sums; the
+ * order is fixed (cells, then ids), so results are deterministic. Needs a cell grid (not
+ * PDX_NL_ALLPAIRS) and FP32 accumulation without bonds. */
+#define PDX_TRAVERSAL_ROWS  0
+#define PDX_TRAVERSAL_CELLS 1
+int pdx_pair_set_traversal(pdx_pair_t p, int32_t mode);
+
 /* ---- accumulation of the per-particle sums (force, energy, virial) ---- */
 /* The precision study of a bead-spring melt: pair arithmetic stays FP32, the sums over a
  * particle's row (and bonds) run in FP32, in 64-bit fixed point (2^32 scale: exact and

@@ -156,6 +156,58 @@ __device__ void grid_sum_finalize(const double (&v)[NV], double *partial, unsign
   if (threadIdx.x == 0) *ticket = 0u;
 }
 
+// Deterministic grid-wide FP64 sums in two launches (no atomics): every block reduces its
+// NV values in a fixed order and stores them in partial[block*NV + c]; then one block of
+// k_sum_partials adds the partials in block order and writes (or adds into) out[c]. A
+// device-scope ticket for a last-block reduction would serialise every block on one address
+// in memory on a multi-XCD part. Must be called by every thread of the block (blockDim.x a
+// multiple of 64, at most 1024).
+template <int NV>
+__device__ void block_partials(const double (&v)[NV], double *partial) {
+  __shared__ double sh[NV][16];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = threadIdx.x / WAVE;
+  const int nwave = blockDim.x / WAVE;
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    double s = wave_sum(v[c]);
+    if (lane == 0) sh[c][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+    for (int w = 0; w < nwave; ++w) s += sh[threadIdx.x][w];
+    partial[(size_t)blockIdx.x * NV + threadIdx.x] = s;
+  }
+}
+
+// One block of 256 threads: out[c] (=, or += with add) = sum over blocks of partial[b*NV + c],
+// thread t taking blocks t, t + 256, ... and a fixed tree over the threads.
+template <int NV>
+__global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ partial, int nblocks, double *out,
+                                                      int add, const int64_t *state) {
+  if (state && !(state[PDX_DN_VALID] != 0 && state[PDX_DN_ERROR] == 0)) return;
+  __shared__ double sh[NV][4];
+  double acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) acc[c] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += 256) {
+#pragma unroll
+    for (int c = 0; c < NV; ++c) acc[c] += partial[(size_t)b * NV + c];
+  }
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    double s = wave_sum(acc[c]);
+    if (lane == 0) sh[c][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    const double s = ((sh[threadIdx.x][0] + sh[threadIdx.x][1]) + sh[threadIdx.x][2]) + sh[threadIdx.x][3];
+    out[threadIdx.x] = add ? out[threadIdx.x] + s : s;
+  }
+}
+
 }  // namespace pdx
 
 // ---------------------------------------------------------------- handles
@@ -175,6 +227,7 @@ struct pdx_nl_s {
   pdx::DevBuf<float> rref;      // 3*N: x | y | z
   pdx::DevBuf<float4> packed;   // N, storage order (x, y, z, q)
   pdx::DevBuf<float4> sorted;   // N, cell order (x, y, z, id bits)
+  pdx::DevBuf<float4> sortedq;  // N, cell order, current (x, y, z, q): the cell traversal's copy
   pdx::DevBuf<float> disp_partial;
   pdx::DevBuf<int32_t> cap_partial;
   pdx::DevBuf<int64_t> stat_partial;  // 2 per block: candidates, accepted
@@ -192,6 +245,7 @@ struct pdx_pair_s {
   double fit_err_e = 0, fit_err_f = 0;
   int lanes = 4;
   int accumulate = PDX_ACC_FP32;      // force / energy / virial accumulator of the row sums
+  int traversal = PDX_TRAVERSAL_ROWS;  // rows of the list, or the list's cells directly
   struct pdx_bonds_s *bonds = nullptr;  // FENE bonds folded into the same per-particle sums
   int32_t exc_n = 0, nexc = 0;          // Coulomb exceptions (CSR, both directions)
   pdx::DevBuf<int32_t> exc_off, exc_j;

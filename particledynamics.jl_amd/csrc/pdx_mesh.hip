@@ -363,6 +363,8 @@ struct SolveArgs {
   int K1, K2, K3, K1h;
   double twopi_L[3];   // 2 pi / L_d
   double inv4a2;       // 1/(4 alpha^2): Vc_ab = G (delta_ab + c k_a k_b), c = -2/k^2 - 2 inv4a2
+  float twopi_Lf[3];   // the same in FP32
+  float inv2a2f;       // 2 inv4a2 = 1/(2 alpha^2)
   double *partial;
   unsigned *ticket;
   double *acc;
@@ -402,26 +404,27 @@ __global__ __launch_bounds__(TPB) void k_fft_z(float2 *spec, FFTPlan pl, const f
       const float Gm = s.G[idx];
       float2 z = r[m3 * T + l];
       if (ENERGY && Gm != 0.0f) {
-        const double w = (m1 == 0 || ((s.K1 % 2 == 0) && m1 == s.K1 / 2)) ? 0.5 : 1.0;
-        const double a = w * ((double)z.x * z.x + (double)z.y * z.y) * (double)Gm;
+        // per-mode terms in FP32 (the precision the coefficients are stored in), sums in FP64
+        const float w = (m1 == 0 || ((s.K1 % 2 == 0) && m1 == s.K1 / 2)) ? 0.5f : 1.0f;
+        const float b2 = w * fmaf(z.x, z.x, z.y * z.y);
+        const float a = b2 * Gm;
         const int s3 = m3 <= K3 / 2 ? m3 : m3 - K3;
         const bool ny1 = (s.K1 % 2 == 0) && (m1 == s.K1 / 2);
         const bool ny3 = (K3 % 2 == 0) && (m3 == K3 / 2);
-        acc[0] += a;
+        acc[0] += (double)a;
         if (s.Vc) {  // tabulated -dG/d eps_ab (the optimal influence functions)
-          const double b2 = w * ((double)z.x * z.x + (double)z.y * z.y);
 #pragma unroll
-          for (int c = 0; c < 6; ++c) acc[1 + c] += b2 * (double)s.Vc[(size_t)c * s.nmodes + idx];
+          for (int c = 0; c < 6; ++c) acc[1 + c] += (double)(b2 * s.Vc[(size_t)c * s.nmodes + idx]);
         } else {
-          const double kx = s.twopi_L[0] * m1, ky = s.twopi_L[1] * s2, kz = s.twopi_L[2] * s3;
-          const double k2 = kx * kx + ky * ky + kz * kz;
-          const double c = -2.0 / k2 - 2.0 * s.inv4a2;
-          acc[1] += a * (1.0 + c * kx * kx);
-          acc[2] += a * (1.0 + c * ky * ky);
-          acc[3] += a * (1.0 + c * kz * kz);
-          acc[4] += (ny1 || ny2) ? 0.0 : a * c * kx * ky;
-          acc[5] += (ny1 || ny3) ? 0.0 : a * c * kx * kz;
-          acc[6] += (ny2 || ny3) ? 0.0 : a * c * ky * kz;
+          const float kx = s.twopi_Lf[0] * (float)m1, ky = s.twopi_Lf[1] * (float)s2, kz = s.twopi_Lf[2] * (float)s3;
+          const float k2 = fmaf(kx, kx, fmaf(ky, ky, kz * kz));
+          const float ac = a * (-2.0f / k2 - s.inv2a2f);
+          acc[1] += (double)fmaf(ac * kx, kx, a);
+          acc[2] += (double)fmaf(ac * ky, ky, a);
+          acc[3] += (double)fmaf(ac * kz, kz, a);
+          acc[4] += (ny1 || ny2) ? 0.0 : (double)(ac * kx * ky);
+          acc[5] += (ny1 || ny3) ? 0.0 : (double)(ac * kx * kz);
+          acc[6] += (ny2 || ny3) ? 0.0 : (double)(ac * ky * kz);
         }
       }
       const float2 gz = cscale(z, Gm);
@@ -452,7 +455,7 @@ __global__ __launch_bounds__(TPB) void k_fft_z(float2 *spec, FFTPlan pl, const f
   if (MODE == 2 && ENERGY) {
     // mesh energy = 1/2 sum_full |b|^2 G = sum_half w |b|^2 G (w = 1/2 on the m1 = 0 and
     // Nyquist planes); virial W_ab = sum_half w |b|^2 Vc_ab
-    grid_sum_finalize<7>(acc, s.partial, s.ticket, s.acc, false);
+    block_partials<7>(acc, s.partial);
   }
 }
 
@@ -658,9 +661,9 @@ __global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__
     }
     __syncthreads();
   }
+  const int nl = min(nlist, SLIST);
   // staged particles: LPP lanes each, lane = one (x, y) line of the stencil walking P z nodes;
   // the products and their order are those of k_spread: ((q w_x) w_y) 2^S, then times w_z
-  const int nl = min(nlist, SLIST);
   const int grp = threadIdx.x / LPP, lane = threadIdx.x % LPP;
   for (int t = grp; t < nl; t += TPB / LPP) {
     const float4 u = list[t];
@@ -1094,6 +1097,8 @@ SolveArgs solve_args(pdx_mesh_t m) {
   s.K1 = m->K1; s.K2 = m->K2; s.K3 = m->K3; s.K1h = m->K1h;
   for (int d = 0; d < 3; ++d) s.twopi_L[d] = 2 * M_PI / m->box[d];
   s.inv4a2 = 1.0 / (4.0 * m->alpha * m->alpha);
+  for (int d = 0; d < 3; ++d) s.twopi_Lf[d] = (float)s.twopi_L[d];
+  s.inv2a2f = (float)(2.0 * s.inv4a2);
   s.partial = m->partial.p;
   s.ticket = m->ticket.p;
   s.acc = m->acc.p;
@@ -1190,6 +1195,9 @@ void launch_z(pdx_mesh_t m, const Plans &pl, hipStream_t st, float2 *spec = null
                        spec, pl.p3, tw3, sa);
   PDX_T_SWITCH(pl.T3, PDX_ZZ)
 #undef PDX_ZZ
+  if (MODE == 2 && ENERGY)  // mesh energy and virial: the z blocks' partials in block order
+    hipLaunchKernelGGL(k_sum_partials<7>, dim3(1), dim3(256), 0, st, (const double *)m->partial.p, gz, m->acc.p, 0,
+                       (const int64_t *)nullptr);
 }
 
 #define PDX_SPREAD_CASE(PP)                                                                                    \

@@ -31,18 +31,18 @@ __device__ __forceinline__ bool rebuild_requested(const int64_t *s) {
 
 // ------------------------------------------------------------------ decide
 // Coverage check (_device_displacement_kernel! + _device_decide_kernel!,
-// DeviceNeighborCUDA.jl:169-236) fused into one launch with a last-block decision.
+// DeviceNeighborCUDA.jl:169-236): per-block maximum squared displacement (plus the float4
+// packing and the histogram clearing), then one block decides (k_decide_final).
 __global__ __launch_bounds__(TPB) void k_decide(int N, const float *__restrict__ x,
                                                  const float *__restrict__ y,
                                                  const float *__restrict__ z,
                                                  const float *__restrict__ q,
                                                  const float *__restrict__ rref, float4 *packed,
                                                  int32_t *cell_counts, int ncell, int allpairs,
-                                                 const float *geom, int64_t *state,
-                                                 float *partial, unsigned *ticket,
-                                                 float threshold, int force) {
+                                                 const float *geom, const int64_t *state,
+                                                 float *partial) {
   const int64_t err = state[PDX_DN_ERROR];
-  if (err != 0) return;  // latched: every block leaves (uniform; the ticket is not touched)
+  if (err != 0) return;  // latched: every block leaves (uniform)
   const int i = blockIdx.x * TPB + threadIdx.x;
   const float L[3] = {geom[0], geom[1], geom[2]};
   const bool valid = state[PDX_DN_VALID] != 0;
@@ -66,7 +66,6 @@ __global__ __launch_bounds__(TPB) void k_decide(int N, const float *__restrict__
     for (int c = i; c < ncell; c += gridDim.x * TPB) cell_counts[c] = 0;
 
   __shared__ float smax[TPB / WAVE];
-  __shared__ bool last;
   float m = wave_max(value);
   if ((threadIdx.x & (WAVE - 1)) == 0) smax[threadIdx.x / WAVE] = m;
   __syncthreads();
@@ -74,21 +73,23 @@ __global__ __launch_bounds__(TPB) void k_decide(int N, const float *__restrict__
     float b = smax[0];
     for (int w = 1; w < TPB / WAVE; ++w) b = fmaxf(b, smax[w]);
     partial[blockIdx.x] = b;
-    __threadfence();
-    last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+}
+
+// The decision from the blocks' maxima (one block).
+__global__ __launch_bounds__(TPB) void k_decide_final(int nblocks, const float *__restrict__ partial, int64_t *state,
+                                                       float threshold, int force) {
+  if (state[PDX_DN_ERROR] != 0) return;
+  __shared__ float smax[TPB / WAVE];
   float v = 0.0f;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += TPB) v = fmaxf(v, __builtin_nontemporal_load(&partial[b]));
+  for (int b = threadIdx.x; b < nblocks; b += TPB) v = fmaxf(v, partial[b]);
   v = wave_max(v);
   if ((threadIdx.x & (WAVE - 1)) == 0) smax[threadIdx.x / WAVE] = v;
   __syncthreads();
   if (threadIdx.x != 0) return;
   float largest = smax[0];
   for (int w = 1; w < TPB / WAVE; ++w) largest = fmaxf(largest, smax[w]);
-  *ticket = 0u;
+  const bool valid = state[PDX_DN_VALID] != 0;
   state[PDX_DN_CHECKS] += 1;
   state[PDX_DN_REQUEST] = 0;
   const bool diag = state[PDX_DN_DIAGNOSTICS] != 0;
@@ -377,17 +378,14 @@ __global__ __launch_bounds__(TPB) void k_rows_allpairs(int N, int cap,
 // ------------------------------------------------------------------ commit
 // Largest row vs cap, build counters (_device_capacity_partial_kernel! + _device_commit_kernel!,
 // DeviceNeighborCUDA.jl:291-388), one launch with a last-block commit.
-__global__ __launch_bounds__(TPB) void k_commit(int N, int cap, const int32_t *counts,
-                                                 const int64_t *candidates, int32_t *partial,
-                                                 int64_t *stat_partial, unsigned *ticket,
-                                                 int64_t *state) {
+__global__ __launch_bounds__(TPB) void k_commit(int N, const int32_t *counts, const int64_t *candidates,
+                                                 int32_t *partial, int64_t *stat_partial, const int64_t *state) {
   if (!rebuild_requested(state)) return;
   const int i = blockIdx.x * TPB + threadIdx.x;
   const int v = i < N ? counts[i] : 0;
   const int64_t cand = (candidates && i < N) ? candidates[i] : 0;
   __shared__ int smax[TPB / WAVE];
   __shared__ long long ssum[2][TPB / WAVE];
-  __shared__ bool last;
   int m = wave_max(v);
   long long sc = cand, sa = v;
   for (int off = 32; off >= 1; off >>= 1) {
@@ -403,20 +401,26 @@ __global__ __launch_bounds__(TPB) void k_commit(int N, int cap, const int32_t *c
     partial[blockIdx.x] = b;
     stat_partial[2 * blockIdx.x] = c0;
     stat_partial[2 * blockIdx.x + 1] = a0;
-    __threadfence();
-    last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last || threadIdx.x != 0) return;
-  __threadfence();
+}
+
+// Capacity commit from the blocks' partials (one wave64; integer sums, so any order is exact).
+__global__ __launch_bounds__(WAVE) void k_commit_final(int nblocks, int cap, const int32_t *__restrict__ partial,
+                                                       const int64_t *__restrict__ stat_partial, int64_t *state) {
+  if (!rebuild_requested(state)) return;
   int largest = 0;
   long long ctot = 0, atot = 0;
-  for (unsigned b = 0; b < gridDim.x; ++b) {
-    largest = max(largest, __builtin_nontemporal_load(&partial[b]));
-    ctot += __builtin_nontemporal_load(&stat_partial[2 * b]);
-    atot += __builtin_nontemporal_load(&stat_partial[2 * b + 1]);
+  for (int b = threadIdx.x; b < nblocks; b += WAVE) {
+    largest = max(largest, partial[b]);
+    ctot += stat_partial[2 * b];
+    atot += stat_partial[2 * b + 1];
   }
-  *ticket = 0u;
+  largest = wave_max(largest);
+  for (int off = 32; off >= 1; off >>= 1) {
+    ctot += __shfl_xor(ctot, off, WAVE);
+    atot += __shfl_xor(atot, off, WAVE);
+  }
+  if (threadIdx.x != 0) return;
   state[PDX_DN_REQUIRED] = largest;
   const bool diag = state[PDX_DN_DIAGNOSTICS] != 0;
   if (diag) {
@@ -519,9 +523,9 @@ int enqueue_rebuild(pdx_nl_t nl, hipStream_t st) {
                        nl->rl2, nl->neighbors_flat.p, nl->counts.p,
                        nl->diagnostics ? nl->candidates.p : nullptr, s);
   }
-  hipLaunchKernelGGL(k_commit, dim3(nb), dim3(TPB), 0, st, N, nl->cap, nl->counts.p,
-                     nl->diagnostics ? nl->candidates.p : nullptr, nl->cap_partial.p, nl->stat_partial.p,
-                     nl->tickets.p + 1, s);
+  hipLaunchKernelGGL(k_commit, dim3(nb), dim3(TPB), 0, st, N, nl->counts.p,
+                     nl->diagnostics ? nl->candidates.p : nullptr, nl->cap_partial.p, nl->stat_partial.p, s);
+  hipLaunchKernelGGL(k_commit_final, dim3(1), dim3(64), 0, st, nb, nl->cap, nl->cap_partial.p, nl->stat_partial.p, s);
   hipLaunchKernelGGL(k_copy_refs, dim3(nb), dim3(TPB), 0, st, N, nl->packed.p, nl->rref.p, s);
   PDX_LAUNCH_CHECK();
   return PDX_OK;
@@ -534,9 +538,11 @@ int enqueue_update(pdx_nl_t nl, const float *x, const float *y, const float *z, 
   const int N = nl->N;
   const float s = (float)nl->skin;
   const float threshold = (0.25f * s) * s;
-  hipLaunchKernelGGL(k_decide, dim3(blocks_for(N, TPB)), dim3(TPB), 0, st, N, x, y, z, q, nl->rref.p,
-                     nl->packed.p, nl->cell_counts.p, nl->ncell, (int)nl->allpairs, nl->geom.p,
-                     nl->state.p, nl->disp_partial.p, nl->tickets.p, threshold, force);
+  const int nb = blocks_for(N, TPB);
+  hipLaunchKernelGGL(k_decide, dim3(nb), dim3(TPB), 0, st, N, x, y, z, q, nl->rref.p, nl->packed.p,
+                     nl->cell_counts.p, nl->ncell, (int)nl->allpairs, nl->geom.p, nl->state.p, nl->disp_partial.p);
+  hipLaunchKernelGGL(k_decide_final, dim3(1), dim3(TPB), 0, st, nb, nl->disp_partial.p, nl->state.p, threshold,
+                     force);
   PDX_LAUNCH_CHECK();
   return enqueue_rebuild(nl, st);
 }
@@ -601,7 +607,7 @@ extern "C" int pdx_nl_create(int32_t n, const double box[3], double cutoff, doub
       (rc = nl->cell_offsets.alloc(nl->ncell + 1)) || (rc = nl->cell_counts.alloc(nl->ncell)) ||
       (rc = nl->particle_ids_sorted.alloc(n)) || (rc = nl->cell_of_particle.alloc(n)) ||
       (rc = nl->cell_rank.alloc(n)) || (rc = nl->rref.alloc((size_t)3 * n)) || (rc = nl->packed.alloc(n)) ||
-      (rc = nl->sorted.alloc(n)) || (rc = nl->disp_partial.alloc(nb)) || (rc = nl->cap_partial.alloc(nb)) ||
+      (rc = nl->sorted.alloc(n)) || (rc = nl->sortedq.alloc(n)) || (rc = nl->disp_partial.alloc(nb)) || (rc = nl->cap_partial.alloc(nb)) ||
       (rc = nl->stat_partial.alloc((size_t)2 * nb)) || (rc = nl->scan_block.alloc(sblocks)) ||
       (rc = nl->candidates.alloc(nl->diagnostics ? n : 1)) || (rc = nl->tickets.alloc(4))) {
     delete nl;

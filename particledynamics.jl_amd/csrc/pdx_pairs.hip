@@ -126,7 +126,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs &a, bool owner, int
   if (ENERGY && a.totals) {
     const double d[7] = {(double)ei, (double)w[0], (double)w[1], (double)w[2],
                          (double)w[3], (double)w[4], (double)w[5]};
-    grid_sum_finalize<7>(d, a.partial, a.ticket, a.totals, a.add != 0);
+    block_partials<7>(d, a.partial);
   }
 }
 
@@ -298,6 +298,242 @@ __global__ __launch_bounds__(64 * S) void k_pair_split(const PairArgs a) {
   pair_epilogue<COUL, ENERGY>(a, active && w == 0, i, qi, sum[0], sum[1], sum[2], ENERGY ? sum[3] : 0.f, v);
 }
 
+// ---- cell traversal: no rows. One workgroup per home cell of the list's grid stages the
+// current (x, y, z, q) of its 27 neighbour cells into LDS (cell order, 2048 candidates per
+// chunk), then each wave takes home particles in turn: 64 candidates at a time are tested
+// against the cutoff, the hits are ballot-compacted into a wave-private LDS queue in candidate
+// order and evaluated 64 at a time with every lane busy. Candidate order (cells z, y, x, then
+// ids inside a cell) and all sums are fixed, so results are deterministic.
+struct CellArgs {
+  const float4 *sortedq;          // current (x, y, z, q) in cell order
+  const int32_t *ids;             // 1-based storage id of cell slot t
+  const int32_t *offsets;         // 1-based cell offsets (ncell + 1)
+  int nx, ny, nz;
+  float cut2;                     // the largest cutoff^2 of the pair terms
+};
+
+constexpr int CSTAGE = 2048;  // staged candidates per chunk (32 KiB)
+constexpr int CHOME = 192;    // home particles per pass (their partial sums live in LDS)
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// cell-ordered copy of the current coordinates and charges (every step, after the list check)
+__global__ __launch_bounds__(TPB) void k_refresh_sorted(int N, const int32_t *__restrict__ ids,
+                                                         const float4 *__restrict__ packed, float4 *sortedq,
+                                                         const int64_t *state) {
+  if (!(state[PDX_DN_VALID] != 0 && state[PDX_DN_ERROR] == 0)) return;
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t < N) sortedq[t] = packed[ids[t] - 1];
+}
+
+template <int MODEL, bool COUL, bool ENERGY, int NP>
+__global__ __launch_bounds__(TPB) void k_pair_cells(const PairArgs a, const CellArgs c) {
+  const int64_t *st = a.state;
+  const bool valid = st[PDX_DN_VALID] != 0 && st[PDX_DN_ERROR] == 0;  // uniform
+  if (!valid) return;
+  constexpr int WPB = TPB / WAVE;
+  constexpr int NACC = ENERGY ? 10 : 3;
+  __shared__ float4 stage[CSTAGE];
+  __shared__ float4 queue[WPB][2 * WAVE];
+  __shared__ float accs[CHOME][NACC];
+  __shared__ int seg_start[27], seg_pre[28];
+  const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int nx = c.nx, ny = c.ny, nz = c.nz;
+  const int home = blockIdx.x;
+  const int cx = home % nx, cy = (home / nx) % ny, cz = home / (nx * ny);
+  const float Lx = a.geom[0], Ly = a.geom[1], Lz = a.geom[2];
+  const float hx = 0.5f * Lx, hy = 0.5f * Ly, hz = 0.5f * Lz;
+  float pe[NP], pf[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    pe[k] = a.pe[k];
+    pf[k] = a.pf[k];
+  }
+  if (threadIdx.x < 27) {
+    const int k = threadIdx.x;
+    int z2 = cz + k / 9 - 1, y2 = cy + (k / 3) % 3 - 1, x2 = cx + k % 3 - 1;
+    z2 += z2 < 0 ? nz : 0; z2 -= z2 >= nz ? nz : 0;
+    y2 += y2 < 0 ? ny : 0; y2 -= y2 >= ny ? ny : 0;
+    x2 += x2 < 0 ? nx : 0; x2 -= x2 >= nx ? nx : 0;
+    const int cc = (z2 * ny + y2) * nx + x2;
+    seg_start[k] = c.offsets[cc] - 1;
+    seg_pre[k + 1] = c.offsets[cc + 1] - c.offsets[cc];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    seg_pre[0] = 0;
+    for (int k = 1; k <= 27; ++k) seg_pre[k] += seg_pre[k - 1];
+  }
+  __syncthreads();
+  const int total = seg_pre[27];
+  const int hs = c.offsets[home] - 1, he = c.offsets[home + 1] - 1;
+  float4 *Q = queue[w];
+  double tot[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int hg = hs; hg < he; hg += CHOME) {  // uniform
+    const int hn = min(CHOME, he - hg);
+    for (int k = threadIdx.x; k < hn * NACC; k += TPB) (&accs[0][0])[k] = 0.f;
+    for (int c0 = 0; c0 < total; c0 += CSTAGE) {  // uniform
+      const int nch = min(CSTAGE, total - c0);
+      __syncthreads();  // the previous chunk is no longer read
+      for (int f = threadIdx.x; f < nch; f += TPB) {
+        const int g = c0 + f;
+        int sg = 0;
+        while (seg_pre[sg + 1] <= g) ++sg;
+        stage[f] = c.sortedq[seg_start[sg] + (g - seg_pre[sg])];
+      }
+      __syncthreads();
+      for (int p = w; p < hn; p += WPB) {
+        const int t = hg + p;
+        const float4 pi = c.sortedq[t];
+        const float qi = pi.w;
+        float ax = 0.f, ay = 0.f, az = 0.f, e = 0.f;
+        float v[6] = {0, 0, 0, 0, 0, 0};
+        auto evaluate = [&](int cnt) {
+          if (lane < cnt) {
+            const float4 d = Q[lane];
+            const float r2 = fmaf(d.x, d.x, fmaf(d.y, d.y, d.z * d.z));
+            float f, uu;
+            pair_terms<MODEL, COUL, ENERGY, NP>(a, pe, pf, qi, d.w, r2, f, uu);
+            ax = fmaf(f, d.x, ax);
+            ay = fmaf(f, d.y, ay);
+            az = fmaf(f, d.z, az);
+            if (ENERGY) {
+              e += uu;
+              const float fxij = f * d.x, fyij = f * d.y, fzij = f * d.z;
+              v[0] = fmaf(d.x, fxij, v[0]); v[1] = fmaf(d.y, fyij, v[1]); v[2] = fmaf(d.z, fzij, v[2]);
+              v[3] = fmaf(d.x, fyij, v[3]); v[4] = fmaf(d.x, fzij, v[4]); v[5] = fmaf(d.y, fzij, v[5]);
+            }
+          }
+        };
+        int qn = 0;  // uniform
+        for (int k0 = 0; k0 < nch; k0 += WAVE) {
+          const int k = k0 + lane;
+          bool hit = false;
+          float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (k < nch) {  // the particle itself has r^2 = 0, which every pair term excludes
+            const float4 pj = stage[k];
+            d.x = periodic_delta(pi.x, pj.x, hx, Lx);
+            d.y = periodic_delta(pi.y, pj.y, hy, Ly);
+            d.z = periodic_delta(pi.z, pj.z, hz, Lz);
+            d.w = pj.w;
+            hit = fmaf(d.x, d.x, fmaf(d.y, d.y, d.z * d.z)) < c.cut2;
+          }
+          const unsigned long long m = __ballot(hit);
+          if (hit) Q[qn + __popcll(m & below)] = d;
+          qn += __popcll(m);
+          if (qn >= WAVE) {
+            wave_lds_sync();
+            evaluate(WAVE);
+            const float4 rest = lane < qn - WAVE ? Q[WAVE + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+            wave_lds_sync();
+            if (lane < qn - WAVE) Q[lane] = rest;
+            qn -= WAVE;
+            wave_lds_sync();
+          }
+        }
+        wave_lds_sync();
+        evaluate(qn);
+        wave_lds_sync();
+        if (COUL && a.exo && c0 == 0 && qi != 0.0f) {
+          const int i = c.ids[t] - 1;
+          for (int q = a.exo[i] + lane; q < a.exo[i + 1]; q += WAVE) {
+            const float4 pj = a.packed[a.exj[q]];
+            const float sc = a.cC * qi * pj.w * (a.exs[q] - 1.0f);
+            const float dx = periodic_delta(pi.x, pj.x, hx, Lx);
+            const float dy = periodic_delta(pi.y, pj.y, hy, Ly);
+            const float dz = periodic_delta(pi.z, pj.z, hz, Lz);
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const float invr = __builtin_amdgcn_rsqf(r2);
+            const float f = sc * invr * invr * invr;
+            ax = fmaf(f, dx, ax);
+            ay = fmaf(f, dy, ay);
+            az = fmaf(f, dz, az);
+            if (ENERGY) {
+              e += sc * invr + (r2 < a.crc2 ? a.cC * qi * pj.w * a.ceshift : 0.0f);
+              const float fxij = f * dx, fyij = f * dy, fzij = f * dz;
+              v[0] = fmaf(dx, fxij, v[0]); v[1] = fmaf(dy, fyij, v[1]); v[2] = fmaf(dz, fzij, v[2]);
+              v[3] = fmaf(dx, fyij, v[3]); v[4] = fmaf(dx, fzij, v[4]); v[5] = fmaf(dy, fzij, v[5]);
+            }
+          }
+        }
+        ax = wave_sum_width(ax, WAVE);
+        ay = wave_sum_width(ay, WAVE);
+        az = wave_sum_width(az, WAVE);
+        if (ENERGY) {
+          e = wave_sum_width(e, WAVE);
+#pragma unroll
+          for (int k = 0; k < 6; ++k) v[k] = wave_sum_width(v[k], WAVE);
+        }
+        if (lane == 0) {  // only this wave touches particle p: chunk sums in chunk order
+          accs[p][0] += ax; accs[p][1] += ay; accs[p][2] += az;
+          if (ENERGY) {
+            accs[p][3] += e;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) accs[p][4 + k] += v[k];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // outputs of the pass: thread per home particle (half energies and half virials)
+    for (int p = threadIdx.x; p < hn; p += TPB) {
+      const int t = hg + p;
+      const int i = c.ids[t] - 1;
+      const float qi = c.sortedq[t].w;
+      const float fx = accs[p][0], fy = accs[p][1], fz = accs[p][2];
+      if (a.add) {
+        a.fx[i] += fx; a.fy[i] += fy; a.fz[i] += fz;
+      } else {
+        a.fx[i] = fx; a.fy[i] = fy; a.fz[i] = fz;
+      }
+      if (ENERGY) {
+        float ei = 0.5f * accs[p][3];
+        if (COUL) ei -= a.halfself * qi * qi;
+        float wv[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) wv[k] = 0.5f * accs[p][4 + k];
+        if (a.Epot) a.Epot[i] = a.add ? a.Epot[i] + ei : ei;
+        if (a.V) {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) {
+            float *q = a.V + (size_t)k * a.N + i;
+            *q = a.add ? *q + wv[k] : wv[k];
+          }
+        }
+        tot[0] += (double)ei;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) tot[1 + k] += (double)wv[k];
+      }
+    }
+  }
+  if (ENERGY && a.totals) block_partials<7>(tot, a.partial);
+}
+
+template <int MODEL, bool COUL, bool ENERGY, int NP>
+void launch_cells_np(dim3 grid, hipStream_t st, const PairArgs &a, const CellArgs &c) {
+  hipLaunchKernelGGL((k_pair_cells<MODEL, COUL, ENERGY, NP>), grid, dim3(TPB), 0, st, a, c);
+}
+
+template <int MODEL>
+void launch_cells(bool coul, bool energy, int np, dim3 grid, hipStream_t st, const PairArgs &a, const CellArgs &c) {
+  if (coul) {
+    if (np <= 16) {
+      if (energy) launch_cells_np<MODEL, true, true, 16>(grid, st, a, c);
+      else launch_cells_np<MODEL, true, false, 16>(grid, st, a, c);
+    } else {
+      if (energy) launch_cells_np<MODEL, true, true, 24>(grid, st, a, c);
+      else launch_cells_np<MODEL, true, false, 24>(grid, st, a, c);
+    }
+  } else {
+    if (energy) launch_cells_np<MODEL, false, true, 1>(grid, st, a, c);
+    else launch_cells_np<MODEL, false, false, 1>(grid, st, a, c);
+  }
+}
+
 // ---- per-particle sums in a chosen precision (the bead-spring precision study). The pair
 // arithmetic is the FP32 arithmetic of k_pair; only the accumulation of the row sums changes:
 // FP32 (fma, as k_pair), 64-bit fixed point at 2^32 (exact, so independent of the order and of
@@ -448,7 +684,7 @@ __global__ __launch_bounds__(TPB) void k_pair_acc(const PairArgs a, int bonded) 
   }
   if (ENERGY && a.totals) {
     const double d[7] = {ei, w[0], w[1], w[2], w[3], w[4], w[5]};
-    grid_sum_finalize<7>(d, a.partial, a.ticket, a.totals, a.add != 0);
+    block_partials<7>(d, a.partial);
   }
 }
 
@@ -704,6 +940,11 @@ extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, 
   // lanes > 0: G lanes of a wave per particle; lanes < 0: the split layout with -lanes waves
   // per block of 64 particles; 0: automatic
   const bool acc_path = p->accumulate != PDX_ACC_FP32 || p->bonds != nullptr;
+  const bool cells = p->traversal == PDX_TRAVERSAL_CELLS;
+  if (cells) {
+    if (acc_path) return fail(PDX_ERR_ARGUMENT, "the cell traversal uses FP32 sums without bonds");
+    if (nl->allpairs) return fail(PDX_ERR_ARGUMENT, "the cell traversal needs a cell grid (not all-pairs rows)");
+  }
   if (acc_path) {
     if (p->spec.coulomb) return fail(PDX_ERR_ARGUMENT, "accumulation modes and bonds are not available with Coulomb");
     if (p->spec.model != PDX_PAIR_LJ && p->spec.model != PDX_PAIR_WCA && p->spec.model != PDX_PAIR_SOFT)
@@ -720,7 +961,8 @@ extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, 
     while (G * 2 <= want && G < 64) G <<= 1;
   }
   if (acc_path) G = std::min(G, 8);
-  const int blocks = G < 0 ? (N + WAVE - 1) / WAVE : (int)(((int64_t)N * G + TPB - 1) / TPB);
+  const int blocks = cells ? nl->ncell
+                           : (G < 0 ? (N + WAVE - 1) / WAVE : (int)(((int64_t)N * G + TPB - 1) / TPB));
   if (energy && totals) {
     if ((size_t)blocks * 7 > p->partial.n) {
       PDX_HIP(hipStreamSynchronize(as_stream(stream)));
@@ -749,10 +991,40 @@ extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, 
   hipStream_t st = as_stream(stream);
   if (p->nexc > 0) {
     if (p->exc_n != N) return fail(PDX_ERR_DIMENSION, "exceptions and neighbor list particle counts differ");
-    if (G < 0) return fail(PDX_ERR_ARGUMENT, "exceptions use the lanes layout (lanes >= 0)");
+    if (G < 0 && !cells) return fail(PDX_ERR_ARGUMENT, "exceptions use the lanes layout (lanes >= 0)");
     a.exo = p->exc_off.p;
     a.exj = p->exc_j.p;
     a.exs = p->exc_s.p;
+  }
+  // FP64 totals: per-block partials, then one block adds them in block order
+  auto finish = [&]() -> int {
+    PDX_LAUNCH_CHECK();
+    if (energy && totals) {
+      hipLaunchKernelGGL(k_sum_partials<7>, dim3(1), dim3(256), 0, st, (const double *)p->partial.p, blocks,
+                         totals, a.add, (const int64_t *)nl->state.p);
+      PDX_LAUNCH_CHECK();
+    }
+    return PDX_OK;
+  };
+  if (cells) {
+    CellArgs c{};
+    c.sortedq = nl->sortedq.p;
+    c.ids = nl->particle_ids_sorted.p;
+    c.offsets = nl->cell_offsets.p;
+    c.nx = nl->nx; c.ny = nl->ny; c.nz = nl->nz;
+    float cut2 = p->spec.model != PDX_PAIR_NONE ? p->rc2 : 0.0f;
+    if (coul) cut2 = std::max(cut2, p->crc2);
+    c.cut2 = cut2;
+    hipLaunchKernelGGL(k_refresh_sorted, dim3((N + TPB - 1) / TPB), dim3(TPB), 0, st, N, nl->particle_ids_sorted.p,
+                       nl->packed.p, nl->sortedq.p, nl->state.p);
+    const dim3 grid(nl->ncell);
+    switch (p->spec.model) {
+      case PDX_PAIR_LJ: launch_cells<PDX_PAIR_LJ>(coul, energy, np, grid, st, a, c); break;
+      case PDX_PAIR_WCA: launch_cells<PDX_PAIR_WCA>(coul, energy, np, grid, st, a, c); break;
+      case PDX_PAIR_SOFT: launch_cells<PDX_PAIR_SOFT>(coul, energy, np, grid, st, a, c); break;
+      default: launch_cells<PDX_PAIR_NONE>(coul, energy, np, grid, st, a, c); break;
+    }
+    return finish();
   }
   if (acc_path) {
     const int bonded = p->bonds != nullptr;
@@ -766,8 +1038,7 @@ extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, 
     if (p->spec.model == PDX_PAIR_LJ) launch_acc<PDX_PAIR_LJ>(p->accumulate, energy, G, dim3(blocks), st, a, bonded);
     else if (p->spec.model == PDX_PAIR_WCA) launch_acc<PDX_PAIR_WCA>(p->accumulate, energy, G, dim3(blocks), st, a, bonded);
     else launch_acc<PDX_PAIR_SOFT>(p->accumulate, energy, G, dim3(blocks), st, a, bonded);
-    PDX_LAUNCH_CHECK();
-    return PDX_OK;
+    return finish();
   }
   switch (p->spec.model) {
     case PDX_PAIR_LJ: launch_model<PDX_PAIR_LJ>(coul, energy, np, G, dim3(blocks), st, a); break;
@@ -775,8 +1046,7 @@ extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, 
     case PDX_PAIR_SOFT: launch_model<PDX_PAIR_SOFT>(coul, energy, np, G, dim3(blocks), st, a); break;
     default: launch_model<PDX_PAIR_NONE>(coul, energy, np, G, dim3(blocks), st, a); break;
   }
-  PDX_LAUNCH_CHECK();
-  return PDX_OK;
+  return finish();
 }
 
 extern "C" int pdx_pair_set_accumulation(pdx_pair_t p, int32_t mode) {
@@ -927,5 +1197,13 @@ extern "C" int pdx_pair_set_exceptions(pdx_pair_t p, int32_t n, int32_t nexc, co
   PDX_HIP(hipMemcpy(p->exc_j.p, jj.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
   PDX_HIP(hipMemcpy(p->exc_s.p, ss.data(), sizeof(float) * m, hipMemcpyHostToDevice));
   p->nexc = (int32_t)(m / 2);
+  return PDX_OK;
+}
+
+extern "C" int pdx_pair_set_traversal(pdx_pair_t p, int32_t mode) {
+  if (!p) return fail(PDX_ERR_ARGUMENT, "null pair model");
+  if (mode != PDX_TRAVERSAL_ROWS && mode != PDX_TRAVERSAL_CELLS)
+    return fail(PDX_ERR_ARGUMENT, "traversal must be PDX_TRAVERSAL_ROWS or PDX_TRAVERSAL_CELLS");
+  p->traversal = mode;
   return PDX_OK;
 }

@@ -21,7 +21,7 @@ def default_capacity(N, box, rlist):
 class ForceEngine:
     def __init__(self, N, box, pair_model=None, electrostatics=None, charges=None, skin=0.4, cap=None,
                  interval=20, lanes=0, diagnostics=False, spread="cells", correction=None, accumulation="fp32",
-                 bonds=None, exceptions=()):
+                 bonds=None, exceptions=(), traversal="rows"):
         import torch
         self.N = int(N)
         self.box = tuple(float(b) for b in box)
@@ -36,7 +36,7 @@ class ForceEngine:
         cap = cap if cap is not None else default_capacity(N, box, rc + skin)
         self.nl = DeviceNeighborWorkspace(N, box, rc, skin, cap, interval=interval, diagnostics=diagnostics)
         self.pairs = PairForce(pair_model, coulomb=self.solver.real_space() if self.solver else None, lanes=lanes,
-                               accumulation=accumulation, bonds=bonds)
+                               accumulation=accumulation, bonds=bonds, traversal=traversal)
         if exceptions:
             if self.solver is None:
                 raise L.ArgumentError("electrostatic exceptions need a mesh")

@@ -67,6 +67,7 @@ def _sig(lib):
         "pdx_bonds_destroy": [vp],
         "pdx_pair_set_bonds": [vp, vp],
         "pdx_bonds_status": [vp, P(C.c_int32), i32, vp],
+        "pdx_pair_set_traversal": [vp, i32],
         "pdx_pair_set_exceptions": [vp, i32, i32, P(C.c_int32), P(C.c_int32), P(C.c_double)],
     }.items():
         f = getattr(lib, name)
@@ -119,7 +120,7 @@ class FENEBonds:
 
 class PairForce:
     def __init__(self, model=None, coulomb: RealSpaceCoulomb | None = None, lanes: int = 0,
-                 accumulation: str = "fp32", bonds: FENEBonds | None = None):
+                 accumulation: str = "fp32", bonds: FENEBonds | None = None, traversal: str = "rows"):
         spec = L.pdx_pair_spec()
         if model is None:
             spec.model = L.PDX_PAIR_NONE
@@ -143,8 +144,17 @@ class PairForce:
         self.coulomb = coulomb
         self.bonds = None
         self.set_accumulation(accumulation)
+        self.set_traversal(traversal)
         if bonds is not None:
             self.set_bonds(bonds)
+
+    def set_traversal(self, mode: str):
+        """"rows": the list's ELL rows; "cells": the list's cell grid directly, one wave per
+        particle with ballot-compacted candidates (no rows are read)."""
+        if mode not in ("rows", "cells"):
+            raise L.ArgumentError("traversal must be 'rows' or 'cells'")
+        L.check(L.lib.pdx_pair_set_traversal(self._h, 0 if mode == "rows" else 1))
+        self.traversal = mode
 
     def set_accumulation(self, mode: str):
         """Precision of the per-particle sums: "fp32", "fixed64" (64-bit fixed point, exact)

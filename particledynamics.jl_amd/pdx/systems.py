@@ -108,6 +108,7 @@ def replica_batch(nrep=512, seed=0, n_range=(256, 8192), density_range=(0.70, 0.
         Lb = np.float32(s["box"][0])
         c = X[offsets[k]:offsets[k + 1]]
         c[c >= Lb / 2] -= Lb
+        c[c < -Lb / 2] += Lb
     return dict(specs=specs, x=X[:, 0].copy(), y=X[:, 1].copy(), z=X[:, 2].copy(), vx=V[:, 0].copy(),
                 vy=V[:, 1].copy(), vz=V[:, 2].copy(), offsets=offsets, N=int(offsets[-1]))
 
@@ -136,9 +137,10 @@ def kg_chains(nchains, length=100, density=0.85, bond=0.97, seed=0, max_cos=-0.6
         pos[:, s] = pos[:, s - 1] + bond * v
         prev = v
     P = _wrap(pos.reshape(-1, 3), L).astype(np.float32)
-    for d in range(3):
+    for d in range(3):  # FP32 rounding may leave a coordinate a hair outside [-L/2, L/2)
         c = P[:, d]
         c[c >= np.float32(L / 2)] -= np.float32(L)
+        c[c < -np.float32(L / 2)] += np.float32(L)
     ids = np.arange(N).reshape(nchains, length)
     bonds = np.stack([ids[:, :-1].reshape(-1), ids[:, 1:].reshape(-1)], 1).astype(np.int32)
     return dict(x=P[:, 0].copy(), y=P[:, 1].copy(), z=P[:, 2].copy(), bonds=bonds, box=(L, L, L), N=N)

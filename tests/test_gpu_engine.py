@@ -10,13 +10,13 @@ from pdx import systems
 pytestmark = pytest.mark.gpu
 
 
-def _setup(seed=0):
+def _setup(seed=0, traversal="rows"):
     import torch
     s = systems.rock_salt(10, 2.82, 0.1, seed=seed)
     spec = pdx.PME(coulomb_prefactor=14.399645, real_cutoff=6.0, real_space_tolerance=1e-5, mesh_spacing=1.0,
                    order=5)
     eng = pdx.ForceEngine(s["N"], s["box"], pdx.LennardJones(0.01, 2.8, 6.0), electrostatics=spec,
-                          charges=s["q"], skin=0.5)
+                          charges=s["q"], skin=0.5, traversal=traversal)
     X, Y, Z, Q = (dev(s[k]) for k in ("x", "y", "z", "q"))
     bufs = [torch.empty(s["N"], device="cuda") for _ in range(4)] + [torch.empty(6, s["N"], device="cuda")]
     return s, eng, (X, Y, Z, Q), bufs
@@ -29,9 +29,10 @@ def _snap(eng, bufs):
     return [b.clone() for b in bufs], tot["energy"], list(tot["virial"])
 
 
-def test_overlap_and_graph_match_serial(cuda):
+@pytest.mark.parametrize("traversal", ["rows", "cells"])
+def test_overlap_and_graph_match_serial(cuda, traversal):
     import torch
-    s, eng, P, bufs = _setup()
+    s, eng, P, bufs = _setup(traversal=traversal)
     eng.evaluate(*P, *bufs, energy=True, overlap=False)
     ref, Eref, Wref = _snap(eng, bufs)
     eng.evaluate(*P, *bufs, energy=True, overlap=True)

@@ -14,7 +14,7 @@ from pdx import systems
 pytestmark = pytest.mark.gpu
 
 
-def _eval(x, y, z, box, model, cap=64, lanes=0, q=None, coulomb=None, skin=0.4, cutoff=None):
+def _eval(x, y, z, box, model, cap=64, lanes=0, q=None, coulomb=None, skin=0.4, cutoff=None, traversal="rows"):
     import torch
     N = len(x)
     X, Y, Z = dev(x), dev(y), dev(z)
@@ -22,7 +22,7 @@ def _eval(x, y, z, box, model, cap=64, lanes=0, q=None, coulomb=None, skin=0.4, 
     rc = cutoff if cutoff is not None else model.cutoff
     ws = pdx.DeviceNeighborWorkspace(N, box, rc, skin, cap)
     ws.update(X, Y, Z, Q)
-    pf = pdx.PairForce(model, coulomb=coulomb, lanes=lanes)
+    pf = pdx.PairForce(model, coulomb=coulomb, lanes=lanes, traversal=traversal)
     fx, fy, fz, E = (torch.empty(N, device="cuda") for _ in range(4))
     V = torch.empty(6, N, device="cuda")
     tot = torch.zeros(7, dtype=torch.float64, device="cuda")
@@ -150,3 +150,46 @@ def test_real_space_coulomb(cuda):
     assert abs(tot[0] - e.sum()) / abs(e.sum()) < 1e-5
     info = pdx.PairForce(None, coulomb=coul).info()
     assert info["force_fit_error"] < 1e-6
+
+
+@pytest.mark.parametrize("kind", ["lj", "wca"])
+def test_cell_traversal_against_direct_sum(cuda, kind):
+    """The list-free cell traversal on the LJ fluid against the FP64 direct sums."""
+    s = systems.fcc_lattice(4000, 0.8442, 0.05, seed=2)
+    model = pdx.LennardJones() if kind == "lj" else pdx.WCA()
+    cap = pdx.default_capacity(4000, s["box"], model.cutoff + 0.4)
+    F, E, V, tot = _eval(s["x"], s["y"], s["z"], s["box"], model, cap=cap, traversal="cells")
+    ref = oracle.pair_direct(s["x"], s["y"], s["z"], s["box"], kind=kind)
+    assert _relrms(F, ref["F"]) < 2e-5
+    assert abs(tot[0] - ref["energy"].sum()) < 1e-5 * abs(ref["energy"].sum())
+    assert np.allclose(tot[1:], ref["virial"].sum(0), rtol=1e-5, atol=1e-5 * abs(ref["virial"][:, 0].sum()))
+    assert np.allclose(E, ref["energy"], rtol=1e-4, atol=1e-5)
+
+
+def test_cell_traversal_matches_rows_with_coulomb(cuda):
+    """Cells and rows visit the same pairs: same forces and totals up to the FP32 sum order;
+    the cell traversal is deterministic (two runs give the same bits)."""
+    s = systems.rock_salt(12, 2.82, 0.1, seed=5)
+    coul = pdx.RealSpaceCoulomb(C=14.399645, cutoff=8.0, alpha=0.39)
+    lj = pdx.LennardJones(0.01, 2.8, 8.0)
+    cap = pdx.default_capacity(s["N"], s["box"], 8.5)
+    args = (s["x"], s["y"], s["z"], s["box"], lj)
+    F1, E1, V1, t1 = _eval(*args, cap=cap, q=s["q"], coulomb=coul, skin=0.5)
+    F2, E2, V2, t2 = _eval(*args, cap=cap, q=s["q"], coulomb=coul, skin=0.5, traversal="cells")
+    F3, E3, V3, t3 = _eval(*args, cap=cap, q=s["q"], coulomb=coul, skin=0.5, traversal="cells")
+    print(f"[cells] rel rms vs rows {_relrms(F2, F1):.2e}", flush=True)
+    assert _relrms(F2, F1) < 1e-6
+    assert np.allclose(t1, t2, rtol=1e-6, atol=1e-6 * abs(t1[0]))
+    assert np.allclose(E2, E1, rtol=1e-5, atol=1e-6) and np.allclose(V2, V1, rtol=1e-4, atol=1e-4)
+    assert np.array_equal(F2, F3) and np.array_equal(t2, t3)
+
+
+def test_cell_traversal_restrictions(cuda):
+    with pytest.raises(pdx.ArgumentError):
+        pdx.PairForce(pdx.WCA(), traversal="tiles")
+    import torch
+    p = pdx.PairForce(pdx.WCA(), traversal="cells", accumulation="fixed64")
+    nl = pdx.DeviceNeighborWorkspace(8, (12.0, 12.0, 12.0), 1.2, 0.3, 64)
+    x = torch.zeros(8, device="cuda")
+    with pytest.raises(pdx.ArgumentError):
+        p.compute(nl, x, x, x, energy=False)
