@@ -181,12 +181,11 @@ __device__ void block_partials(const double (&v)[NV], double *partial) {
   }
 }
 
-// One block of 256 threads: out[c] (=, or += with add) = sum over blocks of partial[b*NV + c],
+// One block of 256 threads (the whole block calls it): out[c] (=, or += with add) = sum over
+// blocks of partial[b*NV + c],
 // thread t taking blocks t, t + 256, ... and a fixed tree over the threads.
 template <int NV>
-__global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ partial, int nblocks, double *out,
-                                                      int add, const int64_t *state) {
-  if (state && !(state[PDX_DN_VALID] != 0 && state[PDX_DN_ERROR] == 0)) return;
+__device__ void sum_partials_block(const double *__restrict__ partial, int nblocks, double *out, int add) {
   __shared__ double sh[NV][4];
   double acc[NV];
 #pragma unroll
@@ -206,6 +205,13 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__
     const double s = ((sh[threadIdx.x][0] + sh[threadIdx.x][1]) + sh[threadIdx.x][2]) + sh[threadIdx.x][3];
     out[threadIdx.x] = add ? out[threadIdx.x] + s : s;
   }
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ partial, int nblocks, double *out,
+                                                      int add, const int64_t *state) {
+  if (state && !(state[PDX_DN_VALID] != 0 && state[PDX_DN_ERROR] == 0)) return;
+  sum_partials_block<NV>(partial, nblocks, out, add);
 }
 
 }  // namespace pdx

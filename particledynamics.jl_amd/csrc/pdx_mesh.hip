@@ -319,8 +319,13 @@ __global__ __launch_bounds__(TPB) void k_fft_x_c2r(const float2 *spec, float *ou
 template <int T, bool BIG>
 __global__ __launch_bounds__(TPB) void k_fft_y(const float2 *src, float2 *spec, int K1h, int K2, FFTPlan pl,
                                                const float2 *__restrict__ tw, float sg, int ikdim, float kscale,
-                                               int K1) {
+                                               int K1, const double *psum, int npsum, double *psum_out) {
   extern __shared__ float2 lds[];
+  // block 0 first adds the z pass's energy / virial partials (one launch fewer on the path)
+  if (psum && blockIdx.x == 0) {
+    sum_partials_block<7>(psum, npsum, psum_out, 0);
+    __syncthreads();
+  }
   float2 *twl = lds, *b0 = lds + K2, *b1 = b0 + (size_t)T * K2;
   load_twiddles(twl, tw, K2);
   const int tiles = (K1h + T - 1) / T;
@@ -693,6 +698,17 @@ __global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__
   }
 }
 
+// Workgroups are dispatched to the 8 XCDs round robin (block b runs on XCD b mod 8), each XCD
+// with its own L2. With `xcd` set, the blocks that share an XCD take a contiguous range of the
+// logical block order, so one XCD's particles (consecutive in storage order, which for lattice-
+// ordered or cell-sorted particles is a spatial slab) touch one slab of the grid.
+__device__ __forceinline__ int xcd_block(int xcd) {
+  const int b = blockIdx.x, nb = gridDim.x;
+  if (!xcd) return b;
+  const int x = b & 7, r = b >> 3, q = nb >> 3, rem = nb & 7;
+  return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + r;
+}
+
 // ---- gather: same lane layout; line partial sums reduced over the particle's lanes.
 template <int P, bool ENERGY>
 __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__ x, const float *__restrict__ y,
@@ -700,9 +716,9 @@ __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__
                                                 const int32_t *__restrict__ order, const float *__restrict__ pot,
                                                 int K1, int K2, int K3, float Lx, float Ly, float Lz,
                                                 float *fx, float *fy, float *fz, float *Epot, float bg, int set,
-                                                float *Vglobal, const double *acc, double ubg) {
+                                                float *Vglobal, const double *acc, double ubg, int xcd) {
   constexpr int LPP = lanes_for(P);
-  const int tid = blockIdx.x * TPB + threadIdx.x;
+  const int tid = xcd_block(xcd) * TPB + threadIdx.x;
   if (Vglobal && tid < 6) Vglobal[tid] = (float)(acc[1 + tid] + (tid < 3 ? ubg : 0.0));
   const int t = tid / LPP;
   const int lane = tid % LPP;
@@ -1078,9 +1094,9 @@ Plans plans_of(pdx_mesh_t m) {
   make_plan(m->K2, pl.p2);
   make_plan(m->K3, pl.p3);
   if (m->T1 == 0) {
-    m->T1 = lines_per_tile(m->K1, 8, "PDX_FFT_TX");
-    m->T2 = lines_per_tile(m->K2, 16, "PDX_FFT_TY");
-    m->T3 = lines_per_tile(m->K3, 16, "PDX_FFT_TZ");
+    m->T1 = lines_per_tile(m->K1, 4, "PDX_FFT_TX");
+    m->T2 = lines_per_tile(m->K2, 8, "PDX_FFT_TY");
+    m->T3 = lines_per_tile(m->K3, 8, "PDX_FFT_TZ");
   }
   pl.T1 = m->T1;
   pl.T2 = m->T2;
@@ -1138,10 +1154,12 @@ int enqueue_forward_xy(pdx_mesh_t m, const Plans &pl, const void *src, bool fixe
 #define PDX_YF(TT)                                                                                         \
   if (pl.p2.big)                                                                                             \
     hipLaunchKernelGGL((k_fft_y<TT, true>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, m->spec_c.p,          \
-                       m->spec_c.p, m->K1h, m->K2, pl.p2, tw2, -1.0f, 0, 0.0f, m->K1);                        \
+                       m->spec_c.p, m->K1h, m->K2, pl.p2, tw2, -1.0f, 0, 0.0f, m->K1, nullptr, 0,      \
+                       nullptr);                        \
   else                                                                                                       \
     hipLaunchKernelGGL((k_fft_y<TT, false>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, m->spec_c.p,         \
-                       m->spec_c.p, m->K1h, m->K2, pl.p2, tw2, -1.0f, 0, 0.0f, m->K1);
+                       m->spec_c.p, m->K1h, m->K2, pl.p2, tw2, -1.0f, 0, 0.0f, m->K1, nullptr, 0,      \
+                       nullptr);
   PDX_T_SWITCH(pl.T2, PDX_YF)
 #undef PDX_YF
   PDX_LAUNCH_CHECK();
@@ -1151,17 +1169,20 @@ int enqueue_forward_xy(pdx_mesh_t m, const Plans &pl, const void *src, bool fixe
 // Inverse y pass from `src` into `spec` (in place when equal; times -i k_d first when ikdim is 1
 // or 2), then the inverse x pass of `spec` into the real grid `dst`.
 int enqueue_inverse_yx(pdx_mesh_t m, const Plans &pl, const float2 *src, float2 *spec, float *dst, int ikdim,
-                       hipStream_t st) {
+                       hipStream_t st, bool with_sum = false) {
+  const double *ps = with_sum ? m->partial.p : nullptr;
+  double *po = with_sum ? m->acc.p : nullptr;
+  const int nps = m->nmode_blocks;
   const float2 *tw1 = m->tw.p, *tw2 = m->tw.p + m->K1;
   const int gy = ((m->K1h + pl.T2 - 1) / pl.T2) * m->K3;
   const float kscale = ikdim == 1 ? (float)(2 * M_PI / m->box[0]) : (float)(2 * M_PI / m->box[1]);
 #define PDX_YI(TT)                                                                                         \
   if (pl.p2.big)                                                                                             \
     hipLaunchKernelGGL((k_fft_y<TT, true>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, src, spec, m->K1h,    \
-                       m->K2, pl.p2, tw2, +1.0f, ikdim, kscale, m->K1);                                      \
+                       m->K2, pl.p2, tw2, +1.0f, ikdim, kscale, m->K1, ps, nps, po);                                      \
   else                                                                                                       \
     hipLaunchKernelGGL((k_fft_y<TT, false>), dim3(gy), dim3(TPB), fft_lds(TT, m->K2), st, src, spec, m->K1h,   \
-                       m->K2, pl.p2, tw2, +1.0f, ikdim, kscale, m->K1);
+                       m->K2, pl.p2, tw2, +1.0f, ikdim, kscale, m->K1, ps, nps, po);
   PDX_T_SWITCH(pl.T2, PDX_YI)
 #undef PDX_YI
   const int nlines = m->K2 * m->K3;
@@ -1195,9 +1216,6 @@ void launch_z(pdx_mesh_t m, const Plans &pl, hipStream_t st, float2 *spec = null
                        spec, pl.p3, tw3, sa);
   PDX_T_SWITCH(pl.T3, PDX_ZZ)
 #undef PDX_ZZ
-  if (MODE == 2 && ENERGY)  // mesh energy and virial: the z blocks' partials in block order
-    hipLaunchKernelGGL(k_sum_partials<7>, dim3(1), dim3(256), 0, st, (const double *)m->partial.p, gz, m->acc.p, 0,
-                       (const int64_t *)nullptr);
 }
 
 #define PDX_SPREAD_CASE(PP)                                                                                    \
@@ -1211,11 +1229,11 @@ void launch_z(pdx_mesh_t m, const Plans &pl, hipStream_t st, float2 *spec = null
     if (energy)                                                                                                 \
       hipLaunchKernelGGL((k_gather<PP, true>), dim3(blocks_of(PP)), dim3(TPB), 0, st, N, x, y, z, q, order,     \
                          m->pot.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], fx, fy, fz, Epot, m->bg, set,         \
-                         Vglobal, m->acc.p, m->ubg);                                                           \
+                         Vglobal, m->acc.p, m->ubg, xcd);                                                      \
     else                                                                                                        \
       hipLaunchKernelGGL((k_gather<PP, false>), dim3(blocks_of(PP)), dim3(TPB), 0, st, N, x, y, z, q, order,    \
                          m->pot.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], fx, fy, fz, Epot, m->bg, set,         \
-                         Vglobal, m->acc.p, m->ubg);                                                           \
+                         Vglobal, m->acc.p, m->ubg, xcd);                                                      \
     break;
 
 #define PDX_GATHER_IK_CASE(PP)                                                                                  \
@@ -1428,7 +1446,8 @@ extern "C" int pdx_mesh_spread_cells(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t str
   const float L[3] = {(float)m->box[0], (float)m->box[1], (float)m->box[2]};
   const float scale = (float)m->fixed_scale;
   auto ntiles = [&](int T) { return ((m->K1 + T - 1) / T) * ((m->K2 + T - 1) / T) * ((m->K3 + T - 1) / T); };
-  const bool small = ntiles(16) < 256;
+  bool small = ntiles(16) < 256;
+  if (const char *e = std::getenv("PDX_SPREAD_TILE")) small = std::atoi(e) == 8;
   const int tiles = ntiles(small ? 8 : 16);
   const float margin = 0.5f * (float)nl->skin;
   hipStream_t st = as_stream(stream);
@@ -1468,7 +1487,7 @@ extern "C" int pdx_mesh_solve(pdx_mesh_t m, int32_t flags, pdx_stream_t stream) 
     if (energy) launch_z<2, true>(m, pl, st);
     else launch_z<2, false>(m, pl, st);
     PDX_LAUNCH_CHECK();
-    return enqueue_inverse_yx(m, pl, m->spec_c.p, m->spec_c.p, m->pot.p, 0, st);
+    return enqueue_inverse_yx(m, pl, m->spec_c.p, m->spec_c.p, m->pot.p, 0, st, energy);
   }
   // ik: G b is kept for the z field; E_d = IFFT(-i k_d G b) for d = z, x, y, and the potential
   // (three field transforms, plus the potential when energies are wanted)
@@ -1476,7 +1495,7 @@ extern "C" int pdx_mesh_solve(pdx_mesh_t m, int32_t flags, pdx_stream_t stream) 
   else launch_z<2, false>(m, pl, st, nullptr, m->spec_ez.p);
   launch_z<3, false>(m, pl, st, m->spec_ez.p);
   PDX_LAUNCH_CHECK();
-  if ((rc = enqueue_inverse_yx(m, pl, m->spec_ez.p, m->spec_ez.p, m->ez.p, 0, st)) ||
+  if ((rc = enqueue_inverse_yx(m, pl, m->spec_ez.p, m->spec_ez.p, m->ez.p, 0, st, energy)) ||
       (rc = enqueue_inverse_yx(m, pl, m->spec_c.p, m->spec_tmp.p, m->ex.p, 1, st)) ||
       (rc = enqueue_inverse_yx(m, pl, m->spec_c.p, m->spec_tmp.p, m->ey.p, 2, st)))
     return rc;
@@ -1495,6 +1514,8 @@ extern "C" int pdx_mesh_gather(pdx_mesh_t m, const float *x, const float *y, con
   if (!energy) Vglobal = nullptr;
   hipStream_t st = as_stream(stream);
   auto blocks_of = [&](int P) { return (int)(((int64_t)N * lanes_for(P) + TPB - 1) / TPB); };
+  int xcd = 1;
+  if (const char *e = std::getenv("PDX_GATHER_XCD")) xcd = std::atoi(e);
   if (m->spec.influence == PDX_INFLUENCE_P3M_IK) {
     if (energy && !m->pot_valid)
       return fail(PDX_ERR_ARGUMENT, "ik gather with energies needs a solve with PDX_ENERGY first");

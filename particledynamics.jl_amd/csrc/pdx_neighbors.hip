@@ -3,15 +3,18 @@
 // Pipeline of one predicated rebuild (all kernels are enqueued every time; each returns at
 // once unless the device state requests a rebuild, so the sequence can be captured in a
 // hipGraph — the "predicated reference path" of src/neighbors/DeviceNeighborCUDA.jl:413-420):
-//   k_decide       coverage check: max |dr|^2 vs skin^2/4, adaptive interval, packing of
-//                  (x, y, z, q), zeroing of the cell histogram (one launch, last-block decide)
+//   k_decide       coverage check: per-block max |dr|^2, packing of (x, y, z, q), zeroing of
+//                  the cell histogram
+//   k_decide_final max over the blocks vs skin^2/4, age, adaptive interval (one block)
 //   k_bin          cell id + histogram; the atomic's return value is the particle's slot
-//   k_scan_local / k_scan_blocks / k_scan_finish   exclusive scan -> 1-based cell_offsets
+//   k_scan_single  (or k_scan_local / k_scan_blocks / k_scan_finish above 16384 cells)
+//                  exclusive scan -> 1-based cell_offsets
 //   k_scatter      particle_ids_sorted (1-based)
 //   k_sort_cells   ascending id inside each cell (rows and traversal become reproducible),
 //                  and the cell-ordered packed positions read by the row kernel
 //   k_rows         27-cell stencil, r^2 <= rlist^2, slot-major rows t*N + i, 1-based ids
-//   k_commit       max row length vs cap, state words (last-block commit)
+//   k_commit       per-block max row length and statistics
+//   k_commit_final max row length vs cap, state words (one wave)
 //   k_copy_refs    reference positions of the successful build
 #include "pdx_internal.hpp"
 
@@ -144,6 +147,48 @@ __global__ __launch_bounds__(TPB) void k_bin(int N, const float4 *__restrict__ p
   const int c = (cz * ny + cy) * nx + cx;
   cell_of_particle[i] = c;
   cell_rank[i] = atomicAdd(&cell_counts[c], 1);
+}
+
+// Reference positions of a successful build (skipped when the commit latched an error).
+__global__ __launch_bounds__(TPB) void k_copy_refs(int N, const float4 *__restrict__ packed,
+                                                    float *rref, const int64_t *state) {
+  if (!rebuild_requested(state)) return;
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i >= N) return;
+  const float4 p = packed[i];
+  rref[i] = p.x;
+  rref[(size_t)N + i] = p.y;
+  rref[(size_t)2 * N + i] = p.z;
+}
+
+// Cell offsets in one launch for grids of up to 16384 cells: 1024 threads, a contiguous run of
+// at most 16 cells each, wave scans of the run totals; 1-based, last entry N+1
+// (_device_offsets_kernel!, DeviceNeighborCUDA.jl:280-289).
+constexpr int SCAN_SINGLE_MAX = 16384;
+__global__ __launch_bounds__(1024) void k_scan_single(int ncell, int N, const int32_t *__restrict__ counts,
+                                                       int32_t *offsets, const int64_t *state) {
+  if (!rebuild_requested(state)) return;
+  __shared__ int32_t wtot[16];
+  const int per = (ncell + 1023) / 1024;
+  const int c0 = min(threadIdx.x * per, ncell), c1 = min(c0 + per, ncell);
+  int32_t run = 0;
+  for (int c = c0; c < c1; ++c) run += counts[c];
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  int32_t v = run;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const int32_t u = __shfl_up(v, o, WAVE);
+    if (lane >= o) v += u;
+  }
+  if (lane == WAVE - 1) wtot[w] = v;
+  __syncthreads();
+  int32_t base = v - run + 1;
+  for (int k = 0; k < w; ++k) base += wtot[k];
+  for (int c = c0; c < c1; ++c) {
+    offsets[c] = base;
+    base += counts[c];
+  }
+  if (threadIdx.x == 0) offsets[ncell] = N + 1;
 }
 
 // Exclusive scan of cell_counts, SCAN_BLOCK cells per block (block-local part).
@@ -450,16 +495,6 @@ __global__ __launch_bounds__(WAVE) void k_commit_final(int nblocks, int cap, con
   }
 }
 
-__global__ __launch_bounds__(TPB) void k_copy_refs(int N, const float4 *__restrict__ packed,
-                                                    float *rref, const int64_t *state) {
-  if (!rebuild_requested(state)) return;
-  const int i = blockIdx.x * TPB + threadIdx.x;
-  if (i >= N) return;
-  const float4 p = packed[i];
-  rref[i] = p.x;
-  rref[(size_t)N + i] = p.y;
-  rref[(size_t)2 * N + i] = p.z;
-}
 
 __global__ void k_advance(int64_t *state) {
   if (state[PDX_DN_VALID] != 0 && state[PDX_DN_ERROR] == 0) state[PDX_DN_STEP] += 1;
@@ -494,11 +529,16 @@ int enqueue_rebuild(pdx_nl_t nl, hipStream_t st) {
     const int sblocks = blocks_for(nl->ncell, SCAN_BLOCK);
     hipLaunchKernelGGL(k_bin, dim3(nb), dim3(TPB), 0, st, N, nl->packed.p, geom, nl->nx, nl->ny, nl->nz,
                        nl->cell_of_particle.p, nl->cell_counts.p, nl->cell_rank.p, s);
-    hipLaunchKernelGGL(k_scan_local, dim3(sblocks), dim3(TPB), 0, st, nl->ncell, nl->cell_counts.p,
-                       nl->cell_offsets.p, nl->scan_block.p, s);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, sblocks, nl->scan_block.p, s);
-    hipLaunchKernelGGL(k_scan_finish, dim3(blocks_for(nl->ncell + 1, TPB)), dim3(TPB), 0, st, nl->ncell, N,
-                       nl->cell_offsets.p, nl->scan_block.p, s);
+    if (nl->ncell <= SCAN_SINGLE_MAX) {
+      hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, nl->ncell, N, nl->cell_counts.p,
+                         nl->cell_offsets.p, s);
+    } else {
+      hipLaunchKernelGGL(k_scan_local, dim3(sblocks), dim3(TPB), 0, st, nl->ncell, nl->cell_counts.p,
+                         nl->cell_offsets.p, nl->scan_block.p, s);
+      hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, sblocks, nl->scan_block.p, s);
+      hipLaunchKernelGGL(k_scan_finish, dim3(blocks_for(nl->ncell + 1, TPB)), dim3(TPB), 0, st, nl->ncell, N,
+                         nl->cell_offsets.p, nl->scan_block.p, s);
+    }
     hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(TPB), 0, st, N, nl->cell_of_particle.p, nl->cell_rank.p,
                        nl->cell_offsets.p, nl->particle_ids_sorted.p, s);
     hipLaunchKernelGGL(k_sort_cells, dim3(nl->ncell), dim3(WAVE), 0, st, nl->ncell, nl->cell_offsets.p,
