@@ -524,6 +524,9 @@ __device__ __forceinline__ float sel(const float (&v)[P], int j) {
 }
 
 constexpr int lanes_for(int P) { return P * P <= 16 ? 16 : (P * P <= 32 ? 32 : 64); }
+// gather: 8 lanes per particle, each walking several (x, y) lines, so the B-spline weights are
+// computed by 8 lanes instead of P^2 and more loads are in flight per lane
+constexpr int gather_lanes(int) { return 8; }
 
 // ---- spread: LPP lanes per particle, lane l owns the (x, y) line (l % P, l / P) of the
 // particle's P x P x P stencil and walks its P z nodes with 64-bit integer atomics (the
@@ -573,7 +576,8 @@ __global__ void k_zero64(unsigned long long *g, size_t n) {
 // 64-bit fixed point with LDS integer atomics (exact, order-independent) and stores the
 // block with plain writes: no global atomics, and no clearing (every node is written).
 // Tile edge: 16 nodes, or 8 on meshes too small to give 256 tiles of 16 (one per CU).
-constexpr int SLIST = 1536;  // particles reaching one block staged in LDS (a few hundred in practice)
+constexpr int SLIST = 768;   // particles reaching one block staged in LDS (a few hundred in practice)
+constexpr int SCHUNK = 64;   // staged particles whose weights are computed together
 template <int P, int STILE>
 __global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__ packed,
                                                       const int32_t *__restrict__ offsets,
@@ -583,7 +587,9 @@ __global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__
                                                       int K2, int K3, float Lx, float Ly, float Lz, float scale) {
   if (state[PDX_DN_ERROR] != 0) return;  // no usable cells / positions: the grid keeps its last value
   constexpr int LPP = lanes_for(P);
-  __shared__ unsigned long long acc[STILE * STILE * STILE];
+  // x rows padded to STILE + 1 nodes: the P x P lanes of a particle then spread over the LDS banks
+  constexpr int SROW = STILE + 1;
+  __shared__ unsigned long long acc[STILE * STILE * SROW];
   __shared__ float4 list[SLIST];
   __shared__ int cstart[64], cpre[65];
   __shared__ int nlist;
@@ -591,7 +597,7 @@ __global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__
   const int b = blockIdx.x;
   const int a1 = (b % tx) * STILE, a2 = ((b / tx) % ty) * STILE, a3 = (b / (tx * ty)) * STILE;
   const int e1 = min(STILE, K1 - a1), e2 = min(STILE, K2 - a2), e3 = min(STILE, K3 - a3);
-  for (int k = threadIdx.x; k < STILE * STILE * STILE; k += TPB) acc[k] = 0ull;
+  for (int k = threadIdx.x; k < STILE * STILE * SROW; k += TPB) acc[k] = 0ull;
   // candidate cells per dimension, in the list's frame (x + L/2) / L
   const int K[3] = {K1, K2, K3}, A[3] = {a1, a2, a3}, E[3] = {e1, e2, e3}, NC[3] = {nx, ny, nz};
   const float Ls[3] = {Lx, Ly, Lz};
@@ -658,7 +664,7 @@ __global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__
             for (int j3 = 0; j3 < P; ++j3) {
               int l3 = o3 - (P - 1) + j3; l3 += l3 < 0 ? K3 : 0; l3 -= l3 >= K3 ? K3 : 0;
               if (l3 >= e3) continue;
-              atomicAdd(&acc[(l3 * STILE + l2) * STILE + l1], (unsigned long long)__float2ll_rn(q12 * wz[j3]));
+              atomicAdd(&acc[(l3 * STILE + l2) * SROW + l1], (unsigned long long)__float2ll_rn(q12 * wz[j3]));
             }
           }
         }
@@ -666,27 +672,41 @@ __global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__
     }
     __syncthreads();
   }
+  // staged particles, SCHUNK at a time: their B-spline weights are computed once (three threads
+  // per particle, one dimension each) into LDS; then LPP lanes per particle, lane = one (x, y)
+  // line of the stencil walking P z nodes. The products and their order are those of
+  // k_spread: ((q w_x) w_y) 2^S, then times w_z.
   const int nl = min(nlist, SLIST);
-  // staged particles: LPP lanes each, lane = one (x, y) line of the stencil walking P z nodes;
-  // the products and their order are those of k_spread: ((q w_x) w_y) 2^S, then times w_z
+  __shared__ float wsh[SCHUNK][3][P];
+  __shared__ int ksh[SCHUNK][3];
   const int grp = threadIdx.x / LPP, lane = threadIdx.x % LPP;
-  for (int t = grp; t < nl; t += TPB / LPP) {
-    const float4 u = list[t];
-    float wx[P], wy[P], wz[P], d[P];
-    const int k1 = bspline<P>(u.x, wx, d);
-    const int k2 = bspline<P>(u.y, wy, d);
-    const int k3 = bspline<P>(u.z, wz, d);
-    for (int l = lane; l < P * P; l += LPP) {
-      const int j1 = l % P, j2 = l / P;
-      int l1 = wrap_node(k1 + j1, K1) - a1; l1 += l1 < 0 ? K1 : 0;
-      int l2 = wrap_node(k2 + j2, K2) - a2; l2 += l2 < 0 ? K2 : 0;
-      if (l1 >= e1 || l2 >= e2) continue;
-      const float q12 = u.w * sel<P>(wx, j1) * sel<P>(wy, j2) * scale;
+  for (int c0 = 0; c0 < nl; c0 += SCHUNK) {
+    const int nc = min(SCHUNK, nl - c0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 3 * nc; e += TPB) {
+      const int t = e / 3, d = e - 3 * t;
+      const float4 u = list[c0 + t];
+      float w[P], dw[P];
+      ksh[t][d] = bspline<P>(d == 0 ? u.x : (d == 1 ? u.y : u.z), w, dw);
 #pragma unroll
-      for (int j3 = 0; j3 < P; ++j3) {
-        int l3 = wrap_node(k3 + j3, K3) - a3; l3 += l3 < 0 ? K3 : 0;
-        if (l3 >= e3) continue;
-        atomicAdd(&acc[(l3 * STILE + l2) * STILE + l1], (unsigned long long)__float2ll_rn(q12 * wz[j3]));
+      for (int j = 0; j < P; ++j) wsh[t][d][j] = w[j];
+    }
+    __syncthreads();
+    for (int t = grp; t < nc; t += TPB / LPP) {
+      const float qv = list[c0 + t].w;
+      const int k1 = ksh[t][0], k2 = ksh[t][1], k3 = ksh[t][2];
+      for (int l = lane; l < P * P; l += LPP) {
+        const int j1 = l % P, j2 = l / P;
+        int l1 = wrap_node(k1 + j1, K1) - a1; l1 += l1 < 0 ? K1 : 0;
+        int l2 = wrap_node(k2 + j2, K2) - a2; l2 += l2 < 0 ? K2 : 0;
+        if (l1 >= e1 || l2 >= e2) continue;
+        const float q12 = qv * wsh[t][0][j1] * wsh[t][1][j2] * scale;
+#pragma unroll
+        for (int j3 = 0; j3 < P; ++j3) {
+          int l3 = wrap_node(k3 + j3, K3) - a3; l3 += l3 < 0 ? K3 : 0;
+          if (l3 >= e3) continue;
+          atomicAdd(&acc[(l3 * STILE + l2) * SROW + l1], (unsigned long long)__float2ll_rn(q12 * wsh[t][2][j3]));
+        }
       }
     }
   }
@@ -694,7 +714,8 @@ __global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__
   const size_t KK = (size_t)K1 * K2;
   for (int k = threadIdx.x; k < STILE * STILE * STILE; k += TPB) {
     const int l1 = k % STILE, l2 = (k / STILE) % STILE, l3 = k / (STILE * STILE);
-    if (l1 < e1 && l2 < e2 && l3 < e3) grid[(size_t)(a3 + l3) * KK + (size_t)(a2 + l2) * K1 + (a1 + l1)] = acc[k];
+    if (l1 < e1 && l2 < e2 && l3 < e3)
+      grid[(size_t)(a3 + l3) * KK + (size_t)(a2 + l2) * K1 + (a1 + l1)] = acc[(l3 * STILE + l2) * SROW + l1];
   }
 }
 
@@ -717,7 +738,7 @@ __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__
                                                 int K1, int K2, int K3, float Lx, float Ly, float Lz,
                                                 float *fx, float *fy, float *fz, float *Epot, float bg, int set,
                                                 float *Vglobal, const double *acc, double ubg, int xcd) {
-  constexpr int LPP = lanes_for(P);
+  constexpr int LPP = gather_lanes(P);
   const int tid = xcd_block(xcd) * TPB + threadIdx.x;
   if (Vglobal && tid < 6) Vglobal[tid] = (float)(acc[1 + tid] + (tid < 3 ? ubg : 0.0));
   const int t = tid / LPP;
@@ -780,7 +801,7 @@ __global__ __launch_bounds__(TPB) void k_gather_ik(int N, const float *__restric
                                                    float Ly, float Lz, float *fx, float *fy, float *fz, float *Epot,
                                                    float bg, int set, float *Vglobal, const double *acc,
                                                    double ubg) {
-  constexpr int LPP = lanes_for(P);
+  constexpr int LPP = gather_lanes(P);
   const int tid = blockIdx.x * TPB + threadIdx.x;
   if (Vglobal && tid < 6) Vglobal[tid] = (float)(acc[1 + tid] + (tid < 3 ? ubg : 0.0));
   const int t = tid / LPP;
@@ -1513,7 +1534,7 @@ extern "C" int pdx_mesh_gather(pdx_mesh_t m, const float *x, const float *y, con
   const bool energy = (flags & PDX_ENERGY) != 0;
   if (!energy) Vglobal = nullptr;
   hipStream_t st = as_stream(stream);
-  auto blocks_of = [&](int P) { return (int)(((int64_t)N * lanes_for(P) + TPB - 1) / TPB); };
+  auto blocks_of = [&](int P) { return (int)(((int64_t)N * gather_lanes(P) + TPB - 1) / TPB); };
   int xcd = 1;
   if (const char *e = std::getenv("PDX_GATHER_XCD")) xcd = std::atoi(e);
   if (m->spec.influence == PDX_INFLUENCE_P3M_IK) {
