@@ -954,9 +954,10 @@ extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, 
   }
   int G = p->lanes;
   if (G == 0) {
-    // about 2^17 threads: every CU busy while a wave's row slots stay in few cache lines
-    // (measured on the 64 000-ion system: 2 lanes beat 1 and 4, and every split layout)
-    const int64_t want = std::max<int64_t>(1, 131072 / std::max(N, 1));
+    // about 2^18 threads: enough waves per SIMD to hide the neighbour gathers while a wave's
+    // row slots stay in few cache lines (measured on the 64 000-ion system: 4 lanes beat 2,
+    // 8 and 16, and every split layout)
+    const int64_t want = std::max<int64_t>(1, 262144 / std::max(N, 1));
     G = 1;
     while (G * 2 <= want && G < 64) G <<= 1;
   }
