@@ -299,6 +299,75 @@ int pdx_pair_set_bonds(pdx_pair_t p, pdx_bonds_t b);
 int pdx_bonds_status(pdx_bonds_t b, int32_t *first_invalid, int32_t reset, pdx_stream_t stream);
 
 /* ======================================================================= */
+/* Cluster-pair lists and the Newton-3 tile kernel                         */
+/* ======================================================================= */
+/* The cluster-pair nonbonded engine (ClusterPairList / ClusterPairEngine,
+ * src/clusterpairs/List.jl:112-179, Engine.jl:5-75): the short-range forces of a pair model
+ * created by pdx_pair_create (LJ / WCA / soft, real-space Coulomb polynomial, Coulomb
+ * exceptions) evaluated over 8 x 8 cluster tiles, each unique pair once.
+ *   slots:   super-cluster S owns slots 64S .. 64S+63 (eight clusters of eight); padding
+ *            slots hold perm = -1 (List.jl:4-8);
+ *   outer:   per super-cluster, the j-clusters (cluster index >= the i-cluster's) whose
+ *            bounding box comes within rlist = rmax + skin of an i-cluster, with an 8-bit
+ *            i-cluster mask, in a segment of `entry_capacity` entries (List.jl:10-13);
+ *   working: the outer entries pruned to rmax + prune_buffer by the force pass itself
+ *            (List.jl:183-188, Kernels.jl:349-394).
+ * Rebuild when a particle moved more than skin/2 since the build, prune when one moved more
+ * than prune_buffer/2 since the last prune (Engine.jl:5-12); both decided on the device. */
+typedef struct pdx_cluster_s *pdx_cluster_t;
+
+/* Device pointers into a cluster list (read-only for callers). */
+typedef struct {
+  int32_t *perm;      /* nslot: slot -> particle (0-based), -1 padding */
+  int32_t *slot_of;   /* N: particle -> slot */
+  float *packed;      /* nslot float4 (x, y, z, q) in slot order, NaN coordinates on padding */
+  int32_t *n_outer;   /* NSmax: outer entries per super-cluster */
+  int32_t *outer;     /* NSmax * entry_capacity: (cj << 8) | i-cluster mask */
+  int32_t *control;   /* 16 control words (PDX_CL_*) */
+} pdx_cluster_arrays;
+
+#define PDX_CL_NS      0  /* super-clusters in use */
+#define PDX_CL_ERROR   1  /* 0, 1 outer-list capacity exceeded, 2 non-finite coordinates */
+#define PDX_CL_REQUIRED 2 /* entries the largest super-cluster needed after an overflow */
+#define PDX_CL_BUILDS  6  /* builds so far */
+#define PDX_CL_PRUNES  7  /* pruning passes so far (each build is followed by one) */
+#define PDX_CL_OUTER   8  /* outer entries of the last build */
+
+/* List for n particles in a centred orthorhombic box (ClusterPairEngine(...), Engine.jl:50-95;
+ * setup!, List.jl:355-418): rmax is the larger of the pair model's cutoffs; every box side
+ * must exceed 2 (rmax + skin); 0 <= prune_buffer <= skin. entry_capacity: outer entries per
+ * super-cluster (0: an estimate from the density). waves: wave64s per super-cluster in the
+ * force pass, 1, 2, 4 or 8 (0: 4). The pair model must outlive the list. */
+int pdx_cluster_create(int32_t n, const double box[3], pdx_pair_t model, double skin, double prune_buffer,
+                       int32_t entry_capacity, int32_t waves, pdx_cluster_t *out);
+int pdx_cluster_destroy(pdx_cluster_t c);
+/* out: N, column grid x, y, z bins, super-cluster bound NSmax, slots, entry capacity, waves. */
+int pdx_cluster_info(pdx_cluster_t c, int64_t out[8]);
+int pdx_cluster_arrays_get(pdx_cluster_t c, pdx_cluster_arrays *out);
+/* Keep the list valid for the current coordinates and pack them in slot order
+ * (update_list!, Engine.jl:41-60): displacement decision, predicated build (clustering,
+ * bounding boxes, search), reference positions; graph-capturable. q may be NULL. */
+int pdx_cluster_update(pdx_cluster_t c, const float *x, const float *y, const float *z, const float *q,
+                       pdx_stream_t stream);
+/* Forces (and with PDX_ENERGY per-particle half energies, half virials and FP64 totals, the
+ * layout of pdx_pair_forces) of the coordinates of the last update (compute_forces!,
+ * Engine.jl:79-92): the pruning pass over the outer list when one is due, else the plain pass
+ * over the working list, then the scatter to storage order (_scatter_kernel!,
+ * Kernels.jl:207-240) with the Coulomb self energy and exceptions. PDX_ADD adds. */
+int pdx_cluster_forces(pdx_cluster_t c, float *fx, float *fy, float *fz, float *Epot, float *V, double *totals,
+                       int32_t flags, pdx_stream_t stream);
+/* The next update rebuilds (request_build!, List.jl:552-558). */
+int pdx_cluster_request_build(pdx_cluster_t c, pdx_stream_t stream);
+/* Synchronising read of the control words: out = NS, outer entries, working entries, builds,
+ * prunes, error, required capacity, capacity. With check, returns PDX_ERR_CAPACITY /
+ * PDX_ERR_COORDINATE for a latched error (forces of that step were not written). */
+int pdx_cluster_status(pdx_cluster_t c, int64_t out[8], int32_t check, pdx_stream_t stream);
+/* Replace the outer/working storage by segments of entry_capacity entries, clear a latched
+ * capacity error and request a build (ensure_capacity!, List.jl:434-450). Host boundary:
+ * synchronises the device. */
+int pdx_cluster_resize(pdx_cluster_t c, int32_t entry_capacity);
+
+/* ======================================================================= */
 /* Learned correction of coarse-mesh PME forces (BF16 MFMA MLP)            */
 /* ======================================================================= */
 /* No counterpart in the reference: the correction and its training are new. */

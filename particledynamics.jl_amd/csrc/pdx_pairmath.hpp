@@ -1,0 +1,62 @@
+// Per-pair physics shared by the ELL-row and cluster-tile traversals: the model terms of one
+// pair at squared distance r2, so both traversals evaluate every pair identically.
+#pragma once
+
+#include "pdx_internal.hpp"
+
+namespace pdx {
+
+// Horner with a compile-time length: the coefficients are uniform kernel arguments held in
+// scalar registers for the whole row loop (a runtime length would re-load them every step).
+template <int NP>
+__device__ __forceinline__ float horner(const float (&c)[NP], float x) {
+  float acc = c[NP - 1];
+#pragma unroll
+  for (int k = NP - 2; k >= 0; --k) acc = fmaf(acc, x, c[k]);
+  return acc;
+}
+
+// Per-visit physics shared by both traversal layouts: force factor f (force on i = f d) and
+// pair energy u of one row entry (LJ/WCA/soft + real-space Coulomb polynomial).
+// A: any argument block with the model fields eps, sig2, rc2, eshift, cC, crc2, csx, ceshift.
+template <int MODEL, bool COUL, bool ENERGY, int NP, typename A>
+__device__ __forceinline__ void pair_terms(const A &a, const float (&pe)[NP], const float (&pf)[NP],
+                                           float qi, float qj, float r2, float &f, float &u) {
+  f = 0.f;
+  u = 0.f;
+  if (MODEL == PDX_PAIR_LJ || MODEL == PDX_PAIR_WCA) {
+    const bool in = (r2 > 0.f) & (r2 < a.rc2);
+    const float invr2 = __builtin_amdgcn_rcpf(r2);
+    const float s2 = a.sig2 * invr2;
+    const float s6 = s2 * s2 * s2;
+    const float s12 = s6 * s6;
+    const float fl = 24.0f * a.eps * (2.0f * s12 - s6) * invr2;
+    f = in ? fl : 0.f;
+    if (ENERGY) {
+      const float ul = 4.0f * a.eps * (s12 - s6) + (MODEL == PDX_PAIR_WCA ? a.eps : -a.eshift);
+      u = in ? ul : 0.f;
+    }
+  } else if (MODEL == PDX_PAIR_SOFT) {
+    const bool in = (r2 > 0.f) & (r2 < a.rc2);
+    const float r = sqrtf(r2);
+    const float sg = sqrtf(a.sig2);
+    const float xx = 1.0f - r / sg;
+    f = in ? (a.eps / sg) * xx / r : 0.f;
+    if (ENERGY) u = in ? 0.5f * a.eps * xx * xx : 0.f;
+  }
+  if (COUL) {
+    const bool inr = (r2 > 0.f) & (r2 < a.crc2);
+    const float x = fmaf(r2, a.csx, -1.0f);
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float rinv3 = rinv * rinv * rinv;
+    const float cq = a.cC * qi * qj;
+    const float fc = cq * (rinv3 - horner<NP>(pf, x));
+    f += inr ? fc : 0.f;
+    if (ENERGY) {
+      const float uc = cq * (rinv + horner<NP>(pe, x) - a.ceshift);
+      u += inr ? uc : 0.f;
+    }
+  }
+}
+
+}  // namespace pdx

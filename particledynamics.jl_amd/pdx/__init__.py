@@ -11,9 +11,10 @@ from .electrostatics import (ElectrostaticsSpec, PME, PPPM, ElectrostaticsSolver
                              ewald_fp64)
 from .engine import ForceEngine, default_capacity
 from .replicas import ReplicaBatch
+from .clusters import ClusterPairEngine
 from . import systems
 
 __all__ = ["ArgumentError", "DeviceError", "DimensionMismatch", "NeighborCapacityError", "check", "last_error",
            "lib", "LIB_PATH", "DeviceNeighborWorkspace", "CellList", "build_neighbors_dense", "LennardJones", "WCA",
            "SoftRepulsive", "RealSpaceCoulomb", "PairForce", "FENEBonds", "DomainError", "ElectrostaticsSpec", "PME", "PPPM", "ElectrostaticsSolver",
-           "alpha_from_tolerance", "fft_friendly", "ewald_fp64", "ForceEngine", "default_capacity", "ReplicaBatch", "systems"]
+           "alpha_from_tolerance", "fft_friendly", "ewald_fp64", "ForceEngine", "default_capacity", "ReplicaBatch", "ClusterPairEngine", "systems"]
