@@ -32,6 +32,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-error", action="store_true", help="skip the FP64 Ewald force-error measurement")
     p.add_argument("--lanes", type=int, default=0)
+    p.add_argument("--no-clusters", action="store_true", help="skip the cluster-pair backend leg")
     p.add_argument("--skin", type=float, default=1.0)
     p.add_argument("--spread", choices=("cells", "atomic"), default="cells")
     p.add_argument("--traversal", choices=("rows", "cells"), default="rows",
@@ -233,9 +234,118 @@ def main():
         out["replicas"] = replica_leg()
     if not args.no_precision:
         out["precision"] = precision_leg(args.steps, args.warmup)
+    if not args.no_clusters:
+        out["clusters"] = cluster_leg(args.steps, args.warmup, args.skin, nl, X, Y, Z, pair_ms)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sysd, C)
     print(json.dumps(out))
+
+
+def _time_events(fn, reps, stream):
+    import torch
+    for _ in range(3):
+        fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def pairs_within(nl, X, Y, Z, box, rc):
+    """Unique pairs closer than rc, counted over the ELL rows of nl (GPU, torch)."""
+    import torch
+    N, cap = nl.N, nl.cap
+    rows = nl.neighbors_flat().view(cap, N).long() - 1
+    cnt = nl.counts().long()
+    P = torch.stack([X, Y, Z], 1).double()
+    Lb = torch.tensor(box, dtype=torch.float64, device="cuda")
+    total = 0
+    for t0 in range(0, cap, 32):
+        r = rows[t0:t0 + 32]
+        valid = (torch.arange(t0, min(t0 + 32, cap), device="cuda")[:, None] < cnt[None, :]) & (r >= 0)
+        d = P[r.clamp(min=0)] - P[None, :, :]
+        d -= Lb * torch.round(d / Lb)
+        total += int(((d * d).sum(-1) < rc * rc)[valid].sum().item())
+    return total // 2
+
+
+def cluster_leg(steps, warmup, skin, nl_rows, X, Y, Z, rows_pair_ms):
+    """The cluster-pair backend (8 x 8 Newton-3 tiles) on the same 64 000-ion PME evaluation, and
+    its force pass against the ELL rows' on that system and on the 4000-atom LJ fluid."""
+    import numpy as np
+    import torch
+
+    import pdx
+    from pdx import systems
+    sysd = systems.rock_salt(40, 2.82, 0.1, seed=0)
+    N, box = sysd["N"], sysd["box"]
+    C = 14.399645
+    spec = pdx.PME(coulomb_prefactor=C, real_cutoff=9.0, real_space_tolerance=1e-5, mesh_spacing=1.0, order=5)
+    lj = pdx.LennardJones(epsilon=0.01, sigma=2.8, cutoff=9.0)
+    eng = pdx.ForceEngine(N, box, lj, electrostatics=spec, charges=sysd["q"], skin=skin, backend="cluster")
+    Q = torch.from_numpy(sysd["q"]).cuda()
+    fx, fy, fz, E = (torch.empty(N, device="cuda") for _ in range(4))
+    V = torch.empty(6, N, device="cuda")
+    eng.capture(X, Y, Z, Q, fx, fy, fz, E, V, energy=True)
+    for _ in range(max(warmup, 1)):
+        eng.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.replay()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    eng.totals()
+    stream = torch.cuda.current_stream()
+    cl = eng.cluster
+    tot = eng.pair_totals
+    pass_ms = _time_events(lambda: cl.forces(fx, fy, fz, E, V, tot, energy=True), 50, stream)
+    pass_f_ms = _time_events(lambda: cl.forces(fx, fy, fz, energy=False), 50, stream)
+    upd_ms = _time_events(lambda: cl.update(X, Y, Z, Q), 50, stream)
+
+    def rebuild():
+        cl.request_build()
+        cl.update(X, Y, Z, Q)
+        cl.forces(fx, fy, fz, E, V, tot, energy=True)
+    build_ms = _time_events(rebuild, 10, stream)
+    tc = cl.tile_counts()
+    inrc = pairs_within(nl_rows, X, Y, Z, box, 9.0)
+    evals = 64 * tc["working_tiles"]
+    fpp = 109
+    c2 = {"ms_per_eval": round(ms, 4), "evals_per_s": round(1e3 / ms, 2), "pair_pass_ms": round(pass_ms, 4),
+          "pair_pass_force_only_ms": round(pass_f_ms, 4), "update_ms": round(upd_ms, 4),
+          "build_prune_ms": round(build_ms, 4), "rows_pair_pass_ms": round(rows_pair_ms, 4),
+          "outer_entries": tc["outer"], "working_entries": tc["working"], "working_tiles": tc["working_tiles"],
+          "tile_pair_evaluations": evals, "unique_pairs_within_cutoff": inrc,
+          "tile_efficiency": round(inrc / max(evals, 1), 4),
+          "tile_tflops_evaluated": round(evals * fpp / (pass_ms * 1e-3) / 1e12, 3),
+          "tile_tflops_in_cutoff": round(inrc * fpp / (pass_ms * 1e-3) / 1e12, 3),
+          "flop_per_pair": fpp, "skin": skin, "prune_buffer": cl.prune_buffer, "waves": cl.info["waves"]}
+    # the 4000-atom LJ fluid (reduced units, rc 2.5, skin 0.4): rows against tiles, force pass with energies
+    s1 = systems.fcc_lattice(4000, 0.8442, 0.05, seed=0)
+    X1, Y1, Z1 = (torch.from_numpy(s1[k]).cuda() for k in ("x", "y", "z"))
+    pf1 = pdx.PairForce(pdx.LennardJones())
+    n1 = 4000
+    g = [torch.empty(n1, device="cuda") for _ in range(4)]
+    V1 = torch.empty(6, n1, device="cuda")
+    t1 = torch.zeros(7, dtype=torch.float64, device="cuda")
+    ws = pdx.DeviceNeighborWorkspace(n1, s1["box"], 2.5, 0.4, pdx.default_capacity(n1, s1["box"], 2.9))
+    ws.update(X1, Y1, Z1)
+    rows1 = _time_events(lambda: pf1.compute(ws, g[0], g[1], g[2], g[3], V1, t1, energy=True), 50, stream)
+    cl1 = pdx.ClusterPairEngine(n1, s1["box"], pf1, 0.4)
+    cl1.update(X1, Y1, Z1)
+    cl1.forces(g[0], g[1], g[2], g[3], V1, t1, energy=True)
+    tiles1 = _time_events(lambda: cl1.forces(g[0], g[1], g[2], g[3], V1, t1, energy=True), 50, stream)
+    tc1 = cl1.tile_counts()
+    c1 = {"rows_pair_pass_ms": round(rows1, 4), "cluster_pair_pass_ms": round(tiles1, 4),
+          "working_tiles": tc1["working_tiles"], "tile_pair_evaluations": 64 * tc1["working_tiles"]}
+    del eng
+    return {"c2": c2, "c1_lj_fluid_4000": c1,
+            "note": "cluster tiles evaluate every pair of an 8 x 8 tile; rows visit only list entries"}
 
 
 def coulomb_error(X, Y, Z, Q, box, spec, C, skin, correction=None, nsample=256):

@@ -324,6 +324,11 @@ typedef struct {
   int32_t *n_outer;   /* NSmax: outer entries per super-cluster */
   int32_t *outer;     /* NSmax * entry_capacity: (cj << 8) | i-cluster mask */
   int32_t *control;   /* 16 control words (PDX_CL_*) */
+  int32_t *order;     /* N: 1-based ids in slot order (the mesh traversal order, mesh_order) */
+  int32_t *working;   /* NSmax * entry_capacity: working entries (cj << 8) | pruned i-cluster mask; chunk k
+                         of super-cluster S (of K = waves * splits) holds n_working[S*K + k] of them from
+                         the start of its share floor(n_outer[S] k / K) of S's segment */
+  int32_t *n_working; /* NSmax * K */
 } pdx_cluster_arrays;
 
 #define PDX_CL_NS      0  /* super-clusters in use */
@@ -341,8 +346,15 @@ typedef struct {
 int pdx_cluster_create(int32_t n, const double box[3], pdx_pair_t model, double skin, double prune_buffer,
                        int32_t entry_capacity, int32_t waves, pdx_cluster_t *out);
 int pdx_cluster_destroy(pdx_cluster_t c);
-/* out: N, column grid x, y, z bins, super-cluster bound NSmax, slots, entry capacity, waves. */
-int pdx_cluster_info(pdx_cluster_t c, int64_t out[8]);
+/* out: N, column grid x, y, z bins, super-cluster bound NSmax, slots, entry capacity, waves,
+ * splits, chunks per super-cluster (waves * splits). */
+int pdx_cluster_info(pdx_cluster_t c, int64_t out[10]);
+/* Launch shape of the force pass: waves (1, 2, 4, 8) per workgroup and splits (1, 2, 4, 8)
+ * workgroups per super-cluster; a super-cluster's entries are cut into waves * splits contiguous
+ * chunks, one per wave (the reference's split blocks for small systems, Kernels.jl:76-86).
+ * Default: 4 waves and splits for about 2048 waves in flight. The next update rebuilds. Host
+ * boundary: synchronises the device. */
+int pdx_cluster_set_launch(pdx_cluster_t c, int32_t waves, int32_t splits);
 int pdx_cluster_arrays_get(pdx_cluster_t c, pdx_cluster_arrays *out);
 /* Keep the list valid for the current coordinates and pack them in slot order
  * (update_list!, Engine.jl:41-60): displacement decision, predicated build (clustering,

@@ -37,7 +37,9 @@ constexpr int CL = 8;    // atoms per cluster
 constexpr int SUP = 64;  // atoms per super-cluster (eight clusters)
 constexpr int KV = 10;   // partial values per atom: fx, fy, fz, energy, six virial components
 constexpr int MAXC = 24;
-constexpr int MAXW = 8;  // wave64s per super-cluster in the force pass
+constexpr int MAXW = 8;  // wave64s per workgroup of the force pass
+constexpr int MAXSPLIT = 8;  // workgroups per super-cluster of the force pass
+constexpr int MAXCHUNK = MAXW * MAXSPLIT;
 
 // control words (device int32)
 enum : int {
@@ -225,6 +227,15 @@ __global__ __launch_bounds__(TPB) void k_cl_order(int N, const int32_t *__restri
   const int i = blockIdx.x * TPB + threadIdx.x;
   if (i >= N) return;
   order[bin_start[bin_of[i]] + bin_rank[i]] = i;
+}
+
+// The mesh traversal order (mesh_order, a copy of the cluster order): 1-based ids in slot order
+// without the padding, i.e. the sorted order itself.
+__global__ __launch_bounds__(TPB) void k_cl_order1(int N, const int32_t *__restrict__ order, int32_t *order1,
+                                                   const int32_t *ctl) {
+  if (!ctl[W_BUILD]) return;
+  const int k = blockIdx.x * TPB + threadIdx.x;
+  if (k < N) order1[k] = order[k] + 1;
 }
 
 // Ascending ids inside every bin (bins hold a few particles): the order, and with it the slots
@@ -563,7 +574,7 @@ struct TileArgs {
   int32_t *went, *wjb, *nw;
   uint8_t *keep;
   float *jbuf, *islot;
-  int ecap, nslot, waves;
+  int ecap, nslot, waves, splits;
   float rp2, rmax2;
 };
 
@@ -572,10 +583,11 @@ template <int MODEL, bool COUL, bool E, bool PRUNE, int NP>
 __global__ __launch_bounds__(WAVE *MAXW) void k_cl_tiles(const TileArgs a) {
   const int32_t *ctl = a.ctl;
   if (ctl[W_ERR] != 0 || (ctl[W_PRUNE] != 0) != PRUNE) return;
-  const int S = blockIdx.x;
+  const int S = blockIdx.x / a.splits, part = blockIdx.x % a.splits;
   if (S >= ctl[W_NS]) return;
   constexpr int NV = E ? KV : 3;
   const int W = a.waves;
+  const int nchunk = W * a.splits;
   const int tid = threadIdx.x, w = tid / WAVE, lane = tid & (WAVE - 1);
   const int ti = lane & 7, tj = lane >> 3;
   const float Lx = a.geom[0], Ly = a.geom[1], Lz = a.geom[2];
@@ -594,17 +606,32 @@ __global__ __launch_bounds__(WAVE *MAXW) void k_cl_tiles(const TileArgs a) {
   for (int c = 0; c < CL; ++c)
 #pragma unroll
     for (int k = 0; k < NV; ++k) si[c][k] = 0.0f;
+  // the entries of S in W x splits contiguous chunks, one per wave
   const int n = a.ne[S];
-  const int e0 = (n * w) / W;
-  const int e1 = PRUNE ? (n * (w + 1)) / W : e0 + a.nw[S * W + w];
+  const int chunk = part * W + w;
+  const int e0 = (n * chunk) / nchunk;
+  const int e1 = PRUNE ? (n * (chunk + 1)) / nchunk : e0 + a.nw[S * nchunk + chunk];
   const size_t seg = (size_t)S * a.ecap;
   int wpos = e0;
+  // the next entry and its j-atom are loaded one iteration ahead (the loads depend on each
+  // other, and an iteration of up to eight tiles covers both latencies)
+  int ent_n = 0, jb_n = 0;
+  float4 xj_n = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (e0 < e1) {
+    ent_n = PRUNE ? a.oent[seg + e0] : a.went[seg + e0];
+    jb_n = PRUNE ? (int)(seg + e0) : a.wjb[seg + e0];
+    xj_n = a.packed[(size_t)(ent_n >> 8) * CL + tj];
+  }
   for (int e = e0; e < e1; ++e) {
-    const int ent = PRUNE ? a.oent[seg + e] : a.went[seg + e];
-    const int jb = PRUNE ? (int)(seg + e) : a.wjb[seg + e];
+    const int ent = ent_n, jb = jb_n;
+    const float4 xj = xj_n;
+    if (e + 1 < e1) {
+      ent_n = PRUNE ? a.oent[seg + e + 1] : a.went[seg + e + 1];
+      jb_n = PRUNE ? (int)(seg + e + 1) : a.wjb[seg + e + 1];
+      xj_n = a.packed[(size_t)(ent_n >> 8) * CL + tj];
+    }
     const int cj = ent >> 8;
     const unsigned mask = (unsigned)ent & 255u;
-    const float4 xj = a.packed[(size_t)cj * CL + tj];
     const int cdiag = cj - S * CL;
     float sj[NV];
 #pragma unroll
@@ -663,7 +690,7 @@ __global__ __launch_bounds__(WAVE *MAXW) void k_cl_tiles(const TileArgs a) {
       if (ti < NV - 8) row[(ti + 8) * CL + tj] = ti == 0 ? sj[8] : sj[NV - 1];
     }
   }
-  if (PRUNE && lane == 0) a.nw[S * W + w] = wpos - e0;
+  if (PRUNE && lane == 0) a.nw[S * nchunk + chunk] = wpos - e0;
   // i-sums over the eight tj lanes, then over the waves in LDS in wave order
   __shared__ float sh[MAXW][KV][SUP];
 #pragma unroll
@@ -682,7 +709,7 @@ __global__ __launch_bounds__(WAVE *MAXW) void k_cl_tiles(const TileArgs a) {
     const int k = idx / SUP, l = idx % SUP;
     float s = sh[0][k][l];
     for (int q = 1; q < W; ++q) s += sh[q][k][l];
-    a.islot[(size_t)k * a.nslot + (size_t)S * SUP + l] = s;
+    a.islot[(size_t)(part * KV + k) * a.nslot + (size_t)S * SUP + l] = s;
   }
 }
 
@@ -711,7 +738,7 @@ struct ScatterArgs {
   const float *exs;
   float *fx, *fy, *fz, *Epot, *V;
   double *partial;
-  int N, nslot, add, totals;
+  int N, nslot, add, totals, splits;
   float cC, crc2, ceshift, halfself;
 };
 
@@ -732,6 +759,10 @@ __global__ __launch_bounds__(TPB) void k_cl_scatter(const ScatterArgs a) {
       float acc[NV];
 #pragma unroll
       for (int k = 0; k < NV; ++k) acc[k] = a.islot[(size_t)k * a.nslot + s];
+      for (int part = 1; part < a.splits; ++part) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += a.islot[(size_t)(part * KV + k) * a.nslot + s];
+      }
       const int c = s >> 3, t = s & 7;
       const int p0 = a.joff[c], p1 = p0 + a.wjcnt[c];
       for (int p = p0; p < p1; ++p) {
@@ -800,12 +831,12 @@ struct pdx_cluster_s {
   pdx_pair_t model = nullptr;
   double rmax = 0, skin = 0, pb = 0, rlist = 0;
   int ncx = 1, ncy = 1, nzb = 1, ncol = 1, nbins = 1, mx = 1, my = 1, allx = 0, ally = 0;
-  int NSmax = 0, nslot = 0, ecap = 0, waves = 4;
+  int NSmax = 0, nslot = 0, ecap = 0, waves = 4, splits = 1;
   int dblocks = 0, sblocks = 0;
   bool updated = false;
   DevBuf<int32_t> ctl;
   DevBuf<float> geom, ref, dpart;
-  DevBuf<int32_t> bin_of, bin_rank, bin_cnt, bin_start, order;
+  DevBuf<int32_t> bin_of, bin_rank, bin_cnt, bin_start, order, order1;
   DevBuf<int32_t> col_sc_off, sc_first, sc_n, sc_col;
   DevBuf<int32_t> perm, slot_of;
   DevBuf<float4> cbc, cbh, sbc, sbh, packed;
@@ -863,6 +894,14 @@ extern "C" int pdx_cluster_create(int32_t n, const double box[3], pdx_pair_t mod
   c->pb = prune_buffer;
   c->rlist = rlist;
   c->waves = waves;
+  // about 2048 waves over the chip: small systems split each super-cluster's entries over
+  // several workgroups
+  {
+    const int64_t ns = (n + SUP - 1) / SUP;
+    int sp = 1;
+    while (sp < MAXSPLIT && ns * waves * sp * 2 <= 2048) sp <<= 1;
+    c->splits = sp;
+  }
   // columns of about one super-cluster edge, z bins of about four particles
   const double V = box[0] * box[1] * box[2];
   const double rho = n / V;
@@ -897,14 +936,14 @@ extern "C" int pdx_cluster_create(int32_t n, const double box[3], pdx_pair_t mod
   if ((rc = c->ctl.alloc(W_LEN)) || (rc = c->geom.alloc(3)) || (rc = c->ref.alloc((size_t)6 * N)) ||
       (rc = c->dpart.alloc((size_t)2 * c->dblocks)) || (rc = c->bin_of.alloc(N)) || (rc = c->bin_rank.alloc(N)) ||
       (rc = c->bin_cnt.alloc(c->nbins)) || (rc = c->bin_start.alloc((size_t)c->nbins + 1)) ||
-      (rc = c->order.alloc(N)) || (rc = c->col_sc_off.alloc((size_t)c->ncol + 1)) ||
+      (rc = c->order.alloc(N)) || (rc = c->order1.alloc(N)) || (rc = c->col_sc_off.alloc((size_t)c->ncol + 1)) ||
       (rc = c->sc_first.alloc(c->NSmax)) || (rc = c->sc_n.alloc(c->NSmax)) || (rc = c->sc_col.alloc(c->NSmax)) ||
       (rc = c->perm.alloc(c->nslot)) || (rc = c->slot_of.alloc(N)) || (rc = c->cbc.alloc((size_t)CL * c->NSmax)) ||
       (rc = c->cbh.alloc((size_t)CL * c->NSmax)) || (rc = c->sbc.alloc(c->NSmax)) || (rc = c->sbh.alloc(c->NSmax)) ||
       (rc = c->packed.alloc(c->nslot)) || (rc = c->ne.alloc(c->NSmax)) ||
-      (rc = c->nw.alloc((size_t)MAXW * c->NSmax)) || (rc = c->jcnt.alloc((size_t)CL * c->NSmax)) ||
+      (rc = c->nw.alloc((size_t)MAXCHUNK * c->NSmax)) || (rc = c->jcnt.alloc((size_t)CL * c->NSmax)) ||
       (rc = c->jcur.alloc((size_t)CL * c->NSmax)) || (rc = c->joff.alloc((size_t)CL * c->NSmax + 1)) ||
-      (rc = c->wjcnt.alloc((size_t)CL * c->NSmax)) || (rc = c->islot.alloc((size_t)KV * c->nslot)) ||
+      (rc = c->wjcnt.alloc((size_t)CL * c->NSmax)) || (rc = c->islot.alloc((size_t)c->splits * KV * c->nslot)) ||
       (rc = c->partial.alloc((size_t)7 * c->sblocks)) || (rc = alloc_entries(c, ecap))) {
     delete c;
     return rc;
@@ -917,7 +956,7 @@ extern "C" int pdx_cluster_create(int32_t n, const double box[3], pdx_pair_t mod
   if (e == hipSuccess) e = hipMemset(c->perm.p, 0xFF, sizeof(int32_t) * c->nslot);
   if (e == hipSuccess) e = hipMemset(c->ref.p, 0, sizeof(float) * 6 * (size_t)N);
   if (e == hipSuccess) e = hipMemset(c->ne.p, 0, sizeof(int32_t) * c->NSmax);
-  if (e == hipSuccess) e = hipMemset(c->nw.p, 0, sizeof(int32_t) * MAXW * c->NSmax);
+  if (e == hipSuccess) e = hipMemset(c->nw.p, 0, sizeof(int32_t) * MAXCHUNK * c->NSmax);
   if (e != hipSuccess) {
     delete c;
     return fail(PDX_ERR_DEVICE, std::string("cluster list set-up failed: ") + hipGetErrorString(e));
@@ -931,10 +970,33 @@ extern "C" int pdx_cluster_destroy(pdx_cluster_t c) {
   return PDX_OK;
 }
 
-extern "C" int pdx_cluster_info(pdx_cluster_t c, int64_t out[8]) {
+extern "C" int pdx_cluster_info(pdx_cluster_t c, int64_t out[10]) {
   if (!c || !out) return fail(PDX_ERR_ARGUMENT, "null argument");
   out[0] = c->N; out[1] = c->ncx; out[2] = c->ncy; out[3] = c->nzb;
   out[4] = c->NSmax; out[5] = c->nslot; out[6] = c->ecap; out[7] = c->waves;
+  out[8] = c->splits; out[9] = (int64_t)c->waves * c->splits;
+  return PDX_OK;
+}
+
+extern "C" int pdx_cluster_set_launch(pdx_cluster_t c, int32_t waves, int32_t splits) {
+  if (!c) return fail(PDX_ERR_ARGUMENT, "null cluster list");
+  if (waves != 1 && waves != 2 && waves != 4 && waves != 8)
+    return fail(PDX_ERR_ARGUMENT, "cluster pairs: waves must be 1, 2, 4 or 8");
+  if (splits < 1 || splits > MAXSPLIT || (splits & (splits - 1)))
+    return fail(PDX_ERR_ARGUMENT, "cluster pairs: splits must be 1, 2, 4 or 8");
+  if (waves == c->waves && splits == c->splits) return PDX_OK;
+  // the working list is laid out per chunk: the next update rebuilds and the pass prunes again
+  PDX_HIP(hipDeviceSynchronize());
+  if ((size_t)splits * KV * c->nslot > c->islot.n) {
+    int rc = c->islot.alloc((size_t)splits * KV * c->nslot);
+    if (rc) return rc;
+  }
+  c->waves = waves;
+  c->splits = splits;
+  int32_t h[W_LEN];
+  PDX_HIP(hipMemcpy(h, c->ctl.p, sizeof(h), hipMemcpyDeviceToHost));
+  h[W_REQUEST] = 1;
+  PDX_HIP(hipMemcpy(c->ctl.p, h, sizeof(h), hipMemcpyHostToDevice));
   return PDX_OK;
 }
 
@@ -946,6 +1008,9 @@ extern "C" int pdx_cluster_arrays_get(pdx_cluster_t c, pdx_cluster_arrays *out) 
   out->n_outer = c->ne.p;
   out->outer = c->oent.p;
   out->control = c->ctl.p;
+  out->order = c->order1.p;
+  out->working = c->went.p;
+  out->n_working = c->nw.p;
   return PDX_OK;
 }
 
@@ -971,6 +1036,8 @@ extern "C" int pdx_cluster_update(pdx_cluster_t c, const float *x, const float *
                      (const int32_t *)ctl);
   hipLaunchKernelGGL(k_cl_sort_bins, dim3(blocks_of(c->nbins, TPB)), dim3(TPB), 0, st, c->nbins,
                      (const int32_t *)c->bin_start.p, c->order.p, (const int32_t *)ctl);
+  hipLaunchKernelGGL(k_cl_order1, dim3(c->dblocks), dim3(TPB), 0, st, N, (const int32_t *)c->order.p, c->order1.p,
+                     (const int32_t *)ctl);
   hipLaunchKernelGGL(k_cl_layout, dim3(1), dim3(1024), 0, st, c->ncol, c->nzb, (const int32_t *)c->bin_start.p,
                      c->col_sc_off.p, c->sc_first.p, c->sc_n.p, c->sc_col.p, ctl);
   hipLaunchKernelGGL(k_cl_assign, dim3(c->NSmax), dim3(WAVE), 0, st, (const int32_t *)ctl,
@@ -1081,11 +1148,12 @@ extern "C" int pdx_cluster_forces(pdx_cluster_t c, float *fx, float *fy, float *
   a.ecap = c->ecap;
   a.nslot = c->nslot;
   a.waves = c->waves;
+  a.splits = c->splits;
   const double rp = c->rmax + c->pb;
   a.rp2 = (float)(rp * rp);
   a.rmax2 = (float)(c->rlist * c->rlist * 1.0001);
   const int np = std::max(p->npe, p->npf);
-  const int grid = c->NSmax, threads = WAVE * c->waves;
+  const int grid = c->NSmax * c->splits, threads = WAVE * c->waves;
   switch (p->spec.model) {
     case PDX_PAIR_LJ: launch_tiles_model<PDX_PAIR_LJ>(coul, energy, np, st, grid, threads, a, c); break;
     case PDX_PAIR_WCA: launch_tiles_model<PDX_PAIR_WCA>(coul, energy, np, st, grid, threads, a, c); break;
@@ -1114,6 +1182,7 @@ extern "C" int pdx_cluster_forces(pdx_cluster_t c, float *fx, float *fy, float *
   s.nslot = c->nslot;
   s.add = (flags & PDX_ADD) ? 1 : 0;
   s.totals = totals != nullptr;
+  s.splits = c->splits;
   s.cC = p->cC; s.crc2 = p->crc2; s.ceshift = p->ceshift; s.halfself = p->halfself;
   if (coul) {
     if (energy) hipLaunchKernelGGL((k_cl_scatter<true, true>), dim3(c->sblocks), dim3(TPB), 0, st, s);
@@ -1143,14 +1212,15 @@ extern "C" int pdx_cluster_status(pdx_cluster_t c, int64_t out[8], int32_t check
   PDX_HIP(hipStreamSynchronize(as_stream(stream)));
   int32_t h[W_LEN];
   PDX_HIP(hipMemcpy(h, c->ctl.p, sizeof(h), hipMemcpyDeviceToHost));
-  std::vector<int32_t> ne(c->NSmax), nw((size_t)MAXW * c->NSmax);
+  std::vector<int32_t> ne(c->NSmax), nw((size_t)MAXCHUNK * c->NSmax);
   PDX_HIP(hipMemcpy(ne.data(), c->ne.p, sizeof(int32_t) * ne.size(), hipMemcpyDeviceToHost));
   PDX_HIP(hipMemcpy(nw.data(), c->nw.p, sizeof(int32_t) * nw.size(), hipMemcpyDeviceToHost));
   int64_t outer = 0, work = 0;
   const int NS = std::min(std::max(h[W_NS], 0), c->NSmax);
   for (int S = 0; S < NS; ++S) {
     outer += ne[S];
-    for (int w = 0; w < c->waves; ++w) work += nw[(size_t)S * c->waves + w];
+    const int nchunk = c->waves * c->splits;
+    for (int k = 0; k < nchunk; ++k) work += nw[(size_t)S * nchunk + k];
   }
   out[0] = h[W_NS]; out[1] = outer; out[2] = work; out[3] = h[W_NBUILD];
   out[4] = h[W_NPRUNE]; out[5] = h[W_ERR]; out[6] = h[W_REQ]; out[7] = c->ecap;
@@ -1176,6 +1246,6 @@ extern "C" int pdx_cluster_resize(pdx_cluster_t c, int32_t entry_capacity) {
   h[W_REQUEST] = 1;
   PDX_HIP(hipMemcpy(c->ctl.p, h, sizeof(h), hipMemcpyHostToDevice));
   PDX_HIP(hipMemset(c->ne.p, 0, sizeof(int32_t) * c->NSmax));
-  PDX_HIP(hipMemset(c->nw.p, 0, sizeof(int32_t) * MAXW * c->NSmax));
+  PDX_HIP(hipMemset(c->nw.p, 0, sizeof(int32_t) * MAXCHUNK * c->NSmax));
   return PDX_OK;
 }

@@ -22,7 +22,8 @@ ClusterStatus = namedtuple("ClusterStatus", "super_clusters outer working builds
 
 class pdx_cluster_arrays(C.Structure):
     _fields_ = [("perm", C.c_void_p), ("slot_of", C.c_void_p), ("packed", C.c_void_p), ("n_outer", C.c_void_p),
-                ("outer", C.c_void_p), ("control", C.c_void_p)]
+                ("outer", C.c_void_p), ("control", C.c_void_p), ("order", C.c_void_p), ("working", C.c_void_p),
+                ("n_working", C.c_void_p)]
 
 
 def _sig(lib):
@@ -38,6 +39,7 @@ def _sig(lib):
         "pdx_cluster_request_build": [vp, vp],
         "pdx_cluster_status": [vp, P(C.c_int64), i32, vp],
         "pdx_cluster_resize": [vp, i32],
+        "pdx_cluster_set_launch": [vp, i32, i32],
     }.items():
         f = getattr(lib, name)
         f.argtypes = args
@@ -68,9 +70,9 @@ class ClusterPairEngine:
         self._refresh()
 
     def _refresh(self):
-        o = (C.c_int64 * 8)()
+        o = (C.c_int64 * 10)()
         L.check(L.lib.pdx_cluster_info(self._h, o))
-        keys = ("N", "ncx", "ncy", "nzb", "NSmax", "nslot", "capacity", "waves")
+        keys = ("N", "ncx", "ncy", "nzb", "NSmax", "nslot", "capacity", "waves", "splits", "chunks")
         self.info = dict(zip(keys, [int(v) for v in o]))
         a = pdx_cluster_arrays()
         L.check(L.lib.pdx_cluster_arrays_get(self._h, C.byref(a)))
@@ -120,6 +122,11 @@ class ClusterPairEngine:
         return {"builds": s.builds, "prunes": s.prunes, "outer": s.outer, "working": s.working,
                 "super_clusters": s.super_clusters}
 
+    def set_launch(self, waves, splits):
+        """Force-pass launch shape: waves per workgroup and workgroups per super-cluster."""
+        L.check(L.lib.pdx_cluster_set_launch(self._h, int(waves), int(splits)))
+        self._refresh()
+
     def resize(self, capacity):
         L.check(L.lib.pdx_cluster_resize(self._h, int(capacity)))
         self._refresh()
@@ -144,10 +151,41 @@ class ClusterPairEngine:
         import torch
         return device_view(self._arrays.packed, (self.info["nslot"], 4), torch.float32)
 
+    def order(self):
+        """1-based particle ids in slot order (the mesh traversal order)."""
+        return device_view(self._arrays.order, (self.N,), _i32())
+
+    @property
+    def order_ptr(self):
+        return self._arrays.order
+
     def outer(self):
         """(n_outer per super-cluster, entries (NSmax, capacity)): entry = (cj << 8) | i-mask."""
         return (device_view(self._arrays.n_outer, (self.info["NSmax"],), _i32()),
                 device_view(self._arrays.outer, (self.info["NSmax"], self.info["capacity"]), _i32()))
+
+
+    def tile_counts(self):
+        """Host counts of the current lists: outer and working entries, and the 8 x 8 tiles
+        (mask bits) of each."""
+        import numpy as np
+        NS = self.status(check=False).super_clusters
+        ne, ent = (t.cpu().numpy() for t in self.outer())
+        K, cap = self.info["chunks"], self.info["capacity"]
+        went = device_view(self._arrays.working, (self.info["NSmax"], cap), _i32()).cpu().numpy()
+        nw = device_view(self._arrays.n_working, (self.info["NSmax"] * K,), _i32()).cpu().numpy()
+        bits = np.array([bin(k).count("1") for k in range(256)])
+        out = dict(outer=0, outer_tiles=0, working=0, working_tiles=0)
+        for S in range(NS):
+            n = int(ne[S])
+            out["outer"] += n
+            out["outer_tiles"] += int(bits[ent[S, :n] & 255].sum())
+            for w in range(K):
+                e0 = (n * w) // K
+                k = int(nw[S * K + w])
+                out["working"] += k
+                out["working_tiles"] += int(bits[went[S, e0:e0 + k] & 255].sum())
+        return out
 
 
 def _i32():

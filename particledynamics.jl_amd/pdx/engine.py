@@ -10,6 +10,7 @@ from . import _lib as L
 from .electrostatics import ElectrostaticsSolver
 from .forces import PairForce
 from .neighbors import DeviceNeighborWorkspace
+from .clusters import ClusterPairEngine
 
 
 def default_capacity(N, box, rlist):
@@ -21,8 +22,14 @@ def default_capacity(N, box, rlist):
 class ForceEngine:
     def __init__(self, N, box, pair_model=None, electrostatics=None, charges=None, skin=0.4, cap=None,
                  interval=20, lanes=0, diagnostics=False, spread="cells", correction=None, accumulation="fp32",
-                 bonds=None, exceptions=(), traversal="rows"):
+                 bonds=None, exceptions=(), traversal="rows", backend="ell", prune_buffer=None, cluster_capacity=0):
+        """backend "ell": the cell list and ELL rows (``pdx_nl_*`` + ``pdx_pair_forces``); "cluster":
+        the cluster-pair list and tiles (``pdx_cluster_*``, the reference's ``ClusterBackend``,
+        src/engine/ClusterBackend.jl:1-40); its mesh spread takes the cluster order."""
         import torch
+        if backend not in ("ell", "cluster"):
+            raise L.ArgumentError("backend must be 'ell' or 'cluster'")
+        self.backend = backend
         self.N = int(N)
         self.box = tuple(float(b) for b in box)
         self.solver = ElectrostaticsSolver(electrostatics, box, charges) if electrostatics is not None else None
@@ -33,10 +40,16 @@ class ForceEngine:
             rc = max(rc, float(electrostatics.real_cutoff))
         self.cutoff = rc
         self.skin = float(skin)
-        cap = cap if cap is not None else default_capacity(N, box, rc + skin)
-        self.nl = DeviceNeighborWorkspace(N, box, rc, skin, cap, interval=interval, diagnostics=diagnostics)
         self.pairs = PairForce(pair_model, coulomb=self.solver.real_space() if self.solver else None, lanes=lanes,
                                accumulation=accumulation, bonds=bonds, traversal=traversal)
+        if backend == "cluster":
+            self.nl = None
+            self.cluster = ClusterPairEngine(N, box, self.pairs, skin, prune_buffer, capacity=cluster_capacity)
+            spread = "atomic"  # the owner-computes spread walks the ELL list's cells
+        else:
+            cap = cap if cap is not None else default_capacity(N, box, rc + skin)
+            self.nl = DeviceNeighborWorkspace(N, box, rc, skin, cap, interval=interval, diagnostics=diagnostics)
+            self.cluster = None
         if exceptions:
             if self.solver is None:
                 raise L.ArgumentError("electrostatic exceptions need a mesh")
@@ -64,12 +77,15 @@ class ForceEngine:
         """Enqueue one force evaluation (no host synchronisation)."""
         import torch
         main = stream if stream is not None else torch.cuda.current_stream()
-        self.nl.update(x, y, z, q, stream=main)
+        if self.cluster is not None:
+            self.cluster.update(x, y, z, q, stream=main)
+            order = self.cluster.order_ptr
+        else:
+            self.nl.update(x, y, z, q, stream=main)
+            order = self.nl._arrays.particle_ids_sorted
         if self.solver is None:
-            self.pairs.compute(self.nl, fx, fy, fz, Epot, V, self.pair_totals if energy else None, energy=energy,
-                               add=False, stream=main)
+            self._pairs(fx, fy, fz, Epot, V, energy, main)
             return
-        order = self.nl._arrays.particle_ids_sorted
         side = self.mesh_stream if overlap else main
         if overlap:
             side.wait_stream(main)
@@ -81,14 +97,26 @@ class ForceEngine:
             from .learned import mesh_features
             mesh_features(self.solver, x, y, z, q, self.feats, stream=side)
         self.solver.solve(energy, stream=side)
-        self.pairs.compute(self.nl, fx, fy, fz, Epot, V, self.pair_totals if energy else None, energy=energy,
-                           add=False, stream=main)
+        self._pairs(fx, fy, fz, Epot, V, energy, main)
         if overlap:
             main.wait_stream(side)
         self.solver.mesh_gather(x, y, z, q, fx, fy, fz, Epot, self.Vglobal if energy else None, energy,
                                 order=order, set=False, stream=main)
         if self.correction is not None:
             self.correction.forward(self.feats, None, fx, fy, fz, stream=main)
+
+    def _pairs(self, fx, fy, fz, Epot, V, energy, stream):
+        tot = self.pair_totals if energy else None
+        if self.cluster is not None:
+            self.cluster.forces(fx, fy, fz, Epot, V, tot, energy=energy, add=False, stream=stream)
+        else:
+            self.pairs.compute(self.nl, fx, fy, fz, Epot, V, tot, energy=energy, add=False, stream=stream)
+
+    def status(self, check=True, stream=None):
+        """Boundary readback of the short-range list (raises on a latched capacity / coordinate error)."""
+        if self.cluster is not None:
+            return self.cluster.status(check=check, stream=stream)
+        return self.nl.status(check=check, stream=stream)
 
     def capture(self, x, y, z, q, fx, fy, fz, Epot=None, V=None, energy=True, overlap=True):
         """Capture one evaluation into a hipGraph (torch.cuda.CUDAGraph) over these buffers;
@@ -109,7 +137,7 @@ class ForceEngine:
 
     def totals(self, stream=None):
         """FP64 energy and virial of the last energy evaluation (boundary readback)."""
-        self.nl.status(check=True, stream=stream)
+        self.status(check=True, stream=stream)
         p = self.pair_totals.cpu().tolist()
         E = p[0]
         W = p[1:7]

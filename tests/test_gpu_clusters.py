@@ -110,6 +110,15 @@ def test_uneven_box_and_cluster_sizes(cuda):
         else:
             assert _relrms(F, base) < 1e-6
     assert eng.info["waves"] == 8
+    # entries split over 1..8 workgroups per super-cluster: same pairs, sums in another order
+    X, Y, Z = dev(s["x"]), dev(s["y"]), dev(s["z"])
+    for splits in (1, 2, 8):
+        eng.set_launch(4, splits)
+        F = _host(*eng.compute(X, Y, Z, None, energy=True)[:3])[0]
+        assert eng.info["splits"] == splits
+        assert _relrms(F, base) < 1e-6, splits
+        st = eng.status()
+        assert st.builds == 2 + (splits > 1) * (1 if splits == 2 else 2)
 
 
 def test_list_covers_every_pair_once(cuda):
@@ -295,3 +304,40 @@ def test_restrictions(cuda):
         ClusterPairEngine(100, (20.0, 20.0, 20.0), pdx.PairForce(pdx.WCA(), accumulation="fp64"), 0.4)
     with pytest.raises(pdx.ArgumentError):
         ClusterPairEngine(100, (20.0, 20.0, 20.0), pf, 0.4, waves=3)
+
+
+def test_engine_cluster_backend_with_mesh(cuda):
+    """ForceEngine with the cluster backend (the reference's ClusterBackend: cluster tiles for the
+    short range, the mesh in the cluster order) against the ELL backend on the same PME system,
+    eager and as a captured graph."""
+    import torch
+    s = systems.rock_salt(12, 2.82, 0.1, seed=2)
+    N, box = s["N"], s["box"]
+    spec = pdx.PME(coulomb_prefactor=14.399645, real_cutoff=8.0, real_space_tolerance=1e-5, mesh_spacing=1.0,
+                   order=5)
+    lj = pdx.LennardJones(0.01, 2.8, 8.0)
+    X, Y, Z, Q = dev(s["x"]), dev(s["y"]), dev(s["z"]), dev(s["q"])
+    res = {}
+    for backend in ("ell", "cluster"):
+        eng = pdx.ForceEngine(N, box, lj, electrostatics=spec, charges=s["q"], skin=0.5, backend=backend)
+        fx, fy, fz, E = (torch.empty(N, device="cuda") for _ in range(4))
+        V = torch.empty(6, N, device="cuda")
+        eng.evaluate(X, Y, Z, Q, fx, fy, fz, E, V, energy=True)
+        tot = eng.totals()
+        F = torch.stack([fx, fy, fz], 1).double().cpu().numpy()
+        res[backend] = (F, E.double().cpu().numpy(), tot)
+        if backend == "cluster":
+            g = eng.capture(X, Y, Z, Q, fx, fy, fz, E, V, energy=True)
+            eager = [t.clone() for t in (fx, fy, fz, E)]
+            for _ in range(3):
+                g.replay()
+                torch.cuda.synchronize()
+                for a, b in zip((fx, fy, fz, E), eager):
+                    assert torch.equal(a, b)
+    (F1, E1, t1), (F2, E2, t2) = res["ell"], res["cluster"]
+    print(f"[clusters] engine rel rms vs ell {_relrms(F2, F1):.2e} dE {abs(t2['energy'] - t1['energy']):.2e}",
+          flush=True)
+    assert _relrms(F2, F1) < 1e-6
+    assert abs(t2["energy"] - t1["energy"]) < 1e-6 * abs(t1["energy"])
+    assert np.allclose(t2["virial"], t1["virial"], rtol=1e-6, atol=1e-6 * abs(t1["energy"]))
+    assert np.allclose(E2, E1, rtol=1e-5, atol=1e-6)
