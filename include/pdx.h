@@ -209,8 +209,9 @@ typedef struct {
  * (ElectrostaticsSolver, Solver.jl:129-158; compute_influence!, Influence.jl:190-227). */
 int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const double box[3], pdx_mesh_t *out);
 int pdx_mesh_destroy(pdx_mesh_t m);
-/* Resolved parameters: out[0..7] = K1, K2, K3, P, alpha, C, self_coefficient, bg (per unit q). */
-int pdx_mesh_info(pdx_mesh_t m, double out[8]);
+/* Resolved parameters: out[0..8] = K1, K2, K3, P, alpha, C, self_coefficient, bg (per unit q),
+ * lines per plane of the plane transforms (0: line passes). */
+int pdx_mesh_info(pdx_mesh_t m, double out[9]);
 /* Host charges: total charge, sum q^2, net-charge policy (set_solver_charges!, Solver.jl:166-185).
  * Also fixes the fixed-point scale of the spread from sum |q|. */
 int pdx_mesh_set_charges(pdx_mesh_t m, const float *q_host, int32_t n);

@@ -294,4 +294,5 @@ struct pdx_mesh_s {
   bool pot_valid = false;                  // ik: the potential grid of the last solve exists
   int nmode_blocks = 0;
   int T1 = 0, T2 = 0, T3 = 0;  // FFT tile widths, fixed at creation
+  int plane_tp = 0;            // plane passes: lines per plane padded to this power of two (0: line passes)
 };

@@ -1,5 +1,5 @@
 // Smooth PME mesh engine on gfx950: B-spline charge spreading into an int64 fixed-point
-// grid, a mixed-radix (2,3,4,5,7,11,13) Stockham 3D FFT in LDS, the influence-function
+// grid, a mixed-radix (2,3,4,5,7,8,11,13) Stockham 3D FFT in LDS, the influence-function
 // convolution with FP64 energy / virial reductions fused into the z transform, and the
 // analytic-differentiation force gather.
 //
@@ -41,7 +41,7 @@ bool make_plan(int K, FFTPlan &pl) {
   pl.K = K;
   pl.nstage = 0;
   int n = K;
-  const int rs[7] = {4, 2, 3, 5, 7, 11, 13};
+  const int rs[8] = {8, 4, 2, 3, 5, 7, 11, 13};
   for (int r : rs) {
     while (n % r == 0) {
       if (pl.nstage >= MAXSTAGE) return false;
@@ -52,7 +52,7 @@ bool make_plan(int K, FFTPlan &pl) {
   int p = 1;
   pl.big = 0;
   for (int s = 0; s < pl.nstage; ++s) {
-    if (pl.radix[s] >= 7) pl.big = 1;
+    if (pl.radix[s] == 7 || pl.radix[s] >= 11) pl.big = 1;
     pl.p[s] = p;
     pl.magic[s] = p > 1 ? (unsigned)((((unsigned long long)1 << 32) + p - 1) / p) : 0u;
     pl.tstride[s] = K / (p * pl.radix[s]);
@@ -95,6 +95,27 @@ template <> __device__ __forceinline__ void dft<4>(float2 (&u)[MAXR], float sg) 
   u[2] = csub(t0, t2);
   u[1] = cadd(t1, t3);
   u[3] = csub(t1, t3);
+}
+// Radix 8 as two radix-4 DFTs: a_k = u_k + u_{k+4} gives the even outputs, b_k = (u_k - u_{k+4})
+// W8^k the odd ones (W8 = exp(sg i pi / 4)).
+template <> __device__ __forceinline__ void dft<8>(float2 (&u)[MAXR], float sg) {
+  const float c = 0.70710678118654752440f;
+  float2 a[MAXR], b[MAXR];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    a[k] = cadd(u[k], u[k + 4]);
+    b[k] = csub(u[k], u[k + 4]);
+  }
+  b[1] = make_float2(c * (b[1].x - sg * b[1].y), c * (b[1].y + sg * b[1].x));
+  b[2] = cmuli(b[2], sg);
+  b[3] = make_float2(c * (-b[3].x - sg * b[3].y), c * (-b[3].y + sg * b[3].x));
+  dft<4>(a, sg);
+  dft<4>(b, sg);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    u[2 * m] = a[m];
+    u[2 * m + 1] = b[m];
+  }
 }
 template <> __device__ __forceinline__ void dft<5>(float2 (&u)[MAXR], float sg) {
   const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
@@ -147,40 +168,56 @@ template <> __device__ __forceinline__ void dft<13>(float2 (&u)[MAXR], float sg)
 
 template <int T> constexpr int ilog2() { return T <= 1 ? 0 : 1 + ilog2<T / 2>(); }
 
-// One Stockham stage over T lines stored [element*T + line]; twiddles exp(-2 pi i m/K) from
-// LDS (conjugated for the inverse, sg = +1).
-template <int R, int T>
+// One Stockham stage over T lines stored [element*S + line] (S >= T: a padded row stride);
+// twiddles exp(-2 pi i m/K) from LDS (conjugated for the inverse, sg = +1).
+template <int R, int T, int S = T>
 __device__ __forceinline__ void stage(const float2 *__restrict__ in, float2 *__restrict__ out, int K, int p,
                                       unsigned magic, int tstride, const float2 *__restrict__ tw, float sg) {
   constexpr int TL = ilog2<T>();
   const int KR = K / R;
   const int nb = KR << TL;
-  for (int idx = threadIdx.x; idx < nb; idx += blockDim.x) {
-    const int l = idx & (T - 1);
-    const int i = idx >> TL;
-    const int q = p == 1 ? i : (int)__umulhi((unsigned)i, magic);
-    const int k = i - q * p;
-    float2 u[MAXR];
+  const int bs = blockDim.x;
+  // two butterflies per iteration (one for the large radices, which need the registers): the LDS
+  // loads of both are issued before either is computed
+  constexpr int U = (R == 7 || R >= 11) ? 1 : 2;
+  for (int idx0 = threadIdx.x; idx0 < nb; idx0 += U * bs) {
+    float2 u[U][MAXR];
+    int jj[U], ll[U], pp = p;
+    bool act[U];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      float2 v = in[(i + r * KR) * T + l];
-      if (r > 0 && k > 0) {
-        float2 w = tw[r * k * tstride];
-        w.y *= -sg;
-        v = cmul(v, w);
+    for (int h = 0; h < U; ++h) {
+      const int idx = idx0 + h * bs;
+      act[h] = idx < nb;
+      const int l = idx & (T - 1);
+      const int i = act[h] ? idx >> TL : 0;
+      const int q = p == 1 ? i : (int)__umulhi((unsigned)i, magic);
+      const int k = i - q * p;
+      jj[h] = q * p * R + k;
+      ll[h] = l;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float2 v = in[(i + r * KR) * S + l];
+        if (r > 0 && k > 0) {
+          float2 w = tw[r * k * tstride];
+          w.y *= -sg;
+          v = cmul(v, w);
+        }
+        u[h][r] = v;
       }
-      u[r] = v;
     }
-    dft<R>(u, sg);
-    const int j = q * p * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) out[(j + r * p) * T + l] = u[r];
+    for (int h = 0; h < U; ++h) {
+      if (!act[h]) continue;
+      dft<R>(u[h], sg);
+#pragma unroll
+      for (int r = 0; r < R; ++r) out[(jj[h] + r * pp) * S + ll[h]] = u[h][r];
+    }
   }
 }
 
 // Full transform of T lines held in buf0; returns the buffer with the result.
 // sg = -1 forward, +1 inverse (unnormalised).
-template <int T, bool BIG>
+template <int T, bool BIG, int S = T>
 __device__ float2 *lds_fft(float2 *buf0, float2 *buf1, const FFTPlan &pl, const float2 *__restrict__ tw, float sg) {
   float2 *a = buf0, *b = buf1;
   for (int s = 0; s < pl.nstage; ++s) {
@@ -188,16 +225,17 @@ __device__ float2 *lds_fft(float2 *buf0, float2 *buf1, const FFTPlan &pl, const 
     const unsigned mg = pl.magic[s];
     const int ts = pl.tstride[s];
     switch (pl.radix[s]) {
-      case 4: stage<4, T>(a, b, pl.K, p, mg, ts, tw, sg); break;
-      case 2: stage<2, T>(a, b, pl.K, p, mg, ts, tw, sg); break;
-      case 3: stage<3, T>(a, b, pl.K, p, mg, ts, tw, sg); break;
-      case 5: stage<5, T>(a, b, pl.K, p, mg, ts, tw, sg); break;
+      case 8: stage<8, T, S>(a, b, pl.K, p, mg, ts, tw, sg); break;
+      case 4: stage<4, T, S>(a, b, pl.K, p, mg, ts, tw, sg); break;
+      case 2: stage<2, T, S>(a, b, pl.K, p, mg, ts, tw, sg); break;
+      case 3: stage<3, T, S>(a, b, pl.K, p, mg, ts, tw, sg); break;
+      case 5: stage<5, T, S>(a, b, pl.K, p, mg, ts, tw, sg); break;
       default:
         if (BIG) {
           switch (pl.radix[s]) {
-            case 7: stage<7, T>(a, b, pl.K, p, mg, ts, tw, sg); break;
-            case 11: stage<11, T>(a, b, pl.K, p, mg, ts, tw, sg); break;
-            default: stage<13, T>(a, b, pl.K, p, mg, ts, tw, sg); break;
+            case 7: stage<7, T, S>(a, b, pl.K, p, mg, ts, tw, sg); break;
+            case 11: stage<11, T, S>(a, b, pl.K, p, mg, ts, tw, sg); break;
+            default: stage<13, T, S>(a, b, pl.K, p, mg, ts, tw, sg); break;
           }
         }
         break;
@@ -214,18 +252,18 @@ __device__ __forceinline__ void load_twiddles(float2 *twl, const float2 *__restr
 
 // Element e -> (global load, LDS store) for e in [0, n), CH loads in flight per thread: all
 // loads of a chunk are issued before the first LDS store waits on them.
-template <int CH, typename V, typename LOAD, typename STORE>
+template <int CH, typename V, int BS = TPB, typename LOAD, typename STORE>
 __device__ __forceinline__ void staged(int n, LOAD load, STORE store) {
-  for (int base = 0; base < n; base += CH * TPB) {
+  for (int base = 0; base < n; base += CH * BS) {
     V v[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      const int e = base + c * TPB + (int)threadIdx.x;
+      const int e = base + c * BS + (int)threadIdx.x;
       if (e < n) v[c] = load(e);
     }
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      const int e = base + c * TPB + (int)threadIdx.x;
+      const int e = base + c * BS + (int)threadIdx.x;
       if (e < n) store(e, v[c]);
     }
   }
@@ -375,39 +413,24 @@ struct SolveArgs {
   double *acc;
 };
 
-// ---- z pass: MODE 0 forward only, 1 inverse only, 2 forward + solve + inverse
-template <int T, int MODE, bool ENERGY, bool BIG>
-__global__ __launch_bounds__(TPB) void k_fft_z(float2 *spec, FFTPlan pl, const float2 *__restrict__ tw,
-                                               SolveArgs s) {
-  extern __shared__ float2 lds[];
+// The influence multiply of one tile of T z-lines at fixed m2 (lines m1 = m10 + l), held in r
+// as [m3*T + l], with the FP64 energy / virial terms (_solve_kernel!, MeshKernels.jl:57-81).
+template <int T, bool ENERGY, int S = T>
+__device__ __forceinline__ void solve_lines(float2 *r, int m10, int m2, const SolveArgs &s, double (&acc)[7]) {
   const int K1h = s.K1h, K2 = s.K2, K3 = s.K3;
-  float2 *twl = lds, *b0 = lds + K3, *b1 = b0 + (size_t)T * K3;
-  load_twiddles(twl, tw, K3);
-  const int tiles = (K1h + T - 1) / T;
-  const int m10 = (blockIdx.x % tiles) * T;
-  const int m2 = blockIdx.x / tiles;
   const size_t plane = (size_t)K1h * K2;
   const size_t base = (size_t)m2 * K1h;
   constexpr int TL = ilog2<T>();
   const int n = T * K3;
-  staged<8, float2>(n, [&](int e) {
-    const int l = e & (T - 1), m3 = e >> TL;
-    return m10 + l < K1h ? spec[base + (size_t)m3 * plane + m10 + l] : make_float2(0.f, 0.f);
-  }, [&](int e, float2 v) { b0[e] = v; });
-  __syncthreads();
-  float2 *r = b0;
-  if (MODE == 0 || MODE == 2) r = lds_fft<T, BIG>(b0, b1, pl, twl, -1.0f);
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  if (MODE == 2) {
     const int s2 = m2 <= K2 / 2 ? m2 : m2 - K2;
     const bool ny2 = (K2 % 2 == 0) && (m2 == K2 / 2);
-    for (int e = threadIdx.x; e < n; e += TPB) {
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
       const int l = e & (T - 1), m3 = e >> TL;
       const int m1 = m10 + l;
       if (m1 >= K1h) continue;
       const size_t idx = base + (size_t)m3 * plane + m1;
       const float Gm = s.G[idx];
-      float2 z = r[m3 * T + l];
+      float2 z = r[m3 * S + l];
       if (ENERGY && Gm != 0.0f) {
         // per-mode terms in FP32 (the precision the coefficients are stored in), sums in FP64
         const float w = (m1 == 0 || ((s.K1 % 2 == 0) && m1 == s.K1 / 2)) ? 0.5f : 1.0f;
@@ -434,8 +457,35 @@ __global__ __launch_bounds__(TPB) void k_fft_z(float2 *spec, FFTPlan pl, const f
       }
       const float2 gz = cscale(z, Gm);
       if (s.ez) s.ez[idx] = gz;
-      r[m3 * T + l] = gz;
+      r[m3 * S + l] = gz;
     }
+}
+
+// ---- z pass: MODE 0 forward only, 1 inverse only, 2 forward + solve + inverse
+template <int T, int MODE, bool ENERGY, bool BIG>
+__global__ __launch_bounds__(TPB) void k_fft_z(float2 *spec, FFTPlan pl, const float2 *__restrict__ tw,
+                                               SolveArgs s) {
+  extern __shared__ float2 lds[];
+  const int K1h = s.K1h, K2 = s.K2, K3 = s.K3;
+  float2 *twl = lds, *b0 = lds + K3, *b1 = b0 + (size_t)T * K3;
+  load_twiddles(twl, tw, K3);
+  const int tiles = (K1h + T - 1) / T;
+  const int m10 = (blockIdx.x % tiles) * T;
+  const int m2 = blockIdx.x / tiles;
+  const size_t plane = (size_t)K1h * K2;
+  const size_t base = (size_t)m2 * K1h;
+  constexpr int TL = ilog2<T>();
+  const int n = T * K3;
+  staged<8, float2>(n, [&](int e) {
+    const int l = e & (T - 1), m3 = e >> TL;
+    return m10 + l < K1h ? spec[base + (size_t)m3 * plane + m10 + l] : make_float2(0.f, 0.f);
+  }, [&](int e, float2 v) { b0[e] = v; });
+  __syncthreads();
+  float2 *r = b0;
+  if (MODE == 0 || MODE == 2) r = lds_fft<T, BIG>(b0, b1, pl, twl, -1.0f);
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (MODE == 2) {
+    solve_lines<T, ENERGY>(r, m10, m2, s, acc);
     __syncthreads();
     float2 *other = (r == b0) ? b1 : b0;
     r = lds_fft<T, BIG>(r, other, pl, twl, +1.0f);
@@ -461,6 +511,163 @@ __global__ __launch_bounds__(TPB) void k_fft_z(float2 *spec, FFTPlan pl, const f
     // mesh energy = 1/2 sum_full |b|^2 G = sum_half w |b|^2 G (w = 1/2 on the m1 = 0 and
     // Nyquist planes); virial W_ab = sum_half w |b|^2 Vc_ab
     block_partials<7>(acc, s.partial);
+  }
+}
+
+// ---- plane passes: one workgroup of PBS threads holds a whole mesh plane in LDS, so the x and
+// y transforms of a z plane run in one launch with no round trip through memory between them,
+// and the z solve takes a whole (m1, m3) plane. Lines are padded to TP (a power of two >= the
+// lines of a plane); the LDS holds two TP x KM ping-pong buffers and the twiddles of the axes.
+// The per-line arithmetic (stages, twiddles, R2C / C2R packing) is that of the line passes, so
+// the potential grid is the same bit for bit.
+constexpr int PBS = 1024;
+
+// Sum of NV values per block over nblocks partials in a fixed order, for any block size (a
+// multiple of 64, at most 1024); the whole block calls it.
+template <int NV>
+__device__ void sum_partials_any(const double *__restrict__ partial, int nblocks, double *out) {
+  __shared__ double sh[NV][16];
+  double acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) acc[c] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < NV; ++c) acc[c] += partial[(size_t)b * NV + c];
+  }
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    const double v = wave_sum(acc[c]);
+    if (lane == 0) sh[c][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double v = 0.0;
+    for (int w = 0; w < nw; ++w) v += sh[threadIdx.x][w];
+    out[threadIdx.x] = v;
+  }
+  __syncthreads();
+}
+
+// Forward x (R2C, two real lines per complex line) and y transforms of z plane n3 of the
+// fixed-point charge grid into the half spectrum.
+template <int TP, bool BIG>
+__global__ __launch_bounds__(PBS) void k_plane_fwd(const unsigned long long *__restrict__ grid, float2 *spec, int K1,
+                                                   int K2, int K1h, int KM, FFTPlan p1, FFTPlan p2,
+                                                   const float2 *__restrict__ tw, double inv_scale) {
+  extern __shared__ float2 lds[];
+  constexpr int TS = TP + 1;  // padded row stride: column walks of the transposes hit distinct banks
+  float2 *tw1 = lds, *tw2 = lds + K1, *b0 = tw2 + K2, *b1 = b0 + (size_t)KM * TS;
+  for (int k = threadIdx.x; k < K1 + K2; k += PBS) lds[k] = tw[k];
+  const int n3 = blockIdx.x;
+  const unsigned long long *g = grid + (size_t)n3 * K1 * K2;
+  const int n = K1 * K2;
+  if (K2 & 1)  // the odd last real line has no partner: its imaginary slot stays zero
+    for (int n1 = threadIdx.x; n1 < K1; n1 += PBS) b0[n1 * TS + (K2 >> 1)] = make_float2(0.f, 0.f);
+  __syncthreads();
+  staged<16, float, PBS>(n, [&](int e) { return (float)((double)(long long)g[e] * inv_scale); },
+                         [&](int e, float v) {
+                           const int n2 = e / K1, n1 = e - n2 * K1;
+                           reinterpret_cast<float *>(&b0[n1 * TS + (n2 >> 1)])[n2 & 1] = v;
+                         });
+  __syncthreads();
+  float2 *r = lds_fft<TP, BIG, TS>(b0, b1, p1, tw1, -1.0f);
+  float2 *o = r == b0 ? b1 : b0;
+  // A[m] = (Z[m] + conj Z[K-m])/2 (line 2c), B[m] = (Z[m] - conj Z[K-m])/(2i) (line 2c+1), stored
+  // as the y lines: element n2, line m1
+  for (int e = threadIdx.x; e < K2 * K1h; e += PBS) {
+    const int n2 = e / K1h, m = e - n2 * K1h;
+    const int c = n2 >> 1;
+    const float2 z = r[m * TS + c];
+    const float2 w = r[(m == 0 ? 0 : K1 - m) * TS + c];
+    o[n2 * TS + m] = (n2 & 1) ? make_float2(0.5f * (z.y + w.y), 0.5f * (w.x - z.x))
+                              : make_float2(0.5f * (z.x + w.x), 0.5f * (z.y - w.y));
+  }
+  __syncthreads();
+  r = lds_fft<TP, BIG, TS>(o, r, p2, tw2, -1.0f);
+  float2 *out = spec + (size_t)n3 * K2 * K1h;
+  for (int e = threadIdx.x; e < K2 * K1h; e += PBS) {
+    const int m2 = e / K1h, m1 = e - m2 * K1h;
+    out[e] = r[m2 * TS + m1];
+  }
+}
+
+// Forward z transform, influence multiply with the FP64 energy / virial partials, inverse z
+// transform of the (m1, m3) plane m2 (all K1h lines in one workgroup).
+template <int TP, bool ENERGY, bool BIG>
+__global__ __launch_bounds__(PBS) void k_plane_z(float2 *spec, FFTPlan p3, const float2 *__restrict__ tw,
+                                                 SolveArgs s) {
+  extern __shared__ float2 lds[];
+  const int K1h = s.K1h, K2 = s.K2, K3 = s.K3;
+  constexpr int TS = TP + 1;
+  float2 *twl = lds, *b0 = lds + K3, *b1 = b0 + (size_t)TS * K3;
+  for (int k = threadIdx.x; k < K3; k += PBS) twl[k] = tw[k];
+  const int m2 = blockIdx.x;
+  const size_t plane = (size_t)K1h * K2;
+  const size_t base = (size_t)m2 * K1h;
+  const int n = K3 * K1h;
+  staged<8, float2, PBS>(n, [&](int e) {
+    const int m3 = e / K1h, m1 = e - m3 * K1h;
+    return spec[base + (size_t)m3 * plane + m1];
+  }, [&](int e, float2 v) {
+    const int m3 = e / K1h, m1 = e - m3 * K1h;
+    b0[m3 * TS + m1] = v;
+  });
+  __syncthreads();
+  float2 *r = lds_fft<TP, BIG, TS>(b0, b1, p3, twl, -1.0f);
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  solve_lines<TP, ENERGY, TS>(r, 0, m2, s, acc);
+  __syncthreads();
+  r = lds_fft<TP, BIG, TS>(r, r == b0 ? b1 : b0, p3, twl, +1.0f);
+  for (int e = threadIdx.x; e < n; e += PBS) {
+    const int m3 = e / K1h, m1 = e - m3 * K1h;
+    spec[base + (size_t)m3 * plane + m1] = r[m3 * TS + m1];
+  }
+  if (ENERGY) block_partials<7>(acc, s.partial);
+}
+
+// Inverse y and x (C2R) transforms of z plane m3 into the potential grid; block 0 first adds
+// the z pass's energy / virial partials.
+template <int TP, bool BIG>
+__global__ __launch_bounds__(PBS) void k_plane_inv(const float2 *__restrict__ spec, float *pot, int K1, int K2,
+                                                   int K1h, int KM, FFTPlan p1, FFTPlan p2,
+                                                   const float2 *__restrict__ tw, const double *psum, int npsum,
+                                                   double *psum_out) {
+  extern __shared__ float2 lds[];
+  if (psum && blockIdx.x == 0) sum_partials_any<7>(psum, npsum, psum_out);
+  constexpr int TS = TP + 1;  // padded row stride: column walks of the transposes hit distinct banks
+  float2 *tw1 = lds, *tw2 = lds + K1, *b0 = tw2 + K2, *b1 = b0 + (size_t)KM * TS;
+  for (int k = threadIdx.x; k < K1 + K2; k += PBS) lds[k] = tw[k];
+  const int m3 = blockIdx.x;
+  const float2 *in = spec + (size_t)m3 * K2 * K1h;
+  staged<8, float2, PBS>(K2 * K1h, [&](int e) { return in[e]; }, [&](int e, float2 v) {
+    const int m2 = e / K1h, m1 = e - m2 * K1h;
+    b0[m2 * TS + m1] = v;
+  });
+  __syncthreads();
+  float2 *r = lds_fft<TP, BIG, TS>(b0, b1, p2, tw2, +1.0f);
+  float2 *o = r == b0 ? b1 : b0;
+  // x lines: Z = A + i B with A, B the half-spectrum lines n2 = 2c, 2c+1 completed by Hermitian
+  // symmetry (imaginary parts of the DC and Nyquist terms dropped)
+  const int npair = (K2 + 1) >> 1;
+  for (int e = threadIdx.x; e < npair * K1; e += PBS) {
+    const int c = e / K1, m = e - c * K1;
+    const int la = 2 * c, lb = la + 1;
+    const bool conj = m >= K1h;
+    const int mm = conj ? K1 - m : m;
+    float2 A = r[la * TS + mm];
+    float2 B = lb < K2 ? r[lb * TS + mm] : make_float2(0.f, 0.f);
+    if (mm == 0 || 2 * mm == K1) { A.y = 0.f; B.y = 0.f; }
+    if (conj) { A.y = -A.y; B.y = -B.y; }
+    o[m * TS + c] = make_float2(A.x - B.y, A.y + B.x);
+  }
+  __syncthreads();
+  r = lds_fft<TP, BIG, TS>(o, r, p1, tw1, +1.0f);
+  float *out = pot + (size_t)m3 * K1 * K2;
+  for (int e = threadIdx.x; e < K1 * K2; e += PBS) {
+    const int n2 = e / K1, n1 = e - n2 * K1;
+    const float2 z = r[n1 * TS + (n2 >> 1)];
+    out[e] = (n2 & 1) ? z.y : z.x;
   }
 }
 
@@ -1239,6 +1446,86 @@ void launch_z(pdx_mesh_t m, const Plans &pl, hipStream_t st, float2 *spec = null
 #undef PDX_ZZ
 }
 
+// Plane passes (the default when a plane fits the LDS): lines of every plane at most 64
+// (K1/2 + 1, ceil(K2/2)), sides at most 128 with radices 2, 3, 4, 5 only, not the ik path;
+// PDX_FFT_PLANE=0 turns them off.
+int plane_lines(pdx_mesh_t m) {
+  if (const char *v = std::getenv("PDX_FFT_PLANE"))
+    if (std::atoi(v) == 0) return 0;
+  if (m->spec.influence == PDX_INFLUENCE_P3M_IK) return 0;
+  if (m->K1 > 128 || m->K2 > 128 || m->K3 > 128) return 0;
+  FFTPlan p;  // radices 7, 11, 13 need more registers than a 1024-thread workgroup leaves
+  for (int K : {m->K1, m->K2, m->K3})
+    if (!make_plan(K, p) || p.big) return 0;
+  const int lines = std::max(m->K1h, (m->K2 + 1) / 2);
+  if (lines > 64) return 0;
+  return lines <= 32 ? 32 : 64;
+}
+
+size_t plane_lds_xy(pdx_mesh_t m) {
+  return ((size_t)m->K1 + m->K2 + (size_t)2 * std::max(m->K1, m->K2) * (m->plane_tp + 1)) * sizeof(float2);
+}
+size_t plane_lds_z(pdx_mesh_t m) { return ((size_t)m->K3 + (size_t)2 * m->K3 * (m->plane_tp + 1)) * sizeof(float2); }
+
+#define PDX_PLANE_SWITCH(CALL)                                    \
+  (void)big;                                                       \
+  if (m->plane_tp == 32) {                                         \
+    CALL(32, false)                                                \
+  } else {                                                         \
+    CALL(64, false)                                                \
+  }
+
+int plane_attributes(pdx_mesh_t m, const Plans &pl) {
+  const size_t lxy = plane_lds_xy(m), lz = plane_lds_z(m);
+  bool big = pl.p1.big || pl.p2.big;
+#define PDX_PA_XY(TP, BB)                                                                                     \
+  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_fwd<TP, BB>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
+                              (int)lxy));                                                                     \
+  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_inv<TP, BB>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
+                              (int)lxy));
+  PDX_PLANE_SWITCH(PDX_PA_XY)
+#undef PDX_PA_XY
+  big = pl.p3.big;
+#define PDX_PA_Z(TP, BB)                                                                                     \
+  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_z<TP, true, BB>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                              (int)lz));                                                                      \
+  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_z<TP, false, BB>, hipFuncAttributeMaxDynamicSharedMemorySize,\
+                              (int)lz));
+  PDX_PLANE_SWITCH(PDX_PA_Z)
+#undef PDX_PA_Z
+  return PDX_OK;
+}
+
+// The SPME / P3M-AD solve as three plane launches.
+int enqueue_plane_solve(pdx_mesh_t m, const Plans &pl, bool energy, hipStream_t st) {
+  const int KM = std::max(m->K1, m->K2);
+  const size_t lxy = plane_lds_xy(m), lz = plane_lds_z(m);
+  const float2 *tw1 = m->tw.p, *tw3 = m->tw.p + m->K1 + m->K2;
+  bool big = pl.p1.big || pl.p2.big;
+#define PDX_PF(TP, BB)                                                                                        \
+  hipLaunchKernelGGL((k_plane_fwd<TP, BB>), dim3(m->K3), dim3(PBS), lxy, st, (const unsigned long long *)m->grid64.p, \
+                     m->spec_c.p, m->K1, m->K2, m->K1h, KM, pl.p1, pl.p2, tw1, m->fixed_inv);
+  PDX_PLANE_SWITCH(PDX_PF)
+#undef PDX_PF
+  SolveArgs sa = solve_args(m);
+  big = pl.p3.big;
+#define PDX_PZ(TP, BB)                                                                                        \
+  if (energy) hipLaunchKernelGGL((k_plane_z<TP, true, BB>), dim3(m->K2), dim3(PBS), lz, st, m->spec_c.p, pl.p3, tw3, sa); \
+  else hipLaunchKernelGGL((k_plane_z<TP, false, BB>), dim3(m->K2), dim3(PBS), lz, st, m->spec_c.p, pl.p3, tw3, sa);
+  PDX_PLANE_SWITCH(PDX_PZ)
+#undef PDX_PZ
+  big = pl.p1.big || pl.p2.big;
+  const double *ps = energy ? m->partial.p : nullptr;
+  double *po = energy ? m->acc.p : nullptr;
+#define PDX_PI(TP, BB)                                                                                        \
+  hipLaunchKernelGGL((k_plane_inv<TP, BB>), dim3(m->K3), dim3(PBS), lxy, st, (const float2 *)m->spec_c.p, m->pot.p, \
+                     m->K1, m->K2, m->K1h, KM, pl.p1, pl.p2, tw1, ps, m->nmode_blocks, po);
+  PDX_PLANE_SWITCH(PDX_PI)
+#undef PDX_PI
+  PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
 #define PDX_SPREAD_CASE(PP)                                                                                    \
   case PP:                                                                                                     \
     hipLaunchKernelGGL(k_spread<PP>, dim3(((int64_t)N * lanes_for(PP) + TPB - 1) / TPB), dim3(TPB), 0, st, N, \
@@ -1340,7 +1627,8 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
   m->bgcoef = M_PI / (2.0 * alpha * alpha);       // background_coefficient (Splits.jl:85)
   const size_t nmodes = (size_t)m->K1h * m->K2 * m->K3;
   const Plans pl = plans_of(m);
-  m->nmode_blocks = ((m->K1h + pl.T3 - 1) / pl.T3) * m->K2;
+  m->plane_tp = plane_lines(m);
+  m->nmode_blocks = m->plane_tp ? m->K2 : ((m->K1h + pl.T3 - 1) / pl.T3) * m->K2;
   int rc;
   if ((rc = m->grid64.alloc(nnodes)) || (rc = m->spec_c.alloc(nmodes)) || (rc = m->pot.alloc(nnodes)) ||
       (rc = m->G.alloc(nmodes)) || (rc = m->tw.alloc(K[0] + K[1] + K[2])) || (rc = m->acc.alloc(8)) ||
@@ -1367,7 +1655,7 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
     delete m;
     return fail(PDX_ERR_DEVICE, "mesh initialisation failed");
   }
-  if ((rc = compute_influence(m)) || (rc = upload_twiddles(m))) {
+  if ((rc = compute_influence(m)) || (rc = upload_twiddles(m)) || (m->plane_tp && (rc = plane_attributes(m, pl)))) {
     delete m;
     return rc;
   }
@@ -1387,10 +1675,11 @@ extern "C" int pdx_mesh_destroy(pdx_mesh_t m) {
   return PDX_OK;
 }
 
-extern "C" int pdx_mesh_info(pdx_mesh_t m, double out[8]) {
+extern "C" int pdx_mesh_info(pdx_mesh_t m, double out[9]) {
   if (!m || !out) return fail(PDX_ERR_ARGUMENT, "null argument");
   out[0] = m->K1; out[1] = m->K2; out[2] = m->K3; out[3] = m->P;
   out[4] = m->alpha; out[5] = m->C; out[6] = m->selfc; out[7] = m->bg;
+  out[8] = m->plane_tp;
   return PDX_OK;
 }
 
@@ -1501,9 +1790,10 @@ extern "C" int pdx_mesh_solve(pdx_mesh_t m, int32_t flags, pdx_stream_t stream) 
   if (!m) return fail(PDX_ERR_ARGUMENT, "null mesh");
   hipStream_t st = as_stream(stream);
   const Plans pl = plans_of(m);
+  const bool energy = (flags & PDX_ENERGY) != 0;
+  if (m->plane_tp) return enqueue_plane_solve(m, pl, energy, st);
   int rc = enqueue_forward_xy(m, pl, m->grid64.p, true, st);
   if (rc) return rc;
-  const bool energy = (flags & PDX_ENERGY) != 0;
   if (m->spec.influence != PDX_INFLUENCE_P3M_IK) {
     if (energy) launch_z<2, true>(m, pl, st);
     else launch_z<2, false>(m, pl, st);

@@ -140,12 +140,13 @@ class ElectrostaticsSolver:
         self._h = h
         self.box = tuple(float(b) for b in box)
         self.set_charges(q)
-        info = (C.c_double * 8)()
+        info = (C.c_double * 9)()
         L.check(L.lib.pdx_mesh_info(self._h, info))
         self.mesh = (int(info[0]), int(info[1]), int(info[2]))
         self.order = int(info[3])
         self.alpha = info[4]
         self.C = info[5]
+        self.plane_lines = int(info[8])
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -102,6 +102,38 @@ def test_spread_cells_equals_atomic_spread(cuda, P, mesh):
         assert torch.equal(ga, gb), f"move {move}"
 
 
+@pytest.mark.parametrize("mesh", [(120, 120, 120), (60, 60, 60), (48, 45, 50), (64, 128, 30)])
+def test_plane_transforms_equal_line_passes(cuda, mesh, monkeypatch):
+    """The plane transforms (x and y of a z plane in one workgroup, the z solve of a y plane in
+    another) give the line passes' potential grid bit for bit, and the FP64 energy and virial up
+    to the order of the partial sums. With an odd K2 the line passes pair real x lines across
+    plane boundaries while the plane pass pairs them inside a plane (the odd one with zero), so
+    there the grids agree to FP32 rounding."""
+    import torch
+    N = 2000
+    box = (30.0, 31.0, 32.0)
+    R, q = systems.random_cloud(N, box, seed=1)
+    X, Y, Z, Q = dev(R[:, 0]), dev(R[:, 1]), dev(R[:, 2]), dev(q)
+    res = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("PDX_FFT_PLANE", flag)
+        s = _solver(mesh, P=5, box=box, N=N, q=q, alpha=0.35, rc=9.0)
+        assert (s.plane_lines > 0) == (flag == "1"), s.plane_lines
+        s.spread(X, Y, Z, Q)
+        s.solve(energy=True)
+        res[flag] = (s.potential_grid().clone(), s.totals())
+    t1, t0 = res["1"][1], res["0"][1]
+    if mesh[1] % 2 == 0:
+        assert torch.equal(res["1"][0], res["0"][0])
+        tol = 1e-12
+    else:
+        d = (res["1"][0] - res["0"][0]).abs().max().item()
+        assert d <= 2e-6 * res["0"][0].abs().max().item(), d
+        tol = 1e-6
+    assert t1["mesh_energy"] == pytest.approx(t0["mesh_energy"], rel=tol)
+    assert np.allclose(t1["virial"], t0["virial"], rtol=tol, atol=tol * abs(t0["mesh_energy"]))
+
+
 def _es_eval(R, q, L, spec, skin_frac=0.075):
     """LJ-free Coulomb evaluation: real space on the rows + mesh, energy and virial."""
     import torch
