@@ -34,7 +34,9 @@ def parse():
     p.add_argument("--lanes", type=int, default=0)
     p.add_argument("--no-clusters", action="store_true", help="skip the cluster-pair backend leg")
     p.add_argument("--skin", type=float, default=1.0)
-    p.add_argument("--spread", choices=("cells", "atomic"), default="cells")
+    p.add_argument("--overlap-with", choices=("spread", "solve"), default="spread",
+                   help="what the pair kernel overlaps: the whole mesh branch, or the solve only")
+    p.add_argument("--spread", choices=("snapshot", "cells", "atomic"), default="snapshot")
     p.add_argument("--traversal", choices=("rows", "cells"), default="rows",
                    help="pair traversal: the list's ELL rows, or its cell grid without rows")
     p.add_argument("--no-learned", action="store_true", help="skip the learned-correction comparison")
@@ -104,7 +106,7 @@ def main():
     spec = pdx.PME(coulomb_prefactor=C, real_cutoff=9.0, real_space_tolerance=1e-5, mesh_spacing=1.0, order=5)
     lj = pdx.LennardJones(epsilon=0.01, sigma=2.8, cutoff=9.0)
     eng = pdx.ForceEngine(N, box, lj, electrostatics=spec, charges=sysd["q"], skin=args.skin, lanes=args.lanes,
-                          spread=args.spread, traversal=args.traversal)
+                          spread=args.spread, traversal=args.traversal, overlap_with=args.overlap_with)
     X, Y, Z, Q = (torch.from_numpy(sysd[k]).cuda() for k in ("x", "y", "z", "q"))
     fx, fy, fz, E = (torch.empty(N, device="cuda") for _ in range(4))
     V = torch.empty(6, N, device="cuda")
@@ -146,7 +148,7 @@ def main():
         marks[1].record(stream)
         pf.compute(nl, fx, fy, fz, E, V, eng.pair_totals, energy=True)
         marks[2].record(stream)
-        if eng.spread_mode == "cells":
+        if eng.spread_mode in ("cells", "snapshot"):
             sol.spread_cells(nl)
         else:
             sol.spread(X, Y, Z, Q, order=order)
@@ -208,7 +210,8 @@ def main():
            "config": {"workload": "rock-salt 64000 ions, LJ + SPME 120^3 P=5, rc 9 A, tol 1e-5, plain PME",
                       "N": N, "mesh": list(sol.mesh), "order": sol.order, "alpha": sol.alpha, "skin": args.skin,
                       "cap": nl.cap, "lanes": args.lanes, "mode": args.mode, "spread": args.spread,
-                      "traversal": args.traversal, "parallelism": "single GPU"},
+                      "traversal": args.traversal, "overlap_with": args.overlap_with,
+                      "parallelism": "single GPU"},
            "ns_per_day": round(ns_day, 2), "ns_per_day_note": f"dt=1 fs, one rebuild every {interval} steps",
            "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()}, "rebuild_ms": round(rebuild_ms, 4),
            "energy_eV": tot["energy"], "roofline": roofline}

@@ -240,6 +240,11 @@ struct pdx_nl_s {
   pdx::DevBuf<int32_t> scan_block;    // per-1024-cell block totals
   pdx::DevBuf<int64_t> candidates;    // N (diagnostics)
   pdx::DevBuf<unsigned> tickets;      // last-block tickets (decide, commit)
+  // cell snapshot for a spread that runs beside the rebuild: offsets and ids of the last
+  // completed build; snap_state[0] = build count copied, [2] = nonzero while none is valid
+  pdx::DevBuf<int32_t> snap_offsets, snap_ids;
+  pdx::DevBuf<int64_t> snap_state;
+  int snap_host = 0;  // 0: none since create / box change, 1: an update followed, 2: snapshot enqueued
 };
 
 struct pdx_pair_s {

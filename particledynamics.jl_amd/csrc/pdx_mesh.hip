@@ -785,8 +785,8 @@ __global__ void k_zero64(unsigned long long *g, size_t n) {
 // Tile edge: 16 nodes, or 8 on meshes too small to give 256 tiles of 16 (one per CU).
 constexpr int SLIST = 768;   // particles reaching one block staged in LDS (a few hundred in practice)
 constexpr int SCHUNK = 64;   // staged particles whose weights are computed together
-template <int P, int STILE>
-__global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__ packed,
+template <int P, int STILE, int SBS>
+__global__ __launch_bounds__(SBS) void k_spread_tiles(const float4 *__restrict__ packed,
                                                       const int32_t *__restrict__ offsets,
                                                       const int32_t *__restrict__ ids,
                                                       const int64_t *__restrict__ state, int nx, int ny,
@@ -804,7 +804,7 @@ __global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__
   const int b = blockIdx.x;
   const int a1 = (b % tx) * STILE, a2 = ((b / tx) % ty) * STILE, a3 = (b / (tx * ty)) * STILE;
   const int e1 = min(STILE, K1 - a1), e2 = min(STILE, K2 - a2), e3 = min(STILE, K3 - a3);
-  for (int k = threadIdx.x; k < STILE * STILE * SROW; k += TPB) acc[k] = 0ull;
+  for (int k = threadIdx.x; k < STILE * STILE * SROW; k += SBS) acc[k] = 0ull;
   // candidate cells per dimension, in the list's frame (x + L/2) / L
   const int K[3] = {K1, K2, K3}, A[3] = {a1, a2, a3}, E[3] = {e1, e2, e3}, NC[3] = {nx, ny, nz};
   const float Ls[3] = {Lx, Ly, Lz};
@@ -842,7 +842,7 @@ __global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__
     __syncthreads();
     const int total = cpre[nc];
     // flat candidate index -> cell (linear search over <= 64 prefix entries) -> particle
-    for (int f = threadIdx.x; f < total; f += TPB) {
+    for (int f = threadIdx.x; f < total; f += SBS) {
       int c = 0;
       while (cpre[c + 1] <= f) ++c;
       const float4 p = packed[ids[cstart[c] + (f - cpre[c])] - 1];
@@ -890,7 +890,7 @@ __global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__
   for (int c0 = 0; c0 < nl; c0 += SCHUNK) {
     const int nc = min(SCHUNK, nl - c0);
     __syncthreads();
-    for (int e = threadIdx.x; e < 3 * nc; e += TPB) {
+    for (int e = threadIdx.x; e < 3 * nc; e += SBS) {
       const int t = e / 3, d = e - 3 * t;
       const float4 u = list[c0 + t];
       float w[P], dw[P];
@@ -899,7 +899,7 @@ __global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__
       for (int j = 0; j < P; ++j) wsh[t][d][j] = w[j];
     }
     __syncthreads();
-    for (int t = grp; t < nc; t += TPB / LPP) {
+    for (int t = grp; t < nc; t += SBS / LPP) {
       const float qv = list[c0 + t].w;
       const int k1 = ksh[t][0], k2 = ksh[t][1], k3 = ksh[t][2];
       for (int l = lane; l < P * P; l += LPP) {
@@ -919,7 +919,7 @@ __global__ __launch_bounds__(TPB) void k_spread_tiles(const float4 *__restrict__
   }
   __syncthreads();
   const size_t KK = (size_t)K1 * K2;
-  for (int k = threadIdx.x; k < STILE * STILE * STILE; k += TPB) {
+  for (int k = threadIdx.x; k < STILE * STILE * STILE; k += SBS) {
     const int l1 = k % STILE, l2 = (k / STILE) % STILE, l3 = k / (STILE * STILE);
     if (l1 < e1 && l2 < e2 && l3 < e3)
       grid[(size_t)(a3 + l3) * KK + (size_t)(a2 + l2) * K1 + (a1 + l1)] = acc[(l3 * STILE + l2) * SROW + l1];
@@ -1746,7 +1746,11 @@ extern "C" int pdx_mesh_spread(pdx_mesh_t m, const float *x, const float *y, con
   return PDX_OK;
 }
 
-extern "C" int pdx_mesh_spread_cells(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t stream) {
+namespace {
+// Owner-computes spread from the cells of nl: the current cells (margin skin/2, the most a
+// particle moved since they were built) or the snapshot of the last completed build (margin skin:
+// one step past a rebuild trigger moves no particle by another skin/2).
+int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t stream) {
   if (!m || !nl) return fail(PDX_ERR_ARGUMENT, "null argument");
   if (nl->N != m->N) return fail(PDX_ERR_DIMENSION, "neighbor list and mesh particle counts differ");
   if (nl->allpairs) return fail(PDX_ERR_ARGUMENT, "all-pairs lists have no cells; use pdx_mesh_spread");
@@ -1759,18 +1763,26 @@ extern "C" int pdx_mesh_spread_cells(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t str
   bool small = ntiles(16) < 256;
   if (const char *e = std::getenv("PDX_SPREAD_TILE")) small = std::atoi(e) == 8;
   const int tiles = ntiles(small ? 8 : 16);
-  const float margin = 0.5f * (float)nl->skin;
+  const float margin = (snapshot ? 1.0f : 0.5f) * (float)nl->skin;
+  const int32_t *offsets = snapshot ? nl->snap_offsets.p : nl->cell_offsets.p;
+  const int32_t *ids = snapshot ? nl->snap_ids.p : nl->particle_ids_sorted.p;
+  const int64_t *guard = snapshot ? nl->snap_state.p : nl->state.p;  // word 2 nonzero: no usable cells
   hipStream_t st = as_stream(stream);
+  // workgroup size: 256 threads by default; 1024 (32 particles at a time per tile) is faster for
+  // the spread alone but slower while it shares the CUs with the pair kernel
+  int sbs = 256;
+  if (const char *e = std::getenv("PDX_SPREAD_BS")) sbs = std::atoi(e) == 256 ? 256 : 1024;
+#define PDX_TSPREAD_L(PP, TT, BB)                                                                              \
+  hipLaunchKernelGGL((k_spread_tiles<PP, TT, BB>), dim3(tiles), dim3(BB), 0, st, nl->packed.p,                 \
+                     offsets, ids, guard, nl->nx, nl->ny, nl->nz,                                              \
+                     margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale);
 #define PDX_TSPREAD(PP)                                                                                        \
   case PP:                                                                                                     \
-    if (small)                                                                                                 \
-      hipLaunchKernelGGL((k_spread_tiles<PP, 8>), dim3(tiles), dim3(TPB), 0, st, nl->packed.p,                  \
-                         nl->cell_offsets.p, nl->particle_ids_sorted.p, nl->state.p, nl->nx, nl->ny, nl->nz,     \
-                         margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale);                    \
-    else                                                                                                       \
-      hipLaunchKernelGGL((k_spread_tiles<PP, 16>), dim3(tiles), dim3(TPB), 0, st, nl->packed.p,                 \
-                         nl->cell_offsets.p, nl->particle_ids_sorted.p, nl->state.p, nl->nx, nl->ny, nl->nz,     \
-                         margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale);                    \
+    if (small) {                                                                                               \
+      if (sbs == 1024) { PDX_TSPREAD_L(PP, 8, 1024) } else { PDX_TSPREAD_L(PP, 8, 256) }                       \
+    } else {                                                                                                   \
+      if (sbs == 1024) { PDX_TSPREAD_L(PP, 16, 1024) } else { PDX_TSPREAD_L(PP, 16, 256) }                     \
+    }                                                                                                          \
     break;
   switch (m->P) {
     PDX_TSPREAD(3)
@@ -1782,8 +1794,20 @@ extern "C" int pdx_mesh_spread_cells(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t str
     default: return fail(PDX_ERR_ARGUMENT, "unsupported order");
   }
 #undef PDX_TSPREAD
+#undef PDX_TSPREAD_L
   PDX_LAUNCH_CHECK();
   return PDX_OK;
+}
+}  // namespace
+
+extern "C" int pdx_mesh_spread_cells(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t stream) {
+  return spread_cells_impl(m, nl, false, stream);
+}
+
+extern "C" int pdx_mesh_spread_snapshot(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t stream) {
+  if (nl && nl->snap_host != 2)
+    return fail(PDX_ERR_STATE, "no cell snapshot: call pdx_nl_snapshot after an update (and after a box change)");
+  return spread_cells_impl(m, nl, true, stream);
 }
 
 extern "C" int pdx_mesh_solve(pdx_mesh_t m, int32_t flags, pdx_stream_t stream) {

@@ -507,6 +507,23 @@ __global__ void k_rearm(int64_t *state) {
 
 __global__ void k_invalidate(int64_t *state) { state[PDX_DN_VALID] = 0; }
 
+// Cell snapshot: offsets and ids of a build that completed since the last snapshot (the build
+// counter moved), then the recorded count. Run after every step's evaluation, so that a spread
+// running beside the next step's rebuild reads cells no rebuild is writing.
+__global__ __launch_bounds__(TPB) void k_snap_copy(int N, int ncell, const int32_t *__restrict__ offsets,
+                                                   const int32_t *__restrict__ ids, int32_t *so, int32_t *sid,
+                                                   const int64_t *state, const int64_t *snap) {
+  if (state[PDX_DN_ERROR] != 0 || state[PDX_DN_BUILDS] == snap[0]) return;
+  const int k = blockIdx.x * TPB + threadIdx.x;
+  if (k <= ncell) so[k] = offsets[k];
+  if (k < N) sid[k] = ids[k];
+}
+__global__ void k_snap_mark(const int64_t *state, int64_t *snap) {
+  if (state[PDX_DN_ERROR] != 0 || state[PDX_DN_BUILDS] == snap[0]) return;
+  snap[0] = state[PDX_DN_BUILDS];
+  snap[2] = 0;
+}
+
 inline int blocks_for(int64_t n, int tpb) { return (int)((n + tpb - 1) / tpb); }
 
 int grid_for_box(const double box[3], double rl, int32_t out[3]) {
@@ -571,8 +588,8 @@ int enqueue_rebuild(pdx_nl_t nl, hipStream_t st) {
   return PDX_OK;
 }
 
-int enqueue_update(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
-                   hipStream_t st, int force) {
+int enqueue_check(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q, hipStream_t st,
+                  int force) {
   if (!nl) return fail(PDX_ERR_ARGUMENT, "null neighbor list");
   if (!x || !y || !z) return fail(PDX_ERR_ARGUMENT, "null coordinate array");
   const int N = nl->N;
@@ -584,6 +601,14 @@ int enqueue_update(pdx_nl_t nl, const float *x, const float *y, const float *z, 
   hipLaunchKernelGGL(k_decide_final, dim3(1), dim3(TPB), 0, st, nb, nl->disp_partial.p, nl->state.p, threshold,
                      force);
   PDX_LAUNCH_CHECK();
+  if (nl->snap_host == 0) nl->snap_host = 1;
+  return PDX_OK;
+}
+
+int enqueue_update(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
+                   hipStream_t st, int force) {
+  int rc = enqueue_check(nl, x, y, z, q, st, force);
+  if (rc) return rc;
   return enqueue_rebuild(nl, st);
 }
 
@@ -649,7 +674,9 @@ extern "C" int pdx_nl_create(int32_t n, const double box[3], double cutoff, doub
       (rc = nl->cell_rank.alloc(n)) || (rc = nl->rref.alloc((size_t)3 * n)) || (rc = nl->packed.alloc(n)) ||
       (rc = nl->sorted.alloc(n)) || (rc = nl->sortedq.alloc(n)) || (rc = nl->disp_partial.alloc(nb)) || (rc = nl->cap_partial.alloc(nb)) ||
       (rc = nl->stat_partial.alloc((size_t)2 * nb)) || (rc = nl->scan_block.alloc(sblocks)) ||
-      (rc = nl->candidates.alloc(nl->diagnostics ? n : 1)) || (rc = nl->tickets.alloc(4))) {
+      (rc = nl->candidates.alloc(nl->diagnostics ? n : 1)) || (rc = nl->tickets.alloc(4)) ||
+      (rc = nl->snap_offsets.alloc((size_t)nl->ncell + 1)) || (rc = nl->snap_ids.alloc(n)) ||
+      (rc = nl->snap_state.alloc(4))) {
     delete nl;
     return rc;
   }
@@ -663,6 +690,8 @@ extern "C" int pdx_nl_create(int32_t n, const double box[3], double cutoff, doub
   for (int32_t k = 0; k < n; ++k) ident[k] = k + 1;
   hipError_t e = hipSuccess;
   if (e == hipSuccess) e = hipMemcpy(nl->state.p, st, sizeof(st), hipMemcpyHostToDevice);
+  const int64_t snap0[4] = {-1, 0, 1, 0};
+  if (e == hipSuccess) e = hipMemcpy(nl->snap_state.p, snap0, sizeof(snap0), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(nl->geom.p, geomf, sizeof(geomf), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(nl->neighbors_flat.p, 0xff, sizeof(int32_t) * (size_t)n * cap);  // -1 padding
   if (e == hipSuccess) e = hipMemset(nl->counts.p, 0, sizeof(int32_t) * n);
@@ -719,6 +748,31 @@ extern "C" int pdx_nl_rebuild(pdx_nl_t nl, const float *x, const float *y, const
   return enqueue_update(nl, x, y, z, q, as_stream(stream), 1);
 }
 
+extern "C" int pdx_nl_check(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
+                            pdx_stream_t stream) {
+  return enqueue_check(nl, x, y, z, q, as_stream(stream), 0);
+}
+
+extern "C" int pdx_nl_build(pdx_nl_t nl, pdx_stream_t stream) {
+  if (!nl) return fail(PDX_ERR_ARGUMENT, "null neighbor list");
+  return enqueue_rebuild(nl, as_stream(stream));
+}
+
+extern "C" int pdx_nl_snapshot(pdx_nl_t nl, pdx_stream_t stream) {
+  if (!nl) return fail(PDX_ERR_ARGUMENT, "null neighbor list");
+  if (nl->allpairs) return fail(PDX_ERR_ARGUMENT, "all-pairs lists have no cells to snapshot");
+  hipStream_t st = as_stream(stream);
+  const int n = std::max(nl->N, nl->ncell + 1);
+  hipLaunchKernelGGL(k_snap_copy, dim3(blocks_for(n, TPB)), dim3(TPB), 0, st, nl->N, nl->ncell,
+                     (const int32_t *)nl->cell_offsets.p, (const int32_t *)nl->particle_ids_sorted.p,
+                     nl->snap_offsets.p, nl->snap_ids.p, (const int64_t *)nl->state.p,
+                     (const int64_t *)nl->snap_state.p);
+  hipLaunchKernelGGL(k_snap_mark, dim3(1), dim3(1), 0, st, (const int64_t *)nl->state.p, nl->snap_state.p);
+  PDX_LAUNCH_CHECK();
+  if (nl->snap_host == 1) nl->snap_host = 2;
+  return PDX_OK;
+}
+
 extern "C" int pdx_nl_advance(pdx_nl_t nl, pdx_stream_t stream) {
   if (!nl) return fail(PDX_ERR_ARGUMENT, "null neighbor list");
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, as_stream(stream), nl->state.p);
@@ -763,7 +817,7 @@ extern "C" int pdx_nl_set_box(pdx_nl_t nl, const double box[3], pdx_stream_t str
       PDX_HIP(hipStreamSynchronize(as_stream(stream)));
       const int ncell = g[0] * g[1] * g[2];
       if ((rc = nl->cell_offsets.alloc(ncell + 1)) || (rc = nl->cell_counts.alloc(ncell)) ||
-          (rc = nl->scan_block.alloc(blocks_for(ncell, SCAN_BLOCK))))
+          (rc = nl->scan_block.alloc(blocks_for(ncell, SCAN_BLOCK))) || (rc = nl->snap_offsets.alloc(ncell + 1)))
         return rc;
       std::vector<int32_t> ones(ncell + 1, 1);
       PDX_HIP(hipMemcpy(nl->cell_offsets.p, ones.data(), sizeof(int32_t) * (ncell + 1), hipMemcpyHostToDevice));
@@ -772,6 +826,12 @@ extern "C" int pdx_nl_set_box(pdx_nl_t nl, const double box[3], pdx_stream_t str
     }
   }
   for (int d = 0; d < 3; ++d) nl->box[d] = box[d];
+  {  // cells of the old box are no cover for the new one: no snapshot until the next build
+    PDX_HIP(hipStreamSynchronize(as_stream(stream)));
+    const int64_t snap0[4] = {-1, 0, 1, 0};
+    PDX_HIP(hipMemcpy(nl->snap_state.p, snap0, sizeof(snap0), hipMemcpyHostToDevice));
+    nl->snap_host = 0;
+  }
   const float geomf[3] = {(float)box[0], (float)box[1], (float)box[2]};
   PDX_HIP(hipMemcpyAsync(nl->geom.p, geomf, sizeof(geomf), hipMemcpyHostToDevice, as_stream(stream)));
   hipLaunchKernelGGL(k_invalidate, dim3(1), dim3(1), 0, as_stream(stream), nl->state.p);

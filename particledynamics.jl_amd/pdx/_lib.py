@@ -110,6 +110,10 @@ def _load():
         "pdx_mesh_set_box": ([vp, P(dbl)], C.c_int),
         "pdx_mesh_spread": ([vp, vp, vp, vp, vp, vp, vp], C.c_int),
         "pdx_mesh_spread_cells": ([vp, vp, vp], C.c_int),
+        "pdx_mesh_spread_snapshot": ([vp, vp, vp], C.c_int),
+        "pdx_nl_check": ([vp, vp, vp, vp, vp, vp], C.c_int),
+        "pdx_nl_build": ([vp, vp], C.c_int),
+        "pdx_nl_snapshot": ([vp, vp], C.c_int),
         "pdx_mesh_solve": ([vp, i32, vp], C.c_int),
         "pdx_mesh_gather": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp], C.c_int),
         "pdx_mesh_totals": ([vp, P(dbl), vp], C.c_int),

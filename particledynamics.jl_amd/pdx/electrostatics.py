@@ -184,6 +184,10 @@ class ElectrostaticsSolver:
         """Owner-computes spread from ``nl``'s cells and packed (x, y, z, q) (no atomics)."""
         L.check(L.lib.pdx_mesh_spread_cells(self._h, nl.handle, L.stream_handle(stream)))
 
+    def spread_snapshot(self, nl, stream=None):
+        """Owner-computes spread from ``nl``'s cell snapshot (may run beside ``nl.build``)."""
+        L.check(L.lib.pdx_mesh_spread_snapshot(self._h, nl.handle, L.stream_handle(stream)))
+
     def solve(self, energy=True, stream=None):
         L.check(L.lib.pdx_mesh_solve(self._h, L.PDX_ENERGY if energy else 0, L.stream_handle(stream)))
 

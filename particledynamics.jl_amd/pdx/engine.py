@@ -21,12 +21,18 @@ def default_capacity(N, box, rlist):
 
 class ForceEngine:
     def __init__(self, N, box, pair_model=None, electrostatics=None, charges=None, skin=0.4, cap=None,
-                 interval=20, lanes=0, diagnostics=False, spread="cells", correction=None, accumulation="fp32",
-                 bonds=None, exceptions=(), traversal="rows", backend="ell", prune_buffer=None, cluster_capacity=0):
+                 interval=20, lanes=0, diagnostics=False, spread="snapshot", correction=None, accumulation="fp32",
+                 bonds=None, exceptions=(), traversal="rows", backend="ell", prune_buffer=None, cluster_capacity=0,
+                 overlap_with="spread"):
         """backend "ell": the cell list and ELL rows (``pdx_nl_*`` + ``pdx_pair_forces``); "cluster":
         the cluster-pair list and tiles (``pdx_cluster_*``, the reference's ``ClusterBackend``,
         src/engine/ClusterBackend.jl:1-40); its mesh spread takes the cluster order."""
         import torch
+        if overlap_with not in ("spread", "solve"):
+            raise L.ArgumentError("overlap_with must be 'spread' or 'solve'")
+        # "spread": the pair forces run beside the whole mesh branch (spread and solve);
+        # "solve": the spread runs first on the main stream, the pairs beside the solve only
+        self.overlap_with = overlap_with
         if backend not in ("ell", "cluster"):
             raise L.ArgumentError("backend must be 'ell' or 'cluster'")
         self.backend = backend
@@ -59,8 +65,12 @@ class ForceEngine:
         # the mesh branch runs on its own stream (default priority), forked after the list
         # check and joined before the gather adds the mesh forces to the pair forces
         self.mesh_stream = torch.cuda.Stream() if self.solver is not None else None
-        if spread not in ("cells", "atomic"):
-            raise L.ArgumentError("spread must be 'cells' or 'atomic'")
+        # "snapshot": the owner-computes spread over the cell snapshot of the last completed
+        # build, forked right after the list check so that it runs beside the rebuild; "cells":
+        # the same spread over the current cells, after the whole list update; "atomic": the
+        # global-atomic spread in the list's order
+        if spread not in ("snapshot", "cells", "atomic"):
+            raise L.ArgumentError("spread must be 'snapshot', 'cells' or 'atomic'")
         self.spread_mode = spread
         # learned force correction of a coarse mesh (an MLP of the charge-grid features): the
         # features are taken on the mesh stream after the spread, the network's output is
@@ -72,27 +82,44 @@ class ForceEngine:
             from .learned import NFEAT
             self.feats = torch.empty(self.N, NFEAT, dtype=torch.float32, device="cuda")
         self._graph = None
+        self._snap_ready = False
 
     def evaluate(self, x, y, z, q, fx, fy, fz, Epot=None, V=None, energy=True, stream=None, overlap=True):
         """Enqueue one force evaluation (no host synchronisation)."""
         import torch
         main = stream if stream is not None else torch.cuda.current_stream()
+        early = (self.cluster is None and self.solver is not None and overlap and self.spread_mode == "snapshot"
+                 and self.overlap_with == "spread")
         if self.cluster is not None:
             self.cluster.update(x, y, z, q, stream=main)
             order = self.cluster.order_ptr
+        elif early and self._snap_ready:
+            self.nl.check(x, y, z, q, stream=main)  # the rebuild kernels follow after the fork
+            order = self.nl._arrays.particle_ids_sorted
         else:
             self.nl.update(x, y, z, q, stream=main)
+            if early:
+                self.nl.snapshot(stream=main)
+                self._snap_ready = True
+                early = False  # this first evaluation's list is complete already
             order = self.nl._arrays.particle_ids_sorted
         if self.solver is None:
             self._pairs(fx, fy, fz, Epot, V, energy, main)
             return
         side = self.mesh_stream if overlap else main
-        if overlap:
+        spread_stream = main if self.overlap_with == "solve" else side
+        if overlap and spread_stream is side:
             side.wait_stream(main)
-        if self.spread_mode == "cells":
-            self.solver.spread_cells(self.nl, stream=side)
+        if self.spread_mode == "snapshot" and self._snap_ready and overlap and self.cluster is None:
+            self.solver.spread_snapshot(self.nl, stream=spread_stream)
+        elif self.spread_mode in ("cells", "snapshot") and self.cluster is None:
+            self.solver.spread_cells(self.nl, stream=spread_stream)
         else:
-            self.solver.spread(x, y, z, q, order, stream=side)
+            self.solver.spread(x, y, z, q, order, stream=spread_stream)
+        if early:
+            self.nl.build(stream=main)
+        if overlap and spread_stream is main:
+            side.wait_stream(main)
         if self.correction is not None:
             from .learned import mesh_features
             mesh_features(self.solver, x, y, z, q, self.feats, stream=side)
@@ -102,6 +129,8 @@ class ForceEngine:
             main.wait_stream(side)
         self.solver.mesh_gather(x, y, z, q, fx, fy, fz, Epot, self.Vglobal if energy else None, energy,
                                 order=order, set=False, stream=main)
+        if self._snap_ready and self.cluster is None:
+            self.nl.snapshot(stream=main)  # cells of this step's build for the next step's spread
         if self.correction is not None:
             self.correction.forward(self.feats, None, fx, fy, fz, stream=main)
 

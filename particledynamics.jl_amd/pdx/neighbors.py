@@ -78,6 +78,18 @@ class DeviceNeighborWorkspace:
     def rebuild(self, x, y, z, q=None, stream=None):
         L.check(L.lib.pdx_nl_rebuild(self._h, L.ptr(x), L.ptr(y), L.ptr(z), L.ptr(q), L.stream_handle(stream)))
 
+    def check(self, x, y, z, q=None, stream=None):
+        """First half of ``update``: coverage check and packing (no rebuild kernels)."""
+        L.check(L.lib.pdx_nl_check(self._h, L.ptr(x), L.ptr(y), L.ptr(z), L.ptr(q), L.stream_handle(stream)))
+
+    def build(self, stream=None):
+        """Second half of ``update``: the rebuild kernels, predicated on the check's decision."""
+        L.check(L.lib.pdx_nl_build(self._h, L.stream_handle(stream)))
+
+    def snapshot(self, stream=None):
+        """Copy the cells of a build completed since the last snapshot (for spread_snapshot)."""
+        L.check(L.lib.pdx_nl_snapshot(self._h, L.stream_handle(stream)))
+
     def advance(self, stream=None):
         L.check(L.lib.pdx_nl_advance(self._h, L.stream_handle(stream)))
 

@@ -134,6 +134,38 @@ def test_plane_transforms_equal_line_passes(cuda, mesh, monkeypatch):
     assert np.allclose(t1["virial"], t0["virial"], rtol=tol, atol=tol * abs(t0["mesh_energy"]))
 
 
+def test_spread_snapshot_equals_atomic_spread(cuda):
+    """The spread over the cell snapshot of the previous build gives the atomic spread's grid bit
+    for bit, also on the step whose check triggers a rebuild (the snapshot is then a build
+    behind, covered by its full-skin margin), and refuses to run before a snapshot exists."""
+    import torch
+    N = 3000
+    R, q = systems.random_cloud(N, L_ES, seed=7)
+    s = _solver((48, 48, 54), P=5, N=N, q=q)
+    X, Y, Z, Q = dev(R[:, 0]), dev(R[:, 1]), dev(R[:, 2]), dev(q)
+    nl = pdx.DeviceNeighborWorkspace(N, L_ES, 4.0, 0.3, 512)
+    nl.update(X, Y, Z, Q)
+    with pytest.raises(pdx.ArgumentError):
+        s.spread_snapshot(nl)
+    nl.snapshot()
+    builds = []
+    for step in range(4):
+        if step:
+            X = X + 0.1
+            X = X - L_ES[0] * torch.floor(X / L_ES[0] + 0.5)
+            nl.check(X, Y, Z, Q)
+        s.spread_snapshot(nl)
+        ga = s.fixed_grid().clone()
+        if step:
+            nl.build()
+        nl.snapshot()
+        s.spread(X, Y, Z, Q)
+        gb = s.fixed_grid().clone()
+        assert torch.equal(ga, gb), step
+        builds.append(nl.status().builds)
+    assert builds[-1] >= 2, builds  # 0.1 per step against skin/2 = 0.15: rebuilds happened
+
+
 def _es_eval(R, q, L, spec, skin_frac=0.075):
     """LJ-free Coulomb evaluation: real space on the rows + mesh, energy and virial."""
     import torch
