@@ -731,9 +731,18 @@ __device__ __forceinline__ float sel(const float (&v)[P], int j) {
 }
 
 constexpr int lanes_for(int P) { return P * P <= 16 ? 16 : (P * P <= 32 ? 32 : 64); }
-// gather: 8 lanes per particle, each walking several (x, y) lines, so the B-spline weights are
+// gather: 8 lanes per particle, each walking several stencil lines, so the B-spline weights are
 // computed by 8 lanes instead of P^2 and more loads are in flight per lane
 constexpr int gather_lanes(int) { return 8; }
+
+// Sum over the eight lanes 8k .. 8k+7 of a wave64, every lane receiving it: quad permutations
+// (xor 1, xor 2) and the half-row mirror (lane l <-> 7 - l) as DPP operands of three adds.
+__device__ __forceinline__ float sum8_lanes(float v) {
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
+  return v;
+}
 
 // ---- spread: LPP lanes per particle, lane l owns the (x, y) line (l % P, l / P) of the
 // particle's P x P x P stencil and walks its P z nodes with 64-bit integer atomics (the
@@ -937,7 +946,8 @@ __device__ __forceinline__ int xcd_block(int xcd) {
   return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + r;
 }
 
-// ---- gather: same lane layout; line partial sums reduced over the particle's lanes.
+// ---- gather: 8 lanes per particle, lane = (y, z) lines of the stencil walked along x; line
+// partial sums reduced over the particle's lanes.
 template <int P, bool ENERGY>
 __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__ x, const float *__restrict__ y,
                                                 const float *__restrict__ z, const float *__restrict__ q,
@@ -959,30 +969,43 @@ __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__
     const int k1 = bspline<P>(grid_coordinate(x[i], Lx, K1), wx, dx);
     const int k2 = bspline<P>(grid_coordinate(y[i], Ly, K2), wy, dy);
     const int k3 = bspline<P>(grid_coordinate(z[i], Lz, K3), wz, dz);
+    // lane j < P = x node k1 + j of every (y, z) line: one load instruction reads, per particle,
+    // the P consecutive nodes of one line (one cache line) instead of P^2 scattered ones; the
+    // P^2 lines are unrolled, so their loads are independent and the y / z weights are registers
     const size_t KK = (size_t)K1 * K2;
-    for (int l = lane; l < P * P; l += LPP) {
-      const int j1 = l % P, j2 = l / P;
-      const int n1 = wrap_node(k1 + j1, K1), n2 = wrap_node(k2 + j2, K2);
-      const size_t b = (size_t)n1 + (size_t)K1 * n2;
-      float s0 = 0.f, sz = 0.f;
+    const int jx = lane < P ? lane : 0;
+    const int n1 = wrap_node(k1 + jx, K1);
+    const float ax = lane < P ? sel<P>(wx, jx) : 0.0f;
+    const float bx = lane < P ? sel<P>(dx, jx) : 0.0f;
+    float c[P * P];
 #pragma unroll
-      for (int j3 = 0; j3 < P; ++j3) {
-        const float c = pot[b + KK * wrap_node(k3 + j3, K3)];
-        s0 = fmaf(wz[j3], c, s0);
-        sz = fmaf(dz[j3], c, sz);
-      }
-      const float ax = sel<P>(wx, j1), ay = sel<P>(wy, j2);
-      const float w12 = ax * ay;
-      phi = fmaf(w12, s0, phi);
-      gz = fmaf(w12, sz, gz);
-      gx = fmaf(sel<P>(dx, j1) * ay, s0, gx);
-      gy = fmaf(ax * sel<P>(dy, j2), s0, gy);
+    for (int j3 = 0; j3 < P; ++j3) {
+      const size_t b3 = KK * wrap_node(k3 + j3, K3) + n1;
+#pragma unroll
+      for (int j2 = 0; j2 < P; ++j2) c[j3 * P + j2] = pot[b3 + (size_t)K1 * wrap_node(k2 + j2, K2)];
     }
+#pragma unroll
+    for (int j3 = 0; j3 < P; ++j3) {
+#pragma unroll
+      for (int j2 = 0; j2 < P; ++j2) {
+        const float v = c[j3 * P + j2];
+        const float w23 = wy[j2] * wz[j3];
+        phi = fmaf(w23, v, phi);
+        gy = fmaf(dy[j2] * wz[j3], v, gy);
+        gz = fmaf(wy[j2] * dz[j3], v, gz);
+      }
+    }
+    // the lane's x weight (or its derivative, for gx) applies to its whole column of lines
+    gx = bx * phi;
+    phi *= ax;
+    gy *= ax;
+    gz *= ax;
   }
-  gx = wave_sum_width(gx, LPP);
-  gy = wave_sum_width(gy, LPP);
-  gz = wave_sum_width(gz, LPP);
-  if (ENERGY) phi = wave_sum_width(phi, LPP);
+  // sums over the particle's 8 lanes: three DPP adds each (no LDS round trips)
+  gx = sum8_lanes(gx);
+  gy = sum8_lanes(gy);
+  gz = sum8_lanes(gz);
+  if (ENERGY) phi = sum8_lanes(phi);
   if (!active || lane != 0) return;
   const float ffx = -(qi * gx * ((float)K1 / Lx));
   const float ffy = -(qi * gy * ((float)K2 / Ly));
