@@ -133,10 +133,12 @@ int pdx_nl_update(pdx_nl_t nl, const float *x, const float *y, const float *z, c
 int pdx_nl_check(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
                  pdx_stream_t stream);
 int pdx_nl_build(pdx_nl_t nl, pdx_stream_t stream);
-/* Copy the cells (offsets, ids) of a build completed since the last snapshot into the list's
- * snapshot (no counterpart in the reference: it lets a spread read cells that no concurrent
- * rebuild writes; see pdx_mesh_spread_snapshot). Enqueue after each step's last use of the
- * list; predicated on the device build counter, graph-capturable. */
+/* The list keeps a snapshot of the cells (offsets, ids) of its last completed build: the
+ * coverage check (pdx_nl_check / pdx_nl_update) copies the cells of a build completed since
+ * the previous check, before this step's rebuild can overwrite them (no counterpart in the
+ * reference: it lets a spread read cells that no concurrent rebuild writes; see
+ * pdx_mesh_spread_snapshot). pdx_nl_snapshot takes the copy explicitly (predicated on the
+ * device build counter, graph-capturable); the check makes it unnecessary. */
 int pdx_nl_snapshot(pdx_nl_t nl, pdx_stream_t stream);
 /* Unconditional rebuild (build_neighbors_dense!, NeighborAllocation.jl:198-214). */
 int pdx_nl_rebuild(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
@@ -239,11 +241,12 @@ int pdx_mesh_spread(pdx_mesh_t m, const float *x, const float *y, const float *z
  * packed (x, y, z, q) of the last pdx_nl_update on the same step; the list must cover the
  * mesh's particles and box. Same grid as pdx_mesh_spread (bitwise). */
 int pdx_mesh_spread_cells(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t stream);
-/* The owner-computes spread from nl's cell snapshot (pdx_nl_snapshot) instead of its current
- * cells, with a full skin of margin: it may run on another stream beside pdx_nl_build of the
- * same step (the snapshot's build is at most one step behind a rebuild trigger). The
- * positions are those packed by this step's pdx_nl_check. Returns PDX_ERR_STATE when no
- * snapshot was taken since the list's creation or box change. Same grid as pdx_mesh_spread. */
+/* The owner-computes spread from nl's cell snapshot instead of its current cells, with a full
+ * skin of margin: it may run on another stream beside pdx_nl_build of the same step (the
+ * snapshot's build is at most one step behind a rebuild trigger). The positions are those
+ * packed by this step's pdx_nl_check. Returns PDX_ERR_STATE before the second check after the
+ * list's creation or a box change (the first builds, the second snapshots the build). Same
+ * grid as pdx_mesh_spread. */
 int pdx_mesh_spread_snapshot(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t stream);
 /* Forward R2C 3D FFT, multiply by G, FP64 mesh energy and virial (with PDX_ENERGY), inverse
  * C2R FFT into the potential grid (mesh_transform!, Solver.jl:305-341; _solve_kernel!,

@@ -244,7 +244,7 @@ struct pdx_nl_s {
   // completed build; snap_state[0] = build count copied, [2] = nonzero while none is valid
   pdx::DevBuf<int32_t> snap_offsets, snap_ids;
   pdx::DevBuf<int64_t> snap_state;
-  int snap_host = 0;  // 0: none since create / box change, 1: an update followed, 2: snapshot enqueued
+  int snap_host = 0;  // checks since create / box change (capped at 2): at 2 a snapshot was taken
 };
 
 struct pdx_pair_s {

@@ -1829,7 +1829,7 @@ extern "C" int pdx_mesh_spread_cells(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t str
 
 extern "C" int pdx_mesh_spread_snapshot(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t stream) {
   if (nl && nl->snap_host != 2)
-    return fail(PDX_ERR_STATE, "no cell snapshot: call pdx_nl_snapshot after an update (and after a box change)");
+    return fail(PDX_ERR_STATE, "no cell snapshot yet: the second list check after creation or a box change takes it");
   return spread_cells_impl(m, nl, true, stream);
 }
 

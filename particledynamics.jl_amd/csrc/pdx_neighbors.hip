@@ -43,10 +43,19 @@ __global__ __launch_bounds__(TPB) void k_decide(int N, const float *__restrict__
                                                  const float *__restrict__ rref, float4 *packed,
                                                  int32_t *cell_counts, int ncell, int allpairs,
                                                  const float *geom, const int64_t *state,
-                                                 float *partial) {
+                                                 float *partial, const int32_t *__restrict__ offsets,
+                                                 const int32_t *__restrict__ ids, int32_t *snap_offsets,
+                                                 int32_t *snap_ids, const int64_t *snap) {
   const int64_t err = state[PDX_DN_ERROR];
   if (err != 0) return;  // latched: every block leaves (uniform)
   const int i = blockIdx.x * TPB + threadIdx.x;
+  // cell snapshot: the cells of a build completed since the last snapshot, copied before this
+  // step's rebuild can overwrite them (k_decide_final records the build); a spread on another
+  // stream may then read them beside the rebuild
+  if (!allpairs && state[PDX_DN_BUILDS] > 0 && state[PDX_DN_BUILDS] != snap[0]) {
+    for (int c = i; c <= ncell; c += gridDim.x * TPB) snap_offsets[c] = offsets[c];
+    if (i < N) snap_ids[i] = ids[i];
+  }
   const float L[3] = {geom[0], geom[1], geom[2]};
   const bool valid = state[PDX_DN_VALID] != 0;
   float value = 0.0f;
@@ -81,7 +90,7 @@ __global__ __launch_bounds__(TPB) void k_decide(int N, const float *__restrict__
 
 // The decision from the blocks' maxima (one block).
 __global__ __launch_bounds__(TPB) void k_decide_final(int nblocks, const float *__restrict__ partial, int64_t *state,
-                                                       float threshold, int force) {
+                                                       float threshold, int force, int64_t *snap, int allpairs) {
   if (state[PDX_DN_ERROR] != 0) return;
   __shared__ float smax[TPB / WAVE];
   float v = 0.0f;
@@ -90,6 +99,10 @@ __global__ __launch_bounds__(TPB) void k_decide_final(int nblocks, const float *
   if ((threadIdx.x & (WAVE - 1)) == 0) smax[threadIdx.x / WAVE] = v;
   __syncthreads();
   if (threadIdx.x != 0) return;
+  if (!allpairs && state[PDX_DN_BUILDS] > 0 && state[PDX_DN_BUILDS] != snap[0]) {  // k_decide copied it
+    snap[0] = state[PDX_DN_BUILDS];
+    snap[2] = 0;
+  }
   float largest = smax[0];
   for (int w = 1; w < TPB / WAVE; ++w) largest = fmaxf(largest, smax[w]);
   const bool valid = state[PDX_DN_VALID] != 0;
@@ -597,11 +610,14 @@ int enqueue_check(pdx_nl_t nl, const float *x, const float *y, const float *z, c
   const float threshold = (0.25f * s) * s;
   const int nb = blocks_for(N, TPB);
   hipLaunchKernelGGL(k_decide, dim3(nb), dim3(TPB), 0, st, N, x, y, z, q, nl->rref.p, nl->packed.p,
-                     nl->cell_counts.p, nl->ncell, (int)nl->allpairs, nl->geom.p, nl->state.p, nl->disp_partial.p);
+                     nl->cell_counts.p, nl->ncell, (int)nl->allpairs, nl->geom.p, nl->state.p, nl->disp_partial.p,
+                     (const int32_t *)nl->cell_offsets.p, (const int32_t *)nl->particle_ids_sorted.p,
+                     nl->snap_offsets.p, nl->snap_ids.p, (const int64_t *)nl->snap_state.p);
   hipLaunchKernelGGL(k_decide_final, dim3(1), dim3(TPB), 0, st, nb, nl->disp_partial.p, nl->state.p, threshold,
-                     force);
+                     force, nl->snap_state.p, (int)nl->allpairs);
   PDX_LAUNCH_CHECK();
-  if (nl->snap_host == 0) nl->snap_host = 1;
+  // the first check after creation / a box change builds; the second one snapshots that build
+  if (nl->snap_host < 2) nl->snap_host += 1;
   return PDX_OK;
 }
 
@@ -769,7 +785,7 @@ extern "C" int pdx_nl_snapshot(pdx_nl_t nl, pdx_stream_t stream) {
                      (const int64_t *)nl->snap_state.p);
   hipLaunchKernelGGL(k_snap_mark, dim3(1), dim3(1), 0, st, (const int64_t *)nl->state.p, nl->snap_state.p);
   PDX_LAUNCH_CHECK();
-  if (nl->snap_host == 1) nl->snap_host = 2;
+  if (nl->snap_host >= 1) nl->snap_host = 2;
   return PDX_OK;
 }
 

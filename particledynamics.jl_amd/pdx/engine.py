@@ -99,9 +99,10 @@ class ForceEngine:
         else:
             self.nl.update(x, y, z, q, stream=main)
             if early:
-                self.nl.snapshot(stream=main)
+                # the next check snapshots this first build; this evaluation spreads from the
+                # current cells after the whole update
                 self._snap_ready = True
-                early = False  # this first evaluation's list is complete already
+                early = False
             order = self.nl._arrays.particle_ids_sorted
         if self.solver is None:
             self._pairs(fx, fy, fz, Epot, V, energy, main)
@@ -110,7 +111,7 @@ class ForceEngine:
         spread_stream = main if self.overlap_with == "solve" else side
         if overlap and spread_stream is side:
             side.wait_stream(main)
-        if self.spread_mode == "snapshot" and self._snap_ready and overlap and self.cluster is None:
+        if early:
             self.solver.spread_snapshot(self.nl, stream=spread_stream)
         elif self.spread_mode in ("cells", "snapshot") and self.cluster is None:
             self.solver.spread_cells(self.nl, stream=spread_stream)
@@ -129,8 +130,6 @@ class ForceEngine:
             main.wait_stream(side)
         self.solver.mesh_gather(x, y, z, q, fx, fy, fz, Epot, self.Vglobal if energy else None, energy,
                                 order=order, set=False, stream=main)
-        if self._snap_ready and self.cluster is None:
-            self.nl.snapshot(stream=main)  # cells of this step's build for the next step's spread
         if self.correction is not None:
             self.correction.forward(self.feats, None, fx, fy, fz, stream=main)
 

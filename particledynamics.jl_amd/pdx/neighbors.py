@@ -87,7 +87,8 @@ class DeviceNeighborWorkspace:
         L.check(L.lib.pdx_nl_build(self._h, L.stream_handle(stream)))
 
     def snapshot(self, stream=None):
-        """Copy the cells of a build completed since the last snapshot (for spread_snapshot)."""
+        """Copy the cells of a build completed since the last snapshot (for spread_snapshot); the
+        check does the same, so this is only needed between checks."""
         L.check(L.lib.pdx_nl_snapshot(self._h, L.stream_handle(stream)))
 
     def advance(self, stream=None):

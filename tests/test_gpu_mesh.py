@@ -146,19 +146,16 @@ def test_spread_snapshot_equals_atomic_spread(cuda):
     nl = pdx.DeviceNeighborWorkspace(N, L_ES, 4.0, 0.3, 512)
     nl.update(X, Y, Z, Q)
     with pytest.raises(pdx.ArgumentError):
-        s.spread_snapshot(nl)
-    nl.snapshot()
+        s.spread_snapshot(nl)  # the first check built the cells; the snapshot comes with the next
     builds = []
     for step in range(4):
         if step:
             X = X + 0.1
             X = X - L_ES[0] * torch.floor(X / L_ES[0] + 0.5)
-            nl.check(X, Y, Z, Q)
+        nl.check(X, Y, Z, Q)  # snapshots the last completed build, then decides
         s.spread_snapshot(nl)
         ga = s.fixed_grid().clone()
-        if step:
-            nl.build()
-        nl.snapshot()
+        nl.build()
         s.spread(X, Y, Z, Q)
         gb = s.fixed_grid().clone()
         assert torch.equal(ga, gb), step
