@@ -128,8 +128,10 @@ int pdx_nl_arrays_get(pdx_nl_t nl, pdx_nl_arrays *out);
 int pdx_nl_update(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
                   pdx_stream_t stream);
 /* The two halves of pdx_nl_update, for callers that overlap work with the rebuild: the
- * coverage check with the packing of this step's coordinates (decide_device_neighbors!,
- * DeviceNeighborCUDA.jl:421-423), then the predicated rebuild (_device_rebuild!, :424-427). */
+ * displacement pass with the packing of this step's coordinates (and the cell snapshot), then
+ * the decision and the predicated rebuild (decide_device_neighbors! + _device_rebuild!,
+ * DeviceNeighborCUDA.jl:421-427). Work that needs only the packed coordinates or the snapshot
+ * may run on another stream between the two. */
 int pdx_nl_check(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
                  pdx_stream_t stream);
 int pdx_nl_build(pdx_nl_t nl, pdx_stream_t stream);

@@ -245,6 +245,7 @@ struct pdx_nl_s {
   pdx::DevBuf<int32_t> snap_offsets, snap_ids;
   pdx::DevBuf<int64_t> snap_state;
   int snap_host = 0;  // checks since create / box change (capped at 2): at 2 a snapshot was taken
+  int check_force = 0;  // the last check was a forced rebuild (pdx_nl_rebuild)
 };
 
 struct pdx_pair_s {

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(TPB) void k_decide(int N, const float *__restrict__
                                                  const float *geom, const int64_t *state,
                                                  float *partial, const int32_t *__restrict__ offsets,
                                                  const int32_t *__restrict__ ids, int32_t *snap_offsets,
-                                                 int32_t *snap_ids, const int64_t *snap) {
+                                                 int32_t *snap_ids, int64_t *snap) {
   const int64_t err = state[PDX_DN_ERROR];
   if (err != 0) return;  // latched: every block leaves (uniform)
   const int i = blockIdx.x * TPB + threadIdx.x;
@@ -55,6 +55,7 @@ __global__ __launch_bounds__(TPB) void k_decide(int N, const float *__restrict__
   if (!allpairs && state[PDX_DN_BUILDS] > 0 && state[PDX_DN_BUILDS] != snap[0]) {
     for (int c = i; c <= ncell; c += gridDim.x * TPB) snap_offsets[c] = offsets[c];
     if (i < N) snap_ids[i] = ids[i];
+    if (i == 0) snap[2] = 0;  // valid once this launch completes (snap[0] is written by k_decide_final)
   }
   const float L[3] = {geom[0], geom[1], geom[2]};
   const bool valid = state[PDX_DN_VALID] != 0;
@@ -99,10 +100,8 @@ __global__ __launch_bounds__(TPB) void k_decide_final(int nblocks, const float *
   if ((threadIdx.x & (WAVE - 1)) == 0) smax[threadIdx.x / WAVE] = v;
   __syncthreads();
   if (threadIdx.x != 0) return;
-  if (!allpairs && state[PDX_DN_BUILDS] > 0 && state[PDX_DN_BUILDS] != snap[0]) {  // k_decide copied it
+  if (!allpairs && state[PDX_DN_BUILDS] > 0 && state[PDX_DN_BUILDS] != snap[0])  // k_decide copied it
     snap[0] = state[PDX_DN_BUILDS];
-    snap[2] = 0;
-  }
   float largest = smax[0];
   for (int w = 1; w < TPB / WAVE; ++w) largest = fmaxf(largest, smax[w]);
   const bool valid = state[PDX_DN_VALID] != 0;
@@ -606,26 +605,35 @@ int enqueue_check(pdx_nl_t nl, const float *x, const float *y, const float *z, c
   if (!nl) return fail(PDX_ERR_ARGUMENT, "null neighbor list");
   if (!x || !y || !z) return fail(PDX_ERR_ARGUMENT, "null coordinate array");
   const int N = nl->N;
-  const float s = (float)nl->skin;
-  const float threshold = (0.25f * s) * s;
   const int nb = blocks_for(N, TPB);
   hipLaunchKernelGGL(k_decide, dim3(nb), dim3(TPB), 0, st, N, x, y, z, q, nl->rref.p, nl->packed.p,
                      nl->cell_counts.p, nl->ncell, (int)nl->allpairs, nl->geom.p, nl->state.p, nl->disp_partial.p,
                      (const int32_t *)nl->cell_offsets.p, (const int32_t *)nl->particle_ids_sorted.p,
-                     nl->snap_offsets.p, nl->snap_ids.p, (const int64_t *)nl->snap_state.p);
-  hipLaunchKernelGGL(k_decide_final, dim3(1), dim3(TPB), 0, st, nb, nl->disp_partial.p, nl->state.p, threshold,
-                     force, nl->snap_state.p, (int)nl->allpairs);
+                     nl->snap_offsets.p, nl->snap_ids.p, nl->snap_state.p);
   PDX_LAUNCH_CHECK();
   // the first check after creation / a box change builds; the second one snapshots that build
   if (nl->snap_host < 2) nl->snap_host += 1;
+  nl->check_force = force;
   return PDX_OK;
+}
+
+// The decision from the check's partial maxima (and the snapshot's record), then the
+// predicated rebuild.
+int enqueue_decide_build(pdx_nl_t nl, hipStream_t st) {
+  const float s = (float)nl->skin;
+  const float threshold = (0.25f * s) * s;
+  const int nb = blocks_for(nl->N, TPB);
+  hipLaunchKernelGGL(k_decide_final, dim3(1), dim3(TPB), 0, st, nb, nl->disp_partial.p, nl->state.p, threshold,
+                     nl->check_force, nl->snap_state.p, (int)nl->allpairs);
+  PDX_LAUNCH_CHECK();
+  return enqueue_rebuild(nl, st);
 }
 
 int enqueue_update(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
                    hipStream_t st, int force) {
   int rc = enqueue_check(nl, x, y, z, q, st, force);
   if (rc) return rc;
-  return enqueue_rebuild(nl, st);
+  return enqueue_decide_build(nl, st);
 }
 
 }  // namespace
@@ -771,7 +779,7 @@ extern "C" int pdx_nl_check(pdx_nl_t nl, const float *x, const float *y, const f
 
 extern "C" int pdx_nl_build(pdx_nl_t nl, pdx_stream_t stream) {
   if (!nl) return fail(PDX_ERR_ARGUMENT, "null neighbor list");
-  return enqueue_rebuild(nl, as_stream(stream));
+  return enqueue_decide_build(nl, as_stream(stream));
 }
 
 extern "C" int pdx_nl_snapshot(pdx_nl_t nl, pdx_stream_t stream) {
