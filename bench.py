@@ -34,8 +34,8 @@ def parse():
     p.add_argument("--lanes", type=int, default=0)
     p.add_argument("--no-clusters", action="store_true", help="skip the cluster-pair backend leg")
     p.add_argument("--skin", type=float, default=1.0)
-    p.add_argument("--overlap-with", choices=("spread", "solve"), default="spread",
-                   help="what the pair kernel overlaps: the whole mesh branch, or the solve only")
+    p.add_argument("--overlap-with", choices=("spread", "solve", "fft"), default="fft",
+                   help="what the pair kernel overlaps: the whole mesh branch, the solve only, or the transforms only")
     p.add_argument("--spread", choices=("snapshot", "cells", "atomic"), default="snapshot")
     p.add_argument("--traversal", choices=("rows", "cells"), default="rows",
                    help="pair traversal: the list's ELL rows, or its cell grid without rows")

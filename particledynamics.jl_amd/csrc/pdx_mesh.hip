@@ -1791,9 +1791,9 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
   const int32_t *ids = snapshot ? nl->snap_ids.p : nl->particle_ids_sorted.p;
   const int64_t *guard = snapshot ? nl->snap_state.p : nl->state.p;  // word 2 nonzero: no usable cells
   hipStream_t st = as_stream(stream);
-  // workgroup size: 256 threads by default; 1024 (32 particles at a time per tile) is faster for
-  // the spread alone but slower while it shares the CUs with the pair kernel
-  int sbs = 256;
+  // workgroup size (pdx_mesh_set_spread_threads): 1024 (32 particles at a time per tile) is the
+  // faster spread on its own; 256 leaves more room beside a concurrent pair kernel
+  int sbs = m->spread_threads;
   if (const char *e = std::getenv("PDX_SPREAD_BS")) sbs = std::atoi(e) == 256 ? 256 : 1024;
 #define PDX_TSPREAD_L(PP, TT, BB)                                                                              \
   hipLaunchKernelGGL((k_spread_tiles<PP, TT, BB>), dim3(tiles), dim3(BB), 0, st, nl->packed.p,                 \
@@ -1822,6 +1822,13 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
   return PDX_OK;
 }
 }  // namespace
+
+extern "C" int pdx_mesh_set_spread_threads(pdx_mesh_t m, int32_t threads) {
+  if (!m) return fail(PDX_ERR_ARGUMENT, "null mesh");
+  if (threads != 256 && threads != 1024) return fail(PDX_ERR_ARGUMENT, "spread threads must be 256 or 1024");
+  m->spread_threads = threads;
+  return PDX_OK;
+}
 
 extern "C" int pdx_mesh_spread_cells(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t stream) {
   return spread_cells_impl(m, nl, false, stream);

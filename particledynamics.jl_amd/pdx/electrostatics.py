@@ -184,6 +184,10 @@ class ElectrostaticsSolver:
         """Owner-computes spread from ``nl``'s cells and packed (x, y, z, q) (no atomics)."""
         L.check(L.lib.pdx_mesh_spread_cells(self._h, nl.handle, L.stream_handle(stream)))
 
+    def set_spread_threads(self, threads):
+        """Workgroup size of the owner-computes spreads (1024 or 256)."""
+        L.check(L.lib.pdx_mesh_set_spread_threads(self._h, int(threads)))
+
     def spread_snapshot(self, nl, stream=None):
         """Owner-computes spread from ``nl``'s cell snapshot (may run beside ``nl.build``)."""
         L.check(L.lib.pdx_mesh_spread_snapshot(self._h, nl.handle, L.stream_handle(stream)))

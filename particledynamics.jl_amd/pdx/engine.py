@@ -23,15 +23,17 @@ class ForceEngine:
     def __init__(self, N, box, pair_model=None, electrostatics=None, charges=None, skin=0.4, cap=None,
                  interval=20, lanes=0, diagnostics=False, spread="snapshot", correction=None, accumulation="fp32",
                  bonds=None, exceptions=(), traversal="rows", backend="ell", prune_buffer=None, cluster_capacity=0,
-                 overlap_with="spread"):
+                 overlap_with="fft"):
         """backend "ell": the cell list and ELL rows (``pdx_nl_*`` + ``pdx_pair_forces``); "cluster":
         the cluster-pair list and tiles (``pdx_cluster_*``, the reference's ``ClusterBackend``,
         src/engine/ClusterBackend.jl:1-40); its mesh spread takes the cluster order."""
         import torch
-        if overlap_with not in ("spread", "solve"):
-            raise L.ArgumentError("overlap_with must be 'spread' or 'solve'")
+        if overlap_with not in ("spread", "solve", "fft"):
+            raise L.ArgumentError("overlap_with must be 'spread', 'solve' or 'fft'")
         # "spread": the pair forces run beside the whole mesh branch (spread and solve);
-        # "solve": the spread runs first on the main stream, the pairs beside the solve only
+        # "solve": the spread runs first on the main stream, the pairs beside the solve only;
+        # "fft": the spread runs on the mesh stream beside the rebuild, the pairs wait for it and
+        # run beside the transforms
         self.overlap_with = overlap_with
         if backend not in ("ell", "cluster"):
             raise L.ArgumentError("backend must be 'ell' or 'cluster'")
@@ -65,6 +67,9 @@ class ForceEngine:
         # the mesh branch runs on its own stream (default priority), forked after the list
         # check and joined before the gather adds the mesh forces to the pair forces
         self.mesh_stream = torch.cuda.Stream() if self.solver is not None else None
+        if self.solver is not None:
+            # a spread that shares the CUs with the pair kernel runs better in smaller workgroups
+            self.solver.set_spread_threads(256 if overlap_with == "spread" else 1024)
         # "snapshot": the owner-computes spread over the cell snapshot of the last completed
         # build, forked right after the list check so that it runs beside the rebuild; "cells":
         # the same spread over the current cells, after the whole list update; "atomic": the
@@ -89,7 +94,7 @@ class ForceEngine:
         import torch
         main = stream if stream is not None else torch.cuda.current_stream()
         early = (self.cluster is None and self.solver is not None and overlap and self.spread_mode == "snapshot"
-                 and self.overlap_with == "spread")
+                 and self.overlap_with in ("spread", "fft"))
         if self.cluster is not None:
             self.cluster.update(x, y, z, q, stream=main)
             order = self.cluster.order_ptr
@@ -124,6 +129,10 @@ class ForceEngine:
         if self.correction is not None:
             from .learned import mesh_features
             mesh_features(self.solver, x, y, z, q, self.feats, stream=side)
+        if overlap and self.overlap_with == "fft":
+            spread_done = torch.cuda.Event()
+            spread_done.record(side)
+            main.wait_event(spread_done)
         self.solver.solve(energy, stream=side)
         self._pairs(fx, fy, fz, Epot, V, energy, main)
         if overlap:
