@@ -593,35 +593,37 @@ __global__ __launch_bounds__(PBS) void k_plane_fwd(const unsigned long long *__r
 }
 
 // Forward z transform, influence multiply with the FP64 energy / virial partials, inverse z
-// transform of the (m1, m3) plane m2 (all K1h lines in one workgroup).
-template <int TP, bool ENERGY, bool BIG>
+// transform of TZ lines m1 = m10 .. m10 + TZ - 1 of the (m1, m3) plane m2: `tiles` workgroups per
+// plane, so that the launch spans every CU.
+template <int TZ, bool ENERGY, bool BIG>
 __global__ __launch_bounds__(PBS) void k_plane_z(float2 *spec, FFTPlan p3, const float2 *__restrict__ tw,
-                                                 SolveArgs s) {
+                                                 SolveArgs s, int tiles) {
   extern __shared__ float2 lds[];
   const int K1h = s.K1h, K2 = s.K2, K3 = s.K3;
-  constexpr int TS = TP + 1;
+  constexpr int TS = TZ + 1;
   float2 *twl = lds, *b0 = lds + K3, *b1 = b0 + (size_t)TS * K3;
   for (int k = threadIdx.x; k < K3; k += PBS) twl[k] = tw[k];
-  const int m2 = blockIdx.x;
+  const int m2 = blockIdx.x / tiles, m10 = (blockIdx.x % tiles) * TZ;
+  const int nl = min(TZ, K1h - m10);
   const size_t plane = (size_t)K1h * K2;
-  const size_t base = (size_t)m2 * K1h;
-  const int n = K3 * K1h;
+  const size_t base = (size_t)m2 * K1h + m10;
+  const int n = K3 * nl;
   staged<8, float2, PBS>(n, [&](int e) {
-    const int m3 = e / K1h, m1 = e - m3 * K1h;
-    return spec[base + (size_t)m3 * plane + m1];
+    const int m3 = e / nl, l = e - m3 * nl;
+    return spec[base + (size_t)m3 * plane + l];
   }, [&](int e, float2 v) {
-    const int m3 = e / K1h, m1 = e - m3 * K1h;
-    b0[m3 * TS + m1] = v;
+    const int m3 = e / nl, l = e - m3 * nl;
+    b0[m3 * TS + l] = v;
   });
   __syncthreads();
-  float2 *r = lds_fft<TP, BIG, TS>(b0, b1, p3, twl, -1.0f);
+  float2 *r = lds_fft<TZ, BIG, TS>(b0, b1, p3, twl, -1.0f);
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  solve_lines<TP, ENERGY, TS>(r, 0, m2, s, acc);
+  solve_lines<TZ, ENERGY, TS>(r, m10, m2, s, acc);
   __syncthreads();
-  r = lds_fft<TP, BIG, TS>(r, r == b0 ? b1 : b0, p3, twl, +1.0f);
+  r = lds_fft<TZ, BIG, TS>(r, r == b0 ? b1 : b0, p3, twl, +1.0f);
   for (int e = threadIdx.x; e < n; e += PBS) {
-    const int m3 = e / K1h, m1 = e - m3 * K1h;
-    spec[base + (size_t)m3 * plane + m1] = r[m3 * TS + m1];
+    const int m3 = e / nl, l = e - m3 * nl;
+    spec[base + (size_t)m3 * plane + l] = r[m3 * TS + l];
   }
   if (ENERGY) block_partials<7>(acc, s.partial);
 }
@@ -1488,7 +1490,10 @@ int plane_lines(pdx_mesh_t m) {
 size_t plane_lds_xy(pdx_mesh_t m) {
   return ((size_t)m->K1 + m->K2 + (size_t)2 * std::max(m->K1, m->K2) * (m->plane_tp + 1)) * sizeof(float2);
 }
-size_t plane_lds_z(pdx_mesh_t m) { return ((size_t)m->K3 + (size_t)2 * m->K3 * (m->plane_tp + 1)) * sizeof(float2); }
+// z solve: lines of a plane in tiles of 32 (two or more workgroups per plane when K1/2 + 1 > 32)
+constexpr int PLANE_TZ = 32;
+int plane_z_tiles(pdx_mesh_t m) { return (m->K1h + PLANE_TZ - 1) / PLANE_TZ; }
+size_t plane_lds_z(pdx_mesh_t m) { return ((size_t)m->K3 + (size_t)2 * m->K3 * (PLANE_TZ + 1)) * sizeof(float2); }
 
 #define PDX_PLANE_SWITCH(CALL)                                    \
   (void)big;                                                       \
@@ -1509,13 +1514,10 @@ int plane_attributes(pdx_mesh_t m, const Plans &pl) {
   PDX_PLANE_SWITCH(PDX_PA_XY)
 #undef PDX_PA_XY
   big = pl.p3.big;
-#define PDX_PA_Z(TP, BB)                                                                                     \
-  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_z<TP, true, BB>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                              (int)lz));                                                                      \
-  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_z<TP, false, BB>, hipFuncAttributeMaxDynamicSharedMemorySize,\
-                              (int)lz));
-  PDX_PLANE_SWITCH(PDX_PA_Z)
-#undef PDX_PA_Z
+  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_z<PLANE_TZ, true, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lz));
+  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_z<PLANE_TZ, false, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lz));
   return PDX_OK;
 }
 
@@ -1531,12 +1533,13 @@ int enqueue_plane_solve(pdx_mesh_t m, const Plans &pl, bool energy, hipStream_t 
   PDX_PLANE_SWITCH(PDX_PF)
 #undef PDX_PF
   SolveArgs sa = solve_args(m);
-  big = pl.p3.big;
-#define PDX_PZ(TP, BB)                                                                                        \
-  if (energy) hipLaunchKernelGGL((k_plane_z<TP, true, BB>), dim3(m->K2), dim3(PBS), lz, st, m->spec_c.p, pl.p3, tw3, sa); \
-  else hipLaunchKernelGGL((k_plane_z<TP, false, BB>), dim3(m->K2), dim3(PBS), lz, st, m->spec_c.p, pl.p3, tw3, sa);
-  PDX_PLANE_SWITCH(PDX_PZ)
-#undef PDX_PZ
+  const int zt = plane_z_tiles(m);
+  if (energy)
+    hipLaunchKernelGGL((k_plane_z<PLANE_TZ, true, false>), dim3(m->K2 * zt), dim3(PBS), lz, st, m->spec_c.p, pl.p3,
+                       tw3, sa, zt);
+  else
+    hipLaunchKernelGGL((k_plane_z<PLANE_TZ, false, false>), dim3(m->K2 * zt), dim3(PBS), lz, st, m->spec_c.p, pl.p3,
+                       tw3, sa, zt);
   big = pl.p1.big || pl.p2.big;
   const double *ps = energy ? m->partial.p : nullptr;
   double *po = energy ? m->acc.p : nullptr;
@@ -1651,7 +1654,7 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
   const size_t nmodes = (size_t)m->K1h * m->K2 * m->K3;
   const Plans pl = plans_of(m);
   m->plane_tp = plane_lines(m);
-  m->nmode_blocks = m->plane_tp ? m->K2 : ((m->K1h + pl.T3 - 1) / pl.T3) * m->K2;
+  m->nmode_blocks = m->plane_tp ? m->K2 * plane_z_tiles(m) : ((m->K1h + pl.T3 - 1) / pl.T3) * m->K2;
   int rc;
   if ((rc = m->grid64.alloc(nnodes)) || (rc = m->spec_c.alloc(nmodes)) || (rc = m->pot.alloc(nnodes)) ||
       (rc = m->G.alloc(nmodes)) || (rc = m->tw.alloc(K[0] + K[1] + K[2])) || (rc = m->acc.alloc(8)) ||
