@@ -1,5 +1,5 @@
 // Smooth PME mesh engine on gfx950: B-spline charge spreading into an int64 fixed-point
-// grid, a mixed-radix (2,3,4,5,7,8,11,13) Stockham 3D FFT in LDS, the influence-function
+// grid, a mixed-radix (2,3,4,5,7,8,11,13,15) Stockham 3D FFT in LDS, the influence-function
 // convolution with FP64 energy / virial reductions fused into the z transform, and the
 // analytic-differentiation force gather.
 //
@@ -41,7 +41,7 @@ bool make_plan(int K, FFTPlan &pl) {
   pl.K = K;
   pl.nstage = 0;
   int n = K;
-  const int rs[8] = {8, 4, 2, 3, 5, 7, 11, 13};
+  const int rs[9] = {8, 15, 4, 2, 3, 5, 7, 11, 13};
   for (int r : rs) {
     while (n % r == 0) {
       if (pl.nstage >= MAXSTAGE) return false;
@@ -52,7 +52,7 @@ bool make_plan(int K, FFTPlan &pl) {
   int p = 1;
   pl.big = 0;
   for (int s = 0; s < pl.nstage; ++s) {
-    if (pl.radix[s] == 7 || pl.radix[s] >= 11) pl.big = 1;
+    if (pl.radix[s] == 7 || pl.radix[s] == 11 || pl.radix[s] == 13) pl.big = 1;
     pl.p[s] = p;
     pl.magic[s] = p > 1 ? (unsigned)((((unsigned long long)1 << 32) + p - 1) / p) : 0u;
     pl.tstride[s] = K / (p * pl.radix[s]);
@@ -71,7 +71,7 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 __device__ __forceinline__ float2 cmuli(float2 a, float sign) { return make_float2(-sign * a.y, sign * a.x); }
 
 // In-place radix-R DFT of u[0..R) with exponent sign `sg` (-1 forward, +1 inverse).
-constexpr int MAXR = 13;
+constexpr int MAXR = 16;
 template <int R> __device__ __forceinline__ void dft(float2 (&u)[MAXR], float sg);
 
 template <> __device__ __forceinline__ void dft<2>(float2 (&u)[MAXR], float) {
@@ -133,6 +133,43 @@ template <> __device__ __forceinline__ void dft<5>(float2 (&u)[MAXR], float sg) 
   u[2] = cadd(r2, i2);
   u[3] = csub(r2, i2);
 }
+// Radix 15 = 3 x 5 (Cooley-Tukey): n = 5 n1 + n2, k = k1 + 3 k2; radix 3 over n1, the twiddles
+// W15^(n2 k1), radix 5 over n2 (W15 = exp(sg 2 pi i / 15), cos / sin of 2 pi m / 15 below).
+template <> __device__ __forceinline__ void dft<15>(float2 (&u)[MAXR], float sg) {
+  const float c[9] = {1.0f, 0.91354545764260086660f, 0.66913060635885823757f, 0.30901699437494745126f,
+                      -0.10452846326765333207f, -0.49999999999999977796f, -0.80901699437494734024f,
+                      -0.97814760073380568883f, -0.97814760073380568883f};
+  const float s[9] = {0.0f, 0.40673664307580015276f, 0.74314482547739413310f, 0.95105651629515353118f,
+                      0.99452189536827340088f, 0.86602540378443870761f, 0.58778525229247324813f,
+                      0.20791169081775931482f, -0.20791169081775906502f};
+  float2 y[3][5];
+#pragma unroll
+  for (int n2 = 0; n2 < 5; ++n2) {
+    float2 a[MAXR];
+    a[0] = u[n2];
+    a[1] = u[5 + n2];
+    a[2] = u[10 + n2];
+    dft<3>(a, sg);
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) {
+      float2 v = a[k1];
+      if (k1 > 0 && n2 > 0) {
+        const int m = n2 * k1;
+        v = cmul(v, make_float2(c[m], sg * s[m]));
+      }
+      y[k1][n2] = v;
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 3; ++k1) {
+    float2 b[MAXR];
+#pragma unroll
+    for (int n2 = 0; n2 < 5; ++n2) b[n2] = y[k1][n2];
+    dft<5>(b, sg);
+#pragma unroll
+    for (int k2 = 0; k2 < 5; ++k2) u[k1 + 3 * k2] = b[k2];
+  }
+}
 // Odd primes 7, 11, 13: direct O(R^2) sum with the roots cos/sin(2 pi k/R) from a table.
 template <int R>
 __device__ __forceinline__ void dft_direct(float2 (&u)[MAXR], float sg, const float *c, const float *s) {
@@ -179,7 +216,7 @@ __device__ __forceinline__ void stage(const float2 *__restrict__ in, float2 *__r
   const int bs = blockDim.x;
   // two butterflies per iteration (one for the large radices, which need the registers): the LDS
   // loads of both are issued before either is computed
-  constexpr int U = (R == 7 || R >= 11) ? 1 : 2;
+  constexpr int U = (R == 7 || R >= 11) ? 1 : 2;  // 7, 11, 13, 15: one butterfly per iteration
   for (int idx0 = threadIdx.x; idx0 < nb; idx0 += U * bs) {
     float2 u[U][MAXR];
     int jj[U], ll[U], pp = p;
@@ -226,6 +263,7 @@ __device__ float2 *lds_fft(float2 *buf0, float2 *buf1, const FFTPlan &pl, const 
     const int ts = pl.tstride[s];
     switch (pl.radix[s]) {
       case 8: stage<8, T, S>(a, b, pl.K, p, mg, ts, tw, sg); break;
+      case 15: stage<15, T, S>(a, b, pl.K, p, mg, ts, tw, sg); break;
       case 4: stage<4, T, S>(a, b, pl.K, p, mg, ts, tw, sg); break;
       case 2: stage<2, T, S>(a, b, pl.K, p, mg, ts, tw, sg); break;
       case 3: stage<3, T, S>(a, b, pl.K, p, mg, ts, tw, sg); break;
