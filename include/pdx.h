@@ -264,6 +264,15 @@ int pdx_mesh_solve(pdx_mesh_t m, int32_t flags, pdx_stream_t stream);
 int pdx_mesh_gather(pdx_mesh_t m, const float *x, const float *y, const float *z, const float *q,
                     const int32_t *order, float *fx, float *fy, float *fz, float *Epot,
                     float *Vglobal, int32_t flags, int32_t set, pdx_stream_t stream);
+/* pdx_mesh_gather at the positions of nl's packed array, for a mesh whose last spread was an
+ * owner-computes spread (pdx_mesh_spread_cells / _snapshot) from nl on the same positions
+ * (PDX_ERR_STATE otherwise): that spread listed each particle under the 16^3 (or 8^3) tile of
+ * its base node, and one workgroup per tile gathers its particles from the tile's potential
+ * staged in LDS (same forces as pdx_mesh_gather up to FP32 rounding). Smooth PME and P3M-ad
+ * meshes (the ik influence has its own gather). Same role as pdx_mesh_gather:
+ * _gather_ad_lines_kernel! (MeshKernels.jl:313-333). */
+int pdx_mesh_gather_cells(pdx_mesh_t m, pdx_nl_t nl, float *fx, float *fy, float *fz, float *Epot,
+                          float *Vglobal, int32_t flags, int32_t set, pdx_stream_t stream);
 /* Boundary readback: out[0] mesh energy, out[1..6] reciprocal virial (with the background
  * term on the diagonal), out[7] self energy, out[8] background energy (FP64). */
 int pdx_mesh_totals(pdx_mesh_t m, double out[9], pdx_stream_t stream);

@@ -302,4 +302,13 @@ struct pdx_mesh_s {
   int T1 = 0, T2 = 0, T3 = 0;  // FFT tile widths, fixed at creation
   int plane_tp = 0;            // plane passes: lines per plane padded to this power of two (0: line passes)
   int spread_threads = 1024;   // workgroup size of the owner-computes spread (256 or 1024)
+  // per-tile particle lists of the last owner-computes spread (read by the tile gather):
+  // tile_entries entries shared by the tiles, tile_n counts; valid for the positions of
+  // tiles_nl's packed array while tiles_nl is set
+  pdx::DevBuf<float4> tile_list;  // (u1, u2, u3, q)
+  pdx::DevBuf<int32_t> tile_ids;  // particle index of each entry
+  pdx::DevBuf<int32_t> tile_n;
+  size_t tile_entries = 0;
+  const void *tiles_nl = nullptr;
+  int tiles_stile = 0, tiles_count = 0, tiles_cap = 0;
 };

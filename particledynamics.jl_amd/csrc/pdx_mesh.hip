@@ -752,6 +752,16 @@ __device__ __forceinline__ int bspline(float u, float (&w)[P], float (&dw)[P]) {
   return (int)fl - (P - 1);
 }
 
+// v rounded to the nearest integer (ties to even) as a 64-bit integer, for |v| < 2^51: the FP64
+// sum v + 1.5 * 2^52 lies in [2^52, 2^53), where the spacing of doubles is 1, so the addition
+// itself rounds v and leaves it in the low mantissa bits. The same value as __float2ll_rn in
+// four instructions instead of a dozen. The spread's fixed-point scale keeps every charge
+// contribution q w_x w_y w_z 2^S below 2^50.
+__device__ __forceinline__ long long fixed_round(float v) {
+  constexpr double M = 6755399441055744.0;  // 1.5 * 2^52
+  return __double_as_longlong((double)v + M) - __double_as_longlong(M);
+}
+
 // Fractional grid coordinate in [0, K) (_grid_coordinate, MeshKernels.jl:9-15).
 __device__ __forceinline__ float grid_coordinate(float x, float L, int K) {
   float s = x / L;
@@ -814,7 +824,7 @@ __global__ __launch_bounds__(TPB) void k_spread(int N, const float *__restrict__
 #pragma unroll
     for (int j3 = 0; j3 < P; ++j3) {
       const int n3 = wrap_node(k3 + j3, K3);
-      const long long v = __float2ll_rn(q12 * wz[j3]);
+      const long long v = fixed_round(q12 * wz[j3]);
       atomicAdd(&grid[b + KK * n3], (unsigned long long)v);
     }
   }
@@ -832,6 +842,9 @@ __global__ void k_zero64(unsigned long long *g, size_t n) {
 // 64-bit fixed point with LDS integer atomics (exact, order-independent) and stores the
 // block with plain writes: no global atomics, and no clearing (every node is written).
 // Tile edge: 16 nodes, or 8 on meshes too small to give 256 tiles of 16 (one per CU).
+// Each block also lists, for the tile gather, the particles whose base node floor(u) lies in
+// its tile (every particle in exactly one tile): entries (u1, u2, u3, q) and the particle index,
+// up to tcap of them, and the count (which may exceed tcap) in tn[tile].
 constexpr int SLIST = 768;   // particles reaching one block staged in LDS (a few hundred in practice)
 constexpr int SCHUNK = 64;   // staged particles whose weights are computed together
 template <int P, int STILE, int SBS>
@@ -840,7 +853,9 @@ __global__ __launch_bounds__(SBS) void k_spread_tiles(const float4 *__restrict__
                                                       const int32_t *__restrict__ ids,
                                                       const int64_t *__restrict__ state, int nx, int ny,
                                                       int nz, float margin, unsigned long long *grid, int K1,
-                                                      int K2, int K3, float Lx, float Ly, float Lz, float scale) {
+                                                      int K2, int K3, float Lx, float Ly, float Lz, float scale,
+                                                      float4 *__restrict__ tlist, int32_t *__restrict__ tids,
+                                                      int32_t *__restrict__ tn, int tcap) {
   if (state[PDX_DN_ERROR] != 0) return;  // no usable cells / positions: the grid keeps its last value
   constexpr int LPP = lanes_for(P);
   // x rows padded to STILE + 1 nodes: the P x P lanes of a particle then spread over the LDS banks
@@ -870,98 +885,158 @@ __global__ __launch_bounds__(SBS) void k_spread_tiles(const float4 *__restrict__
   // (the cell range is at most 64 cells: a block plus its halo spans < 4 cells per dimension
   // whenever the cells are wider than (STILE + P + skin) / 3 mesh spacings; otherwise the
   // remaining cells are taken in further rounds of 64)
-  if (threadIdx.x == 0) nlist = 0;
-  for (int round = 0; round * 64 < ncand; ++round) {
-    const int nc = min(64, ncand - round * 64);
-    if (threadIdx.x < nc) {
-      const int cc = round * 64 + threadIdx.x;
-      int cx = c0[0] + cc % cn[0], cy = c0[1] + (cc / cn[0]) % cn[1], cz = c0[2] + cc / (cn[0] * cn[1]);
-      cx = ((cx % nx) + nx) % nx;
-      cy = ((cy % ny) + ny) % ny;
-      cz = ((cz % nz) + nz) % nz;
-      const int c = (cz * ny + cy) * nx + cx;
-      cstart[threadIdx.x] = offsets[c] - 1;
-      cpre[threadIdx.x + 1] = offsets[c + 1] - offsets[c];
+  __shared__ int nown;
+  if (threadIdx.x == 0) {
+    nlist = 0;
+    nown = 0;
+  }
+  float4 *own = tlist + (size_t)b * tcap;
+  int32_t *oid = tids + (size_t)b * tcap;
+  // one candidate particle: listed for the gather when its base node lies in the tile, staged
+  // for the spread when its stencil reaches the tile (and its charge is nonzero)
+  auto take = [&](int i, const float4 p) {
+    const float u1 = grid_coordinate(p.x, Lx, K1), u2 = grid_coordinate(p.y, Ly, K2),
+                u3 = grid_coordinate(p.z, Lz, K3);
+    int o1 = (int)floorf(u1) - a1; o1 += o1 < 0 ? K1 : 0;
+    int o2 = (int)floorf(u2) - a2; o2 += o2 < 0 ? K2 : 0;
+    int o3 = (int)floorf(u3) - a3; o3 += o3 < 0 ? K3 : 0;
+    if (o1 < e1 && o2 < e2 && o3 < e3) {
+      const int k = atomicAdd(&nown, 1);
+      if (k < tcap) {
+        own[k] = make_float4(u1, u2, u3, p.w);
+        oid[k] = i;
+      }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      cpre[0] = 0;
-      for (int k = 1; k <= nc; ++k) cpre[k] += cpre[k - 1];
+    if (p.w == 0.0f) return;
+    if (o1 > e1 + P - 2 || o2 > e2 + P - 2 || o3 > e3 + P - 2) return;
+    const int slot = atomicAdd(&nlist, 1);
+    if (slot < SLIST) {
+      list[slot] = make_float4(u1, u2, u3, p.w);
+      return;
     }
-    __syncthreads();
-    const int total = cpre[nc];
-    // flat candidate index -> cell (linear search over <= 64 prefix entries) -> particle
-    for (int f = threadIdx.x; f < total; f += SBS) {
-      int c = 0;
-      while (cpre[c + 1] <= f) ++c;
-      const float4 p = packed[ids[cstart[c] + (f - cpre[c])] - 1];
-      if (p.w == 0.0f) continue;
-      const float u1 = grid_coordinate(p.x, Lx, K1), u2 = grid_coordinate(p.y, Ly, K2),
-                  u3 = grid_coordinate(p.z, Lz, K3);
-      int o1 = (int)floorf(u1) - a1; o1 += o1 < 0 ? K1 : 0;
-      int o2 = (int)floorf(u2) - a2; o2 += o2 < 0 ? K2 : 0;
-      int o3 = (int)floorf(u3) - a3; o3 += o3 < 0 ? K3 : 0;
-      if (o1 > e1 + P - 2 || o2 > e2 + P - 2 || o3 > e3 + P - 2) continue;
-      const int slot = atomicAdd(&nlist, 1);
-      if (slot < SLIST) {
-        list[slot] = make_float4(u1, u2, u3, p.w);
-      } else {  // list full: this thread spreads the particle itself
-        float wx[P], wy[P], wz[P], d[P];
-        bspline<P>(u1, wx, d);
-        bspline<P>(u2, wy, d);
-        bspline<P>(u3, wz, d);
-        for (int j2 = 0; j2 < P; ++j2) {
-          int l2 = o2 - (P - 1) + j2; l2 += l2 < 0 ? K2 : 0; l2 -= l2 >= K2 ? K2 : 0;
-          if (l2 >= e2) continue;
-          for (int j1 = 0; j1 < P; ++j1) {
-            int l1 = o1 - (P - 1) + j1; l1 += l1 < 0 ? K1 : 0; l1 -= l1 >= K1 ? K1 : 0;
-            if (l1 >= e1) continue;
-            const float q12 = p.w * wx[j1] * wy[j2] * scale;
-            for (int j3 = 0; j3 < P; ++j3) {
-              int l3 = o3 - (P - 1) + j3; l3 += l3 < 0 ? K3 : 0; l3 -= l3 >= K3 ? K3 : 0;
-              if (l3 >= e3) continue;
-              atomicAdd(&acc[(l3 * STILE + l2) * SROW + l1], (unsigned long long)__float2ll_rn(q12 * wz[j3]));
-            }
-          }
+    // list full: this thread spreads the particle itself
+    float wx[P], wy[P], wz[P], d[P];
+    bspline<P>(u1, wx, d);
+    bspline<P>(u2, wy, d);
+    bspline<P>(u3, wz, d);
+    for (int j2 = 0; j2 < P; ++j2) {
+      int l2 = o2 - (P - 1) + j2; l2 += l2 < 0 ? K2 : 0; l2 -= l2 >= K2 ? K2 : 0;
+      if (l2 >= e2) continue;
+      for (int j1 = 0; j1 < P; ++j1) {
+        int l1 = o1 - (P - 1) + j1; l1 += l1 < 0 ? K1 : 0; l1 -= l1 >= K1 ? K1 : 0;
+        if (l1 >= e1) continue;
+        const float q12 = p.w * wx[j1] * wy[j2] * scale;
+        for (int j3 = 0; j3 < P; ++j3) {
+          int l3 = o3 - (P - 1) + j3; l3 += l3 < 0 ? K3 : 0; l3 -= l3 >= K3 ? K3 : 0;
+          if (l3 >= e3) continue;
+          atomicAdd(&acc[(l3 * STILE + l2) * SROW + l1], (unsigned long long)fixed_round(q12 * wz[j3]));
         }
       }
     }
+  };
+  for (int round = 0; round * 64 < ncand; ++round) {
+    const int nc = min(64, ncand - round * 64);
+    if (threadIdx.x < 64) {  // wave 0: the round's cells and the inclusive scan of their sizes
+      int cnt = 0;
+      if (threadIdx.x < nc) {
+        const int cc = round * 64 + threadIdx.x;
+        int cx = c0[0] + cc % cn[0], cy = c0[1] + (cc / cn[0]) % cn[1], cz = c0[2] + cc / (cn[0] * cn[1]);
+        cx = ((cx % nx) + nx) % nx;
+        cy = ((cy % ny) + ny) % ny;
+        cz = ((cz % nz) + nz) % nz;
+        const int c = (cz * ny + cy) * nx + cx;
+        cstart[threadIdx.x] = offsets[c] - 1;
+        cnt = offsets[c + 1] - offsets[c];
+      }
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(cnt, o, 64);
+        if (threadIdx.x >= o) cnt += u;
+      }
+      cpre[threadIdx.x + 1] = cnt;
+      if (threadIdx.x == 0) cpre[0] = 0;
+    }
+    __syncthreads();
+    const int total = cpre[nc];
+    // flat candidate index f -> its cell c (the last c < nc with cpre[c] <= f: six halvings)
+    // -> particle; CB candidates per thread and pass, their loads issued together (two memory
+    // round trips per pass: ids, then positions)
+    auto cell_of = [&](int f) {
+      int c = 0;
+#pragma unroll
+      for (int h = 32; h >= 1; h >>= 1)
+        if (c + h < nc && cpre[c + h] <= f) c += h;
+      return cstart[c] + (f - cpre[c]);
+    };
+    constexpr int CB = 4;
+    for (int f0 = threadIdx.x; f0 < total; f0 += CB * SBS) {
+      int ic[CB];
+      float4 pc[CB];
+#pragma unroll
+      for (int u = 0; u < CB; ++u) {
+        const int f = f0 + u * SBS;
+        ic[u] = f < total ? ids[cell_of(f)] - 1 : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < CB; ++u) pc[u] = packed[ic[u] >= 0 ? ic[u] : ic[0]];
+#pragma unroll
+      for (int u = 0; u < CB; ++u)
+        if (ic[u] >= 0) take(ic[u], pc[u]);
+    }
     __syncthreads();
   }
-  // staged particles, SCHUNK at a time: their B-spline weights are computed once (three threads
-  // per particle, one dimension each) into LDS; then LPP lanes per particle, lane = one (x, y)
-  // line of the stencil walking P z nodes. The products and their order are those of
-  // k_spread: ((q w_x) w_y) 2^S, then times w_z.
+  if (threadIdx.x == 0) tn[b] = nown;
+  // staged particles, SCHUNK at a time: per particle and dimension (three threads each) the
+  // B-spline weights and the tile-local index of each stencil node (a byte; 0xFF: outside the
+  // tile) go to LDS; then LPP lanes per particle, lane = one (x, y) line of the stencil walking
+  // its P z nodes. The products and their order are those of k_spread: ((q w_x) w_y) 2^S, then
+  // times w_z, rounded to the nearest integer.
   const int nl = min(nlist, SLIST);
-  __shared__ float wsh[SCHUNK][3][P];
-  __shared__ int ksh[SCHUNK][3];
+  __shared__ float wsh[SCHUNK][3][8];
+  __shared__ uint2 lsh[SCHUNK][3];
   const int grp = threadIdx.x / LPP, lane = threadIdx.x % LPP;
+  auto node_byte = [](const uint2 x, int j) { return ((j < 4 ? x.x : x.y) >> (8 * (j & 3))) & 0xFFu; };
   for (int c0 = 0; c0 < nl; c0 += SCHUNK) {
     const int nc = min(SCHUNK, nl - c0);
     __syncthreads();
     for (int e = threadIdx.x; e < 3 * nc; e += SBS) {
       const int t = e / 3, d = e - 3 * t;
       const float4 u = list[c0 + t];
+      const int K = d == 0 ? K1 : (d == 1 ? K2 : K3);
+      const int a = d == 0 ? a1 : (d == 1 ? a2 : a3);
+      const int ee = d == 0 ? e1 : (d == 1 ? e2 : e3);
       float w[P], dw[P];
-      ksh[t][d] = bspline<P>(d == 0 ? u.x : (d == 1 ? u.y : u.z), w, dw);
+      const int k = bspline<P>(d == 0 ? u.x : (d == 1 ? u.y : u.z), w, dw);
+      unsigned lo = 0xFFFFFFFFu, hi = 0xFFFFFFFFu;
 #pragma unroll
-      for (int j = 0; j < P; ++j) wsh[t][d][j] = w[j];
+      for (int j = 0; j < P; ++j) {
+        wsh[t][d][j] = w[j];
+        int l = wrap_node(k + j, K) - a;
+        l += l < 0 ? K : 0;
+        const unsigned bj = l < ee ? (unsigned)l : 0xFFu;
+        if (j < 4) lo = (lo & ~(0xFFu << (8 * j))) | (bj << (8 * j));
+        else hi = (hi & ~(0xFFu << (8 * (j - 4)))) | (bj << (8 * (j - 4)));
+      }
+      lsh[t][d] = make_uint2(lo, hi);
     }
     __syncthreads();
     for (int t = grp; t < nc; t += SBS / LPP) {
       const float qv = list[c0 + t].w;
-      const int k1 = ksh[t][0], k2 = ksh[t][1], k3 = ksh[t][2];
       for (int l = lane; l < P * P; l += LPP) {
         const int j1 = l % P, j2 = l / P;
-        int l1 = wrap_node(k1 + j1, K1) - a1; l1 += l1 < 0 ? K1 : 0;
-        int l2 = wrap_node(k2 + j2, K2) - a2; l2 += l2 < 0 ? K2 : 0;
-        if (l1 >= e1 || l2 >= e2) continue;
+        const unsigned l1 = node_byte(lsh[t][0], j1), l2 = node_byte(lsh[t][1], j2);
+        if (l1 == 0xFFu || l2 == 0xFFu) continue;
         const float q12 = qv * wsh[t][0][j1] * wsh[t][1][j2] * scale;
+        float wz[P];
+#pragma unroll
+        for (int j3 = 0; j3 < P; ++j3) wz[j3] = wsh[t][2][j3];
+        const uint2 x3 = lsh[t][2];
+        unsigned long long *line = acc + l2 * SROW + l1;
 #pragma unroll
         for (int j3 = 0; j3 < P; ++j3) {
-          int l3 = wrap_node(k3 + j3, K3) - a3; l3 += l3 < 0 ? K3 : 0;
-          if (l3 >= e3) continue;
-          atomicAdd(&acc[(l3 * STILE + l2) * SROW + l1], (unsigned long long)__float2ll_rn(q12 * wsh[t][2][j3]));
+          const unsigned l3 = node_byte(x3, j3);
+          if (l3 == 0xFFu) continue;
+          atomicAdd(line + l3 * (STILE * SROW), (unsigned long long)fixed_round(q12 * wz[j3]));
         }
       }
     }
@@ -1056,6 +1131,184 @@ __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__
   } else if (qi != 0.0f) {
     fx[i] += ffx; fy[i] += ffy; fz[i] += ffz;
     if (ENERGY && Epot) Epot[i] += 0.5f * qi * phi + bg * qi;
+  }
+}
+
+// ---- tile gather: the forces at the positions of the last owner-computes spread, from the
+// particle lists it wrote (k_spread_tiles). One workgroup per tile, in three phases with one
+// memory round trip each: (1) the potential of the nodes the tile's particles reach (the tile
+// and the P-1 nodes below it in each dimension, (STILE+P-1)^3 values) and the tile's list are
+// loaded, every load issued before the first LDS store; (2) each listed particle is gathered
+// from LDS with the lane layout of k_gather (lane = x node, (y, z) lines unrolled, DPP sums)
+// into per-particle results in LDS; (3) the results are written (or added) to the particles'
+// outputs. Tiles whose list overflowed (tn > tcap) are left to the workgroups past the tiles:
+// they walk all particles and gather those of overflowed tiles from global memory, one
+// particle per thread. Per particle the result is k_gather's up to FP32 rounding.
+constexpr int GCHUNK = 256;  // listed particles per LDS chunk (a tile holds ~N/tiles)
+template <int P, int STILE, bool ENERGY>
+__global__ __launch_bounds__(TPB) void k_gather_tiles(int N, const float4 *__restrict__ packed,
+                                                      const float4 *__restrict__ tlist,
+                                                      const int32_t *__restrict__ tids,
+                                                      const int32_t *__restrict__ tn, int tcap, int tiles,
+                                                      const float *__restrict__ pot, int K1, int K2, int K3,
+                                                      float Lx, float Ly, float Lz, float *fx, float *fy,
+                                                      float *fz, float *Epot, float bg, int set, float *Vglobal,
+                                                      const double *acc, double ubg) {
+  constexpr int R = STILE + P - 1;
+  __shared__ float reg[R * R * R];
+  __shared__ float4 pl[GCHUNK];
+  __shared__ int pid[GCHUNK];
+  __shared__ float res[4][GCHUNK];
+  if (Vglobal && blockIdx.x == 0 && threadIdx.x < 6)
+    Vglobal[threadIdx.x] = (float)(acc[1 + threadIdx.x] + (threadIdx.x < 3 ? ubg : 0.0));
+  const int tx = (K1 + STILE - 1) / STILE, ty = (K2 + STILE - 1) / STILE;
+  const float s1 = (float)K1 / Lx, s2 = (float)K2 / Ly, s3 = (float)K3 / Lz;
+  if ((int)blockIdx.x >= tiles) {
+    const int stride = (gridDim.x - tiles) * TPB;
+    const size_t KK = (size_t)K1 * K2;
+    for (int i = (blockIdx.x - tiles) * TPB + threadIdx.x; i < N; i += stride) {
+      const float4 p = packed[i];
+      const float u1 = grid_coordinate(p.x, Lx, K1), u2 = grid_coordinate(p.y, Ly, K2),
+                  u3 = grid_coordinate(p.z, Lz, K3);
+      const int t = (int)floorf(u1) / STILE + tx * ((int)floorf(u2) / STILE + ty * ((int)floorf(u3) / STILE));
+      if (tn[t] <= tcap) continue;
+      float wx[P], wy[P], wz[P], dx[P], dy[P], dz[P];
+      const int k1 = bspline<P>(u1, wx, dx), k2 = bspline<P>(u2, wy, dy), k3 = bspline<P>(u3, wz, dz);
+      float phi = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+      for (int j3 = 0; j3 < P; ++j3)
+#pragma unroll
+        for (int j2 = 0; j2 < P; ++j2) {
+          const size_t b = KK * wrap_node(k3 + j3, K3) + (size_t)K1 * wrap_node(k2 + j2, K2);
+          float lp = 0.f, lg = 0.f;
+#pragma unroll
+          for (int j1 = 0; j1 < P; ++j1) {
+            const float v = pot[b + wrap_node(k1 + j1, K1)];
+            lp = fmaf(wx[j1], v, lp);
+            lg = fmaf(dx[j1], v, lg);
+          }
+          phi = fmaf(wy[j2] * wz[j3], lp, phi);
+          gx = fmaf(wy[j2] * wz[j3], lg, gx);
+          gy = fmaf(dy[j2] * wz[j3], lp, gy);
+          gz = fmaf(wy[j2] * dz[j3], lp, gz);
+        }
+      const float qi = p.w;
+      if (set) {
+        fx[i] = -(qi * gx * s1); fy[i] = -(qi * gy * s2); fz[i] = -(qi * gz * s3);
+        if (ENERGY && Epot) Epot[i] = 0.5f * qi * phi + bg * qi;
+      } else if (qi != 0.0f) {
+        fx[i] -= qi * gx * s1; fy[i] -= qi * gy * s2; fz[i] -= qi * gz * s3;
+        if (ENERGY && Epot) Epot[i] += 0.5f * qi * phi + bg * qi;
+      }
+    }
+    return;
+  }
+  const int b = blockIdx.x;
+  const int n = tn[b];
+  if (n <= 0 || n > tcap) return;
+  const int a1 = (b % tx) * STILE, a2 = ((b / tx) % ty) * STILE, a3 = (b / (tx * ty)) * STILE;
+  const int r1 = min(STILE, K1 - a1) + P - 1, r2 = min(STILE, K2 - a2) + P - 1, r3 = min(STILE, K3 - a3) + P - 1;
+  const float4 *own = tlist + (size_t)b * tcap;
+  const int32_t *oid = tids + (size_t)b * tcap;
+  // phase 1: region node l (per dimension) = grid node a - (P-1) + l, wrapped below zero
+  constexpr int NLD = (R * R * R + TPB - 1) / TPB;
+  float v[NLD];
+#pragma unroll
+  for (int j = 0; j < NLD; ++j) {
+    const int k = threadIdx.x + j * TPB;
+    const int l1 = k % R, l2 = (k / R) % R, l3 = k / (R * R);
+    v[j] = 0.f;
+    if (k < R * R * R && l1 < r1 && l2 < r2 && l3 < r3) {
+      int g1 = a1 - (P - 1) + l1; g1 += g1 < 0 ? K1 : 0;
+      int g2 = a2 - (P - 1) + l2; g2 += g2 < 0 ? K2 : 0;
+      int g3 = a3 - (P - 1) + l3; g3 += g3 < 0 ? K3 : 0;
+      v[j] = pot[((size_t)g3 * K2 + g2) * K1 + g1];
+    }
+  }
+  static_assert(GCHUNK == TPB, "one list entry per thread and chunk");
+  float4 e0 = make_float4(0.f, 0.f, 0.f, 0.f);
+  int i0 = -1;
+  if (threadIdx.x < n) {
+    e0 = own[threadIdx.x];
+    i0 = oid[threadIdx.x];
+  }
+#pragma unroll
+  for (int j = 0; j < NLD; ++j) {
+    const int k = threadIdx.x + j * TPB;
+    if (k < R * R * R) reg[k] = v[j];
+  }
+  constexpr int LPP = gather_lanes(P);
+  const int lane = threadIdx.x % LPP;
+  for (int c0 = 0; c0 < n; c0 += GCHUNK) {
+    const int nc = min(GCHUNK, n - c0);
+    if (c0 > 0) {
+      __syncthreads();  // the previous chunk's results are written out
+      e0 = make_float4(0.f, 0.f, 0.f, 0.f);
+      i0 = -1;
+      if (threadIdx.x < nc) {
+        e0 = own[c0 + threadIdx.x];
+        i0 = oid[c0 + threadIdx.x];
+      }
+    }
+    pl[threadIdx.x] = e0;
+    pid[threadIdx.x] = i0;
+    __syncthreads();
+    // phase 2 (t is uniform over a particle's LPP lanes, so the lane sums see only active lanes)
+    for (int t = threadIdx.x / LPP; t < nc; t += TPB / LPP) {
+      const float4 e = pl[t];
+      const int i = pid[t];
+      const float qi = (i >= 0 && i < N) ? e.w : 0.0f;
+      float phi = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+      if (qi != 0.0f) {
+        float wx[P], wy[P], wz[P], dx[P], dy[P], dz[P];
+        // base node floor(u) = k + P - 1 lies in the tile: local offsets 0 .. STILE-1
+        const int o1 = min(max(bspline<P>(e.x, wx, dx) + (P - 1) - a1, 0), STILE - 1);
+        const int o2 = min(max(bspline<P>(e.y, wy, dy) + (P - 1) - a2, 0), STILE - 1);
+        const int o3 = min(max(bspline<P>(e.z, wz, dz) + (P - 1) - a3, 0), STILE - 1);
+        const int jx = lane < P ? lane : 0;
+        const float ax = lane < P ? sel<P>(wx, jx) : 0.0f;
+        const float bx = lane < P ? sel<P>(dx, jx) : 0.0f;
+        const float *base = reg + (o3 * R + o2) * R + o1 + jx;
+#pragma unroll
+        for (int j3 = 0; j3 < P; ++j3) {
+#pragma unroll
+          for (int j2 = 0; j2 < P; ++j2) {
+            const float val = base[(j3 * R + j2) * R];
+            const float w23 = wy[j2] * wz[j3];
+            phi = fmaf(w23, val, phi);
+            gy = fmaf(dy[j2] * wz[j3], val, gy);
+            gz = fmaf(wy[j2] * dz[j3], val, gz);
+          }
+        }
+        gx = bx * phi;
+        phi *= ax;
+        gy *= ax;
+        gz *= ax;
+      }
+      gx = sum8_lanes(gx);
+      gy = sum8_lanes(gy);
+      gz = sum8_lanes(gz);
+      if (ENERGY) phi = sum8_lanes(phi);
+      if (lane == 0) {
+        res[0][t] = -(qi * gx * s1);
+        res[1][t] = -(qi * gy * s2);
+        res[2][t] = -(qi * gz * s3);
+        res[3][t] = 0.5f * qi * phi + bg * qi;
+      }
+    }
+    __syncthreads();
+    // phase 3: one listed particle per thread
+    const int i = pid[threadIdx.x];
+    if (threadIdx.x < nc && i >= 0 && i < N) {
+      const float qi = pl[threadIdx.x].w;
+      if (set) {
+        fx[i] = res[0][threadIdx.x]; fy[i] = res[1][threadIdx.x]; fz[i] = res[2][threadIdx.x];
+        if (ENERGY && Epot) Epot[i] = res[3][threadIdx.x];
+      } else if (qi != 0.0f) {
+        fx[i] += res[0][threadIdx.x]; fy[i] += res[1][threadIdx.x]; fz[i] += res[2][threadIdx.x];
+        if (ENERGY && Epot) Epot[i] += res[3][threadIdx.x];
+      }
+    }
   }
 }
 
@@ -1712,6 +1965,20 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
     delete m;
     return rc;
   }
+  {
+    // tile lists: twice the mean occupancy plus 64 entries per tile, for the larger tile count
+    const int nt8 = ((K[0] + 7) / 8) * ((K[1] + 7) / 8) * ((K[2] + 7) / 8);
+    m->tile_entries = (size_t)2 * n + (size_t)64 * nt8;
+    if ((rc = m->tile_list.alloc(m->tile_entries)) || (rc = m->tile_ids.alloc(m->tile_entries)) ||
+        (rc = m->tile_n.alloc(nt8))) {
+      delete m;
+      return rc;
+    }
+    if (hipMemset(m->tile_n.p, 0, sizeof(int32_t) * nt8) != hipSuccess) {
+      delete m;
+      return fail(PDX_ERR_DEVICE, "mesh initialisation failed");
+    }
+  }
   if (hipMemset(m->grid64.p, 0, sizeof(unsigned long long) * nnodes) != hipSuccess ||
       hipMemset(m->acc.p, 0, sizeof(double) * 8) != hipSuccess ||
       hipMemset(m->ticket.p, 0, sizeof(unsigned)) != hipSuccess ||
@@ -1750,8 +2017,9 @@ extern "C" int pdx_mesh_info(pdx_mesh_t m, double out[9]) {
 extern "C" int pdx_mesh_set_charges(pdx_mesh_t m, const float *q, int32_t n) {
   if (!m || (!q && n > 0)) return fail(PDX_ERR_ARGUMENT, "null argument");
   if (n != m->N) return fail(PDX_ERR_DIMENSION, "charge count differs from the mesh particle count");
-  double Q = 0, qabs = 0, qsq = 0;
+  double Q = 0, qabs = 0, qsq = 0, qmax = 0;
   for (int i = 0; i < n; ++i) {
+    qmax = std::max(qmax, (double)std::fabs(q[i]));
     if (!std::isfinite(q[i])) return fail(PDX_ERR_ARGUMENT, "Charges must be finite.");
     Q += q[i];
     qabs += std::fabs((double)q[i]);
@@ -1765,9 +2033,12 @@ extern "C" int pdx_mesh_set_charges(pdx_mesh_t m, const float *q, int32_t n) {
   m->qabs = qabs;
   m->qsq = qsq;
   // fixed-point scale 2^S: every node sum is bounded by sum |q| (weights sum to 1 per
-  // particle), so |node| * 2^S < 2^62 cannot overflow; S <= 40 keeps 2^-40 resolution.
+  // particle), so |node| * 2^S < 2^62 cannot overflow; S <= 40 keeps 2^-40 resolution, and
+  // S <= 50 - log2 max|q| keeps every single contribution q w w w 2^S below 2^50 (the range
+  // of the spreads' FP64 rounding; no change for charges up to 1024)
   int S = (int)std::floor(62.0 - std::log2(std::max(qabs, 1.0)));
   S = std::min(S, 40);
+  S = std::min(S, (int)std::floor(50.0 - std::log2(std::max(qmax, 1.0))));
   m->fixed_scale = std::ldexp(1.0, S);
   m->fixed_inv = std::ldexp(1.0, -S);
   refresh_background(m);
@@ -1782,6 +2053,7 @@ extern "C" int pdx_mesh_set_box(pdx_mesh_t m, const double box[3]) {
   if (!(2 * m->spec.real_cutoff < mn)) return fail(PDX_ERR_ARGUMENT, "real_cutoff must stay below half the shortest box length.");
   PDX_HIP(hipDeviceSynchronize());
   for (int d = 0; d < 3; ++d) m->box[d] = box[d];
+  m->tiles_nl = nullptr;
   int rc = compute_influence(m);
   if (rc) return rc;
   refresh_background(m);
@@ -1792,6 +2064,7 @@ extern "C" int pdx_mesh_spread(pdx_mesh_t m, const float *x, const float *y, con
                                const int32_t *order, pdx_stream_t stream) {
   if (!m || !x || !y || !z || !q) return fail(PDX_ERR_ARGUMENT, "null argument");
   const int N = m->N;
+  m->tiles_nl = nullptr;  // this spread writes no tile lists
   const float L[3] = {(float)m->box[0], (float)m->box[1], (float)m->box[2]};
   const float scale = (float)m->fixed_scale;
   hipStream_t st = as_stream(stream);
@@ -1836,10 +2109,17 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
   // faster spread on its own; 256 leaves more room beside a concurrent pair kernel
   int sbs = m->spread_threads;
   if (const char *e = std::getenv("PDX_SPREAD_BS")) sbs = std::atoi(e) == 256 ? 256 : 1024;
+  int tcap = (int)std::min<size_t>(m->tile_entries / tiles, INT32_MAX);
+  if (const char *e = std::getenv("PDX_TILE_CAP")) tcap = std::max(0, std::min(tcap, std::atoi(e)));  // tests: overflow
+  m->tiles_nl = nl;
+  m->tiles_stile = small ? 8 : 16;
+  m->tiles_count = tiles;
+  m->tiles_cap = tcap;
 #define PDX_TSPREAD_L(PP, TT, BB)                                                                              \
   hipLaunchKernelGGL((k_spread_tiles<PP, TT, BB>), dim3(tiles), dim3(BB), 0, st, nl->packed.p,                 \
                      offsets, ids, guard, nl->nx, nl->ny, nl->nz,                                              \
-                     margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale);
+                     margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale, m->tile_list.p,   \
+                     m->tile_ids.p, m->tile_n.p, tcap);
 #define PDX_TSPREAD(PP)                                                                                        \
   case PP:                                                                                                     \
     if (small) {                                                                                               \
@@ -1946,6 +2226,48 @@ extern "C" int pdx_mesh_gather(pdx_mesh_t m, const float *x, const float *y, con
     PDX_GATHER_CASE(8)
     default: return fail(PDX_ERR_ARGUMENT, "unsupported order");
   }
+  PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
+extern "C" int pdx_mesh_gather_cells(pdx_mesh_t m, pdx_nl_t nl, float *fx, float *fy, float *fz, float *Epot,
+                                     float *Vglobal, int32_t flags, int32_t set, pdx_stream_t stream) {
+  if (!m || !nl || !fx || !fy || !fz) return fail(PDX_ERR_ARGUMENT, "null argument");
+  if (m->spec.influence == PDX_INFLUENCE_P3M_IK)
+    return fail(PDX_ERR_ARGUMENT, "the tile gather differentiates the potential (ad); ik meshes use pdx_mesh_gather");
+  if (m->tiles_nl != (const void *)nl)
+    return fail(PDX_ERR_STATE, "the last spread of this mesh was not an owner-computes spread from this list");
+  const int N = m->N;
+  const float L[3] = {(float)m->box[0], (float)m->box[1], (float)m->box[2]};
+  const bool energy = (flags & PDX_ENERGY) != 0;
+  if (!energy) Vglobal = nullptr;
+  hipStream_t st = as_stream(stream);
+  // workgroups past the tiles: the overflow walk, four particles per thread
+  const int extra = (int)std::max<int64_t>(1, std::min<int64_t>(256, ((int64_t)N + 4 * TPB - 1) / (4 * TPB)));
+  const dim3 grid(m->tiles_count + extra);
+#define PDX_GT_L(PP, TT, EE)                                                                                   \
+  hipLaunchKernelGGL((k_gather_tiles<PP, TT, EE>), grid, dim3(TPB), 0, st, N, nl->packed.p, m->tile_list.p,    \
+                     m->tile_ids.p, m->tile_n.p, m->tiles_cap, m->tiles_count, m->pot.p, m->K1, m->K2, m->K3, L[0], L[1],     \
+                     L[2], fx, fy, fz, Epot, m->bg, set, Vglobal, m->acc.p, m->ubg);
+#define PDX_GT(PP)                                                                                             \
+  case PP:                                                                                                     \
+    if (m->tiles_stile == 8) {                                                                                 \
+      if (energy) { PDX_GT_L(PP, 8, true) } else { PDX_GT_L(PP, 8, false) }                                    \
+    } else {                                                                                                   \
+      if (energy) { PDX_GT_L(PP, 16, true) } else { PDX_GT_L(PP, 16, false) }                                  \
+    }                                                                                                          \
+    break;
+  switch (m->P) {
+    PDX_GT(3)
+    PDX_GT(4)
+    PDX_GT(5)
+    PDX_GT(6)
+    PDX_GT(7)
+    PDX_GT(8)
+    default: return fail(PDX_ERR_ARGUMENT, "unsupported order");
+  }
+#undef PDX_GT
+#undef PDX_GT_L
   PDX_LAUNCH_CHECK();
   return PDX_OK;
 }

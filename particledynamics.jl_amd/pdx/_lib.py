@@ -117,6 +117,7 @@ def _load():
         "pdx_nl_snapshot": ([vp, vp], C.c_int),
         "pdx_mesh_solve": ([vp, i32, vp], C.c_int),
         "pdx_mesh_gather": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp], C.c_int),
+        "pdx_mesh_gather_cells": ([vp, vp, vp, vp, vp, vp, vp, i32, i32, vp], C.c_int),
         "pdx_mesh_totals": ([vp, P(dbl), vp], C.c_int),
         "pdx_mesh_copy_grid": ([vp, i32, vp, vp], C.c_int),
         "pdx_mesh_fft": ([vp, i32, vp, vp, vp], C.c_int),

@@ -205,6 +205,14 @@ class ElectrostaticsSolver:
                                       L.ptr(fy), L.ptr(fz), L.ptr(Epot), L.ptr(Vglobal),
                                       L.PDX_ENERGY if energy else 0, 1 if set else 0, L.stream_handle(stream)))
 
+    def gather_cells(self, nl, fx, fy, fz, Epot=None, Vglobal=None, energy=True, set=False, stream=None):
+        """``mesh_gather`` at the positions of the last ``spread_cells`` / ``spread_snapshot`` from
+        ``nl``: one workgroup per mesh tile gathers the particles that spread listed under it
+        from the tile's potential staged in LDS (smooth PME and P3M-ad)."""
+        L.check(L.lib.pdx_mesh_gather_cells(self._h, nl.handle, L.ptr(fx), L.ptr(fy), L.ptr(fz), L.ptr(Epot),
+                                            L.ptr(Vglobal), L.PDX_ENERGY if energy else 0, 1 if set else 0,
+                                            L.stream_handle(stream)))
+
     def compute_mesh(self, x, y, z, q, fx, fy, fz, Epot=None, Vglobal=None, energy=True, order=None, set=False,
                      stream=None):
         self.mesh_transform(x, y, z, q, energy, order, stream)

@@ -23,7 +23,7 @@ class ForceEngine:
     def __init__(self, N, box, pair_model=None, electrostatics=None, charges=None, skin=0.4, cap=None,
                  interval=20, lanes=0, diagnostics=False, spread="snapshot", correction=None, accumulation="fp32",
                  bonds=None, exceptions=(), traversal="rows", backend="ell", prune_buffer=None, cluster_capacity=0,
-                 overlap_with="fft"):
+                 overlap_with="fft", gather="tiles"):
         """backend "ell": the cell list and ELL rows (``pdx_nl_*`` + ``pdx_pair_forces``); "cluster":
         the cluster-pair list and tiles (``pdx_cluster_*``, the reference's ``ClusterBackend``,
         src/engine/ClusterBackend.jl:1-40); its mesh spread takes the cluster order."""
@@ -77,6 +77,12 @@ class ForceEngine:
         if spread not in ("snapshot", "cells", "atomic"):
             raise L.ArgumentError("spread must be 'snapshot', 'cells' or 'atomic'")
         self.spread_mode = spread
+        # "tiles": after an owner-computes spread, the gather runs tile by tile from the particle
+        # lists that spread wrote (smooth PME / P3M-ad); "particles": the per-particle gather
+        if gather not in ("tiles", "particles"):
+            raise L.ArgumentError("gather must be 'tiles' or 'particles'")
+        self._tile_gather = (gather == "tiles" and electrostatics is not None
+                             and getattr(electrostatics, "influence", "spme") != "p3m_ik")
         # learned force correction of a coarse mesh (an MLP of the charge-grid features): the
         # features are taken on the mesh stream after the spread, the network's output is
         # added to the forces after the gather; energies and virials are not corrected
@@ -137,8 +143,13 @@ class ForceEngine:
         self._pairs(fx, fy, fz, Epot, V, energy, main)
         if overlap:
             main.wait_stream(side)
-        self.solver.mesh_gather(x, y, z, q, fx, fy, fz, Epot, self.Vglobal if energy else None, energy,
-                                order=order, set=False, stream=main)
+        if self._tile_gather and self.cluster is None and self.spread_mode in ("cells", "snapshot"):
+            # the owner-computes spread listed the particles per tile: gather them tile by tile
+            self.solver.gather_cells(self.nl, fx, fy, fz, Epot, self.Vglobal if energy else None, energy,
+                                     set=False, stream=main)
+        else:
+            self.solver.mesh_gather(x, y, z, q, fx, fy, fz, Epot, self.Vglobal if energy else None, energy,
+                                    order=order, set=False, stream=main)
         if self.correction is not None:
             self.correction.forward(self.feats, None, fx, fy, fz, stream=main)
 

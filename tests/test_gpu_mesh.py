@@ -102,6 +102,57 @@ def test_spread_cells_equals_atomic_spread(cuda, P, mesh):
         assert torch.equal(ga, gb), f"move {move}"
 
 
+@pytest.mark.parametrize("P,mesh,cap", [(5, (20, 22, 24), None), (4, (33, 16, 40), None), (8, (48, 48, 54), None),
+                                         (5, (112, 128, 128), None), (5, (112, 128, 128), "3"), (6, (48, 48, 54), "0")])
+def test_gather_cells_equals_particle_gather(cuda, P, mesh, cap, monkeypatch):
+    """The tile gather (particles listed per tile by the owner-computes spread, potential staged in
+    LDS) gives the per-particle gather's forces, energies and virial to FP32 rounding, written
+    (set) and added; with a tile capacity of 3 (or 0) most (all) tiles overflow and their particles
+    take the in-kernel per-particle walk. Refused after a spread that wrote no tile lists."""
+    import torch
+    if cap is not None:
+        monkeypatch.setenv("PDX_TILE_CAP", cap)
+    N = 3000
+    R, q = systems.random_cloud(N, L_ES, seed=P + 3)
+    q[::97] = 0.0  # neutral particles get zero mesh forces (written with set)
+    nz = q != 0
+    q[nz] -= q[nz].mean()
+    s = _solver(mesh, P=P, N=N, q=q)
+    X, Y, Z, Q = dev(R[:, 0]), dev(R[:, 1]), dev(R[:, 2]), dev(q)
+    nl = pdx.DeviceNeighborWorkspace(N, L_ES, 4.0, 0.3, 512)
+    nl.update(X, Y, Z, Q)
+    s.spread_cells(nl)
+    s.solve(energy=True)
+    out = {}
+    for mode in ("particles", "tiles"):
+        f = [torch.full((N,), 7.0, device="cuda") for _ in range(4)]
+        Vg = torch.zeros(6, device="cuda")
+        if mode == "particles":
+            s.mesh_gather(X, Y, Z, Q, f[0], f[1], f[2], f[3], Vg, energy=True, set=True)
+        else:
+            s.gather_cells(nl, f[0], f[1], f[2], f[3], Vg, energy=True, set=True)
+        a = [t.clone() for t in f]
+        if mode == "particles":
+            s.mesh_gather(X, Y, Z, Q, f[0], f[1], f[2], f[3], None, energy=True, set=False)
+        else:
+            s.gather_cells(nl, f[0], f[1], f[2], f[3], None, energy=True, set=False)
+        out[mode] = (a, [t.clone() for t in f], Vg.clone())
+    (a0, b0, V0), (a1, b1, V1) = out["particles"], out["tiles"]
+    # the tile path keeps k_gather's lane layout and sum order; the overflow walk sums each
+    # x run first (another FP32 rounding of the same 125-term sums)
+    tol = 2e-6 if cap is None else 1e-5
+    for k in range(4):
+        scale = a0[k].abs().max().item()
+        assert (a1[k] - a0[k]).abs().max().item() <= tol * scale, k
+        assert (b1[k] - b0[k]).abs().max().item() <= 2 * tol * scale, k
+        zq = torch.from_numpy(q == 0).cuda()
+        assert torch.equal(a1[k][zq], torch.zeros_like(a1[k][zq]))
+    assert torch.equal(V0, V1)
+    s.spread(X, Y, Z, Q)
+    with pytest.raises(pdx.ArgumentError):
+        s.gather_cells(nl, a1[0], a1[1], a1[2], None, None, energy=False)
+
+
 @pytest.mark.parametrize("mesh", [(120, 120, 120), (60, 60, 60), (48, 45, 50), (64, 128, 30)])
 def test_plane_transforms_equal_line_passes(cuda, mesh, monkeypatch):
     """The plane transforms (x and y of a z plane in one workgroup, the z solve of a y plane in
