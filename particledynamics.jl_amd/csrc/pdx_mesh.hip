@@ -845,10 +845,39 @@ __global__ void k_zero64(unsigned long long *g, size_t n) {
 // Each block also lists, for the tile gather, the particles whose base node floor(u) lies in
 // its tile (every particle in exactly one tile): entries (u1, u2, u3, q) and the particle index,
 // up to tcap of them, and the count (which may exceed tcap) in tn[tile].
+// One particle straight into the tile's accumulator (o = floor(u) - tile origin, wrapped), in the
+// product order of k_spread; the path of particles beyond the staged list.
+template <int P, int STILE>
+__device__ __forceinline__ void spread_direct(unsigned long long *acc, float u1, float u2, float u3,
+                                                        float q, int o1, int o2, int o3, int e1, int e2, int e3,
+                                                        int K1, int K2, int K3, float scale) {
+  constexpr int SROW = STILE + 1;
+  float wx[P], wy[P], wz[P], d[P];
+  bspline<P>(u1, wx, d);
+  bspline<P>(u2, wy, d);
+  bspline<P>(u3, wz, d);
+  for (int j2 = 0; j2 < P; ++j2) {
+    int l2 = o2 - (P - 1) + j2; l2 += l2 < 0 ? K2 : 0; l2 -= l2 >= K2 ? K2 : 0;
+    if (l2 >= e2) continue;
+    for (int j1 = 0; j1 < P; ++j1) {
+      int l1 = o1 - (P - 1) + j1; l1 += l1 < 0 ? K1 : 0; l1 -= l1 >= K1 ? K1 : 0;
+      if (l1 >= e1) continue;
+      const float q12 = q * wx[j1] * wy[j2] * scale;
+      for (int j3 = 0; j3 < P; ++j3) {
+        int l3 = o3 - (P - 1) + j3; l3 += l3 < 0 ? K3 : 0; l3 -= l3 >= K3 ? K3 : 0;
+        if (l3 >= e3) continue;
+        atomicAdd(&acc[(l3 * STILE + l2) * SROW + l1], (unsigned long long)fixed_round(q12 * wz[j3]));
+      }
+    }
+  }
+}
+
 constexpr int SLIST = 768;   // particles reaching one block staged in LDS (a few hundred in practice)
 constexpr int SCHUNK = 64;   // staged particles whose weights are computed together
 template <int P, int STILE, int SBS>
-__global__ __launch_bounds__(SBS) void k_spread_tiles(const float4 *__restrict__ packed,
+// (1024-thread workgroups of orders up to 5 are held to 64 VGPRs, 8 waves per SIMD: two of them,
+// 32 waves, then fit on a CU; higher orders would spill)
+__global__ __launch_bounds__(SBS, (SBS == 1024 && P <= 5) ? 8 : 1) void k_spread_tiles(const float4 *__restrict__ packed,
                                                       const int32_t *__restrict__ offsets,
                                                       const int32_t *__restrict__ ids,
                                                       const int64_t *__restrict__ state, int nx, int ny,
@@ -915,24 +944,7 @@ __global__ __launch_bounds__(SBS) void k_spread_tiles(const float4 *__restrict__
       return;
     }
     // list full: this thread spreads the particle itself
-    float wx[P], wy[P], wz[P], d[P];
-    bspline<P>(u1, wx, d);
-    bspline<P>(u2, wy, d);
-    bspline<P>(u3, wz, d);
-    for (int j2 = 0; j2 < P; ++j2) {
-      int l2 = o2 - (P - 1) + j2; l2 += l2 < 0 ? K2 : 0; l2 -= l2 >= K2 ? K2 : 0;
-      if (l2 >= e2) continue;
-      for (int j1 = 0; j1 < P; ++j1) {
-        int l1 = o1 - (P - 1) + j1; l1 += l1 < 0 ? K1 : 0; l1 -= l1 >= K1 ? K1 : 0;
-        if (l1 >= e1) continue;
-        const float q12 = p.w * wx[j1] * wy[j2] * scale;
-        for (int j3 = 0; j3 < P; ++j3) {
-          int l3 = o3 - (P - 1) + j3; l3 += l3 < 0 ? K3 : 0; l3 -= l3 >= K3 ? K3 : 0;
-          if (l3 >= e3) continue;
-          atomicAdd(&acc[(l3 * STILE + l2) * SROW + l1], (unsigned long long)fixed_round(q12 * wz[j3]));
-        }
-      }
-    }
+    spread_direct<P, STILE>(acc, u1, u2, u3, p.w, o1, o2, o3, e1, e2, e3, K1, K2, K3, scale);
   };
   for (int round = 0; round * 64 < ncand; ++round) {
     const int nc = min(64, ncand - round * 64);
@@ -968,7 +980,7 @@ __global__ __launch_bounds__(SBS) void k_spread_tiles(const float4 *__restrict__
         if (c + h < nc && cpre[c + h] <= f) c += h;
       return cstart[c] + (f - cpre[c]);
     };
-    constexpr int CB = 4;
+    constexpr int CB = 2;
     for (int f0 = threadIdx.x; f0 < total; f0 += CB * SBS) {
       int ic[CB];
       float4 pc[CB];
@@ -1099,16 +1111,19 @@ __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__
 #pragma unroll
       for (int j2 = 0; j2 < P; ++j2) c[j3 * P + j2] = pot[b3 + (size_t)K1 * wrap_node(k2 + j2, K2)];
     }
+    // per z node the y sums A = sum wy v, B = sum dy v; phi = sum wz A, gy = sum wz B,
+    // gz = sum dz A (two multiply-adds per node instead of six)
 #pragma unroll
     for (int j3 = 0; j3 < P; ++j3) {
+      float A = 0.f, B = 0.f;
 #pragma unroll
       for (int j2 = 0; j2 < P; ++j2) {
-        const float v = c[j3 * P + j2];
-        const float w23 = wy[j2] * wz[j3];
-        phi = fmaf(w23, v, phi);
-        gy = fmaf(dy[j2] * wz[j3], v, gy);
-        gz = fmaf(wy[j2] * dz[j3], v, gz);
+        A = fmaf(wy[j2], c[j3 * P + j2], A);
+        B = fmaf(dy[j2], c[j3 * P + j2], B);
       }
+      phi = fmaf(wz[j3], A, phi);
+      gy = fmaf(wz[j3], B, gy);
+      gz = fmaf(dz[j3], A, gz);
     }
     // the lane's x weight (or its derivative, for gx) applies to its whole column of lines
     gx = bx * phi;
@@ -1144,7 +1159,7 @@ __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__
 // outputs. Tiles whose list overflowed (tn > tcap) are left to the workgroups past the tiles:
 // they walk all particles and gather those of overflowed tiles from global memory, one
 // particle per thread. Per particle the result is k_gather's up to FP32 rounding.
-constexpr int GCHUNK = 256;  // listed particles per LDS chunk (a tile holds ~N/tiles)
+constexpr int GCHUNK = 128;  // listed particles per LDS chunk (a tile holds ~N/tiles)
 template <int P, int STILE, bool ENERGY>
 __global__ __launch_bounds__(TPB) void k_gather_tiles(int N, const float4 *__restrict__ packed,
                                                       const float4 *__restrict__ tlist,
@@ -1159,6 +1174,8 @@ __global__ __launch_bounds__(TPB) void k_gather_tiles(int N, const float4 *__res
   __shared__ float4 pl[GCHUNK];
   __shared__ int pid[GCHUNK];
   __shared__ float res[4][GCHUNK];
+  __shared__ float gw[GCHUNK][3][P], gd[GCHUNK][3][P];  // weights and derivatives per dimension
+  __shared__ int go[GCHUNK][3];                         // tile-local base nodes
   if (Vglobal && blockIdx.x == 0 && threadIdx.x < 6)
     Vglobal[threadIdx.x] = (float)(acc[1 + threadIdx.x] + (threadIdx.x < 3 ? ubg : 0.0));
   const int tx = (K1 + STILE - 1) / STILE, ty = (K2 + STILE - 1) / STILE;
@@ -1210,25 +1227,33 @@ __global__ __launch_bounds__(TPB) void k_gather_tiles(int N, const float4 *__res
   const int r1 = min(STILE, K1 - a1) + P - 1, r2 = min(STILE, K2 - a2) + P - 1, r3 = min(STILE, K3 - a3) + P - 1;
   const float4 *own = tlist + (size_t)b * tcap;
   const int32_t *oid = tids + (size_t)b * tcap;
-  // phase 1: region node l (per dimension) = grid node a - (P-1) + l, wrapped below zero
+  // phase 1: region node l (per dimension) = grid node a - (P-1) + l, wrapped below zero; the
+  // thread's region index k advances by TPB per load, its (l1, l2, l3) digits incrementally
   constexpr int NLD = (R * R * R + TPB - 1) / TPB;
+  constexpr int D1 = TPB % R, D2 = (TPB / R) % R, D3 = TPB / (R * R);
   float v[NLD];
+  {
+    int l1 = threadIdx.x % R, l2 = (threadIdx.x / R) % R, l3 = threadIdx.x / (R * R);
 #pragma unroll
-  for (int j = 0; j < NLD; ++j) {
-    const int k = threadIdx.x + j * TPB;
-    const int l1 = k % R, l2 = (k / R) % R, l3 = k / (R * R);
-    v[j] = 0.f;
-    if (k < R * R * R && l1 < r1 && l2 < r2 && l3 < r3) {
-      int g1 = a1 - (P - 1) + l1; g1 += g1 < 0 ? K1 : 0;
-      int g2 = a2 - (P - 1) + l2; g2 += g2 < 0 ? K2 : 0;
-      int g3 = a3 - (P - 1) + l3; g3 += g3 < 0 ? K3 : 0;
-      v[j] = pot[((size_t)g3 * K2 + g2) * K1 + g1];
+    for (int j = 0; j < NLD; ++j) {
+      v[j] = 0.f;
+      if (l3 < r3 && l1 < r1 && l2 < r2) {
+        int g1 = a1 - (P - 1) + l1; g1 += g1 < 0 ? K1 : 0;
+        int g2 = a2 - (P - 1) + l2; g2 += g2 < 0 ? K2 : 0;
+        int g3 = a3 - (P - 1) + l3; g3 += g3 < 0 ? K3 : 0;
+        v[j] = pot[((size_t)g3 * K2 + g2) * K1 + g1];
+      }
+      l1 += D1;
+      if (l1 >= R) { l1 -= R; ++l2; }
+      l2 += D2;
+      if (l2 >= R) { l2 -= R; ++l3; }
+      l3 += D3;
     }
   }
-  static_assert(GCHUNK == TPB, "one list entry per thread and chunk");
+  static_assert(GCHUNK <= TPB, "at most one list entry per thread and chunk");
   float4 e0 = make_float4(0.f, 0.f, 0.f, 0.f);
   int i0 = -1;
-  if (threadIdx.x < n) {
+  if (threadIdx.x < min(n, GCHUNK)) {
     e0 = own[threadIdx.x];
     i0 = oid[threadIdx.x];
   }
@@ -1250,35 +1275,57 @@ __global__ __launch_bounds__(TPB) void k_gather_tiles(int N, const float4 *__res
         i0 = oid[c0 + threadIdx.x];
       }
     }
-    pl[threadIdx.x] = e0;
-    pid[threadIdx.x] = i0;
+    if (threadIdx.x < GCHUNK) {
+      pl[threadIdx.x] = e0;
+      pid[threadIdx.x] = i0;
+    }
     __syncthreads();
-    // phase 2 (t is uniform over a particle's LPP lanes, so the lane sums see only active lanes)
+    // weights: one thread per particle and dimension; base node floor(u) = k + P - 1 lies in
+    // the tile: local offsets 0 .. STILE-1
+    for (int e = threadIdx.x; e < 3 * nc; e += TPB) {
+      const int t = e / 3, d = e - 3 * t;
+      const float4 u = pl[t];
+      float w[P], dw[P];
+      const int a = d == 0 ? a1 : (d == 1 ? a2 : a3);
+      go[t][d] = min(max(bspline<P>(d == 0 ? u.x : (d == 1 ? u.y : u.z), w, dw) + (P - 1) - a, 0), STILE - 1);
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        gw[t][d][j] = w[j];
+        gd[t][d][j] = dw[j];
+      }
+    }
+    __syncthreads();
+    // phase 2 (t is uniform over a particle's LPP lanes, so the lane sums see only active lanes):
+    // per z node, the y sums of the lane's x column, A = sum wy v and B = sum dy v; then
+    // phi = sum wz A, gy = sum wz B, gz = sum dz A
     for (int t = threadIdx.x / LPP; t < nc; t += TPB / LPP) {
       const float4 e = pl[t];
       const int i = pid[t];
       const float qi = (i >= 0 && i < N) ? e.w : 0.0f;
       float phi = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
       if (qi != 0.0f) {
-        float wx[P], wy[P], wz[P], dx[P], dy[P], dz[P];
-        // base node floor(u) = k + P - 1 lies in the tile: local offsets 0 .. STILE-1
-        const int o1 = min(max(bspline<P>(e.x, wx, dx) + (P - 1) - a1, 0), STILE - 1);
-        const int o2 = min(max(bspline<P>(e.y, wy, dy) + (P - 1) - a2, 0), STILE - 1);
-        const int o3 = min(max(bspline<P>(e.z, wz, dz) + (P - 1) - a3, 0), STILE - 1);
+        float wy[P], wz[P], dy[P], dz[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          wy[j] = gw[t][1][j]; dy[j] = gd[t][1][j];
+          wz[j] = gw[t][2][j]; dz[j] = gd[t][2][j];
+        }
         const int jx = lane < P ? lane : 0;
-        const float ax = lane < P ? sel<P>(wx, jx) : 0.0f;
-        const float bx = lane < P ? sel<P>(dx, jx) : 0.0f;
-        const float *base = reg + (o3 * R + o2) * R + o1 + jx;
+        const float ax = lane < P ? gw[t][0][jx] : 0.0f;
+        const float bx = lane < P ? gd[t][0][jx] : 0.0f;
+        const float *base = reg + (go[t][2] * R + go[t][1]) * R + go[t][0] + jx;
 #pragma unroll
         for (int j3 = 0; j3 < P; ++j3) {
+          float A = 0.f, B = 0.f;
 #pragma unroll
           for (int j2 = 0; j2 < P; ++j2) {
             const float val = base[(j3 * R + j2) * R];
-            const float w23 = wy[j2] * wz[j3];
-            phi = fmaf(w23, val, phi);
-            gy = fmaf(dy[j2] * wz[j3], val, gy);
-            gz = fmaf(wy[j2] * dz[j3], val, gz);
+            A = fmaf(wy[j2], val, A);
+            B = fmaf(dy[j2], val, B);
           }
+          phi = fmaf(wz[j3], A, phi);
+          gy = fmaf(wz[j3], B, gy);
+          gz = fmaf(dz[j3], A, gz);
         }
         gx = bx * phi;
         phi *= ax;
@@ -1298,7 +1345,7 @@ __global__ __launch_bounds__(TPB) void k_gather_tiles(int N, const float4 *__res
     }
     __syncthreads();
     // phase 3: one listed particle per thread
-    const int i = pid[threadIdx.x];
+    const int i = threadIdx.x < GCHUNK ? pid[threadIdx.x] : -1;
     if (threadIdx.x < nc && i >= 0 && i < N) {
       const float qi = pl[threadIdx.x].w;
       if (set) {
