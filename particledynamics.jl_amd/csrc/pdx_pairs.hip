@@ -119,11 +119,28 @@ __global__ __launch_bounds__(TPB) void k_pair(const PairArgs a) {
       float4 pj[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) pj[u] = a.packed[jj[u]];
+      // plain differences; the compensated periodic form (equal to them whenever no wrap is
+      // needed) only in the waves where some lane's pair crosses the periodic seam
+      float ddx[U], ddy[U], ddz[U];
+      bool wrap = false;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const float dx = periodic_delta(pi.x, pj[u].x, hx, Lx);
-        const float dy = periodic_delta(pi.y, pj[u].y, hy, Ly);
-        const float dz = periodic_delta(pi.z, pj[u].z, hz, Lz);
+        ddx[u] = pi.x - pj[u].x;
+        ddy[u] = pi.y - pj[u].y;
+        ddz[u] = pi.z - pj[u].z;
+        wrap |= (fabsf(ddx[u]) > hx) | (fabsf(ddy[u]) > hy) | (fabsf(ddz[u]) > hz);
+      }
+      if (__ballot(wrap) != 0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          ddx[u] = periodic_delta(pi.x, pj[u].x, hx, Lx);
+          ddy[u] = periodic_delta(pi.y, pj[u].y, hy, Ly);
+          ddz[u] = periodic_delta(pi.z, pj[u].z, hz, Lz);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const float dx = ddx[u], dy = ddy[u], dz = ddz[u];
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         float f, uu;
         pair_terms<MODEL, COUL, ENERGY, NP>(a, pe, pf, qi, pj[u].w, r2, f, uu);
