@@ -84,6 +84,31 @@ __device__ __forceinline__ float periodic_delta(float xi, float xj, float halfL,
   return d;
 }
 
+// The periodic differences pi - pj[u] of U row entries: plain differences (periodic_delta
+// returns exactly these when |d| <= L/2), and periodic_delta itself only when some lane of the
+// wave has an entry across the periodic seam; the compensated form then runs with selects.
+template <int U>
+__device__ __forceinline__ void row_deltas(const float4 &pi, const float4 (&pj)[U], float hx, float hy, float hz,
+                                           float Lx, float Ly, float Lz, float (&dx)[U], float (&dy)[U],
+                                           float (&dz)[U]) {
+  bool wrap = false;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    dx[u] = pi.x - pj[u].x;
+    dy[u] = pi.y - pj[u].y;
+    dz[u] = pi.z - pj[u].z;
+    wrap |= (fabsf(dx[u]) > hx) | (fabsf(dy[u]) > hy) | (fabsf(dz[u]) > hz);
+  }
+  if (__ballot(wrap) != 0) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      dx[u] = periodic_delta(pi.x, pj[u].x, hx, Lx);
+      dy[u] = periodic_delta(pi.y, pj[u].y, hy, Ly);
+      dz[u] = periodic_delta(pi.z, pj[u].z, hz, Lz);
+    }
+  }
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);

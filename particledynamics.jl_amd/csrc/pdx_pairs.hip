@@ -119,25 +119,8 @@ __global__ __launch_bounds__(TPB) void k_pair(const PairArgs a) {
       float4 pj[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) pj[u] = a.packed[jj[u]];
-      // plain differences; the compensated periodic form (equal to them whenever no wrap is
-      // needed) only in the waves where some lane's pair crosses the periodic seam
       float ddx[U], ddy[U], ddz[U];
-      bool wrap = false;
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        ddx[u] = pi.x - pj[u].x;
-        ddy[u] = pi.y - pj[u].y;
-        ddz[u] = pi.z - pj[u].z;
-        wrap |= (fabsf(ddx[u]) > hx) | (fabsf(ddy[u]) > hy) | (fabsf(ddz[u]) > hz);
-      }
-      if (__ballot(wrap) != 0) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          ddx[u] = periodic_delta(pi.x, pj[u].x, hx, Lx);
-          ddy[u] = periodic_delta(pi.y, pj[u].y, hy, Ly);
-          ddz[u] = periodic_delta(pi.z, pj[u].z, hz, Lz);
-        }
-      }
+      row_deltas<U>(pi, pj, hx, hy, hz, Lx, Ly, Lz, ddx, ddy, ddz);
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const float dx = ddx[u], dy = ddy[u], dz = ddz[u];
@@ -565,11 +548,11 @@ __global__ __launch_bounds__(TPB) void k_pair_acc(const PairArgs a, int bonded) 
       float4 pj[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) pj[u] = a.packed[jj[u]];
+      float ddx[U], ddy[U], ddz[U];
+      row_deltas<U>(pi, pj, hx, hy, hz, Lx, Ly, Lz, ddx, ddy, ddz);
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const float dx = periodic_delta(pi.x, pj[u].x, hx, Lx);
-        const float dy = periodic_delta(pi.y, pj[u].y, hy, Ly);
-        const float dz = periodic_delta(pi.z, pj[u].z, hz, Lz);
+        const float dx = ddx[u], dy = ddy[u], dz = ddz[u];
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         float f, uu;
         pair_terms<MODEL, false, ENERGY, 1>(a, none, none, 0.f, 0.f, r2, f, uu);
