@@ -587,6 +587,11 @@ __device__ void sum_partials_any(const double *__restrict__ partial, int nblocks
   __syncthreads();
 }
 
+// The plane passes are the critical path of an evaluation whose pair kernel runs beside them:
+// their waves take the SIMD's issue slots first (wave priority 3 of 0..3; the pair kernel's
+// waves keep 0 and use what is left), so sharing a CU slows the transform less.
+__device__ __forceinline__ void mesh_chain_priority() { __builtin_amdgcn_s_setprio(3); }
+
 // Forward x (R2C, two real lines per complex line) and y transforms of z plane n3 of the
 // fixed-point charge grid into the half spectrum.
 template <int TP, bool BIG>
@@ -594,6 +599,7 @@ __global__ __launch_bounds__(PBS) void k_plane_fwd(const unsigned long long *__r
                                                    int K2, int K1h, int KM, FFTPlan p1, FFTPlan p2,
                                                    const float2 *__restrict__ tw, double inv_scale) {
   extern __shared__ float2 lds[];
+  mesh_chain_priority();
   constexpr int TS = TP + 1;  // padded row stride: column walks of the transposes hit distinct banks
   float2 *tw1 = lds, *tw2 = lds + K1, *b0 = tw2 + K2, *b1 = b0 + (size_t)KM * TS;
   for (int k = threadIdx.x; k < K1 + K2; k += PBS) lds[k] = tw[k];
@@ -637,6 +643,7 @@ template <int TZ, bool ENERGY, bool BIG>
 __global__ __launch_bounds__(PBS) void k_plane_z(float2 *spec, FFTPlan p3, const float2 *__restrict__ tw,
                                                  SolveArgs s, int tiles) {
   extern __shared__ float2 lds[];
+  mesh_chain_priority();
   const int K1h = s.K1h, K2 = s.K2, K3 = s.K3;
   constexpr int TS = TZ + 1;
   float2 *twl = lds, *b0 = lds + K3, *b1 = b0 + (size_t)TS * K3;
@@ -674,6 +681,7 @@ __global__ __launch_bounds__(PBS) void k_plane_inv(const float2 *__restrict__ sp
                                                    const float2 *__restrict__ tw, const double *psum, int npsum,
                                                    double *psum_out) {
   extern __shared__ float2 lds[];
+  mesh_chain_priority();
   if (psum && blockIdx.x == 0) sum_partials_any<7>(psum, npsum, psum_out);
   constexpr int TS = TP + 1;  // padded row stride: column walks of the transposes hit distinct banks
   float2 *tw1 = lds, *tw2 = lds + K1, *b0 = tw2 + K2, *b1 = b0 + (size_t)KM * TS;
