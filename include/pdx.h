@@ -250,8 +250,9 @@ int pdx_mesh_spread_cells(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t stream);
  * list's creation or a box change (the first builds, the second snapshots the build). Same
  * grid as pdx_mesh_spread. */
 int pdx_mesh_spread_snapshot(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t stream);
-/* Workgroup size of the owner-computes spreads: 1024 (the default: 32 particles at a time per
- * tile, the faster spread on its own) or 256 (more room for a kernel running beside it). */
+/* Workgroup size of the owner-computes spreads: 512 (the default: 16 particles at a time per
+ * tile), 1024 (32 at a time, twice the waves to launch) or 256 (more room for a kernel running
+ * beside it). */
 int pdx_mesh_set_spread_threads(pdx_mesh_t m, int32_t threads);
 /* Forward R2C 3D FFT, multiply by G, FP64 mesh energy and virial (with PDX_ENERGY), inverse
  * C2R FFT into the potential grid (mesh_transform!, Solver.jl:305-341; _solve_kernel!,

@@ -2160,10 +2160,14 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
   const int32_t *ids = snapshot ? nl->snap_ids.p : nl->particle_ids_sorted.p;
   const int64_t *guard = snapshot ? nl->snap_state.p : nl->state.p;  // word 2 nonzero: no usable cells
   hipStream_t st = as_stream(stream);
-  // workgroup size (pdx_mesh_set_spread_threads): 1024 (32 particles at a time per tile) is the
-  // faster spread on its own; 256 leaves more room beside a concurrent pair kernel
+  // workgroup size (pdx_mesh_set_spread_threads): 512 (16 particles at a time per tile; half
+  // the waves of 1024 to launch, which the evaluation measures faster), 1024, or 256 (more room
+  // beside a concurrent pair kernel)
   int sbs = m->spread_threads;
-  if (const char *e = std::getenv("PDX_SPREAD_BS")) sbs = std::atoi(e) == 256 ? 256 : 1024;
+  if (const char *e = std::getenv("PDX_SPREAD_BS")) {
+    const int v = std::atoi(e);
+    sbs = (v == 256 || v == 1024) ? v : 512;
+  }
   int tcap = (int)std::min<size_t>(m->tile_entries / tiles, INT32_MAX);
   if (const char *e = std::getenv("PDX_TILE_CAP")) tcap = std::max(0, std::min(tcap, std::atoi(e)));  // tests: overflow
   m->tiles_nl = nl;
@@ -2178,9 +2182,13 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
 #define PDX_TSPREAD(PP)                                                                                        \
   case PP:                                                                                                     \
     if (small) {                                                                                               \
-      if (sbs == 1024) { PDX_TSPREAD_L(PP, 8, 1024) } else { PDX_TSPREAD_L(PP, 8, 256) }                       \
+      if (sbs == 1024) { PDX_TSPREAD_L(PP, 8, 1024) }                                                          \
+      else if (sbs == 512) { PDX_TSPREAD_L(PP, 8, 512) }                                                       \
+      else { PDX_TSPREAD_L(PP, 8, 256) }                                                                       \
     } else {                                                                                                   \
-      if (sbs == 1024) { PDX_TSPREAD_L(PP, 16, 1024) } else { PDX_TSPREAD_L(PP, 16, 256) }                     \
+      if (sbs == 1024) { PDX_TSPREAD_L(PP, 16, 1024) }                                                         \
+      else if (sbs == 512) { PDX_TSPREAD_L(PP, 16, 512) }                                                      \
+      else { PDX_TSPREAD_L(PP, 16, 256) }                                                                      \
     }                                                                                                          \
     break;
   switch (m->P) {
@@ -2201,7 +2209,8 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
 
 extern "C" int pdx_mesh_set_spread_threads(pdx_mesh_t m, int32_t threads) {
   if (!m) return fail(PDX_ERR_ARGUMENT, "null mesh");
-  if (threads != 256 && threads != 1024) return fail(PDX_ERR_ARGUMENT, "spread threads must be 256 or 1024");
+  if (threads != 256 && threads != 512 && threads != 1024)
+    return fail(PDX_ERR_ARGUMENT, "spread threads must be 256, 512 or 1024");
   m->spread_threads = threads;
   return PDX_OK;
 }

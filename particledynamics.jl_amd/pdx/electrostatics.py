@@ -185,7 +185,7 @@ class ElectrostaticsSolver:
         L.check(L.lib.pdx_mesh_spread_cells(self._h, nl.handle, L.stream_handle(stream)))
 
     def set_spread_threads(self, threads):
-        """Workgroup size of the owner-computes spreads (1024 or 256)."""
+        """Workgroup size of the owner-computes spreads (512, 1024 or 256)."""
         L.check(L.lib.pdx_mesh_set_spread_threads(self._h, int(threads)))
 
     def spread_snapshot(self, nl, stream=None):

@@ -69,7 +69,7 @@ class ForceEngine:
         self.mesh_stream = torch.cuda.Stream() if self.solver is not None else None
         if self.solver is not None:
             # a spread that shares the CUs with the pair kernel runs better in smaller workgroups
-            self.solver.set_spread_threads(256 if overlap_with == "spread" else 1024)
+            self.solver.set_spread_threads(256 if overlap_with == "spread" else 512)
         # "snapshot": the owner-computes spread over the cell snapshot of the last completed
         # build, forked right after the list check so that it runs beside the rebuild; "cells":
         # the same spread over the current cells, after the whole list update; "atomic": the

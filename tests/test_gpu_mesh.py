@@ -153,6 +153,27 @@ def test_gather_cells_equals_particle_gather(cuda, P, mesh, cap, monkeypatch):
         s.gather_cells(nl, a1[0], a1[1], a1[2], None, None, energy=False)
 
 
+@pytest.mark.parametrize("mesh", [(112, 128, 128), (20, 22, 24)])
+def test_spread_workgroup_sizes_agree(cuda, mesh):
+    """The owner-computes spread gives the same fixed-point grid bit for bit with 256-, 512- and
+    1024-thread tiles (16^3 and 8^3 tiles), and refuses other sizes."""
+    import torch
+    N = 3000
+    R, q = systems.random_cloud(N, L_ES, seed=11)
+    s = _solver(mesh, P=5, N=N, q=q)
+    X, Y, Z, Q = dev(R[:, 0]), dev(R[:, 1]), dev(R[:, 2]), dev(q)
+    nl = pdx.DeviceNeighborWorkspace(N, L_ES, 4.0, 0.3, 512)
+    nl.update(X, Y, Z, Q)
+    grids = []
+    for threads in (256, 512, 1024):
+        s.set_spread_threads(threads)
+        s.spread_cells(nl)
+        grids.append(s.fixed_grid().clone())
+    assert torch.equal(grids[0], grids[1]) and torch.equal(grids[1], grids[2])
+    with pytest.raises(pdx.ArgumentError):
+        s.set_spread_threads(128)
+
+
 @pytest.mark.parametrize("mesh", [(120, 120, 120), (60, 60, 60), (48, 45, 50), (64, 128, 30)])
 def test_plane_transforms_equal_line_passes(cuda, mesh, monkeypatch):
     """The plane transforms (x and y of a z plane in one workgroup, the z solve of a y plane in
