@@ -254,6 +254,12 @@ int pdx_mesh_spread_snapshot(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t stream);
  * tile), 1024 (32 at a time, twice the waves to launch) or 256 (more room for a kernel running
  * beside it). */
 int pdx_mesh_set_spread_threads(pdx_mesh_t m, int32_t threads);
+/* Workgroup size of the plane passes of pdx_mesh_solve (meshes whose planes fit LDS): 1024 (the
+ * default, the faster transform on its own) or 512 (smaller workgroups that share the CUs better
+ * with a kernel running beside the transform, such as the pair forces of the same evaluation).
+ * The same potential bit for bit; the FP64 energy and virial to FP64 rounding (their partial
+ * sums are grouped by workgroup). No counterpart in the reference (a launch shape). */
+int pdx_mesh_set_transform_threads(pdx_mesh_t m, int32_t threads);
 /* Forward R2C 3D FFT, multiply by G, FP64 mesh energy and virial (with PDX_ENERGY), inverse
  * C2R FFT into the potential grid (mesh_transform!, Solver.jl:305-341; _solve_kernel!,
  * MeshKernels.jl:57-81). */

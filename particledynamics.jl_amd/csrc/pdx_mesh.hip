@@ -558,7 +558,7 @@ __global__ __launch_bounds__(TPB) void k_fft_z(float2 *spec, FFTPlan pl, const f
 // lines of a plane); the LDS holds two TP x KM ping-pong buffers and the twiddles of the axes.
 // The per-line arithmetic (stages, twiddles, R2C / C2R packing) is that of the line passes, so
 // the potential grid is the same bit for bit.
-constexpr int PBS = 1024;
+constexpr int PBS = 1024;  // the default workgroup size of the plane passes (512 selectable)
 
 // Sum of NV values per block over nblocks partials in a fixed order, for any block size (a
 // multiple of 64, at most 1024); the whole block calls it.
@@ -594,22 +594,22 @@ __device__ __forceinline__ void mesh_chain_priority() { __builtin_amdgcn_s_setpr
 
 // Forward x (R2C, two real lines per complex line) and y transforms of z plane n3 of the
 // fixed-point charge grid into the half spectrum.
-template <int TP, bool BIG>
-__global__ __launch_bounds__(PBS) void k_plane_fwd(const unsigned long long *__restrict__ grid, float2 *spec, int K1,
+template <int TP, bool BIG, int BS>
+__global__ __launch_bounds__(BS) void k_plane_fwd(const unsigned long long *__restrict__ grid, float2 *spec, int K1,
                                                    int K2, int K1h, int KM, FFTPlan p1, FFTPlan p2,
                                                    const float2 *__restrict__ tw, double inv_scale) {
   extern __shared__ float2 lds[];
   mesh_chain_priority();
   constexpr int TS = TP + 1;  // padded row stride: column walks of the transposes hit distinct banks
   float2 *tw1 = lds, *tw2 = lds + K1, *b0 = tw2 + K2, *b1 = b0 + (size_t)KM * TS;
-  for (int k = threadIdx.x; k < K1 + K2; k += PBS) lds[k] = tw[k];
+  for (int k = threadIdx.x; k < K1 + K2; k += BS) lds[k] = tw[k];
   const int n3 = blockIdx.x;
   const unsigned long long *g = grid + (size_t)n3 * K1 * K2;
   const int n = K1 * K2;
   if (K2 & 1)  // the odd last real line has no partner: its imaginary slot stays zero
-    for (int n1 = threadIdx.x; n1 < K1; n1 += PBS) b0[n1 * TS + (K2 >> 1)] = make_float2(0.f, 0.f);
+    for (int n1 = threadIdx.x; n1 < K1; n1 += BS) b0[n1 * TS + (K2 >> 1)] = make_float2(0.f, 0.f);
   __syncthreads();
-  staged<16, float, PBS>(n, [&](int e) { return (float)((double)(long long)g[e] * inv_scale); },
+  staged<16, float, BS>(n, [&](int e) { return (float)((double)(long long)g[e] * inv_scale); },
                          [&](int e, float v) {
                            const int n2 = e / K1, n1 = e - n2 * K1;
                            reinterpret_cast<float *>(&b0[n1 * TS + (n2 >> 1)])[n2 & 1] = v;
@@ -619,7 +619,7 @@ __global__ __launch_bounds__(PBS) void k_plane_fwd(const unsigned long long *__r
   float2 *o = r == b0 ? b1 : b0;
   // A[m] = (Z[m] + conj Z[K-m])/2 (line 2c), B[m] = (Z[m] - conj Z[K-m])/(2i) (line 2c+1), stored
   // as the y lines: element n2, line m1
-  for (int e = threadIdx.x; e < K2 * K1h; e += PBS) {
+  for (int e = threadIdx.x; e < K2 * K1h; e += BS) {
     const int n2 = e / K1h, m = e - n2 * K1h;
     const int c = n2 >> 1;
     const float2 z = r[m * TS + c];
@@ -630,7 +630,7 @@ __global__ __launch_bounds__(PBS) void k_plane_fwd(const unsigned long long *__r
   __syncthreads();
   r = lds_fft<TP, BIG, TS>(o, r, p2, tw2, -1.0f);
   float2 *out = spec + (size_t)n3 * K2 * K1h;
-  for (int e = threadIdx.x; e < K2 * K1h; e += PBS) {
+  for (int e = threadIdx.x; e < K2 * K1h; e += BS) {
     const int m2 = e / K1h, m1 = e - m2 * K1h;
     out[e] = r[m2 * TS + m1];
   }
@@ -639,21 +639,21 @@ __global__ __launch_bounds__(PBS) void k_plane_fwd(const unsigned long long *__r
 // Forward z transform, influence multiply with the FP64 energy / virial partials, inverse z
 // transform of TZ lines m1 = m10 .. m10 + TZ - 1 of the (m1, m3) plane m2: `tiles` workgroups per
 // plane, so that the launch spans every CU.
-template <int TZ, bool ENERGY, bool BIG>
-__global__ __launch_bounds__(PBS) void k_plane_z(float2 *spec, FFTPlan p3, const float2 *__restrict__ tw,
+template <int TZ, bool ENERGY, bool BIG, int BS>
+__global__ __launch_bounds__(BS) void k_plane_z(float2 *spec, FFTPlan p3, const float2 *__restrict__ tw,
                                                  SolveArgs s, int tiles) {
   extern __shared__ float2 lds[];
   mesh_chain_priority();
   const int K1h = s.K1h, K2 = s.K2, K3 = s.K3;
   constexpr int TS = TZ + 1;
   float2 *twl = lds, *b0 = lds + K3, *b1 = b0 + (size_t)TS * K3;
-  for (int k = threadIdx.x; k < K3; k += PBS) twl[k] = tw[k];
+  for (int k = threadIdx.x; k < K3; k += BS) twl[k] = tw[k];
   const int m2 = blockIdx.x / tiles, m10 = (blockIdx.x % tiles) * TZ;
   const int nl = min(TZ, K1h - m10);
   const size_t plane = (size_t)K1h * K2;
   const size_t base = (size_t)m2 * K1h + m10;
   const int n = K3 * nl;
-  staged<8, float2, PBS>(n, [&](int e) {
+  staged<8, float2, BS>(n, [&](int e) {
     const int m3 = e / nl, l = e - m3 * nl;
     return spec[base + (size_t)m3 * plane + l];
   }, [&](int e, float2 v) {
@@ -666,7 +666,7 @@ __global__ __launch_bounds__(PBS) void k_plane_z(float2 *spec, FFTPlan p3, const
   solve_lines<TZ, ENERGY, TS>(r, m10, m2, s, acc);
   __syncthreads();
   r = lds_fft<TZ, BIG, TS>(r, r == b0 ? b1 : b0, p3, twl, +1.0f);
-  for (int e = threadIdx.x; e < n; e += PBS) {
+  for (int e = threadIdx.x; e < n; e += BS) {
     const int m3 = e / nl, l = e - m3 * nl;
     spec[base + (size_t)m3 * plane + l] = r[m3 * TS + l];
   }
@@ -675,8 +675,8 @@ __global__ __launch_bounds__(PBS) void k_plane_z(float2 *spec, FFTPlan p3, const
 
 // Inverse y and x (C2R) transforms of z plane m3 into the potential grid; block 0 first adds
 // the z pass's energy / virial partials.
-template <int TP, bool BIG>
-__global__ __launch_bounds__(PBS) void k_plane_inv(const float2 *__restrict__ spec, float *pot, int K1, int K2,
+template <int TP, bool BIG, int BS>
+__global__ __launch_bounds__(BS) void k_plane_inv(const float2 *__restrict__ spec, float *pot, int K1, int K2,
                                                    int K1h, int KM, FFTPlan p1, FFTPlan p2,
                                                    const float2 *__restrict__ tw, const double *psum, int npsum,
                                                    double *psum_out) {
@@ -685,10 +685,10 @@ __global__ __launch_bounds__(PBS) void k_plane_inv(const float2 *__restrict__ sp
   if (psum && blockIdx.x == 0) sum_partials_any<7>(psum, npsum, psum_out);
   constexpr int TS = TP + 1;  // padded row stride: column walks of the transposes hit distinct banks
   float2 *tw1 = lds, *tw2 = lds + K1, *b0 = tw2 + K2, *b1 = b0 + (size_t)KM * TS;
-  for (int k = threadIdx.x; k < K1 + K2; k += PBS) lds[k] = tw[k];
+  for (int k = threadIdx.x; k < K1 + K2; k += BS) lds[k] = tw[k];
   const int m3 = blockIdx.x;
   const float2 *in = spec + (size_t)m3 * K2 * K1h;
-  staged<8, float2, PBS>(K2 * K1h, [&](int e) { return in[e]; }, [&](int e, float2 v) {
+  staged<8, float2, BS>(K2 * K1h, [&](int e) { return in[e]; }, [&](int e, float2 v) {
     const int m2 = e / K1h, m1 = e - m2 * K1h;
     b0[m2 * TS + m1] = v;
   });
@@ -698,7 +698,7 @@ __global__ __launch_bounds__(PBS) void k_plane_inv(const float2 *__restrict__ sp
   // x lines: Z = A + i B with A, B the half-spectrum lines n2 = 2c, 2c+1 completed by Hermitian
   // symmetry (imaginary parts of the DC and Nyquist terms dropped)
   const int npair = (K2 + 1) >> 1;
-  for (int e = threadIdx.x; e < npair * K1; e += PBS) {
+  for (int e = threadIdx.x; e < npair * K1; e += BS) {
     const int c = e / K1, m = e - c * K1;
     const int la = 2 * c, lb = la + 1;
     const bool conj = m >= K1h;
@@ -712,7 +712,7 @@ __global__ __launch_bounds__(PBS) void k_plane_inv(const float2 *__restrict__ sp
   __syncthreads();
   r = lds_fft<TP, BIG, TS>(o, r, p1, tw1, +1.0f);
   float *out = pot + (size_t)m3 * K1 * K2;
-  for (int e = threadIdx.x; e < K1 * K2; e += PBS) {
+  for (int e = threadIdx.x; e < K1 * K2; e += BS) {
     const int n2 = e / K1, n1 = e - n2 * K1;
     const float2 z = r[n1 * TS + (n2 >> 1)];
     out[e] = (n2 & 1) ? z.y : z.x;
@@ -1852,50 +1852,62 @@ size_t plane_lds_z(pdx_mesh_t m) { return ((size_t)m->K3 + (size_t)2 * m->K3 * (
 int plane_attributes(pdx_mesh_t m, const Plans &pl) {
   const size_t lxy = plane_lds_xy(m), lz = plane_lds_z(m);
   bool big = pl.p1.big || pl.p2.big;
-#define PDX_PA_XY(TP, BB)                                                                                     \
-  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_fwd<TP, BB>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
+#define PDX_PA_XY_BS(TP, BB, BS)                                                                              \
+  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_fwd<TP, BB, BS>, hipFuncAttributeMaxDynamicSharedMemorySize,\
                               (int)lxy));                                                                     \
-  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_inv<TP, BB>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
+  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_inv<TP, BB, BS>, hipFuncAttributeMaxDynamicSharedMemorySize,\
                               (int)lxy));
+#define PDX_PA_XY(TP, BB) PDX_PA_XY_BS(TP, BB, 1024) PDX_PA_XY_BS(TP, BB, 512)
   PDX_PLANE_SWITCH(PDX_PA_XY)
 #undef PDX_PA_XY
+#undef PDX_PA_XY_BS
   big = pl.p3.big;
-  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_z<PLANE_TZ, true, false>,
+  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_z<PLANE_TZ, true, false, 1024>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lz));
-  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_z<PLANE_TZ, false, false>,
+  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_z<PLANE_TZ, false, false, 1024>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lz));
+  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_z<PLANE_TZ, true, false, 512>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lz));
+  PDX_HIP(hipFuncSetAttribute((const void *)k_plane_z<PLANE_TZ, false, false, 512>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lz));
   return PDX_OK;
 }
 
-// The SPME / P3M-AD solve as three plane launches.
-int enqueue_plane_solve(pdx_mesh_t m, const Plans &pl, bool energy, hipStream_t st) {
+// The SPME / P3M-AD solve as three plane launches, in workgroups of BS threads.
+template <int BS>
+int enqueue_plane_solve_bs(pdx_mesh_t m, const Plans &pl, bool energy, hipStream_t st) {
   const int KM = std::max(m->K1, m->K2);
   const size_t lxy = plane_lds_xy(m), lz = plane_lds_z(m);
   const float2 *tw1 = m->tw.p, *tw3 = m->tw.p + m->K1 + m->K2;
   bool big = pl.p1.big || pl.p2.big;
 #define PDX_PF(TP, BB)                                                                                        \
-  hipLaunchKernelGGL((k_plane_fwd<TP, BB>), dim3(m->K3), dim3(PBS), lxy, st, (const unsigned long long *)m->grid64.p, \
+  hipLaunchKernelGGL((k_plane_fwd<TP, BB, BS>), dim3(m->K3), dim3(BS), lxy, st, (const unsigned long long *)m->grid64.p, \
                      m->spec_c.p, m->K1, m->K2, m->K1h, KM, pl.p1, pl.p2, tw1, m->fixed_inv);
   PDX_PLANE_SWITCH(PDX_PF)
 #undef PDX_PF
   SolveArgs sa = solve_args(m);
   const int zt = plane_z_tiles(m);
   if (energy)
-    hipLaunchKernelGGL((k_plane_z<PLANE_TZ, true, false>), dim3(m->K2 * zt), dim3(PBS), lz, st, m->spec_c.p, pl.p3,
+    hipLaunchKernelGGL((k_plane_z<PLANE_TZ, true, false, BS>), dim3(m->K2 * zt), dim3(BS), lz, st, m->spec_c.p, pl.p3,
                        tw3, sa, zt);
   else
-    hipLaunchKernelGGL((k_plane_z<PLANE_TZ, false, false>), dim3(m->K2 * zt), dim3(PBS), lz, st, m->spec_c.p, pl.p3,
+    hipLaunchKernelGGL((k_plane_z<PLANE_TZ, false, false, BS>), dim3(m->K2 * zt), dim3(BS), lz, st, m->spec_c.p, pl.p3,
                        tw3, sa, zt);
   big = pl.p1.big || pl.p2.big;
   const double *ps = energy ? m->partial.p : nullptr;
   double *po = energy ? m->acc.p : nullptr;
 #define PDX_PI(TP, BB)                                                                                        \
-  hipLaunchKernelGGL((k_plane_inv<TP, BB>), dim3(m->K3), dim3(PBS), lxy, st, (const float2 *)m->spec_c.p, m->pot.p, \
+  hipLaunchKernelGGL((k_plane_inv<TP, BB, BS>), dim3(m->K3), dim3(BS), lxy, st, (const float2 *)m->spec_c.p, m->pot.p, \
                      m->K1, m->K2, m->K1h, KM, pl.p1, pl.p2, tw1, ps, m->nmode_blocks, po);
   PDX_PLANE_SWITCH(PDX_PI)
 #undef PDX_PI
   PDX_LAUNCH_CHECK();
   return PDX_OK;
+}
+
+int enqueue_plane_solve(pdx_mesh_t m, const Plans &pl, bool energy, hipStream_t st) {
+  if (m->transform_threads == 512) return enqueue_plane_solve_bs<512>(m, pl, energy, st);
+  return enqueue_plane_solve_bs<PBS>(m, pl, energy, st);
 }
 
 #define PDX_SPREAD_CASE(PP)                                                                                    \
@@ -2212,6 +2224,13 @@ extern "C" int pdx_mesh_set_spread_threads(pdx_mesh_t m, int32_t threads) {
   if (threads != 256 && threads != 512 && threads != 1024)
     return fail(PDX_ERR_ARGUMENT, "spread threads must be 256, 512 or 1024");
   m->spread_threads = threads;
+  return PDX_OK;
+}
+
+extern "C" int pdx_mesh_set_transform_threads(pdx_mesh_t m, int32_t threads) {
+  if (!m) return fail(PDX_ERR_ARGUMENT, "null mesh");
+  if (threads != 512 && threads != 1024) return fail(PDX_ERR_ARGUMENT, "transform threads must be 512 or 1024");
+  m->transform_threads = threads;
   return PDX_OK;
 }
 

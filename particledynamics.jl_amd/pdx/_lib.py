@@ -111,6 +111,7 @@ def _load():
         "pdx_mesh_spread": ([vp, vp, vp, vp, vp, vp, vp], C.c_int),
         "pdx_mesh_spread_cells": ([vp, vp, vp], C.c_int),
         "pdx_mesh_spread_snapshot": ([vp, vp, vp], C.c_int),
+        "pdx_mesh_set_transform_threads": ([vp, i32], C.c_int),
         "pdx_mesh_set_spread_threads": ([vp, i32], C.c_int),
         "pdx_nl_check": ([vp, vp, vp, vp, vp, vp], C.c_int),
         "pdx_nl_build": ([vp, vp], C.c_int),

@@ -188,6 +188,11 @@ class ElectrostaticsSolver:
         """Workgroup size of the owner-computes spreads (512, 1024 or 256)."""
         L.check(L.lib.pdx_mesh_set_spread_threads(self._h, int(threads)))
 
+    def set_transform_threads(self, threads):
+        """Workgroup size of the plane passes of the solve (1024 or 512): the same potential,
+        the FP64 energy and virial to FP64 rounding."""
+        L.check(L.lib.pdx_mesh_set_transform_threads(self._h, int(threads)))
+
     def spread_snapshot(self, nl, stream=None):
         """Owner-computes spread from ``nl``'s cell snapshot (may run beside ``nl.build``)."""
         L.check(L.lib.pdx_mesh_spread_snapshot(self._h, nl.handle, L.stream_handle(stream)))

@@ -70,6 +70,9 @@ class ForceEngine:
         if self.solver is not None:
             # a spread that shares the CUs with the pair kernel runs better in smaller workgroups
             self.solver.set_spread_threads(256 if overlap_with == "spread" else 512)
+            # the pair kernel runs beside the transforms: they take 512-thread workgroups, which
+            # share the CUs with it better than 1024-thread ones (slower alone, faster together)
+            self.solver.set_transform_threads(512)
         # "snapshot": the owner-computes spread over the cell snapshot of the last completed
         # build, forked right after the list check so that it runs beside the rebuild; "cells":
         # the same spread over the current cells, after the whole list update; "atomic": the

@@ -203,6 +203,32 @@ def test_plane_transforms_equal_line_passes(cuda, mesh, monkeypatch):
     assert np.allclose(t1["virial"], t0["virial"], rtol=1e-6, atol=1e-6 * abs(t0["mesh_energy"]))
 
 
+@pytest.mark.parametrize("mesh", [(120, 120, 120), (48, 45, 50), (64, 128, 30)])
+def test_plane_workgroup_sizes_agree(cuda, mesh):
+    """The plane passes give the same potential bit for bit in 512- and 1024-thread workgroups
+    (every line runs the same butterflies), and the FP64 energy and virial to FP64 rounding (the
+    threads' partial sums group the modes by workgroup size); other sizes are refused."""
+    import torch
+    N = 2000
+    box = (30.0, 31.0, 32.0)
+    R, q = systems.random_cloud(N, box, seed=4)
+    X, Y, Z, Q = dev(R[:, 0]), dev(R[:, 1]), dev(R[:, 2]), dev(q)
+    s = _solver(mesh, P=5, box=box, N=N, q=q, alpha=0.35, rc=9.0)
+    assert s.plane_lines > 0
+    res = []
+    for threads in (1024, 512):
+        s.set_transform_threads(threads)
+        s.spread(X, Y, Z, Q)
+        s.solve(energy=True)
+        res.append((s.potential_grid().clone(), s.totals()))
+    assert torch.equal(res[0][0], res[1][0])
+    e0, e1 = res[0][1]["mesh_energy"], res[1][1]["mesh_energy"]
+    assert abs(e1 - e0) <= 1e-12 * abs(e0)
+    assert np.allclose(res[1][1]["virial"], res[0][1]["virial"], rtol=1e-12, atol=1e-12 * abs(e0))
+    with pytest.raises(pdx.ArgumentError):
+        s.set_transform_threads(256)
+
+
 def test_spread_snapshot_equals_atomic_spread(cuda):
     """The spread over the cell snapshot of the previous build gives the atomic spread's grid bit
     for bit, also on the step whose check triggers a rebuild (the snapshot is then a build
