@@ -881,7 +881,7 @@ __device__ __forceinline__ void spread_direct(unsigned long long *acc, float u1,
 }
 
 constexpr int SLIST = 768;   // particles reaching one block staged in LDS (a few hundred in practice)
-constexpr int SCHUNK = 64;   // staged particles whose weights are computed together
+constexpr int SCHUNK = 128;  // staged particles whose weights are computed together
 template <int P, int STILE, int SBS>
 // (1024-thread workgroups of orders up to 5 are held to 64 VGPRs, 8 waves per SIMD: two of them,
 // 32 waves, then fit on a CU; higher orders would spill)
