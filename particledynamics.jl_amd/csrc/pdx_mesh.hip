@@ -1168,8 +1168,9 @@ __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__
 // they walk all particles and gather those of overflowed tiles from global memory, one
 // particle per thread. Per particle the result is k_gather's up to FP32 rounding.
 constexpr int GCHUNK = 128;  // listed particles per LDS chunk (a tile holds ~N/tiles)
+constexpr int GTB = 512;     // threads per tile workgroup (measured faster than 256 and 128)
 template <int P, int STILE, bool ENERGY>
-__global__ __launch_bounds__(TPB) void k_gather_tiles(int N, const float4 *__restrict__ packed,
+__global__ __launch_bounds__(GTB) void k_gather_tiles(int N, const float4 *__restrict__ packed,
                                                       const float4 *__restrict__ tlist,
                                                       const int32_t *__restrict__ tids,
                                                       const int32_t *__restrict__ tn, int tcap, int tiles,
@@ -1189,9 +1190,9 @@ __global__ __launch_bounds__(TPB) void k_gather_tiles(int N, const float4 *__res
   const int tx = (K1 + STILE - 1) / STILE, ty = (K2 + STILE - 1) / STILE;
   const float s1 = (float)K1 / Lx, s2 = (float)K2 / Ly, s3 = (float)K3 / Lz;
   if ((int)blockIdx.x >= tiles) {
-    const int stride = (gridDim.x - tiles) * TPB;
+    const int stride = (gridDim.x - tiles) * GTB;
     const size_t KK = (size_t)K1 * K2;
-    for (int i = (blockIdx.x - tiles) * TPB + threadIdx.x; i < N; i += stride) {
+    for (int i = (blockIdx.x - tiles) * GTB + threadIdx.x; i < N; i += stride) {
       const float4 p = packed[i];
       const float u1 = grid_coordinate(p.x, Lx, K1), u2 = grid_coordinate(p.y, Ly, K2),
                   u3 = grid_coordinate(p.z, Lz, K3);
@@ -1236,9 +1237,9 @@ __global__ __launch_bounds__(TPB) void k_gather_tiles(int N, const float4 *__res
   const float4 *own = tlist + (size_t)b * tcap;
   const int32_t *oid = tids + (size_t)b * tcap;
   // phase 1: region node l (per dimension) = grid node a - (P-1) + l, wrapped below zero; the
-  // thread's region index k advances by TPB per load, its (l1, l2, l3) digits incrementally
-  constexpr int NLD = (R * R * R + TPB - 1) / TPB;
-  constexpr int D1 = TPB % R, D2 = (TPB / R) % R, D3 = TPB / (R * R);
+  // thread's region index k advances by GTB per load, its (l1, l2, l3) digits incrementally
+  constexpr int NLD = (R * R * R + GTB - 1) / GTB;
+  constexpr int D1 = GTB % R, D2 = (GTB / R) % R, D3 = GTB / (R * R);
   float v[NLD];
   {
     int l1 = threadIdx.x % R, l2 = (threadIdx.x / R) % R, l3 = threadIdx.x / (R * R);
@@ -1258,7 +1259,7 @@ __global__ __launch_bounds__(TPB) void k_gather_tiles(int N, const float4 *__res
       l3 += D3;
     }
   }
-  static_assert(GCHUNK <= TPB, "at most one list entry per thread and chunk");
+  static_assert(GCHUNK <= GTB, "at most one list entry per thread and chunk");
   float4 e0 = make_float4(0.f, 0.f, 0.f, 0.f);
   int i0 = -1;
   if (threadIdx.x < min(n, GCHUNK)) {
@@ -1267,7 +1268,7 @@ __global__ __launch_bounds__(TPB) void k_gather_tiles(int N, const float4 *__res
   }
 #pragma unroll
   for (int j = 0; j < NLD; ++j) {
-    const int k = threadIdx.x + j * TPB;
+    const int k = threadIdx.x + j * GTB;
     if (k < R * R * R) reg[k] = v[j];
   }
   constexpr int LPP = gather_lanes(P);
@@ -1290,7 +1291,7 @@ __global__ __launch_bounds__(TPB) void k_gather_tiles(int N, const float4 *__res
     __syncthreads();
     // weights: one thread per particle and dimension; base node floor(u) = k + P - 1 lies in
     // the tile: local offsets 0 .. STILE-1
-    for (int e = threadIdx.x; e < 3 * nc; e += TPB) {
+    for (int e = threadIdx.x; e < 3 * nc; e += GTB) {
       const int t = e / 3, d = e - 3 * t;
       const float4 u = pl[t];
       float w[P], dw[P];
@@ -1306,7 +1307,7 @@ __global__ __launch_bounds__(TPB) void k_gather_tiles(int N, const float4 *__res
     // phase 2 (t is uniform over a particle's LPP lanes, so the lane sums see only active lanes):
     // per z node, the y sums of the lane's x column, A = sum wy v and B = sum dy v; then
     // phi = sum wz A, gy = sum wz B, gz = sum dz A
-    for (int t = threadIdx.x / LPP; t < nc; t += TPB / LPP) {
+    for (int t = threadIdx.x / LPP; t < nc; t += GTB / LPP) {
       const float4 e = pl[t];
       const int i = pid[t];
       const float qi = (i >= 0 && i < N) ? e.w : 0.0f;
@@ -2326,10 +2327,10 @@ extern "C" int pdx_mesh_gather_cells(pdx_mesh_t m, pdx_nl_t nl, float *fx, float
   if (!energy) Vglobal = nullptr;
   hipStream_t st = as_stream(stream);
   // workgroups past the tiles: the overflow walk, four particles per thread
-  const int extra = (int)std::max<int64_t>(1, std::min<int64_t>(256, ((int64_t)N + 4 * TPB - 1) / (4 * TPB)));
+  const int extra = (int)std::max<int64_t>(1, std::min<int64_t>(256, ((int64_t)N + 4 * GTB - 1) / (4 * GTB)));
   const dim3 grid(m->tiles_count + extra);
 #define PDX_GT_L(PP, TT, EE)                                                                                   \
-  hipLaunchKernelGGL((k_gather_tiles<PP, TT, EE>), grid, dim3(TPB), 0, st, N, nl->packed.p, m->tile_list.p,    \
+  hipLaunchKernelGGL((k_gather_tiles<PP, TT, EE>), grid, dim3(GTB), 0, st, N, nl->packed.p, m->tile_list.p,    \
                      m->tile_ids.p, m->tile_n.p, m->tiles_cap, m->tiles_count, m->pot.p, m->K1, m->K2, m->K3, L[0], L[1],     \
                      L[2], fx, fy, fz, Epot, m->bg, set, Vglobal, m->acc.p, m->ubg);
 #define PDX_GT(PP)                                                                                             \
