@@ -8,12 +8,17 @@
  *   or_neighbor_rows  test/neighbor_reference.jl:6-16        (rows by brute force)
  *   or_pair_direct    test/gpu_force_reference.jl:38-80,      (direct FP64 pair sums)
  *                     test/pair_reference_gpu.jl:25-64
+ *   or_pair_typed     test/pair_reference_gpu.jl:8-64        (mixed sigma, pair matrix, 2-D)
  *   or_ewald          test/ewald_reference.jl:50-126         (FP64 Ewald, general cell)
  *   or_erfc           test/ewald_reference.jl:20-40          (series / continued fraction)
  *   or_ewald_sampled  test/ewald_reference.jl:50-126 for an orthorhombic box, forces on a
  *                     sample only (minimum image + the images rcut reaches), for large N
  *   or_bspline        src/electrostatics/Windows.jl:188-223  (B-spline weights, FP64)
- *   or_cell_pairs     cell-list restatement used only to time a same-algorithm CPU baseline
+ *   or_cell_pairs     LJ / WCA (src/engine/PairModels.jl:13-54) + real-space Ewald
+ *                     (src/electrostatics/RealSpace.jl:36-118) over a cell list, FP64
+ *   or_mesh_spread,   B-spline spread and analytic-derivative gather
+ *   or_mesh_gather    (src/electrostatics/MeshKernels.jl:247-276, 313-333), FP64; with the
+ *                     influence function and FFT of oracle.py (mesh.py) the CPU SPME / P3M-AD
  *   or_kg_forces      Kremer-Grest melt: WCA over all pairs (bonded pairs included) by cells
  *                     plus FENE bonds (_fene3_EV!, src/bonded/FENEBondCUDA.jl), FP64
  *
@@ -31,6 +36,16 @@
  *   bspline — partition of unity, sum dw = 0, derivative by differences, M_4 closed form.
  *   kg      — WCA part equals or_pair_direct; FENE two-bead closed form; F = -dE/dx and the
  *             virial W_aa = -dE/d eps_aa by central differences.
+ *   typed   — style 2 with equal diameters and style 3 with one type equal or_pair_direct;
+ *             a two-type matrix of the Lorentz means equals style 2; a dimer at
+ *             r = 2^(1/6) sigma_ij has zero force; F = -dE/dx; Newton's third law; 2-D equals
+ *             3-D with every z equal.
+ *   cell_pairs — LJ / WCA part equals or_pair_direct; the Coulomb part equals a brute-force
+ *             minimum-image erfc sum; F = -dE/dx by central differences.
+ *   mesh    — the CPU SPME total (cell_pairs + mesh + self) against or_ewald on the
+ *             reference's 120-charge case; F = -dE/dx and W_aa = -dE/d eps_aa by central
+ *             differences for SPME and P3M-AD; spread conserves charge; the P3M-AD error
+ *             converges in the alias order.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -122,6 +137,64 @@ int or_pair_direct(int n, const double *x, const double *y, const double *z, con
 /* ---------------------------------------------------------------- erfc */
 /* erfc(x): series of erf for x < 2, backward continued fraction beyond
  * (_erfc_ref, test/ewald_reference.jl:20-40). */
+/* Typed pair models (test/pair_reference_gpu.jl:8-64): style 1 uniform (sigma, epsilon, cutoff),
+ * style 2 mixed sigma (sigma_ij = (sigma_i + sigma_j)/2, rc = factor sigma_ij, one epsilon), style 3
+ * pair matrix (sigma, epsilon, rc by 1-based type id; ntypes x ntypes, column-major). kind 1 LJ
+ * (unshifted), 2 WCA (uniform rc^2 = 1.2599211 sigma^2; styles 2 and 3 take their rc; u + eps),
+ * 3 soft. D = 2 ignores z. Per sample 10 values: F(3), half energy, 6 half virials. */
+int or_pair_typed(int n, int D, const double *x, const double *y, const double *z, const double *box, int kind,
+                  int style, double sigma, double epsilon, double cutoff, const double *sigma_p, double factor,
+                  const int *typeid, int ntypes, const double *sig_m, const double *eps_m, const double *rc_m,
+                  int nsample, const int *ids, double *out) {
+  if (style == 3)
+    for (int i = 0; i < n; ++i)
+      if (typeid[i] < 1 || typeid[i] > ntypes) return -1;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int s = 0; s < nsample; ++s) {
+    const int i = ids[s];
+    double v[10] = {0};
+    for (int j = 0; j < n; ++j) {
+      if (j == i) continue;
+      double dx = x[i] - x[j]; dx -= box[0] * jl_round(dx / box[0]);
+      double dy = y[i] - y[j]; dy -= box[1] * jl_round(dy / box[1]);
+      double dz = 0.0;
+      if (D == 3) { dz = z[i] - z[j]; dz -= box[2] * jl_round(dz / box[2]); }
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      double sg, ep, rc2;
+      if (style == 1) {
+        sg = sigma; ep = epsilon;
+        rc2 = kind == 2 ? 1.2599211 * sigma * sigma : (kind == 3 ? sigma * sigma : cutoff * cutoff);
+      } else if (style == 2) {
+        sg = 0.5 * (sigma_p[i] + sigma_p[j]); ep = epsilon;
+        const double rc = factor * sg;
+        rc2 = rc * rc;
+      } else {
+        const int a = typeid[i] - 1, b = typeid[j] - 1;
+        sg = sig_m[a + (size_t)ntypes * b]; ep = eps_m[a + (size_t)ntypes * b];
+        const double rc = rc_m[a + (size_t)ntypes * b];
+        rc2 = rc * rc;
+      }
+      if (!(0.0 < r2 && r2 < rc2)) continue;
+      double a, u;
+      if (kind == 3) {
+        const double rr = sqrt(r2), xx = 1.0 - rr / sg;
+        a = (ep / sg) * xx / rr;
+        u = 0.5 * ep * xx * xx;
+      } else {
+        const double s6 = pow(sg * sg / r2, 3.0);
+        a = 24.0 * ep * s6 * (2.0 * s6 - 1.0) / r2;
+        u = 4.0 * ep * s6 * (s6 - 1.0) + (kind == 2 ? ep : 0.0);
+      }
+      v[0] += a * dx; v[1] += a * dy; v[2] += a * dz;
+      v[3] += 0.5 * u;
+      v[4] += 0.5 * a * dx * dx; v[5] += 0.5 * a * dy * dy; v[6] += 0.5 * a * dz * dz;
+      v[7] += 0.5 * a * dx * dy; v[8] += 0.5 * a * dx * dz; v[9] += 0.5 * a * dy * dz;
+    }
+    for (int k = 0; k < 10; ++k) out[10 * (size_t)s + k] = v[k];
+  }
+  return 0;
+}
+
 double or_erfc(double x) {
   if (x < 0) return 2.0 - or_erfc(-x);
   if (x < 2.0) {
@@ -417,13 +490,30 @@ int or_bspline(int P, double u, double *w, double *dw) {
   return (int)fl - (P - 1);
 }
 
-/* ---------------------------------------------------------------- cell-list CPU baseline */
-/* Same-algorithm CPU LJ force evaluation (cell list of width >= rc, full rows, FP64) used
- * only to time a fair CPU baseline beside the brute-force oracle. Returns the pair count. */
-long or_cell_lj(int n, const double *x, const double *y, const double *z, const double *box, double sigma,
-                double epsilon, double cutoff, double *fx, double *fy, double *fz, double *energy) {
+/* ---------------------------------------------------------------- same-algorithm CPU evaluation */
+/* Cell list of width >= the largest cutoff (27 cells, full rows, FP64) with the production pair
+ * conventions: LJ (kind 1, energy shifted to zero at rc_lj when shift != 0) or WCA (kind 2,
+ * rc^2 = 1.2599211 sigma^2, u + eps) (src/engine/PairModels.jl:13-54, src/engine/PairMath.jl:16-39),
+ * plus the real-space Ewald term C q_i q_j erfc(alpha r)/r inside rc_coul when C != 0
+ * (src/electrostatics/RealSpace.jl:36-118). Per particle: force F (3n, xyz), half energy E (n) and
+ * half virial V (6n: xx yy zz xy xz yz per particle). With or_mesh_spread / or_mesh_gather and an
+ * FFT this is the FP64 CPU restatement of the device SPME evaluation, used to compare the device
+ * at equal parameters and to time a same-algorithm CPU baseline. Returns the number of pairs
+ * within either cutoff, each counted once. */
+long or_cell_pairs(int n, const double *x, const double *y, const double *z, const double *q, const double *box,
+                   int kind, double sigma, double epsilon, double rc_lj, int shift, double C, double alpha,
+                   double rc_coul, double *F, double *E, double *V) {
+  const double rlj = kind == 2 ? sqrt(1.2599211) * sigma : (kind == 1 ? rc_lj : 0.0);
+  const double rlj2 = kind == 2 ? 1.2599211 * sigma * sigma : rlj * rlj;
+  const double rcc = C != 0.0 ? rc_coul : 0.0, rcc2 = rcc * rcc;
+  const double rmax = rlj > rcc ? rlj : rcc;
+  double ushift = 0.0;
+  if (kind == 1 && shift) {
+    const double s6 = pow(sigma / rc_lj, 6.0);
+    ushift = 4.0 * epsilon * s6 * (s6 - 1.0);
+  }
   int nc[3];
-  for (int d = 0; d < 3; ++d) { nc[d] = (int)floor(box[d] / cutoff); if (nc[d] < 3) nc[d] = 3; }
+  for (int d = 0; d < 3; ++d) { nc[d] = (int)floor(box[d] / rmax); if (nc[d] < 3) nc[d] = 3; }
   const int ncell = nc[0] * nc[1] * nc[2];
   int *head = (int *)malloc(sizeof(int) * ncell), *next = (int *)malloc(sizeof(int) * n);
   int *cell = (int *)malloc(sizeof(int) * n);
@@ -440,14 +530,13 @@ long or_cell_lj(int n, const double *x, const double *y, const double *z, const 
     cell[i] = (ci[2] * nc[1] + ci[1]) * nc[0] + ci[0];
   }
   for (int i = n - 1; i >= 0; --i) { next[i] = head[cell[i]]; head[cell[i]] = i; }
-  const double rc2 = cutoff * cutoff;
+  const double tsp = 2.0 / sqrt(M_PI);
   long pairs = 0;
-  double etot = 0.0;
-#pragma omp parallel for schedule(static) reduction(+ : pairs, etot)
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : pairs)
   for (int i = 0; i < n; ++i) {
     const int c = cell[i];
     const int cx = c % nc[0], cy = (c / nc[0]) % nc[1], cz = c / (nc[0] * nc[1]);
-    double ax = 0, ay = 0, az = 0, e = 0;
+    double ax = 0, ay = 0, az = 0, e = 0, v[6] = {0, 0, 0, 0, 0, 0};
     for (int oz = -1; oz <= 1; ++oz)
       for (int oy = -1; oy <= 1; ++oy)
         for (int ox = -1; ox <= 1; ++ox) {
@@ -459,20 +548,129 @@ long or_cell_lj(int n, const double *x, const double *y, const double *z, const 
             double dy = y[i] - y[j]; dy -= box[1] * jl_round(dy / box[1]);
             double dz = z[i] - z[j]; dz -= box[2] * jl_round(dz / box[2]);
             const double r2 = dx * dx + dy * dy + dz * dz;
-            if (!(r2 < rc2)) continue;
-            const double s2 = sigma * sigma / r2, s6 = s2 * s2 * s2;
-            const double a = 24.0 * epsilon * s6 * (2.0 * s6 - 1.0) / r2;
+            double a = 0.0, u = 0.0;
+            int hit = 0;
+            if (kind != 0 && r2 < rlj2) {
+              const double s2 = sigma * sigma / r2, s6 = s2 * s2 * s2;
+              a += 24.0 * epsilon * s6 * (2.0 * s6 - 1.0) / r2;
+              u += 4.0 * epsilon * s6 * (s6 - 1.0) + (kind == 2 ? epsilon : -ushift);
+              hit = 1;
+            }
+            if (C != 0.0 && r2 < rcc2) {
+              const double r = sqrt(r2), qq = C * q[i] * q[j];
+              const double ec = erfc(alpha * r) / r;
+              a += qq * (ec + tsp * alpha * exp(-alpha * alpha * r2)) / r2;
+              u += qq * ec;
+              hit = 1;
+            }
+            if (!hit) continue;
             ax += a * dx; ay += a * dy; az += a * dz;
-            e += 2.0 * epsilon * s6 * (s6 - 1.0);
-            ++pairs;
+            e += 0.5 * u;
+            v[0] += 0.5 * a * dx * dx; v[1] += 0.5 * a * dy * dy; v[2] += 0.5 * a * dz * dz;
+            v[3] += 0.5 * a * dx * dy; v[4] += 0.5 * a * dx * dz; v[5] += 0.5 * a * dy * dz;
+            if (j > i) ++pairs;
           }
         }
-    fx[i] = ax; fy[i] = ay; fz[i] = az;
-    etot += e;
+    F[3 * i] = ax; F[3 * i + 1] = ay; F[3 * i + 2] = az;
+    E[i] = e;
+    for (int k = 0; k < 6; ++k) V[6 * (size_t)i + k] = v[k];
   }
-  *energy = etot;
   free(head); free(next); free(cell);
   return pairs;
+}
+
+/* Fractional mesh coordinate u = K (x/L - floor(x/L)) in [0, K) (_grid_coordinate,
+ * src/electrostatics/MeshKernels.jl:9-15): node 0 sits at x = 0. */
+static double mesh_u(double x, double L, int K) {
+  double s = x / L;
+  s -= floor(s);
+  double u = s * K;
+  if (!isfinite(u)) return 0.0;
+  return u >= K ? u - K : u;
+}
+static int mesh_wrap(int n, int K) { return ((n % K) + K) % K; }
+
+/* Charge assignment Q(n) = sum_i q_i M_P(u_i - n) on the grid (K1, K2, K3), x fastest
+ * (_spread_lines_kernel!, src/electrostatics/MeshKernels.jl:247-276), FP64. Each OpenMP thread
+ * owns a range of z planes and adds only into them, so the sums are deterministic. */
+int or_mesh_spread(int n, const double *x, const double *y, const double *z, const double *q, const double *box,
+                   const int *K, int P, double *grid) {
+  if (P < 2 || P > 12) return -1;
+  const int K1 = K[0], K2 = K[1], K3 = K[2];
+  const size_t KK = (size_t)K1 * K2;
+  memset(grid, 0, sizeof(double) * KK * K3);
+#pragma omp parallel
+  {
+    int nt = 1, t = 0;
+#ifdef _OPENMP
+    nt = omp_get_num_threads();
+    t = omp_get_thread_num();
+#endif
+    const int z0 = (int)((long)K3 * t / nt), z1 = (int)((long)K3 * (t + 1) / nt);
+    double wx[16], wy[16], wz[16], d[16];
+    for (int i = 0; i < n; ++i) {
+      if (q[i] == 0.0) continue;
+      const int k3 = or_bspline(P, mesh_u(z[i], box[2], K3), wz, d);
+      int any = 0;
+      for (int j3 = 0; j3 < P; ++j3) {
+        const int n3 = mesh_wrap(k3 + j3, K3);
+        if (n3 >= z0 && n3 < z1) any = 1;
+      }
+      if (!any) continue;
+      const int k1 = or_bspline(P, mesh_u(x[i], box[0], K1), wx, d);
+      const int k2 = or_bspline(P, mesh_u(y[i], box[1], K2), wy, d);
+      for (int j3 = 0; j3 < P; ++j3) {
+        const int n3 = mesh_wrap(k3 + j3, K3);
+        if (n3 < z0 || n3 >= z1) continue;
+        for (int j2 = 0; j2 < P; ++j2) {
+          const int n2 = mesh_wrap(k2 + j2, K2);
+          const double qw = q[i] * wz[j3] * wy[j2];
+          double *line = grid + KK * n3 + (size_t)K1 * n2;
+          for (int j1 = 0; j1 < P; ++j1) line[mesh_wrap(k1 + j1, K1)] += qw * wx[j1];
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+/* Gather with the analytic B-spline derivative (_gather_ad_lines_kernel!,
+ * src/electrostatics/MeshKernels.jl:313-333): phi_i = sum_n M(u_i - n) pot(n) and
+ * F_i = -q_i (K_d / L_d) sum_n dM/du_d pot(n). F (3n, xyz), phi (n). */
+int or_mesh_gather(int n, const double *x, const double *y, const double *z, const double *q, const double *box,
+                   const int *K, int P, const double *pot, double *F, double *phi) {
+  if (P < 2 || P > 12) return -1;
+  const int K1 = K[0], K2 = K[1], K3 = K[2];
+  const size_t KK = (size_t)K1 * K2;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    double wx[16], wy[16], wz[16], dx[16], dy[16], dz[16];
+    const int k1 = or_bspline(P, mesh_u(x[i], box[0], K1), wx, dx);
+    const int k2 = or_bspline(P, mesh_u(y[i], box[1], K2), wy, dy);
+    const int k3 = or_bspline(P, mesh_u(z[i], box[2], K3), wz, dz);
+    double p = 0, gx = 0, gy = 0, gz = 0;
+    for (int j3 = 0; j3 < P; ++j3) {
+      const int n3 = mesh_wrap(k3 + j3, K3);
+      for (int j2 = 0; j2 < P; ++j2) {
+        const double *line = pot + KK * n3 + (size_t)K1 * mesh_wrap(k2 + j2, K2);
+        double lp = 0, lg = 0;
+        for (int j1 = 0; j1 < P; ++j1) {
+          const double v = line[mesh_wrap(k1 + j1, K1)];
+          lp += wx[j1] * v;
+          lg += dx[j1] * v;
+        }
+        p += wz[j3] * wy[j2] * lp;
+        gx += wz[j3] * wy[j2] * lg;
+        gy += wz[j3] * dy[j2] * lp;
+        gz += dz[j3] * wy[j2] * lp;
+      }
+    }
+    phi[i] = p;
+    F[3 * i] = -q[i] * gx * K1 / box[0];
+    F[3 * i + 1] = -q[i] * gy * K2 / box[1];
+    F[3 * i + 2] = -q[i] * gz * K3 / box[2];
+  }
+  return 0;
 }
 
 int or_num_threads(void) {

@@ -28,14 +28,18 @@ def _load():
     d, i = C.c_double, C.c_int
     lib.or_neighbor_rows.argtypes = [i, i, P(d), P(d), P(d), P(d), P(d), i, P(i), P(i), P(d)]
     lib.or_pair_direct.argtypes = [i, P(d), P(d), P(d), P(d), i, d, d, d, i, P(i), P(d)]
+    lib.or_pair_typed.argtypes = [i, i, P(d), P(d), P(d), P(d), i, i, d, d, d, P(d), d, P(i), i, P(d), P(d), P(d),
+                                  i, P(i), P(d)]
     lib.or_erfc.argtypes = [d]
     lib.or_erfc.restype = d
     lib.or_ewald.argtypes = [i, P(d), P(d), P(d), d, d, d, i, i, P(i), P(i), P(d), P(d), P(d), P(d)]
     lib.or_ewald_sampled.argtypes = [i, P(d), P(d), P(d), P(d), P(d), d, d, d, i, P(i), P(d)]
     lib.or_bspline.argtypes = [i, d, P(d), P(d)]
     lib.or_bspline.restype = i
-    lib.or_cell_lj.argtypes = [i, P(d), P(d), P(d), P(d), d, d, d, P(d), P(d), P(d), P(d)]
-    lib.or_cell_lj.restype = C.c_long
+    lib.or_cell_pairs.argtypes = [i, P(d), P(d), P(d), P(d), P(d), i, d, d, d, i, d, d, d, P(d), P(d), P(d)]
+    lib.or_cell_pairs.restype = C.c_long
+    lib.or_mesh_spread.argtypes = [i, P(d), P(d), P(d), P(d), P(d), P(i), i, P(d)]
+    lib.or_mesh_gather.argtypes = [i, P(d), P(d), P(d), P(d), P(d), P(i), i, P(d), P(d), P(d)]
     lib.or_kg_forces.argtypes = [i, P(d), P(d), P(d), P(d), d, d, i, P(i), P(i), d, d, P(d), P(d), P(d), P(d)]
     lib.or_kg_forces.restype = C.c_long
     lib.or_num_threads.restype = i
@@ -95,6 +99,34 @@ def pair_direct(x, y, z, box, kind="lj", sigma=1.0, epsilon=1.0, cutoff=2.5, ids
     return dict(F=out[:, 0:3], energy=out[:, 3], virial=out[:, 4:10], pair_scale=out[:, 10], counts=out[:, 11])
 
 
+def pair_typed(coords, box, kind="lj", style="uniform", sigma=1.0, epsilon=1.0, cutoff=2.5, sigma_particle=None,
+               factor=1.0, typeid=None, sigma_pair=None, epsilon_pair=None, rcut_pair=None, ids=None):
+    """FP64 direct sums of the typed pair models (test/pair_reference_gpu.jl:8-64). coords: (x, y)
+    or (x, y, z); style "uniform", "mixed" (per-particle sigma, rc = factor sigma_ij) or "matrix"
+    (1-based typeid, ntypes x ntypes sigma / epsilon / rcut). Returns dict F (n, D), energy (n),
+    virial (n, 6)."""
+    D = len(coords)
+    n = len(coords[0])
+    k = {"lj": 1, "wca": 2, "soft": 3}[kind]
+    st = {"uniform": 1, "mixed": 2, "matrix": 3}[style]
+    ids = np.arange(n) if ids is None else np.asarray(ids)
+    cs = [_d(c) for c in coords] + [_d(np.zeros(n))] * (3 - D)
+    bx = _d(list(box)[:D] + [1.0] * (3 - D))
+    sp = _d(sigma_particle if sigma_particle is not None else np.ones(1))
+    ti = _i(typeid if typeid is not None else np.ones(1, dtype=np.int32))
+    nt = 1 if sigma_pair is None else np.asarray(sigma_pair).shape[0]
+    m = [_d(np.asarray(a, dtype=np.float64).T.reshape(-1)) if a is not None else _d(np.ones(1))
+         for a in (sigma_pair, epsilon_pair, rcut_pair)]  # column-major
+    ia = _i(ids)
+    out = np.zeros((len(ids), 10))
+    rc = _lib.or_pair_typed(n, D, cs[0][1], cs[1][1], cs[2][1], bx[1], k, st, float(sigma), float(epsilon),
+                            float(cutoff), sp[1], float(factor), ti[1], int(nt), m[0][1], m[1][1], m[2][1], len(ids),
+                            ia[1], out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != 0:
+        raise ValueError("type id out of range")
+    return dict(F=out[:, :D], energy=out[:, 3], virial=out[:, 4:10])
+
+
 def ewald(R, q, H, C_=1.0, alpha=None, digits=16.0, background=False, exceptions=()):
     """FP64 Ewald (test/ewald_reference.jl:50-126). R: (n,3); H: 3x3 cell (columns = box vectors).
     Returns (E, F (n,3), W 3x3)."""
@@ -136,16 +168,23 @@ def bspline(P, u):
     return k0, w, dw
 
 
-def cell_lj(x, y, z, box, sigma=1.0, epsilon=1.0, cutoff=2.5):
-    """Same-algorithm CPU LJ evaluation (cell list, FP64) for the CPU baseline timing."""
+def cell_pairs(x, y, z, q, box, kind="none", sigma=1.0, epsilon=1.0, cutoff=2.5, shift=False, C_=0.0, alpha=0.0,
+               rc_coul=0.0):
+    """FP64 cell-list pair evaluation: LJ / WCA (kind "lj" / "wca" / "none") plus the real-space
+    Ewald term C q_i q_j erfc(alpha r)/r inside rc_coul when C_ != 0. Returns dict F (n,3),
+    E (n, half energies), V (n,6 half virials), pairs (each counted once)."""
+    k = {"none": 0, "lj": 1, "wca": 2}[kind]
     n = len(x)
     xa, ya, za, bx = _d(x), _d(y), _d(z), _d(box)
-    fx, fy, fz = np.zeros(n), np.zeros(n), np.zeros(n)
-    e = C.c_double()
+    qa = _d(q if q is not None else np.zeros(n))
+    F = np.zeros(3 * n)
+    E = np.zeros(n)
+    V = np.zeros(6 * n)
     P = C.POINTER(C.c_double)
-    pairs = _lib.or_cell_lj(n, xa[1], ya[1], za[1], bx[1], sigma, epsilon, cutoff, fx.ctypes.data_as(P),
-                            fy.ctypes.data_as(P), fz.ctypes.data_as(P), C.byref(e))
-    return np.stack([fx, fy, fz], 1), e.value, pairs
+    pairs = _lib.or_cell_pairs(n, xa[1], ya[1], za[1], qa[1], bx[1], k, float(sigma), float(epsilon), float(cutoff),
+                               1 if shift else 0, float(C_), float(alpha), float(rc_coul), F.ctypes.data_as(P),
+                               E.ctypes.data_as(P), V.ctypes.data_as(P))
+    return dict(F=F.reshape(n, 3), E=E, V=V.reshape(n, 6), pairs=int(pairs))
 
 
 def vv_steps(x, y, z, vx, vy, vz, box, kind="lj", sigma=1.0, epsilon=1.0, cutoff=2.5, dt=0.005, mass=1.0,

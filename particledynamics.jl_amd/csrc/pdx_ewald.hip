@@ -75,10 +75,10 @@ __global__ __launch_bounds__(TPB) void k_real_force(EwaldArgs a, const int32_t *
 }
 
 // One block per particle: real-space energy and virial (halves of full rows).
-__global__ __launch_bounds__(TPB) void k_real_energy(EwaldArgs a, double *partial, unsigned *ticket, double *out) {
+__global__ __launch_bounds__(TPB) void k_real_energy(EwaldArgs a, double *partial) {
   double f[3] = {0, 0, 0}, ev[7] = {0, 0, 0, 0, 0, 0, 0};
   real_terms(a, blockIdx.x, threadIdx.x, TPB, f, ev);
-  grid_sum_finalize<7>(ev, partial, ticket, out, false);
+  block_partials<7>(ev, partial);
 }
 
 // Structure factor S(k) = sum_j q_j e^{i k.r_j}, one block per wave vector.
@@ -99,8 +99,7 @@ __global__ __launch_bounds__(TPB) void k_structure(EwaldArgs a, const double *kv
 
 // Reciprocal energy and virial over the half space (x2).
 __global__ __launch_bounds__(TPB) void k_recip_energy(int nk, const double *kv, const double2 *S, double CoverV,
-                                                      double inv4a2, double *partial, unsigned *ticket,
-                                                      double *out) {
+                                                      double inv4a2, double *partial) {
   const int k = blockIdx.x * TPB + threadIdx.x;
   double ev[7] = {0, 0, 0, 0, 0, 0, 0};
   if (k < nk) {
@@ -113,7 +112,7 @@ __global__ __launch_bounds__(TPB) void k_recip_energy(int nk, const double *kv, 
     ev[1] = Ek * (1.0 - b * kx * kx); ev[2] = Ek * (1.0 - b * ky * ky); ev[3] = Ek * (1.0 - b * kz * kz);
     ev[4] = -Ek * b * kx * ky; ev[5] = -Ek * b * kx * kz; ev[6] = -Ek * b * ky * kz;
   }
-  grid_sum_finalize<7>(ev, partial, ticket, out, false);
+  block_partials<7>(ev, partial);
 }
 
 // Reciprocal force on sampled particles, one block per sample, threads over wave vectors.
@@ -173,15 +172,15 @@ extern "C" int pdx_ewald_fp64(int32_t n, const double *x, const double *y, const
   const int nk = (int)(kv.size() / 3);
   const double V = box[0] * box[1] * box[2];
   const double CoverV = C / V, inv4a2 = 1.0 / (4 * alpha * alpha);
+  // a synchronous boundary call (the error judge, not an enqueue of the force path): its
+  // workspace lives for the call
   DevBuf<double> dkv, part, acc;
   DevBuf<double2> S;
-  DevBuf<unsigned> ticket;
   int rc;
   const int ekblocks = std::max(1, (nk + TPB - 1) / TPB);
   if ((rc = dkv.alloc(std::max<size_t>(kv.size(), 3))) || (rc = S.alloc(std::max(nk, 1))) ||
-      (rc = part.alloc((size_t)7 * std::max(n, ekblocks))) || (rc = acc.alloc(16)) || (rc = ticket.alloc(1)))
+      (rc = part.alloc((size_t)7 * std::max(n, ekblocks))) || (rc = acc.alloc(16)))
     return rc;
-  PDX_HIP(hipMemsetAsync(ticket.p, 0, sizeof(unsigned), st));
   PDX_HIP(hipMemsetAsync(acc.p, 0, sizeof(double) * 16, st));
   if (nk > 0) PDX_HIP(hipMemcpyAsync(dkv.p, kv.data(), sizeof(double) * kv.size(), hipMemcpyHostToDevice, st));
   if (nk > 0) hipLaunchKernelGGL(k_structure, dim3(nk), dim3(TPB), 0, st, a, dkv.p, S.p);
@@ -191,10 +190,15 @@ extern "C" int pdx_ewald_fp64(int32_t n, const double *x, const double *y, const
       hipLaunchKernelGGL(k_recip_force, dim3(nsample), dim3(TPB), 0, st, a, nk, dkv.p, S.p, CoverV, inv4a2, sample, F);
   }
   if (want_energy) {
-    hipLaunchKernelGGL(k_real_energy, dim3(n), dim3(TPB), 0, st, a, part.p, ticket.p, acc.p);
-    if (nk > 0)
-      hipLaunchKernelGGL(k_recip_energy, dim3(ekblocks), dim3(TPB), 0, st, nk, dkv.p, S.p, CoverV, inv4a2, part.p,
-                         ticket.p, acc.p + 8);
+    // per-block FP64 partials, then one block sums them in block order (no device-scope tickets)
+    hipLaunchKernelGGL(k_real_energy, dim3(n), dim3(TPB), 0, st, a, part.p);
+    hipLaunchKernelGGL(k_sum_partials<7>, dim3(1), dim3(256), 0, st, (const double *)part.p, n, acc.p, 0,
+                       (const int64_t *)nullptr);
+    if (nk > 0) {
+      hipLaunchKernelGGL(k_recip_energy, dim3(ekblocks), dim3(TPB), 0, st, nk, dkv.p, S.p, CoverV, inv4a2, part.p);
+      hipLaunchKernelGGL(k_sum_partials<7>, dim3(1), dim3(256), 0, st, (const double *)part.p, ekblocks, acc.p + 8, 0,
+                         (const int64_t *)nullptr);
+    }
   }
   PDX_LAUNCH_CHECK();
   PDX_HIP(hipStreamSynchronize(st));

@@ -129,58 +129,6 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-// Deterministic grid-wide FP64 sum of NV values per thread, in one launch: every block
-// reduces its values in a fixed order and stores them in `partial[block*NV + c]`; the
-// last block to finish (atomic ticket) sums the partials in block order and writes (or
-// adds into) out[c], then resets the ticket for the next launch. Must be called by every
-// thread of every block (blockDim.x a multiple of 64, at most 1024).
-template <int NV>
-__device__ void grid_sum_finalize(const double (&v)[NV], double *partial, unsigned *ticket,
-                                  double *out, bool add) {
-  __shared__ double sh[NV][16];
-  __shared__ bool last;
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = threadIdx.x / WAVE;
-  const int nwave = blockDim.x / WAVE;
-#pragma unroll
-  for (int c = 0; c < NV; ++c) {
-    double s = wave_sum(v[c]);
-    if (lane == 0) sh[c][wave] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x < NV) {
-    double s = 0.0;
-    for (int w = 0; w < nwave; ++w) s += sh[threadIdx.x][w];
-    partial[(size_t)blockIdx.x * NV + threadIdx.x] = s;
-  }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // Fixed-order sum over blocks: thread t takes blocks t, t+blockDim, ... then a fixed tree.
-  double acc[NV];
-#pragma unroll
-  for (int c = 0; c < NV; ++c) acc[c] = 0.0;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-#pragma unroll
-    for (int c = 0; c < NV; ++c) acc[c] += __builtin_nontemporal_load(&partial[(size_t)b * NV + c]);
-  }
-#pragma unroll
-  for (int c = 0; c < NV; ++c) {
-    double s = wave_sum(acc[c]);
-    if (lane == 0) sh[c][wave] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x < NV) {
-    double s = 0.0;
-    for (int w = 0; w < nwave; ++w) s += sh[threadIdx.x][w];
-    if (out) out[threadIdx.x] = add ? out[threadIdx.x] + s : s;
-  }
-  if (threadIdx.x == 0) *ticket = 0u;
-}
-
 // Deterministic grid-wide FP64 sums in two launches (no atomics): every block reduces its
 // NV values in a fixed order and stores them in partial[block*NV + c]; then one block of
 // k_sum_partials adds the partials in block order and writes (or adds into) out[c]. A
@@ -259,14 +207,16 @@ struct pdx_nl_s {
   pdx::DevBuf<float4> packed;   // N, storage order (x, y, z, q)
   pdx::DevBuf<float4> sorted;   // N, cell order (x, y, z, id bits)
   pdx::DevBuf<float4> sortedq;  // N, cell order, current (x, y, z, q): the cell traversal's copy
-  pdx::DevBuf<float> disp_partial;
+  pdx::DevBuf<float> disp_partial;  // ndisp per-block maxima of the last check
+  int32_t ndisp = 0;
   pdx::DevBuf<int32_t> cap_partial;
   pdx::DevBuf<int64_t> stat_partial;  // 2 per block: candidates, accepted
   pdx::DevBuf<int32_t> scan_block;    // per-1024-cell block totals
   pdx::DevBuf<int64_t> candidates;    // N (diagnostics)
-  pdx::DevBuf<unsigned> tickets;      // last-block tickets (decide, commit)
   // cell snapshot for a spread that runs beside the rebuild: offsets and ids of the last
-  // completed build; snap_state[0] = build count copied, [2] = nonzero while none is valid
+  // completed build; snap_state[0] = build count copied, [2] = nonzero while none is valid,
+  // [3] = nonzero when the last snapshot spread found a particle beyond its margin (written by
+  // that spread, read by its fallback spread and by the tile gather of the same evaluation)
   pdx::DevBuf<int32_t> snap_offsets, snap_ids;
   pdx::DevBuf<int64_t> snap_state;
   int snap_host = 0;  // checks since create / box change (capped at 2): at 2 a snapshot was taken
@@ -288,7 +238,6 @@ struct pdx_pair_s {
   pdx::DevBuf<int32_t> exc_off, exc_j;
   pdx::DevBuf<float> exc_s;
   pdx::DevBuf<double> partial;
-  pdx::DevBuf<unsigned> ticket;
 };
 
 struct pdx_bonds_s {
@@ -315,7 +264,6 @@ struct pdx_mesh_s {
   pdx::DevBuf<float2> tw;                  // twiddles exp(-2 pi i m / K) for K1 | K2 | K3
   pdx::DevBuf<double> acc;                 // 8: mesh energy + 6 virial + spare
   pdx::DevBuf<double> partial;
-  pdx::DevBuf<unsigned> ticket;
   // P3M: virial coefficients Vc (6 x half spectrum, SoA) of the optimal influence function;
   // ik: the z-field spectrum, a scratch spectrum and the three real field grids
   pdx::DevBuf<float> Vc;
@@ -336,5 +284,6 @@ struct pdx_mesh_s {
   pdx::DevBuf<int32_t> tile_n;
   size_t tile_entries = 0;
   const void *tiles_nl = nullptr;
+  bool tiles_snapshot = false;  // the lists came from a snapshot spread (its word 3 may send the gather per particle)
   int tiles_stile = 0, tiles_count = 0, tiles_cap = 0;
 };

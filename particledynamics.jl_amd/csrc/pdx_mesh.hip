@@ -447,7 +447,6 @@ struct SolveArgs {
   float twopi_Lf[3];   // the same in FP32
   float inv2a2f;       // 2 inv4a2 = 1/(2 alpha^2)
   double *partial;
-  unsigned *ticket;
   double *acc;
 };
 
@@ -806,6 +805,30 @@ __device__ __forceinline__ float sum8_lanes(float v) {
 // particle's P x P x P stencil and walks its P z nodes with 64-bit integer atomics (the
 // sum is exact and independent of the order of arrival).
 template <int P>
+__device__ __forceinline__ void spread_particle(int lane, float px, float py, float pz, float qi,
+                                                unsigned long long *grid, int K1, int K2, int K3, float Lx,
+                                                float Ly, float Lz, float scale) {
+  constexpr int LPP = lanes_for(P);
+  float wx[P], wy[P], wz[P], d[P];
+  const int k1 = bspline<P>(grid_coordinate(px, Lx, K1), wx, d);
+  const int k2 = bspline<P>(grid_coordinate(py, Ly, K2), wy, d);
+  const int k3 = bspline<P>(grid_coordinate(pz, Lz, K3), wz, d);
+  const size_t KK = (size_t)K1 * K2;
+  for (int l = lane; l < P * P; l += LPP) {
+    const int j1 = l % P, j2 = l / P;
+    const int n1 = wrap_node(k1 + j1, K1), n2 = wrap_node(k2 + j2, K2);
+    const float q12 = qi * sel<P>(wx, j1) * sel<P>(wy, j2) * scale;
+    const size_t b = (size_t)n1 + (size_t)K1 * n2;
+#pragma unroll
+    for (int j3 = 0; j3 < P; ++j3) {
+      const int n3 = wrap_node(k3 + j3, K3);
+      const long long v = fixed_round(q12 * wz[j3]);
+      atomicAdd(&grid[b + KK * n3], (unsigned long long)v);
+    }
+  }
+}
+
+template <int P>
 __global__ __launch_bounds__(TPB) void k_spread(int N, const float *__restrict__ x, const float *__restrict__ y,
                                                 const float *__restrict__ z, const float *__restrict__ q,
                                                 const int32_t *__restrict__ order, unsigned long long *grid,
@@ -819,22 +842,26 @@ __global__ __launch_bounds__(TPB) void k_spread(int N, const float *__restrict__
   const int i = order ? order[t] - 1 : t;
   const float qi = q[i];
   if (qi == 0.0f) return;
-  float wx[P], wy[P], wz[P], d[P];
-  const int k1 = bspline<P>(grid_coordinate(x[i], Lx, K1), wx, d);
-  const int k2 = bspline<P>(grid_coordinate(y[i], Ly, K2), wy, d);
-  const int k3 = bspline<P>(grid_coordinate(z[i], Lz, K3), wz, d);
-  const size_t KK = (size_t)K1 * K2;
-  for (int l = lane; l < P * P; l += LPP) {
-    const int j1 = l % P, j2 = l / P;
-    const int n1 = wrap_node(k1 + j1, K1), n2 = wrap_node(k2 + j2, K2);
-    const float q12 = qi * sel<P>(wx, j1) * sel<P>(wy, j2) * scale;
-    const size_t b = (size_t)n1 + (size_t)K1 * n2;
-#pragma unroll
-    for (int j3 = 0; j3 < P; ++j3) {
-      const int n3 = wrap_node(k3 + j3, K3);
-      const long long v = fixed_round(q12 * wz[j3]);
-      atomicAdd(&grid[b + KK * n3], (unsigned long long)v);
-    }
+  spread_particle<P>(lane, x[i], y[i], z[i], qi, grid, K1, K2, K3, Lx, Ly, Lz, scale);
+}
+
+// ---- fallback of the snapshot spread: when the snapshot spread found a particle further from
+// its snapshot cell than the snapshot's margin (stale[3] != 0, written by that spread, whose
+// tiles then only cleared the grid), every particle of `packed` is spread with global 64-bit
+// atomics, as k_spread; otherwise every workgroup leaves at once. Grid-stride, so a small grid
+// suffices for this rare path.
+template <int P>
+__global__ __launch_bounds__(TPB) void k_spread_stale(int N, const float4 *__restrict__ packed,
+                                                      const int64_t *__restrict__ stale, unsigned long long *grid,
+                                                      int K1, int K2, int K3, float Lx, float Ly, float Lz,
+                                                      float scale) {
+  if (stale[3] == 0) return;
+  constexpr int LPP = lanes_for(P);
+  const int64_t total = (int64_t)N * LPP;
+  for (int64_t tid = (int64_t)blockIdx.x * TPB + threadIdx.x; tid < total; tid += (int64_t)gridDim.x * TPB) {
+    const int i = (int)(tid / LPP), lane = (int)(tid % LPP);
+    const float4 p = packed[i];
+    if (p.w != 0.0f) spread_particle<P>(lane, p.x, p.y, p.z, p.w, grid, K1, K2, K3, Lx, Ly, Lz, scale);
   }
 }
 
@@ -892,7 +919,9 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && P <= 5) ? 8 : 1) void k_spread
                                                       int nz, float margin, unsigned long long *grid, int K1,
                                                       int K2, int K3, float Lx, float Ly, float Lz, float scale,
                                                       float4 *__restrict__ tlist, int32_t *__restrict__ tids,
-                                                      int32_t *__restrict__ tn, int tcap) {
+                                                      int32_t *__restrict__ tn, int tcap,
+                                                      const float *__restrict__ disp, int ndisp, float stale2,
+                                                      int64_t *stale_word) {
   if (state[PDX_DN_ERROR] != 0) return;  // no usable cells / positions: the grid keeps its last value
   constexpr int LPP = lanes_for(P);
   // x rows padded to STILE + 1 nodes: the P x P lanes of a particle then spread over the LDS banks
@@ -906,6 +935,20 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && P <= 5) ? 8 : 1) void k_spread
   const int a1 = (b % tx) * STILE, a2 = ((b / tx) % ty) * STILE, a3 = (b / (tx * ty)) * STILE;
   const int e1 = min(STILE, K1 - a1), e2 = min(STILE, K2 - a2), e3 = min(STILE, K3 - a3);
   for (int k = threadIdx.x; k < STILE * STILE * SROW; k += SBS) acc[k] = 0ull;
+  // snapshot cells (disp != null): the check's per-block maxima of |x - x_build|^2 bound the
+  // motion since the snapshot's build. A maximum beyond the margin (or -1: not measured) makes
+  // the cells unusable: every tile then only clears its nodes, and the fallback spread and the
+  // per-particle gather take over for this evaluation (word 3 of the snapshot state).
+  bool stale = false;
+  if (disp) {
+    int s = 0;
+    for (int k = threadIdx.x; k < ndisp; k += SBS) {
+      const float d = disp[k];
+      s |= !(d >= 0.0f && d <= stale2);
+    }
+    stale = __syncthreads_or(s) != 0;
+    if (b == 0 && threadIdx.x == 0) stale_word[3] = stale ? 1 : 0;
+  }
   // candidate cells per dimension, in the list's frame (x + L/2) / L
   const int K[3] = {K1, K2, K3}, A[3] = {a1, a2, a3}, E[3] = {e1, e2, e3}, NC[3] = {nx, ny, nz};
   const float Ls[3] = {Lx, Ly, Lz};
@@ -954,7 +997,7 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && P <= 5) ? 8 : 1) void k_spread
     // list full: this thread spreads the particle itself
     spread_direct<P, STILE>(acc, u1, u2, u3, p.w, o1, o2, o3, e1, e2, e3, K1, K2, K3, scale);
   };
-  for (int round = 0; round * 64 < ncand; ++round) {
+  for (int round = 0; !stale && round * 64 < ncand; ++round) {
     const int nc = min(64, ncand - round * 64);
     if (threadIdx.x < 64) {  // wave 0: the round's cells and the inclusive scan of their sizes
       int cnt = 0;
@@ -1005,13 +1048,13 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && P <= 5) ? 8 : 1) void k_spread
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) tn[b] = nown;
+  if (threadIdx.x == 0) tn[b] = stale ? 0 : nown;
   // staged particles, SCHUNK at a time: per particle and dimension (three threads each) the
   // B-spline weights and the tile-local index of each stencil node (a byte; 0xFF: outside the
   // tile) go to LDS; then LPP lanes per particle, lane = one (x, y) line of the stencil walking
   // its P z nodes. The products and their order are those of k_spread: ((q w_x) w_y) 2^S, then
   // times w_z, rounded to the nearest integer.
-  const int nl = min(nlist, SLIST);
+  const int nl = stale ? 0 : min(nlist, SLIST);
   __shared__ float wsh[SCHUNK][3][8];
   __shared__ uint2 lsh[SCHUNK][3];
   const int grp = threadIdx.x / LPP, lane = threadIdx.x % LPP;
@@ -1177,7 +1220,8 @@ __global__ __launch_bounds__(GTB) void k_gather_tiles(int N, const float4 *__res
                                                       const float *__restrict__ pot, int K1, int K2, int K3,
                                                       float Lx, float Ly, float Lz, float *fx, float *fy,
                                                       float *fz, float *Epot, float bg, int set, float *Vglobal,
-                                                      const double *acc, double ubg) {
+                                                      const double *acc, double ubg,
+                                                      const int64_t *__restrict__ stale_word) {
   constexpr int R = STILE + P - 1;
   __shared__ float reg[R * R * R];
   __shared__ float4 pl[GCHUNK];
@@ -1189,6 +1233,8 @@ __global__ __launch_bounds__(GTB) void k_gather_tiles(int N, const float4 *__res
     Vglobal[threadIdx.x] = (float)(acc[1 + threadIdx.x] + (threadIdx.x < 3 ? ubg : 0.0));
   const int tx = (K1 + STILE - 1) / STILE, ty = (K2 + STILE - 1) / STILE;
   const float s1 = (float)K1 / Lx, s2 = (float)K2 / Ly, s3 = (float)K3 / Lz;
+  // a stale snapshot spread wrote no lists: the walk past the tiles then gathers every particle
+  const bool stale = stale_word && stale_word[3] != 0;
   if ((int)blockIdx.x >= tiles) {
     const int stride = (gridDim.x - tiles) * GTB;
     const size_t KK = (size_t)K1 * K2;
@@ -1197,7 +1243,7 @@ __global__ __launch_bounds__(GTB) void k_gather_tiles(int N, const float4 *__res
       const float u1 = grid_coordinate(p.x, Lx, K1), u2 = grid_coordinate(p.y, Ly, K2),
                   u3 = grid_coordinate(p.z, Lz, K3);
       const int t = (int)floorf(u1) / STILE + tx * ((int)floorf(u2) / STILE + ty * ((int)floorf(u3) / STILE));
-      if (tn[t] <= tcap) continue;
+      if (!stale && tn[t] <= tcap) continue;
       float wx[P], wy[P], wz[P], dx[P], dy[P], dz[P];
       const int k1 = bspline<P>(u1, wx, dx), k2 = bspline<P>(u2, wy, dy), k3 = bspline<P>(u3, wz, dz);
       float phi = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
@@ -1229,6 +1275,7 @@ __global__ __launch_bounds__(GTB) void k_gather_tiles(int N, const float4 *__res
     }
     return;
   }
+  if (stale) return;
   const int b = blockIdx.x;
   const int n = tn[b];
   if (n <= 0 || n > tcap) return;
@@ -1716,7 +1763,6 @@ SolveArgs solve_args(pdx_mesh_t m) {
   for (int d = 0; d < 3; ++d) s.twopi_Lf[d] = (float)s.twopi_L[d];
   s.inv2a2f = (float)(2.0 * s.inv4a2);
   s.partial = m->partial.p;
-  s.ticket = m->ticket.p;
   s.acc = m->acc.p;
   return s;
 }
@@ -2017,7 +2063,7 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
   int rc;
   if ((rc = m->grid64.alloc(nnodes)) || (rc = m->spec_c.alloc(nmodes)) || (rc = m->pot.alloc(nnodes)) ||
       (rc = m->G.alloc(nmodes)) || (rc = m->tw.alloc(K[0] + K[1] + K[2])) || (rc = m->acc.alloc(8)) ||
-      (rc = m->partial.alloc((size_t)7 * m->nmode_blocks)) || (rc = m->ticket.alloc(1))) {
+      (rc = m->partial.alloc((size_t)7 * m->nmode_blocks))) {
     delete m;
     return rc;
   }
@@ -2049,7 +2095,6 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
   }
   if (hipMemset(m->grid64.p, 0, sizeof(unsigned long long) * nnodes) != hipSuccess ||
       hipMemset(m->acc.p, 0, sizeof(double) * 8) != hipSuccess ||
-      hipMemset(m->ticket.p, 0, sizeof(unsigned)) != hipSuccess ||
       hipMemset(m->pot.p, 0, sizeof(float) * nnodes) != hipSuccess) {
     delete m;
     return fail(PDX_ERR_DEVICE, "mesh initialisation failed");
@@ -2133,6 +2178,7 @@ extern "C" int pdx_mesh_spread(pdx_mesh_t m, const float *x, const float *y, con
   if (!m || !x || !y || !z || !q) return fail(PDX_ERR_ARGUMENT, "null argument");
   const int N = m->N;
   m->tiles_nl = nullptr;  // this spread writes no tile lists
+  m->tiles_snapshot = false;
   const float L[3] = {(float)m->box[0], (float)m->box[1], (float)m->box[2]};
   const float scale = (float)m->fixed_scale;
   hipStream_t st = as_stream(stream);
@@ -2187,11 +2233,17 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
   m->tiles_stile = small ? 8 : 16;
   m->tiles_count = tiles;
   m->tiles_cap = tcap;
+  // snapshot: the check's displacement maxima decide whether the snapshot still covers every
+  // particle (|x - x_build| <= margin); the fallback spread runs only when it does not
+  const float *disp = snapshot ? nl->disp_partial.p : nullptr;
+  const int ndisp = nl->ndisp;
+  const float stale2 = margin * margin;
+  m->tiles_snapshot = snapshot;
 #define PDX_TSPREAD_L(PP, TT, BB)                                                                              \
   hipLaunchKernelGGL((k_spread_tiles<PP, TT, BB>), dim3(tiles), dim3(BB), 0, st, nl->packed.p,                 \
                      offsets, ids, guard, nl->nx, nl->ny, nl->nz,                                              \
                      margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale, m->tile_list.p,   \
-                     m->tile_ids.p, m->tile_n.p, tcap);
+                     m->tile_ids.p, m->tile_n.p, tcap, disp, ndisp, stale2, nl->snap_state.p);
 #define PDX_TSPREAD(PP)                                                                                        \
   case PP:                                                                                                     \
     if (small) {                                                                                               \
@@ -2216,6 +2268,25 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
 #undef PDX_TSPREAD
 #undef PDX_TSPREAD_L
   PDX_LAUNCH_CHECK();
+  if (snapshot) {
+    const int fb = (int)std::min<int64_t>(1024, ((int64_t)m->N * lanes_for(8) + TPB - 1) / TPB);
+#define PDX_STALE(PP)                                                                                          \
+  case PP:                                                                                                     \
+    hipLaunchKernelGGL(k_spread_stale<PP>, dim3(fb), dim3(TPB), 0, st, m->N, nl->packed.p,                    \
+                       (const int64_t *)nl->snap_state.p, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale); \
+    break;
+    switch (m->P) {
+      PDX_STALE(3)
+      PDX_STALE(4)
+      PDX_STALE(5)
+      PDX_STALE(6)
+      PDX_STALE(7)
+      PDX_STALE(8)
+      default: return fail(PDX_ERR_ARGUMENT, "unsupported order");
+    }
+#undef PDX_STALE
+    PDX_LAUNCH_CHECK();
+  }
   return PDX_OK;
 }
 }  // namespace
@@ -2332,7 +2403,8 @@ extern "C" int pdx_mesh_gather_cells(pdx_mesh_t m, pdx_nl_t nl, float *fx, float
 #define PDX_GT_L(PP, TT, EE)                                                                                   \
   hipLaunchKernelGGL((k_gather_tiles<PP, TT, EE>), grid, dim3(GTB), 0, st, N, nl->packed.p, m->tile_list.p,    \
                      m->tile_ids.p, m->tile_n.p, m->tiles_cap, m->tiles_count, m->pot.p, m->K1, m->K2, m->K3, L[0], L[1],     \
-                     L[2], fx, fy, fz, Epot, m->bg, set, Vglobal, m->acc.p, m->ubg);
+                     L[2], fx, fy, fz, Epot, m->bg, set, Vglobal, m->acc.p, m->ubg,                        \
+                     m->tiles_snapshot ? (const int64_t *)nl->snap_state.p : nullptr);
 #define PDX_GT(PP)                                                                                             \
   case PP:                                                                                                     \
     if (m->tiles_stile == 8) {                                                                                 \

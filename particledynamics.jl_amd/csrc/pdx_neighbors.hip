@@ -85,7 +85,9 @@ __global__ __launch_bounds__(TPB) void k_decide(int N, const float *__restrict__
   if (threadIdx.x == 0) {
     float b = smax[0];
     for (int w = 1; w < TPB / WAVE; ++w) b = fmaxf(b, smax[w]);
-    partial[blockIdx.x] = b;
+    // -1: not measured (no valid list to measure against); a snapshot spread reading these
+    // maxima then cannot bound the motion since the snapshot's build
+    partial[blockIdx.x] = (!valid && b != INFINITY) ? -1.0f : b;
   }
 }
 
@@ -690,6 +692,7 @@ extern "C" int pdx_nl_create(int32_t n, const double box[3], double cutoff, doub
     nl->row_lanes = G;
   }
   const int nb = blocks_for(n, TPB);
+  nl->ndisp = nb;
   const int sblocks = blocks_for(nl->ncell, SCAN_BLOCK);
   if ((rc = nl->state.alloc(PDX_DN_NSTATE)) || (rc = nl->geom.alloc(3)) ||
       (rc = nl->neighbors_flat.alloc((size_t)n * cap)) || (rc = nl->counts.alloc(n)) ||
@@ -698,7 +701,7 @@ extern "C" int pdx_nl_create(int32_t n, const double box[3], double cutoff, doub
       (rc = nl->cell_rank.alloc(n)) || (rc = nl->rref.alloc((size_t)3 * n)) || (rc = nl->packed.alloc(n)) ||
       (rc = nl->sorted.alloc(n)) || (rc = nl->sortedq.alloc(n)) || (rc = nl->disp_partial.alloc(nb)) || (rc = nl->cap_partial.alloc(nb)) ||
       (rc = nl->stat_partial.alloc((size_t)2 * nb)) || (rc = nl->scan_block.alloc(sblocks)) ||
-      (rc = nl->candidates.alloc(nl->diagnostics ? n : 1)) || (rc = nl->tickets.alloc(4)) ||
+      (rc = nl->candidates.alloc(nl->diagnostics ? n : 1)) ||
       (rc = nl->snap_offsets.alloc((size_t)nl->ncell + 1)) || (rc = nl->snap_ids.alloc(n)) ||
       (rc = nl->snap_state.alloc(4))) {
     delete nl;
@@ -721,7 +724,6 @@ extern "C" int pdx_nl_create(int32_t n, const double box[3], double cutoff, doub
   if (e == hipSuccess) e = hipMemset(nl->counts.p, 0, sizeof(int32_t) * n);
   if (e == hipSuccess) e = hipMemset(nl->cell_counts.p, 0, sizeof(int32_t) * nl->ncell);
   if (e == hipSuccess) e = hipMemcpy(nl->cell_offsets.p, ones.data(), sizeof(int32_t) * (nl->ncell + 1), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(nl->tickets.p, 0, sizeof(unsigned) * 4);
   if (e == hipSuccess) e = hipMemcpy(nl->particle_ids_sorted.p, ident.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(nl->cell_of_particle.p, 0, sizeof(int32_t) * n);
   if (e == hipSuccess) e = hipMemset(nl->packed.p, 0, sizeof(float4) * n);

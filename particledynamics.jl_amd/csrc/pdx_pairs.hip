@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <string>
 
 using namespace pdx;
 
@@ -30,7 +31,6 @@ struct PairArgs {
   float *fx, *fy, *fz, *Epot, *V;
   double *totals;
   double *partial;
-  unsigned *ticket;
   int add;
   float eps, sig2, rc2, eshift;   // LJ / WCA / soft
   float cC, crc2, csx, ceshift, halfself;
@@ -859,12 +859,6 @@ extern "C" int pdx_pair_create(const pdx_pair_spec *spec, pdx_pair_t *out) {
     p->fit_err_f = ferr / fmax;
   }
   p->lanes = sp.lanes;
-  int rc;
-  if ((rc = p->ticket.alloc(1))) { delete p; return rc; }
-  if (hipMemset(p->ticket.p, 0, sizeof(unsigned)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
-    delete p;
-    return fail(PDX_ERR_DEVICE, "pair model initialisation failed");
-  }
   *out = p;
   return PDX_OK;
 }
@@ -880,8 +874,12 @@ extern "C" int pdx_pair_info(pdx_pair_t p, double out[4]) {
   return PDX_OK;
 }
 
-extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, float *fz, float *Epot,
-                               float *V, double *totals, int32_t flags, pdx_stream_t stream) {
+namespace {
+// fixed_partial == nullptr: the model's own partial buffer, grown here when too small (a host
+// boundary: it synchronises and allocates); otherwise that caller-provided workspace of
+// fixed_blocks per-block slots, and a launch needing more is refused.
+int pair_forces_impl(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, float *fz, float *Epot, float *V,
+                     double *totals, int32_t flags, pdx_stream_t stream, double *fixed_partial, int fixed_blocks) {
   if (!p || !nl) return fail(PDX_ERR_ARGUMENT, "null pair model or neighbor list");
   if (!fx || !fy || !fz) return fail(PDX_ERR_ARGUMENT, "null force array");
   const bool energy = (flags & PDX_ENERGY) != 0;
@@ -913,13 +911,19 @@ extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, 
   if (acc_path) G = std::min(G, 8);
   const int blocks = cells ? nl->ncell
                            : (G < 0 ? (N + WAVE - 1) / WAVE : (int)(((int64_t)N * G + TPB - 1) / TPB));
+  double *partial = fixed_partial;
   if (energy && totals) {
-    if ((size_t)blocks * 7 > p->partial.n) {
+    if (fixed_partial) {
+      if (blocks > fixed_blocks)
+        return fail(PDX_ERR_ARGUMENT, std::to_string(blocks) + " workgroups exceed the fixed workspace of " +
+                                          std::to_string(fixed_blocks) + " (fewer particles or lanes, or pdx_pair_forces)");
+    } else if ((size_t)blocks * 7 > p->partial.n) {
       PDX_HIP(hipStreamSynchronize(as_stream(stream)));
       int rc = p->partial.alloc((size_t)blocks * 7);
       if (rc) return rc;
     }
   }
+  if (!fixed_partial) partial = p->partial.p;
   PairArgs a{};
   a.N = N;
   a.packed = nl->packed.p;
@@ -930,8 +934,7 @@ extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, 
   a.fx = fx; a.fy = fy; a.fz = fz;
   a.Epot = Epot; a.V = V;
   a.totals = totals;
-  a.partial = p->partial.p;
-  a.ticket = p->ticket.p;
+  a.partial = partial;
   a.add = (flags & PDX_ADD) ? 1 : 0;
   a.eps = p->eps; a.sig2 = p->sig2; a.rc2 = p->rc2; a.eshift = p->eshift;
   a.cC = p->cC; a.crc2 = p->crc2; a.csx = p->csx; a.ceshift = p->ceshift; a.halfself = p->halfself;
@@ -950,7 +953,7 @@ extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, 
   auto finish = [&]() -> int {
     PDX_LAUNCH_CHECK();
     if (energy && totals) {
-      hipLaunchKernelGGL(k_sum_partials<7>, dim3(1), dim3(256), 0, st, (const double *)p->partial.p, blocks,
+      hipLaunchKernelGGL(k_sum_partials<7>, dim3(1), dim3(256), 0, st, (const double *)partial, blocks,
                          totals, a.add, (const int64_t *)nl->state.p);
       PDX_LAUNCH_CHECK();
     }
@@ -997,6 +1000,12 @@ extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, 
     default: launch_model<PDX_PAIR_NONE>(coul, energy, np, G, dim3(blocks), st, a); break;
   }
   return finish();
+}
+}  // namespace
+
+extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, float *fz, float *Epot,
+                               float *V, double *totals, int32_t flags, pdx_stream_t stream) {
+  return pair_forces_impl(p, nl, fx, fy, fz, Epot, V, totals, flags, stream, nullptr, 0);
 }
 
 extern "C" int pdx_pair_set_accumulation(pdx_pair_t p, int32_t mode) {

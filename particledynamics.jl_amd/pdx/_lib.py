@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libpdx.so")
@@ -174,3 +175,10 @@ def stream_handle(stream=None):
     import torch
     s = stream if stream is not None else torch.cuda.current_stream()
     return C.c_void_p(s.cuda_stream)
+
+
+def adopt(owner, handle, destroy):
+    """Destroy ``handle`` with ``destroy`` when ``owner`` is collected, or at interpreter exit
+    while the library is still loaded (weakref.finalize runs its exit hook before modules are
+    torn down, so no handle outlives the process unreleased)."""
+    return weakref.finalize(owner, destroy, handle)

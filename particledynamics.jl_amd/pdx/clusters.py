@@ -62,6 +62,7 @@ class ClusterPairEngine:
         L.check(L.lib.pdx_cluster_create(int(n), L.dbl3(box), pair._h, float(skin), float(prune_buffer),
                                          int(capacity), int(waves), C.byref(h)))
         self._h = h
+        self._release = L.adopt(self, h, L.lib.pdx_cluster_destroy)
         self.pair = pair  # keeps the model alive
         self.N = int(n)
         self.box = tuple(float(b) for b in box)
@@ -78,11 +79,6 @@ class ClusterPairEngine:
         L.check(L.lib.pdx_cluster_arrays_get(self._h, C.byref(a)))
         self._arrays = a
 
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            L.lib.pdx_cluster_destroy(h)
-            self._h = None
 
     # ---- enqueue (never synchronises)
     def update(self, x, y, z, q=None, stream=None):

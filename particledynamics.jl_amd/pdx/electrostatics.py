@@ -138,6 +138,7 @@ class ElectrostaticsSolver:
         cs = spec.c_spec()
         L.check(L.lib.pdx_mesh_create(C.byref(cs), self.N, L.dbl3(box), C.byref(h)))
         self._h = h
+        self._release = L.adopt(self, h, L.lib.pdx_mesh_destroy)
         self.box = tuple(float(b) for b in box)
         self.set_charges(q)
         info = (C.c_double * 9)()
@@ -148,11 +149,6 @@ class ElectrostaticsSolver:
         self.C = info[5]
         self.plane_lines = int(info[8])
 
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            L.lib.pdx_mesh_destroy(h)
-            self._h = None
 
     @property
     def handle(self):

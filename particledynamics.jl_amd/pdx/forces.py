@@ -95,17 +95,13 @@ class FENEBonds:
         L.check(L.lib.pdx_bonds_create(int(n), len(b), bi.ctypes.data_as(P), bj.ctypes.data_as(P), float(k),
                                        float(R0), C.byref(h)))
         self._h = h
+        self._release = L.adopt(self, h, L.lib.pdx_bonds_destroy)
         self.n, self.nbond, self.k, self.R0 = int(n), len(b), float(k), float(R0)
 
     @property
     def handle(self):
         return self._h
 
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            L.lib.pdx_bonds_destroy(h)
-            self._h = None
 
     def status(self, reset=True, stream=None):
         """Raise ``DomainError`` naming the first particle whose bond left the FENE domain
@@ -140,6 +136,7 @@ class PairForce:
         h = C.c_void_p()
         L.check(L.lib.pdx_pair_create(C.byref(spec), C.byref(h)))
         self._h = h
+        self._release = L.adopt(self, h, L.lib.pdx_pair_destroy)
         self.model = model
         self.coulomb = coulomb
         self.bonds = None
@@ -186,11 +183,6 @@ class PairForce:
         L.check(L.lib.pdx_pair_set_bonds(self._h, bonds.handle if bonds is not None else None))
         self.bonds = bonds  # kept alive while attached
 
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            L.lib.pdx_pair_destroy(h)
-            self._h = None
 
     def info(self):
         out = (C.c_double * 4)()

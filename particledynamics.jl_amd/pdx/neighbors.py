@@ -46,6 +46,7 @@ class DeviceNeighborWorkspace:
         L.check(L.lib.pdx_nl_create(int(N), L.dbl3(box), float(cutoff), float(skin), int(cap), int(interval),
                                     flags, C.byref(h)))
         self._h = h
+        self._release = L.adopt(self, h, L.lib.pdx_nl_destroy)
         self.N = int(N)
         self.box = tuple(float(b) for b in box)
         self.cutoff = float(cutoff)
@@ -65,11 +66,6 @@ class DeviceNeighborWorkspace:
     def handle(self):
         return self._h
 
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            L.lib.pdx_nl_destroy(h)
-            self._h = None
 
     # ---- enqueue (never synchronises)
     def update(self, x, y, z, q=None, stream=None):

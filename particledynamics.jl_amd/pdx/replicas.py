@@ -70,6 +70,7 @@ class ReplicaBatch:
         h = C.c_void_p()
         L.check(L.lib.pdx_replicas_create(len(specs), arr, int(block), int(stencil), C.byref(h)))
         self._h = h
+        self._release = L.adopt(self, h, L.lib.pdx_replicas_destroy)
         self.specs = list(specs)
         self.nrep = len(specs)
         off = (C.c_int64 * (self.nrep + 1))()
@@ -77,11 +78,6 @@ class ReplicaBatch:
         self.offsets = np.array(off[:], dtype=np.int64)
         self.total = int(self.offsets[-1])
 
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            L.lib.pdx_replicas_destroy(h)
-            self._h = None
 
     def info(self):
         o = (C.c_int64 * 8)()

@@ -67,13 +67,30 @@ def maxwell_velocities(N, temperature, mass=1.0, seed=0):
     return v.astype(np.float32)
 
 
+def _fcc_counts(n):
+    """FCC cell counts (mx, my, mz) of a lattice with the fewest sites 4 mx my mz >= n (counts
+    within one of (n/4)^(1/3)): n sites drawn from it sit at nearly the target density, not on a
+    lattice up to twice as dense as a cube of m^3 cells would give."""
+    c = (n / 4.0) ** (1.0 / 3.0)
+    lo, hi = max(1, int(np.floor(c)) - 1), int(np.ceil(c)) + 1
+    best = None
+    for mx in range(lo, hi + 1):
+        for my in range(lo, hi + 1):
+            for mz in range(lo, hi + 1):
+                M = 4 * mx * my * mz
+                key = (M, max(mx, my, mz) - min(mx, my, mz))
+                if M >= n and (best is None or key < best[0]):
+                    best = (key, (mx, my, mz))
+    return best[1]
+
+
 def replica_batch(nrep=512, seed=0, n_range=(256, 8192), density_range=(0.70, 0.90), temperature=0.72,
                   models=("wca", "lj"), jitter=0.05):
     """Seeded batch of independent LJ / WCA boxes (the batched-replica config): per replica,
     n log-uniform in ``n_range``, density uniform in ``density_range``, the model drawn
     uniformly from ``models`` (LJ: eps = sigma = 1, rc = 2.5, energy shifted; WCA: eps =
-    sigma = 1), atoms on a seeded random subset of the sites of the smallest FCC lattice
-    that holds them, Gaussian jitter, Maxwell velocities at ``temperature`` with zero net
+    sigma = 1), atoms on a seeded random subset of the sites of the FCC lattice with the fewest
+    sites that holds them (at most a few percent denser than the target), Gaussian jitter, Maxwell velocities at ``temperature`` with zero net
     momentum. Returns specs (for ``ReplicaBatch``), concatenated FP32 x, y, z, vx, vy, vz
     and the offsets."""
     rng = np.random.default_rng(seed)
@@ -85,11 +102,10 @@ def replica_batch(nrep=512, seed=0, n_range=(256, 8192), density_range=(0.70, 0.
         rho = float(rng.uniform(*density_range))
         model = models[int(rng.integers(len(models)))]
         Lb = (n / rho) ** (1.0 / 3.0)
-        m = 1
-        while 4 * m ** 3 < n:
-            m += 1
-        g = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1).reshape(-1, 3)
-        sites = ((g[:, None, :] + basis[None, :, :]).reshape(-1, 3) + 0.25) * (Lb / m) - Lb / 2
+        mx, my, mz = _fcc_counts(n)
+        g = np.stack(np.meshgrid(np.arange(mx), np.arange(my), np.arange(mz), indexing="ij"), -1).reshape(-1, 3)
+        cell = Lb / np.array([mx, my, mz], dtype=np.float64)
+        sites = ((g[:, None, :] + basis[None, :, :]).reshape(-1, 3) + 0.25) * cell - Lb / 2
         sites = sites[np.sort(rng.choice(len(sites), n, replace=False))]
         pos = _wrap(sites + rng.normal(0.0, jitter, size=sites.shape), Lb)
         v = rng.normal(0.0, np.sqrt(temperature), size=(n, 3))

@@ -1,0 +1,158 @@
+"""The benchmark configurations at their own sizes, against the oracles:
+  * the 64 000-ion jittered rock salt with LJ + SPME 120^3, P = 5, rc 9 A (the ionic workload):
+    every ion against the FP64 CPU restatement of the same algorithm, and sampled ions against
+    the independent FP64 oracles (LJ direct sums + Ewald);
+  * the 1 048 600-bead Kremer-Grest melt, all three accumulation modes, every bead against the
+    FP64 oracle;
+  * the 512-box replica batch, 1000 velocity-Verlet steps in one launch: boxes run alone
+    (the largest ones included) are bitwise equal to their batch members, and the replica with
+    the largest energy drift follows the FP64 Verlet oracle."""
+import math
+import time
+
+import numpy as np
+import pytest
+
+import mesh as cpu_mesh
+import oracle
+import pdx
+from conftest import dev
+from pdx import melt, replicas, systems
+
+pytestmark = pytest.mark.gpu
+C_ION = 14.399645
+
+
+def _relrms(a, b):
+    return math.sqrt(((a - b) ** 2).sum() / (b ** 2).sum())
+
+
+def test_rock_salt_64k_lj_spme(cuda):
+    import torch
+    s = systems.rock_salt(40, 2.82, 0.1, seed=0)
+    N, box = s["N"], s["box"]
+    spec = pdx.PME(coulomb_prefactor=C_ION, real_cutoff=9.0, real_space_tolerance=1e-5, mesh_spacing=1.0, order=5)
+    lj = pdx.LennardJones(epsilon=0.01, sigma=2.8, cutoff=9.0)
+    eng = pdx.ForceEngine(N, box, lj, electrostatics=spec, charges=s["q"], skin=1.0)
+    assert tuple(eng.solver.mesh) == (120, 120, 120)
+    X, Y, Z, Q = (dev(s[k]) for k in ("x", "y", "z", "q"))
+    f = [torch.empty(N, device="cuda") for _ in range(3)]
+    E = torch.empty(N, device="cuda")
+    V = torch.empty(6, N, device="cuda")
+    eng.evaluate(X, Y, Z, Q, *f, E, V, energy=True)
+    eng.evaluate(X, Y, Z, Q, *f, E, V, energy=True)  # the engine's steady path (snapshot spread)
+    tot = eng.totals()
+    F = torch.stack(f, 1).double().cpu().numpy()
+    x, y, z, q = (s[k].astype(np.float64) for k in ("x", "y", "z", "q"))
+    alpha = eng.solver.alpha
+    t0 = time.perf_counter()
+    ref = cpu_mesh.evaluate(x, y, z, q, box, C_ION, alpha, 9.0, (120, 120, 120), 5,
+                            lj=dict(kind="lj", sigma=2.8, epsilon=0.01, cutoff=9.0))
+    t_cpu = time.perf_counter() - t0
+    Wr = ref["W"]
+    dF = _relrms(F, ref["F"])
+    dE = abs(tot["energy"] - ref["E"]) / abs(ref["E"])
+    dW = np.abs(np.array(tot["virial"]) - Wr).max() / abs(Wr[:3].sum())
+    dEp = np.abs(E.double().cpu().numpy() - ref["Epot"]).max() / np.abs(ref["Epot"]).max()
+    print(f"[c2] vs FP64 same algorithm ({t_cpu:.1f} s CPU): dF={dF:.2e} dE={dE:.2e} dW={dW:.2e} "
+          f"dEpot={dEp:.2e}", flush=True)
+    assert dF < 3e-6 and dE < 1e-6 and dW < 1e-5 and dEp < 1e-5
+    # sampled ions against the independent oracles: LJ direct sums + FP64 Ewald
+    ids = np.sort(np.random.default_rng(1).choice(N, 256, replace=False))
+    Flj = oracle.pair_direct(x, y, z, box, kind="lj", sigma=2.8, epsilon=0.01, cutoff=9.0, ids=ids)["F"]
+    Fc = oracle.ewald_sampled(x, y, z, q, box, C_=C_ION, alpha=0.15, digits=12.0, ids=ids)
+    Fs = F[ids]
+    e_total = _relrms(Fs, Flj + Fc)
+    e_coul = _relrms(Fs - Flj, Fc)  # the Coulomb part: the method's error at this mesh and order
+    print(f"[c2] vs FP64 Ewald + LJ direct (256 ions): total {e_total:.2e}, Coulomb {e_coul:.2e}", flush=True)
+    assert e_coul < 6e-4
+    assert e_total < 6e-4
+
+
+@pytest.fixture(scope="module")
+def kg_full():
+    sysd = systems.kg_chains(10486, length=100, density=0.85, seed=0)
+    X, Y, Z, bonds = melt.push_off(sysd, iterations=300)
+    ref = oracle.kg_forces(sysd["x"], sysd["y"], sysd["z"], sysd["box"], sysd["bonds"])
+    return sysd, (X, Y, Z, bonds), ref
+
+
+def test_kg_melt_full_size(cuda, kg_full):
+    """1 048 600 beads: per accumulation mode, forces on every bead within the reference's FP32
+    bound (2e-5 relative RMS, test/test_device_engine_pairs_gpu.jl:124-125), FP64 energy to 1e-6
+    and virial to 1e-5; 64-bit fixed point and FP64 sums agree to FP32 rounding of the sum."""
+    import torch
+    sysd, (X, Y, Z, bonds), ref = kg_full
+    N, box = sysd["N"], sysd["box"]
+    assert N == 1048600
+    Fn = np.linalg.norm(ref["F"])
+    got = {}
+    for mode in ("fp32", "fixed64", "fp64"):
+        eng = pdx.ForceEngine(N, box, pdx.WCA(epsilon=1.0, sigma=1.0), skin=0.4, accumulation=mode, bonds=bonds,
+                              cap=2 * pdx.default_capacity(N, box, 2 ** (1 / 6) + 0.4))
+        f = [torch.empty(N, device="cuda") for _ in range(3)]
+        E = torch.empty(N, device="cuda")
+        V = torch.empty(6, N, device="cuda")
+        eng.evaluate(X, Y, Z, None, *f, E, V, energy=True)
+        tot = eng.totals()
+        F = torch.stack(f, 1).double().cpu().numpy()
+        err = np.linalg.norm(F - ref["F"]) / Fn
+        dE = abs(tot["pair_energy"] - ref["energy"]) / abs(ref["energy"])
+        dW = np.linalg.norm(np.array(tot["pair_virial"]) - ref["W"]) / np.linalg.norm(ref["W"])
+        print(f"[kg full] {mode}: force {err:.3e} energy {dE:.2e} virial {dW:.2e}", flush=True)
+        assert err < 2e-5 and dE < 1e-6 and dW < 1e-5, mode
+        got[mode] = F
+        bonds.status()
+        del eng
+    assert np.abs(got["fixed64"] - got["fp64"]).max() <= 1e-6 * np.abs(got["fp64"]).max()
+
+
+def test_replica_batch_full_size(cuda):
+    """512 boxes (n log-uniform in [256, 8192]), 1000 steps in one launch; the three largest and
+    three random boxes re-run alone give bitwise the batch's positions, velocities, forces and
+    energies; the box with the largest energy drift follows the FP64 Verlet oracle for 100 steps
+    and drifts, over the 1000 steps, as the FP64 integration of the first 100 steps suggests
+    (at most 2e-3 per atom in total energy)."""
+    import torch
+    b = systems.replica_batch(512, seed=0)
+    rb = pdx.ReplicaBatch(b["specs"])
+    st = replicas.device_state(b)
+    init = replicas.clone_state(st)
+    e0 = rb.forces(st).cpu().numpy()
+    out = rb.run(st, 1000, dt=0.005, energy=True).cpu().numpy()
+    ns = np.array([sp["n"] for sp in b["specs"]])
+    assert ns.min() >= 256 and ns.max() <= 8192 and ns.max() > 7000
+    drift = np.abs((out[:, 0] + out[:, 1]) - (e0[:, 0] + e0[:, 1])) / ns
+    print(f"[replicas full] atoms {ns.sum()} drift per atom: max {drift.max():.2e} median {np.median(drift):.2e}",
+          flush=True)
+    assert drift.max() < 2e-3
+    rng = np.random.default_rng(4)
+    sample = list(np.argsort(ns)[-3:]) + list(rng.choice(512, 3, replace=False))
+    for k in sample:
+        s = b["specs"][k]
+        a, e = rb.offsets[k], rb.offsets[k + 1]
+        one = pdx.ReplicaBatch([s])
+        st1 = {n: v[a:e].clone() for n, v in init.items()}
+        one.forces(st1)
+        o1 = one.run(st1, 1000, dt=0.005, energy=True).cpu().numpy()
+        for n in st:
+            assert torch.equal(st[n][a:e], st1[n]), (k, n)
+        assert np.array_equal(out[k], o1[0]), k
+    # the worst-drift replica against FP64 velocity Verlet over 100 steps
+    k = int(np.argmax(drift))
+    s = b["specs"][k]
+    a, e = rb.offsets[k], rb.offsets[k + 1]
+    h0 = {n: v[a:e].cpu().numpy().astype(np.float64) for n, v in init.items()}
+    one = pdx.ReplicaBatch([s])
+    st1 = {n: v[a:e].clone() for n, v in init.items()}
+    one.forces(st1)
+    one.run(st1, 100, dt=0.005)
+    Xr, Vr, *_ = oracle.vv_steps(h0["x"], h0["y"], h0["z"], h0["vx"], h0["vy"], h0["vz"], s["box"], kind=s["model"],
+                                 cutoff=s.get("cutoff", 2.5), dt=0.005, nsteps=100)
+    L = s["box"][0]
+    d = np.stack([st1[c].cpu().numpy() for c in ("x", "y", "z")], 1) - Xr
+    d -= L * np.round(d / L)
+    dv = np.stack([st1[c].cpu().numpy() for c in ("vx", "vy", "vz")], 1) - Vr
+    print(f"[replicas full] worst drift box {k}: {s['model']} n={s['n']} rho={s['density']:.3f} "
+          f"max|dx|={np.abs(d).max():.2e} max|dv|={np.abs(dv).max():.2e}", flush=True)
+    assert np.abs(d).max() < 1e-3 and np.abs(dv).max() < 1e-2

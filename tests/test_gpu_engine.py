@@ -62,3 +62,67 @@ def test_energy_total_consistent(cuda):
     # per-particle energies (pair halves + self + mesh halves) sum to the FP64 total
     assert abs(got[3].double().sum().item() - E) < 1e-5 * abs(E)
     assert np.isfinite(W).all()
+
+
+def _fresh(s, traversal="rows"):
+    import torch
+    spec = pdx.PME(coulomb_prefactor=14.399645, real_cutoff=6.0, real_space_tolerance=1e-5, mesh_spacing=1.0,
+                   order=5)
+    eng = pdx.ForceEngine(s["N"], s["box"], pdx.LennardJones(0.01, 2.8, 6.0), electrostatics=spec,
+                          charges=s["q"], skin=0.5, traversal=traversal)
+    bufs = [torch.empty(s["N"], device="cuda") for _ in range(4)] + [torch.empty(6, s["N"], device="cuda")]
+    return eng, bufs
+
+
+def _unrelated(seed):
+    # another jittered crystal, translated by several lattice spacings: every ion is far from
+    # where the previous build binned it, and no two ions come close
+    s = systems.rock_salt(10, 2.82, 0.1, seed=seed)
+    L = s["box"][0]
+    for k, d in zip("xyz", (9.3, -7.1, 11.4)):
+        r = s[k].astype(np.float64) + d
+        s[k] = (r - L * np.floor(r / L + 0.5)).astype(np.float32)
+    return s
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_engine_on_unrelated_configuration(cuda, graph):
+    """One engine evaluated on configuration A, then on an unrelated configuration B, gives B's
+    forces, energy and virial: the spread from the cells of A's build (valid only while every
+    particle moved less than the skin since it) is replaced by the per-particle spread and gather
+    for that evaluation. Compared with a fresh engine on B."""
+    import torch
+    a = systems.rock_salt(10, 2.82, 0.1, seed=0)
+    b = _unrelated(7)
+    eng, bufs = _fresh(a)
+    PA = [dev(a[k]) for k in ("x", "y", "z", "q")]
+    PB = [dev(b[k]) for k in ("x", "y", "z", "q")]
+    work = [t.clone() for t in PA]
+    eng.evaluate(*work, *bufs, energy=True)
+    eng.evaluate(*work, *bufs, energy=True)  # the second check takes the snapshot of A's build
+    if graph:
+        eng.capture(*work, *bufs, energy=True)
+        eng.replay()
+    torch.cuda.synchronize()
+    for t, u in zip(work, PB):
+        t.copy_(u)
+    for rep in range(2):  # B with A's cells (stale snapshot), then B with B's own snapshot
+        if graph:
+            eng.replay()
+        else:
+            eng.evaluate(*work, *bufs, energy=True)
+        got, E, W = _snap(eng, bufs)
+        ref_eng, ref_bufs = _fresh(b)
+        for _ in range(rep + 1):
+            ref_eng.evaluate(*PB, *ref_bufs, energy=True)
+        ref, Eref, Wref = _snap(ref_eng, ref_bufs)
+        F = torch.stack(got[:3]).double()
+        Fr = torch.stack(ref[:3]).double()
+        err = ((F - Fr).norm() / Fr.norm()).item()
+        assert err < 1e-6, f"evaluation {rep}: force rel. difference {err:.3e}"
+        assert abs(E - Eref) < 1e-7 * abs(Eref), (rep, E, Eref)
+        for w, wr in zip(W, Wref):
+            assert abs(w - wr) < 1e-6 * max(abs(wr), abs(Eref)), (rep, W, Wref)
+        if rep == 1:  # both spread from a valid snapshot of B's build: the same kernels, bitwise
+            for x, y in zip(got, ref):
+                assert torch.equal(x, y)

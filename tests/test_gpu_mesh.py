@@ -7,6 +7,7 @@ import math
 import numpy as np
 import pytest
 
+import mesh as cpu_mesh
 import oracle
 import pdx
 from conftest import dev
@@ -329,8 +330,11 @@ def test_p3m_against_ewald(cuda, diff):
 
 def test_optimal_influence_on_a_coarse_mesh(cuda):
     """On a coarse mesh the ik-optimal influence function gives a far lower force error than
-    SPME with the same B-spline assignment, and the AD-optimal one about the same (SPME and
-    P3M-AD are known to be close; Ballenegger, Cerda & Holm 2012)."""
+    SPME with the same B-spline assignment. The AD-optimal one is ~12 % worse than SPME on this
+    configuration (and ~10 % worse averaged over 40 random 120-charge clouds): that is the
+    method's own error, not the device's. The FP64 CPU restatement of the same algorithm
+    (oracle/mesh.py, whose alias sums are converged at order 2: order 3 changes the error by
+    < 1e-5 relative) gives the same errors, so the device errors must equal the FP64 ones to 1 %."""
     R, q = systems.random_cloud(120, L_ES, seed=3)
     R32 = R.astype(np.float32).astype(np.float64)
     q32 = q.astype(np.float32).astype(np.float64)
@@ -343,9 +347,15 @@ def test_optimal_influence_on_a_coarse_mesh(cuda):
                             differentiation="ik"))):
         _, F, _, _ = _es_eval(R, q, L_ES, spec)
         errs[name] = _relrms(F, Fref)
-    print(f"[influence] coarse-mesh force errors {errs}", flush=True)
+    cpu = {}
+    for name in ("spme", "ad"):
+        o = cpu_mesh.evaluate(R32[:, 0], R32[:, 1], R32[:, 2], q32, L_ES, 1.7, 0.9, 4.0, (20, 20, 24), 4,
+                              influence="spme" if name == "spme" else "p3m_ad")
+        cpu[name] = _relrms(o["F"], Fref)
+    print(f"[influence] coarse-mesh force errors {errs}, FP64 same algorithm {cpu}", flush=True)
     assert errs["ik"] < 0.5 * errs["spme"]
-    assert errs["ad"] < 1.25 * errs["spme"]
+    for name in ("spme", "ad"):
+        assert abs(errs[name] - cpu[name]) < 0.01 * cpu[name], name
 
 
 def test_exceptions_against_ewald(cuda):
@@ -414,3 +424,97 @@ def test_device_ewald_matches_oracle(cuda):
     assert np.allclose([W[0], W[1], W[2], W[3], W[4], W[5]],
                        [Wref[0, 0], Wref[1, 1], Wref[2, 2], Wref[0, 1], Wref[0, 2], Wref[1, 2]],
                        rtol=1e-9, atol=1e-10 * abs(np.trace(Wref)))
+
+
+def _cpu_vs_device(R, q, L, C, alpha, rc, mesh, P, influence, background=False):
+    R32 = R.astype(np.float32).astype(np.float64)
+    q32 = np.asarray(q, dtype=np.float32).astype(np.float64)
+    if influence == "spme":
+        spec = pdx.PME(coulomb_prefactor=C, real_cutoff=rc, alpha=alpha, mesh=mesh, order=P,
+                       net_charge="background" if background else "error")
+    else:
+        spec = pdx.PPPM(coulomb_prefactor=C, real_cutoff=rc, alpha=alpha, mesh=mesh, order=P,
+                        net_charge="background" if background else "error")
+    E, F, W, Ep = _es_eval(R, q, L, spec)
+    ref = cpu_mesh.evaluate(R32[:, 0], R32[:, 1], R32[:, 2], q32, L, C, alpha, rc, mesh, P,
+                            influence="spme" if influence == "spme" else "p3m_ad", background=background)
+    return (E, F, W, Ep), ref
+
+
+@pytest.mark.parametrize("influence", ["spme", "ad"])
+@pytest.mark.parametrize("case", ["cloud120", "strain80"])
+def test_mesh_equals_fp64_same_algorithm(cuda, influence, case):
+    """The device evaluation (FP32 arithmetic, fixed-point spread, FP64 energy/virial sums) against
+    the FP64 CPU restatement of the same algorithm at equal mesh, order and alpha (oracle/mesh.py,
+    itself pinned against FP64 Ewald and by F = -dE/dx, W_aa = -dE/d eps_aa): forces to 3e-6
+    relative RMS, energy to 1e-6, virial to 1e-5 of its trace, per-particle energies to 1e-5."""
+    if case == "cloud120":
+        R, q = systems.random_cloud(120, L_ES, seed=3)
+        L, C, alpha, rc, mesh, P = L_ES, 1.7, 0.9, 4.0, (48, 48, 54), 6
+    else:  # the reference's derivative / strain case (test/test_electrostatics.jl:341-375)
+        L = (13.0, 13.5, 14.0)
+        R, q = systems.random_cloud(80, L, seed=9)
+        C, alpha, rc, mesh, P = 1.0, 0.9, 3.8, (24, 25, 27), 5
+    (E, F, W, Ep), ref = _cpu_vs_device(R, q, L, C, alpha, rc, mesh, P, influence)
+    Wr = ref["W"]
+    Wrm = np.array([[Wr[0], Wr[3], Wr[4]], [Wr[3], Wr[1], Wr[5]], [Wr[4], Wr[5], Wr[2]]])
+    dF = _relrms(F, ref["F"])
+    dE = abs(E - ref["E"]) / abs(ref["E"])
+    dW = np.abs(W - Wrm).max() / abs(np.trace(Wrm))
+    print(f"[fp64 same algorithm] {influence} {case}: dF={dF:.2e} dE={dE:.2e} dW={dW:.2e}", flush=True)
+    assert dF < 3e-6
+    assert dE < 1e-6
+    assert dW < 1e-5
+    assert np.abs(Ep - ref["Epot"]).max() < 1e-5 * np.abs(ref["Epot"]).max()
+
+
+def test_net_charge_background_and_zero_charges(cuda):
+    """Net charge (test/test_electrostatics.jl:398-415): refused by default; with the uniform
+    neutralising background, energy and forces against FP64 Ewald with background < 3e-5 and the
+    virial < 3e-4 of its trace, the background energy -C pi Q^2 / (2 alpha^2 V) to 1e-12; the
+    per-particle energies carry the background share. All charges zero: no force, energy or virial."""
+    import torch
+    C, rc = 1.3, 4.0
+    R, q = systems.random_cloud(100, L_ES, seed=13)
+    qn = q + 0.05
+    with pytest.raises(pdx.ArgumentError, match="net charge"):
+        pdx.ForceEngine(100, L_ES, None, electrostatics=pdx.PME(coulomb_prefactor=C, real_cutoff=rc, alpha=0.9,
+                                                                 mesh=(48, 48, 54), order=6), charges=qn)
+    R32 = R.astype(np.float32).astype(np.float64)
+    q32 = qn.astype(np.float32).astype(np.float64)
+    Eb, Fb, Wb = oracle.ewald(R32, q32, np.diag(L_ES), C_=C, background=True)
+    spec = pdx.PME(coulomb_prefactor=C, real_cutoff=rc, alpha=0.9, mesh=(48, 48, 54), order=6,
+                   net_charge="background")
+    N = 100
+    eng = pdx.ForceEngine(N, L_ES, None, electrostatics=spec, charges=qn, skin=0.3,
+                          cap=max(64, pdx.default_capacity(N, L_ES, 4.4)))
+    X, Y, Z, Q = dev(R[:, 0]), dev(R[:, 1]), dev(R[:, 2]), dev(qn)
+    fx, fy, fz, E = (torch.empty(N, device="cuda") for _ in range(4))
+    eng.evaluate(X, Y, Z, Q, fx, fy, fz, E, None, energy=True)
+    tot = eng.totals()
+    F = torch.stack([fx, fy, fz], 1).double().cpu().numpy()
+    W = np.array(tot["virial"])
+    Wm = np.array([[W[0], W[3], W[4]], [W[3], W[1], W[5]], [W[4], W[5], W[2]]])
+    print(f"[background] dE={abs(tot['energy'] - Eb) / abs(Eb):.2e} dF={_relrms(F, Fb):.2e} "
+          f"dW={np.abs(Wm - Wb).max() / abs(np.trace(Wb)):.2e}", flush=True)
+    assert abs(tot["energy"] - Eb) / abs(Eb) < 3e-5
+    assert _relrms(F, Fb) < 3e-5
+    assert np.abs(Wm - Wb).max() / abs(np.trace(Wb)) < 3e-4
+    Q32 = float(np.float32(qn).astype(np.float64).sum())
+    ubg = -C * math.pi * Q32 ** 2 / (2 * 0.81 * np.prod(L_ES))
+    assert tot["background_energy"] == pytest.approx(ubg, rel=1e-12)
+    assert abs(E.double().sum().item() - tot["energy"]) < 1e-5 * abs(tot["energy"])
+    # the same against the FP64 same-algorithm evaluation with background
+    ref = cpu_mesh.evaluate(R32[:, 0], R32[:, 1], R32[:, 2], q32, L_ES, C, 0.9, rc, (48, 48, 54), 6, background=True)
+    assert abs(tot["energy"] - ref["E"]) < 1e-6 * abs(ref["E"])
+    assert _relrms(F, ref["F"]) < 3e-6
+    # zero charges
+    z = np.zeros(N)
+    eng0 = pdx.ForceEngine(N, L_ES, None, electrostatics=pdx.PME(coulomb_prefactor=C, real_cutoff=rc, alpha=0.9,
+                                                                  mesh=(48, 48, 54), order=6), charges=z,
+                           skin=0.3, cap=max(64, pdx.default_capacity(N, L_ES, 4.4)))
+    eng0.evaluate(X, Y, Z, dev(z), fx, fy, fz, E, None, energy=True)
+    t0 = eng0.totals()
+    assert t0["energy"] == 0 and all(w == 0 for w in t0["virial"])
+    for t in (fx, fy, fz, E):
+        assert torch.count_nonzero(t).item() == 0

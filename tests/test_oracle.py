@@ -347,3 +347,201 @@ def test_kg_oracle_gradient_and_strain():
         bm = tuple(np.array(box) * s)
         em = oracle.kg_forces(*(R * s).T, bm, bonds)["energy"]
         assert abs(-(ep - em) / (2 * h) - d["W"][a]) < 1e-5 * max(1.0, abs(d["W"][a]))
+
+
+# ---------------------------------------------------------------- same-algorithm CPU evaluation
+import mesh as cpu_mesh  # noqa: E402  (oracle/mesh.py)
+
+L_ES = (14.0, 15.0, 16.0)
+
+
+def _cloud(n, L, seed):
+    rng = np.random.default_rng(seed)
+    R = rng.random((n, 3)) * np.asarray(L) - np.asarray(L) / 2
+    q = rng.standard_normal(n)
+    q -= q.mean()
+    return R, q
+
+
+def test_cell_pairs_equal_direct_sums():
+    """The cell-list pair sums equal the brute-force direct sums (LJ shifted and not, WCA)."""
+    rng = np.random.default_rng(5)
+    n, L = 500, 9.0
+    P = rng.random((n, 3)) * L - L / 2
+    for kind, shift in (("lj", False), ("lj", True), ("wca", False)):
+        c = oracle.cell_pairs(P[:, 0], P[:, 1], P[:, 2], None, (L, L, L), kind=kind, cutoff=2.5, shift=shift)
+        d = oracle.pair_direct(P[:, 0], P[:, 1], P[:, 2], (L, L, L), kind=kind, cutoff=2.5)
+        scale = np.abs(d["F"]).max()
+        assert np.abs(c["F"] - d["F"]).max() < 1e-12 * scale
+        E = d["energy"]
+        if kind == "lj" and shift:  # the direct sums are unshifted: 1/2 u(rc) per pair end
+            s6 = (1 / 2.5) ** 6
+            E = E - 0.5 * 4 * (s6 * s6 - s6) * d["counts"]
+        assert np.abs(c["E"] - E).max() < 1e-12 * np.abs(E).max()
+        assert np.abs(c["V"] - d["virial"]).max() < 1e-12 * np.abs(d["virial"]).max()
+
+
+def test_cell_pairs_coulomb_equals_brute_force():
+    """Real-space Ewald term against a numpy minimum-image erfc sum; F = -dE/dx."""
+    from scipy.special import erfc
+    R, q = _cloud(200, L_ES, 11)
+    C_, alpha, rc = 1.7, 0.9, 4.0
+    c = oracle.cell_pairs(R[:, 0], R[:, 1], R[:, 2], q, L_ES, C_=C_, alpha=alpha, rc_coul=rc)
+    L = np.asarray(L_ES)
+    d = R[:, None, :] - R[None, :, :]
+    d -= L * np.round(d / L)
+    r = np.sqrt((d ** 2).sum(-1))
+    np.fill_diagonal(r, np.inf)
+    inside = r < rc
+    rs = np.where(inside, r, 1.0)
+    qq = C_ * q[:, None] * q[None, :]
+    u = np.where(inside, qq * erfc(alpha * rs) / rs, 0.0)
+    a = np.where(inside, qq * (erfc(alpha * rs) / rs + 2 / math.sqrt(math.pi) * alpha
+                               * np.exp(-(alpha * rs) ** 2)) / rs ** 2, 0.0)
+    F = (a[:, :, None] * d).sum(1)
+    assert np.abs(c["F"] - F).max() < 1e-12 * np.abs(F).max()
+    assert np.abs(c["E"] - 0.5 * u.sum(1)).max() < 1e-12 * np.abs(u).max()
+    W = 0.5 * np.einsum("ij,ija,ijb->ab", a, d, d)
+    Vt = c["V"].sum(0)
+    assert np.allclose(Vt, [W[0, 0], W[1, 1], W[2, 2], W[0, 1], W[0, 2], W[1, 2]], rtol=1e-12, atol=1e-12 * abs(W).max())
+
+
+def test_cpu_spme_against_ewald():
+    """The CPU SPME (real space + mesh + self) on the reference's 120-charge case
+    (test/test_electrostatics.jl:299-340): force rel. RMS < 3e-5, energy < 1e-4."""
+    R, q = _cloud(120, L_ES, 3)
+    Er, Fr, Wr = oracle.ewald(R, q, np.diag(L_ES), C_=1.7)
+    out = cpu_mesh.evaluate(R[:, 0], R[:, 1], R[:, 2], q, L_ES, 1.7, 0.9, 4.0, (48, 48, 54), 6)
+    err = math.sqrt(((out["F"] - Fr) ** 2).sum() / (Fr ** 2).sum())
+    assert err < 3e-5
+    assert abs(out["E"] - Er) / abs(Er) < 1e-4
+    W = out["W"]
+    Wm = np.array([[W[0], W[3], W[4]], [W[3], W[1], W[5]], [W[4], W[5], W[2]]])
+    assert np.abs(Wm - Wr).max() / abs(np.trace(Wr)) < 3e-4
+    assert abs(out["Epot"].sum() - out["E"]) < 1e-10 * abs(out["E"])
+    # converged: order 8 on a finer mesh reaches the reference's "< 1e-6" (electrostatics.md:231)
+    fine = cpu_mesh.evaluate(R[:, 0], R[:, 1], R[:, 2], q, L_ES, 1.7, 0.9, 4.0, (72, 72, 80), 8)
+    assert math.sqrt(((fine["F"] - Fr) ** 2).sum() / (Fr ** 2).sum()) < 2e-6
+    # the spread conserves charge: the k = 0 sum of the grid is sum q
+    g = cpu_mesh.spread(R[:, 0], R[:, 1], R[:, 2], np.abs(q), L_ES, (48, 48, 54), 6)
+    assert abs(g.sum() - np.abs(q).sum()) < 1e-12 * np.abs(q).sum()
+
+
+@pytest.mark.parametrize("influence", ["spme", "p3m_ad"])
+def test_cpu_mesh_gradient_and_strain(influence):
+    """F = -dE/dx and W_aa = -dE/d eps_aa at fixed fractional coordinates for the discrete
+    energy of the same algorithm (test/test_electrostatics.jl:341-375), FP64."""
+    C_, rc, alpha, K, P = 1.0, 3.8, 0.9, (24, 25, 27), 5
+    L = (13.0, 13.5, 14.0)
+    R, q = _cloud(80, L, 9)
+
+    def ev(Rr, Ll):
+        return cpu_mesh.evaluate(Rr[:, 0], Rr[:, 1], Rr[:, 2], q, Ll, C_, alpha, rc, K, P, influence=influence)
+
+    base = ev(R, L)
+    Fmax = np.abs(base["F"]).max()
+    h = 1e-5
+    for i, d in ((2, 0), (39, 1), (76, 2)):
+        Rp = R.copy(); Rp[i, d] += h
+        Rm = R.copy(); Rm[i, d] -= h
+        fd = -(ev(Rp, L)["E"] - ev(Rm, L)["E"]) / (2 * h)
+        assert abs(fd - base["F"][i, d]) < 1e-6 * Fmax, (i, d, fd, base["F"][i, d])
+    W = base["W"]
+    tr = abs(W[0] + W[1] + W[2])
+    for a in range(3):
+        eps = 1e-5
+        Es = []
+        for e in (eps, -eps):
+            s = np.ones(3); s[a] += e
+            Es.append(ev(R * s, tuple(np.asarray(L) * s))["E"])
+        fd = -(Es[0] - Es[1]) / (2 * eps)
+        assert abs(fd - W[a]) < 1e-6 * max(abs(W[a]), tr), (a, fd, W[a])
+
+
+def test_p3m_ad_alias_convergence():
+    """The optimal AD influence function's alias sums converge: the force error against FP64
+    Ewald changes by < 1 % from alias order 2 to 3, and the order-2 error is the converged
+    method's error. On the reference's 120-charge case at a coarse mesh the converged AD error
+    is within 25 % of SPME's: both use the same B-spline assignment and analytic derivative, and
+    the optimal function minimises the error averaged over uncorrelated pair positions, not the
+    error of one configuration (Ballenegger, Cerda & Holm 2012, the AD and SPME curves)."""
+    R, q = _cloud(120, L_ES, 3)
+    _, Fr, _ = oracle.ewald(R, q, np.diag(L_ES), C_=1.7)
+
+    def err(influence, nmax=2):
+        o = cpu_mesh.evaluate(R[:, 0], R[:, 1], R[:, 2], q, L_ES, 1.7, 0.9, 4.0, (20, 20, 24), 4,
+                              influence=influence, alias_order=nmax)
+        return math.sqrt(((o["F"] - Fr) ** 2).sum() / (Fr ** 2).sum())
+
+    e1, e2, e3 = err("p3m_ad", 1), err("p3m_ad", 2), err("p3m_ad", 3)
+    spme = err("spme")
+    assert abs(e3 - e2) < 0.01 * e2
+    assert abs(e2 - e1) < 0.1 * e2
+    assert e2 < 1.25 * spme
+
+
+# ---------------------------------------------------------------- typed pair models
+def test_pair_typed_reduces_to_uniform():
+    """Mixed sigma with equal diameters (factor = rc / sigma) and a one-type matrix equal the
+    uniform direct sums; a two-type matrix of the Lorentz means equals the mixed-sigma model."""
+    rng = np.random.default_rng(8)
+    n, L = 400, 9.0
+    P = rng.random((n, 3)) * L - L / 2
+    for kind in ("lj", "wca"):
+        sig = 1.1
+        rc = 2.5 * sig if kind == "lj" else 1.2599211 ** 0.5 * sig
+        d = oracle.pair_direct(P[:, 0], P[:, 1], P[:, 2], (L, L, L), kind=kind, sigma=sig, cutoff=rc)
+        m = oracle.pair_typed((P[:, 0], P[:, 1], P[:, 2]), (L, L, L), kind=kind, style="mixed",
+                              sigma_particle=np.full(n, sig), factor=rc / sig)
+        t = oracle.pair_typed((P[:, 0], P[:, 1], P[:, 2]), (L, L, L), kind=kind, style="matrix",
+                              typeid=np.ones(n, np.int32), sigma_pair=[[sig]], epsilon_pair=[[1.0]], rcut_pair=[[rc]])
+        for r in (m, t):
+            assert np.allclose(r["F"], d["F"], rtol=1e-12, atol=1e-12 * np.abs(d["F"]).max())
+            assert np.allclose(r["energy"], d["energy"], rtol=1e-12, atol=1e-14)
+            assert np.allclose(r["virial"], d["virial"][:, :6], rtol=1e-12, atol=1e-12 * np.abs(d["virial"]).max())
+    typ = rng.integers(1, 3, n).astype(np.int32)
+    sp = np.array([0.9, 1.3])
+    S = 0.5 * (sp[:, None] + sp[None, :])
+    mixed = oracle.pair_typed((P[:, 0], P[:, 1], P[:, 2]), (L, L, L), kind="lj", style="mixed",
+                              sigma_particle=sp[typ - 1], factor=2.5)
+    mat = oracle.pair_typed((P[:, 0], P[:, 1], P[:, 2]), (L, L, L), kind="lj", style="matrix", typeid=typ,
+                            sigma_pair=S, epsilon_pair=np.ones((2, 2)), rcut_pair=2.5 * S)
+    assert np.allclose(mat["F"], mixed["F"], rtol=1e-12, atol=1e-12 * np.abs(mixed["F"]).max())
+
+
+def test_pair_typed_dimer_and_gradient():
+    """A dimer at the LJ minimum 2^(1/6) sigma_ij has zero force; F = -dE/dx and Newton's third
+    law on an asymmetric matrix of types; 2-D equals 3-D with all z equal."""
+    sp = np.array([0.8, 1.4])
+    rmin = 2 ** (1 / 6) * 0.5 * (sp[0] + sp[1])
+    r = oracle.pair_typed((np.array([0.0, rmin]), np.zeros(2), np.zeros(2)), (10.0, 10.0, 10.0), kind="lj",
+                          style="mixed", sigma_particle=sp, factor=2.5)
+    assert np.abs(r["F"]).max() < 1e-12
+    rng = np.random.default_rng(2)
+    n, L = 60, 6.0
+    P = rng.random((n, 3)) * L - L / 2
+    typ = rng.integers(1, 4, n).astype(np.int32)
+    Sg = np.array([[1.0, 0.9, 1.2], [0.9, 0.8, 1.1], [1.2, 1.1, 1.3]])
+    Ep = np.array([[1.0, 0.5, 2.0], [0.5, 1.5, 0.7], [2.0, 0.7, 1.2]])
+    Rc = 2.2 * Sg
+
+    def total(Q):
+        o = oracle.pair_typed((Q[:, 0], Q[:, 1], Q[:, 2]), (L, L, L), kind="lj", style="matrix", typeid=typ,
+                              sigma_pair=Sg, epsilon_pair=Ep, rcut_pair=Rc)
+        return o
+
+    base = total(P)
+    assert np.abs(base["F"].sum(0)).max() < 1e-9 * np.abs(base["F"]).max()
+    h = 1e-6
+    for i, d in ((3, 0), (17, 1), (44, 2)):
+        Pp = P.copy(); Pp[i, d] += h
+        Pm = P.copy(); Pm[i, d] -= h
+        fd = -(total(Pp)["energy"].sum() - total(Pm)["energy"].sum()) / (2 * h)
+        assert abs(fd - base["F"][i, d]) < 1e-6 * np.abs(base["F"]).max()
+    P2 = P.copy()
+    P2[:, 2] = 0.0
+    a = oracle.pair_typed((P2[:, 0], P2[:, 1]), (L, L), kind="wca", style="mixed", sigma_particle=sp[typ % 2],
+                          factor=1.2599211 ** 0.5)
+    b = oracle.pair_typed((P2[:, 0], P2[:, 1], P2[:, 2]), (L, L, L), kind="wca", style="mixed",
+                          sigma_particle=sp[typ % 2], factor=1.2599211 ** 0.5)
+    assert np.allclose(a["F"], b["F"][:, :2], rtol=1e-13, atol=1e-13) and np.allclose(a["energy"], b["energy"])
