@@ -72,12 +72,22 @@ extern "C" {
 /* ---- neighbor-list flags ---- */
 #define PDX_NL_ALLPAIRS     1  /* rows from all pairs, one image (DenseNeighborCUDA.jl:128-151) */
 #define PDX_NL_DIAGNOSTICS  2  /* count candidates / accepted pairs (DeviceNeighborCUDA.jl:17-22) */
+#define PDX_NL_2D           4  /* D = 2 (DenseNeighborCUDA.jl:77): box[2] is ignored, z may be NULL (all z = 0);
+                                  rows, pair forces and the rebuild policy as in 3-D, fz = 0; no mesh */
 
 /* ---- pair models (src/engine/PairModels.jl:13-14, 45-54) ---- */
 #define PDX_PAIR_NONE  0
 #define PDX_PAIR_LJ    1  /* f/r = 24e(2s^12-s^6)/r^2, u = 4e(s^12-s^6) - u_c */
 #define PDX_PAIR_WCA   2  /* same force, u + e, rc^2 = 1.2599211 sigma^2 */
 #define PDX_PAIR_SOFT  3  /* harmonic repulsion, rc = sigma */
+/* typed models (PairModels.jl:16-19, 61-81); per-particle data set by pdx_pair_set_sigmas /
+ * pdx_pair_set_types, ELL rows only (lanes >= 0, rows traversal, FP32 sums) */
+#define PDX_PAIR_MIXED_LJ    4  /* MixedSigmaLJ: sigma_ij = (sigma_i + sigma_j)/2, rc = cutoff * sigma_ij
+                                   (spec.cutoff is the rcut factor), one epsilon; with shift, u - u(rc)
+                                   (the same for every pair: sigma/rc is constant) */
+#define PDX_PAIR_MIXED_WCA   5  /* MixedSigmaWCA: the same pairs, u + epsilon */
+#define PDX_PAIR_MATRIX_LJ   6  /* PairMatrixLJ{SHIFT}: sigma, epsilon, rc by type pair; shift: u - u(rc_ab) */
+#define PDX_PAIR_MATRIX_WCA  7  /* PairMatrixWCA: the same, u + epsilon_ab */
 
 /* ---- evaluation flags ---- */
 #define PDX_ENERGY  1  /* also per-particle energy / virial and FP64 totals */
@@ -142,6 +152,20 @@ int pdx_nl_build(pdx_nl_t nl, pdx_stream_t stream);
  * pdx_mesh_spread_snapshot). pdx_nl_snapshot takes the copy explicitly (predicated on the
  * device build counter, graph-capturable); the check makes it unnecessary. */
 int pdx_nl_snapshot(pdx_nl_t nl, pdx_stream_t stream);
+/* Velocity-Verlet step around a force evaluation (the reference's fused NVE step: prefix
+ * kick-drift + displacement, src/engine/Fusion.jl:1-6, Run.jl:247-294):
+ * pdx_nl_check_kick_drift = v += c_i f, x += dt v wrapped into [-L/2, L/2)
+ * (_nve_kick_drift3_kernel!, src/simulation/NVEFusion.jl:3-24), then pdx_nl_check on the new x
+ * (x, y, z, vx, vy, vz updated in place; fx, fy, fz hold the forces of the old x);
+ * pdx_nve_kick = the closing v += c_i f with the new forces (_nve_kick3_kernel!,
+ * src/SimulationCore.jl:581-591), advancing the list's step counter. c_i = dt / (2 m): inv_mass
+ * (device, per particle, c_i = dt/2 inv_mass[i]) or, when NULL, the scalar mass. Both apply
+ * only while the list is valid (a latched error stops the motion). 2-D lists ignore z arrays. */
+int pdx_nl_check_kick_drift(pdx_nl_t nl, float *x, float *y, float *z, const float *q, float *vx, float *vy,
+                            float *vz, const float *fx, const float *fy, const float *fz, double dt, double mass,
+                            const float *inv_mass, pdx_stream_t stream);
+int pdx_nve_kick(pdx_nl_t nl, float *vx, float *vy, float *vz, const float *fx, const float *fy, const float *fz,
+                 double dt, double mass, const float *inv_mass, pdx_stream_t stream);
 /* Unconditional rebuild (build_neighbors_dense!, NeighborAllocation.jl:198-214). */
 int pdx_nl_rebuild(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
                    pdx_stream_t stream);
@@ -193,6 +217,29 @@ int pdx_pair_info(pdx_pair_t p, double out[4]);
  * is invalid (_device_if_valid!, DeviceNeighborCUDA.jl:157-162). */
 int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, float *fz, float *Epot,
                     float *V, double *totals, int32_t flags, pdx_stream_t stream);
+
+/* Per-particle diameters of the mixed-sigma models (data[1] of MixedSigmaLJ/WCA,
+ * PairModels.jl:36): n host values in storage order, finite and positive, copied to the device.
+ * Host boundary (synchronises); set again after the caller reorders its particles. */
+int pdx_pair_set_sigmas(pdx_pair_t p, int32_t n, const float *sigma);
+/* Type data of the pair-matrix models (lj_type_table, src/clusterpairs/PairFunctor.jl:166-179;
+ * data of PairMatrixLJ/WCA, PairModels.jl:37): ntypes x ntypes column-major sigma, epsilon and
+ * rcut (host, 1 <= ntypes <= 64), and the 1-based type id of each of the n particles (host,
+ * storage order). Copied to the device; host boundary. */
+int pdx_pair_set_types(pdx_pair_t p, int32_t n, int32_t ntypes, const int32_t *typ, const double *sigma,
+                       const double *epsilon, const double *rcut);
+/* The largest pair cutoff of the model (its list must reach it): uniform models their cutoff,
+ * mixed sigma the factor times the largest diameter, pair matrices the largest rcut; with
+ * Coulomb at least the real-space cutoff. Needs the sigmas / types of a typed model first. */
+int pdx_pair_cutoff(pdx_pair_t p, double *rc);
+
+/* pdx_pair_forces with the position source explicit (the reference's pair-force call takes the
+ * coordinate arrays, Run._enqueue_pair_force!, src/engine/Run.jl:215-238): x, y, z, q must be the
+ * arrays the last pdx_nl_update / pdx_nl_check of nl packed (compared by address; PDX_ERR_STATE
+ * otherwise, q may be NULL when that check had none). The kernels read the packed copy. */
+int pdx_pair_compute(pdx_pair_t p, pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
+                     float *fx, float *fy, float *fz, float *Epot, float *V, double *totals, int32_t flags,
+                     pdx_stream_t stream);
 
 /* ======================================================================= */
 /* Smooth PME mesh engine (B-spline spread, 3D FFT, influence, gather)     */
@@ -334,6 +381,11 @@ int pdx_bonds_create(int32_t n, int32_t nbond, const int32_t *bi, const int32_t 
 int pdx_bonds_destroy(pdx_bonds_t b);
 /* Attach (or detach with NULL) bonds to a pair model: pdx_pair_forces then adds them. */
 int pdx_pair_set_bonds(pdx_pair_t p, pdx_bonds_t b);
+/* Bonded exclusions (BondExclusionFeature, src/engine/Bonded.jl:60-92; the reference's
+ * exclude_bonded): with exclude != 0, pairs joined by a bond of the attached FENE bonds get no
+ * pair (LJ / WCA / soft) term; the bond term stays. Uncharged models only (charged pairs are
+ * excluded through pdx_pair_set_exceptions). */
+int pdx_pair_set_bond_exclusion(pdx_pair_t p, int32_t exclude);
 /* Domain check of the last evaluations (the reference's DomainError,
  * FENEBondCUDA.jl:14-39): first_invalid receives the lowest particle id with a bond at
  * r >= R0, or -1; with reset the device word is cleared. Synchronises the stream. Returns

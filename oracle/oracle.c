@@ -673,6 +673,18 @@ int or_mesh_gather(int n, const double *x, const double *y, const double *z, con
   return 0;
 }
 
+/* Sets the OpenMP thread count of later calls; returns the previous one (1 without OpenMP). */
+int or_set_threads(int n) {
+#ifdef _OPENMP
+  const int prev = omp_get_max_threads();
+  if (n > 0) omp_set_num_threads(n);
+  return prev;
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
 int or_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();
@@ -683,7 +695,8 @@ int or_num_threads(void) {
 
 /* ---------------------------------------------------------------- Kremer-Grest melt */
 /* WCA (rc^2 = 1.2599211 sigma^2, u + eps) between every pair within the cutoff, bonded
- * pairs included (exclude_bonded = false, src/SimulationState.jl:152-155), found with a cell
+ * pairs included (exclude_bonded = false, src/SimulationState.jl:152-155) or, with
+ * exclude_bonded, skipped (BondExclusionFeature, src/engine/Bonded.jl:60-92), found with a cell
  * list of width >= rc, plus FENE bonds u = -k R0^2/2 ln(1 - r^2/R0^2), f/r = -k/(1 - r^2/R0^2)
  * (_fene3_EV!, src/bonded/FENEBondCUDA.jl). Bonds are pairs (bi[b], bj[b]), 0-based. Writes
  * F (3n, xyz interleaved), per-particle half energies E (n), total energy, and the virial
@@ -691,7 +704,7 @@ int or_num_threads(void) {
  * index) when a bond has r >= R0, else the number of WCA pairs (each counted once). */
 long or_kg_forces(int n, const double *x, const double *y, const double *z, const double *box, double sigma,
                   double epsilon, int nbond, const int *bi, const int *bj, double k, double R0, double *F,
-                  double *E, double *energy, double *W) {
+                  double *E, double *energy, double *W, int exclude_bonded) {
   const double rc = pow(2.0, 1.0 / 6.0) * sigma, rc2 = 1.2599211 * sigma * sigma, R02 = R0 * R0;
   int nc[3];
   for (int d = 0; d < 3; ++d) { nc[d] = (int)floor(box[d] / rc); if (nc[d] < 3) nc[d] = 3; }
@@ -711,6 +724,18 @@ long or_kg_forces(int n, const double *x, const double *y, const double *z, cons
     cell[i] = (ci[2] * nc[1] + ci[1]) * nc[0] + ci[0];
   }
   for (int i = n - 1; i >= 0; --i) { next[i] = head[cell[i]]; head[cell[i]] = i; }
+  /* bonded exclusions (exclude_bonded): partners of each particle, CSR */
+  int *boff = (int *)calloc((size_t)n + 1, sizeof(int)), *bpart = (int *)malloc(sizeof(int) * (2 * (size_t)nbond + 1));
+  for (int b = 0; b < nbond; ++b) { ++boff[bi[b] + 1]; ++boff[bj[b] + 1]; }
+  for (int i = 0; i < n; ++i) boff[i + 1] += boff[i];
+  {
+    int *fill = (int *)calloc((size_t)n, sizeof(int));
+    for (int b = 0; b < nbond; ++b) {
+      bpart[boff[bi[b]] + fill[bi[b]]++] = bj[b];
+      bpart[boff[bj[b]] + fill[bj[b]]++] = bi[b];
+    }
+    free(fill);
+  }
   long pairs = 0;
   double etot = 0.0, w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0;
 #pragma omp parallel for schedule(static) reduction(+ : pairs, etot, w0, w1, w2, w3, w4, w5)
@@ -730,6 +755,11 @@ long or_kg_forces(int n, const double *x, const double *y, const double *z, cons
             double dz = z[i] - z[j]; dz -= box[2] * jl_round(dz / box[2]);
             const double r2 = dx * dx + dy * dy + dz * dz;
             if (!(r2 < rc2)) continue;
+            if (exclude_bonded) {
+              int bonded = 0;
+              for (int t = boff[i]; t < boff[i + 1]; ++t) bonded |= bpart[t] == j;
+              if (bonded) continue;
+            }
             const double s2 = sigma * sigma / r2, s6 = s2 * s2 * s2;
             const double a = 24.0 * epsilon * s6 * (2.0 * s6 - 1.0) / r2;
             ax += a * dx; ay += a * dy; az += a * dz;
@@ -762,6 +792,6 @@ long or_kg_forces(int n, const double *x, const double *y, const double *z, cons
   }
   *energy = etot;
   W[0] = w0; W[1] = w1; W[2] = w2; W[3] = w3; W[4] = w4; W[5] = w5;
-  free(head); free(next); free(cell);
+  free(head); free(next); free(cell); free(boff); free(bpart);
   return bad >= 0 ? -1 - bad : pairs;
 }

@@ -40,9 +40,11 @@ def _load():
     lib.or_cell_pairs.restype = C.c_long
     lib.or_mesh_spread.argtypes = [i, P(d), P(d), P(d), P(d), P(d), P(i), i, P(d)]
     lib.or_mesh_gather.argtypes = [i, P(d), P(d), P(d), P(d), P(d), P(i), i, P(d), P(d), P(d)]
-    lib.or_kg_forces.argtypes = [i, P(d), P(d), P(d), P(d), d, d, i, P(i), P(i), d, d, P(d), P(d), P(d), P(d)]
+    lib.or_kg_forces.argtypes = [i, P(d), P(d), P(d), P(d), d, d, i, P(i), P(i), d, d, P(d), P(d), P(d), P(d), i]
     lib.or_kg_forces.restype = C.c_long
     lib.or_num_threads.restype = i
+    lib.or_set_threads.argtypes = [i]
+    lib.or_set_threads.restype = i
     return lib
 
 
@@ -61,6 +63,11 @@ def _i(a):
 
 def num_threads():
     return _lib.or_num_threads()
+
+
+def set_threads(n):
+    """OpenMP threads of later oracle calls; returns the previous count."""
+    return _lib.or_set_threads(int(n))
 
 
 def erfc(x):
@@ -221,7 +228,7 @@ def vv_steps(x, y, z, vx, vy, vz, box, kind="lj", sigma=1.0, epsilon=1.0, cutoff
     return X, V, d["F"], pe, ke, w
 
 
-def kg_forces(x, y, z, box, bonds, sigma=1.0, epsilon=1.0, k=30.0, R0=1.5):
+def kg_forces(x, y, z, box, bonds, sigma=1.0, epsilon=1.0, k=30.0, R0=1.5, exclude_bonded=False):
     """FP64 Kremer-Grest forces: WCA over all pairs (bonded ones included) plus FENE bonds.
     bonds: (nb, 2) int array, 0-based. Returns dict F (n,3), E (n), energy, W (6), pairs;
     raises ValueError naming the first bond with r >= R0."""
@@ -236,7 +243,7 @@ def kg_forces(x, y, z, box, bonds, sigma=1.0, epsilon=1.0, k=30.0, R0=1.5):
     P = C.POINTER(C.c_double)
     r = _lib.or_kg_forces(n, xa[1], ya[1], za[1], bx[1], float(sigma), float(epsilon), len(b), bi[1], bj[1],
                           float(k), float(R0), F.ctypes.data_as(P), E.ctypes.data_as(P), C.byref(e),
-                          W.ctypes.data_as(P))
+                          W.ctypes.data_as(P), 1 if exclude_bonded else 0)
     if r < 0:
         raise ValueError(f"FENE bond {-1 - r} has r >= R0")
     return dict(F=F.reshape(n, 3), E=E, energy=e.value, W=W, pairs=int(r))

@@ -872,6 +872,8 @@ extern "C" int pdx_cluster_create(int32_t n, const double box[3], pdx_pair_t mod
     return fail(PDX_ERR_ARGUMENT, "cluster pairs use FP32 sums without bonds");
   if (model->spec.model == PDX_PAIR_NONE && !model->spec.coulomb)
     return fail(PDX_ERR_ARGUMENT, "cluster pairs need a pair model or Coulomb");
+  if (model->spec.model > PDX_PAIR_SOFT)
+    return fail(PDX_ERR_ARGUMENT, "cluster pairs take the uniform LJ / WCA / soft models; typed models use the ELL rows");
   if (!(skin >= 0.0) || !(prune_buffer >= 0.0) || prune_buffer > skin)
     return fail(PDX_ERR_ARGUMENT, "cluster pairs need 0 <= prune_buffer <= skin.");
   if (waves == 0) waves = 4;

@@ -196,6 +196,7 @@ struct pdx_nl_s {
   int32_t interval = 20;
   int32_t row_lanes = 1;  // lanes per particle of the row build
   bool allpairs = false, diagnostics = false;
+  int dim = 3;  // 2: no z coordinates (z = 0), box[2] is the list's own z extent of three radii
   double box[3] = {0, 0, 0};
   double cutoff = 0, skin = 0;
   float rl2 = 0;  // (cutoff + skin)^2 in FP32, as the reference's rl2
@@ -220,6 +221,7 @@ struct pdx_nl_s {
   pdx::DevBuf<int32_t> snap_offsets, snap_ids;
   pdx::DevBuf<int64_t> snap_state;
   int snap_host = 0;  // checks since create / box change (capped at 2): at 2 a snapshot was taken
+  const void *packed_src[4] = {nullptr, nullptr, nullptr, nullptr};  // x, y, z, q of the last check
   int check_force = 0;  // the last check was a forced rebuild (pdx_nl_rebuild)
 };
 
@@ -234,10 +236,19 @@ struct pdx_pair_s {
   int accumulate = PDX_ACC_FP32;      // force / energy / virial accumulator of the row sums
   int traversal = PDX_TRAVERSAL_ROWS;  // rows of the list, or the list's cells directly
   struct pdx_bonds_s *bonds = nullptr;  // FENE bonds folded into the same per-particle sums
+  bool exclude_bonded = false;          // bonded pairs get no pair term (exclude_bonded)
   int32_t exc_n = 0, nexc = 0;          // Coulomb exceptions (CSR, both directions)
   pdx::DevBuf<int32_t> exc_off, exc_j;
   pdx::DevBuf<float> exc_s;
   pdx::DevBuf<double> partial;
+  // typed models: per-particle diameters (mixed sigma), or type ids and the type-pair table
+  // (sig2, eps, rc2, eshift) of the pair matrices; their particle count
+  pdx::DevBuf<float> sigp;
+  pdx::DevBuf<int32_t> typ;
+  pdx::DevBuf<float4> tab;
+  int32_t typed_n = 0, ntypes = 0;
+  float rfac2 = 0;        // mixed sigma: (rc / sigma_ij)^2
+  double typed_rmax = 0;  // the largest pair cutoff of the typed data
 };
 
 struct pdx_bonds_s {

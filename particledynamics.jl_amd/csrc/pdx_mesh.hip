@@ -2204,6 +2204,7 @@ namespace {
 int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t stream) {
   if (!m || !nl) return fail(PDX_ERR_ARGUMENT, "null argument");
   if (nl->N != m->N) return fail(PDX_ERR_DIMENSION, "neighbor list and mesh particle counts differ");
+  if (nl->dim != 3) return fail(PDX_ERR_ARGUMENT, "the mesh is three-dimensional; 2-D lists have no mesh");
   if (nl->allpairs) return fail(PDX_ERR_ARGUMENT, "all-pairs lists have no cells; use pdx_mesh_spread");
   for (int d = 0; d < 3; ++d)
     if (std::fabs(nl->box[d] - m->box[d]) > 1e-12 * m->box[d])

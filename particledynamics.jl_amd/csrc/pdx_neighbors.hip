@@ -33,19 +33,32 @@ __device__ __forceinline__ bool rebuild_requested(const int64_t *s) {
 }
 
 // ------------------------------------------------------------------ decide
+// Velocity-Verlet prefix of a step, fused into the check: v += c_i f, x += dt v wrapped into
+// [-L/2, L/2) (_nve_kick_drift3_kernel!, src/simulation/NVEFusion.jl:3-24; the reference fuses
+// it with the displacement pass the same way, src/engine/Fusion.jl:1-6). Applied only while the
+// list is valid (_device_if_valid!, DeviceNeighborCUDA.jl:157-162).
+struct KickDrift {
+  float *vx, *vy, *vz;
+  const float *fx, *fy, *fz;
+  const float *inv_mass;  // per particle, or null: c for every particle
+  float c, half_dt, dt;
+};
+
 // Coverage check (_device_displacement_kernel! + _device_decide_kernel!,
 // DeviceNeighborCUDA.jl:169-236): per-block maximum squared displacement (plus the float4
-// packing and the histogram clearing), then one block decides (k_decide_final).
-__global__ __launch_bounds__(TPB) void k_decide(int N, const float *__restrict__ x,
-                                                 const float *__restrict__ y,
-                                                 const float *__restrict__ z,
+// packing and the histogram clearing), then one block decides (k_decide_final). With KD the
+// step's kick-drift moves x, y, z (in place) first.
+template <bool KD>
+__global__ __launch_bounds__(TPB) void k_decide(int N, float *__restrict__ x,
+                                                 float *__restrict__ y,
+                                                 float *__restrict__ z,
                                                  const float *__restrict__ q,
                                                  const float *__restrict__ rref, float4 *packed,
                                                  int32_t *cell_counts, int ncell, int allpairs,
                                                  const float *geom, const int64_t *state,
                                                  float *partial, const int32_t *__restrict__ offsets,
                                                  const int32_t *__restrict__ ids, int32_t *snap_offsets,
-                                                 int32_t *snap_ids, int64_t *snap) {
+                                                 int32_t *snap_ids, int64_t *snap, KickDrift kd) {
   const int64_t err = state[PDX_DN_ERROR];
   if (err != 0) return;  // latched: every block leaves (uniform)
   const int i = blockIdx.x * TPB + threadIdx.x;
@@ -61,7 +74,25 @@ __global__ __launch_bounds__(TPB) void k_decide(int N, const float *__restrict__
   const bool valid = state[PDX_DN_VALID] != 0;
   float value = 0.0f;
   if (i < N) {
-    const float c[3] = {x[i], y[i], z[i]};
+    float c[3] = {x[i], y[i], z ? z[i] : 0.0f};  // 2-D lists: no z array, every z is 0
+    if (KD && valid) {
+      const float ci = kd.inv_mass ? kd.half_dt * kd.inv_mass[i] : kd.c;
+      float *v[3] = {kd.vx, kd.vy, kd.vz};
+      const float *f[3] = {kd.fx, kd.fy, kd.fz};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (d == 2 && !z) break;
+        const float u = fmaf(ci, f[d][i], v[d][i]);
+        v[d][i] = u;
+        float p = fmaf(kd.dt, u, c[d]);
+        p -= (p >= 0.5f * L[d]) ? L[d] : 0.0f;
+        p += (p < -0.5f * L[d]) ? L[d] : 0.0f;
+        c[d] = p;
+      }
+      x[i] = c[0];
+      y[i] = c[1];
+      if (z) z[i] = c[2];
+    }
     // z, then y, then x: the nested muladd order of the reference (threshold ties included)
     for (int d = 2; d >= 0; --d) {
       const float h = 0.5f * L[d];
@@ -514,6 +545,21 @@ __global__ void k_advance(int64_t *state) {
   if (state[PDX_DN_VALID] != 0 && state[PDX_DN_ERROR] == 0) state[PDX_DN_STEP] += 1;
 }
 
+// Closing kick of a velocity-Verlet step, v += c_i f (_nve_kick3_kernel!, SimulationCore.jl:
+// 581-591), while the list is valid; the first thread also advances the step counter (the
+// reference folds advance_device_neighbor_step! into the kick, Run.jl:281-285).
+__global__ __launch_bounds__(TPB) void k_kick(int N, KickDrift kd, int dim, int64_t *state) {
+  if (!(state[PDX_DN_VALID] != 0 && state[PDX_DN_ERROR] == 0)) return;
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i < N) {
+    const float ci = kd.inv_mass ? kd.half_dt * kd.inv_mass[i] : kd.c;
+    kd.vx[i] = fmaf(ci, kd.fx[i], kd.vx[i]);
+    kd.vy[i] = fmaf(ci, kd.fy[i], kd.vy[i]);
+    if (dim == 3) kd.vz[i] = fmaf(ci, kd.fz[i], kd.vz[i]);
+  }
+  if (i == 0) state[PDX_DN_STEP] += 1;
+}
+
 __global__ void k_rearm(int64_t *state) {
   state[PDX_DN_ERROR] = 0;
   state[PDX_DN_REQUEST] = 1;
@@ -603,15 +649,23 @@ int enqueue_rebuild(pdx_nl_t nl, hipStream_t st) {
 }
 
 int enqueue_check(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q, hipStream_t st,
-                  int force) {
+                  int force, const KickDrift *kd = nullptr) {
   if (!nl) return fail(PDX_ERR_ARGUMENT, "null neighbor list");
-  if (!x || !y || !z) return fail(PDX_ERR_ARGUMENT, "null coordinate array");
+  if (!x || !y || (!z && nl->dim == 3)) return fail(PDX_ERR_ARGUMENT, "null coordinate array");
   const int N = nl->N;
   const int nb = blocks_for(N, TPB);
-  hipLaunchKernelGGL(k_decide, dim3(nb), dim3(TPB), 0, st, N, x, y, z, q, nl->rref.p, nl->packed.p,
-                     nl->cell_counts.p, nl->ncell, (int)nl->allpairs, nl->geom.p, nl->state.p, nl->disp_partial.p,
-                     (const int32_t *)nl->cell_offsets.p, (const int32_t *)nl->particle_ids_sorted.p,
-                     nl->snap_offsets.p, nl->snap_ids.p, nl->snap_state.p);
+  nl->packed_src[0] = x; nl->packed_src[1] = y; nl->packed_src[2] = z; nl->packed_src[3] = q;
+  float *xw = const_cast<float *>(x), *yw = const_cast<float *>(y), *zw = const_cast<float *>(z);
+  if (kd)
+    hipLaunchKernelGGL(k_decide<true>, dim3(nb), dim3(TPB), 0, st, N, xw, yw, zw, q, nl->rref.p, nl->packed.p,
+                       nl->cell_counts.p, nl->ncell, (int)nl->allpairs, nl->geom.p, nl->state.p, nl->disp_partial.p,
+                       (const int32_t *)nl->cell_offsets.p, (const int32_t *)nl->particle_ids_sorted.p,
+                       nl->snap_offsets.p, nl->snap_ids.p, nl->snap_state.p, *kd);
+  else
+    hipLaunchKernelGGL(k_decide<false>, dim3(nb), dim3(TPB), 0, st, N, xw, yw, zw, q, nl->rref.p, nl->packed.p,
+                       nl->cell_counts.p, nl->ncell, (int)nl->allpairs, nl->geom.p, nl->state.p, nl->disp_partial.p,
+                       (const int32_t *)nl->cell_offsets.p, (const int32_t *)nl->particle_ids_sorted.p,
+                       nl->snap_offsets.p, nl->snap_ids.p, nl->snap_state.p, KickDrift{});
   PDX_LAUNCH_CHECK();
   // the first check after creation / a box change builds; the second one snapshots that build
   if (nl->snap_host < 2) nl->snap_host += 1;
@@ -659,14 +713,23 @@ extern "C" int pdx_nl_create(int32_t n, const double box[3], double cutoff, doub
   nl->skin = skin;
   nl->allpairs = (flags & PDX_NL_ALLPAIRS) != 0;
   nl->diagnostics = (flags & PDX_NL_DIAGNOSTICS) != 0;
+  nl->dim = (flags & PDX_NL_2D) ? 2 : 3;
   for (int d = 0; d < 3; ++d) nl->box[d] = box[d];
   const float rlf = std::max(1e-6f, (float)cutoff + (float)skin);
   nl->rl2 = rlf * rlf;
+  // 2-D (DenseNeighborCUDA.jl:77, D = 2): every z is 0 and the list's z extent is three list
+  // radii, so the z layers of the stencil beside the particles' own are empty and no pair has a
+  // z image: the 3-D kernels then build the 2-D rows, which equal the 9-cell 2-D stencil's
+  double box3[3] = {box[0], box[1], box[2]};
+  if (nl->dim == 2) {
+    box3[2] = 3.0 * (double)rlf * (1.0 + 1e-6);
+    nl->box[2] = box3[2];
+  }
   int32_t g[3];
-  int rc = grid_for_box(box, rlf, g);
+  int rc = grid_for_box(box3, rlf, g);
   if (rc) { delete nl; return rc; }
   if (nl->allpairs) {
-    const double mn = std::min(box[0], std::min(box[1], box[2]));
+    const double mn = std::min(box3[0], std::min(box3[1], box3[2]));
     if (!(2.0 * rlf <= mn)) {
       delete nl;
       return fail(PDX_ERR_ARGUMENT, "All-pairs neighbor rows need cutoff+skin <= L/2 (minimum image).");
@@ -710,7 +773,7 @@ extern "C" int pdx_nl_create(int32_t n, const double box[3], double cutoff, doub
   int64_t st[PDX_DN_NSTATE] = {0};
   st[PDX_DN_INTERVAL] = interval;
   st[PDX_DN_DIAGNOSTICS] = nl->diagnostics ? 1 : 0;
-  const float geomf[3] = {(float)box[0], (float)box[1], (float)box[2]};
+  const float geomf[3] = {(float)box3[0], (float)box3[1], (float)box3[2]};
   std::vector<int32_t> ones(nl->ncell + 1, 1);
   // identity traversal order until the first build (mesh kernels may read it before then)
   std::vector<int32_t> ident(n);
@@ -779,6 +842,45 @@ extern "C" int pdx_nl_check(pdx_nl_t nl, const float *x, const float *y, const f
   return enqueue_check(nl, x, y, z, q, as_stream(stream), 0);
 }
 
+namespace {
+int make_kick_drift(pdx_nl_t nl, float *vx, float *vy, float *vz, const float *fx, const float *fy,
+                    const float *fz, double dt, double mass, const float *inv_mass, KickDrift &kd) {
+  if (!nl) return fail(PDX_ERR_ARGUMENT, "null neighbor list");
+  if (!vx || !vy || !fx || !fy || (nl->dim == 3 && (!vz || !fz)))
+    return fail(PDX_ERR_ARGUMENT, "null velocity or force array");
+  if (!(std::isfinite(dt) && dt > 0)) return fail(PDX_ERR_ARGUMENT, "dt must be finite and positive");
+  if (!inv_mass && !(std::isfinite(mass) && mass > 0))
+    return fail(PDX_ERR_ARGUMENT, "Inertial integrators require mass > 0");
+  kd.vx = vx; kd.vy = vy; kd.vz = vz;
+  kd.fx = fx; kd.fy = fy; kd.fz = fz;
+  kd.inv_mass = inv_mass;
+  kd.dt = (float)dt;
+  kd.half_dt = (float)(0.5 * dt);
+  kd.c = inv_mass ? 0.0f : (float)dt / 2.0f / (float)mass;  // (dt / 2) / m in FP32, as Run.jl:249
+  return PDX_OK;
+}
+}  // namespace
+
+extern "C" int pdx_nl_check_kick_drift(pdx_nl_t nl, float *x, float *y, float *z, const float *q, float *vx,
+                                       float *vy, float *vz, const float *fx, const float *fy, const float *fz,
+                                       double dt, double mass, const float *inv_mass, pdx_stream_t stream) {
+  KickDrift kd{};
+  int rc = make_kick_drift(nl, vx, vy, vz, fx, fy, fz, dt, mass, inv_mass, kd);
+  if (rc) return rc;
+  return enqueue_check(nl, x, y, z, q, as_stream(stream), 0, &kd);
+}
+
+extern "C" int pdx_nve_kick(pdx_nl_t nl, float *vx, float *vy, float *vz, const float *fx, const float *fy,
+                            const float *fz, double dt, double mass, const float *inv_mass, pdx_stream_t stream) {
+  KickDrift kd{};
+  int rc = make_kick_drift(nl, vx, vy, vz, fx, fy, fz, dt, mass, inv_mass, kd);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_kick, dim3(blocks_for(nl->N, TPB)), dim3(TPB), 0, as_stream(stream), nl->N, kd, nl->dim,
+                     nl->state.p);
+  PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
 extern "C" int pdx_nl_build(pdx_nl_t nl, pdx_stream_t stream) {
   if (!nl) return fail(PDX_ERR_ARGUMENT, "null neighbor list");
   return enqueue_decide_build(nl, as_stream(stream));
@@ -830,11 +932,12 @@ extern "C" int pdx_nl_resize_capacity(pdx_nl_t nl, int32_t cap) {
 extern "C" int pdx_nl_set_box(pdx_nl_t nl, const double box[3], pdx_stream_t stream) {
   if (!nl || !box) return fail(PDX_ERR_ARGUMENT, "null argument");
   const float rlf = std::sqrt(nl->rl2);
+  double box3[3] = {box[0], box[1], nl->dim == 2 ? nl->box[2] : box[2]};  // 2-D: z extent kept
   int32_t g[3];
-  int rc = grid_for_box(box, rlf, g);
+  int rc = grid_for_box(box3, rlf, g);
   if (rc) return rc;
   if (nl->allpairs) {
-    const double mn = std::min(box[0], std::min(box[1], box[2]));
+    const double mn = std::min(box3[0], std::min(box3[1], box3[2]));
     if (!(2.0 * rlf <= mn)) return fail(PDX_ERR_ARGUMENT, "All-pairs neighbor rows need cutoff+skin <= L/2.");
   } else {
     if (g[0] < 3 || g[1] < 3 || g[2] < 3)
@@ -851,14 +954,14 @@ extern "C" int pdx_nl_set_box(pdx_nl_t nl, const double box[3], pdx_stream_t str
       nl->nx = g[0]; nl->ny = g[1]; nl->nz = g[2]; nl->ncell = ncell;
     }
   }
-  for (int d = 0; d < 3; ++d) nl->box[d] = box[d];
+  for (int d = 0; d < 3; ++d) nl->box[d] = box3[d];
   {  // cells of the old box are no cover for the new one: no snapshot until the next build
     PDX_HIP(hipStreamSynchronize(as_stream(stream)));
     const int64_t snap0[4] = {-1, 0, 1, 0};
     PDX_HIP(hipMemcpy(nl->snap_state.p, snap0, sizeof(snap0), hipMemcpyHostToDevice));
     nl->snap_host = 0;
   }
-  const float geomf[3] = {(float)box[0], (float)box[1], (float)box[2]};
+  const float geomf[3] = {(float)box3[0], (float)box3[1], (float)box3[2]};
   PDX_HIP(hipMemcpyAsync(nl->geom.p, geomf, sizeof(geomf), hipMemcpyHostToDevice, as_stream(stream)));
   hipLaunchKernelGGL(k_invalidate, dim3(1), dim3(1), 0, as_stream(stream), nl->state.p);
   PDX_HIP(hipStreamSynchronize(as_stream(stream)));

@@ -16,24 +16,44 @@ __device__ __forceinline__ float horner(const float (&c)[NP], float x) {
   return acc;
 }
 
+// LJ-form coefficients of one pair: uniform models take them from the arguments, the typed
+// ones (mixed sigma, pair matrix) form them per pair.
+struct LJCoef {
+  float eps, sig2, rc2, eshift;
+};
+
+__host__ __device__ constexpr bool lj_form(int m) {
+  return m == PDX_PAIR_LJ || m == PDX_PAIR_WCA || m == PDX_PAIR_MIXED_LJ || m == PDX_PAIR_MIXED_WCA ||
+         m == PDX_PAIR_MATRIX_LJ || m == PDX_PAIR_MATRIX_WCA;
+}
+__host__ __device__ constexpr bool wca_energy(int m) {
+  return m == PDX_PAIR_WCA || m == PDX_PAIR_MIXED_WCA || m == PDX_PAIR_MATRIX_WCA;
+}
+__host__ __device__ constexpr bool typed_model(int m) {
+  return m == PDX_PAIR_MIXED_LJ || m == PDX_PAIR_MIXED_WCA || m == PDX_PAIR_MATRIX_LJ || m == PDX_PAIR_MATRIX_WCA;
+}
+
 // Per-visit physics shared by both traversal layouts: force factor f (force on i = f d) and
-// pair energy u of one row entry (LJ/WCA/soft + real-space Coulomb polynomial).
-// A: any argument block with the model fields eps, sig2, rc2, eshift, cC, crc2, csx, ceshift.
+// pair energy u of one row entry (LJ/WCA/soft + real-space Coulomb polynomial), with the LJ
+// coefficients c (PairModels.jl:39-81: LJ u - u_c, WCA u + eps).
+// A: any argument block with the Coulomb fields cC, crc2, csx, ceshift (and, for the soft
+// model, eps, sig2, rc2).
 template <int MODEL, bool COUL, bool ENERGY, int NP, typename A>
-__device__ __forceinline__ void pair_terms(const A &a, const float (&pe)[NP], const float (&pf)[NP],
-                                           float qi, float qj, float r2, float &f, float &u) {
+__device__ __forceinline__ void pair_terms_c(const A &a, const LJCoef &c, const float (&pe)[NP],
+                                             const float (&pf)[NP], float qi, float qj, float r2, float &f,
+                                             float &u) {
   f = 0.f;
   u = 0.f;
-  if (MODEL == PDX_PAIR_LJ || MODEL == PDX_PAIR_WCA) {
-    const bool in = (r2 > 0.f) & (r2 < a.rc2);
+  if (lj_form(MODEL)) {
+    const bool in = (r2 > 0.f) & (r2 < c.rc2);
     const float invr2 = __builtin_amdgcn_rcpf(r2);
-    const float s2 = a.sig2 * invr2;
+    const float s2 = c.sig2 * invr2;
     const float s6 = s2 * s2 * s2;
     const float s12 = s6 * s6;
-    const float fl = 24.0f * a.eps * (2.0f * s12 - s6) * invr2;
+    const float fl = 24.0f * c.eps * (2.0f * s12 - s6) * invr2;
     f = in ? fl : 0.f;
     if (ENERGY) {
-      const float ul = 4.0f * a.eps * (s12 - s6) + (MODEL == PDX_PAIR_WCA ? a.eps : -a.eshift);
+      const float ul = 4.0f * c.eps * (s12 - s6) + (wca_energy(MODEL) ? c.eps : -c.eshift);
       u = in ? ul : 0.f;
     }
   } else if (MODEL == PDX_PAIR_SOFT) {
@@ -57,6 +77,14 @@ __device__ __forceinline__ void pair_terms(const A &a, const float (&pe)[NP], co
       u += inr ? uc : 0.f;
     }
   }
+}
+
+// The uniform models: coefficients from the argument block (fields eps, sig2, rc2, eshift).
+template <int MODEL, bool COUL, bool ENERGY, int NP, typename A>
+__device__ __forceinline__ void pair_terms(const A &a, const float (&pe)[NP], const float (&pf)[NP],
+                                           float qi, float qj, float r2, float &f, float &u) {
+  const LJCoef c{a.eps, a.sig2, a.rc2, a.eshift};
+  pair_terms_c<MODEL, COUL, ENERGY, NP>(a, c, pe, pf, qi, qj, r2, f, u);
 }
 
 }  // namespace pdx

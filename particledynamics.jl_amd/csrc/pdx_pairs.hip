@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <string>
+#include <vector>
 
 using namespace pdx;
 
@@ -43,7 +44,26 @@ struct PairArgs {
   const int32_t *boff, *bpart;
   int32_t *binvalid;
   float bk, bR02;
+  // typed models (k_pair only): per-particle diameters, or 1-based types and the type-pair table
+  const float *sigp;
+  const int32_t *typ;
+  const float4 *tab;  // (sig2, eps, rc2, eshift), row a*ntypes + b
+  int ntypes;
+  float rfac2;
 };
+
+// The LJ coefficients of the pair (i, j) of a typed model from i's context (its diameter, or its
+// 0-based type times ntypes) and j's datum (diameter or 0-based type).
+template <int MODEL>
+__device__ __forceinline__ LJCoef typed_coef(const PairArgs &a, float ci, float cj, int ti, int tj) {
+  if (MODEL == PDX_PAIR_MIXED_LJ || MODEL == PDX_PAIR_MIXED_WCA) {
+    const float s = 0.5f * (ci + cj);
+    const float s2 = s * s;
+    return LJCoef{a.eps, s2, a.rfac2 * s2, a.eshift};
+  }
+  const float4 t = a.tab[ti + tj];
+  return LJCoef{t.y, t.x, t.z, t.w};
+}
 
 // Writes of one particle's sums (lane that owns it) and the FP64 totals of the launch.
 template <bool COUL, bool ENERGY>
@@ -101,11 +121,16 @@ __global__ __launch_bounds__(TPB) void k_pair(const PairArgs a) {
     pe[k] = a.pe[k];
     pf[k] = a.pf[k];
   }
+  constexpr bool TYPED = typed_model(MODEL);
+  constexpr bool MIXED = MODEL == PDX_PAIR_MIXED_LJ || MODEL == PDX_PAIR_MIXED_WCA;
   if (active) {
     const float4 pi = a.packed[i];
     qi = pi.w;
     const int n = a.counts[i];
     const int32_t *row = a.nbr + i;
+    // typed models: i's diameter, or its type's row of the table
+    const float ci = MIXED ? a.sigp[i] : 0.f;
+    const int ti = (TYPED && !MIXED) ? (a.typ[i] - 1) * a.ntypes : 0;
     // U row entries per lane and iteration: U independent gathers in flight. Padding entries
     // point at i itself (r^2 = 0 is outside every model), so no branch is needed.
     constexpr int U = 4;
@@ -119,6 +144,13 @@ __global__ __launch_bounds__(TPB) void k_pair(const PairArgs a) {
       float4 pj[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) pj[u] = a.packed[jj[u]];
+      float cj[U];
+      int tj[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        cj[u] = MIXED ? a.sigp[jj[u]] : 0.f;
+        tj[u] = (TYPED && !MIXED) ? a.typ[jj[u]] - 1 : 0;
+      }
       float ddx[U], ddy[U], ddz[U];
       row_deltas<U>(pi, pj, hx, hy, hz, Lx, Ly, Lz, ddx, ddy, ddz);
 #pragma unroll
@@ -126,7 +158,11 @@ __global__ __launch_bounds__(TPB) void k_pair(const PairArgs a) {
         const float dx = ddx[u], dy = ddy[u], dz = ddz[u];
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         float f, uu;
-        pair_terms<MODEL, COUL, ENERGY, NP>(a, pe, pf, qi, pj[u].w, r2, f, uu);
+        if (TYPED)
+          pair_terms_c<MODEL, COUL, ENERGY, NP>(a, typed_coef<MODEL>(a, ci, cj[u], ti, tj[u]), pe, pf, qi, pj[u].w,
+                                                r2, f, uu);
+        else
+          pair_terms<MODEL, COUL, ENERGY, NP>(a, pe, pf, qi, pj[u].w, r2, f, uu);
         ax = fmaf(f, dx, ax);
         ay = fmaf(f, dy, ay);
         az = fmaf(f, dz, az);
@@ -537,6 +573,17 @@ __global__ __launch_bounds__(TPB) void k_pair_acc(const PairArgs a, int bonded) 
     const float4 pi = a.packed[i];
     const int n = a.counts[i];
     const int32_t *row = a.nbr + i;
+    // bonded exclusions (BondExclusionFeature, src/engine/Bonded.jl:60-92): a row entry that is
+    // one of i's bond partners gets no pair term; the first four partners live in registers
+    const bool excl = (bonded & 2) != 0;
+    int bp[4] = {-1, -1, -1, -1};
+    int bx0 = 0, nbx = 0;
+    if (excl) {
+      bx0 = a.boff[i];
+      nbx = a.boff[i + 1] - bx0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) bp[k] = k < nbx ? a.bpart[bx0 + k] : -1;
+    }
     constexpr int U = 4;
     for (int t0 = lane; t0 < n; t0 += U * G) {
       int jj[U];
@@ -556,6 +603,12 @@ __global__ __launch_bounds__(TPB) void k_pair_acc(const PairArgs a, int bonded) 
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         float f, uu;
         pair_terms<MODEL, false, ENERGY, 1>(a, none, none, 0.f, 0.f, r2, f, uu);
+        if (excl) {
+          bool ex = (jj[u] == bp[0]) | (jj[u] == bp[1]) | (jj[u] == bp[2]) | (jj[u] == bp[3]);
+          for (int k = 4; k < nbx; ++k) ex |= a.bpart[bx0 + k] == jj[u];
+          f = ex ? 0.f : f;
+          uu = ex ? 0.f : uu;
+        }
         ax.add_prod(f, dx);
         ay.add_prod(f, dy);
         az.add_prod(f, dz);
@@ -567,7 +620,7 @@ __global__ __launch_bounds__(TPB) void k_pair_acc(const PairArgs a, int bonded) 
         }
       }
     }
-    if (bonded) {
+    if (bonded & 1) {
       const int b0 = a.boff[i], nb = a.boff[i + 1] - b0;
       const float R02 = a.bR02;
       for (int t = lane; t < nb; t += G) {
@@ -684,6 +737,36 @@ void launch_g(int G, dim3 grid, hipStream_t st, const PairArgs &a) {
   }
 }
 
+// Typed models: the lanes layout only.
+template <int MODEL, bool COUL, bool ENERGY, int NP>
+void launch_typed_g(int G, dim3 grid, hipStream_t st, const PairArgs &a) {
+  switch (G) {
+    case 1: hipLaunchKernelGGL((k_pair<MODEL, COUL, ENERGY, 1, NP>), grid, dim3(TPB), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_pair<MODEL, COUL, ENERGY, 2, NP>), grid, dim3(TPB), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_pair<MODEL, COUL, ENERGY, 4, NP>), grid, dim3(TPB), 0, st, a); break;
+    case 8: hipLaunchKernelGGL((k_pair<MODEL, COUL, ENERGY, 8, NP>), grid, dim3(TPB), 0, st, a); break;
+    case 16: hipLaunchKernelGGL((k_pair<MODEL, COUL, ENERGY, 16, NP>), grid, dim3(TPB), 0, st, a); break;
+    case 32: hipLaunchKernelGGL((k_pair<MODEL, COUL, ENERGY, 32, NP>), grid, dim3(TPB), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_pair<MODEL, COUL, ENERGY, 64, NP>), grid, dim3(TPB), 0, st, a); break;
+  }
+}
+
+template <int MODEL>
+void launch_typed(bool coul, bool energy, int np, int G, dim3 grid, hipStream_t st, const PairArgs &a) {
+  if (coul) {
+    if (np <= 16) {
+      if (energy) launch_typed_g<MODEL, true, true, 16>(G, grid, st, a);
+      else launch_typed_g<MODEL, true, false, 16>(G, grid, st, a);
+    } else {
+      if (energy) launch_typed_g<MODEL, true, true, 24>(G, grid, st, a);
+      else launch_typed_g<MODEL, true, false, 24>(G, grid, st, a);
+    }
+  } else {
+    if (energy) launch_typed_g<MODEL, false, true, 1>(G, grid, st, a);
+    else launch_typed_g<MODEL, false, false, 1>(G, grid, st, a);
+  }
+}
+
 // Coulomb polynomials are zero-padded to 16 or 24 monomials (Horner over exact zeros).
 template <int MODEL>
 void launch_model(bool coul, bool energy, int np, int G, dim3 grid, hipStream_t st, const PairArgs &a) {
@@ -780,13 +863,16 @@ extern "C" int pdx_pair_create(const pdx_pair_spec *spec, pdx_pair_t *out) {
   if (!spec || !out) return fail(PDX_ERR_ARGUMENT, "null argument");
   *out = nullptr;
   const pdx_pair_spec &sp = *spec;
-  if (sp.model < PDX_PAIR_NONE || sp.model > PDX_PAIR_SOFT) return fail(PDX_ERR_ARGUMENT, "unknown pair model");
-  if (sp.model != PDX_PAIR_NONE) {
-    if (!(std::isfinite(sp.epsilon) && std::isfinite(sp.sigma) && sp.sigma > 0))
+  if (sp.model < PDX_PAIR_NONE || sp.model > PDX_PAIR_MATRIX_WCA) return fail(PDX_ERR_ARGUMENT, "unknown pair model");
+  const bool typed = typed_model(sp.model);
+  if (sp.model != PDX_PAIR_NONE && sp.model != PDX_PAIR_MATRIX_LJ && sp.model != PDX_PAIR_MATRIX_WCA) {
+    if (!(std::isfinite(sp.epsilon) && (typed || (std::isfinite(sp.sigma) && sp.sigma > 0))))
       return fail(PDX_ERR_ARGUMENT, "pair epsilon and sigma must be finite, sigma positive");
-    if (sp.model == PDX_PAIR_LJ && !(std::isfinite(sp.cutoff) && sp.cutoff > 0))
-      return fail(PDX_ERR_ARGUMENT, "LJ cutoff must be finite and positive");
+    if ((sp.model == PDX_PAIR_LJ || sp.model == PDX_PAIR_MIXED_LJ || sp.model == PDX_PAIR_MIXED_WCA) &&
+        !(std::isfinite(sp.cutoff) && sp.cutoff > 0))
+      return fail(PDX_ERR_ARGUMENT, "LJ cutoff (the rcut factor of the mixed-sigma models) must be finite and positive");
   }
+  if (typed && sp.lanes < 0) return fail(PDX_ERR_ARGUMENT, "typed pair models use the lanes layout (lanes >= 0)");
   if (sp.coulomb) {
     if (!(sp.coulomb_C > 0 && std::isfinite(sp.coulomb_C)))
       return fail(PDX_ERR_ARGUMENT, "coulomb_prefactor must be finite and positive");
@@ -814,6 +900,14 @@ extern "C" int pdx_pair_create(const pdx_pair_spec *spec, pdx_pair_t *out) {
     p->rc2 = 1.2599211f * p->sig2;  // UniformWCA constant (PairModels.jl:16)
   } else if (sp.model == PDX_PAIR_SOFT) {
     p->rc2 = p->sig2;
+  } else if (sp.model == PDX_PAIR_MIXED_LJ || sp.model == PDX_PAIR_MIXED_WCA) {
+    // rc = factor sigma_ij: (rc/sigma_ij)^2 = factor^2, and the LJ shift u(rc) is one constant
+    const float f = (float)sp.cutoff;
+    p->rfac2 = f * f;
+    if (sp.model == PDX_PAIR_MIXED_LJ && sp.shift) {
+      const float s2 = 1.0f / (f * f), s6 = s2 * s2 * s2;
+      p->eshift = 4.0f * eps * (s6 * s6 - s6);
+    }
   }
   if (sp.coulomb) {
     const double rc = sp.coulomb_cutoff, rc2 = rc * rc, a = sp.alpha;
@@ -888,6 +982,15 @@ int pair_forces_impl(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, float *fz,
   // per block of 64 particles; 0: automatic
   const bool acc_path = p->accumulate != PDX_ACC_FP32 || p->bonds != nullptr;
   const bool cells = p->traversal == PDX_TRAVERSAL_CELLS;
+  const bool typed = typed_model(p->spec.model);
+  if (typed) {
+    if (acc_path || cells)
+      return fail(PDX_ERR_ARGUMENT, "typed pair models use the rows traversal with FP32 sums and no bonds");
+    if (p->typed_n != N)
+      return fail(PDX_ERR_DIMENSION, p->typed_n == 0 ? "typed pair model without its per-particle data "
+                                                       "(pdx_pair_set_sigmas / pdx_pair_set_types)"
+                                                     : "typed pair data and neighbor list particle counts differ");
+  }
   if (cells) {
     if (acc_path) return fail(PDX_ERR_ARGUMENT, "the cell traversal uses FP32 sums without bonds");
     if (nl->allpairs) return fail(PDX_ERR_ARGUMENT, "the cell traversal needs a cell grid (not all-pairs rows)");
@@ -938,6 +1041,7 @@ int pair_forces_impl(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, float *fz,
   a.add = (flags & PDX_ADD) ? 1 : 0;
   a.eps = p->eps; a.sig2 = p->sig2; a.rc2 = p->rc2; a.eshift = p->eshift;
   a.cC = p->cC; a.crc2 = p->crc2; a.csx = p->csx; a.ceshift = p->ceshift; a.halfself = p->halfself;
+  a.sigp = p->sigp.p; a.typ = p->typ.p; a.tab = p->tab.p; a.ntypes = p->ntypes; a.rfac2 = p->rfac2;
   for (int k = 0; k < MAXC; ++k) { a.pe[k] = p->pe[k]; a.pf[k] = p->pf[k]; }
   const bool coul = p->spec.coulomb != 0;
   const int np = std::max(p->npe, p->npf);
@@ -980,7 +1084,7 @@ int pair_forces_impl(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, float *fz,
     return finish();
   }
   if (acc_path) {
-    const int bonded = p->bonds != nullptr;
+    const int bonded = (p->bonds != nullptr ? 1 : 0) | (p->bonds != nullptr && p->exclude_bonded ? 2 : 0);
     if (bonded) {
       a.boff = p->bonds->offsets.p;
       a.bpart = p->bonds->partners.p;
@@ -994,6 +1098,10 @@ int pair_forces_impl(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, float *fz,
     return finish();
   }
   switch (p->spec.model) {
+    case PDX_PAIR_MIXED_LJ: launch_typed<PDX_PAIR_MIXED_LJ>(coul, energy, np, G, dim3(blocks), st, a); break;
+    case PDX_PAIR_MIXED_WCA: launch_typed<PDX_PAIR_MIXED_WCA>(coul, energy, np, G, dim3(blocks), st, a); break;
+    case PDX_PAIR_MATRIX_LJ: launch_typed<PDX_PAIR_MATRIX_LJ>(coul, energy, np, G, dim3(blocks), st, a); break;
+    case PDX_PAIR_MATRIX_WCA: launch_typed<PDX_PAIR_MATRIX_WCA>(coul, energy, np, G, dim3(blocks), st, a); break;
     case PDX_PAIR_LJ: launch_model<PDX_PAIR_LJ>(coul, energy, np, G, dim3(blocks), st, a); break;
     case PDX_PAIR_WCA: launch_model<PDX_PAIR_WCA>(coul, energy, np, G, dim3(blocks), st, a); break;
     case PDX_PAIR_SOFT: launch_model<PDX_PAIR_SOFT>(coul, energy, np, G, dim3(blocks), st, a); break;
@@ -1006,6 +1114,101 @@ int pair_forces_impl(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, float *fz,
 extern "C" int pdx_pair_forces(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, float *fz, float *Epot,
                                float *V, double *totals, int32_t flags, pdx_stream_t stream) {
   return pair_forces_impl(p, nl, fx, fy, fz, Epot, V, totals, flags, stream, nullptr, 0);
+}
+
+extern "C" int pdx_pair_compute(pdx_pair_t p, pdx_nl_t nl, const float *x, const float *y, const float *z,
+                                const float *q, float *fx, float *fy, float *fz, float *Epot, float *V, double *totals,
+                                int32_t flags, pdx_stream_t stream) {
+  if (!p || !nl) return fail(PDX_ERR_ARGUMENT, "null pair model or neighbor list");
+  if (!x || !y || (!z && nl->dim == 3)) return fail(PDX_ERR_ARGUMENT, "null coordinate array");
+  if (nl->packed_src[0] != x || nl->packed_src[1] != y || nl->packed_src[2] != z || nl->packed_src[3] != q)
+    return fail(PDX_ERR_STATE, "the coordinates (and charges) differ from those the list's last check packed; "
+                               "call pdx_nl_update / pdx_nl_check with them first");
+  return pair_forces_impl(p, nl, fx, fy, fz, Epot, V, totals, flags, stream, nullptr, 0);
+}
+
+extern "C" int pdx_pair_set_sigmas(pdx_pair_t p, int32_t n, const float *sigma) {
+  if (!p || (!sigma && n > 0)) return fail(PDX_ERR_ARGUMENT, "null argument");
+  if (p->spec.model != PDX_PAIR_MIXED_LJ && p->spec.model != PDX_PAIR_MIXED_WCA)
+    return fail(PDX_ERR_ARGUMENT, "per-particle diameters belong to the mixed-sigma models");
+  if (n <= 0) return fail(PDX_ERR_ARGUMENT, "particle count must be positive");
+  float smax = 0.f;
+  for (int32_t i = 0; i < n; ++i) {
+    if (!(std::isfinite(sigma[i]) && sigma[i] > 0)) return fail(PDX_ERR_ARGUMENT, "diameters must be finite and positive");
+    smax = std::max(smax, sigma[i]);
+  }
+  PDX_HIP(hipDeviceSynchronize());
+  int rc = p->sigp.alloc(n);
+  if (rc) return rc;
+  PDX_HIP(hipMemcpy(p->sigp.p, sigma, sizeof(float) * n, hipMemcpyHostToDevice));
+  p->typed_n = n;
+  p->typed_rmax = p->spec.cutoff * smax;
+  return PDX_OK;
+}
+
+extern "C" int pdx_pair_set_types(pdx_pair_t p, int32_t n, int32_t ntypes, const int32_t *typ, const double *sigma,
+                                  const double *epsilon, const double *rcut) {
+  if (!p || !typ || !sigma || !epsilon || !rcut) return fail(PDX_ERR_ARGUMENT, "null argument");
+  if (p->spec.model != PDX_PAIR_MATRIX_LJ && p->spec.model != PDX_PAIR_MATRIX_WCA)
+    return fail(PDX_ERR_ARGUMENT, "type tables belong to the pair-matrix models");
+  if (n <= 0) return fail(PDX_ERR_ARGUMENT, "particle count must be positive");
+  if (ntypes < 1 || ntypes > 64) return fail(PDX_ERR_ARGUMENT, "ntypes must be in 1..64");
+  for (int32_t i = 0; i < n; ++i)
+    if (typ[i] < 1 || typ[i] > ntypes) return fail(PDX_ERR_ARGUMENT, "type ids must lie in 1..ntypes");
+  // rows a*ntypes + b of (sigma^2, epsilon, rc^2, u(rc) with the shift) in FP32, from the
+  // column-major matrices (element (a, b) at a + ntypes b)
+  std::vector<float4> tab((size_t)ntypes * ntypes);
+  double rmax = 0;
+  for (int a = 0; a < ntypes; ++a)
+    for (int b = 0; b < ntypes; ++b) {
+      const size_t k = (size_t)a + (size_t)ntypes * b;
+      const double sg = sigma[k], ep = epsilon[k], rc = rcut[k];
+      if (!(std::isfinite(sg) && sg > 0 && std::isfinite(ep) && std::isfinite(rc) && rc >= 0))
+        return fail(PDX_ERR_ARGUMENT, "pair sigma must be positive, epsilon and rcut finite, rcut >= 0");
+      const float s2 = (float)sg * (float)sg, r2 = (float)rc * (float)rc;
+      float esh = 0.f;
+      if (p->spec.model == PDX_PAIR_MATRIX_LJ && p->spec.shift && rc > 0) {
+        // lj_cutoff_energy (PairModels.jl:20-25) per pair
+        const float x2 = s2 / r2, x6 = x2 * x2 * x2;
+        esh = 4.0f * (float)ep * (x6 * x6 - x6);
+      }
+      tab[(size_t)a * ntypes + b] = make_float4(s2, (float)ep, r2, esh);
+      rmax = std::max(rmax, rc);
+    }
+  PDX_HIP(hipDeviceSynchronize());
+  int rc;
+  if ((rc = p->typ.alloc(n)) || (rc = p->tab.alloc(tab.size()))) return rc;
+  PDX_HIP(hipMemcpy(p->typ.p, typ, sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  PDX_HIP(hipMemcpy(p->tab.p, tab.data(), sizeof(float4) * tab.size(), hipMemcpyHostToDevice));
+  p->typed_n = n;
+  p->ntypes = ntypes;
+  p->typed_rmax = rmax;
+  return PDX_OK;
+}
+
+extern "C" int pdx_pair_cutoff(pdx_pair_t p, double *rc) {
+  if (!p || !rc) return fail(PDX_ERR_ARGUMENT, "null argument");
+  double r = 0;
+  switch (p->spec.model) {
+    case PDX_PAIR_LJ: r = p->spec.cutoff; break;
+    case PDX_PAIR_WCA: r = std::sqrt(1.2599211) * p->spec.sigma; break;
+    case PDX_PAIR_SOFT: r = p->spec.sigma; break;
+    case PDX_PAIR_NONE: r = 0; break;
+    default:
+      if (p->typed_n == 0) return fail(PDX_ERR_STATE, "typed pair model without its per-particle data");
+      r = p->typed_rmax;
+  }
+  if (p->spec.coulomb) r = std::max(r, p->spec.coulomb_cutoff);
+  *rc = r;
+  return PDX_OK;
+}
+
+extern "C" int pdx_pair_set_bond_exclusion(pdx_pair_t p, int32_t exclude) {
+  if (!p) return fail(PDX_ERR_ARGUMENT, "null pair model");
+  if (exclude && p->spec.coulomb)
+    return fail(PDX_ERR_ARGUMENT, "bonded exclusions of charged pairs go through the Coulomb exceptions");
+  p->exclude_bonded = exclude != 0;
+  return PDX_OK;
 }
 
 extern "C" int pdx_pair_set_accumulation(pdx_pair_t p, int32_t mode) {

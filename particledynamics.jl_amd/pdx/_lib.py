@@ -24,7 +24,9 @@ PDX_ERR_STATE = 6
 PDX_DN_NSTATE = 23
 PDX_NL_ALLPAIRS = 1
 PDX_NL_DIAGNOSTICS = 2
+PDX_NL_2D = 4
 PDX_PAIR_NONE, PDX_PAIR_LJ, PDX_PAIR_WCA, PDX_PAIR_SOFT = 0, 1, 2, 3
+PDX_PAIR_MIXED_LJ, PDX_PAIR_MIXED_WCA, PDX_PAIR_MATRIX_LJ, PDX_PAIR_MATRIX_WCA = 4, 5, 6, 7
 PDX_ENERGY = 1
 PDX_ADD = 2
 PDX_INFLUENCE_SPME = 1
@@ -104,6 +106,10 @@ def _load():
         "pdx_pair_destroy": ([vp], C.c_int),
         "pdx_pair_info": ([vp, P(dbl)], C.c_int),
         "pdx_pair_forces": ([vp, vp, vp, vp, vp, vp, vp, vp, i32, vp], C.c_int),
+        "pdx_pair_compute": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp], C.c_int),
+        "pdx_pair_set_sigmas": ([vp, i32, P(C.c_float)], C.c_int),
+        "pdx_pair_set_types": ([vp, i32, i32, P(i32), P(dbl), P(dbl), P(dbl)], C.c_int),
+        "pdx_pair_cutoff": ([vp, P(dbl)], C.c_int),
         "pdx_mesh_create": ([P(pdx_mesh_spec), i32, P(dbl), P(vp)], C.c_int),
         "pdx_mesh_destroy": ([vp], C.c_int),
         "pdx_mesh_info": ([vp, P(dbl)], C.c_int),
@@ -116,6 +122,8 @@ def _load():
         "pdx_mesh_set_spread_threads": ([vp, i32], C.c_int),
         "pdx_nl_check": ([vp, vp, vp, vp, vp, vp], C.c_int),
         "pdx_nl_build": ([vp, vp], C.c_int),
+        "pdx_nl_check_kick_drift": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, vp, vp], C.c_int),
+        "pdx_nve_kick": ([vp, vp, vp, vp, vp, vp, vp, dbl, dbl, vp, vp], C.c_int),
         "pdx_nl_snapshot": ([vp, vp], C.c_int),
         "pdx_mesh_solve": ([vp, i32, vp], C.c_int),
         "pdx_mesh_gather": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp], C.c_int),

@@ -77,6 +77,13 @@ class ElectrostaticsSpec:
     def C(self):
         return self.coulomb_prefactor / self.relative_permittivity
 
+    def resolve_mesh(self, box):
+        """(K1, K2, K3): the given mesh, or the smallest 2,3,5,7-smooth count covering each box
+        length at the spacing, at least the order (resolve_mesh, Spec.jl:141-146)."""
+        if self.mesh is not None:
+            return tuple(int(k) for k in self.mesh)
+        return tuple(max(fft_friendly(int(math.ceil(b / self.mesh_spacing - 1e-9))), int(self.order)) for b in box)
+
     def resolved_alpha(self):
         return float(self.alpha) if self.alpha is not None else alpha_from_tolerance(self.real_cutoff,
                                                                                      self.real_space_tolerance)

@@ -14,7 +14,10 @@ from .clusters import ClusterPairEngine
 
 
 def default_capacity(N, box, rlist):
-    """ceil(1.5 (4 pi/3) rlist^3 rho + 64) (src/engine/StateEngine.jl:463)."""
+    """ceil(1.5 (4 pi/3) rlist^3 rho + 64) (src/engine/StateEngine.jl:463); in 2-D (two box
+    lengths) the disc, ceil(1.5 pi rlist^2 rho + 64)."""
+    if len(box) == 2:
+        return int(math.ceil(1.5 * math.pi * rlist ** 2 * N / (box[0] * box[1]) + 64))
     rho = N / (box[0] * box[1] * box[2])
     return int(math.ceil(1.5 * (4.0 * math.pi / 3.0) * rlist ** 3 * rho + 64))
 
@@ -23,7 +26,7 @@ class ForceEngine:
     def __init__(self, N, box, pair_model=None, electrostatics=None, charges=None, skin=0.4, cap=None,
                  interval=20, lanes=0, diagnostics=False, spread="snapshot", correction=None, accumulation="fp32",
                  bonds=None, exceptions=(), traversal="rows", backend="ell", prune_buffer=None, cluster_capacity=0,
-                 overlap_with="fft", gather="tiles"):
+                 overlap_with="fft", gather="tiles", exclude_bonded=False):
         """backend "ell": the cell list and ELL rows (``pdx_nl_*`` + ``pdx_pair_forces``); "cluster":
         the cluster-pair list and tiles (``pdx_cluster_*``, the reference's ``ClusterBackend``,
         src/engine/ClusterBackend.jl:1-40); its mesh spread takes the cluster order."""
@@ -40,6 +43,8 @@ class ForceEngine:
         self.backend = backend
         self.N = int(N)
         self.box = tuple(float(b) for b in box)
+        if len(self.box) == 2 and (electrostatics is not None or backend != "ell"):
+            raise L.ArgumentError("2-D systems (two box lengths) run on the ELL rows without a mesh")
         self.solver = ElectrostaticsSolver(electrostatics, box, charges) if electrostatics is not None else None
         rc = 0.0
         if pair_model is not None:
@@ -50,6 +55,10 @@ class ForceEngine:
         self.skin = float(skin)
         self.pairs = PairForce(pair_model, coulomb=self.solver.real_space() if self.solver else None, lanes=lanes,
                                accumulation=accumulation, bonds=bonds, traversal=traversal)
+        if exclude_bonded:
+            if bonds is None:
+                raise L.ArgumentError("exclude_bonded needs bonds")
+            self.pairs.set_bond_exclusion(True)
         if backend == "cluster":
             self.nl = None
             self.cluster = ClusterPairEngine(N, box, self.pairs, skin, prune_buffer, capacity=cluster_capacity)
@@ -98,17 +107,38 @@ class ForceEngine:
         self._graph = None
         self._snap_ready = False
 
-    def evaluate(self, x, y, z, q, fx, fy, fz, Epot=None, V=None, energy=True, stream=None, overlap=True):
-        """Enqueue one force evaluation (no host synchronisation)."""
+    def evaluate(self, x, y, z, q, fx, fy, fz, Epot=None, V=None, energy=True, stream=None, overlap=True,
+                 kick_drift=None):
+        """Enqueue one force evaluation (no host synchronisation). ``kick_drift``: dict(vx, vy, vz,
+        dt, mass, inv_mass) moves x, y, z first by the velocity-Verlet kick-drift with the forces
+        fx, fy, fz hold (fused into the list check), as ``step`` does."""
         import torch
         main = stream if stream is not None else torch.cuda.current_stream()
         early = (self.cluster is None and self.solver is not None and overlap and self.spread_mode == "snapshot"
                  and self.overlap_with in ("spread", "fft"))
+        if kick_drift is not None and self.cluster is not None:
+            raise L.ArgumentError("the fused velocity-Verlet step runs on the ELL backend")
+
+        def check():
+            if kick_drift is None:
+                self.nl.check(x, y, z, q, stream=main)
+            else:
+                kd = kick_drift
+                self.nl.check_kick_drift(x, y, z, q, kd["vx"], kd["vy"], kd["vz"], fx, fy, fz, kd["dt"],
+                                         kd.get("mass", 1.0), kd.get("inv_mass"), stream=main)
+
         if self.cluster is not None:
             self.cluster.update(x, y, z, q, stream=main)
             order = self.cluster.order_ptr
         elif early and self._snap_ready:
-            self.nl.check(x, y, z, q, stream=main)  # the rebuild kernels follow after the fork
+            check()  # the rebuild kernels follow after the fork
+            order = self.nl._arrays.particle_ids_sorted
+        elif kick_drift is not None:
+            check()
+            self.nl.build(stream=main)
+            if early:
+                self._snap_ready = True
+                early = False
             order = self.nl._arrays.particle_ids_sorted
         else:
             self.nl.update(x, y, z, q, stream=main)
@@ -119,7 +149,7 @@ class ForceEngine:
                 early = False
             order = self.nl._arrays.particle_ids_sorted
         if self.solver is None:
-            self._pairs(fx, fy, fz, Epot, V, energy, main)
+            self._pairs(x, y, z, q, fx, fy, fz, Epot, V, energy, main)
             return
         side = self.mesh_stream if overlap else main
         spread_stream = main if self.overlap_with == "solve" else side
@@ -143,7 +173,7 @@ class ForceEngine:
             spread_done.record(side)
             main.wait_event(spread_done)
         self.solver.solve(energy, stream=side)
-        self._pairs(fx, fy, fz, Epot, V, energy, main)
+        self._pairs(x, y, z, q, fx, fy, fz, Epot, V, energy, main)
         if overlap:
             main.wait_stream(side)
         if self._tile_gather and self.cluster is None and self.spread_mode in ("cells", "snapshot"):
@@ -156,12 +186,40 @@ class ForceEngine:
         if self.correction is not None:
             self.correction.forward(self.feats, None, fx, fy, fz, stream=main)
 
-    def _pairs(self, fx, fy, fz, Epot, V, energy, stream):
+    def _pairs(self, x, y, z, q, fx, fy, fz, Epot, V, energy, stream):
         tot = self.pair_totals if energy else None
         if self.cluster is not None:
             self.cluster.forces(fx, fy, fz, Epot, V, tot, energy=energy, add=False, stream=stream)
         else:
-            self.pairs.compute(self.nl, fx, fy, fz, Epot, V, tot, energy=energy, add=False, stream=stream)
+            self.pairs.compute_at(self.nl, x, y, z, q, fx, fy, fz, Epot, V, tot, energy=energy, add=False,
+                                  stream=stream)
+
+    def step(self, x, y, z, q, vx, vy, vz, fx, fy, fz, dt, mass=1.0, inv_mass=None, Epot=None, V=None,
+             energy=False, stream=None, overlap=True):
+        """One velocity-Verlet step (the reference's NVE step, src/engine/Run.jl:247-294): kick-drift
+        fused into the list check, the force evaluation of the new positions, the closing kick
+        (which also advances the list's step counter). fx, fy, fz must hold the forces of the
+        current positions (an ``evaluate`` first); they hold the new forces afterwards."""
+        import torch
+        main = stream if stream is not None else torch.cuda.current_stream()
+        kd = dict(vx=vx, vy=vy, vz=vz, dt=dt, mass=mass, inv_mass=inv_mass)
+        # (a learned correction is part of the forces the closing kick applies: evaluate adds it)
+        self.evaluate(x, y, z, q, fx, fy, fz, Epot, V, energy, stream=main, overlap=overlap, kick_drift=kd)
+        self.nl.kick(vx, vy, vz, fx, fy, fz, dt, mass, inv_mass, stream=main)
+
+    def capture_step(self, x, y, z, q, vx, vy, vz, fx, fy, fz, dt, mass=1.0, inv_mass=None, Epot=None, V=None,
+                     energy=False, overlap=True):
+        """Capture one ``step`` into a hipGraph; ``replay()`` then advances one step per launch. The
+        warm-up step before the capture advances the system by one step."""
+        import torch
+        self.step(x, y, z, q, vx, vy, vz, fx, fy, fz, dt, mass, inv_mass, Epot, V, energy, overlap=overlap)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.step(x, y, z, q, vx, vy, vz, fx, fy, fz, dt, mass, inv_mass, Epot, V, energy, overlap=overlap)
+        torch.cuda.synchronize()
+        self._graph = g
+        return g
 
     def status(self, check=True, stream=None):
         """Boundary readback of the short-range list (raises on a latched capacity / coordinate error)."""

@@ -46,6 +46,60 @@ class SoftRepulsive:
         return self.sigma
 
 
+class MixedSigmaLJ:
+    """Per-particle diameters, sigma_ij = (sigma_i + sigma_j)/2, rc = rcut_factor sigma_ij, one
+    epsilon (``MixedSigmaLJ``, src/engine/PairModels.jl:16,61-65); ``shift``: u - u(rc), the same
+    constant for every pair."""
+    model = L.PDX_PAIR_MIXED_LJ
+
+    def __init__(self, sigmas, epsilon=1.0, rcut_factor=2.5, shift=False):
+        import numpy as np
+        self.sigmas = np.ascontiguousarray(sigmas, dtype=np.float32)
+        self.epsilon, self.rcut_factor, self.shift = float(epsilon), float(rcut_factor), bool(shift)
+
+    @property
+    def cutoff(self):
+        return self.rcut_factor * float(self.sigmas.max())
+
+
+class MixedSigmaWCA(MixedSigmaLJ):
+    """``MixedSigmaWCA`` (PairModels.jl:17,66-70): the mixed-sigma pairs with u + epsilon; the
+    reference takes the factor as given (2^(1/6) for a WCA cut)."""
+    model = L.PDX_PAIR_MIXED_WCA
+
+    def __init__(self, sigmas, epsilon=1.0, rcut_factor=2 ** (1 / 6)):
+        super().__init__(sigmas, epsilon, rcut_factor, False)
+
+
+class PairMatrixLJ:
+    """sigma, epsilon and rc by type pair (``PairMatrixLJ{SHIFT}``, PairModels.jl:18,71-77):
+    ``types`` 1-based per particle, ``sigma`` / ``epsilon`` / ``rcut`` ntypes x ntypes."""
+    model = L.PDX_PAIR_MATRIX_LJ
+
+    def __init__(self, types, sigma, epsilon, rcut, shift=False):
+        import numpy as np
+        self.types = np.ascontiguousarray(types, dtype=np.int32)
+        self.sigma = np.asarray(sigma, dtype=np.float64)
+        self.epsilon_m = np.asarray(epsilon, dtype=np.float64)
+        self.rcut = np.asarray(rcut, dtype=np.float64)
+        self.shift = bool(shift)
+        nt = self.sigma.shape[0]
+        if self.sigma.shape != (nt, nt) or self.epsilon_m.shape != (nt, nt) or self.rcut.shape != (nt, nt):
+            raise L.DimensionMismatch("sigma, epsilon and rcut must be ntypes x ntypes")
+
+    @property
+    def cutoff(self):
+        return float(self.rcut.max())
+
+
+class PairMatrixWCA(PairMatrixLJ):
+    """``PairMatrixWCA`` (PairModels.jl:19,78-81): the matrix pairs with u + epsilon_ab."""
+    model = L.PDX_PAIR_MATRIX_WCA
+
+    def __init__(self, types, sigma, epsilon, rcut):
+        super().__init__(types, sigma, epsilon, rcut, False)
+
+
 @dataclass(frozen=True)
 class RealSpaceCoulomb:
     """Short-range Ewald Coulomb C q_i q_j erfc(alpha r)/r inside ``cutoff``."""
@@ -66,6 +120,7 @@ def _sig(lib):
         "pdx_bonds_create": [i32, i32, P(C.c_int32), P(C.c_int32), f64, f64, P(vp)],
         "pdx_bonds_destroy": [vp],
         "pdx_pair_set_bonds": [vp, vp],
+        "pdx_pair_set_bond_exclusion": [vp, i32],
         "pdx_bonds_status": [vp, P(C.c_int32), i32, vp],
         "pdx_pair_set_traversal": [vp, i32],
         "pdx_pair_set_exceptions": [vp, i32, i32, P(C.c_int32), P(C.c_int32), P(C.c_double)],
@@ -120,6 +175,15 @@ class PairForce:
         spec = L.pdx_pair_spec()
         if model is None:
             spec.model = L.PDX_PAIR_NONE
+        elif isinstance(model, MixedSigmaLJ):
+            spec.model = model.model
+            spec.epsilon = model.epsilon
+            spec.sigma = 1.0
+            spec.cutoff = model.rcut_factor
+            spec.shift = 1 if model.shift else 0
+        elif isinstance(model, PairMatrixLJ):
+            spec.model = model.model
+            spec.shift = 1 if model.shift else 0
         else:
             spec.model = model.model
             spec.epsilon = float(model.epsilon)
@@ -138,6 +202,16 @@ class PairForce:
         self._h = h
         self._release = L.adopt(self, h, L.lib.pdx_pair_destroy)
         self.model = model
+        if isinstance(model, MixedSigmaLJ):
+            L.check(L.lib.pdx_pair_set_sigmas(h, len(model.sigmas), model.sigmas.ctypes.data_as(C.POINTER(C.c_float))))
+        elif isinstance(model, PairMatrixLJ):
+            import numpy as np
+            cm = [np.asfortranarray(m).reshape(-1, order="F") for m in (model.sigma, model.epsilon_m, model.rcut)]
+            cm = [np.ascontiguousarray(m) for m in cm]
+            D = C.POINTER(C.c_double)
+            L.check(L.lib.pdx_pair_set_types(h, len(model.types), model.sigma.shape[0],
+                                             model.types.ctypes.data_as(C.POINTER(C.c_int32)), cm[0].ctypes.data_as(D),
+                                             cm[1].ctypes.data_as(D), cm[2].ctypes.data_as(D)))
         self.coulomb = coulomb
         self.bonds = None
         self.set_accumulation(accumulation)
@@ -179,6 +253,11 @@ class PairForce:
                                               b.ctypes.data_as(C.POINTER(C.c_int32)),
                                               c.ctypes.data_as(C.POINTER(C.c_double))))
 
+    def set_bond_exclusion(self, exclude: bool):
+        """Bonded pairs feel no pair term (the reference's ``exclude_bonded``)."""
+        L.check(L.lib.pdx_pair_set_bond_exclusion(self._h, 1 if exclude else 0))
+        self.exclude_bonded = bool(exclude)
+
     def set_bonds(self, bonds: FENEBonds | None):
         L.check(L.lib.pdx_pair_set_bonds(self._h, bonds.handle if bonds is not None else None))
         self.bonds = bonds  # kept alive while attached
@@ -189,6 +268,21 @@ class PairForce:
         L.check(L.lib.pdx_pair_info(self._h, out))
         return dict(n_energy_terms=int(out[0]), n_force_terms=int(out[1]), energy_fit_error=out[2],
                     force_fit_error=out[3])
+
+    def cutoff(self):
+        """The largest pair cutoff of the model (``pdx_pair_cutoff``)."""
+        r = C.c_double()
+        L.check(L.lib.pdx_pair_cutoff(self._h, C.byref(r)))
+        return r.value
+
+    def compute_at(self, nl, x, y, z, q, fx, fy, fz, Epot=None, V=None, totals=None, energy=True, add=False,
+                   stream=None):
+        """Forces at the given coordinates (``pdx_pair_compute``): x, y, z, q must be the arrays
+        the list's last ``update`` / ``check`` packed."""
+        flags = (L.PDX_ENERGY if energy else 0) | (L.PDX_ADD if add else 0)
+        L.check(L.lib.pdx_pair_compute(self._h, nl.handle, L.ptr(x), L.ptr(y), L.ptr(z), L.ptr(q), L.ptr(fx),
+                                       L.ptr(fy), L.ptr(fz), L.ptr(Epot), L.ptr(V), L.ptr(totals), flags,
+                                       L.stream_handle(stream)))
 
     def compute(self, nl, fx, fy, fz, Epot=None, V=None, totals=None, energy=True, add=False, stream=None):
         """Enqueue the pair kernel over ``nl``'s rows (positions from ``nl.update``)."""

@@ -41,9 +41,15 @@ class DeviceNeighborWorkspace:
     """GPU-owned cell list and ELL rows for ``N`` particles in a centred orthorhombic box."""
 
     def __init__(self, N, box, cutoff, skin, cap, interval=20, allpairs=False, diagnostics=True):
+        """``box`` of two lengths makes a 2-D list (DenseNeighborCUDA.jl:77): z may then be None."""
         h = C.c_void_p()
-        flags = (L.PDX_NL_ALLPAIRS if allpairs else 0) | (L.PDX_NL_DIAGNOSTICS if diagnostics else 0)
-        L.check(L.lib.pdx_nl_create(int(N), L.dbl3(box), float(cutoff), float(skin), int(cap), int(interval),
+        if len(box) not in (2, 3):
+            raise L.ArgumentError("box must have 2 or 3 lengths")
+        self.dim = len(box)
+        flags = ((L.PDX_NL_ALLPAIRS if allpairs else 0) | (L.PDX_NL_DIAGNOSTICS if diagnostics else 0)
+                 | (L.PDX_NL_2D if self.dim == 2 else 0))
+        box3 = tuple(box) + ((1.0,) if self.dim == 2 else ())
+        L.check(L.lib.pdx_nl_create(int(N), L.dbl3(box3), float(cutoff), float(skin), int(cap), int(interval),
                                     flags, C.byref(h)))
         self._h = h
         self._release = L.adopt(self, h, L.lib.pdx_nl_destroy)
@@ -78,6 +84,18 @@ class DeviceNeighborWorkspace:
         """First half of ``update``: coverage check and packing (no rebuild kernels)."""
         L.check(L.lib.pdx_nl_check(self._h, L.ptr(x), L.ptr(y), L.ptr(z), L.ptr(q), L.stream_handle(stream)))
 
+    def check_kick_drift(self, x, y, z, q, vx, vy, vz, fx, fy, fz, dt, mass=1.0, inv_mass=None, stream=None):
+        """``check`` preceded by the velocity-Verlet kick-drift (v += dt/(2m) f, x += dt v, wrapped),
+        fused into the same pass; x, y, z, vx, vy, vz change in place."""
+        L.check(L.lib.pdx_nl_check_kick_drift(self._h, L.ptr(x), L.ptr(y), L.ptr(z), L.ptr(q), L.ptr(vx), L.ptr(vy),
+                                              L.ptr(vz), L.ptr(fx), L.ptr(fy), L.ptr(fz), float(dt), float(mass),
+                                              L.ptr(inv_mass), L.stream_handle(stream)))
+
+    def kick(self, vx, vy, vz, fx, fy, fz, dt, mass=1.0, inv_mass=None, stream=None):
+        """Closing kick v += dt/(2m) f of a velocity-Verlet step; advances the step counter."""
+        L.check(L.lib.pdx_nve_kick(self._h, L.ptr(vx), L.ptr(vy), L.ptr(vz), L.ptr(fx), L.ptr(fy), L.ptr(fz),
+                                   float(dt), float(mass), L.ptr(inv_mass), L.stream_handle(stream)))
+
     def build(self, stream=None):
         """Second half of ``update``: the rebuild kernels, predicated on the check's decision."""
         L.check(L.lib.pdx_nl_build(self._h, L.stream_handle(stream)))
@@ -99,7 +117,10 @@ class DeviceNeighborWorkspace:
         self._refresh_info()
 
     def set_box(self, box, stream=None):
-        L.check(L.lib.pdx_nl_set_box(self._h, L.dbl3(box), L.stream_handle(stream)))
+        if len(box) != self.dim:
+            raise L.DimensionMismatch("the box has another dimension than the list")
+        box3 = tuple(box) + ((1.0,) if self.dim == 2 else ())
+        L.check(L.lib.pdx_nl_set_box(self._h, L.dbl3(box3), L.stream_handle(stream)))
         self.box = tuple(float(b) for b in box)
         self._refresh_info()
 
@@ -155,7 +176,7 @@ class DeviceNeighborWorkspace:
 
 def build_neighbors_dense(x, y, z, box, cutoff, skin, cap, q=None, allpairs=False, stream=None):
     """Build rows now and check capacity (raises ``NeighborCapacityError``)."""
-    ws = DeviceNeighborWorkspace(x.numel(), box, cutoff, skin, cap, allpairs=allpairs)
+    ws = DeviceNeighborWorkspace(x.numel(), box, cutoff, skin, cap, allpairs=allpairs)  # len(box) 2: 2-D
     ws.rebuild(x, y, z, q, stream=stream)
     ws.status(check=True, stream=stream)
     return ws

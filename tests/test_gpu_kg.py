@@ -110,3 +110,31 @@ def test_modes_need_pair_model_without_coulomb(cuda):
     x = torch.zeros(8, device="cuda")
     with pytest.raises(pdx.ArgumentError):
         p.compute(nl, x, x, x, energy=False)
+
+
+def test_bonded_exclusion_against_fp64(cuda, small_melt):
+    """exclude_bonded (BondExclusionFeature, src/engine/Bonded.jl:60-92): bonded beads feel only
+    their FENE bond; every accumulation mode against the FP64 oracle with the same exclusion."""
+    import torch
+    sysd, _ = small_melt
+    ref = oracle.kg_forces(sysd["x"], sysd["y"], sysd["z"], sysd["box"], sysd["bonds"], exclude_bonded=True)
+    full = oracle.kg_forces(sysd["x"], sysd["y"], sysd["z"], sysd["box"], sysd["bonds"])
+    assert np.abs(full["F"] - ref["F"]).max() > 1e-3 * np.abs(ref["F"]).max()  # the exclusion matters here
+    N, box = sysd["N"], sysd["box"]
+    X, Y, Z = dev(sysd["x"]), dev(sysd["y"]), dev(sysd["z"])
+    for mode in MODES:
+        eng = pdx.ForceEngine(N, box, pdx.WCA(epsilon=1.0, sigma=1.0), skin=0.4, accumulation=mode,
+                              bonds=pdx.FENEBonds(N, sysd["bonds"]), exclude_bonded=True)
+        f = [torch.empty(N, device="cuda") for _ in range(3)]
+        E = torch.empty(N, device="cuda")
+        eng.evaluate(X, Y, Z, None, *f, E, None, energy=True)
+        tot = eng.totals()
+        F = torch.stack(f, 1).double().cpu().numpy()
+        err = np.linalg.norm(F - ref["F"]) / np.linalg.norm(ref["F"])
+        print(f"[kg exclusion] {mode}: force {err:.2e}", flush=True)
+        assert err < 2e-6, mode
+        assert tot["pair_energy"] == pytest.approx(ref["energy"], rel=1e-6)
+        for c in range(6):
+            assert tot["pair_virial"][c] == pytest.approx(ref["W"][c], rel=2e-5, abs=1e-3 * abs(ref["W"][0]))
+    with pytest.raises(pdx.ArgumentError):
+        pdx.ForceEngine(N, box, pdx.WCA(), exclude_bonded=True)

@@ -138,3 +138,44 @@ def test_rebuild_policy(cuda):
         ws.advance()
     ws.update(X + 0.201, Y, Z)
     assert ws.status().builds == 3  # aged out
+
+
+def test_rows_and_forces_2d(cuda):
+    """2-D lists (DenseNeighborCUDA.jl:77, D = 2): rows bit-exact against the 2-D brute force
+    (tie band empty), and LJ forces / energies against the 2-D direct sums of
+    test/pair_reference_gpu.jl (oracle.pair_typed, D = 2); z is absent (NULL)."""
+    import torch
+    cutoff, skin = 2.5, 0.3
+    rl = float(np.float32(cutoff) + np.float32(skin))
+    band = 4 * _ulp32(np.float32(rl) * np.float32(rl))
+    n, L = 3000, 60.0  # area density 0.83
+    for seed in range(20):
+        rng = np.random.default_rng(100 + seed)
+        m = int(math.ceil(math.sqrt(n)))
+        g = np.stack(np.meshgrid(np.arange(m), np.arange(m), indexing="ij"), -1).reshape(-1, 2)[:n]
+        P = (g + 0.5) * (L / m) - L / 2 + rng.normal(0, 0.05, (n, 2))
+        P = (P - L * np.floor(P / L + 0.5)).astype(np.float32)
+        exp, counts, margin = oracle.neighbor_rows((P[:, 0].astype(np.float64), P[:, 1].astype(np.float64)), (L, L),
+                                                   np.full(n, rl))
+        if margin > band:
+            break
+    assert margin > band
+    X, Y = dev(P[:, 0]), dev(P[:, 1])
+    cap = pdx.default_capacity(n, (L, L), rl)
+    ws = pdx.build_neighbors_dense(X, Y, None, (L, L), cutoff, skin, cap)
+    assert ws.rows_host() == [list(r) for r in exp]
+    eng = pdx.ForceEngine(n, (L, L), pdx.LennardJones(1.0, 1.0, 2.5), skin=skin)
+    f = [torch.empty(n, device="cuda") for _ in range(3)]
+    E = torch.empty(n, device="cuda")
+    eng.evaluate(X, Y, None, None, *f, E, None, energy=True)
+    tot = eng.totals()
+    ref = oracle.pair_typed((P[:, 0].astype(np.float64), P[:, 1].astype(np.float64)), (L, L), kind="lj", cutoff=2.5)
+    F = torch.stack(f[:2], 1).double().cpu().numpy()
+    err = np.linalg.norm(F - ref["F"]) / np.linalg.norm(ref["F"])
+    print(f"[rows 2d] seed={seed} force {err:.2e}", flush=True)
+    assert err < 2e-5
+    assert torch.count_nonzero(f[2]).item() == 0
+    assert abs(tot["energy"] - ref["energy"].sum()) < 1e-5 * abs(ref["energy"].sum())
+    with pytest.raises(pdx.ArgumentError):
+        pdx.ForceEngine(n, (L, L), None, electrostatics=pdx.PME(coulomb_prefactor=1.0, real_cutoff=2.5, alpha=1.0,
+                                                               mesh=(32, 32, 32), order=4), charges=np.zeros(n))

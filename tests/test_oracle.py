@@ -545,3 +545,36 @@ def test_pair_typed_dimer_and_gradient():
     b = oracle.pair_typed((P2[:, 0], P2[:, 1], P2[:, 2]), (L, L, L), kind="wca", style="mixed",
                           sigma_particle=sp[typ % 2], factor=1.2599211 ** 0.5)
     assert np.allclose(a["F"], b["F"][:, :2], rtol=1e-13, atol=1e-13) and np.allclose(a["energy"], b["energy"])
+
+
+def test_kg_oracle_bond_exclusion():
+    """With exclude_bonded the WCA terms of bonded pairs vanish: the difference to the full sums is
+    exactly the WCA of the bonded pairs (direct two-body evaluation)."""
+    rng = np.random.default_rng(4)
+    n, L = 200, 12.0
+    m = 6  # a jittered lattice of 216 sites with bonded pairs along x
+    g = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1).reshape(-1, 3)[:n]
+    P = (g + 0.5) * (L / m) - L / 2 + rng.normal(0, 0.05, (n, 3))
+    bonds = np.array([[i, i + 1] for i in range(0, n - 1, 2)], dtype=np.int32)
+    # bonded partners within the WCA range so the exclusion matters
+    for i, j in bonds:
+        P[j] = P[i] + np.array([0.95, 0.0, 0.0])
+    P = P - L * np.floor(P / L + 0.5)
+    full = oracle.kg_forces(P[:, 0], P[:, 1], P[:, 2], (L, L, L), bonds)
+    ex = oracle.kg_forces(P[:, 0], P[:, 1], P[:, 2], (L, L, L), bonds, exclude_bonded=True)
+    dF = full["F"] - ex["F"]
+    two = np.zeros_like(dF)
+    e2 = 0.0
+    for i, j in bonds:
+        d = P[i] - P[j]
+        d -= L * np.round(d / L)
+        r2 = (d * d).sum()
+        if r2 < 1.2599211:
+            s6 = (1 / r2) ** 3
+            a = 24 * s6 * (2 * s6 - 1) / r2
+            two[i] += a * d
+            two[j] -= a * d
+            e2 += 4 * s6 * (s6 - 1) + 1
+    assert np.abs(dF - two).max() < 1e-9 * np.abs(two).max()
+    assert abs((full["energy"] - ex["energy"]) - e2) < 1e-9 * abs(e2)
+    assert np.abs(two).max() > 1.0
