@@ -169,6 +169,16 @@ int pdx_nve_kick(pdx_nl_t nl, float *vx, float *vy, float *vz, const float *fx, 
 /* Unconditional rebuild (build_neighbors_dense!, NeighborAllocation.jl:198-214). */
 int pdx_nl_rebuild(pdx_nl_t nl, const float *x, const float *y, const float *z, const float *q,
                    pdx_stream_t stream);
+/* Spatial order of the list's last build (the permutation OrderingWorkspace applies,
+ * src/engine/Ordering.jl:19-79): perm (device, N) receives, slot by slot, the 0-based id of the
+ * particle that belongs there, in cell order (cells z, y, x) and inside a cell by position (x, y,
+ * z; the id only for coincident particles), so the order is a function of the configuration, not
+ * of the labels. The caller permutes its particle arrays by it (new[s] = old[perm[s]]) and then
+ * calls pdx_nl_invalidate. Host boundary (reads the state; PDX_ERR_STATE without a valid list). */
+int pdx_nl_spatial_order(pdx_nl_t nl, int32_t *perm, pdx_stream_t stream);
+/* The particles were relabelled (reordered): the next check rebuilds, and no cell snapshot is kept
+ * until the second check after it. Host boundary. */
+int pdx_nl_invalidate(pdx_nl_t nl, pdx_stream_t stream);
 /* Advance the device step counter (advance_device_neighbor_step!, DeviceNeighborCUDA.jl:404-407). */
 int pdx_nl_advance(pdx_nl_t nl, pdx_stream_t stream);
 /* Clear a latched capacity error and request the failed rebuild again (rearm_capacity_rebuild!, :622). */

@@ -535,13 +535,19 @@ __global__ __launch_bounds__(TPB) void k_cl_jsort(const int32_t *ctl, const int3
 
 // Coordinates and charges in slot order, NaN on padding (_pack_kernel!, List.jl:366-404), and
 // the reference positions of a build / prune.
+// Typed models: the slot's diameter (mixed sigma) or 0-based type (pair matrix), from the pair
+// model's per-particle data; padding slots take 1.0 / type 0 (their NaN coordinates drop them).
 __global__ __launch_bounds__(TPB) void k_cl_pack(int nslot, int N, const int32_t *__restrict__ perm,
                                                  const float *__restrict__ x, const float *__restrict__ y,
                                                  const float *__restrict__ z, const float *__restrict__ q,
-                                                 float4 *packed, float *ref, const int32_t *ctl) {
+                                                 float4 *packed, float *ref, const int32_t *ctl,
+                                                 const float *__restrict__ sigp, const int32_t *__restrict__ typ,
+                                                 float *saux, int32_t *styp) {
   const int s = blockIdx.x * TPB + threadIdx.x;
   if (s >= nslot) return;
   const int i = perm[s];
+  if (sigp) saux[s] = i < 0 ? 1.0f : sigp[i];
+  if (typ) styp[s] = i < 0 ? 0 : typ[i] - 1;
   if (i < 0) {
     packed[s] = make_float4(NAN, NAN, NAN, 0.0f);
     return;
@@ -565,6 +571,11 @@ __global__ void k_cl_set(int32_t *ctl, int word, int32_t v) { ctl[word] = v; }
 // ---------------------------------------------------------------- force pass
 struct TileArgs {
   float eps, sig2, rc2, eshift;     // LJ / WCA / soft
+  float rfac2;                      // mixed sigma: (rc / sigma_ij)^2
+  const float *saux;                // typed: slot diameters (mixed sigma)
+  const int32_t *styp;              // typed: slot 0-based types (pair matrix)
+  const float4 *tab;                // typed: type-pair table (sig2, eps, rc2, eshift)
+  int ntypes;
   float cC, crc2, csx, ceshift;     // real-space Coulomb
   float pe[MAXC], pf[MAXC];
   const float4 *packed;
@@ -601,6 +612,19 @@ __global__ __launch_bounds__(WAVE *MAXW) void k_cl_tiles(const TileArgs a) {
   float4 xi[CL];
 #pragma unroll
   for (int c = 0; c < CL; ++c) xi[c] = a.packed[(size_t)S * SUP + c * CL + ti];
+  // typed models: the i atoms' diameters or type rows (lane ti of each cluster c)
+  constexpr bool TYPED = typed_model(MODEL);
+  constexpr bool MIXED = MODEL == PDX_PAIR_MIXED_LJ || MODEL == PDX_PAIR_MIXED_WCA;
+  float cai[TYPED ? CL : 1];
+  int tai[TYPED ? CL : 1];
+  if constexpr (TYPED) {
+#pragma unroll
+    for (int c = 0; c < CL; ++c) {
+      const size_t sl = (size_t)S * SUP + c * CL + ti;
+      cai[c] = MIXED ? a.saux[sl] : 0.0f;
+      tai[c] = MIXED ? 0 : a.styp[sl] * a.ntypes;
+    }
+  }
   float si[CL][NV];
 #pragma unroll
   for (int c = 0; c < CL; ++c)
@@ -617,18 +641,30 @@ __global__ __launch_bounds__(WAVE *MAXW) void k_cl_tiles(const TileArgs a) {
   // other, and an iteration of up to eight tiles covers both latencies)
   int ent_n = 0, jb_n = 0;
   float4 xj_n = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float caj_n = 0.0f;
+  int taj_n = 0;
   if (e0 < e1) {
     ent_n = PRUNE ? a.oent[seg + e0] : a.went[seg + e0];
     jb_n = PRUNE ? (int)(seg + e0) : a.wjb[seg + e0];
     xj_n = a.packed[(size_t)(ent_n >> 8) * CL + tj];
+    if constexpr (TYPED) {
+      if (MIXED) caj_n = a.saux[(size_t)(ent_n >> 8) * CL + tj];
+      else taj_n = a.styp[(size_t)(ent_n >> 8) * CL + tj];
+    }
   }
   for (int e = e0; e < e1; ++e) {
     const int ent = ent_n, jb = jb_n;
     const float4 xj = xj_n;
+    const float caj = caj_n;
+    const int taj = taj_n;
     if (e + 1 < e1) {
       ent_n = PRUNE ? a.oent[seg + e + 1] : a.went[seg + e + 1];
       jb_n = PRUNE ? (int)(seg + e + 1) : a.wjb[seg + e + 1];
       xj_n = a.packed[(size_t)(ent_n >> 8) * CL + tj];
+      if constexpr (TYPED) {
+        if (MIXED) caj_n = a.saux[(size_t)(ent_n >> 8) * CL + tj];
+        else taj_n = a.styp[(size_t)(ent_n >> 8) * CL + tj];
+      }
     }
     const int cj = ent >> 8;
     const unsigned mask = (unsigned)ent & 255u;
@@ -650,7 +686,11 @@ __global__ __launch_bounds__(WAVE *MAXW) void k_cl_tiles(const TileArgs a) {
       const bool ok = (r2 < a.rmax2) && (c != cdiag || tj > ti);
       if (PRUNE && __ballot(ok && r2 < a.rp2) != 0) pm |= 1u << c;
       float f, u;
-      pair_terms<MODEL, COUL, E, NP>(a, pe, pf, xi[c].w, xj.w, r2, f, u);
+      if constexpr (TYPED)
+        pair_terms_c<MODEL, COUL, E, NP>(a, typed_coef<MODEL>(a, cai[c], caj, tai[c], taj), pe, pf, xi[c].w, xj.w,
+                                         r2, f, u);
+      else
+        pair_terms<MODEL, COUL, E, NP>(a, pe, pf, xi[c].w, xj.w, r2, f, u);
       f = ok ? f : 0.0f;
       dx = ok ? dx : 0.0f;
       dy = ok ? dy : 0.0f;
@@ -845,6 +885,8 @@ struct pdx_cluster_s {
   DevBuf<int32_t> jcnt, jcur, joff, jlist, wjlist, wjcnt;
   DevBuf<float> jbuf, islot;
   DevBuf<double> partial;
+  DevBuf<float> saux;    // typed models: diameter per slot (mixed sigma)
+  DevBuf<int32_t> styp;  // typed models: 0-based type per slot (pair matrix)
 };
 
 namespace {
@@ -872,8 +914,6 @@ extern "C" int pdx_cluster_create(int32_t n, const double box[3], pdx_pair_t mod
     return fail(PDX_ERR_ARGUMENT, "cluster pairs use FP32 sums without bonds");
   if (model->spec.model == PDX_PAIR_NONE && !model->spec.coulomb)
     return fail(PDX_ERR_ARGUMENT, "cluster pairs need a pair model or Coulomb");
-  if (model->spec.model > PDX_PAIR_SOFT)
-    return fail(PDX_ERR_ARGUMENT, "cluster pairs take the uniform LJ / WCA / soft models; typed models use the ELL rows");
   if (!(skin >= 0.0) || !(prune_buffer >= 0.0) || prune_buffer > skin)
     return fail(PDX_ERR_ARGUMENT, "cluster pairs need 0 <= prune_buffer <= skin.");
   if (waves == 0) waves = 4;
@@ -882,7 +922,12 @@ extern "C" int pdx_cluster_create(int32_t n, const double box[3], pdx_pair_t mod
   for (int d = 0; d < 3; ++d)
     if (!(box[d] > 0.0)) return fail(PDX_ERR_ARGUMENT, "box lengths must be positive");
   double rmax = 0.0;
-  if (model->spec.model != PDX_PAIR_NONE) rmax = std::sqrt((double)model->rc2);
+  if (typed_model(model->spec.model)) {
+    if (model->typed_n == 0) return fail(PDX_ERR_STATE, "typed pair model without its per-particle data");
+    rmax = model->typed_rmax;
+  } else if (model->spec.model != PDX_PAIR_NONE) {
+    rmax = std::sqrt((double)model->rc2);
+  }
   if (model->spec.coulomb) rmax = std::max(rmax, std::sqrt((double)model->crc2));
   const double rlist = rmax + skin;
   if (std::min(box[0], std::min(box[1], box[2])) <= 2.0 * rlist)
@@ -1062,8 +1107,22 @@ extern "C" int pdx_cluster_update(pdx_cluster_t c, const float *x, const float *
                      (const int32_t *)c->oent.p, c->ecap, (const int32_t *)c->joff.p, c->jcur.p, c->jlist.p);
   hipLaunchKernelGGL(k_cl_jsort, dim3(blocks_of((int64_t)CL * c->NSmax, TPB)), dim3(TPB), 0, st,
                      (const int32_t *)ctl, (const int32_t *)c->joff.p, c->jlist.p);
+  const pdx_pair_s *pm = c->model;
+  const bool mixed = pm->spec.model == PDX_PAIR_MIXED_LJ || pm->spec.model == PDX_PAIR_MIXED_WCA;
+  const bool matrix = pm->spec.model == PDX_PAIR_MATRIX_LJ || pm->spec.model == PDX_PAIR_MATRIX_WCA;
+  if ((mixed || matrix) && pm->typed_n != N)
+    return fail(PDX_ERR_DIMENSION, "typed pair data and cluster list particle counts differ");
+  if (mixed && (int64_t)c->saux.n < c->nslot) {
+    int rc = c->saux.alloc(c->nslot);
+    if (rc) return rc;
+  }
+  if (matrix && (int64_t)c->styp.n < c->nslot) {
+    int rc = c->styp.alloc(c->nslot);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(k_cl_pack, dim3(c->sblocks), dim3(TPB), 0, st, c->nslot, N, (const int32_t *)c->perm.p, x, y,
-                     z, q, c->packed.p, c->ref.p, (const int32_t *)ctl);
+                     z, q, c->packed.p, c->ref.p, (const int32_t *)ctl, mixed ? (const float *)pm->sigp.p : nullptr,
+                     matrix ? (const int32_t *)pm->typ.p : nullptr, c->saux.p, c->styp.p);
   PDX_LAUNCH_CHECK();
   c->updated = true;
   return PDX_OK;
@@ -1131,6 +1190,9 @@ extern "C" int pdx_cluster_forces(pdx_cluster_t c, float *fx, float *fy, float *
   hipStream_t st = as_stream(stream);
   TileArgs a{};
   a.eps = p->eps; a.sig2 = p->sig2; a.rc2 = p->rc2; a.eshift = p->eshift;
+  a.rfac2 = p->rfac2; a.saux = c->saux.p; a.styp = c->styp.p; a.tab = p->tab.p; a.ntypes = p->ntypes;
+  if (typed_model(p->spec.model) && p->typed_n != c->N)
+    return fail(PDX_ERR_DIMENSION, "typed pair data and cluster list particle counts differ");
   a.cC = p->cC; a.crc2 = p->crc2; a.csx = p->csx; a.ceshift = p->ceshift;
   for (int k = 0; k < MAXC; ++k) {
     a.pe[k] = p->pe[k];
@@ -1157,6 +1219,12 @@ extern "C" int pdx_cluster_forces(pdx_cluster_t c, float *fx, float *fy, float *
   const int np = std::max(p->npe, p->npf);
   const int grid = c->NSmax * c->splits, threads = WAVE * c->waves;
   switch (p->spec.model) {
+    case PDX_PAIR_MIXED_LJ: launch_tiles_model<PDX_PAIR_MIXED_LJ>(coul, energy, np, st, grid, threads, a, c); break;
+    case PDX_PAIR_MIXED_WCA: launch_tiles_model<PDX_PAIR_MIXED_WCA>(coul, energy, np, st, grid, threads, a, c); break;
+    case PDX_PAIR_MATRIX_LJ: launch_tiles_model<PDX_PAIR_MATRIX_LJ>(coul, energy, np, st, grid, threads, a, c); break;
+    case PDX_PAIR_MATRIX_WCA:
+      launch_tiles_model<PDX_PAIR_MATRIX_WCA>(coul, energy, np, st, grid, threads, a, c);
+      break;
     case PDX_PAIR_LJ: launch_tiles_model<PDX_PAIR_LJ>(coul, energy, np, st, grid, threads, a, c); break;
     case PDX_PAIR_WCA: launch_tiles_model<PDX_PAIR_WCA>(coul, energy, np, st, grid, threads, a, c); break;
     case PDX_PAIR_SOFT: launch_tiles_model<PDX_PAIR_SOFT>(coul, energy, np, st, grid, threads, a, c); break;

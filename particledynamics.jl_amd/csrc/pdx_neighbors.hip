@@ -567,6 +567,41 @@ __global__ void k_rearm(int64_t *state) {
 
 __global__ void k_invalidate(int64_t *state) { state[PDX_DN_VALID] = 0; }
 
+// Spatial order of the last build (the permutation of OrderingWorkspace, src/engine/Ordering.jl:
+// 19-79): slots in cell order (cells z, y, x), inside a cell by position (x, then y, then z, then
+// the id for coincident particles), so the order depends on the configuration and not on the
+// particles' labels. One wave per cell ranks its particles (cells hold a few dozen); perm[slot] =
+// 0-based particle id. Cells beyond the LDS stage rank from global memory.
+__global__ __launch_bounds__(WAVE) void k_spatial_order(int ncell, const int32_t *__restrict__ offsets,
+                                                        const int32_t *__restrict__ ids,
+                                                        const float4 *__restrict__ packed, int32_t *perm) {
+  __shared__ float4 sh[SORT_STAGE / 2];
+  const int c = blockIdx.x;
+  const int s = offsets[c] - 1, e = offsets[c + 1] - 1, n = e - s;
+  auto key = [&](int k) {
+    const int i = ids[s + k] - 1;
+    const float4 p = packed[i];
+    return make_float4(p.x, p.y, p.z, __int_as_float(i));
+  };
+  const bool staged = n <= SORT_STAGE / 2;
+  if (staged) {
+    for (int k = threadIdx.x; k < n; k += WAVE) sh[k] = key(k);
+    __syncthreads();
+  }
+  auto before = [](const float4 a, const float4 b) {
+    if (a.x != b.x) return a.x < b.x;
+    if (a.y != b.y) return a.y < b.y;
+    if (a.z != b.z) return a.z < b.z;
+    return __float_as_int(a.w) < __float_as_int(b.w);
+  };
+  for (int k = threadIdx.x; k < n; k += WAVE) {
+    const float4 me = staged ? sh[k] : key(k);
+    int rank = 0;
+    for (int m = 0; m < n; ++m) rank += before(staged ? sh[m] : key(m), me);
+    perm[s + rank] = __float_as_int(me.w);
+  }
+}
+
 // Cell snapshot: offsets and ids of a build that completed since the last snapshot (the build
 // counter moved), then the recorded count. Run after every step's evaluation, so that a spread
 // running beside the next step's rebuild reads cells no rebuild is writing.
@@ -878,6 +913,35 @@ extern "C" int pdx_nve_kick(pdx_nl_t nl, float *vx, float *vy, float *vz, const 
   hipLaunchKernelGGL(k_kick, dim3(blocks_for(nl->N, TPB)), dim3(TPB), 0, as_stream(stream), nl->N, kd, nl->dim,
                      nl->state.p);
   PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
+extern "C" int pdx_nl_spatial_order(pdx_nl_t nl, int32_t *perm, pdx_stream_t stream) {
+  if (!nl || !perm) return fail(PDX_ERR_ARGUMENT, "null argument");
+  if (nl->allpairs) return fail(PDX_ERR_ARGUMENT, "all-pairs lists have no cells to order by");
+  int64_t st[PDX_DN_NSTATE];
+  hipStream_t s = as_stream(stream);
+  PDX_HIP(hipStreamSynchronize(s));
+  PDX_HIP(hipMemcpy(st, nl->state.p, sizeof(st), hipMemcpyDeviceToHost));
+  if (st[PDX_DN_ERROR] != 0 || st[PDX_DN_VALID] == 0)
+    return fail(PDX_ERR_STATE, "spatial order needs a valid list (update it first)");
+  hipLaunchKernelGGL(k_spatial_order, dim3(nl->ncell), dim3(WAVE), 0, s, nl->ncell,
+                     (const int32_t *)nl->cell_offsets.p, (const int32_t *)nl->particle_ids_sorted.p,
+                     (const float4 *)nl->packed.p, perm);
+  PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
+extern "C" int pdx_nl_invalidate(pdx_nl_t nl, pdx_stream_t stream) {
+  if (!nl) return fail(PDX_ERR_ARGUMENT, "null neighbor list");
+  hipStream_t s = as_stream(stream);
+  PDX_HIP(hipStreamSynchronize(s));
+  const int64_t snap0[4] = {-1, 0, 1, 0};
+  PDX_HIP(hipMemcpy(nl->snap_state.p, snap0, sizeof(snap0), hipMemcpyHostToDevice));
+  nl->snap_host = 0;
+  hipLaunchKernelGGL(k_invalidate, dim3(1), dim3(1), 0, s, nl->state.p);
+  PDX_LAUNCH_CHECK();
+  PDX_HIP(hipStreamSynchronize(s));
   return PDX_OK;
 }
 

@@ -79,6 +79,20 @@ __device__ __forceinline__ void pair_terms_c(const A &a, const LJCoef &c, const 
   }
 }
 
+// The LJ coefficients of the pair (i, j) of a typed model (PairModels.jl:61-81) from i's
+// context (its diameter, or its 0-based type times ntypes) and j's datum (diameter or 0-based
+// type). A: fields eps, eshift, rfac2 (mixed sigma) and tab, rows (sig2, eps, rc2, eshift).
+template <int MODEL, typename A>
+__device__ __forceinline__ LJCoef typed_coef(const A &a, float ci, float cj, int ti, int tj) {
+  if (MODEL == PDX_PAIR_MIXED_LJ || MODEL == PDX_PAIR_MIXED_WCA) {
+    const float s = 0.5f * (ci + cj);
+    const float s2 = s * s;
+    return LJCoef{a.eps, s2, a.rfac2 * s2, a.eshift};
+  }
+  const float4 t = a.tab[ti + tj];
+  return LJCoef{t.y, t.x, t.z, t.w};
+}
+
 // The uniform models: coefficients from the argument block (fields eps, sig2, rc2, eshift).
 template <int MODEL, bool COUL, bool ENERGY, int NP, typename A>
 __device__ __forceinline__ void pair_terms(const A &a, const float (&pe)[NP], const float (&pf)[NP],

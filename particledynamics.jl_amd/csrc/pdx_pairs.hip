@@ -52,18 +52,6 @@ struct PairArgs {
   float rfac2;
 };
 
-// The LJ coefficients of the pair (i, j) of a typed model from i's context (its diameter, or its
-// 0-based type times ntypes) and j's datum (diameter or 0-based type).
-template <int MODEL>
-__device__ __forceinline__ LJCoef typed_coef(const PairArgs &a, float ci, float cj, int ti, int tj) {
-  if (MODEL == PDX_PAIR_MIXED_LJ || MODEL == PDX_PAIR_MIXED_WCA) {
-    const float s = 0.5f * (ci + cj);
-    const float s2 = s * s;
-    return LJCoef{a.eps, s2, a.rfac2 * s2, a.eshift};
-  }
-  const float4 t = a.tab[ti + tj];
-  return LJCoef{t.y, t.x, t.z, t.w};
-}
 
 // Writes of one particle's sums (lane that owns it) and the FP64 totals of the launch.
 template <bool COUL, bool ENERGY>

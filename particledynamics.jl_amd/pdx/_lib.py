@@ -122,6 +122,8 @@ def _load():
         "pdx_mesh_set_spread_threads": ([vp, i32], C.c_int),
         "pdx_nl_check": ([vp, vp, vp, vp, vp, vp], C.c_int),
         "pdx_nl_build": ([vp, vp], C.c_int),
+        "pdx_nl_spatial_order": ([vp, vp, vp], C.c_int),
+        "pdx_nl_invalidate": ([vp, vp], C.c_int),
         "pdx_nl_check_kick_drift": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, vp, vp], C.c_int),
         "pdx_nve_kick": ([vp, vp, vp, vp, vp, vp, vp, dbl, dbl, vp, vp], C.c_int),
         "pdx_nl_snapshot": ([vp, vp], C.c_int),

@@ -165,6 +165,7 @@ class ElectrostaticsSolver:
         import numpy as np
         q = np.ascontiguousarray(np.asarray(charges, dtype=np.float32))
         L.check(L.lib.pdx_mesh_set_charges(self._h, q.ctypes.data_as(C.POINTER(C.c_float)), int(q.size)))
+        self.charges = q
         self.qtotal = float(q.astype(np.float64).sum())
         self.qsq = float((q.astype(np.float64) ** 2).sum())
 

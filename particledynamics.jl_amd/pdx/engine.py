@@ -67,6 +67,8 @@ class ForceEngine:
             cap = cap if cap is not None else default_capacity(N, box, rc + skin)
             self.nl = DeviceNeighborWorkspace(N, box, rc, skin, cap, interval=interval, diagnostics=diagnostics)
             self.cluster = None
+        self._exceptions = list(exceptions)
+        self._bond_pairs = None if bonds is None else bonds.pairs
         if exceptions:
             if self.solver is None:
                 raise L.ArgumentError("electrostatic exceptions need a mesh")
@@ -220,6 +222,52 @@ class ForceEngine:
         torch.cuda.synchronize()
         self._graph = g
         return g
+
+    def reorder(self, x, y, z, q=None, *extra, tags=None):
+        """Spatial reordering (OrderingWorkspace / enqueue_ordering!, src/engine/Ordering.jl:19-79),
+        at a host boundary: the particles are permuted into the spatial order of the last build
+        (cells, then position inside a cell), so storage neighbours are space neighbours and the
+        order no longer depends on how the particles were labelled. x, y, z, q and every tensor in
+        ``extra`` (velocities, forces, per-particle arrays of N rows) are permuted in place, as are
+        ``tags`` (slot -> original id, int tensor); the engine's own per-particle data (charges,
+        typed pair data, bonds, exceptions) follows. Returns the permutation (slot -> old slot).
+        The list rebuilds at the next evaluation."""
+        import numpy as np
+        import torch
+        if self.cluster is not None:
+            raise L.ArgumentError("spatial reordering runs on the ELL backend")
+        torch.cuda.synchronize()
+        self.nl.status(check=True)
+        perm = self.nl.spatial_order()
+        for t in (x, y, z, q) + tuple(extra) + ((tags,) if tags is not None else ()):
+            if t is not None:
+                t.copy_(t[perm])
+        p = perm.cpu().numpy()
+        inv = np.empty_like(p)
+        inv[p] = np.arange(len(p))
+        model = self.pairs.model
+        from .forces import MixedSigmaLJ, PairMatrixLJ
+        if isinstance(model, MixedSigmaLJ):
+            model.sigmas = np.ascontiguousarray(model.sigmas[p])
+            self.pairs.set_sigmas(model.sigmas)
+        elif isinstance(model, PairMatrixLJ):
+            model.types = np.ascontiguousarray(model.types[p])
+            self.pairs._set_types(model)
+        if self.pairs.bonds is not None:
+            from .forces import FENEBonds
+            b = self.pairs.bonds
+            pairs = inv[self._bond_pairs]
+            self._bond_pairs = pairs
+            nb = FENEBonds(self.N, pairs, k=b.k, R0=b.R0)
+            self.pairs.set_bonds(nb)
+        if self._exceptions:
+            self._exceptions = [(int(inv[i - 1]) + 1, int(inv[j - 1]) + 1) + tuple(r) for i, j, *r in self._exceptions]
+            self.pairs.set_exceptions(self.N, self._exceptions)
+        if self.solver is not None:
+            self.solver.set_charges(self.solver.charges[p])
+        self.nl.invalidate()
+        self._snap_ready = False
+        return perm
 
     def status(self, check=True, stream=None):
         """Boundary readback of the short-range list (raises on a latched capacity / coordinate error)."""

@@ -152,6 +152,7 @@ class FENEBonds:
         self._h = h
         self._release = L.adopt(self, h, L.lib.pdx_bonds_destroy)
         self.n, self.nbond, self.k, self.R0 = int(n), len(b), float(k), float(R0)
+        self.pairs = b.copy()
 
     @property
     def handle(self):
@@ -205,13 +206,7 @@ class PairForce:
         if isinstance(model, MixedSigmaLJ):
             L.check(L.lib.pdx_pair_set_sigmas(h, len(model.sigmas), model.sigmas.ctypes.data_as(C.POINTER(C.c_float))))
         elif isinstance(model, PairMatrixLJ):
-            import numpy as np
-            cm = [np.asfortranarray(m).reshape(-1, order="F") for m in (model.sigma, model.epsilon_m, model.rcut)]
-            cm = [np.ascontiguousarray(m) for m in cm]
-            D = C.POINTER(C.c_double)
-            L.check(L.lib.pdx_pair_set_types(h, len(model.types), model.sigma.shape[0],
-                                             model.types.ctypes.data_as(C.POINTER(C.c_int32)), cm[0].ctypes.data_as(D),
-                                             cm[1].ctypes.data_as(D), cm[2].ctypes.data_as(D)))
+            self._set_types(model)
         self.coulomb = coulomb
         self.bonds = None
         self.set_accumulation(accumulation)
@@ -268,6 +263,20 @@ class PairForce:
         L.check(L.lib.pdx_pair_info(self._h, out))
         return dict(n_energy_terms=int(out[0]), n_force_terms=int(out[1]), energy_fit_error=out[2],
                     force_fit_error=out[3])
+
+    def _set_types(self, model):
+        import numpy as np
+        cm = [np.ascontiguousarray(np.asarray(m, dtype=np.float64).reshape(-1, order="F"))
+              for m in (model.sigma, model.epsilon_m, model.rcut)]
+        D = C.POINTER(C.c_double)
+        L.check(L.lib.pdx_pair_set_types(self._h, len(model.types), model.sigma.shape[0],
+                                         model.types.ctypes.data_as(C.POINTER(C.c_int32)), cm[0].ctypes.data_as(D),
+                                         cm[1].ctypes.data_as(D), cm[2].ctypes.data_as(D)))
+
+    def set_sigmas(self, sigmas):
+        import numpy as np
+        s = np.ascontiguousarray(sigmas, dtype=np.float32)
+        L.check(L.lib.pdx_pair_set_sigmas(self._h, len(s), s.ctypes.data_as(C.POINTER(C.c_float))))
 
     def cutoff(self):
         """The largest pair cutoff of the model (``pdx_pair_cutoff``)."""

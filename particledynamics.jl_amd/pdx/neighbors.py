@@ -96,6 +96,18 @@ class DeviceNeighborWorkspace:
         L.check(L.lib.pdx_nve_kick(self._h, L.ptr(vx), L.ptr(vy), L.ptr(vz), L.ptr(fx), L.ptr(fy), L.ptr(fz),
                                    float(dt), float(mass), L.ptr(inv_mass), L.stream_handle(stream)))
 
+    def spatial_order(self, stream=None):
+        """Permutation into the spatial order of the last build (``pdx_nl_spatial_order``): int64
+        device tensor, slot -> 0-based particle id."""
+        import torch
+        perm = torch.empty(self.N, dtype=torch.int32, device="cuda")
+        L.check(L.lib.pdx_nl_spatial_order(self._h, L.ptr(perm), L.stream_handle(stream)))
+        return perm.long()
+
+    def invalidate(self, stream=None):
+        """Particles were relabelled: the next check rebuilds (``pdx_nl_invalidate``)."""
+        L.check(L.lib.pdx_nl_invalidate(self._h, L.stream_handle(stream)))
+
     def build(self, stream=None):
         """Second half of ``update``: the rebuild kernels, predicated on the check's decision."""
         L.check(L.lib.pdx_nl_build(self._h, L.stream_handle(stream)))

@@ -126,3 +126,68 @@ def test_engine_on_unrelated_configuration(cuda, graph):
         if rep == 1:  # both spread from a valid snapshot of B's build: the same kernels, bitwise
             for x, y in zip(got, ref):
                 assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("system", ["rock_salt", "melt"])
+def test_spatial_reorder_is_label_independent(cuda, system):
+    """Spatial reordering (src/engine/Ordering.jl:19-79): a configuration and a random relabelling
+    of it, each reordered by its engine, end in the same storage order bit for bit (cells, then
+    position inside a cell), and then give bitwise equal rows, forces, energies and virials; the
+    tags map every slot back to its original particle."""
+    import torch
+    if system == "rock_salt":
+        s = systems.rock_salt(12, 2.82, 0.1, seed=2)
+        N = s["N"]
+        arrays = {k: s[k] for k in ("x", "y", "z", "q")}
+
+        def engine(q):
+            spec = pdx.PME(coulomb_prefactor=14.399645, real_cutoff=6.0, real_space_tolerance=1e-5,
+                           mesh_spacing=1.0, order=5)
+            return pdx.ForceEngine(N, s["box"], pdx.LennardJones(0.01, 2.8, 6.0), electrostatics=spec, charges=q,
+                                   skin=0.5)
+    else:
+        from pdx import melt
+        s = systems.kg_chains(60, length=30, density=0.85, seed=5)
+        melt.push_off(s, iterations=100)
+        N = s["N"]
+        arrays = {k: s[k] for k in ("x", "y", "z")}
+        arrays["q"] = None
+
+        def engine(q, bonds):
+            return pdx.ForceEngine(N, s["box"], pdx.WCA(), skin=0.4, bonds=pdx.FENEBonds(N, bonds),
+                                   accumulation="fixed64")
+    rng = np.random.default_rng(9)
+    sigma = rng.permutation(N)
+    inv = np.argsort(sigma)
+    results = []
+    for relabel in (False, True):
+        idx = sigma if relabel else np.arange(N)
+        P = [dev(arrays[k][idx]) for k in ("x", "y", "z")]
+        Q = dev(arrays["q"][idx]) if arrays["q"] is not None else None
+        tags = torch.from_numpy(idx.astype(np.int64)).cuda()  # slot -> original id
+        if system == "rock_salt":
+            eng = engine(arrays["q"][idx])
+        else:
+            bonds = inv[s["bonds"]] if relabel else s["bonds"]
+            eng = engine(None, bonds)
+        f = [torch.empty(N, device="cuda") for _ in range(4)]
+        V = torch.empty(6, N, device="cuda")
+        eng.evaluate(*P, Q, *f, V, energy=True)
+        eng.reorder(*P, Q, tags=tags)
+        eng.evaluate(*P, Q, *f, V, energy=True)
+        eng.evaluate(*P, Q, *f, V, energy=True)
+        tot = eng.totals()
+        rows = eng.nl.rows_host()
+        results.append(([t.clone() for t in P], tags.clone(), [t.clone() for t in f], V.clone(), tot["energy"],
+                        list(tot["virial"]), rows))
+    (Pa, ta, fa, Va, Ea, Wa, ra), (Pb, tb, fb, Vb, Eb, Wb, rb) = results
+    for a, b in zip(Pa, Pb):
+        assert torch.equal(a, b)
+    assert torch.equal(ta, tb)  # both slots hold the same original particle
+    assert ra == rb
+    for a, b in zip(fa, fb):
+        assert torch.equal(a, b)
+    assert torch.equal(Va, Vb) and Ea == Eb and Wa == Wb
+    # tags: the slot's coordinates are the tagged particle's
+    x0 = arrays["x"][ta.cpu().numpy()]
+    assert np.array_equal(Pa[0].cpu().numpy(), x0)

@@ -145,10 +145,42 @@ def test_explicit_positions_and_refusals(cuda):
     with pytest.raises(pdx.ArgumentError):
         pdx.PairForce(pdx.MixedSigmaLJ(sig), lanes=-4)
     with pytest.raises(pdx.ArgumentError):
-        pdx.ClusterPairEngine(500, (L, L, L), pdx.PairForce(pdx.MixedSigmaLJ(sig)), 0.3, 0.1)
-    with pytest.raises(pdx.ArgumentError):
         pdx.PairMatrixLJ(np.array([1, 2, 4], np.int32), np.ones((3, 3)), np.ones((3, 3)), np.ones((3, 3)))
         pdx.PairForce(pdx.PairMatrixLJ(np.array([1, 2, 4], np.int32), np.ones((3, 3)), np.ones((3, 3)),
                                        np.ones((3, 3))))
     with pytest.raises(pdx.DimensionMismatch):
         pf.compute(pdx.DeviceNeighborWorkspace(400, (L, L, L), 2.5, 0.3, 64), *f)
+
+
+@pytest.mark.parametrize("style", ["mixed", "matrix"])
+def test_typed_cluster_tiles(cuda, style):
+    """The per-type table on the cluster tiles (lj_type_table, src/clusterpairs/PairFunctor.jl:
+    166-179; the mixed-sigma kind LJ_MIXED): forces, energy and virial against the typed FP64 direct
+    sums, as on the rows."""
+    import torch
+    P, L, rng = _system(seed=5)
+    n = len(P)
+    if style == "mixed":
+        sig = np.where(rng.random(n) < 0.5, 0.9, 1.25).astype(np.float32)
+        model = pdx.MixedSigmaLJ(sig, epsilon=1.1, rcut_factor=2.5)
+        kw = dict(style="mixed", epsilon=1.1, sigma_particle=sig.astype(np.float64), factor=2.5)
+    else:
+        typ = rng.integers(1, 4, n).astype(np.int32)
+        S = np.array([[1.0, 0.9, 1.2], [0.9, 0.8, 1.1], [1.2, 1.1, 1.3]])
+        Ep = np.array([[1.0, 0.5, 2.0], [0.5, 1.5, 0.7], [2.0, 0.7, 1.2]])
+        Rc = 2.2 * S
+        model = pdx.PairMatrixLJ(typ, S, Ep, Rc)
+        kw = dict(style="matrix", typeid=typ, sigma_pair=S, epsilon_pair=Ep, rcut_pair=Rc)
+    eng = pdx.ForceEngine(n, (L, L, L), model, skin=0.3, backend="cluster", prune_buffer=0.1)
+    X, Y, Z = (dev(P[:, d]) for d in range(3))
+    f = [torch.empty(n, device="cuda") for _ in range(3)]
+    E = torch.empty(n, device="cuda")
+    V = torch.empty(6, n, device="cuda")
+    eng.evaluate(X, Y, Z, None, *f, E, V, energy=True)
+    tot = eng.totals()
+    Pd = P.astype(np.float64)
+    ref = oracle.pair_typed((Pd[:, 0], Pd[:, 1], Pd[:, 2]), (L, L, L), kind="lj", **kw)
+    F = torch.stack(f, 1).double().cpu().numpy()
+    t = np.array([tot["pair_energy"]] + list(tot["pair_virial"]))
+    err = _check(F, E.double().cpu().numpy(), V.double().cpu().numpy().T, t, ref["F"], ref["energy"], ref["virial"])
+    print(f"[typed] cluster {style}: force {err:.2e}", flush=True)
