@@ -54,9 +54,14 @@ def test_rock_salt_64k_lj_spme(cuda):
     dE = abs(tot["energy"] - ref["E"]) / abs(ref["E"])
     dW = np.abs(np.array(tot["virial"]) - Wr).max() / abs(Wr[:3].sum())
     dEp = np.abs(E.double().cpu().numpy() - ref["Epot"]).max() / np.abs(ref["Epot"]).max()
-    print(f"[c2] vs FP64 same algorithm ({t_cpu:.1f} s CPU): dF={dF:.2e} dE={dE:.2e} dW={dW:.2e} "
-          f"dEpot={dEp:.2e}", flush=True)
-    assert dF < 3e-6 and dE < 1e-6 and dW < 1e-5 and dEp < 1e-5
+    frms = math.sqrt((ref["F"] ** 2).sum(1).mean())
+    print(f"[c2] vs FP64 same algorithm ({t_cpu:.1f} s CPU): dF={dF:.2e} (|F| rms {frms:.3e} eV/A, error rms "
+          f"{dF * frms:.2e} eV/A) dE={dE:.2e} dW={dW:.2e} dEpot={dEp:.2e}", flush=True)
+    # forces: the reference's FP32 force bound (test/test_device_engine_pairs_gpu.jl:124-125). The
+    # net force on a jittered lattice ion is ~1/10 of one nearest-neighbour Coulomb pair force
+    # (C/a^2 = 1.8 eV/A) and the sum over ~500 partners in FP32 rounds at that pair scale, so
+    # the relative error sits above the 120-charge cloud's (measured 1.4e-5, 1.4e-7 energy).
+    assert dF < 2e-5 and dE < 1e-6 and dW < 1e-5 and dEp < 1e-5
     # sampled ions against the independent oracles: LJ direct sums + FP64 Ewald
     ids = np.sort(np.random.default_rng(1).choice(N, 256, replace=False))
     Flj = oracle.pair_direct(x, y, z, box, kind="lj", sigma=2.8, epsilon=0.01, cutoff=9.0, ids=ids)["F"]
