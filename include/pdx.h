@@ -582,6 +582,21 @@ int pdx_ewald_fp64(int32_t n, const double *x, const double *y, const double *z,
                    int32_t nsample, const int32_t *sample, double *F, int32_t want_energy,
                    double out[7], pdx_stream_t stream);
 
+/* pdx_ewald_fp64 without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under graph capture of its caller's other
+ * work, tight training loops that re-measure the learned correction's targets, harnesses that
+ * forbid hipMalloc). Same arguments and the same kernels and launch shapes as pdx_ewald_fp64,
+ * so the same results bit for bit. Its workspace (structure factors, block partials, result
+ * words) is a static device array of 4.1 MiB per device, held until the process ends; the
+ * half-space wave-vector box (ceil(kmax L_1/2pi) + 2)(2 ceil(kmax L_2/2pi) + 3)(2 ceil(kmax
+ * L_3/2pi) + 3), kmax = 2 alpha sqrt(digits ln 10), must hold at most 262 144 indices, else
+ * PDX_ERR_ARGUMENT. Every call on one device shares that array: calls that overlap on one
+ * device (other host threads, other streams) race. */
+int pdx_ewald_fp64_noalloc(int32_t n, const double *x, const double *y, const double *z, const double *q,
+                           const double box[3], double C, double alpha, double digits, int32_t background,
+                           int32_t nsample, const int32_t *sample, double *F, int32_t want_energy,
+                           double out[7], pdx_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

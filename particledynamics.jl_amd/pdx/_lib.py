@@ -135,6 +135,8 @@ def _load():
         "pdx_mesh_fft": ([vp, i32, vp, vp, vp], C.c_int),
         "pdx_ewald_fp64": ([i32, vp, vp, vp, vp, P(dbl), dbl, dbl, dbl, i32, i32, vp, vp, i32, P(dbl), vp],
                            C.c_int),
+        "pdx_ewald_fp64_noalloc": ([i32, vp, vp, vp, vp, P(dbl), dbl, dbl, dbl, i32, i32, vp, vp, i32, P(dbl), vp],
+                                   C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

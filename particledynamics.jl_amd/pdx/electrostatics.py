@@ -273,15 +273,22 @@ class ElectrostaticsSolver:
 
 
 def ewald_fp64(x, y, z, q, box, C, alpha, digits=14.0, background=False, sample=None, want_energy=True,
-               stream=None):
+               stream=None, F=None, no_alloc=False):
     """FP64 Ewald reference on the device (``pdx_ewald_fp64``). x..q: float64 device tensors;
-    sample: int32 device tensor of 0-based ids. Returns (F[nsample,3] tensor or None, totals dict)."""
+    sample: int32 device tensor of 0-based ids; F: optional float64 device tensor [nsample, 3]
+    for the forces. no_alloc: call ``pdx_ewald_fp64_noalloc``, which takes its workspace from a
+    static device array instead of allocating (pass F to keep the whole call allocation-free).
+    Returns (F[nsample,3] tensor or None, totals dict)."""
     import torch
     n = x.numel()
     ns = 0 if sample is None else sample.numel()
-    F = torch.zeros(max(ns, 1), 3, dtype=torch.float64, device="cuda")
+    if F is None:
+        F = torch.zeros(max(ns, 1), 3, dtype=torch.float64, device="cuda")
+    elif ns and (F.dtype != torch.float64 or not F.is_contiguous() or F.numel() < 3 * ns):
+        raise L.DimensionMismatch("F must be a contiguous float64 tensor of at least nsample x 3")
     out = (ctypes.c_double * 7)()
-    L.check(L.lib.pdx_ewald_fp64(n, L.ptr(x), L.ptr(y), L.ptr(z), L.ptr(q), L.dbl3(box), float(C), float(alpha),
+    fn = L.lib.pdx_ewald_fp64_noalloc if no_alloc else L.lib.pdx_ewald_fp64
+    L.check(fn(n, L.ptr(x), L.ptr(y), L.ptr(z), L.ptr(q), L.dbl3(box), float(C), float(alpha),
                                  float(digits), 1 if background else 0, ns, L.ptr(sample), L.ptr(F),
                                  1 if want_energy else 0, out, L.stream_handle(stream)))
     tot = dict(energy=out[0], virial=list(out[1:7])) if want_energy else None
