@@ -17,10 +17,15 @@ Energy E = 1/2 sum_m |S(m)|^2 G(m), virial W_ab = 1/2 sum_m |S(m)|^2 Vc_ab(m) (t
 spectrum weighted 1/2 at m1 = 0 and the x Nyquist), self energy -C alpha/sqrt(pi) sum q^2,
 background -C pi Q^2 / (2 alpha^2 V) (src/electrostatics/Solver.jl:382-429).
 
+ESP (the prolate split and window of oracle/prolate.py) runs through the same transform and the
+alias-sum influence with the window's transform (MeshESP, evaluate_esp at the end of this file).
+
 Pinned in tests/test_oracle.py: the total (real space + mesh + self) against or_ewald on the
-reference's 120-charge case (test/test_electrostatics.jl:299-340); F = -dE/dx and
-W_aa = -dE/d eps_aa by central differences (test/test_electrostatics.jl:341-375); charge
-conservation of the spread; the P3M-AD error converging in the alias order.
+reference's 120-charge case (test/test_electrostatics.jl:299-340), for SPME and for ESP at the
+reference's ESP case (tolerance 1e-6, 32 x 32 x 36, width 8: forces < 1e-5, energy < 1e-6);
+F = -dE/dx and W_aa = -dE/d eps_aa by central differences (test/test_electrostatics.jl:341-375),
+for ESP with the background; charge conservation of the spread; the P3M-AD error converging in
+the alias order.
 """
 from __future__ import annotations
 
@@ -100,17 +105,34 @@ def _alias(P, K, nmax):
     return sn ** P  # [m, n + nmax]
 
 
+def _gaussian_spectral(alpha):
+    i4a2 = 1.0 / (4 * alpha ** 2)
+
+    def g_of(qs):
+        g = 4 * np.pi * np.exp(-qs * i4a2) / qs
+        return g, g * (-2.0 / qs - 2.0 * i4a2)
+    return g_of
+
+
 def influence_p3m_ad(K, box, C_, alpha, P, nmax=2):
     """Optimal influence function for analytic differentiation (and its -dG/d eps_ab)."""
     K1, K2, K3 = K
+    return influence_alias_ad(K, box, C_, (_alias(P, K1, nmax), _alias(P, K2, nmax), _alias(P, K3, nmax)),
+                              _gaussian_spectral(alpha), nmax)
+
+
+def influence_alias_ad(K, box, C_, tables, spectral, nmax):
+    """AD-optimal influence G = (C/V) sum U^2 g k_n^2 / (sum U^2 . sum U^2 k_n^2) over |n_d| <= nmax
+    for window tables U_d[m, n + nmax] and a spectral function g(k^2) -> (g, g'(k)/k)
+    (_influence_alias_kernel!, src/electrostatics/Influence.jl:77-181), with its -dG/d eps_ab."""
+    K1, K2, K3 = K
     (m1, m2, m3), (kx, ky, kz), (ny1, ny2, ny3) = _modes(K, box)
     V = box[0] * box[1] * box[2]
-    U1, U2, U3 = _alias(P, K1, nmax)[m1], _alias(P, K2, nmax)[m2], _alias(P, K3, nmax)[m3]
+    U1, U2, U3 = tables[0][m1], tables[1][m2], tables[2][m3]
     shape = (K3, K2, K1 // 2 + 1)
     A = np.zeros(shape); B = np.zeros(shape); Cc = np.zeros(shape)
     a = [np.zeros(shape) for _ in range(6)]
     c = [np.zeros(shape) for _ in range(6)]
-    i4a2 = 1.0 / (4 * alpha ** 2)
     for i1, n1 in enumerate(range(-nmax, nmax + 1)):
         q1 = kx + 2 * np.pi * K1 * n1 / box[0]
         u1 = U1[:, i1][None, None, :]
@@ -125,8 +147,9 @@ def influence_p3m_ad(K, box, C_, alpha, P, nmax=2):
                 qq = q1 ** 2 + q2 ** 2 + q3 ** 2
                 nz = qq != 0
                 qs = np.where(nz, qq, 1.0)
-                g = np.where(nz, 4 * np.pi * np.exp(-qs * i4a2) / qs, 0.0)
-                gpk = g * (-2.0 / qs - 2.0 * i4a2)
+                g, gpk = spectral(qs)
+                g = np.where(nz, g, 0.0)
+                gpk = np.where(nz, gpk, 0.0)
                 A += U2t * g * qq
                 Cc += U2t * qq
                 h = U2t * (gpk * qq + 2 * g)
@@ -240,3 +263,153 @@ def evaluate(x, y, z, q, box, C_, alpha, rc, K, P, influence="spme", alias_order
     return dict(E=float(pr["E"].sum()) + Em + Es + Eb, F=pr["F"] + Fm, W=Wp + Wm + Wb, Epot=Ep,
                 mesh_energy=Em, self_energy=Es, background_energy=Eb, pair_energy=float(pr["E"].sum()),
                 pairs=pr["pairs"])
+
+
+# ---------------------------------------------------------------------------------------------
+# ESP (Ewald summation with prolates): the same evaluation with the prolate split and window of
+# oracle/prolate.py. Spread and gather walk each particle's centred P^3 stencil in FP64
+# (_spread_lines_kernel! / _gather_kernel! with window_stencil, src/electrostatics/MeshKernels.jl:
+# 247-333, Windows.jl:239-250); the influence is the AD-optimal alias sum over the window's
+# transform (:esp) or the plain deconvolution g / phi^2 (:esp_deconvolution, the alias sum at
+# nmax = 0); the real space sums C q_i q_j E_s(r/rc)/r over the pairs within rc (Splits.jl:91-130);
+# self energy -(C/2) chi(0)/rc sum q^2 and background -C pi rc^2 m_2 Q^2/V (Splits.jl:67-86).
+
+
+def _grid_coordinate(x, L, K):
+    s = np.asarray(x, dtype=np.float64) / L
+    s = s - np.floor(s)
+    u = s * K
+    return np.where(u >= K, u - K, u)
+
+
+def _stencils(x, y, z, box, K, window):
+    out = []
+    for c, L, Kd in zip((x, y, z), box, K):
+        k0, w, dw = window.stencil(_grid_coordinate(c, L, Kd))
+        nodes = (k0[:, None] + np.arange(window.P)[None, :]) % Kd
+        out.append((nodes, w, dw))
+    return out
+
+
+def esp_spread(x, y, z, q, box, K, window):
+    """FP64 charge grid of the prolate window, shape (K3, K2, K1)."""
+    K1, K2, K3 = K
+    (n1, w1, _), (n2, w2, _), (n3, w3, _) = _stencils(x, y, z, box, K, window)
+    q = np.asarray(q, dtype=np.float64)
+    grid = np.zeros(K1 * K2 * K3)
+    for j3 in range(window.P):
+        for j2 in range(window.P):
+            idx = (n3[:, j3] * K2 + n2[:, j2])[:, None] * K1 + n1
+            grid += np.bincount(idx.ravel(), weights=(q[:, None] * w3[:, j3, None] * w2[:, j2, None] * w1).ravel(),
+                                minlength=grid.size)
+    return grid.reshape(K3, K2, K1)
+
+
+def esp_gather(x, y, z, q, box, K, window, pot):
+    """Forces -q grad phi (n, 3) and phi (n) from the potential grid (K3, K2, K1)."""
+    K1, K2, K3 = K
+    (n1, w1, d1), (n2, w2, d2), (n3, w3, d3) = _stencils(x, y, z, box, K, window)
+    p = np.asarray(pot, dtype=np.float64).ravel()
+    phi = np.zeros(len(x))
+    g = np.zeros((len(x), 3))
+    for j3 in range(window.P):
+        for j2 in range(window.P):
+            v = p[(n3[:, j3] * K2 + n2[:, j2])[:, None] * K1 + n1]
+            a = (w1 * v).sum(1)
+            b = (d1 * v).sum(1)
+            phi += w3[:, j3] * w2[:, j2] * a
+            g[:, 0] += w3[:, j3] * w2[:, j2] * b
+            g[:, 1] += w3[:, j3] * d2[:, j2] * a
+            g[:, 2] += d3[:, j3] * w2[:, j2] * a
+    q = np.asarray(q, dtype=np.float64)
+    F = -q[:, None] * g * (np.asarray(K, dtype=np.float64) / np.asarray(box, dtype=np.float64))[None, :]
+    return F, phi
+
+
+class MeshESP:
+    """Influence function and window of one ESP choice (K, box, C, rc, P, c, c1)."""
+
+    def __init__(self, K, box, C_, rc, P, split_shape, window_shape, influence="esp", alias_order=2):
+        import prolate
+        self.K = tuple(int(k) for k in K)
+        self.box = tuple(float(b) for b in box)
+        self.C = float(C_)
+        self.split = prolate.Split(rc, split_shape)
+        self.window = prolate.Window(P, window_shape)
+        nmax = 0 if influence == "esp_deconvolution" else int(alias_order)
+        tables = tuple(self.window.alias_table(k, nmax) for k in self.K) if nmax > 0 else \
+            tuple(self.window.transform(2 * np.pi * np.where(np.arange(k) <= k // 2, np.arange(k), np.arange(k) - k)
+                                        / k)[:, None] for k in self.K)
+        self.G, self.Vc = influence_alias_ad(self.K, self.box, self.C, tables,
+                                             lambda qs: self.split.spectral(np.sqrt(qs)), nmax)
+        K1 = self.K[0]
+        w = np.ones(K1 // 2 + 1)
+        w[0] = 0.5
+        if K1 % 2 == 0:
+            w[-1] = 0.5
+        self.w = w[None, None, :]
+
+    def evaluate(self, x, y, z, q):
+        """Mesh energy, forces (n, 3), per-particle 1/2 q phi, virial (6)."""
+        import scipy.fft as sf
+        grid = esp_spread(x, y, z, q, self.box, self.K, self.window)
+        S = sf.rfftn(grid, workers=_workers())
+        b2 = self.w * (S.real ** 2 + S.imag ** 2)
+        E = float((b2 * self.G).sum())
+        W = np.array([float((b2 * v).sum()) for v in self.Vc])
+        pot = sf.irfftn(S * self.G, s=grid.shape, workers=_workers()) * grid.size
+        F, phi = esp_gather(x, y, z, q, self.box, self.K, self.window, pot)
+        return E, F, 0.5 * np.asarray(q, dtype=np.float64) * phi, W
+
+
+def esp_real_space(x, y, z, q, box, C_, split):
+    """C q_i q_j E_s(r/rc)/r over the pairs within rc (minimum image, rc < L/2): per-particle
+    half energies, forces (n, 3) and the virial (6: xx yy zz xy xz yz)."""
+    from scipy.spatial import cKDTree
+    L = np.asarray(box, dtype=np.float64)
+    R = np.stack([np.asarray(c, dtype=np.float64) for c in (x, y, z)], 1)
+    Rw = R - L * np.floor(R / L)
+    Rw = np.where(Rw >= L, Rw - L, Rw)
+    pairs = cKDTree(Rw, boxsize=L).query_pairs(split.rc, output_type="ndarray")
+    n = len(R)
+    E = np.zeros(n)
+    F = np.zeros((n, 3))
+    W = np.zeros(6)
+    if len(pairs):
+        i, j = pairs[:, 0], pairs[:, 1]
+        d = R[i] - R[j]
+        d -= L * np.round(d / L)
+        r = np.sqrt((d * d).sum(1))
+        e, f = split.pair(r)
+        qq = C_ * np.asarray(q, dtype=np.float64)[i] * np.asarray(q, dtype=np.float64)[j]
+        fv = (qq * f)[:, None] * d
+        for c in range(3):
+            F[:, c] += np.bincount(i, weights=fv[:, c], minlength=n) - np.bincount(j, weights=fv[:, c], minlength=n)
+        eh = 0.5 * qq * e
+        E += np.bincount(i, weights=eh, minlength=n) + np.bincount(j, weights=eh, minlength=n)
+        for k, (a, b) in enumerate(((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))):
+            W[k] = float((qq * f * d[:, a] * d[:, b]).sum())
+    return E, F, W
+
+
+def evaluate_esp(x, y, z, q, box, C_, rc, K, P, split_shape, window_shape, influence="esp", alias_order=2,
+                 background=False, mesh=None):
+    """Full FP64 ESP Coulomb evaluation: real space + mesh + self (+ background). Returns dict E,
+    F (n, 3), W (6), Epot (n) and the parts."""
+    m = mesh or MeshESP(K, box, C_, rc, P, split_shape, window_shape, influence, alias_order)
+    qd = np.asarray(q, dtype=np.float64)
+    Er, Fr, Wr = esp_real_space(x, y, z, q, box, C_, m.split)
+    Em, Fm, Epm, Wm = m.evaluate(x, y, z, q)
+    sc = m.split.self_coefficient()
+    Es = -0.5 * C_ * sc * float((qd ** 2).sum())
+    Ep = Er + Epm - 0.5 * C_ * sc * qd ** 2
+    Eb = 0.0
+    Wb = np.zeros(6)
+    if background:
+        V = box[0] * box[1] * box[2]
+        Q = float(qd.sum())
+        Eb = -C_ * m.split.background_coefficient() * Q * Q / V
+        Ep = Ep - C_ * m.split.background_coefficient() * Q / V * qd
+        Wb[:3] = Eb
+    return dict(E=float(Er.sum()) + Em + Es + Eb, F=Fr + Fm, W=Wr + Wm + Wb, Epot=Ep, mesh_energy=Em,
+                self_energy=Es, background_energy=Eb, pair_energy=float(Er.sum()))

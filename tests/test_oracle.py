@@ -578,3 +578,139 @@ def test_kg_oracle_bond_exclusion():
     assert np.abs(dF - two).max() < 1e-9 * np.abs(two).max()
     assert abs((full["energy"] - ex["energy"]) - e2) < 1e-9 * abs(e2)
     assert np.abs(two).max() > 1.0
+
+
+# ---------------------------------------------------------------- ESP (prolate split and window)
+import prolate  # noqa: E402
+
+
+def test_prolate_shapes_of_table_2():
+    """Golden shapes of Table 2 of Liang et al., as the reference pins them
+    (test/test_electrostatics.jl:54-57)."""
+    for eps, c in ((1e-3, 9.5392), (5e-4, 10.290), (1e-4, 12.024), (5e-5, 12.762), (1e-5, 14.471)):
+        assert prolate.shape_for_tolerance(eps) == pytest.approx(c, rel=5e-4)
+
+
+def test_prolate_function_identities():
+    """psi_0^c is an eigenfunction of the finite Fourier transform on [-1, 1] (the ratio of its
+    transform to itself is constant), its derivative is the central difference, psi(0) = 1, and
+    its integral is the quadrature of its values (test/test_electrostatics.jl:58-63)."""
+    pf = prolate.ProlateFunction(10.0)
+    x, w = prolate.gauss_legendre(160)
+    ratio = lambda t: float((w * pf(x) * np.cos(10.0 * t * x)).sum() / pf(t))
+    assert ratio(0.2) == pytest.approx(ratio(0.9), rel=1e-10)
+    assert float(pf.derivative(0.3)) == pytest.approx(float((pf(0.3 + 1e-6) - pf(0.3 - 1e-6)) / 2e-6), rel=1e-7)
+    assert float(pf(0.0)) == pytest.approx(1.0, abs=1e-14)
+    xq, wq = prolate.gauss_legendre(64, 0.0, 0.7)
+    assert float(pf.integral(0.7)) == pytest.approx(float((wq * pf(xq)).sum()), rel=1e-13)
+    # the tolerance rule: psi(1) / ||psi|| = eps at the rule's shape
+    c = prolate.shape_for_tolerance(1e-4)
+    p = prolate.ProlateFunction(c)
+    assert float(p(1.0)) / p.l2_norm() == pytest.approx(1e-4, rel=1e-8)
+
+
+def test_prolate_window_weights():
+    """The centred stencil: u = 10.5, P = 6 starts at node 8 with weights symmetric about the
+    particle, w_3 = psi(1/6) - psi(1) (test/test_electrostatics.jl:88-91); weights vanish at the
+    support edge, so they are continuous where the stencil base changes (even P at integer u, odd
+    P at half-integer u), and dw is the u-derivative of w."""
+    pw = prolate.Window(6, 12.0)
+    k0, w, _ = pw.stencil(np.array([10.5]))
+    assert k0[0] == 8
+    assert w[0, 2] == pytest.approx(w[0, 3], rel=1e-12)
+    assert w[0, 2] == pytest.approx(float(pw.pf(1 / 6)) - pw.edge, rel=1e-12)
+    for P, eps in ((4, 2.5e-4), (5, 5e-4), (6, 5e-5), (8, 5e-6)):
+        win = prolate.Window(P, prolate.shape_for_tolerance(eps))
+        u0 = 7.0 if P % 2 == 0 else 7.5
+        kL, wL, _ = win.stencil(np.array([u0 - 1e-12]))
+        kR, wR, _ = win.stencil(np.array([u0]))
+        nodes = {}
+        for k, ww, sgn in ((kL[0], wL[0], 1), (kR[0], wR[0], -1)):
+            for j in range(P):
+                nodes[k + j] = nodes.get(k + j, 0.0) + sgn * ww[j]
+        assert kL[0] != kR[0]
+        assert max(abs(v) for v in nodes.values()) < 1e-10
+        u = np.array([13.37])
+        _, wp, _ = win.stencil(u + 1e-7)
+        _, wm, _ = win.stencil(u - 1e-7)
+        _, _, dw = win.stencil(u)
+        assert np.abs((wp - wm) / 2e-7 - dw).max() < 1e-6
+
+
+def test_prolate_split_identities():
+    """E_s(0) = 1, E_s vanishes at and beyond the cutoff, E_s(t) = 1 - chi(0) t + O(t^3), the force
+    factor is -dE/dr / r (test/test_electrostatics.jl:281-297), and the long-range correction is
+    1/r - E_s/r with fcorr = -(1/r) d corr/dr."""
+    sp = prolate.Split(3.0, 11.0)
+    assert float(sp.Es(0.0)) == 1.0
+    assert float(sp.Es(1.0)) == pytest.approx(0.0, abs=1e-15)
+    e, f = sp.pair(np.array([3.0, 3.5]))
+    assert (e == 0).all() and (f == 0).all()
+    assert float(sp.pair(np.array([1e-3]))[0][0]) * 1e-3 == pytest.approx(1 - sp.chi0 * 1e-3 / 3.0, abs=1e-9)
+    r = np.linspace(0.3, 2.9, 40)
+    h = 1e-6
+    dE = (sp.pair(r + h)[0] - sp.pair(r - h)[0]) / (2 * h)
+    assert np.abs(-dE - sp.pair(r)[1] * r).max() < 1e-7
+    for rr in (0.5, 1.7, 2.9):
+        corr, fcorr = sp.correction(rr * rr)
+        assert corr == pytest.approx(1 / rr - float(sp.pair(np.array([rr]))[0][0]), rel=1e-10)
+        cp = sp.correction((rr + h) ** 2)[0]
+        cm = sp.correction((rr - h) ** 2)[0]
+        assert fcorr == pytest.approx(-(cp - cm) / (2 * h) / rr, rel=1e-6)
+
+
+def test_cpu_esp_against_ewald():
+    """The FP64 ESP evaluation (real space + mesh + self) on the reference's 120-charge case
+    (test/test_electrostatics.jl:299-320: ESP at tolerance 1e-6, mesh 32 x 32 x 36, width 8):
+    forces < 1e-5 relative RMS, energy < 1e-6, virial < 1e-4 of its trace; the per-particle
+    energies sum to the total; the plain deconvolution is less accurate than the optimal one."""
+    R, q = _cloud(120, L_ES, 3)
+    Er, Fr, Wr = oracle.ewald(R, q, np.diag(L_ES), C_=1.7)
+    c, c1 = prolate.shape_for_tolerance(1e-6), prolate.shape_for_tolerance(5e-7)
+    out = cpu_mesh.evaluate_esp(R[:, 0], R[:, 1], R[:, 2], q, L_ES, 1.7, 4.0, (32, 32, 36), 8, c, c1)
+    err = math.sqrt(((out["F"] - Fr) ** 2).sum() / (Fr ** 2).sum())
+    assert err < 1e-5
+    assert abs(out["E"] - Er) / abs(Er) < 1e-6
+    W = out["W"]
+    Wm = np.array([[W[0], W[3], W[4]], [W[3], W[1], W[5]], [W[4], W[5], W[2]]])
+    assert np.abs(Wm - Wr).max() / abs(np.trace(Wr)) < 1e-4
+    assert abs(out["Epot"].sum() - out["E"]) < 1e-10 * abs(out["E"])
+    dec = cpu_mesh.evaluate_esp(R[:, 0], R[:, 1], R[:, 2], q, L_ES, 1.7, 4.0, (32, 32, 36), 8, c, c1,
+                                influence="esp_deconvolution")
+    assert math.sqrt(((dec["F"] - Fr) ** 2).sum() / (Fr ** 2).sum()) > err
+
+
+def test_cpu_esp_gradient_and_strain():
+    """F = -dE/dx and W_aa = -dE/d eps_aa of the discrete ESP energy (test/test_electrostatics.jl:
+    341-375), with the net-charge background: the energy of the same algorithm is differentiated."""
+    C_, rc, K, P = 1.0, 3.8, (20, 21, 22), 6
+    L = (13.0, 13.5, 14.0)
+    R, q = _cloud(80, L, 9)
+    q = q + 0.02
+    c, c1 = prolate.shape_for_tolerance(1e-4), prolate.shape_for_tolerance(5e-5)
+    mesh = lambda Ll: cpu_mesh.MeshESP(K, Ll, C_, rc, P, c, c1)
+
+    def ev(Rr, Ll, m=None):
+        return cpu_mesh.evaluate_esp(Rr[:, 0], Rr[:, 1], Rr[:, 2], q, Ll, C_, rc, K, P, c, c1, background=True,
+                                     mesh=m)
+
+    m0 = mesh(L)
+    base = ev(R, L, m0)
+    Fmax = np.abs(base["F"]).max()
+    h = 1e-5
+    for i, d in ((2, 0), (39, 1), (76, 2)):
+        Rp = R.copy(); Rp[i, d] += h
+        Rm = R.copy(); Rm[i, d] -= h
+        fd = -(ev(Rp, L, m0)["E"] - ev(Rm, L, m0)["E"]) / (2 * h)
+        assert abs(fd - base["F"][i, d]) < 1e-6 * Fmax, (i, d, fd, base["F"][i, d])
+    W = base["W"]
+    tr = abs(W[0] + W[1] + W[2])
+    for a in range(3):
+        eps = 1e-5
+        Es = []
+        for e in (eps, -eps):
+            s = np.ones(3); s[a] += e
+            Ls = tuple(np.asarray(L) * s)
+            Es.append(ev(R * s, Ls, mesh(Ls))["E"])
+        fd = -(Es[0] - Es[1]) / (2 * eps)
+        assert abs(fd - W[a]) < 1e-6 * max(abs(W[a]), tr), (a, fd, W[a])
