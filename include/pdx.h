@@ -201,13 +201,15 @@ typedef struct {
   int32_t shift;          /* LJ potential-shift modifier (PairModels.jl:20-25) */
   double epsilon, sigma, cutoff;   /* cutoff: LJ only; WCA/SOFT fix their own */
   /* real-space Ewald Coulomb, C q_i q_j (1/r + P_e(r^2)) (PairModels.jl:90-110, PairFunctor.jl:127-156) */
-  int32_t coulomb;        /* 0 off, 1 Gaussian (Ewald) split */
+  int32_t coulomb;        /* 0 off, 1 Gaussian (Ewald) split, 2 prolate split (ESP; split_shape) */
   int32_t coulomb_shift;  /* potential-shift modifier on the Coulomb energy (Spec.jl:25-29) */
   double coulomb_C;       /* prefactor / relative permittivity */
   double coulomb_cutoff;  /* real-space cutoff */
   double alpha;           /* Ewald splitting parameter */
   int32_t lanes;          /* traversal: 0 automatic; 1..64 lanes of a wave per particle; -2,-4,-8,-16:
                              blocks of 64 particles whose rows are split over that many waves */
+  double split_shape;     /* prolate split: bandlimit c of ProlateSplit(cutoff, c) (Splits.jl:48-60); the
+                             real-space pair C q_i q_j E_s(r/rc)/r vanishes at rc, self coefficient chi(0)/rc */
 } pdx_pair_spec;
 
 /* Build a pair model (UniformLJ/UniformWCA/ChargedPairs constructors, PairModels.jl:16-18,
@@ -258,6 +260,8 @@ int pdx_pair_compute(pdx_pair_t p, pdx_nl_t nl, const float *x, const float *y, 
 #define PDX_INFLUENCE_SPME   1  /* smooth PME: G = (C/V) g(k) D1 D2 D3, analytic differentiation */
 #define PDX_INFLUENCE_P3M_AD 2  /* optimal influence for analytic differentiation (alias sums) */
 #define PDX_INFLUENCE_P3M_IK 3  /* optimal influence for ik differentiation, three field transforms */
+#define PDX_INFLUENCE_ESP    4  /* ESP: prolate split and window, optimal AD influence (alias sums) */
+#define PDX_INFLUENCE_ESP_DECONVOLUTION 5  /* ESP with the plain window deconvolution */
 #define PDX_NET_CHARGE_ERROR 0
 #define PDX_NET_CHARGE_BACKGROUND 1
 
@@ -267,20 +271,32 @@ typedef struct {
   double real_cutoff;
   double alpha;                  /* Gaussian split; <= 0: from real_space_tolerance */
   double real_space_tolerance;   /* erfc(alpha rc) = tol (alpha_from_tolerance, Spec.jl:154-163) */
-  int32_t order;                 /* B-spline order P, 3..12 */
+  int32_t order;                 /* B-spline order P, 3..12; ESP: prolate window width, 4..8 */
   int32_t mesh[3];               /* K1, K2, K3; mesh[0] <= 0: from mesh_spacing (resolve_mesh, Spec.jl:141-146) */
   double mesh_spacing;
   int32_t influence;             /* PDX_INFLUENCE_* (INFLUENCE_KINDS, Spec.jl:5-24) */
   int32_t net_charge;            /* PDX_NET_CHARGE_* */
   double net_charge_tolerance;
   int32_t real_space_modifier;   /* 0 none, 1 potential shift */
-  int32_t alias_order;           /* P3M alias sums |n_d| <= alias_order, 0..6 (Spec.jl:80) */
+  int32_t alias_order;           /* P3M / ESP alias sums |n_d| <= alias_order, 0..6 (Spec.jl:80) */
+  /* ESP (ProlateSplit / ProlateWindow, Splits.jl:30-60, Windows.jl:51-80): the split's bandlimit c
+   * (the real-space part vanishes at real_cutoff) and the window's c1; the window must satisfy
+   * order <= 2 c1/pi - 1/2 (Spec.jl:74-75) and be resolved in FP32 (its Nyquist attenuation at
+   * least 0.02 eps32^(2/3), Windows.jl:310-330). Ignored by the other influence kinds. */
+  double split_shape;
+  double window_shape;
 } pdx_mesh_spec;
 
 /* Allocate the mesh workspace and compute the FP64 influence function for the box
  * (ElectrostaticsSolver, Solver.jl:129-158; compute_influence!, Influence.jl:190-227). */
 int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const double box[3], pdx_mesh_t *out);
 int pdx_mesh_destroy(pdx_mesh_t m);
+/* ESP defaults of a target relative force error (the ESP descriptor's rules for an FP32 state,
+ * src/workflow/Electrostatics.jl:102-215,288-301): out[0] = split shape c with psi(1)/||psi|| = tol,
+ * out[1] = window width (order > 0 keeps it; else Table 2 of Liang et al.: 5, 5, 6, 7, 8 for
+ * 1e-3 .. 1e-5), out[2] = window shape c(tol/2), or the smallest larger shape an FP32 mesh
+ * resolves. The default mesh spacing is pi real_cutoff / c. Host only. */
+int pdx_esp_rule(double tolerance, int32_t order, double out[3]);
 /* Resolved parameters: out[0..8] = K1, K2, K3, P, alpha, C, self_coefficient, bg (per unit q),
  * lines per plane of the plane transforms (0: line passes). */
 int pdx_mesh_info(pdx_mesh_t m, double out[9]);

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "pdx.h"
+#include "pdx_prolate.hpp"
 
 namespace pdx {
 
@@ -283,6 +284,13 @@ struct pdx_mesh_s {
   pdx::DevBuf<double> Utab;                // alias tables (2 nmax + 1) x K per dimension
   bool pot_valid = false;                  // ik: the potential grid of the last solve exists
   int nmode_blocks = 0;
+  // window of the spread and gather: wkey = P (B-spline order P) or 16 + P (prolate window of
+  // width P, ESP) with its FP32 node polynomials; ESP's split quadrature (xq | wchi, 128 each)
+  int wkey = 0;
+  pdx::WinCoef wc{};
+  double split_shape = 0, window_shape = 0;
+  int window_degree = 0;
+  pdx::DevBuf<double> espq;
   int T1 = 0, T2 = 0, T3 = 0;  // FFT tile widths, fixed at creation
   int plane_tp = 0;            // plane passes: lines per plane padded to this power of two (0: line passes)
   int spread_threads = 512;    // workgroup size of the owner-computes spread (256, 512 or 1024)

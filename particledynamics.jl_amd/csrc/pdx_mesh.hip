@@ -12,6 +12,7 @@
 // Lines of a pass are staged in LDS as [element*T + line] for T adjacent lines, so global
 // loads and stores of a tile are contiguous runs of T complex values.
 #include "pdx_internal.hpp"
+#include "pdx_prolate.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -759,6 +760,75 @@ __device__ __forceinline__ int bspline(float u, float (&w)[P], float (&dw)[P]) {
   return (int)fl - (P - 1);
 }
 
+// ---- window codes of the spread and gather kernels: WP = P, the order-P B-spline, or
+// PROLATE + P, the prolate window of width P (ESP). Both windows cover P nodes ending at a base
+// node: the B-spline's floor(u), the prolate window's centred stencil floor(u) + P/2 (even P) or
+// floor(u + 1/2) + (P-1)/2 (odd P), i.e. the base of u + P/2. The tile spread and gather own a
+// particle by its base node, so they serve both windows alike.
+constexpr int PROLATE = 16;
+__host__ __device__ constexpr int win_width(int WP) { return WP & 15; }
+__host__ __device__ constexpr bool win_prolate(int WP) { return WP >= PROLATE; }
+// the shift of the stencil against the B-spline's (the base node of u is that of u + shift)
+template <int WP> __device__ __forceinline__ float win_shift() { return win_prolate(WP) ? 0.5f * win_width(WP) : 0.f; }
+
+template <int WP>
+__device__ __forceinline__ int window_base(float u) {
+  constexpr int P = win_width(WP);
+  if (!win_prolate(WP)) return (int)floorf(u);
+  if (P % 2 == 0) return (int)floorf(u) + P / 2;
+  return (int)floorf(u + 0.5f) + (P - 1) / 2;
+}
+
+// Prolate window weights (window_stencil of ProlatePolyStencil, src/electrostatics/Windows.jl:
+// 158-166,231-250): tau = offset of u from the centre of its cell; stored node h carries
+// E_h(t) + tau O_h(t) (t = tau^2), its mirror node P-1-h carries E_h(t) - tau O_h(t); the
+// u-derivatives tau Ed_h(t) +- Od_h(t) are the exact derivatives of the same polynomials.
+template <int P>
+__device__ __forceinline__ int prolate_weights(float u, float (&w)[P], float (&dw)[P], const WinCoef &c) {
+  constexpr int H = (P + 1) / 2;
+  float tau;
+  int k0;
+  if (P % 2 == 0) {
+    const float fl = floorf(u);
+    tau = (u - fl) - 0.5f;
+    k0 = (int)fl - (P / 2 - 1);
+  } else {
+    const float fl = floorf(u + 0.5f);
+    tau = u - fl;
+    k0 = (int)fl - (P - 1) / 2;
+  }
+  const float t = tau * tau;
+#pragma unroll
+  for (int h = 0; h < H; ++h) {
+    float E = c.e[WIN_TERMS - 1][h], O = c.o[WIN_TERMS - 1][h];
+    float Ed = c.ed[WIN_TERMS - 1][h], Od = c.od[WIN_TERMS - 1][h];
+#pragma unroll
+    for (int j = WIN_TERMS - 2; j >= 0; --j) {
+      E = fmaf(E, t, c.e[j][h]);
+      O = fmaf(O, t, c.o[j][h]);
+      Ed = fmaf(Ed, t, c.ed[j][h]);
+      Od = fmaf(Od, t, c.od[j][h]);
+    }
+    const float to = tau * O, ted = tau * Ed;
+    w[h] = E + to;
+    dw[h] = ted + Od;
+    if (P - 1 - h != h) {
+      w[P - 1 - h] = E - to;
+      dw[P - 1 - h] = ted - Od;
+    }
+  }
+  return k0;
+}
+
+// Weights and u-derivatives of the P = win_width(WP) nodes k0 .. k0 + P - 1 (k0 returned,
+// unwrapped: k0 + P - 1 = window_base<WP>(u)).
+template <int WP>
+__device__ __forceinline__ int window(float u, float (&w)[win_width(WP)], float (&dw)[win_width(WP)],
+                                      const WinCoef &wc) {
+  if constexpr (win_prolate(WP)) return prolate_weights<win_width(WP)>(u, w, dw, wc);
+  else return bspline<win_width(WP)>(u, w, dw);
+}
+
 // v rounded to the nearest integer (ties to even) as a 64-bit integer, for |v| < 2^51: the FP64
 // sum v + 1.5 * 2^52 lies in [2^52, 2^53), where the spacing of doubles is 1, so the addition
 // itself rounds v and leaves it in the low mantissa bits. The same value as __float2ll_rn in
@@ -804,15 +874,16 @@ __device__ __forceinline__ float sum8_lanes(float v) {
 // ---- spread: LPP lanes per particle, lane l owns the (x, y) line (l % P, l / P) of the
 // particle's P x P x P stencil and walks its P z nodes with 64-bit integer atomics (the
 // sum is exact and independent of the order of arrival).
-template <int P>
+template <int WP>
 __device__ __forceinline__ void spread_particle(int lane, float px, float py, float pz, float qi,
                                                 unsigned long long *grid, int K1, int K2, int K3, float Lx,
-                                                float Ly, float Lz, float scale) {
+                                                float Ly, float Lz, float scale, const WinCoef &wc) {
+  constexpr int P = win_width(WP);
   constexpr int LPP = lanes_for(P);
   float wx[P], wy[P], wz[P], d[P];
-  const int k1 = bspline<P>(grid_coordinate(px, Lx, K1), wx, d);
-  const int k2 = bspline<P>(grid_coordinate(py, Ly, K2), wy, d);
-  const int k3 = bspline<P>(grid_coordinate(pz, Lz, K3), wz, d);
+  const int k1 = window<WP>(grid_coordinate(px, Lx, K1), wx, d, wc);
+  const int k2 = window<WP>(grid_coordinate(py, Ly, K2), wy, d, wc);
+  const int k3 = window<WP>(grid_coordinate(pz, Lz, K3), wz, d, wc);
   const size_t KK = (size_t)K1 * K2;
   for (int l = lane; l < P * P; l += LPP) {
     const int j1 = l % P, j2 = l / P;
@@ -828,13 +899,13 @@ __device__ __forceinline__ void spread_particle(int lane, float px, float py, fl
   }
 }
 
-template <int P>
+template <int WP>
 __global__ __launch_bounds__(TPB) void k_spread(int N, const float *__restrict__ x, const float *__restrict__ y,
                                                 const float *__restrict__ z, const float *__restrict__ q,
                                                 const int32_t *__restrict__ order, unsigned long long *grid,
                                                 int K1, int K2, int K3, float Lx, float Ly, float Lz,
-                                                float scale) {
-  constexpr int LPP = lanes_for(P);
+                                                float scale, const WinCoef wc) {
+  constexpr int LPP = lanes_for(win_width(WP));
   const int tid = blockIdx.x * TPB + threadIdx.x;
   const int t = tid / LPP;
   const int lane = tid % LPP;
@@ -842,7 +913,7 @@ __global__ __launch_bounds__(TPB) void k_spread(int N, const float *__restrict__
   const int i = order ? order[t] - 1 : t;
   const float qi = q[i];
   if (qi == 0.0f) return;
-  spread_particle<P>(lane, x[i], y[i], z[i], qi, grid, K1, K2, K3, Lx, Ly, Lz, scale);
+  spread_particle<WP>(lane, x[i], y[i], z[i], qi, grid, K1, K2, K3, Lx, Ly, Lz, scale, wc);
 }
 
 // ---- fallback of the snapshot spread: when the snapshot spread found a particle further from
@@ -850,18 +921,18 @@ __global__ __launch_bounds__(TPB) void k_spread(int N, const float *__restrict__
 // tiles then only cleared the grid), every particle of `packed` is spread with global 64-bit
 // atomics, as k_spread; otherwise every workgroup leaves at once. Grid-stride, so a small grid
 // suffices for this rare path.
-template <int P>
+template <int WP>
 __global__ __launch_bounds__(TPB) void k_spread_stale(int N, const float4 *__restrict__ packed,
                                                       const int64_t *__restrict__ stale, unsigned long long *grid,
                                                       int K1, int K2, int K3, float Lx, float Ly, float Lz,
-                                                      float scale) {
+                                                      float scale, const WinCoef wc) {
   if (stale[3] == 0) return;
-  constexpr int LPP = lanes_for(P);
+  constexpr int LPP = lanes_for(win_width(WP));
   const int64_t total = (int64_t)N * LPP;
   for (int64_t tid = (int64_t)blockIdx.x * TPB + threadIdx.x; tid < total; tid += (int64_t)gridDim.x * TPB) {
     const int i = (int)(tid / LPP), lane = (int)(tid % LPP);
     const float4 p = packed[i];
-    if (p.w != 0.0f) spread_particle<P>(lane, p.x, p.y, p.z, p.w, grid, K1, K2, K3, Lx, Ly, Lz, scale);
+    if (p.w != 0.0f) spread_particle<WP>(lane, p.x, p.y, p.z, p.w, grid, K1, K2, K3, Lx, Ly, Lz, scale, wc);
   }
 }
 
@@ -882,15 +953,16 @@ __global__ void k_zero64(unsigned long long *g, size_t n) {
 // up to tcap of them, and the count (which may exceed tcap) in tn[tile].
 // One particle straight into the tile's accumulator (o = floor(u) - tile origin, wrapped), in the
 // product order of k_spread; the path of particles beyond the staged list.
-template <int P, int STILE>
+template <int WP, int STILE>
 __device__ __forceinline__ void spread_direct(unsigned long long *acc, float u1, float u2, float u3,
                                                         float q, int o1, int o2, int o3, int e1, int e2, int e3,
-                                                        int K1, int K2, int K3, float scale) {
+                                                        int K1, int K2, int K3, float scale, const WinCoef &wc) {
+  constexpr int P = win_width(WP);
   constexpr int SROW = STILE + 1;
   float wx[P], wy[P], wz[P], d[P];
-  bspline<P>(u1, wx, d);
-  bspline<P>(u2, wy, d);
-  bspline<P>(u3, wz, d);
+  window<WP>(u1, wx, d, wc);
+  window<WP>(u2, wy, d, wc);
+  window<WP>(u3, wz, d, wc);
   for (int j2 = 0; j2 < P; ++j2) {
     int l2 = o2 - (P - 1) + j2; l2 += l2 < 0 ? K2 : 0; l2 -= l2 >= K2 ? K2 : 0;
     if (l2 >= e2) continue;
@@ -909,10 +981,10 @@ __device__ __forceinline__ void spread_direct(unsigned long long *acc, float u1,
 
 constexpr int SLIST = 768;   // particles reaching one block staged in LDS (a few hundred in practice)
 constexpr int SCHUNK = 128;  // staged particles whose weights are computed together
-template <int P, int STILE, int SBS>
-// (1024-thread workgroups of orders up to 5 are held to 64 VGPRs, 8 waves per SIMD: two of them,
-// 32 waves, then fit on a CU; higher orders would spill)
-__global__ __launch_bounds__(SBS, (SBS == 1024 && P <= 5) ? 8 : 1) void k_spread_tiles(const float4 *__restrict__ packed,
+template <int WP, int STILE, int SBS>
+// (1024-thread workgroups of B-spline orders up to 5 are held to 64 VGPRs, 8 waves per SIMD: two
+// of them, 32 waves, then fit on a CU; higher orders would spill)
+__global__ __launch_bounds__(SBS, (SBS == 1024 && WP <= 5) ? 8 : 1) void k_spread_tiles(const float4 *__restrict__ packed,
                                                       const int32_t *__restrict__ offsets,
                                                       const int32_t *__restrict__ ids,
                                                       const int64_t *__restrict__ state, int nx, int ny,
@@ -921,8 +993,9 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && P <= 5) ? 8 : 1) void k_spread
                                                       float4 *__restrict__ tlist, int32_t *__restrict__ tids,
                                                       int32_t *__restrict__ tn, int tcap,
                                                       const float *__restrict__ disp, int ndisp, float stale2,
-                                                      int64_t *stale_word) {
+                                                      int64_t *stale_word, const WinCoef wc) {
   if (state[PDX_DN_ERROR] != 0) return;  // no usable cells / positions: the grid keeps its last value
+  constexpr int P = win_width(WP);
   constexpr int LPP = lanes_for(P);
   // x rows padded to STILE + 1 nodes: the P x P lanes of a particle then spread over the LDS banks
   constexpr int SROW = STILE + 1;
@@ -949,14 +1022,15 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && P <= 5) ? 8 : 1) void k_spread
     stale = __syncthreads_or(s) != 0;
     if (b == 0 && threadIdx.x == 0) stale_word[3] = stale ? 1 : 0;
   }
-  // candidate cells per dimension, in the list's frame (x + L/2) / L
+  // candidate cells per dimension, in the list's frame (x + L/2) / L: a particle at u reaches the
+  // block when the base node of u + shift lies in [A, A + E + P - 2]
   const int K[3] = {K1, K2, K3}, A[3] = {a1, a2, a3}, E[3] = {e1, e2, e3}, NC[3] = {nx, ny, nz};
   const float Ls[3] = {Lx, Ly, Lz};
   int c0[3], cn[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const float f0 = (float)A[d] / (float)K[d] + 0.5f - margin / Ls[d] - 1e-5f;
-    const float f1 = (float)(A[d] + E[d] + P - 1) / (float)K[d] + 0.5f + margin / Ls[d] + 1e-5f;
+    const float f0 = ((float)A[d] - win_shift<WP>()) / (float)K[d] + 0.5f - margin / Ls[d] - 1e-5f;
+    const float f1 = ((float)(A[d] + E[d] + P - 1) - win_shift<WP>()) / (float)K[d] + 0.5f + margin / Ls[d] + 1e-5f;
     const int lo = (int)floorf(f0 * NC[d]);
     c0[d] = lo;
     cn[d] = min((int)floorf(f1 * NC[d]) - lo + 1, NC[d]);
@@ -977,9 +1051,9 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && P <= 5) ? 8 : 1) void k_spread
   auto take = [&](int i, const float4 p) {
     const float u1 = grid_coordinate(p.x, Lx, K1), u2 = grid_coordinate(p.y, Ly, K2),
                 u3 = grid_coordinate(p.z, Lz, K3);
-    int o1 = (int)floorf(u1) - a1; o1 += o1 < 0 ? K1 : 0;
-    int o2 = (int)floorf(u2) - a2; o2 += o2 < 0 ? K2 : 0;
-    int o3 = (int)floorf(u3) - a3; o3 += o3 < 0 ? K3 : 0;
+    const int o1 = wrap_node(window_base<WP>(u1) - a1, K1);
+    const int o2 = wrap_node(window_base<WP>(u2) - a2, K2);
+    const int o3 = wrap_node(window_base<WP>(u3) - a3, K3);
     if (o1 < e1 && o2 < e2 && o3 < e3) {
       const int k = atomicAdd(&nown, 1);
       if (k < tcap) {
@@ -995,7 +1069,7 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && P <= 5) ? 8 : 1) void k_spread
       return;
     }
     // list full: this thread spreads the particle itself
-    spread_direct<P, STILE>(acc, u1, u2, u3, p.w, o1, o2, o3, e1, e2, e3, K1, K2, K3, scale);
+    spread_direct<WP, STILE>(acc, u1, u2, u3, p.w, o1, o2, o3, e1, e2, e3, K1, K2, K3, scale, wc);
   };
   for (int round = 0; !stale && round * 64 < ncand; ++round) {
     const int nc = min(64, ncand - round * 64);
@@ -1069,7 +1143,7 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && P <= 5) ? 8 : 1) void k_spread
       const int a = d == 0 ? a1 : (d == 1 ? a2 : a3);
       const int ee = d == 0 ? e1 : (d == 1 ? e2 : e3);
       float w[P], dw[P];
-      const int k = bspline<P>(d == 0 ? u.x : (d == 1 ? u.y : u.z), w, dw);
+      const int k = window<WP>(d == 0 ? u.x : (d == 1 ? u.y : u.z), w, dw, wc);
       unsigned lo = 0xFFFFFFFFu, hi = 0xFFFFFFFFu;
 #pragma unroll
       for (int j = 0; j < P; ++j) {
@@ -1126,13 +1200,15 @@ __device__ __forceinline__ int xcd_block(int xcd) {
 
 // ---- gather: 8 lanes per particle, lane = (y, z) lines of the stencil walked along x; line
 // partial sums reduced over the particle's lanes.
-template <int P, bool ENERGY>
+template <int WP, bool ENERGY>
 __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__ x, const float *__restrict__ y,
                                                 const float *__restrict__ z, const float *__restrict__ q,
                                                 const int32_t *__restrict__ order, const float *__restrict__ pot,
                                                 int K1, int K2, int K3, float Lx, float Ly, float Lz,
                                                 float *fx, float *fy, float *fz, float *Epot, float bg, int set,
-                                                float *Vglobal, const double *acc, double ubg, int xcd) {
+                                                float *Vglobal, const double *acc, double ubg, int xcd,
+                                                const WinCoef wc) {
+  constexpr int P = win_width(WP);
   constexpr int LPP = gather_lanes(P);
   const int tid = xcd_block(xcd) * TPB + threadIdx.x;
   if (Vglobal && tid < 6) Vglobal[tid] = (float)(acc[1 + tid] + (tid < 3 ? ubg : 0.0));
@@ -1144,9 +1220,9 @@ __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__
   float phi = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
   if (active && qi != 0.0f) {
     float wx[P], wy[P], wz[P], dx[P], dy[P], dz[P];
-    const int k1 = bspline<P>(grid_coordinate(x[i], Lx, K1), wx, dx);
-    const int k2 = bspline<P>(grid_coordinate(y[i], Ly, K2), wy, dy);
-    const int k3 = bspline<P>(grid_coordinate(z[i], Lz, K3), wz, dz);
+    const int k1 = window<WP>(grid_coordinate(x[i], Lx, K1), wx, dx, wc);
+    const int k2 = window<WP>(grid_coordinate(y[i], Ly, K2), wy, dy, wc);
+    const int k3 = window<WP>(grid_coordinate(z[i], Lz, K3), wz, dz, wc);
     // lane j < P = x node k1 + j of every (y, z) line: one load instruction reads, per particle,
     // the P consecutive nodes of one line (one cache line) instead of P^2 scattered ones; the
     // P^2 lines are unrolled, so their loads are independent and the y / z weights are registers
@@ -1212,7 +1288,7 @@ __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__
 // particle per thread. Per particle the result is k_gather's up to FP32 rounding.
 constexpr int GCHUNK = 128;  // listed particles per LDS chunk (a tile holds ~N/tiles)
 constexpr int GTB = 512;     // threads per tile workgroup (measured faster than 256 and 128)
-template <int P, int STILE, bool ENERGY>
+template <int WP, int STILE, bool ENERGY>
 __global__ __launch_bounds__(GTB) void k_gather_tiles(int N, const float4 *__restrict__ packed,
                                                       const float4 *__restrict__ tlist,
                                                       const int32_t *__restrict__ tids,
@@ -1221,7 +1297,8 @@ __global__ __launch_bounds__(GTB) void k_gather_tiles(int N, const float4 *__res
                                                       float Lx, float Ly, float Lz, float *fx, float *fy,
                                                       float *fz, float *Epot, float bg, int set, float *Vglobal,
                                                       const double *acc, double ubg,
-                                                      const int64_t *__restrict__ stale_word) {
+                                                      const int64_t *__restrict__ stale_word, const WinCoef wc) {
+  constexpr int P = win_width(WP);
   constexpr int R = STILE + P - 1;
   __shared__ float reg[R * R * R];
   __shared__ float4 pl[GCHUNK];
@@ -1242,10 +1319,11 @@ __global__ __launch_bounds__(GTB) void k_gather_tiles(int N, const float4 *__res
       const float4 p = packed[i];
       const float u1 = grid_coordinate(p.x, Lx, K1), u2 = grid_coordinate(p.y, Ly, K2),
                   u3 = grid_coordinate(p.z, Lz, K3);
-      const int t = (int)floorf(u1) / STILE + tx * ((int)floorf(u2) / STILE + ty * ((int)floorf(u3) / STILE));
+      const int t = wrap_node(window_base<WP>(u1), K1) / STILE +
+                    tx * (wrap_node(window_base<WP>(u2), K2) / STILE + ty * (wrap_node(window_base<WP>(u3), K3) / STILE));
       if (!stale && tn[t] <= tcap) continue;
       float wx[P], wy[P], wz[P], dx[P], dy[P], dz[P];
-      const int k1 = bspline<P>(u1, wx, dx), k2 = bspline<P>(u2, wy, dy), k3 = bspline<P>(u3, wz, dz);
+      const int k1 = window<WP>(u1, wx, dx, wc), k2 = window<WP>(u2, wy, dy, wc), k3 = window<WP>(u3, wz, dz, wc);
       float phi = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll
       for (int j3 = 0; j3 < P; ++j3)
@@ -1336,14 +1414,16 @@ __global__ __launch_bounds__(GTB) void k_gather_tiles(int N, const float4 *__res
       pid[threadIdx.x] = i0;
     }
     __syncthreads();
-    // weights: one thread per particle and dimension; base node floor(u) = k + P - 1 lies in
+    // weights: one thread per particle and dimension; the base node k + P - 1 (wrapped) lies in
     // the tile: local offsets 0 .. STILE-1
     for (int e = threadIdx.x; e < 3 * nc; e += GTB) {
       const int t = e / 3, d = e - 3 * t;
       const float4 u = pl[t];
       float w[P], dw[P];
       const int a = d == 0 ? a1 : (d == 1 ? a2 : a3);
-      go[t][d] = min(max(bspline<P>(d == 0 ? u.x : (d == 1 ? u.y : u.z), w, dw) + (P - 1) - a, 0), STILE - 1);
+      const int K = d == 0 ? K1 : (d == 1 ? K2 : K3);
+      const int k = window<WP>(d == 0 ? u.x : (d == 1 ? u.y : u.z), w, dw, wc);
+      go[t][d] = min(max(wrap_node(k + (P - 1) - a, K), 0), STILE - 1);
 #pragma unroll
       for (int j = 0; j < P; ++j) {
         gw[t][d][j] = w[j];
@@ -1482,12 +1562,38 @@ __global__ __launch_bounds__(TPB) void k_gather_ik(int N, const float *__restric
 // half-spectrum mode (_influence_alias_kernel!, src/electrostatics/Influence.jl:77-181):
 //   AD: G = (C/V) sum U^2 g k_n^2 / (sum U^2 . sum U^2 k_n^2)
 //   ik: G = (C/V) sum U^2 g (D.k_n) / (|D|^2 (sum U^2)^2), D = k with Nyquist components zeroed
-// with g(k) = 4 pi exp(-k^2/(4 alpha^2)) / k^2, and the virial coefficients Vc_ab = -dG/d eps_ab
-// at fixed fractional coordinates (Nyquist shears zeroed). U tables: U_d[m (2 nmax + 1) + n + nmax].
+// with g(k) = 4 pi exp(-k^2/(4 alpha^2)) / k^2 (Gaussian split) or 4 pi c^(k rc)/k^2 with
+// c^(u) = int_0^1 chi(x) cos(u x) dx by the nq-node quadrature (xq, wchi) (prolate split,
+// spectral_g, src/electrostatics/Splits.jl:220-240), and the virial coefficients
+// Vc_ab = -dG/d eps_ab at fixed fractional coordinates (Nyquist shears zeroed). U tables:
+// U_d[m (2 nmax + 1) + n + nmax], the window transform at the alias frequencies (B-spline sinc^P
+// or the prolate window's phi^). With nmax = 0 and U = phi^ this is ESP's plain deconvolution
+// G = g / phi^^2 and its virial G delta_ab + (C/V) D g'(k)/k k_a k_b.
+__device__ __forceinline__ void spectral_g(double qq, double i4a2, const double *__restrict__ xq,
+                                           const double *__restrict__ wchi, int nq, double rc, double &g,
+                                           double &gpk) {
+  if (nq == 0) {
+    g = 4.0 * M_PI * exp(-qq * i4a2) / qq;
+    gpk = g * (-2.0 / qq - 2.0 * i4a2);  // g'(k)/k
+    return;
+  }
+  const double k = sqrt(qq), u = k * rc;
+  double c = 0.0, dc = 0.0;
+  for (int j = 0; j < nq; ++j) {
+    double sn, cs;
+    sincos(u * xq[j], &sn, &cs);
+    c += wchi[j] * cs;
+    dc -= wchi[j] * xq[j] * sn;
+  }
+  g = 4.0 * M_PI * c / qq;
+  gpk = 4.0 * M_PI * (rc * dc / qq - 2.0 * c / (qq * k)) / k;
+}
+
 __global__ void k_influence_alias(float *G, float *Vc, size_t nmodes, const double *__restrict__ U1t,
                                   const double *__restrict__ U2t, const double *__restrict__ U3t, int K1, int K2,
                                   int K3, double L1, double L2, double L3, double CoverV, double alpha, int nmax,
-                                  int ik) {
+                                  int ik, const double *__restrict__ xq, const double *__restrict__ wchi, int nq,
+                                  double rcs) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nmodes) return;
   const int K1h = K1 / 2 + 1;
@@ -1523,8 +1629,8 @@ __global__ void k_influence_alias(float *G, float *Vc, size_t nmodes, const doub
         B += U2tot;
         const double qq = q1 * q1 + q2 * q2 + q3 * q3;
         if (qq == 0.0) continue;
-        const double g = 4.0 * M_PI * exp(-qq * i4a2) / qq;
-        const double gpk = g * (-2.0 / qq - 2.0 * i4a2);  // g'(k)/k
+        double g, gpk;
+        spectral_g(qq, i4a2, xq, wchi, nq, rcs, g, gpk);
         if (ik) {
           const double dq = dx * q1 + dy * q2 + dz * q3;
           A += U2tot * g * dq;
@@ -1676,22 +1782,46 @@ std::vector<double> alias_table(int P, int K, int nmax) {
   return U;
 }
 
+bool is_esp(const pdx_mesh_s *m) {
+  return m->spec.influence == PDX_INFLUENCE_ESP || m->spec.influence == PDX_INFLUENCE_ESP_DECONVOLUTION;
+}
+int alias_order_of(const pdx_mesh_s *m) {
+  return m->spec.influence == PDX_INFLUENCE_ESP_DECONVOLUTION ? 0 : m->spec.alias_order;
+}
+
+// Prolate window transform at the alias frequencies, phi^(2 pi (s(m) + K n)/K) (window_alias_table,
+// src/electrostatics/Windows.jl:401-411; with nmax = 0 the deconvolution's phi^, Windows.jl:340-352).
+std::vector<double> prolate_alias_table(const ProlateWindow &w, int K, int nmax) {
+  const int W = 2 * nmax + 1;
+  std::vector<double> U((size_t)K * W);
+  const int nq = nmax == 0 ? 192 : 256;
+  for (int m = 0; m < K; ++m) {
+    const int sm = m <= K / 2 ? m : m - K;
+    for (int n = -nmax; n <= nmax; ++n) U[(size_t)m * W + n + nmax] = w.transform(2 * M_PI * (sm + (double)K * n) / K, nq);
+  }
+  return U;
+}
+
 int compute_influence(pdx_mesh_t m) {
   const int K1 = m->K1, K2 = m->K2, K3 = m->K3, K1h = m->K1h;
   const double L1 = m->box[0], L2 = m->box[1], L3 = m->box[2];
   const double CoverV = m->C / (L1 * L2 * L3);
   if (m->spec.influence != PDX_INFLUENCE_SPME) {
-    const int nmax = m->spec.alias_order, W = 2 * nmax + 1;
+    const int nmax = alias_order_of(m), W = 2 * nmax + 1;
     std::vector<double> U;
+    ProlateWindow pw;
+    if (is_esp(m) && !pw.make(m->P, m->window_shape)) return fail(PDX_ERR_ARGUMENT, "prolate window set-up failed");
     for (int K : {K1, K2, K3}) {
-      const std::vector<double> u = alias_table(m->P, K, nmax);
+      const std::vector<double> u = is_esp(m) ? prolate_alias_table(pw, K, nmax) : alias_table(m->P, K, nmax);
       U.insert(U.end(), u.begin(), u.end());
     }
     PDX_HIP(hipMemcpy(m->Utab.p, U.data(), sizeof(double) * U.size(), hipMemcpyHostToDevice));
     const size_t nmodes = (size_t)K1h * K2 * K3;
     hipLaunchKernelGGL(k_influence_alias, dim3((unsigned)((nmodes + 127) / 128)), dim3(128), 0, 0, m->G.p, m->Vc.p,
                        nmodes, m->Utab.p, m->Utab.p + (size_t)K1 * W, m->Utab.p + (size_t)(K1 + K2) * W, K1, K2,
-                       K3, L1, L2, L3, CoverV, m->alpha, nmax, m->spec.influence == PDX_INFLUENCE_P3M_IK ? 1 : 0);
+                       K3, L1, L2, L3, CoverV, m->alpha, nmax, m->spec.influence == PDX_INFLUENCE_P3M_IK ? 1 : 0,
+                       is_esp(m) ? (const double *)m->espq.p : nullptr,
+                       is_esp(m) ? (const double *)m->espq.p + 128 : nullptr, is_esp(m) ? 128 : 0, m->spec.real_cutoff);
     PDX_LAUNCH_CHECK();
     PDX_HIP(hipDeviceSynchronize());
     return PDX_OK;
@@ -1759,7 +1889,7 @@ SolveArgs solve_args(pdx_mesh_t m) {
   s.nmodes = (size_t)m->K1h * m->K2 * m->K3;
   s.K1 = m->K1; s.K2 = m->K2; s.K3 = m->K3; s.K1h = m->K1h;
   for (int d = 0; d < 3; ++d) s.twopi_L[d] = 2 * M_PI / m->box[d];
-  s.inv4a2 = 1.0 / (4.0 * m->alpha * m->alpha);
+  s.inv4a2 = m->alpha > 0 ? 1.0 / (4.0 * m->alpha * m->alpha) : 0.0;  // Gaussian closed form only (no Vc table)
   for (int d = 0; d < 3; ++d) s.twopi_Lf[d] = (float)s.twopi_L[d];
   s.inv2a2f = (float)(2.0 * s.inv4a2);
   s.partial = m->partial.p;
@@ -1957,10 +2087,15 @@ int enqueue_plane_solve(pdx_mesh_t m, const Plans &pl, bool energy, hipStream_t 
   return enqueue_plane_solve_bs<PBS>(m, pl, energy, st);
 }
 
+// the window codes compiled: B-spline orders 3..8 and prolate widths 4..8
+#define PDX_WINDOWS(CASE) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) \
+  CASE(PROLATE + 4) CASE(PROLATE + 5) CASE(PROLATE + 6) CASE(PROLATE + 7) CASE(PROLATE + 8)
+
 #define PDX_SPREAD_CASE(PP)                                                                                    \
   case PP:                                                                                                     \
-    hipLaunchKernelGGL(k_spread<PP>, dim3(((int64_t)N * lanes_for(PP) + TPB - 1) / TPB), dim3(TPB), 0, st, N, \
-                       x, y, z, q, order, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale);         \
+    hipLaunchKernelGGL(k_spread<PP>, dim3(((int64_t)N * lanes_for(win_width(PP)) + TPB - 1) / TPB), dim3(TPB), \
+                       0, st, N, x, y, z, q, order, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale,  \
+                       m->wc);                                                                                 \
     break;
 
 #define PDX_GATHER_CASE(PP)                                                                                     \
@@ -1968,11 +2103,11 @@ int enqueue_plane_solve(pdx_mesh_t m, const Plans &pl, bool energy, hipStream_t 
     if (energy)                                                                                                 \
       hipLaunchKernelGGL((k_gather<PP, true>), dim3(blocks_of(PP)), dim3(TPB), 0, st, N, x, y, z, q, order,     \
                          m->pot.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], fx, fy, fz, Epot, m->bg, set,         \
-                         Vglobal, m->acc.p, m->ubg, xcd);                                                      \
+                         Vglobal, m->acc.p, m->ubg, xcd, m->wc);                                               \
     else                                                                                                        \
       hipLaunchKernelGGL((k_gather<PP, false>), dim3(blocks_of(PP)), dim3(TPB), 0, st, N, x, y, z, q, order,    \
                          m->pot.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], fx, fy, fz, Epot, m->bg, set,         \
-                         Vglobal, m->acc.p, m->ubg, xcd);                                                      \
+                         Vglobal, m->acc.p, m->ubg, xcd, m->wc);                                               \
     break;
 
 #define PDX_GATHER_IK_CASE(PP)                                                                                  \
@@ -1999,13 +2134,22 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
     return fail(PDX_ERR_ARGUMENT, "relative_permittivity must be finite and positive.");
   if (!(std::isfinite(sp.real_cutoff) && sp.real_cutoff > 0))
     return fail(PDX_ERR_ARGUMENT, "real_cutoff must be finite and positive.");
-  if (sp.influence != PDX_INFLUENCE_SPME && sp.influence != PDX_INFLUENCE_P3M_AD && sp.influence != PDX_INFLUENCE_P3M_IK)
+  const bool esp = sp.influence == PDX_INFLUENCE_ESP || sp.influence == PDX_INFLUENCE_ESP_DECONVOLUTION;
+  if (sp.influence != PDX_INFLUENCE_SPME && sp.influence != PDX_INFLUENCE_P3M_AD && sp.influence != PDX_INFLUENCE_P3M_IK &&
+      !esp)
     return fail(PDX_ERR_ARGUMENT, "Unknown influence kind.");
   if (sp.influence != PDX_INFLUENCE_SPME && (sp.alias_order < 0 || sp.alias_order > 6))
     return fail(PDX_ERR_ARGUMENT, "alias_order must be between 0 and 6.");
-  if (sp.order < 3 || sp.order > 12)
-    return fail(PDX_ERR_ARGUMENT, "B-spline assignment order must be between 3 and 12.");
-  if (sp.order > 8) return fail(PDX_ERR_ARGUMENT, "Assignment orders above 8 are not compiled in this build.");
+  if (esp) {
+    if (sp.order < 2 || sp.order > 16)
+      return fail(PDX_ERR_ARGUMENT, "Prolate window width must be between 2 and 16 grid points.");
+    if (sp.order < 4 || sp.order > 8)
+      return fail(PDX_ERR_ARGUMENT, "Prolate window widths other than 4..8 grid points are not compiled in this build.");
+  } else {
+    if (sp.order < 3 || sp.order > 12)
+      return fail(PDX_ERR_ARGUMENT, "B-spline assignment order must be between 3 and 12.");
+    if (sp.order > 8) return fail(PDX_ERR_ARGUMENT, "Assignment orders above 8 are not compiled in this build.");
+  }
   if (sp.net_charge != PDX_NET_CHARGE_ERROR && sp.net_charge != PDX_NET_CHARGE_BACKGROUND)
     return fail(PDX_ERR_ARGUMENT, "net_charge must be :error or :background.");
   if (sp.real_space_modifier != 0 && sp.real_space_modifier != 1)
@@ -2016,8 +2160,37 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
   const double mn = std::min(box[0], std::min(box[1], box[2]));
   if (!(2 * sp.real_cutoff < mn))
     return fail(PDX_ERR_ARGUMENT, "real_cutoff must be smaller than half the shortest box length (minimum-image real space).");
+  // ESP: the prolate split and window (ProlateSplit / ProlateWindow, Splits.jl:48-60, Windows.jl:64-71;
+  // the spec checks of Spec.jl:66-76, the FP32 resolution bound of Windows.jl:310-330)
+  ProlateSplit psplit;
+  ProlateWindow pwin;
+  WinCoef wcoef{};
+  int wdeg = 0;
+  if (esp) {
+    if (!(std::isfinite(sp.split_shape) && sp.split_shape > 0))
+      return fail(PDX_ERR_ARGUMENT, "ESP split shape must be finite and positive.");
+    if (!(std::isfinite(sp.window_shape) && sp.window_shape > 0))
+      return fail(PDX_ERR_ARGUMENT, "Prolate window shape must be finite and positive.");
+    if (!psplit.make(sp.real_cutoff, sp.split_shape) || !pwin.make(sp.order, sp.window_shape))
+      return fail(PDX_ERR_ARGUMENT, "prolate function set-up failed (shape out of range)");
+    if (sp.order > prolate_max_order(sp.window_shape))
+      return fail(PDX_ERR_ARGUMENT, "A prolate window of shape " + std::to_string(sp.window_shape) + " supports at most " +
+                                        std::to_string(prolate_max_order(sp.window_shape)) +
+                                        " grid points (P <= 2c/pi - 1/2); got " + std::to_string(sp.order) +
+                                        ". Use a larger window shape (tighter tolerance) or a narrower window.");
+    if (!(pwin.nyquist_attenuation() >= prolate_min_attenuation_fp32()))
+      return fail(PDX_ERR_ARGUMENT, "The prolate window is not resolved by an FP32 mesh: its Nyquist attenuation " +
+                                        std::to_string(pwin.nyquist_attenuation()) + " is below 0.02 eps32^(2/3); "
+                                        "use a larger window shape.");
+    double ew, ed;
+    wdeg = pwin.node_polynomials(wcoef, ew, ed);
+    if (wdeg < 0)
+      return fail(PDX_ERR_ARGUMENT, "the prolate window's node polynomials need more than degree 12 in FP32");
+  }
   double alpha = sp.alpha;
-  if (!(alpha > 0)) {
+  if (esp) {
+    alpha = 0.0;
+  } else if (!(alpha > 0)) {
     const double tol = sp.real_space_tolerance;
     if (!(tol > 0 && tol < 1)) return fail(PDX_ERR_ARGUMENT, "real_space_tolerance must lie in (0, 1).");
     double lo = 0.0, hi = 10.0;  // alpha_from_tolerance (Spec.jl:154-163): bisection of erfc(x) = tol
@@ -2027,7 +2200,7 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
     }
     alpha = 0.5 * (lo + hi) / sp.real_cutoff;
   }
-  if (!(std::isfinite(alpha) && alpha > 0)) return fail(PDX_ERR_ARGUMENT, "Ewald alpha must be finite and positive.");
+  if (!esp && !(std::isfinite(alpha) && alpha > 0)) return fail(PDX_ERR_ARGUMENT, "Ewald alpha must be finite and positive.");
   int K[3];
   for (int d = 0; d < 3; ++d) {
     if (sp.mesh[0] > 0) {
@@ -2054,8 +2227,13 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
   for (int d = 0; d < 3; ++d) m->box[d] = box[d];
   m->alpha = alpha;
   m->C = sp.coulomb_prefactor / sp.relative_permittivity;
-  m->selfc = 2.0 * alpha / std::sqrt(M_PI);      // self_coefficient (Splits.jl:67)
-  m->bgcoef = M_PI / (2.0 * alpha * alpha);       // background_coefficient (Splits.jl:85)
+  m->selfc = esp ? psplit.self_coefficient() : 2.0 * alpha / std::sqrt(M_PI);  // self_coefficient (Splits.jl:67-68)
+  m->bgcoef = esp ? psplit.background_coefficient() : M_PI / (2.0 * alpha * alpha);  // background_coefficient (Splits.jl:85-86)
+  m->wkey = esp ? PROLATE + sp.order : sp.order;
+  m->wc = wcoef;
+  m->split_shape = esp ? sp.split_shape : 0.0;
+  m->window_shape = esp ? sp.window_shape : 0.0;
+  m->window_degree = wdeg;
   const size_t nmodes = (size_t)m->K1h * m->K2 * m->K3;
   const Plans pl = plans_of(m);
   m->plane_tp = plane_lines(m);
@@ -2069,9 +2247,24 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
   }
   if (sp.influence != PDX_INFLUENCE_SPME &&
       ((rc = m->Vc.alloc(6 * nmodes)) ||
-       (rc = m->Utab.alloc((size_t)(K[0] + K[1] + K[2]) * (2 * sp.alias_order + 1))))) {
+       (rc = m->Utab.alloc((size_t)(K[0] + K[1] + K[2]) * (2 * alias_order_of(m) + 1))))) {
     delete m;
     return rc;
+  }
+  if (esp) {
+    std::vector<double> q(256);
+    for (int j = 0; j < 128; ++j) {
+      q[j] = psplit.xq[j];
+      q[128 + j] = psplit.wchi[j];
+    }
+    if ((rc = m->espq.alloc(256))) {
+      delete m;
+      return rc;
+    }
+    if (hipMemcpy(m->espq.p, q.data(), sizeof(double) * 256, hipMemcpyHostToDevice) != hipSuccess) {
+      delete m;
+      return fail(PDX_ERR_DEVICE, "mesh initialisation failed");
+    }
   }
   if (sp.influence == PDX_INFLUENCE_P3M_IK &&
       ((rc = m->spec_ez.alloc(nmodes)) || (rc = m->spec_tmp.alloc(nmodes)) || (rc = m->ex.alloc(nnodes)) ||
@@ -2111,6 +2304,51 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
     return fail(PDX_ERR_DEVICE, "mesh initialisation failed");
   }
   *out = m;
+  return PDX_OK;
+}
+
+extern "C" int pdx_esp_rule(double tolerance, int32_t order, double out[3]) {
+  // the ESP defaults of a target relative force error (electrostatics_spec(::ESP, Float32) and
+  // esp_order_for_tolerance / esp_window_shape, src/workflow/Electrostatics.jl:177-215,288-301)
+  if (!out) return fail(PDX_ERR_ARGUMENT, "null argument");
+  if (!(tolerance > 0 && tolerance < 1)) return fail(PDX_ERR_ARGUMENT, "ESP tolerance must lie in (0, 1).");
+  const double c = prolate_shape_for_tolerance(tolerance);
+  if (!(c > 0)) return fail(PDX_ERR_ARGUMENT, "Tolerance is out of the range of a prolate split.");
+  int P = order;
+  if (P <= 0) {
+    const double d = -std::log10(tolerance);
+    if (d <= 3.35) P = 5;
+    else if (d <= 4.05) P = 6;
+    else if (d <= 4.35) P = 7;
+    else if (d <= 5.05) P = 8;
+    else {
+      const double c1 = prolate_shape_for_tolerance(tolerance / 2);
+      P = std::min(std::max((int)std::floor(2 * c1 / M_PI - 1), 8), 16);
+    }
+  }
+  // window shape: c(eps/2), or the smallest larger shape an FP32 mesh resolves
+  auto resolved = [&](double shape) {
+    ProlateWindow w;
+    return w.make(P, shape) && w.nyquist_attenuation() >= prolate_min_attenuation_fp32();
+  };
+  double c1 = prolate_shape_for_tolerance(tolerance / 2);
+  if (!(c1 > 0)) return fail(PDX_ERR_ARGUMENT, "Tolerance is out of the range of a prolate window.");
+  if (!resolved(c1)) {
+    double lo = c1, hi = c1 + 0.5;
+    while (!resolved(hi)) {
+      lo = hi;
+      hi += 0.5;
+      if (!(hi < 80)) return fail(PDX_ERR_ARGUMENT, "No prolate window of this width is resolved in FP32.");
+    }
+    while (hi - lo > 1e-3) {
+      const double mid = 0.5 * (lo + hi);
+      (resolved(mid) ? hi : lo) = mid;
+    }
+    c1 = hi;
+  }
+  out[0] = c;
+  out[1] = P;
+  out[2] = c1;
   return PDX_OK;
 }
 
@@ -2184,13 +2422,8 @@ extern "C" int pdx_mesh_spread(pdx_mesh_t m, const float *x, const float *y, con
   hipStream_t st = as_stream(stream);
   const size_t nn = (size_t)m->K1 * m->K2 * m->K3;
   hipLaunchKernelGGL(k_zero64, dim3((nn + TPB - 1) / TPB), dim3(TPB), 0, st, m->grid64.p, nn);
-  switch (m->P) {
-    PDX_SPREAD_CASE(3)
-    PDX_SPREAD_CASE(4)
-    PDX_SPREAD_CASE(5)
-    PDX_SPREAD_CASE(6)
-    PDX_SPREAD_CASE(7)
-    PDX_SPREAD_CASE(8)
+  switch (m->wkey) {
+    PDX_WINDOWS(PDX_SPREAD_CASE)
     default: return fail(PDX_ERR_ARGUMENT, "unsupported order");
   }
   PDX_LAUNCH_CHECK();
@@ -2244,7 +2477,7 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
   hipLaunchKernelGGL((k_spread_tiles<PP, TT, BB>), dim3(tiles), dim3(BB), 0, st, nl->packed.p,                 \
                      offsets, ids, guard, nl->nx, nl->ny, nl->nz,                                              \
                      margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale, m->tile_list.p,   \
-                     m->tile_ids.p, m->tile_n.p, tcap, disp, ndisp, stale2, nl->snap_state.p);
+                     m->tile_ids.p, m->tile_n.p, tcap, disp, ndisp, stale2, nl->snap_state.p, m->wc);
 #define PDX_TSPREAD(PP)                                                                                        \
   case PP:                                                                                                     \
     if (small) {                                                                                               \
@@ -2257,13 +2490,8 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
       else { PDX_TSPREAD_L(PP, 16, 256) }                                                                      \
     }                                                                                                          \
     break;
-  switch (m->P) {
-    PDX_TSPREAD(3)
-    PDX_TSPREAD(4)
-    PDX_TSPREAD(5)
-    PDX_TSPREAD(6)
-    PDX_TSPREAD(7)
-    PDX_TSPREAD(8)
+  switch (m->wkey) {
+    PDX_WINDOWS(PDX_TSPREAD)
     default: return fail(PDX_ERR_ARGUMENT, "unsupported order");
   }
 #undef PDX_TSPREAD
@@ -2274,15 +2502,11 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
 #define PDX_STALE(PP)                                                                                          \
   case PP:                                                                                                     \
     hipLaunchKernelGGL(k_spread_stale<PP>, dim3(fb), dim3(TPB), 0, st, m->N, nl->packed.p,                    \
-                       (const int64_t *)nl->snap_state.p, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale); \
+                       (const int64_t *)nl->snap_state.p, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale, \
+                       m->wc);                                                                                 \
     break;
-    switch (m->P) {
-      PDX_STALE(3)
-      PDX_STALE(4)
-      PDX_STALE(5)
-      PDX_STALE(6)
-      PDX_STALE(7)
-      PDX_STALE(8)
+    switch (m->wkey) {
+      PDX_WINDOWS(PDX_STALE)
       default: return fail(PDX_ERR_ARGUMENT, "unsupported order");
     }
 #undef PDX_STALE
@@ -2373,13 +2597,8 @@ extern "C" int pdx_mesh_gather(pdx_mesh_t m, const float *x, const float *y, con
     PDX_LAUNCH_CHECK();
     return PDX_OK;
   }
-  switch (m->P) {
-    PDX_GATHER_CASE(3)
-    PDX_GATHER_CASE(4)
-    PDX_GATHER_CASE(5)
-    PDX_GATHER_CASE(6)
-    PDX_GATHER_CASE(7)
-    PDX_GATHER_CASE(8)
+  switch (m->wkey) {
+    PDX_WINDOWS(PDX_GATHER_CASE)
     default: return fail(PDX_ERR_ARGUMENT, "unsupported order");
   }
   PDX_LAUNCH_CHECK();
@@ -2405,7 +2624,7 @@ extern "C" int pdx_mesh_gather_cells(pdx_mesh_t m, pdx_nl_t nl, float *fx, float
   hipLaunchKernelGGL((k_gather_tiles<PP, TT, EE>), grid, dim3(GTB), 0, st, N, nl->packed.p, m->tile_list.p,    \
                      m->tile_ids.p, m->tile_n.p, m->tiles_cap, m->tiles_count, m->pot.p, m->K1, m->K2, m->K3, L[0], L[1],     \
                      L[2], fx, fy, fz, Epot, m->bg, set, Vglobal, m->acc.p, m->ubg,                        \
-                     m->tiles_snapshot ? (const int64_t *)nl->snap_state.p : nullptr);
+                     m->tiles_snapshot ? (const int64_t *)nl->snap_state.p : nullptr, m->wc);
 #define PDX_GT(PP)                                                                                             \
   case PP:                                                                                                     \
     if (m->tiles_stile == 8) {                                                                                 \
@@ -2414,13 +2633,8 @@ extern "C" int pdx_mesh_gather_cells(pdx_mesh_t m, pdx_nl_t nl, float *fx, float
       if (energy) { PDX_GT_L(PP, 16, true) } else { PDX_GT_L(PP, 16, false) }                                  \
     }                                                                                                          \
     break;
-  switch (m->P) {
-    PDX_GT(3)
-    PDX_GT(4)
-    PDX_GT(5)
-    PDX_GT(6)
-    PDX_GT(7)
-    PDX_GT(8)
+  switch (m->wkey) {
+    PDX_WINDOWS(PDX_GT)
     default: return fail(PDX_ERR_ARGUMENT, "unsupported order");
   }
 #undef PDX_GT

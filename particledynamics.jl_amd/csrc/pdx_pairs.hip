@@ -866,7 +866,11 @@ extern "C" int pdx_pair_create(const pdx_pair_spec *spec, pdx_pair_t *out) {
       return fail(PDX_ERR_ARGUMENT, "coulomb_prefactor must be finite and positive");
     if (!(sp.coulomb_cutoff > 0 && std::isfinite(sp.coulomb_cutoff)))
       return fail(PDX_ERR_ARGUMENT, "real_cutoff must be finite and positive");
-    if (!(sp.alpha > 0 && std::isfinite(sp.alpha))) return fail(PDX_ERR_ARGUMENT, "Ewald alpha must be finite and positive.");
+    if (sp.coulomb != 1 && sp.coulomb != 2) return fail(PDX_ERR_ARGUMENT, "coulomb must be 0, 1 (Gaussian) or 2 (prolate)");
+    if (sp.coulomb == 1 && !(sp.alpha > 0 && std::isfinite(sp.alpha)))
+      return fail(PDX_ERR_ARGUMENT, "Ewald alpha must be finite and positive.");
+    if (sp.coulomb == 2 && !(sp.split_shape > 0 && std::isfinite(sp.split_shape)))
+      return fail(PDX_ERR_ARGUMENT, "ESP split shape must be finite and positive.");
   }
   if (sp.lanes > 0 && (sp.lanes > 64 || (sp.lanes & (sp.lanes - 1))))
     return fail(PDX_ERR_ARGUMENT, "lanes per particle must be a power of two in 1..64");
@@ -899,17 +903,29 @@ extern "C" int pdx_pair_create(const pdx_pair_spec *spec, pdx_pair_t *out) {
   }
   if (sp.coulomb) {
     const double rc = sp.coulomb_cutoff, rc2 = rc * rc, a = sp.alpha;
+    // the split's long-range part 1/r - E_s/r and its force factor as functions of s = r^2:
+    // Gaussian (erf(alpha r)/r) or prolate (a(t)/r by quadrature, Splits.jl:163-202)
+    ProlateSplit ps;
+    if (sp.coulomb == 2 && !ps.make(rc, sp.split_shape)) {
+      delete p;
+      return fail(PDX_ERR_ARGUMENT, "prolate split set-up failed");
+    }
+    auto correction = [&](double s, double &corr, double &fcorr) {
+      if (sp.coulomb == 2) ps.correction(s, corr, fcorr);
+      else ewald_correction(a, s, corr, fcorr);
+    };
     p->cC = (float)sp.coulomb_C;
     p->crc2 = (float)rc2;
     p->csx = (float)(2.0 / rc2);
-    p->ceshift = sp.coulomb_shift ? (float)(std::erfc(a * rc) / rc) : 0.0f;
-    p->halfself = (float)(0.5 * sp.coulomb_C * 2.0 * a / std::sqrt(M_PI));
+    // cutoff energy E_s(rc)/rc: erfc(alpha rc)/rc, exactly zero for the prolate split (Splits.jl:77-78)
+    p->ceshift = (sp.coulomb_shift && sp.coulomb == 1) ? (float)(std::erfc(a * rc) / rc) : 0.0f;
+    p->halfself = (float)(0.5 * sp.coulomb_C * (sp.coulomb == 2 ? ps.self_coefficient() : 2.0 * a / std::sqrt(M_PI)));
     const int nodes = 64;
     std::vector<double> fe(nodes), ff(nodes);
     for (int k = 0; k < nodes; ++k) {
       const double x = std::cos(M_PI * (k + 0.5) / nodes);
       double corr, fcorr;
-      ewald_correction(a, (x + 1.0) * rc2 / 2.0, corr, fcorr);
+      correction((x + 1.0) * rc2 / 2.0, corr, fcorr);
       fe[k] = -corr;
       ff[k] = fcorr;
     }
@@ -917,7 +933,7 @@ extern "C" int pdx_pair_create(const pdx_pair_spec *spec, pdx_pair_t *out) {
     std::vector<double> me = chebyshev_monomial(fe, 1e-8), mf = chebyshev_monomial(ff, 1e-7);
     if ((int)me.size() > MAXC || (int)mf.size() > MAXC) {
       delete p;
-      return fail(PDX_ERR_ARGUMENT, "Ewald correction polynomial needs more than 24 terms (alpha*cutoff too large)");
+      return fail(PDX_ERR_ARGUMENT, "the split's long-range polynomial needs more than 24 terms (alpha*cutoff or the split shape too large)");
     }
     p->npe = (int)me.size();
     p->npf = (int)mf.size();
@@ -928,7 +944,7 @@ extern "C" int pdx_pair_create(const pdx_pair_spec *spec, pdx_pair_t *out) {
     for (int k = 0; k <= 2000; ++k) {
       const double x = -1.0 + 2.0 * k / 2000.0;
       double corr, fcorr;
-      ewald_correction(a, (x + 1.0) * rc2 / 2.0, corr, fcorr);
+      correction((x + 1.0) * rc2 / 2.0, corr, fcorr);
       float he = p->pe[p->npe - 1], hf = p->pf[p->npf - 1];
       for (int q = p->npe - 2; q >= 0; --q) he = fmaf(he, (float)x, p->pe[q]);
       for (int q = p->npf - 2; q >= 0; --q) hf = fmaf(hf, (float)x, p->pf[q]);

@@ -8,8 +8,8 @@ from ._lib import (ArgumentError, DeviceError, DimensionMismatch, NeighborCapaci
 from .neighbors import DeviceNeighborWorkspace, CellList, build_neighbors_dense
 from .forces import (LennardJones, WCA, SoftRepulsive, MixedSigmaLJ, MixedSigmaWCA, PairMatrixLJ, PairMatrixWCA,
                      RealSpaceCoulomb, PairForce, FENEBonds, DomainError)
-from .electrostatics import (ElectrostaticsSpec, PME, PPPM, ElectrostaticsSolver, alpha_from_tolerance, fft_friendly,
-                             ewald_fp64)
+from .electrostatics import (ElectrostaticsSpec, PME, PPPM, ESP, esp_rule, ElectrostaticsSolver, alpha_from_tolerance,
+                             fft_friendly, ewald_fp64)
 from .engine import ForceEngine, default_capacity
 from .replicas import ReplicaBatch
 from .clusters import ClusterPairEngine
@@ -17,5 +17,6 @@ from . import systems
 
 __all__ = ["ArgumentError", "DeviceError", "DimensionMismatch", "NeighborCapacityError", "check", "last_error",
            "lib", "LIB_PATH", "DeviceNeighborWorkspace", "CellList", "build_neighbors_dense", "LennardJones", "WCA",
-           "SoftRepulsive", "MixedSigmaLJ", "MixedSigmaWCA", "PairMatrixLJ", "PairMatrixWCA", "RealSpaceCoulomb", "PairForce", "FENEBonds", "DomainError", "ElectrostaticsSpec", "PME", "PPPM", "ElectrostaticsSolver",
+           "SoftRepulsive", "MixedSigmaLJ", "MixedSigmaWCA", "PairMatrixLJ", "PairMatrixWCA", "RealSpaceCoulomb", "PairForce", "FENEBonds", "DomainError", "ElectrostaticsSpec", "PME", "PPPM", "ESP", "esp_rule",
+           "ElectrostaticsSolver",
            "alpha_from_tolerance", "fft_friendly", "ewald_fp64", "ForceEngine", "default_capacity", "ReplicaBatch", "ClusterPairEngine", "systems"]

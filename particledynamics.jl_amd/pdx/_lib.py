@@ -32,6 +32,8 @@ PDX_ADD = 2
 PDX_INFLUENCE_SPME = 1
 PDX_INFLUENCE_P3M_AD = 2
 PDX_INFLUENCE_P3M_IK = 3
+PDX_INFLUENCE_ESP = 4
+PDX_INFLUENCE_ESP_DECONVOLUTION = 5
 
 
 class ArgumentError(ValueError):
@@ -67,7 +69,7 @@ class pdx_pair_spec(C.Structure):
     _fields_ = [("model", C.c_int32), ("shift", C.c_int32), ("epsilon", C.c_double), ("sigma", C.c_double),
                 ("cutoff", C.c_double), ("coulomb", C.c_int32), ("coulomb_shift", C.c_int32),
                 ("coulomb_C", C.c_double), ("coulomb_cutoff", C.c_double), ("alpha", C.c_double),
-                ("lanes", C.c_int32)]
+                ("lanes", C.c_int32), ("split_shape", C.c_double)]
 
 
 class pdx_mesh_spec(C.Structure):
@@ -75,7 +77,8 @@ class pdx_mesh_spec(C.Structure):
                 ("real_cutoff", C.c_double), ("alpha", C.c_double), ("real_space_tolerance", C.c_double),
                 ("order", C.c_int32), ("mesh", C.c_int32 * 3), ("mesh_spacing", C.c_double),
                 ("influence", C.c_int32), ("net_charge", C.c_int32), ("net_charge_tolerance", C.c_double),
-                ("real_space_modifier", C.c_int32), ("alias_order", C.c_int32)]
+                ("real_space_modifier", C.c_int32), ("alias_order", C.c_int32), ("split_shape", C.c_double),
+                ("window_shape", C.c_double)]
 
 
 def _load():
@@ -113,6 +116,7 @@ def _load():
         "pdx_mesh_create": ([P(pdx_mesh_spec), i32, P(dbl), P(vp)], C.c_int),
         "pdx_mesh_destroy": ([vp], C.c_int),
         "pdx_mesh_info": ([vp, P(dbl)], C.c_int),
+        "pdx_esp_rule": ([dbl, i32, P(dbl)], C.c_int),
         "pdx_mesh_set_charges": ([vp, P(C.c_float), i32], C.c_int),
         "pdx_mesh_set_box": ([vp, P(dbl)], C.c_int),
         "pdx_mesh_spread": ([vp, vp, vp, vp, vp, vp, vp], C.c_int),

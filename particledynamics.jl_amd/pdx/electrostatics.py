@@ -58,17 +58,26 @@ class ElectrostaticsSpec:
     net_charge_tolerance: float = 1e-8
     influence: str = "spme"
     alias_order: int = 2
+    split_shape: float | None = None   # ESP: the prolate split's bandlimit c
+    window_shape: float | None = None  # ESP: the prolate window's shape c1 (``order`` is its width)
+
+    @property
+    def is_esp(self):
+        return self.influence in ("esp", "esp_deconvolution")
 
     def __post_init__(self):
         if self.mesh is None and self.mesh_spacing is None:
             raise L.ArgumentError("Provide either `mesh=(K1,K2,K3)` or `mesh_spacing`.")
-        if self.alpha is None and self.real_space_tolerance is None:
+        if self.influence in ("esp", "esp_deconvolution"):
+            if self.split_shape is None or self.window_shape is None:
+                raise L.ArgumentError("ESP requires a prolate split and a prolate window.")
+        elif self.alpha is None and self.real_space_tolerance is None:
             raise L.ArgumentError("Provide `alpha` or `real_space_tolerance`.")
         if self.net_charge not in ("error", "background"):
             raise L.ArgumentError("net_charge must be :error or :background")
         if self.real_space_modifier not in ("none", "potential_shift"):
             raise L.ArgumentError("real_space_modifier must be :none or :potential_shift")
-        if self.influence not in ("spme", "p3m_ad", "p3m_ik"):
+        if self.influence not in ("spme", "p3m_ad", "p3m_ik", "esp", "esp_deconvolution"):
             raise L.ArgumentError(f"Unknown influence kind {self.influence}.")
         if not 0 <= int(self.alias_order) <= 6:
             raise L.ArgumentError("alias_order must be between 0 and 6.")
@@ -85,6 +94,8 @@ class ElectrostaticsSpec:
         return tuple(max(fft_friendly(int(math.ceil(b / self.mesh_spacing - 1e-9))), int(self.order)) for b in box)
 
     def resolved_alpha(self):
+        if self.is_esp:
+            return None
         return float(self.alpha) if self.alpha is not None else alpha_from_tolerance(self.real_cutoff,
                                                                                      self.real_space_tolerance)
 
@@ -103,7 +114,10 @@ class ElectrostaticsSpec:
             s.mesh[0] = 0
         s.mesh_spacing = float(self.mesh_spacing or 0.0)
         s.influence = {"spme": L.PDX_INFLUENCE_SPME, "p3m_ad": L.PDX_INFLUENCE_P3M_AD,
-                       "p3m_ik": L.PDX_INFLUENCE_P3M_IK}[self.influence]
+                       "p3m_ik": L.PDX_INFLUENCE_P3M_IK, "esp": L.PDX_INFLUENCE_ESP,
+                       "esp_deconvolution": L.PDX_INFLUENCE_ESP_DECONVOLUTION}[self.influence]
+        s.split_shape = float(self.split_shape or 0.0)
+        s.window_shape = float(self.window_shape or 0.0)
         s.alias_order = int(self.alias_order)
         s.net_charge = 1 if self.net_charge == "background" else 0
         s.net_charge_tolerance = float(self.net_charge_tolerance)
@@ -134,6 +148,42 @@ def PME(*, coulomb_prefactor, real_cutoff, alpha=None, real_space_tolerance=None
                               net_charge=net_charge, real_space_modifier=real_space_modifier)
 
 
+def esp_rule(tolerance, order=None):
+    """(split shape c, window width P, window shape c1) of the ESP rules for a target relative force
+    error in an FP32 state (``pdx_esp_rule``; src/workflow/Electrostatics.jl:177-215,288-301)."""
+    out = (ctypes.c_double * 3)()
+    L.check(L.lib.pdx_esp_rule(float(tolerance), int(order or 0), out))
+    return float(out[0]), int(out[1]), float(out[2])
+
+
+def ESP(*, coulomb_prefactor, real_cutoff, tolerance=1e-4, split_shape=None, window_shape=None, order=None,
+        influence="optimal", alias_order=2, mesh=None, mesh_spacing=None, relative_permittivity=1.0,
+        net_charge="error", real_space_modifier="none"):
+    """Ewald summation with prolates (the workflow descriptor ESP, src/workflow/Electrostatics.jl:
+    102-154, 288-301): a prolate split of 1/r whose real-space part vanishes at ``real_cutoff`` and
+    a prolate assignment window of ``order`` grid points. Defaults from ``tolerance``: the split
+    shape c with psi(1)/||psi|| = tolerance, the width of Table 2 of Liang et al., the window shape
+    c(tolerance/2) (raised where an FP32 mesh would not resolve it) and the spacing pi rc / c.
+    ``influence``: "optimal" (AD-optimal, alias sums over |n_d| <= alias_order) or "deconvolution"."""
+    if split_shape is None or window_shape is None or order is None:
+        if tolerance is None:
+            raise L.ArgumentError("ESP needs `tolerance`, or explicit `split_shape`, `window_shape` and `order`.")
+        c, P, c1 = esp_rule(tolerance, order)
+        split_shape = c if split_shape is None else split_shape
+        order = P if order is None else order
+        window_shape = c1 if window_shape is None else window_shape
+    if mesh is None and mesh_spacing is None:
+        mesh_spacing = math.pi * float(real_cutoff) / float(split_shape)
+    if influence not in ("optimal", "deconvolution"):
+        raise L.ArgumentError(f"ESP influence must be :optimal or :deconvolution; got {influence}.")
+    return ElectrostaticsSpec(coulomb_prefactor=coulomb_prefactor, real_cutoff=real_cutoff, order=int(order), mesh=mesh,
+                              mesh_spacing=mesh_spacing, relative_permittivity=relative_permittivity,
+                              net_charge=net_charge, real_space_modifier=real_space_modifier,
+                              influence="esp" if influence == "optimal" else "esp_deconvolution",
+                              alias_order=alias_order, split_shape=float(split_shape),
+                              window_shape=float(window_shape))
+
+
 class ElectrostaticsSolver:
     def __init__(self, spec: ElectrostaticsSpec, box, charges):
         """``charges``: host sequence in storage order (set_solver_charges!, Solver.jl:166-185)."""
@@ -152,8 +202,9 @@ class ElectrostaticsSolver:
         L.check(L.lib.pdx_mesh_info(self._h, info))
         self.mesh = (int(info[0]), int(info[1]), int(info[2]))
         self.order = int(info[3])
-        self.alpha = info[4]
+        self.alpha = info[4] if not spec.is_esp else None
         self.C = info[5]
+        self.self_coefficient = info[6]
         self.plane_lines = int(info[8])
 
 
@@ -176,6 +227,9 @@ class ElectrostaticsSolver:
     def real_space(self):
         """The matching short-range term for a ``PairForce``."""
         from .forces import RealSpaceCoulomb
+        if self.spec.is_esp:
+            return RealSpaceCoulomb(C=self.C, cutoff=self.spec.real_cutoff, alpha=0.0, split="prolate",
+                                    shape=self.spec.split_shape)
         return RealSpaceCoulomb(C=self.C, cutoff=self.spec.real_cutoff, alpha=self.alpha,
                                 shift=self.spec.real_space_modifier == "potential_shift")
 
@@ -234,13 +288,17 @@ class ElectrostaticsSolver:
         return dict(mesh_energy=out[0], virial=list(out[1:7]), self_energy=out[7], background_energy=out[8])
 
     def self_energy(self):
-        return -0.5 * self.C * (2.0 * self.alpha / math.sqrt(math.pi)) * self.qsq
+        """-(C/2) self_coefficient sum q^2 (self_energy, Solver.jl:421-423; the coefficient
+        2 alpha/sqrt(pi) of the Gaussian split or chi(0)/rc of the prolate one)."""
+        return -0.5 * self.C * self.self_coefficient * self.qsq
 
     def background_energy(self):
+        """-C background_coefficient Q^2 / V with the neutralising background, else 0."""
         if self.spec.net_charge != "background":
             return 0.0
-        V = self.box[0] * self.box[1] * self.box[2]
-        return -self.C * (math.pi / (2 * self.alpha ** 2)) * self.qtotal ** 2 / V
+        info = (C.c_double * 9)()
+        L.check(L.lib.pdx_mesh_info(self._h, info))
+        return info[7] * self.qtotal
 
     def fixed_grid(self, stream=None):
         import torch

@@ -102,11 +102,15 @@ class PairMatrixWCA(PairMatrixLJ):
 
 @dataclass(frozen=True)
 class RealSpaceCoulomb:
-    """Short-range Ewald Coulomb C q_i q_j erfc(alpha r)/r inside ``cutoff``."""
+    """Short-range Coulomb C q_i q_j E_s(r)/r inside ``cutoff``: the Gaussian (Ewald) split
+    E_s = erfc(alpha r), or the prolate split of ESP (``split="prolate"``, bandlimit ``shape``),
+    whose E_s vanishes at the cutoff (src/electrostatics/Splits.jl:91-130)."""
     C: float
     cutoff: float
     alpha: float
     shift: bool = False
+    split: str = "gaussian"
+    shape: float = 0.0
 
 
 ACCUMULATION = {"fp32": 0, "fixed64": 1, "fp64": 2}
@@ -192,11 +196,14 @@ class PairForce:
             spec.cutoff = float(getattr(model, "cutoff", 0.0))
             spec.shift = 1 if getattr(model, "shift", False) else 0
         if coulomb is not None:
-            spec.coulomb = 1
+            if coulomb.split not in ("gaussian", "prolate"):
+                raise L.ArgumentError("the real-space split must be 'gaussian' or 'prolate'")
+            spec.coulomb = 2 if coulomb.split == "prolate" else 1
             spec.coulomb_shift = 1 if coulomb.shift else 0
             spec.coulomb_C = float(coulomb.C)
             spec.coulomb_cutoff = float(coulomb.cutoff)
             spec.alpha = float(coulomb.alpha)
+            spec.split_shape = float(coulomb.shape)
         spec.lanes = int(lanes)
         h = C.c_void_p()
         L.check(L.lib.pdx_pair_create(C.byref(spec), C.byref(h)))

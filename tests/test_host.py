@@ -42,3 +42,30 @@ def test_systems():
     v = systems.rock_salt(10, 2.82, 0.1, seed=0, vacancy_fraction=0.05)
     assert v["N"] == 1000 - 2 * 25 and v["q"].sum() == 0
     assert pdx.default_capacity(4000, s["box"], 2.9) == math.ceil(1.5 * 4 * math.pi / 3 * 2.9 ** 3 * 0.8442 + 64)
+
+
+def test_esp_descriptor_rules_and_refusals():
+    """The ESP descriptor's defaults (src/workflow/Electrostatics.jl:177-215,288-301): at 1e-4 the
+    split shape of Table 2 (12.024), width 6, window shape c(5e-5), spacing pi rc / c; the mesh
+    refuses, before any device work, a window wider than 2 c1/pi - 1/2 (Spec.jl:74-75), a width
+    this build does not compile, and a missing shape."""
+    c, P, c1 = pdx.esp_rule(1e-4)
+    assert c == pytest.approx(12.024, rel=5e-4) and P == 6 and c1 == pytest.approx(12.762, rel=5e-4)
+    assert pdx.esp_rule(1e-3)[1] == 5 and pdx.esp_rule(1e-5)[1] == 8 and pdx.esp_rule(1e-4, 7)[1] == 7
+    s = pdx.ESP(coulomb_prefactor=14.4, real_cutoff=9.0, tolerance=1e-4)
+    assert s.order == 6 and s.influence == "esp"
+    assert s.mesh_spacing == pytest.approx(math.pi * 9.0 / c)
+    assert s.resolve_mesh((112.8, 112.8, 112.8)) == (48, 48, 48)
+    cs = s.c_spec()
+    assert cs.influence == 4 and cs.split_shape == pytest.approx(c) and cs.window_shape == pytest.approx(c1)
+    assert pdx.ESP(coulomb_prefactor=1.0, real_cutoff=4.0, influence="deconvolution").influence == "esp_deconvolution"
+    with pytest.raises(pdx.ArgumentError, match="supports at most"):
+        pdx.ElectrostaticsSolver(pdx.ESP(coulomb_prefactor=1.0, real_cutoff=4.0, split_shape=12.0, window_shape=10.0,
+                                         order=8, mesh=(24, 24, 24)), (14.0, 15.0, 16.0), np.zeros(10))
+    with pytest.raises(pdx.ArgumentError, match="not compiled"):
+        pdx.ElectrostaticsSolver(pdx.ESP(coulomb_prefactor=1.0, real_cutoff=4.0, tolerance=1e-6, order=10,
+                                         mesh=(24, 24, 24)), (14.0, 15.0, 16.0), np.zeros(10))
+    with pytest.raises(pdx.ArgumentError):
+        pdx.ESP(coulomb_prefactor=1.0, real_cutoff=4.0, tolerance=None)
+    with pytest.raises(pdx.ArgumentError):
+        pdx.ESP(coulomb_prefactor=1.0, real_cutoff=4.0, influence="ik")
