@@ -22,6 +22,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "particledynamics.jl_amd"))
 
 HBM_PEAK_GBS = 8000.0
+
+
+def _log(msg):
+    """Progress of the long legs on stderr (stdout carries only the JSON line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
 FP32_PEAK_TFLOPS = 157.3
 DT_FS = 1.0
 
@@ -40,6 +46,13 @@ def parse():
     p.add_argument("--pair-only", action="store_true",
                    help="run only the headline system's pair kernel (`steps` launches) for profiling")
     p.add_argument("--pair-rc", type=float, default=9.0, help="real-space cutoff of --pair-only")
+    p.add_argument("--published-only", action="store_true",
+                   help="run only the Coulomb force calls at the reference's published methods and meshes")
+    p.add_argument("--c2-only", action="store_true",
+                   help="run only the 64 000-ion plain SPME leg (without the cluster backend): the profiling runs")
+    p.add_argument("--eval-only", default=None, metavar="RC,ORDER,MESH",
+                   help="run only `steps` graph replays of the headline system's plain PME evaluation at this "
+                        "real-space cutoff, order and cubic mesh (the profiling runs), e.g. 10,6,160")
     p.add_argument("--skin", type=float, default=1.0)
     p.add_argument("--no-replicas", action="store_true", help="skip the batched-replica run")
     p.add_argument("--no-precision", action="store_true", help="skip the bead-spring accumulation study")
@@ -101,22 +114,38 @@ def main():
     if args.pair_only:
         pair_only(args)
         return
+    if args.eval_only:
+        eval_only(args)
+        return
+    if args.published_only:
+        print(json.dumps({"published_coulomb": published_leg(args.steps, args.warmup, args.skin)}))
+        return
+    if args.c2_only:
+        args.no_clusters = True
+        print(json.dumps({"c2_rock_salt_64k": c2_leg(args)}))
+        return
     head = headline_leg(args.c5_sites, args.steps, args.warmup, args.skin, quick=args.quick)
     out = head.pop("line")
     if not args.no_small_c5:
+        _log("60 800-ion learned comparison")
         out["learned_60800"] = headline_leg(40, args.steps, args.warmup, args.skin, quick=args.quick)["line"]
         for k in ("metric", "unit", "n_gpus", "steps", "warmup", "higher_is_better", "scaling", "vs_baseline",
                   "dtype", "data", "roofline"):
             out["learned_60800"].pop(k, None)
     if not args.no_c2:
+        _log("64 000-ion plain SPME leg")
         out["c2_rock_salt_64k"] = c2_leg(args)
     if not args.no_published:
+        _log("published Coulomb force calls")
         out["published_coulomb"] = published_leg(args.steps, args.warmup, args.skin)
     if not args.no_replicas:
+        _log("512 replicas")
         out["replicas"] = replica_leg()
     if not args.no_precision:
+        _log("bead-spring precision study")
         out["precision"] = precision_leg(args.steps, args.warmup)
     if not args.no_cpu_baseline:
+        _log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(head["sysd"], head["plain_spec"])
     print(json.dumps(out))
 
@@ -206,6 +235,7 @@ def plain_search(P, box, skin, ids, Fref, quick=False):
                          (1.6, 1.45, 1.3, 1.2, 1.1, 1.0, 0.92, 0.85, 0.78, 0.7, 0.62, 0.55)})
             for K in Ks:
                 err = _coulomb_error(P, box, _spec(rc, order, K), skin, ids, Fref)
+                _log(f"plain PME rc {rc} order {order} mesh {K}: error {err:.3e}")
                 trials.append({"rc": rc, "order": order, "mesh": K, "err": err})
                 if err <= TARGET_ERR:
                     ms = _time_eval(P, box, _spec(rc, order, K), skin, 20, 3)
@@ -223,6 +253,7 @@ def train_correction(nsites, skin, coarse, fine, configs=2, steps=4000, batch=81
 
     from pdx import learned
     train = [_ion_system(nsites, 11 + k) for k in range(configs)]
+    _log(f"training data: {configs} configurations of {train[0]['N']} ions")
     t0 = time.perf_counter()
     F, T = learned.residual_dataset(train, coarse, fine, skin)
     mlp = learned.MLP(seed=0)
@@ -230,6 +261,7 @@ def train_correction(nsites, skin, coarse, fine, configs=2, steps=4000, batch=81
     mlp.set_norm(mu, isd, sc)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    _log(f"training the correction on {int(F.shape[0])} samples")
     hist = learned.train(mlp, F, T, steps=steps, batch=batch, lr=2e-3)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
@@ -250,6 +282,7 @@ def headline_leg(nsites, steps, warmup, skin, quick=False):
     N, box = sysd["N"], sysd["box"]
     P = _tensors(sysd)
     ids = torch.from_numpy(np.sort(np.random.default_rng(1).choice(N, 256, replace=False)).astype(np.int32)).cuda()
+    _log(f"headline system {N} ions: FP64 Ewald reference of 256 ions")
     t0 = time.perf_counter()
     Fref = _ewald_reference(P, box, ids)
     torch.cuda.synchronize()
@@ -259,6 +292,7 @@ def headline_leg(nsites, steps, warmup, skin, quick=False):
                                   mesh_spacing=h, order=order)
     coarse, fine = mk(2.0, 4), mk(0.5, 6)
     mlp, train_info = train_correction(nsites, skin, coarse, fine, configs=1 if quick else 2)
+    _log("coarse mesh and corrected errors")
     err_coarse = _coulomb_error(P, box, coarse, skin, ids, Fref)
     err_corr = _coulomb_error(P, box, coarse, skin, ids, Fref, correction=mlp)
     err_fine = _coulomb_error(P, box, fine, skin, ids, Fref)
@@ -281,6 +315,7 @@ def headline_leg(nsites, steps, warmup, skin, quick=False):
         t = min(trials, key=lambda r: r["err"])
         variant, spec, corr = "plain PME (target not reached)", _spec(t["rc"], t["order"], t["mesh"]), None
         chosen = t
+    _log(f"timing {steps} evaluations of {variant}")
     # the headline: exactly `steps` evaluations of the chosen variant, graph replays
     lj = pdx.LennardJones(epsilon=0.01, sigma=2.8, cutoff=9.0)
     eng = pdx.ForceEngine(N, box, lj, electrostatics=spec, charges=sysd["q"], skin=skin, correction=corr)
@@ -420,6 +455,21 @@ def pair_only(args):
     eng.status()
     print(json.dumps({"pair_only": True, "N": N, "launches": args.steps,
                       "row_visits": int(eng.nl.counts().to(torch.int64).sum().item())}))
+
+
+def eval_only(args):
+    """`steps` graph replays (after `warmup`) of the headline system's full plain PME evaluation
+    at the given cutoff, order and mesh, for kernel traces and counters."""
+    import torch
+    rc, order, K = args.eval_only.split(",")
+    sysd = _ion_system(args.c5_sites, 10)
+    P = _tensors(sysd)
+    ms, eng, _ = _time_eval(P, sysd["box"], _spec(float(rc), int(order), int(K)), args.skin, args.steps, args.warmup,
+                            keep=True)
+    torch.cuda.synchronize()
+    print(json.dumps({"eval_only": True, "N": sysd["N"], "rc": float(rc), "order": int(order), "mesh": int(K),
+                      "replays": args.steps, "ms_per_eval": round(ms, 4)}))
+    del eng
 
 
 def c2_leg(args):
@@ -587,10 +637,23 @@ def cluster_leg(steps, warmup, skin, nl_rows, X, Y, Z, rows_pair_ms):
             "note": "cluster tiles evaluate every pair of an 8 x 8 tile; rows visit only list entries"}
 
 
-def coulomb_error(X, Y, Z, Q, box, spec, C, skin, correction=None, nsample=256):
-    """Relative RMS Coulomb-force error of one mesh setting (optionally with the learned
-    correction) against FP64 Ewald (alpha 0.25, 12 digits) on sampled ions."""
+def coulomb_reference(X, Y, Z, Q, box, C, nsample=256):
+    """FP64 Ewald Coulomb forces on `nsample` seeded ions (device kernel, alpha 17/L at 12 digits:
+    the real space stays inside one image) and their ids."""
     import numpy as np
+    import torch
+
+    import pdx
+    N = X.numel()
+    ids = torch.from_numpy(np.sort(np.random.default_rng(1).choice(N, nsample, replace=False)).astype(np.int32)).cuda()
+    Fref, _ = pdx.ewald_fp64(*(t.double() for t in (X, Y, Z, Q)), box, C, 17.0 / min(box), digits=12.0, sample=ids,
+                             want_energy=False)
+    return ids, Fref
+
+
+def coulomb_error(X, Y, Z, Q, box, spec, C, skin, correction=None, ref=None):
+    """Relative RMS Coulomb-force error of one mesh setting (optionally with the learned
+    correction) against FP64 Ewald on sampled ions (``coulomb_reference``, or ``ref``)."""
     import torch
 
     import pdx
@@ -599,51 +662,68 @@ def coulomb_error(X, Y, Z, Q, box, spec, C, skin, correction=None, nsample=256):
                           correction=correction)
     g = [torch.empty(N, device="cuda") for _ in range(3)]
     eng.evaluate(X, Y, Z, Q, *g, None, None, energy=False)
-    ids = torch.from_numpy(np.sort(np.random.default_rng(1).choice(N, nsample, replace=False)).astype(np.int32)).cuda()
-    Fref, _ = pdx.ewald_fp64(*(t.double() for t in (X, Y, Z, Q)), box, C, 0.25, digits=12.0, sample=ids,
-                             want_energy=False)
+    ids, Fref = ref if ref is not None else coulomb_reference(X, Y, Z, Q, box, C)
     Fg = torch.stack(g, 1).double()[ids.long()]
+    del eng
     return math.sqrt(((Fg - Fref) ** 2).sum().item() / (Fref ** 2).sum().item())
 
 
-# The reference's published Coulomb force calls (docs/reference/manual/electrostatics.md:248-253):
-# 64 000 ions, measured force error 1e-4, rc 9 A, FP32, RTX 3090, one force call each.
-PUBLISHED_64K = (("SPME", "spme", 150, 1.14), ("P3M-AD", "p3m_ad", 150, 1.09), ("P3M-ik", "p3m_ik", 120, 1.15))
+# The reference's published Coulomb force calls (docs/reference/manual/electrostatics.md:248-256):
+# ions in a dense +-1 e melt, rc 9 A, FP32, RTX 3090, one force call each at a measured force
+# error of 1e-4 (or 1e-5): (name, method, sites per side, mesh, reference ms, target error).
+PUBLISHED = (("SPME", "spme", 40, 150, 1.14, 1e-4), ("P3M-AD", "p3m_ad", 40, 150, 1.09, 1e-4),
+             ("P3M-ik", "p3m_ik", 40, 120, 1.15, 1e-4), ("ESP", "esp", 40, 50, 0.74, 1e-4),
+             ("ESP 1e-5", "esp", 40, 63, 1.6, 1e-5), ("ESP 512k", "esp", 80, 100, 7.6, 1e-4),
+             ("SPME 512k", "spme", 80, 300, 13.5, 1e-4))
 
 
 def published_leg(steps, warmup, skin):
-    """The reference's published 64 000-ion Coulomb force calls at the same methods and meshes, on
-    64 000 +-1 e ions (rock salt with 0.5 A random displacements: disordered like the reference's
-    melt, whose relative force errors a near-perfect lattice would inflate), Coulomb only, rc 9 A:
-    per method the order and real-space tolerance (alpha) with the smallest measured force error
-    against FP64 Ewald are kept, and one force call (list check with predicated rebuild, real-space
-    pairs, spread, solve, gather; forces only) is timed as a graph replay. "matched" says whether
-    our error is within the reference's 1e-4."""
-    import numpy as np
+    """The reference's published Coulomb force calls at the same methods, meshes and sizes, on
+    64 000 (40^3 sites) and 512 000 (80^3) +-1 e ions (rock salt with 0.5 A random displacements:
+    disordered like the reference's melt, whose relative force errors a near-perfect lattice would
+    inflate), Coulomb only, rc 9 A: per method the setting with the smallest measured force error
+    against FP64 Ewald is kept (PME / P3M: order and real-space tolerance; ESP: the tolerance of
+    its rules, which sets the split and window shapes and the width), and one force call (list
+    check with predicated rebuild, real-space pairs, spread, solve, gather; forces only) is timed
+    as a graph replay. "matched" says whether our error is within the reference's target."""
     import torch
 
     import pdx
     from pdx import systems
     C = 14.399645
-    sysd = systems.rock_salt(40, 2.82, 0.5, seed=0)
-    N, box = sysd["N"], sysd["box"]
-    X, Y, Z, Q = (torch.from_numpy(sysd[k]).cuda() for k in ("x", "y", "z", "q"))
+    res = {"systems": "+-1 e ions (rock salt, 0.5 A random displacements), Coulomb only, rc 9 A",
+           "reference": "RTX 3090, dense +-1 e melt, measured force error 1e-4 / 1e-5 (electrostatics.md:248-256)"}
+    cache = {}
 
     def spec_of(kind, K, P, tol):
         if kind == "spme":
             return pdx.PME(coulomb_prefactor=C, real_cutoff=9.0, real_space_tolerance=tol, mesh=(K, K, K), order=P)
+        if kind == "esp":
+            return pdx.ESP(coulomb_prefactor=C, real_cutoff=9.0, tolerance=tol, mesh=(K, K, K))
         return pdx.PPPM(coulomb_prefactor=C, real_cutoff=9.0, real_space_tolerance=tol, mesh=(K, K, K), order=P,
                         differentiation=kind[-2:])
 
-    res = {"N": N, "system": "64 000 +-1 e ions (rock salt, 0.5 A random displacements), Coulomb only, rc 9 A",
-           "reference": "RTX 3090, dense +-1 e melt, measured force error 1e-4 (electrostatics.md:248-253)"}
-    for name, kind, K, ref_ms in PUBLISHED_64K:
+    for name, kind, sites, K, ref_ms, target in PUBLISHED:
+        if sites not in cache:
+            sysd = systems.rock_salt(sites, 2.82, 0.5, seed=0)
+            P4 = tuple(torch.from_numpy(sysd[k]).cuda() for k in ("x", "y", "z", "q"))
+            cache[sites] = (sysd, P4, coulomb_reference(*P4, sysd["box"], C))
+        sysd, (X, Y, Z, Q), ref = cache[sites]
+        N, box = sysd["N"], sysd["box"]
         trials = []
-        for P in ((4, 5) if kind == "spme" else (5,)):
-            for tol in (3e-5, 1e-5, 3e-6, 1e-6):
-                spec = spec_of(kind, K, P, tol)
-                trials.append((coulomb_error(X, Y, Z, Q, box, spec, C, skin), tol, spec, P))
+        if kind == "esp":
+            for tol in (target, target / 2, target / 4):
+                if pdx.esp_rule(tol)[1] > 8:  # widths above 8 are not compiled
+                    continue
+                spec = spec_of(kind, K, None, tol)
+                trials.append((coulomb_error(X, Y, Z, Q, box, spec, C, skin, ref=ref), tol, spec, spec.order))
+        else:
+            for P in ((4, 5) if kind == "spme" else (5,)):
+                for tol in (3e-5, 1e-5, 3e-6, 1e-6):
+                    spec = spec_of(kind, K, P, tol)
+                    trials.append((coulomb_error(X, Y, Z, Q, box, spec, C, skin, ref=ref), tol, spec, P))
         err, tol, spec, _ = min(trials, key=lambda t: t[0])
+        _log(f"published {name}: error {err:.2e}")
         eng = pdx.ForceEngine(N, box, None, electrostatics=spec, charges=sysd["q"], skin=skin)
         fx, fy, fz = (torch.empty(N, device="cuda") for _ in range(3))
         eng.capture(X, Y, Z, Q, fx, fy, fz, None, None, energy=False)
@@ -655,11 +735,16 @@ def published_leg(steps, warmup, skin):
             eng.replay()
         torch.cuda.synchronize()
         ms = 1e3 * (time.perf_counter() - a) / steps
-        res[name] = {"mesh": K, "order": eng.solver.order, "real_space_tolerance": tol,
-                     "alpha": round(spec.resolved_alpha(), 5), "ms_per_force_call": round(ms, 4),
-                     "force_rel_rms_vs_ewald": err, "matched": bool(err <= 1e-4),
-                     "reference_ms": ref_ms, "reference_over_ours": round(ref_ms / ms, 2),
-                     "errors_by_order_and_tolerance": {f"P{P} {t:g}": e for e, t, _, P in trials}}
+        entry = {"N": N, "mesh": K, "order": eng.solver.order, "ms_per_force_call": round(ms, 4),
+                 "force_rel_rms_vs_ewald": err, "target": target, "matched": bool(err <= target),
+                 "reference_ms": ref_ms, "reference_over_ours": round(ref_ms / ms, 2)}
+        if kind == "esp":
+            entry.update(tolerance=tol, split_shape=round(spec.split_shape, 4), window_shape=round(spec.window_shape, 4),
+                         errors_by_tolerance={f"{t:g}": e for e, t, _, _ in trials})
+        else:
+            entry.update(real_space_tolerance=tol, alpha=round(spec.resolved_alpha(), 5),
+                         errors_by_order_and_tolerance={f"P{P} {t:g}": e for e, t, _, P in trials})
+        res[name] = entry
         del eng
     return res
 
