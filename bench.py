@@ -189,15 +189,30 @@ def _coulomb_error(P, box, spec, skin, ids, Fref, correction=None):
     return err
 
 
-def _time_eval(P, box, spec, skin, steps, warmup, correction=None, keep=False):
-    """Full evaluation (LJ + Coulomb, energy and virial) as a graph replay: ms per evaluation."""
+TUNED = {}  # (N, rc, order, mesh, corrected) -> autotune result of that evaluation
+
+
+def _time_eval(P, box, spec, skin, steps, warmup, correction=None, keep=False, tune=False):
+    """Full evaluation (LJ + Coulomb, energy and virial) as a graph replay: ms per evaluation.
+    tune: first measure the launch-shape candidates of pdx.autotune on this system and keep the
+    fastest (set-up, outside the timing)."""
     import torch
 
     import pdx
+    from pdx import autotune
     N = P[0].numel()
     lj = pdx.LennardJones(epsilon=0.01, sigma=2.8, cutoff=9.0)
     eng = pdx.ForceEngine(N, box, lj, electrostatics=spec, charges=P[3].cpu().numpy(), skin=skin,
                           correction=correction)
+    if tune:
+        key = (N, spec.real_cutoff, spec.order, eng.solver.mesh[0], correction is not None)
+        if key not in TUNED:
+            TUNED[key] = autotune.tune(eng, *P, steps=10, warmup=3, rounds=2)
+            _log(f"tuned N {N} rc {spec.real_cutoff} order {spec.order} mesh {eng.solver.mesh[0]}"
+                 f"{' + correction' if correction is not None else ''}: {TUNED[key]['best']} "
+                 f"{TUNED[key]['ms']:.4f} ms")
+        else:
+            autotune.apply(eng, TUNED[key]["best"])
     bufs = [torch.empty(N, device="cuda") for _ in range(4)] + [torch.empty(6, N, device="cuda")]
     eng.capture(*P, *bufs, energy=True)
     for _ in range(max(warmup, 1)):
@@ -238,7 +253,7 @@ def plain_search(P, box, skin, ids, Fref, quick=False):
                 _log(f"plain PME rc {rc} order {order} mesh {K}: error {err:.3e}")
                 trials.append({"rc": rc, "order": order, "mesh": K, "err": err})
                 if err <= TARGET_ERR:
-                    ms = _time_eval(P, box, _spec(rc, order, K), skin, 20, 3)
+                    ms = _time_eval(P, box, _spec(rc, order, K), skin, 20, 3, tune=True)
                     feasible.append({"rc": rc, "order": order, "mesh": K, "err": err, "ms_per_eval": round(ms, 4)})
                     trials[-1]["ms_per_eval"] = round(ms, 4)
                     break
@@ -296,8 +311,8 @@ def headline_leg(nsites, steps, warmup, skin, quick=False):
     err_coarse = _coulomb_error(P, box, coarse, skin, ids, Fref)
     err_corr = _coulomb_error(P, box, coarse, skin, ids, Fref, correction=mlp)
     err_fine = _coulomb_error(P, box, fine, skin, ids, Fref)
-    ms_coarse = _time_eval(P, box, coarse, skin, 20, 3)
-    ms_corr = _time_eval(P, box, coarse, skin, 20, 3, correction=mlp)
+    ms_coarse = _time_eval(P, box, coarse, skin, 20, 3, tune=True)
+    ms_corr = _time_eval(P, box, coarse, skin, 20, 3, correction=mlp, tune=True)
     corrected = {"mesh": coarse.resolve_mesh(box)[0], "order": 4, "rc": 9.0, "err": err_corr,
                  "ms_per_eval": round(ms_corr, 4), "train": train_info}
     coarse_only = {"mesh": coarse.resolve_mesh(box)[0], "order": 4, "rc": 9.0, "err": err_coarse,
@@ -316,9 +331,14 @@ def headline_leg(nsites, steps, warmup, skin, quick=False):
         variant, spec, corr = "plain PME (target not reached)", _spec(t["rc"], t["order"], t["mesh"]), None
         chosen = t
     _log(f"timing {steps} evaluations of {variant}")
-    # the headline: exactly `steps` evaluations of the chosen variant, graph replays
+    # the headline: exactly `steps` evaluations of the chosen variant, graph replays, at the launch
+    # shapes the autotuner measured fastest for it
+    from pdx import autotune
     lj = pdx.LennardJones(epsilon=0.01, sigma=2.8, cutoff=9.0)
     eng = pdx.ForceEngine(N, box, lj, electrostatics=spec, charges=sysd["q"], skin=skin, correction=corr)
+    tkey = (N, spec.real_cutoff, spec.order, eng.solver.mesh[0], corr is not None)
+    if tkey in TUNED:
+        autotune.apply(eng, TUNED[tkey]["best"])
     bufs = [torch.empty(N, device="cuda") for _ in range(4)] + [torch.empty(6, N, device="cuda")]
     eng.capture(*P, *bufs, energy=True)
     for _ in range(max(warmup, 1)):
@@ -342,7 +362,7 @@ def headline_leg(nsites, steps, warmup, skin, quick=False):
                        "N": N, "variant": variant, "mesh": list(eng.solver.mesh), "order": eng.solver.order,
                        "real_cutoff": eng.solver.spec.real_cutoff, "alpha": eng.solver.alpha, "skin": skin,
                        "cap": eng.nl.cap, "rebuild_interval": st.interval, "mode": "graph",
-                       "parallelism": "single GPU"},
+                       "launch": autotune.current(eng), "parallelism": "single GPU"},
             "force_rel_rms_vs_ewald": chosen["err"],
             "force_error_sample": "Coulomb forces of 256 ions vs FP64 Ewald (device, alpha 17/L, 12 digits)",
             "ns_per_day": md["ns_per_day"], "md_step": md,
@@ -351,7 +371,11 @@ def headline_leg(nsites, steps, warmup, skin, quick=False):
                            "speedup_corrected_vs_plain": (round(best["ms_per_eval"] / ms_corr, 3) if plain_ok
                                                           else None),
                            "error_ratio_corrected_vs_target": round(err_corr / TARGET_ERR, 2),
-                           "plain_search": trials, "ewald_reference_s": round(t_ref, 2)},
+                           "plain_search": trials, "ewald_reference_s": round(t_ref, 2),
+                           "autotune": {f"N{k[0]} rc{k[1]} P{k[2]} K{k[3]}{' corrected' if k[4] else ''}":
+                                        {"best": v["best"], "ms": round(v["ms"], 4),
+                                         "candidates": [[c, m] for c, m in v["table"]]}
+                                        for k, v in TUNED.items() if k[0] == N}},
             "roofline": roofline}
     del eng
     return {"line": line, "sysd": sysd, "plain_spec": (best or chosen)}

@@ -187,6 +187,10 @@ int pdx_nl_rearm(pdx_nl_t nl, pdx_stream_t stream);
 int pdx_nl_resize_capacity(pdx_nl_t nl, int32_t cap);
 /* Change the box (set_device_box!, :89-116); the next check rebuilds. Host boundary. */
 int pdx_nl_set_box(pdx_nl_t nl, const double box[3], pdx_stream_t stream);
+/* Lanes per particle of the row build (the builder lanes of ENGINE_SETTINGS, src/engine/
+ * StateEngine.jl:7-27): 1, 2, 4, 8, 16 or 32; the default fills about 2^19 threads. The rows are
+ * the same set per particle whatever the lanes. A launch shape, set before capturing a graph. */
+int pdx_nl_set_row_lanes(pdx_nl_t nl, int32_t lanes);
 /* Boundary readback of the state words (device_neighbor_status, :639-651). Synchronises
  * `stream`. Returns PDX_ERR_CAPACITY / PDX_ERR_COORDINATE / PDX_ERR_STATE when check != 0
  * and the list is unusable; state is filled in every case. */
@@ -219,6 +223,10 @@ int pdx_pair_create(const pdx_pair_spec *spec, pdx_pair_t *out);
 int pdx_pair_destroy(pdx_pair_t p);
 /* Polynomial sizes and fit error: out[0]=n_pe, out[1]=n_pf, out[2]=max rel err (energy), out[3]=(force). */
 int pdx_pair_info(pdx_pair_t p, double out[4]);
+/* Traversal shape of later pair-force calls (the force lanes of ENGINE_SETTINGS, src/engine/
+ * StateEngine.jl:7-27): the `lanes` of pdx_pair_spec (0 automatic). A launch shape, set before
+ * capturing a graph: the first call with more workgroups may grow the energy workspace. */
+int pdx_pair_set_lanes(pdx_pair_t p, int32_t lanes);
 
 /* Forces (+ per-particle half energies and half virials with PDX_ENERGY) over the rows of nl
  * (_lj3_ell_kernel_virial!, LJELLEngineKernels.jl:44-79; _pair_cooperative_kernel!,

@@ -1048,3 +1048,11 @@ extern "C" int pdx_nl_status(pdx_nl_t nl, int64_t state[PDX_DN_NSTATE], int32_t 
   if (state[PDX_DN_VALID] == 0) return fail(PDX_ERR_STATE, "Device ELL has no valid list");
   return PDX_OK;
 }
+
+extern "C" int pdx_nl_set_row_lanes(pdx_nl_t nl, int32_t lanes) {
+  if (!nl) return fail(PDX_ERR_ARGUMENT, "null neighbor list");
+  if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32)
+    return fail(PDX_ERR_ARGUMENT, "row-build lanes must be 1, 2, 4, 8, 16 or 32");
+  nl->row_lanes = lanes;
+  return PDX_OK;
+}

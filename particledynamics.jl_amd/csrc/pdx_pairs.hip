@@ -1366,6 +1366,18 @@ extern "C" int pdx_pair_set_exceptions(pdx_pair_t p, int32_t n, int32_t nexc, co
   return PDX_OK;
 }
 
+extern "C" int pdx_pair_set_lanes(pdx_pair_t p, int32_t lanes) {
+  if (!p) return fail(PDX_ERR_ARGUMENT, "null pair model");
+  if (lanes > 0 && (lanes > 64 || (lanes & (lanes - 1))))
+    return fail(PDX_ERR_ARGUMENT, "lanes per particle must be a power of two in 1..64");
+  if (lanes < 0 && !(lanes == -2 || lanes == -4 || lanes == -8 || lanes == -16))
+    return fail(PDX_ERR_ARGUMENT, "split layouts use 2, 4, 8 or 16 waves (lanes = -2, -4, -8, -16)");
+  if (lanes < 0 && (typed_model(p->spec.model) || p->nexc > 0 || p->bonds || p->accumulate != PDX_ACC_FP32))
+    return fail(PDX_ERR_ARGUMENT, "typed models, exceptions, bonds and accumulation modes use the lanes layout (lanes >= 0)");
+  p->lanes = lanes;
+  return PDX_OK;
+}
+
 extern "C" int pdx_pair_set_traversal(pdx_pair_t p, int32_t mode) {
   if (!p) return fail(PDX_ERR_ARGUMENT, "null pair model");
   if (mode != PDX_TRAVERSAL_ROWS && mode != PDX_TRAVERSAL_CELLS)

@@ -108,6 +108,8 @@ def _load():
         "pdx_pair_create": ([P(pdx_pair_spec), P(vp)], C.c_int),
         "pdx_pair_destroy": ([vp], C.c_int),
         "pdx_pair_info": ([vp, P(dbl)], C.c_int),
+        "pdx_pair_set_lanes": ([vp, i32], C.c_int),
+        "pdx_nl_set_row_lanes": ([vp, i32], C.c_int),
         "pdx_pair_forces": ([vp, vp, vp, vp, vp, vp, vp, vp, i32, vp], C.c_int),
         "pdx_pair_compute": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp], C.c_int),
         "pdx_pair_set_sigmas": ([vp, i32, P(C.c_float)], C.c_int),

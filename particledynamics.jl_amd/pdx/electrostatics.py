@@ -206,6 +206,7 @@ class ElectrostaticsSolver:
         self.C = info[5]
         self.self_coefficient = info[6]
         self.plane_lines = int(info[8])
+        self.spread_threads, self.transform_threads = 512, 1024
 
 
     @property
@@ -245,11 +246,13 @@ class ElectrostaticsSolver:
     def set_spread_threads(self, threads):
         """Workgroup size of the owner-computes spreads (512, 1024 or 256)."""
         L.check(L.lib.pdx_mesh_set_spread_threads(self._h, int(threads)))
+        self.spread_threads = int(threads)
 
     def set_transform_threads(self, threads):
         """Workgroup size of the plane passes of the solve (1024 or 512): the same potential,
         the FP64 energy and virial to FP64 rounding."""
         L.check(L.lib.pdx_mesh_set_transform_threads(self._h, int(threads)))
+        self.transform_threads = int(threads)
 
     def spread_snapshot(self, nl, stream=None):
         """Owner-computes spread from ``nl``'s cell snapshot (may run beside ``nl.build``)."""

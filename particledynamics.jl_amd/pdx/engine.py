@@ -26,7 +26,7 @@ class ForceEngine:
     def __init__(self, N, box, pair_model=None, electrostatics=None, charges=None, skin=0.4, cap=None,
                  interval=20, lanes=0, diagnostics=False, spread="snapshot", correction=None, accumulation="fp32",
                  bonds=None, exceptions=(), traversal="rows", backend="ell", prune_buffer=None, cluster_capacity=0,
-                 overlap_with="fft", gather="tiles", exclude_bonded=False):
+                 overlap_with="fft", gather="tiles", exclude_bonded=False, settings=None):
         """backend "ell": the cell list and ELL rows (``pdx_nl_*`` + ``pdx_pair_forces``); "cluster":
         the cluster-pair list and tiles (``pdx_cluster_*``, the reference's ``ClusterBackend``,
         src/engine/ClusterBackend.jl:1-40); its mesh spread takes the cluster order."""
@@ -108,6 +108,14 @@ class ForceEngine:
             self.feats = torch.empty(self.N, NFEAT, dtype=torch.float32, device="cuda")
         self._graph = None
         self._snap_ready = False
+        # launch shapes (pdx.autotune): a setting dict, or "auto" for the measured table's setting
+        # of this system's class (none: the defaults above)
+        if settings is not None:
+            from . import autotune
+            if settings == "auto":
+                settings = autotune.settings_for(self.N, self.box, self.cutoff, self.solver is not None)
+            if settings:
+                autotune.apply(self, settings)
 
     def evaluate(self, x, y, z, q, fx, fy, fz, Epot=None, V=None, energy=True, stream=None, overlap=True,
                  kick_drift=None):

@@ -221,6 +221,12 @@ class PairForce:
         if bonds is not None:
             self.set_bonds(bonds)
 
+    def set_lanes(self, lanes: int):
+        """Lanes per particle of the row traversal (0 automatic; the force lanes of the engine
+        settings), set before a graph capture."""
+        L.check(L.lib.pdx_pair_set_lanes(self._h, int(lanes)))
+        self.lanes = int(lanes)
+
     def set_traversal(self, mode: str):
         """"rows": the list's ELL rows; "cells": the list's cell grid directly, one wave per
         particle with ballot-compacted candidates (no rows are read)."""

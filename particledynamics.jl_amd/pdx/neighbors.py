@@ -104,6 +104,11 @@ class DeviceNeighborWorkspace:
         L.check(L.lib.pdx_nl_spatial_order(self._h, L.ptr(perm), L.stream_handle(stream)))
         return perm.long()
 
+    def set_row_lanes(self, lanes: int):
+        """Lanes per particle of the row build (the builder lanes of the engine settings)."""
+        L.check(L.lib.pdx_nl_set_row_lanes(self._h, int(lanes)))
+        self.row_lanes_set = int(lanes)
+
     def invalidate(self, stream=None):
         """Particles were relabelled: the next check rebuilds (``pdx_nl_invalidate``)."""
         L.check(L.lib.pdx_nl_invalidate(self._h, L.stream_handle(stream)))

@@ -69,3 +69,21 @@ def test_esp_descriptor_rules_and_refusals():
         pdx.ESP(coulomb_prefactor=1.0, real_cutoff=4.0, tolerance=None)
     with pytest.raises(pdx.ArgumentError):
         pdx.ESP(coulomb_prefactor=1.0, real_cutoff=4.0, influence="ik")
+
+
+def test_autotune_classes_and_table(tmp_path):
+    """The reference's classes (engine_settings, src/engine/StateEngine.jl:47-56): size N <= 30 000,
+    <= 300 000, larger; neighbours rho 4 pi/3 rc^3 with borders 9.8, 36, 72; a saved table is read
+    back and looked up by class."""
+    from pdx import autotune
+    L = (4000 / 0.8442) ** (1 / 3)
+    assert autotune.classify(4000, (L, L, L), 2.5) == (1, 3)       # LJ fluid: 55 neighbours
+    assert autotune.classify(64000, (112.8,) * 3, 9.0) == (2, 4)   # rock salt: 135 within 9 A
+    assert autotune.classify(486400, (225.6,) * 3, 10.0) == (3, 4)
+    assert autotune.classify(1000, (100.0,) * 3, 2.5) == (1, 1)
+    path = str(tmp_path / "t.json")
+    autotune.save({"2,4,mesh": {"force_lanes": 4, "overlap": "fft"}}, path)
+    t = autotune.load(path)
+    assert autotune.settings_for(64000, (112.8,) * 3, 9.0, True, table=t)["force_lanes"] == 4
+    assert autotune.settings_for(64000, (112.8,) * 3, 9.0, False, table=t) is None
+    assert autotune.load(str(tmp_path / "missing.json")) == {}
