@@ -46,14 +46,15 @@ def parse():
     p.add_argument("--pair-only", action="store_true",
                    help="run only the headline system's pair kernel (`steps` launches) for profiling")
     p.add_argument("--pair-rc", type=float, default=9.0, help="real-space cutoff of --pair-only")
-    p.add_argument("--published-only", action="store_true",
-                   help="run only the Coulomb force calls at the reference's published methods and meshes")
-    p.add_argument("--c2-only", action="store_true",
-                   help="run only the 64 000-ion plain SPME leg (without the cluster backend): the profiling runs")
+    p.add_argument("--only", choices=("c2", "published", "replicas", "precision"), default=None,
+                   help="run only one extra leg (the profiling runs): the 64 000-ion plain SPME leg with its cluster "
+                        "backend, the published Coulomb force calls, the 512 replicas or the bead-spring study")
     p.add_argument("--eval-only", default=None, metavar="RC,ORDER,MESH",
                    help="run only `steps` graph replays of the headline system's plain PME evaluation at this "
                         "real-space cutoff, order and cubic mesh (the profiling runs), e.g. 10,6,160")
     p.add_argument("--skin", type=float, default=1.0)
+    p.add_argument("--no-reorder", action="store_true",
+                   help="keep the headline systems in their generated (lattice) storage order")
     p.add_argument("--no-replicas", action="store_true", help="skip the batched-replica run")
     p.add_argument("--no-precision", action="store_true", help="skip the bead-spring accumulation study")
     p.add_argument("--no-published", action="store_true",
@@ -117,18 +118,20 @@ def main():
     if args.eval_only:
         eval_only(args)
         return
-    if args.published_only:
-        print(json.dumps({"published_coulomb": published_leg(args.steps, args.warmup, args.skin)}))
+    if args.only:
+        leg = {"c2": ("c2_rock_salt_64k", lambda: c2_leg(args)),
+               "published": ("published_coulomb", lambda: published_leg(args.steps, args.warmup, args.skin)),
+               "replicas": ("replicas", replica_leg),
+               "precision": ("precision", lambda: precision_leg(args.steps, args.warmup))}[args.only]
+        print(json.dumps({leg[0]: leg[1]()}))
         return
-    if args.c2_only:
-        args.no_clusters = True
-        print(json.dumps({"c2_rock_salt_64k": c2_leg(args)}))
-        return
-    head = headline_leg(args.c5_sites, args.steps, args.warmup, args.skin, quick=args.quick)
+    head = headline_leg(args.c5_sites, args.steps, args.warmup, args.skin, quick=args.quick,
+                        reorder=not args.no_reorder)
     out = head.pop("line")
     if not args.no_small_c5:
         _log("60 800-ion learned comparison")
-        out["learned_60800"] = headline_leg(40, args.steps, args.warmup, args.skin, quick=args.quick)["line"]
+        out["learned_60800"] = headline_leg(40, args.steps, args.warmup, args.skin, quick=args.quick,
+                                            reorder=not args.no_reorder)["line"]
         for k in ("metric", "unit", "n_gpus", "steps", "warmup", "higher_is_better", "scaling", "vs_baseline",
                   "dtype", "data", "roofline"):
             out["learned_60800"].pop(k, None)
@@ -192,7 +195,7 @@ def _coulomb_error(P, box, spec, skin, ids, Fref, correction=None):
 TUNED = {}  # (N, rc, order, mesh, corrected) -> autotune result of that evaluation
 
 
-def _time_eval(P, box, spec, skin, steps, warmup, correction=None, keep=False, tune=False):
+def _time_eval(P, box, spec, skin, steps, warmup, correction=None, keep=False, tune=False, settings=None):
     """Full evaluation (LJ + Coulomb, energy and virial) as a graph replay: ms per evaluation.
     tune: first measure the launch-shape candidates of pdx.autotune on this system and keep the
     fastest (set-up, outside the timing)."""
@@ -203,7 +206,7 @@ def _time_eval(P, box, spec, skin, steps, warmup, correction=None, keep=False, t
     N = P[0].numel()
     lj = pdx.LennardJones(epsilon=0.01, sigma=2.8, cutoff=9.0)
     eng = pdx.ForceEngine(N, box, lj, electrostatics=spec, charges=P[3].cpu().numpy(), skin=skin,
-                          correction=correction)
+                          correction=correction, settings=settings)
     if tune:
         key = (N, spec.real_cutoff, spec.order, eng.solver.mesh[0], correction is not None)
         if key not in TUNED:
@@ -286,14 +289,40 @@ def train_correction(nsites, skin, coarse, fine, configs=2, steps=4000, batch=81
     return mlp, info
 
 
-def headline_leg(nsites, steps, warmup, skin, quick=False):
+def spatially_ordered(sysd, skin):
+    """The system relabelled into the spatial order of the engine's cell list (ForceEngine.reorder,
+    the reference's spatial reordering, src/engine/Ordering.jl:19-79, which its published timings
+    use): storage neighbours are then space neighbours. Host-side set-up of the inputs."""
+    import numpy as np
+    import torch
+
+    import pdx
+    N, box = sysd["N"], sysd["box"]
+    X, Y, Z, Q = _tensors(sysd)
+    eng = pdx.ForceEngine(N, box, pdx.LennardJones(epsilon=0.01, sigma=2.8, cutoff=9.0),
+                          electrostatics=_spec(10.0, 4, pdx.fft_friendly(int(math.ceil(box[0] / 2.0)))),
+                          charges=sysd["q"], skin=skin)
+    f = [torch.empty(N, device="cuda") for _ in range(3)]
+    eng.evaluate(X, Y, Z, Q, *f, None, None, energy=False)
+    perm = eng.reorder(X, Y, Z, Q).cpu().numpy()
+    del eng
+    out = dict(sysd)
+    for k in ("x", "y", "z", "q", "mass"):
+        out[k] = np.ascontiguousarray(np.asarray(sysd[k])[perm])
+    return out
+
+
+def headline_leg(nsites, steps, warmup, skin, quick=False, reorder=True):
     """Learned-corrected coarse PME against plain PME tuned to the same 1e-4 force error, on the
-    held-out configuration (seed 10) of `nsites`^3 sites with 5 % vacated."""
+    held-out configuration (seed 10) of `nsites`^3 sites with 5 % vacated (spatially ordered
+    unless reorder is False)."""
     import numpy as np
     import torch
 
     import pdx
     sysd = _ion_system(nsites, 10)
+    if reorder:
+        sysd = spatially_ordered(sysd, skin)
     N, box = sysd["N"], sysd["box"]
     P = _tensors(sysd)
     ids = torch.from_numpy(np.sort(np.random.default_rng(1).choice(N, 256, replace=False)).astype(np.int32)).cuda()
@@ -362,6 +391,7 @@ def headline_leg(nsites, steps, warmup, skin, quick=False):
                        "N": N, "variant": variant, "mesh": list(eng.solver.mesh), "order": eng.solver.order,
                        "real_cutoff": eng.solver.spec.real_cutoff, "alpha": eng.solver.alpha, "skin": skin,
                        "cap": eng.nl.cap, "rebuild_interval": st.interval, "mode": "graph",
+                       "storage_order": "spatial (ForceEngine.reorder at set-up)" if reorder else "lattice",
                        "launch": autotune.current(eng), "parallelism": "single GPU"},
             "force_rel_rms_vs_ewald": chosen["err"],
             "force_error_sample": "Coulomb forces of 256 ions vs FP64 Ewald (device, alpha 17/L, 12 digits)",
@@ -465,11 +495,13 @@ def pair_only(args):
 
     import pdx
     sysd = _ion_system(args.c5_sites, 10)
+    if not args.no_reorder:
+        sysd = spatially_ordered(sysd, args.skin)
     N, box = sysd["N"], sysd["box"]
     P = _tensors(sysd)
     spec = _spec(args.pair_rc, 5, pdx.fft_friendly(int(math.ceil(box[0]))))
     lj = pdx.LennardJones(epsilon=0.01, sigma=2.8, cutoff=9.0)
-    eng = pdx.ForceEngine(N, box, lj, electrostatics=spec, charges=sysd["q"], skin=args.skin)
+    eng = pdx.ForceEngine(N, box, lj, electrostatics=spec, charges=sysd["q"], skin=args.skin, settings="auto")
     f = [torch.empty(N, device="cuda") for _ in range(4)]
     V = torch.empty(6, N, device="cuda")
     eng.nl.update(*P)
@@ -477,22 +509,27 @@ def pair_only(args):
         eng.pairs.compute_at(eng.nl, *P, *f, V, None, energy=True)
     torch.cuda.synchronize()
     eng.status()
-    print(json.dumps({"pair_only": True, "N": N, "launches": args.steps,
+    from pdx import autotune
+    print(json.dumps({"pair_only": True, "N": N, "launches": args.steps, "launch": autotune.current(eng),
                       "row_visits": int(eng.nl.counts().to(torch.int64).sum().item())}))
 
 
 def eval_only(args):
     """`steps` graph replays (after `warmup`) of the headline system's full plain PME evaluation
-    at the given cutoff, order and mesh, for kernel traces and counters."""
+    at the given cutoff, order and mesh, at the launch shapes of the measured settings table
+    (pdx/engine_settings.json, what the headline's tuning chose), for kernel traces and counters."""
     import torch
+    from pdx import autotune
     rc, order, K = args.eval_only.split(",")
     sysd = _ion_system(args.c5_sites, 10)
+    if not args.no_reorder:
+        sysd = spatially_ordered(sysd, args.skin)
     P = _tensors(sysd)
     ms, eng, _ = _time_eval(P, sysd["box"], _spec(float(rc), int(order), int(K)), args.skin, args.steps, args.warmup,
-                            keep=True)
+                            keep=True, settings="auto")
     torch.cuda.synchronize()
     print(json.dumps({"eval_only": True, "N": sysd["N"], "rc": float(rc), "order": int(order), "mesh": int(K),
-                      "replays": args.steps, "ms_per_eval": round(ms, 4)}))
+                      "replays": args.steps, "ms_per_eval": round(ms, 4), "launch": autotune.current(eng)}))
     del eng
 
 
@@ -706,8 +743,9 @@ def published_leg(steps, warmup, skin):
     64 000 (40^3 sites) and 512 000 (80^3) +-1 e ions (rock salt with 0.5 A random displacements:
     disordered like the reference's melt, whose relative force errors a near-perfect lattice would
     inflate), Coulomb only, rc 9 A: per method the setting with the smallest measured force error
-    against FP64 Ewald is kept (PME / P3M: order and real-space tolerance; ESP: the tolerance of
-    its rules, which sets the split and window shapes and the width), and one force call (list
+    against FP64 Ewald is kept (PME / P3M: order and real-space tolerance; ESP: the loosest tolerance
+    of its rules that meets the target, which sets the split and window shapes and the width), and
+    one force call (list
     check with predicated rebuild, real-space pairs, spread, solve, gather; forces only) is timed
     as a graph replay. "matched" says whether our error is within the reference's target."""
     import torch
@@ -746,7 +784,13 @@ def published_leg(steps, warmup, skin):
                 for tol in (3e-5, 1e-5, 3e-6, 1e-6):
                     spec = spec_of(kind, K, P, tol)
                     trials.append((coulomb_error(X, Y, Z, Q, box, spec, C, skin, ref=ref), tol, spec, P))
-        err, tol, spec, _ = min(trials, key=lambda t: t[0])
+        if kind == "esp":
+            # the loosest tolerance that meets the target: its rules give the narrowest window
+            # (cheapest spread and gather) at this mesh; the most accurate one if none does
+            ok = [t for t in trials if t[0] <= target]
+            err, tol, spec, _ = max(ok, key=lambda t: t[1]) if ok else min(trials, key=lambda t: t[0])
+        else:
+            err, tol, spec, _ = min(trials, key=lambda t: t[0])
         _log(f"published {name}: error {err:.2e}")
         eng = pdx.ForceEngine(N, box, None, electrostatics=spec, charges=sysd["q"], skin=skin)
         fx, fy, fz = (torch.empty(N, device="cuda") for _ in range(3))

@@ -364,7 +364,13 @@ __global__ __launch_bounds__(WAVE) void k_sort_cells(int ncell, const int32_t *o
 // cooperative placement as _cooperative_rows3_kernel!, src/engine/Builder.jl:39-129). Slots are
 // written slot-major at t*N + i with 1-based ids; the count keeps growing past cap so the
 // commit can report the required capacity.
-template <int G>
+// STAGE (opt-in): the workgroup's rows are kept in LDS as [slot][particle] (TPB/G particles, cap
+// slots, at most 64 KiB) and written out slot by slot after the search, so a row slot of
+// consecutive particles is one contiguous store; written straight from the search, the slots of
+// a wave's particles advance at different rates and the 4-byte stores land in partially filled
+// lines (measured: 8.6x the row bytes at 486 400 ions, 2.0x staged). Same rows either way.
+constexpr size_t ROW_STAGE_BYTES = 65536;
+template <int G, bool STAGE>
 __global__ __launch_bounds__(TPB) void k_rows(int N, int cap, int nx, int ny, int nz,
                                                const float4 *__restrict__ sorted,
                                                const int32_t *__restrict__ offsets,
@@ -372,64 +378,92 @@ __global__ __launch_bounds__(TPB) void k_rows(int N, int cap, int nx, int ny, in
                                                const float *geom, float rl2, int32_t *nbr,
                                                int32_t *counts, int64_t *candidates,
                                                const int64_t *state) {
-  if (!rebuild_requested(state)) return;
+  constexpr int PB = TPB / G;  // particles per workgroup
+  extern __shared__ int32_t srow[];  // STAGE: [cap][PB]
+  __shared__ int sfound[PB], sid[PB];
+  __shared__ int smax;
+  if (!rebuild_requested(state)) return;  // uniform over the grid
   const int gid = blockIdx.x * TPB + threadIdx.x;
   const int t = gid / G;
   const int lg = gid % G;
+  const int pb = threadIdx.x / G;
   const int wl = threadIdx.x & (WAVE - 1);
   const unsigned long long gmask = (G == WAVE) ? ~0ull : (((1ull << G) - 1ull) << (wl - lg));
   const unsigned long long below = (1ull << wl) - 1ull;
-  if (t >= N) return;  // whole groups leave together (G divides the block)
+  const bool active = t < N;  // whole groups are active or not (G divides the block)
   const float Lx = geom[0], Ly = geom[1], Lz = geom[2];
   const float hx = 0.5f * Lx, hy = 0.5f * Ly, hz = 0.5f * Lz;
-  const float4 pi = sorted[t];
-  const int i = __float_as_int(pi.w);
-  const int c0 = cell_of_particle[i];
-  const int cx = c0 % nx, cy = (c0 / nx) % ny, cz = c0 / (nx * ny);
-  int found = 0;
+  int found = 0, i = -1;
   int64_t tested = 0;
-  for (int oz = -1; oz <= 1; ++oz) {
-    int cz2 = cz + oz; cz2 -= (cz2 >= nz) ? nz : 0; cz2 += (cz2 < 0) ? nz : 0;
-    for (int oy = -1; oy <= 1; ++oy) {
-      int cy2 = cy + oy; cy2 -= (cy2 >= ny) ? ny : 0; cy2 += (cy2 < 0) ? ny : 0;
-      for (int ox = -1; ox <= 1; ++ox) {
-        int cx2 = cx + ox; cx2 -= (cx2 >= nx) ? nx : 0; cx2 += (cx2 < 0) ? nx : 0;
-        const int c = (cz2 * ny + cy2) * nx + cx2;
-        const int s = offsets[c] - 1, e = offsets[c + 1] - 1;
-        for (int k0 = s; k0 < e; k0 += G) {
-          const int k = k0 + lg;
-          bool acc = false, cand = false;
-          int j = 0;
-          if (k < e) {
-            const float4 pj = sorted[k];
-            j = __float_as_int(pj.w);
-            cand = j != i;
-            const float dx = mic_fast(pj.x - pi.x, hx, Lx);
-            const float dy = mic_fast(pj.y - pi.y, hy, Ly);
-            const float dz = mic_fast(pj.z - pi.z, hz, Lz);
-            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            acc = cand && (r2 <= rl2);
-          }
-          if (G == 1) {
-            tested += cand;
-            if (acc) {
-              if (found < cap) nbr[(size_t)found * N + i] = j + 1;
-              ++found;
+  if (active) {
+    const float4 pi = sorted[t];
+    i = __float_as_int(pi.w);
+    const int c0 = cell_of_particle[i];
+    const int cx = c0 % nx, cy = (c0 / nx) % ny, cz = c0 / (nx * ny);
+    for (int oz = -1; oz <= 1; ++oz) {
+      int cz2 = cz + oz; cz2 -= (cz2 >= nz) ? nz : 0; cz2 += (cz2 < 0) ? nz : 0;
+      for (int oy = -1; oy <= 1; ++oy) {
+        int cy2 = cy + oy; cy2 -= (cy2 >= ny) ? ny : 0; cy2 += (cy2 < 0) ? ny : 0;
+        for (int ox = -1; ox <= 1; ++ox) {
+          int cx2 = cx + ox; cx2 -= (cx2 >= nx) ? nx : 0; cx2 += (cx2 < 0) ? nx : 0;
+          const int c = (cz2 * ny + cy2) * nx + cx2;
+          const int s = offsets[c] - 1, e = offsets[c + 1] - 1;
+          for (int k0 = s; k0 < e; k0 += G) {
+            const int k = k0 + lg;
+            bool acc = false, cand = false;
+            int j = 0;
+            if (k < e) {
+              const float4 pj = sorted[k];
+              j = __float_as_int(pj.w);
+              cand = j != i;
+              const float dx = mic_fast(pj.x - pi.x, hx, Lx);
+              const float dy = mic_fast(pj.y - pi.y, hy, Ly);
+              const float dz = mic_fast(pj.z - pi.z, hz, Lz);
+              const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+              acc = cand && (r2 <= rl2);
             }
-          } else {
-            const unsigned long long am = __ballot(acc) & gmask;
-            tested += __popcll(__ballot(cand) & gmask);
-            if (acc) {
-              const int slot = found + __popcll(am & below);
-              if (slot < cap) nbr[(size_t)slot * N + i] = j + 1;
+            if (G == 1) {
+              tested += cand;
+              if (acc) {
+                if (found < cap) {
+                  if (STAGE) srow[found * PB + pb] = j + 1;
+                  else nbr[(size_t)found * N + i] = j + 1;
+                }
+                ++found;
+              }
+            } else {
+              const unsigned long long am = __ballot(acc) & gmask;
+              tested += __popcll(__ballot(cand) & gmask);
+              if (acc) {
+                const int slot = found + __popcll(am & below);
+                if (slot < cap) {
+                  if (STAGE) srow[slot * PB + pb] = j + 1;
+                  else nbr[(size_t)slot * N + i] = j + 1;
+                }
+              }
+              found += __popcll(am);
             }
-            found += __popcll(am);
           }
         }
       }
     }
   }
-  if (lg == 0) {
+  if (STAGE) {
+    if (lg == 0) {
+      sfound[pb] = active ? min(found, cap) : 0;
+      sid[pb] = i;
+    }
+    if (threadIdx.x == 0) smax = 0;
+    __syncthreads();
+    if (lg == 0 && active) atomicMax(&smax, min(found, cap));
+    __syncthreads();
+    const int ns = smax;
+    for (int e = threadIdx.x; e < ns * PB; e += TPB) {
+      const int sl = e / PB, q = e - sl * PB;
+      if (sl < sfound[q]) nbr[(size_t)sl * N + sid[q]] = srow[sl * PB + q];
+    }
+  }
+  if (active && lg == 0) {
     counts[i] = found;
     if (candidates) candidates[i] = tested;
   }
@@ -655,12 +689,30 @@ int enqueue_rebuild(pdx_nl_t nl, hipStream_t st) {
                        nl->cell_offsets.p, nl->particle_ids_sorted.p, s);
     hipLaunchKernelGGL(k_sort_cells, dim3(nl->ncell), dim3(WAVE), 0, st, nl->ncell, nl->cell_offsets.p,
                        nl->particle_ids_sorted.p, nl->packed.p, nl->sorted.p, s);
-    const int G = nl->row_lanes;
+    // lanes per particle: the list's row lanes; with PDX_ROW_STAGE=1 the rows are staged in LDS and
+    // the lanes raised until the workgroup's rows fit the 64 KiB stage. Staging cuts the build's HBM
+    // writes at 486 400 ions from 3.97 GB to 0.93 GB but not its time (1.13 ms at 32 lanes against
+    // 1.12 ms written straight at 8), so the default writes straight from the search.
+    int G = nl->row_lanes;
+    bool stage = false;
+    if (const char *v = std::getenv("PDX_ROW_STAGE")) stage = std::atoi(v) != 0;
+    if (stage) {
+      int g = G;
+      while (g < 32 && (size_t)(TPB / g) * nl->cap * sizeof(int32_t) > ROW_STAGE_BYTES) g <<= 1;
+      if ((size_t)(TPB / g) * nl->cap * sizeof(int32_t) <= ROW_STAGE_BYTES) G = g;
+      else stage = false;
+    }
     const int rb = (int)(((int64_t)N * G + TPB - 1) / TPB);
+    const size_t lds = stage ? (size_t)(TPB / G) * nl->cap * sizeof(int32_t) : 0;
 #define PDX_ROWS(GG)                                                                                       \
-  hipLaunchKernelGGL(k_rows<GG>, dim3(rb), dim3(TPB), 0, st, N, nl->cap, nl->nx, nl->ny, nl->nz, nl->sorted.p, \
-                     nl->cell_offsets.p, nl->cell_of_particle.p, geom, nl->rl2, nl->neighbors_flat.p,           \
-                     nl->counts.p, nl->diagnostics ? nl->candidates.p : nullptr, s);
+  if (stage)                                                                                               \
+    hipLaunchKernelGGL((k_rows<GG, true>), dim3(rb), dim3(TPB), lds, st, N, nl->cap, nl->nx, nl->ny, nl->nz, \
+                       nl->sorted.p, nl->cell_offsets.p, nl->cell_of_particle.p, geom, nl->rl2,            \
+                       nl->neighbors_flat.p, nl->counts.p, nl->diagnostics ? nl->candidates.p : nullptr, s); \
+  else                                                                                                     \
+    hipLaunchKernelGGL((k_rows<GG, false>), dim3(rb), dim3(TPB), 0, st, N, nl->cap, nl->nx, nl->ny, nl->nz, \
+                       nl->sorted.p, nl->cell_offsets.p, nl->cell_of_particle.p, geom, nl->rl2,            \
+                       nl->neighbors_flat.p, nl->counts.p, nl->diagnostics ? nl->candidates.p : nullptr, s);
     switch (G) {
       case 1: PDX_ROWS(1) break;
       case 2: PDX_ROWS(2) break;
@@ -778,11 +830,15 @@ extern "C" int pdx_nl_create(int32_t n, const double box[3], double cutoff, doub
   }
   nl->nx = g[0]; nl->ny = g[1]; nl->nz = g[2];
   nl->ncell = g[0] * g[1] * g[2];
-  // lanes per particle of the row build: fill ~2^19 resident threads (256 CUs x 2048), at
-  // most 32; PDX_ROW_LANES overrides (tuning)
+  // lanes per particle of the row build: enough to fill ~2^19 resident threads (256 CUs x 2048),
+  // and at least one lane per ~256 candidates (27 cells of the mean occupancy): measured at 486 400
+  // ions (1 700 candidates per particle), 8 lanes build in 1.12 ms against 1.65 ms for 2; at most
+  // 32; PDX_ROW_LANES overrides (tuning)
   {
     int G = 1;
     while ((int64_t)n * G < 524288 && G < 32) G <<= 1;
+    const double cand = 27.0 * (double)n / (double)std::max(1, nl->ncell);
+    while (G < 32 && cand > 256.0 * G) G <<= 1;
     if (const char *v = std::getenv("PDX_ROW_LANES")) {
       const int e = std::atoi(v);
       if (e == 1 || e == 2 || e == 4 || e == 8 || e == 16 || e == 32) G = e;

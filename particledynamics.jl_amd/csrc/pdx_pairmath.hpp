@@ -44,9 +44,12 @@ __device__ __forceinline__ void pair_terms_c(const A &a, const LJCoef &c, const 
                                              float &u) {
   f = 0.f;
   u = 0.f;
+  // with Coulomb, 1/r^2 is the square of the 1/r the Coulomb term needs (one quarter-rate
+  // transcendental per pair instead of two)
+  const float rinv = COUL ? __builtin_amdgcn_rsqf(r2) : 0.f;
   if (lj_form(MODEL)) {
     const bool in = (r2 > 0.f) & (r2 < c.rc2);
-    const float invr2 = __builtin_amdgcn_rcpf(r2);
+    const float invr2 = COUL ? rinv * rinv : __builtin_amdgcn_rcpf(r2);
     const float s2 = c.sig2 * invr2;
     const float s6 = s2 * s2 * s2;
     const float s12 = s6 * s6;
@@ -67,7 +70,6 @@ __device__ __forceinline__ void pair_terms_c(const A &a, const LJCoef &c, const 
   if (COUL) {
     const bool inr = (r2 > 0.f) & (r2 < a.crc2);
     const float x = fmaf(r2, a.csx, -1.0f);
-    const float rinv = __builtin_amdgcn_rsqf(r2);
     const float rinv3 = rinv * rinv * rinv;
     const float cq = a.cC * qi * qj;
     const float fc = cq * (rinv3 - horner<NP>(pf, x));

@@ -755,11 +755,15 @@ void launch_typed(bool coul, bool energy, int np, int G, dim3 grid, hipStream_t 
   }
 }
 
-// Coulomb polynomials are zero-padded to 16 or 24 monomials (Horner over exact zeros).
+// Coulomb polynomials are zero-padded to 14, 16 or 24 monomials (Horner over exact zeros); the
+// 9-10 A real-space fits at 1e-5 take 13-14.
 template <int MODEL>
 void launch_model(bool coul, bool energy, int np, int G, dim3 grid, hipStream_t st, const PairArgs &a) {
   if (coul) {
-    if (np <= 16) {
+    if (np <= 14) {
+      if (energy) launch_g<MODEL, true, true, 14>(G, grid, st, a);
+      else launch_g<MODEL, true, false, 14>(G, grid, st, a);
+    } else if (np <= 16) {
       if (energy) launch_g<MODEL, true, true, 16>(G, grid, st, a);
       else launch_g<MODEL, true, false, 16>(G, grid, st, a);
     } else {

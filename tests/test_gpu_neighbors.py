@@ -179,3 +179,38 @@ def test_rows_and_forces_2d(cuda):
     with pytest.raises(pdx.ArgumentError):
         pdx.ForceEngine(n, (L, L), None, electrostatics=pdx.PME(coulomb_prefactor=1.0, real_cutoff=2.5, alpha=1.0,
                                                                mesh=(32, 32, 32), order=4), charges=np.zeros(n))
+
+
+@pytest.mark.parametrize("case", ["fcc4000", "rocksalt64k"])
+def test_rows_staged_equal_direct_and_lanes(cuda, case, monkeypatch):
+    """The row build staged in LDS (the default) writes the same rows, slot for slot, as the build
+    that stores straight from the search (PDX_ROW_STAGE=0), for every builder-lane count (rows are
+    identical across builder lanes, src/engine/StateEngine.jl:15-19), in storage and spatial order."""
+    import torch
+    if case == "fcc4000":
+        s = systems.fcc_lattice(4000, 0.8442, 0.05, seed=3)
+        cutoff, skin = 2.5, 0.4
+    else:
+        s = systems.rock_salt(40, 2.82, 0.1, seed=0)
+        cutoff, skin = 9.0, 1.0
+    N, box = len(s["x"]), s["box"]
+    cap = pdx.default_capacity(N, box, cutoff + skin)
+    X, Y, Z = dev(s["x"]), dev(s["y"]), dev(s["z"])
+
+    def rows(stage, lanes=None):
+        monkeypatch.setenv("PDX_ROW_STAGE", "1" if stage else "0")
+        ws = pdx.DeviceNeighborWorkspace(N, box, cutoff, skin, cap)
+        if lanes:
+            ws.set_row_lanes(lanes)
+        ws.update(X, Y, Z, None)
+        st = ws.status()
+        cnt = ws.counts().clone()
+        flat = ws.neighbors_flat().view(cap, N).clone()
+        mask = torch.arange(cap, device="cuda")[:, None] < cnt[None, :]
+        return cnt, torch.where(mask, flat, torch.zeros_like(flat)), st.builds
+
+    c0, r0, b0 = rows(False)
+    assert b0 == 1
+    for stage, lanes in ((True, None), (True, 1), (True, 2), (True, 32), (False, 8)):
+        c1, r1, _ = rows(stage, lanes)
+        assert torch.equal(c0, c1) and torch.equal(r0, r1), (stage, lanes)
