@@ -385,14 +385,34 @@ __global__ __launch_bounds__(TPB) void k_mlp_grad(MLPDev m, const float *__restr
   if (lane == 0) P[NPARAM] = (float)loss;
 }
 
-// Fixed-order reduction of the per-wave partials (one thread per parameter) -> grad, loss.
-__global__ void k_mlp_reduce(const float *partial, int nwave, float *grad, double *loss, int n) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k > NPARAM) return;
+// Fixed-order reduction of the per-wave partials -> grad, loss. A block takes RED_P consecutive
+// parameters (coalesced across lanes) and RED_S slices of the waves: slice s sums waves s, s +
+// RED_S, ... in FP64, and the slices are added in slice order, so the result depends only on nwave.
+constexpr int RED_P = 64, RED_S = 16;
+__global__ __launch_bounds__(RED_P *RED_S) void k_mlp_reduce(const float *partial, int nwave, float *grad,
+                                                               double *loss, int n) {
+  __shared__ double acc[RED_S][RED_P];
+  const int j = threadIdx.x % RED_P, sl = threadIdx.x / RED_P;
+  const int k = blockIdx.x * RED_P + j;
   double s = 0.0;
-  for (int w = 0; w < nwave; ++w) s += partial[(size_t)w * (NPARAM + 1) + k];
-  if (k < NPARAM) grad[k] = (float)s;
-  else if (loss) *loss = s / (double)(n * NOUT);
+  if (k <= NPARAM) {
+    int w = sl;
+    for (; w + 3 * RED_S < nwave; w += 4 * RED_S) {
+      const float a = partial[(size_t)w * (NPARAM + 1) + k], b = partial[(size_t)(w + RED_S) * (NPARAM + 1) + k],
+                  c = partial[(size_t)(w + 2 * RED_S) * (NPARAM + 1) + k],
+                  d = partial[(size_t)(w + 3 * RED_S) * (NPARAM + 1) + k];
+      s += a; s += b; s += c; s += d;
+    }
+    for (; w < nwave; w += RED_S) s += partial[(size_t)w * (NPARAM + 1) + k];
+  }
+  acc[sl][j] = s;
+  __syncthreads();
+  if (sl == 0 && k <= NPARAM) {
+    double t = acc[0][j];
+    for (int q = 1; q < RED_S; ++q) t += acc[q][j];
+    if (k < NPARAM) grad[k] = (float)t;
+    else if (loss) *loss = t / (double)(n * NOUT);
+  }
 }
 
 // Adam on the FP32 master parameters, then the BF16 / padded device copies.
@@ -557,7 +577,7 @@ extern "C" int pdx_mlp_gradient(pdx_mlp_t m, const float *feats, const float *ta
     if (rc) return rc;
   }
   hipLaunchKernelGGL(k_mlp_grad, dim3(blocks), dim3(TPB), 0, st, dev_of(m), feats, targets, rows, n, m->partial.p);
-  hipLaunchKernelGGL(k_mlp_reduce, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, st, m->partial.p, nwave,
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((NPARAM + 1 + RED_P - 1) / RED_P), dim3(RED_P * RED_S), 0, st, m->partial.p, nwave,
                      grad ? grad : m->grad.p, loss ? loss : m->loss.p, n);
   PDX_LAUNCH_CHECK();
   return PDX_OK;
