@@ -408,7 +408,72 @@ def headline_leg(nsites, steps, warmup, skin, quick=False, reorder=True):
                                         for k, v in TUNED.items() if k[0] == N}},
             "roofline": roofline}
     del eng
+    plain_spec = _spec(chosen["rc"], chosen["order"], chosen["mesh"]) if "rc" in chosen else spec
+    line["comparison"]["frontier"] = frontier(P, box, skin, trials, corrected, coarse_only, quick)
+    line["storage_order"] = storage_order_leg(nsites, skin, plain_spec, line["ms_per_step"] if reorder else None)
     return {"line": line, "sysd": sysd, "plain_spec": (best or chosen)}
+
+
+def frontier(P, box, skin, trials, corrected, coarse_only, quick=False):
+    """The measured error / speed front of plain PME around the corrected variant: for each error
+    level (the corrected variant's error, 1e-3, 3e-4), the smallest mesh of every searched cutoff
+    and order that reaches it, timed with the measured launch table; then the fastest plain
+    configuration at the corrected variant's error, against which the corrected variant is judged."""
+    levels = sorted({corrected["err"], 1e-3, 3e-4}, reverse=True)
+    picked = {}
+    for lev in levels:
+        for rc, order in sorted({(t["rc"], t["order"]) for t in trials}):
+            ok = [t for t in trials if t["rc"] == rc and t["order"] == order and t["err"] <= lev]
+            if ok:
+                t = min(ok, key=lambda r: r["mesh"])
+                picked[(rc, order, t["mesh"])] = t
+        if quick:
+            break
+    points = []
+    for (rc, order, K), t in sorted(picked.items()):
+        ms = t.get("ms_per_eval")
+        if ms is None:
+            ms = round(_time_eval(P, box, _spec(rc, order, K), skin, 20, 3, settings="auto"), 4)
+        points.append({"rc": rc, "order": order, "mesh": K, "err": t["err"], "ms_per_eval": ms})
+        _log(f"frontier: rc {rc} order {order} mesh {K}: error {t['err']:.3e}, {ms:.4f} ms")
+    points.sort(key=lambda r: r["err"])
+    pareto, best_ms = [], math.inf
+    for r in sorted(points, key=lambda r: r["err"]):  # non-dominated: faster than every more accurate point
+        if r["ms_per_eval"] < best_ms:
+            pareto.append(r)
+            best_ms = r["ms_per_eval"]
+    at = [r for r in points if r["err"] <= corrected["err"]]
+    eq = min(at, key=lambda r: r["ms_per_eval"]) if at else None
+    return {"plain": points, "plain_pareto": pareto, "corrected": {k: corrected[k] for k in ("mesh", "order", "rc",
+                                                                                               "err", "ms_per_eval")},
+            "coarse_only": coarse_only, "plain_at_corrected_error": eq,
+            "speedup_corrected_vs_plain_at_its_error": (round(eq["ms_per_eval"] / corrected["ms_per_eval"], 3)
+                                                         if eq else None),
+            "note": "plain timings use the measured launch table (settings 'auto'); the search's target "
+                    "configurations keep their own tuned times"}
+
+
+def storage_order_leg(nsites, skin, spec, ms_spatial):
+    """The chosen plain evaluation on the same configuration stored three ways: lattice order, a
+    random permutation, and that permutation put back into spatial order by ForceEngine.reorder
+    (label-independent, so it lands on the same order as the headline's)."""
+    import numpy as np
+    lat = _ion_system(nsites, 10)
+    box = lat["box"]
+    perm = np.random.default_rng(5).permutation(lat["N"])
+    shuf = dict(lat)
+    for k in ("x", "y", "z", "q", "mass"):
+        shuf[k] = np.ascontiguousarray(np.asarray(lat[k])[perm])
+    out = {"spatial (headline)": ms_spatial}
+    for name, sysd in (("lattice", lat), ("shuffled", shuf), ("shuffled, then reordered", None)):
+        if sysd is None:
+            sysd = spatially_ordered(shuf, skin)
+        ms = _time_eval(_tensors(sysd), box, spec, skin, 20, 3, settings="auto")
+        out[name] = round(ms, 4)
+        _log(f"storage order {name}: {ms:.4f} ms")
+    out["note"] = ("ms per evaluation of the plain configuration, graph replays, measured launch table; "
+                   "'shuffled' is a random relabelling of the same ions")
+    return out
 
 
 def md_leg(eng, sysd, P, bufs, steps, warmup):

@@ -721,8 +721,12 @@ __global__ __launch_bounds__(BS) void k_plane_inv(const float2 *__restrict__ spe
 
 // ---- B-spline weights (window_stencil, src/electrostatics/Windows.jl:209-223): order-P
 // cardinal B-spline weights w[j] of nodes k0 + j (ascending) and derivatives dw[j].
+// Every kernel that spreads or gathers must get bitwise the same weights (the tiled spread equals
+// the atomic one bit for bit), so no product is left for the compiler to fuse or not as the
+// surrounding code suggests: contraction is off and the fused steps are written as fmaf.
 template <int P>
 __device__ __forceinline__ int bspline(float u, float (&w)[P], float (&dw)[P]) {
+#pragma clang fp contract(off)
   const float fl = floorf(u);
   const float t = u - fl;
   float a[P];  // a[j] = M_n(t + j), built up from M_2
@@ -737,7 +741,7 @@ __device__ __forceinline__ int bspline(float u, float (&w)[P], float (&dw)[P]) {
     for (int j = n; j >= 0; --j) {
       const float left = j < n ? a[j] : 0.f;
       const float right = j >= 1 ? a[j - 1] : 0.f;
-      a[j] = ((t + j) * left + ((float)(n + 1) - t - j) * right) * inv;
+      a[j] = fmaf(t + j, left, ((float)(n + 1) - t - j) * right) * inv;
     }
   }
   // a = M_{P-1}(t + j), j = 0..P-2; final step to M_P and derivatives
@@ -747,7 +751,7 @@ __device__ __forceinline__ int bspline(float u, float (&w)[P], float (&dw)[P]) {
   for (int j = 0; j < P; ++j) {
     const float left = j < P - 1 ? a[j] : 0.f;
     const float right = j >= 1 ? a[j - 1] : 0.f;
-    full[j] = ((t + j) * left + ((float)P - t - j) * right) * inv;
+    full[j] = fmaf(t + j, left, ((float)P - t - j) * right) * inv;
   }
 #pragma unroll
   for (int i = 0; i < P; ++i) {
@@ -783,8 +787,10 @@ __device__ __forceinline__ int window_base(float u) {
 // 158-166,231-250): tau = offset of u from the centre of its cell; stored node h carries
 // E_h(t) + tau O_h(t) (t = tau^2), its mirror node P-1-h carries E_h(t) - tau O_h(t); the
 // u-derivatives tau Ed_h(t) +- Od_h(t) are the exact derivatives of the same polynomials.
+// Contraction is off and every fused step is an explicit fmaf, as in bspline above.
 template <int P>
 __device__ __forceinline__ int prolate_weights(float u, float (&w)[P], float (&dw)[P], const WinCoef &c) {
+#pragma clang fp contract(off)
   constexpr int H = (P + 1) / 2;
   float tau;
   int k0;
@@ -809,12 +815,11 @@ __device__ __forceinline__ int prolate_weights(float u, float (&w)[P], float (&d
       Ed = fmaf(Ed, t, c.ed[j][h]);
       Od = fmaf(Od, t, c.od[j][h]);
     }
-    const float to = tau * O, ted = tau * Ed;
-    w[h] = E + to;
-    dw[h] = ted + Od;
+    w[h] = fmaf(tau, O, E);
+    dw[h] = fmaf(tau, Ed, Od);
     if (P - 1 - h != h) {
-      w[P - 1 - h] = E - to;
-      dw[P - 1 - h] = ted - Od;
+      w[P - 1 - h] = fmaf(-tau, O, E);
+      dw[P - 1 - h] = fmaf(tau, Ed, -Od);
     }
   }
   return k0;

@@ -6,7 +6,11 @@
     FP64 oracle;
   * the 512-box replica batch, 1000 velocity-Verlet steps in one launch: boxes run alone
     (the largest ones included) are bitwise equal to their batch members, and the replica with
-    the largest energy drift follows the FP64 Verlet oracle."""
+    the largest energy drift follows the FP64 Verlet oracle;
+  * the 486 400-ion learned-correction workload (80^3 rock-salt sites, 5 % vacated): plain PME at
+    its tuned setting and the 480^3 training target within 1e-4 of FP64 Ewald on sampled ions,
+    and the correction trained on one configuration lowering the 120^3 coarse-mesh error of a
+    held-out one."""
 import math
 import time
 
@@ -161,3 +165,42 @@ def test_replica_batch_full_size(cuda):
     print(f"[replicas full] worst drift box {k}: {s['model']} n={s['n']} rho={s['density']:.3f} "
           f"max|dx|={np.abs(d).max():.2e} max|dv|={np.abs(dv).max():.2e}", flush=True)
     assert np.abs(d).max() < 1e-3 and np.abs(dv).max() < 1e-2
+
+
+def test_learned_correction_full_size(cuda):
+    """C5 at 486 400 ions. Errors are relative RMS Coulomb forces against the device FP64 Ewald
+    sums (alpha 17 / L, 12 digits) on 256 sampled ions of the held-out configuration."""
+    import torch
+    from pdx import learned
+    mk = lambda rc, order, **kw: pdx.PME(coulomb_prefactor=C_ION, real_cutoff=rc, real_space_tolerance=1e-5,
+                                         order=order, **kw)
+    plain = mk(10.0, 6, mesh=(160, 160, 160))
+    coarse, fine = mk(9.0, 4, mesh_spacing=2.0), mk(9.0, 6, mesh_spacing=0.5)
+    train = systems.rock_salt(80, 2.82, 0.1, seed=11, vacancy_fraction=0.05)
+    held = systems.rock_salt(80, 2.82, 0.1, seed=10, vacancy_fraction=0.05)
+    N, box = held["N"], held["box"]
+    assert N == 486400
+    assert coarse.resolve_mesh(box)[0] == 120 and fine.resolve_mesh(box)[0] == 480
+    ids = torch.from_numpy(np.sort(np.random.default_rng(1).choice(N, 256, replace=False)).astype(np.int32)).cuda()
+    P = [torch.from_numpy(held[k]).cuda().double() for k in ("x", "y", "z", "q")]
+    Fref, _ = pdx.ewald_fp64(*P, box, C_ION, 17.0 / min(box), digits=12.0, sample=ids, want_energy=False)
+    Fref = Fref.cpu().numpy()
+
+    def err(F):
+        return _relrms(F[ids.long()].double().cpu().numpy(), Fref)
+
+    e_plain = err(learned.coulomb_forces(plain, held)[0])
+    e_fine = err(learned.coulomb_forces(fine, held)[0])
+    F, T = learned.residual_dataset([train], coarse, fine)
+    m = learned.MLP(seed=0)
+    m.set_norm(*learned.fit_standardisation(F, T))
+    learned.train(m, F, T, steps=4000, batch=8192, lr=2e-3)
+    del F, T
+    Fc, ec, (X, Y, Z, Q) = learned.coulomb_forces(coarse, held)
+    corr = torch.empty_like(Fc)
+    m.forward(learned.mesh_features(ec.solver, X, Y, Z, Q), corr)
+    e_coarse, e_corr = err(Fc), err(Fc + corr)
+    print(f"[c5] plain 160^3 P6 {e_plain:.3e}, fine 480^3 {e_fine:.3e}, coarse 120^3 {e_coarse:.3e} -> "
+          f"corrected {e_corr:.3e}", flush=True)
+    assert e_plain < 1e-4 and e_fine < 1e-4
+    assert e_corr < 0.5 * e_coarse
