@@ -537,6 +537,17 @@ int pdx_mlp_set_norm(pdx_mlp_t m, const float *mu, const float *isd, float out_s
  * fy, fz are given the correction is added to them. */
 int pdx_mlp_forward(pdx_mlp_t m, const float *feats, int32_t n, float *out, float *fx, float *fy, float *fz,
                     pdx_stream_t stream);
+/* Inference straight from the mesh (no counterpart in the reference): the features of
+ * pdx_mesh_features are formed from mesh's charge grid of its last spread inside the network's
+ * kernel, one 16-particle tile at a time, so no N x 80 feature rows are written or read. Call
+ * after the spread (the grid holds until the next spread; the transform only reads it). out
+ * (N x 3) and/or fx, fy, fz (added to) as in pdx_mlp_forward; bitwise the same values. n <= the
+ * mesh's particle count. */
+int pdx_mlp_forward_mesh(pdx_mlp_t m, pdx_mesh_t mesh, const float *x, const float *y, const float *z,
+                         const float *q, int32_t n, float *out, float *fx, float *fy, float *fz, pdx_stream_t stream);
+/* fx, fy, fz += the columns of out (N x 3): a correction computed ahead on another stream, added
+ * once the forces are complete. */
+int pdx_mlp_add(const float *out, int32_t n, float *fx, float *fy, float *fz, pdx_stream_t stream);
 /* Gradient of loss = mean (MLP(x') - t / out_scale)^2 over the batch rows[0..n) (device ids;
  * NULL: rows 0..n-1) of feats / targets (N x 3): grad (device, nparams) and loss (device
  * double), both may be NULL (internal buffers). Per-wave partials are reduced in a fixed order. */

@@ -64,6 +64,25 @@ constexpr int WAVE = 64;
 
 // ---------------------------------------------------------------- device helpers
 
+// Fractional grid coordinate in [0, K) (_grid_coordinate, src/electrostatics/MeshKernels.jl:9-15).
+__device__ __forceinline__ float grid_coordinate(float x, float L, int K) {
+  float s = x / L;
+  s -= floorf(s);
+  float u = s * (float)K;
+  if (!isfinite(u)) return 0.f;
+  return u >= (float)K ? u - (float)K : u;
+}
+// A 64-bit fixed-point grid value as FP32 times inv (a power of two): the high and low words
+// converted separately and joined by one fused step, the feature definition of the learned
+// correction (the same arithmetic in every kernel that forms features).
+__device__ __forceinline__ float fixed_to_float(unsigned long long v, float inv) {
+  const int hi = (int)((long long)v >> 32);
+  const unsigned lo = (unsigned)v;
+  return fmaf((float)hi, 4294967296.0f, (float)lo) * inv;
+}
+// Mesh node index n wrapped into [0, K) for n in [-K, 2K).
+__device__ __forceinline__ int wrap_node(int n, int K) { return n < 0 ? n + K : (n >= K ? n - K : n); }
+
 // Minimum image for coordinates in one periodic image of [-L/2, L/2)
 // (mic_fast, src/neighbors/NeighborTypes.jl:65-69): branch-free select.
 __device__ __forceinline__ float mic_fast(float dx, float halfL, float L) {

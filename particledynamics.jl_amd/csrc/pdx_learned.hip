@@ -27,10 +27,8 @@ constexpr int OW1 = 0, OB1 = OW1 + NIN * NH, OW2 = OB1 + NH, OB2 = OW2 + NH * NH
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ short f2bf(float f) {  // round to nearest even
-  unsigned u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
+__device__ __forceinline__ short f2bf(float f) {  // round to nearest even (v_cvt_pk_bf16_f32)
+  return __builtin_bit_cast(short, (__bf16)f);
 }
 __device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((unsigned)(unsigned short)b) << 16); }
 
@@ -43,9 +41,10 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU with the hardware reciprocal (1 ulp): the activations are rounded to BF16 next
+__device__ __forceinline__ float silu(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ float dsilu(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-z));
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -78,14 +77,43 @@ __device__ void load_weights(WLds &L, const MLPDev &m) {
   for (int k = threadIdx.x; k < NIN; k += blockDim.x) { L.mu[k] = m.mu[k]; L.isd[k] = m.isd[k]; }
 }
 
+// Inference weights in LDS in MFMA B-fragment order: fragment f of a [K][N] matrix is its 16 x 16
+// block (k0, j0) = (16 (f / (N/16)), 16 (f % (N/16))), and lane l keeps that block's
+// W[k0 + 4 (l/16) + e][j0 + l % 16], e = 0..3, as one 8-byte vector: one LDS read per MFMA operand.
+struct __align__(16) WFrag {
+  bf16x4 w1[(NIN / 16) * (NH / 16) * WAVE], w2[(NH / 16) * (NH / 16) * WAVE], w3[(NH / 16) * (NOP / 16) * WAVE];
+  float b1[NH], b2[NH], b3[NOP], mu[NIN], isd[NIN];
+};
+
+__device__ void load_weights(WFrag &L, const MLPDev &m) {
+  auto put = [&](bf16x4 *dst, const short *src, int K, int N) {
+    const int nf = (K / 16) * (N / 16);
+    for (int q = threadIdx.x; q < nf * WAVE; q += blockDim.x) {
+      const int f = q / WAVE, l = q % WAVE;
+      const int k0 = 16 * (f / (N / 16)) + 4 * (l / 16), j = 16 * (f % (N / 16)) + l % 16;
+      bf16x4 v;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[e] = src[(k0 + e) * N + j];
+      dst[q] = v;
+    }
+  };
+  put(L.w1, m.w1, NIN, NH);
+  put(L.w2, m.w2, NH, NH);
+  put(L.w3, m.w3, NH, NOP);
+  for (int k = threadIdx.x; k < NH; k += blockDim.x) { L.b1[k] = m.b1[k]; L.b2[k] = m.b2[k]; }
+  for (int k = threadIdx.x; k < NOP; k += blockDim.x) L.b3[k] = m.b3[k];
+  for (int k = threadIdx.x; k < NIN; k += blockDim.x) { L.mu[k] = m.mu[k]; L.isd[k] = m.isd[k]; }
+}
+
 // Standardised input tile (16 particles x NIN) into the wave's LDS scratch as BF16 [i][k] (ld NIN).
 // Each lane loads its float4 pieces of the tile first (all loads in flight at once), then
 // standardises and converts them.
 constexpr int NV4 = NIN / 4;              // float4 per feature row
 constexpr int XPER = 16 * NV4 / WAVE;     // float4 per lane per tile
 static_assert((16 * NV4) % WAVE == 0, "tile loads must divide evenly over the wave");
+template <class W>
 __device__ __forceinline__ void stage_inputs(short *xs, const float *__restrict__ feats, const int32_t *rows,
-                                             int base, int n, const WLds &L, int lane) {
+                                             int base, int n, const W &L, int lane) {
   float4 v[XPER];
 #pragma unroll
   for (int u = 0; u < XPER; ++u) {
@@ -108,6 +136,73 @@ __device__ __forceinline__ void stage_inputs(short *xs, const float *__restrict_
     d[3] = f2bf(in ? (v[u].w - L.mu[k + 3]) * L.isd[k + 3] : 0.f);
   }
 }
+
+// The same standardised tile formed straight from the charge grid of the mesh's last spread, with
+// the feature definition and arithmetic of the mesh feature pass (pdx_mesh_features); all loads
+// are issued before the first conversion waits on them.
+struct MeshFeat {
+  const unsigned long long *grid;
+  const float *x, *y, *z, *q;
+  int K1, K2, K3;
+  float Lx, Ly, Lz;
+  float inv_scale;
+};
+// Lane l of a wave staging a 16-particle tile takes particle l / 4 and the grid x offset l % 4
+// (features k = l % 4 + 4 s): four adjacent lanes read four adjacent nodes of one x row.
+__device__ __forceinline__ int feat_particle(int lane) { return lane >> 2; }
+__device__ __forceinline__ int feat_xoff(int lane) { return lane & 3; }
+// One particle's position and charge as loaded (lane l holds particle base + l / 4).
+struct PosQ {
+  float x, y, z, q;
+};
+__device__ __forceinline__ PosQ load_posq(const MeshFeat &g, int base, int n, int lane) {
+  const int p = base + feat_particle(lane);
+  PosQ r{0.f, 0.f, 0.f, 0.f};
+  if (p < n) r = PosQ{g.x[p], g.y[p], g.z[p], g.q[p]};
+  return r;
+}
+// Issue the lane's 16 grid loads (feature k = s0 + 4 s: x offset s0, y offset s & 3, z offset s >> 2).
+__device__ __forceinline__ void load_grid(unsigned long long (&gv)[16], const MeshFeat &g, const PosQ &pq, bool in,
+                                          int s0) {
+  const float f1 = floorf(grid_coordinate(pq.x, g.Lx, g.K1)), f2 = floorf(grid_coordinate(pq.y, g.Ly, g.K2)),
+              f3 = floorf(grid_coordinate(pq.z, g.Lz, g.K3));
+  const int n1 = wrap_node((int)f1 - 3 + s0, g.K1);
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int n2 = wrap_node((int)f2 - 3 + (s & 3), g.K2), n3 = wrap_node((int)f3 - 3 + (s >> 2), g.K3);
+    gv[s] = in ? g.grid[(size_t)n3 * g.K1 * g.K2 + (size_t)n2 * g.K1 + n1] : 0ull;
+  }
+}
+// The standardised BF16 features of the lane's particle into the tile, from its loads.
+template <class W>
+__device__ __forceinline__ void stage_features(short *xs, const MeshFeat &g, const PosQ &pq,
+                                               const unsigned long long (&gv)[16], bool in, const W &L,
+                                               int lane) {
+  const int i = feat_particle(lane), s0 = feat_xoff(lane);
+  const float qi = pq.q;
+  const float u1 = grid_coordinate(pq.x, g.Lx, g.K1), u2 = grid_coordinate(pq.y, g.Ly, g.K2),
+              u3 = grid_coordinate(pq.z, g.Lz, g.K3);
+  const float f1 = floorf(u1), f2 = floorf(u2), f3 = floorf(u3);
+  short *d = xs + i * NIN;
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int k = s0 + 4 * s;
+    const float f = qi * fixed_to_float(gv[s], g.inv_scale);
+    d[k] = f2bf(in ? (f - L.mu[k]) * L.isd[k] : 0.f);
+  }
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {  // k = 64 + s0 + 4 s: q (u - floor u) per dimension, q, q^2, zeros
+    const int e = s0 + 4 * s, k = 64 + e;
+    float f = 0.f;
+    f = e == 0 ? qi * (u1 - f1) : f;
+    f = e == 1 ? qi * (u2 - f2) : f;
+    f = e == 2 ? qi * (u3 - f3) : f;
+    f = e == 3 ? qi : f;
+    f = e == 4 ? qi * qi : f;
+    d[k] = f2bf(in ? (f - L.mu[k]) * L.isd[k] : 0.f);
+  }
+}
+static_assert(NIN == 80, "the mesh staging forms 64 grid features and 16 others");
 
 // A operand (rows i, columns k0..k0+15) of a [16][ld] BF16 LDS tile.
 __device__ __forceinline__ bf16x4 a_from(const short *t, int k0, int lane, int ld = NH) {
@@ -134,44 +229,67 @@ __device__ __forceinline__ bf16x4 bt_from(const short *w, int ld, int k0, int j0
   return b;
 }
 
+// B operands of the three layers (rows k0..k0+15, columns 16 nt..): from the row-major weights of
+// the training kernels, or from the fragment-ordered inference weights.
+__device__ __forceinline__ bf16x4 w1_op(const WLds &L, int k0, int nt, int lane) { return b_from(L.w1, NH, k0, nt * 16, lane); }
+__device__ __forceinline__ bf16x4 w2_op(const WLds &L, int k0, int nt, int lane) { return b_from(L.w2, NH, k0, nt * 16, lane); }
+__device__ __forceinline__ bf16x4 w3_op(const WLds &L, int k0, int lane) { return b_from(L.w3, NOP, k0, 0, lane); }
+__device__ __forceinline__ bf16x4 w1_op(const WFrag &L, int k0, int nt, int lane) {
+  return L.w1[((k0 / 16) * (NH / 16) + nt) * WAVE + lane];
+}
+__device__ __forceinline__ bf16x4 w2_op(const WFrag &L, int k0, int nt, int lane) {
+  return L.w2[((k0 / 16) * (NH / 16) + nt) * WAVE + lane];
+}
+__device__ __forceinline__ bf16x4 w3_op(const WFrag &L, int k0, int lane) { return L.w3[(k0 / 16) * WAVE + lane]; }
+
 // Forward of one 16-particle tile: returns Z1, Z2 (pre-activations, C layout, 4 N-tiles) and
 // the output (one tile, columns 0..2 valid); H1/H2 are left in the wave's LDS tiles.
-__device__ __forceinline__ void forward_tile(const WLds &L, short *xs, short *h1s, short *h2s, int lane,
+template <class W>
+__device__ __forceinline__ void forward_tile(const W &L, short *xs, short *h1s, short *h2s, int lane,
                                              f32x4 (&z1)[4], f32x4 (&z2)[4], f32x4 &y) {
-  // layer 1: [16 x NIN] x [NIN x NH]
+  // layer 1: [16 x NIN] x [NIN x NH]; the four column tiles' accumulation chains advance together
+  // (independent MFMAs back to back), each summed over k in ascending order
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) z1[nt] = f32x4{0, 0, 0, 0};
+#pragma unroll
+  for (int k0 = 0; k0 < NIN; k0 += 16) {
+    const bf16x4 a = a_from(xs, k0, lane, NIN);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) z1[nt] = mfma(a, w1_op(L, k0, nt, lane), z1[nt]);
+  }
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
-    f32x4 c = {0, 0, 0, 0};
-#pragma unroll
-    for (int k0 = 0; k0 < NIN; k0 += 16) c = mfma(a_from(xs, k0, lane, NIN), b_from(L.w1, NH, k0, nt * 16, lane), c);
     const int j = nt * 16 + lane % 16;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      c[e] += L.b1[j];
-      h1s[(4 * (lane / 16) + e) * NH + j] = f2bf(silu(c[e]));
+      z1[nt][e] += L.b1[j];
+      h1s[(4 * (lane / 16) + e) * NH + j] = f2bf(silu(z1[nt][e]));
     }
-    z1[nt] = c;
   }
   wave_sync();
   // layer 2: [16 x NH] x [NH x NH]
 #pragma unroll
-  for (int nt = 0; nt < 4; ++nt) {
-    f32x4 c = {0, 0, 0, 0};
+  for (int nt = 0; nt < 4; ++nt) z2[nt] = f32x4{0, 0, 0, 0};
 #pragma unroll
-    for (int k0 = 0; k0 < NH; k0 += 16) c = mfma(a_from(h1s, k0, lane), b_from(L.w2, NH, k0, nt * 16, lane), c);
+  for (int k0 = 0; k0 < NH; k0 += 16) {
+    const bf16x4 a = a_from(h1s, k0, lane);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) z2[nt] = mfma(a, w2_op(L, k0, nt, lane), z2[nt]);
+  }
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
     const int j = nt * 16 + lane % 16;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      c[e] += L.b2[j];
-      h2s[(4 * (lane / 16) + e) * NH + j] = f2bf(silu(c[e]));
+      z2[nt][e] += L.b2[j];
+      h2s[(4 * (lane / 16) + e) * NH + j] = f2bf(silu(z2[nt][e]));
     }
-    z2[nt] = c;
   }
   wave_sync();
   // layer 3: [16 x NH] x [NH x NOP]
   f32x4 c = {0, 0, 0, 0};
 #pragma unroll
-  for (int k0 = 0; k0 < NH; k0 += 16) c = mfma(a_from(h2s, k0, lane), b_from(L.w3, NOP, k0, 0, lane), c);
+  for (int k0 = 0; k0 < NH; k0 += 16) c = mfma(a_from(h2s, k0, lane), w3_op(L, k0, lane), c);
 #pragma unroll
   for (int e = 0; e < 4; ++e) c[e] += L.b3[lane % 16];
   y = c;
@@ -180,7 +298,7 @@ __device__ __forceinline__ void forward_tile(const WLds &L, short *xs, short *h1
 // Inference: out = out_scale * MLP(standardised features); added to fx, fy, fz when given.
 __global__ __launch_bounds__(TPB) void k_mlp_forward(MLPDev m, const float *__restrict__ feats, int n,
                                                       float *out, float *fx, float *fy, float *fz) {
-  __shared__ WLds L;
+  __shared__ WFrag L;
   __shared__ short xtiles[TPB / WAVE][16 * NIN];
   __shared__ short tiles[TPB / WAVE][2][16 * NH];
   load_weights(L, m);
@@ -208,6 +326,55 @@ __global__ __launch_bounds__(TPB) void k_mlp_forward(MLPDev m, const float *__re
     }
     wave_sync();
   }
+}
+
+// Inference from the mesh: the feature tile of each 16-particle tile is formed from the charge
+// grid in the wave's LDS (no feature rows in memory); out (n x 3) and/or fx, fy, fz as in
+// k_mlp_forward.
+__global__ __launch_bounds__(TPB) void k_mlp_forward_mesh(MLPDev m, const MeshFeat g, int n, float *out, float *fx,
+                                                           float *fy, float *fz) {
+  __shared__ WFrag L;
+  __shared__ short xtiles[TPB / WAVE][16 * NIN];
+  __shared__ short tiles[TPB / WAVE][2][16 * NH];
+  load_weights(L, m);
+  __syncthreads();
+  const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+  short *xs = xtiles[w], *h1s = tiles[w][0], *h2s = tiles[w][1];
+  const int ntile = (n + 15) / 16;
+  const int s0 = feat_xoff(lane);
+  for (int t = blockIdx.x * (TPB / WAVE) + w; t < ntile; t += gridDim.x * (TPB / WAVE)) {
+    const int base = t * 16;
+    const bool in = base + feat_particle(lane) < n;
+    const PosQ pc = load_posq(g, base, n, lane);
+    unsigned long long gc[16];
+    load_grid(gc, g, pc, in, s0);
+    stage_features(xs, g, pc, gc, in, L, lane);
+    wave_sync();
+    f32x4 z1[4], z2[4], y;
+    forward_tile(L, xs, h1s, h2s, lane, z1, z2, y);
+    const int j = lane % 16;
+    if (j < NOUT) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int p = base + 4 * (lane / 16) + e;
+        if (p >= n) continue;
+        const float v = m.out_scale * y[e];
+        if (out) out[(size_t)p * NOUT + j] = v;
+        float *f = j == 0 ? fx : (j == 1 ? fy : fz);
+        if (f) f[p] += v;
+      }
+    }
+    wave_sync();
+  }
+}
+
+__global__ __launch_bounds__(TPB) void k_add_correction(int n, const float *__restrict__ out, float *fx, float *fy,
+                                                        float *fz) {
+  const int p = blockIdx.x * TPB + threadIdx.x;
+  if (p >= n) return;
+  fx[p] += out[(size_t)p * NOUT + 0];
+  fy[p] += out[(size_t)p * NOUT + 1];
+  fz[p] += out[(size_t)p * NOUT + 2];
 }
 
 // Training gradients: per wave, forward + backward over its tiles of the batch, weight and
@@ -559,6 +726,38 @@ extern "C" int pdx_mlp_forward(pdx_mlp_t m, const float *feats, int32_t n, float
   const int blocks = std::max(1, std::min((ntile + 3) / 4, 2048));
   hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(TPB), 0, as_stream(stream), dev_of(m), feats, n, out, fx, fy,
                      fz);
+  PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
+extern "C" int pdx_mlp_forward_mesh(pdx_mlp_t m, pdx_mesh_t mesh, const float *x, const float *y, const float *z,
+                                    const float *q, int32_t n, float *out, float *fx, float *fy, float *fz,
+                                    pdx_stream_t stream) {
+  if (!m || !mesh || !x || !y || !z || !q) return fail(PDX_ERR_ARGUMENT, "null argument");
+  if (n <= 0) return PDX_OK;
+  if (n > mesh->N) return fail(PDX_ERR_ARGUMENT, "more particles than the mesh was made for");
+  if ((fx || fy || fz) && !(fx && fy && fz)) return fail(PDX_ERR_ARGUMENT, "give all three force arrays or none");
+  if (!out && !fx) return fail(PDX_ERR_ARGUMENT, "give out, or fx, fy and fz");
+  if (mesh->K1 < 4 || mesh->K2 < 4 || mesh->K3 < 4)
+    return fail(PDX_ERR_ARGUMENT, "features need at least 4 mesh nodes a side");
+  MeshFeat g{mesh->grid64.p, x, y, z, q, mesh->K1, mesh->K2, mesh->K3, (float)mesh->box[0], (float)mesh->box[1],
+             (float)mesh->box[2], (float)mesh->fixed_inv};
+  // one resident round: the kernel fits three 4-wave workgroups per CU (LDS), and each wave loops
+  // over its tiles, so no second round of short-lived waves forms a tail
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int ntile = (n + 15) / 16;
+  const int blocks = std::max(1, std::min((ntile + 3) / 4, 3 * cus));
+  hipLaunchKernelGGL(k_mlp_forward_mesh, dim3(blocks), dim3(TPB), 0, as_stream(stream), dev_of(m), g, n, out, fx, fy,
+                     fz);
+  PDX_LAUNCH_CHECK();
+  return PDX_OK;
+}
+
+extern "C" int pdx_mlp_add(const float *out, int32_t n, float *fx, float *fy, float *fz, pdx_stream_t stream) {
+  if (!out || !fx || !fy || !fz) return fail(PDX_ERR_ARGUMENT, "null argument");
+  if (n <= 0) return PDX_OK;
+  hipLaunchKernelGGL(k_add_correction, dim3((n + TPB - 1) / TPB), dim3(TPB), 0, as_stream(stream), n, out, fx, fy, fz);
   PDX_LAUNCH_CHECK();
   return PDX_OK;
 }

@@ -844,15 +844,6 @@ __device__ __forceinline__ long long fixed_round(float v) {
   return __double_as_longlong((double)v + M) - __double_as_longlong(M);
 }
 
-// Fractional grid coordinate in [0, K) (_grid_coordinate, MeshKernels.jl:9-15).
-__device__ __forceinline__ float grid_coordinate(float x, float L, int K) {
-  float s = x / L;
-  s -= floorf(s);
-  float u = s * (float)K;
-  if (!isfinite(u)) return 0.f;
-  return u >= (float)K ? u - (float)K : u;
-}
-__device__ __forceinline__ int wrap_node(int n, int K) { return n < 0 ? n + K : (n >= K ? n - K : n); }
 
 template <int P>
 __device__ __forceinline__ float sel(const float (&v)[P], int j) {
@@ -1686,7 +1677,7 @@ constexpr int NFEAT = 80;
 __global__ __launch_bounds__(TPB) void k_features(int N, const float *__restrict__ x, const float *__restrict__ y,
                                                   const float *__restrict__ z, const float *__restrict__ q,
                                                   const unsigned long long *__restrict__ grid, int K1, int K2,
-                                                  int K3, float Lx, float Ly, float Lz, double inv_scale,
+                                                  int K3, float Lx, float Ly, float Lz, float inv_scale,
                                                   float *feats) {
   const int i = blockIdx.x * (TPB / WAVE) + threadIdx.x / WAVE;
   const int lane = threadIdx.x % WAVE;
@@ -1699,8 +1690,7 @@ __global__ __launch_bounds__(TPB) void k_features(int N, const float *__restrict
   const int n2 = wrap_node((int)f2 - 3 + ((lane >> 2) & 3), K2);
   const int n3 = wrap_node((int)f3 - 3 + (lane >> 4), K3);
   float *f = feats + (size_t)i * NFEAT;
-  const double v = (double)(long long)grid[(size_t)n3 * K1 * K2 + (size_t)n2 * K1 + n1] * inv_scale;
-  f[lane] = qi * (float)v;
+  f[lane] = qi * fixed_to_float(grid[(size_t)n3 * K1 * K2 + (size_t)n2 * K1 + n1], inv_scale);
   if (lane < NFEAT - 64) {
     float e = 0.f;
     e = lane == 0 ? qi * (u1 - f1) : e;
@@ -2666,7 +2656,7 @@ extern "C" int pdx_mesh_features(pdx_mesh_t m, const float *x, const float *y, c
   if (m->K1 < 4 || m->K2 < 4 || m->K3 < 4) return fail(PDX_ERR_ARGUMENT, "features need at least 4 mesh nodes a side");
   const float L[3] = {(float)m->box[0], (float)m->box[1], (float)m->box[2]};
   hipLaunchKernelGGL(k_features, dim3((m->N + TPB / WAVE - 1) / (TPB / WAVE)), dim3(TPB), 0, as_stream(stream), m->N, x, y, z, q,
-                     m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], m->fixed_inv, feats);
+                     m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], (float)m->fixed_inv, feats);
   PDX_LAUNCH_CHECK();
   return PDX_OK;
 }

@@ -97,15 +97,13 @@ class ForceEngine:
             raise L.ArgumentError("gather must be 'tiles' or 'particles'")
         self._tile_gather = (gather == "tiles" and electrostatics is not None
                              and getattr(electrostatics, "influence", "spme") != "p3m_ik")
-        # learned force correction of a coarse mesh (an MLP of the charge-grid features): the
-        # features are taken on the mesh stream after the spread, the network's output is
-        # added to the forces after the gather; energies and virials are not corrected
+        # learned force correction of a coarse mesh (an MLP of the charge-grid features, formed
+        # from the grid inside the network's kernel): it runs on the main stream once the spread
+        # is done, beside the transforms, and adds its output to the forces; energies and
+        # virials are not corrected
         self.correction = correction
-        if correction is not None:
-            if self.solver is None:
-                raise L.ArgumentError("a learned correction needs a mesh")
-            from .learned import NFEAT
-            self.feats = torch.empty(self.N, NFEAT, dtype=torch.float32, device="cuda")
+        if correction is not None and self.solver is None:
+            raise L.ArgumentError("a learned correction needs a mesh")
         self._graph = None
         self._snap_ready = False
         # launch shapes (pdx.autotune): a setting dict, or "auto" for the measured table's setting
@@ -175,17 +173,20 @@ class ForceEngine:
             self.nl.build(stream=main)
         if overlap and spread_stream is main:
             side.wait_stream(main)
-        if self.correction is not None:
-            from .learned import mesh_features
-            mesh_features(self.solver, x, y, z, q, self.feats, stream=side)
         if overlap and self.overlap_with == "fft":
             spread_done = torch.cuda.Event()
             spread_done.record(side)
             main.wait_event(spread_done)
         self.solver.solve(energy, stream=side)
         self._pairs(x, y, z, q, fx, fy, fz, Epot, V, energy, main)
+        # the main stream has the spread behind it when it ran the spread or waited for it
+        spread_seen = spread_stream is main or (overlap and self.overlap_with == "fft")
+        if self.correction is not None and spread_seen:
+            self.correction.forward_mesh(self.solver, x, y, z, q, fx=fx, fy=fy, fz=fz, stream=main)
         if overlap:
             main.wait_stream(side)
+        if self.correction is not None and overlap and not spread_seen:
+            self.correction.forward_mesh(self.solver, x, y, z, q, fx=fx, fy=fy, fz=fz, stream=main)
         if self._tile_gather and self.cluster is None and self.spread_mode in ("cells", "snapshot"):
             # the owner-computes spread listed the particles per tile: gather them tile by tile
             self.solver.gather_cells(self.nl, fx, fy, fz, Epot, self.Vglobal if energy else None, energy,
@@ -193,8 +194,6 @@ class ForceEngine:
         else:
             self.solver.mesh_gather(x, y, z, q, fx, fy, fz, Epot, self.Vglobal if energy else None, energy,
                                     order=order, set=False, stream=main)
-        if self.correction is not None:
-            self.correction.forward(self.feats, None, fx, fy, fz, stream=main)
 
     def _pairs(self, x, y, z, q, fx, fy, fz, Epot, V, energy, stream):
         tot = self.pair_totals if energy else None

@@ -113,6 +113,34 @@ def test_features_from_charge_grid(cuda):
     assert f1[0, :64].sum() == pytest.approx(0.49, rel=1e-6)
 
 
+def test_forward_from_mesh_equals_feature_rows(cuda):
+    """The network fed straight from the charge grid (features formed inside its kernel) gives
+    bitwise the outputs of the feature pass + the network on the stored rows, including a ragged
+    last tile; the separate add reproduces the in-kernel add."""
+    import torch
+    L = (14.0, 15.0, 16.0)
+    for n in (1000, 37):
+        R, q = systems.random_cloud(n, L, seed=7 + n)
+        s = pdx.ElectrostaticsSolver(pdx.PME(coulomb_prefactor=1.0, real_cutoff=4.0, alpha=0.9, mesh=(20, 22, 24),
+                                             order=4, net_charge="background"), L, q)
+        X, Y, Z, Q = dev(R[:, 0]), dev(R[:, 1]), dev(R[:, 2]), dev(q)
+        s.spread(X, Y, Z, Q)
+        p, mu, isd = _rand_net(3)
+        m = learned.MLP(seed=0)
+        m.set_params(p)
+        m.set_norm(mu, isd, 0.7)
+        a = torch.empty(n, learned.NOUT, device="cuda")
+        m.forward(learned.mesh_features(s, X, Y, Z, Q), a)
+        b = torch.full((n, learned.NOUT), float("nan"), device="cuda")
+        m.forward_mesh(s, X, Y, Z, Q, out=b)
+        assert torch.equal(a, b), f"n {n}"
+        f0 = [torch.rand(n, device="cuda") for _ in range(3)]
+        f1 = [t.clone() for t in f0]
+        m.forward_mesh(s, X, Y, Z, Q, fx=f0[0], fy=f0[1], fz=f0[2])
+        learned.MLP.add(b, *f1)
+        assert all(torch.equal(u, v) for u, v in zip(f0, f1))
+
+
 def test_training_fits_a_known_map(cuda):
     """The device Adam loop on the MFMA kernels drives the loss down on a learnable target."""
     import torch
