@@ -183,9 +183,10 @@ def test_rows_and_forces_2d(cuda):
 
 @pytest.mark.parametrize("case", ["fcc4000", "rocksalt64k"])
 def test_rows_staged_equal_direct_and_lanes(cuda, case, monkeypatch):
-    """The row build staged in LDS (the default) writes the same rows, slot for slot, as the build
-    that stores straight from the search (PDX_ROW_STAGE=0), for every builder-lane count (rows are
-    identical across builder lanes, src/engine/StateEngine.jl:15-19), in storage and spatial order."""
+    """The row build staged in LDS writes the same rows, slot for slot, as the build that stores
+    straight from the search (the default), for every builder-lane count (rows are identical across
+    builder lanes, src/engine/StateEngine.jl:15-19), in storage and spatial order; so does the
+    cell-staged build (PDX_ROW_CELLS=1)."""
     import torch
     if case == "fcc4000":
         s = systems.fcc_lattice(4000, 0.8442, 0.05, seed=3)
@@ -197,8 +198,9 @@ def test_rows_staged_equal_direct_and_lanes(cuda, case, monkeypatch):
     cap = pdx.default_capacity(N, box, cutoff + skin)
     X, Y, Z = dev(s["x"]), dev(s["y"]), dev(s["z"])
 
-    def rows(stage, lanes=None):
+    def rows(stage, lanes=None, cells=False):
         monkeypatch.setenv("PDX_ROW_STAGE", "1" if stage else "0")
+        monkeypatch.setenv("PDX_ROW_CELLS", "1" if cells else "0")
         ws = pdx.DeviceNeighborWorkspace(N, box, cutoff, skin, cap)
         if lanes:
             ws.set_row_lanes(lanes)
@@ -209,8 +211,12 @@ def test_rows_staged_equal_direct_and_lanes(cuda, case, monkeypatch):
         mask = torch.arange(cap, device="cuda")[:, None] < cnt[None, :]
         return cnt, torch.where(mask, flat, torch.zeros_like(flat)), st.builds
 
-    c0, r0, b0 = rows(False)
+    c0, r0, b0 = rows(False)  # k_rows
     assert b0 == 1
     for stage, lanes in ((True, None), (True, 1), (True, 2), (True, 32), (False, 8)):
         c1, r1, _ = rows(stage, lanes)
         assert torch.equal(c0, c1) and torch.equal(r0, r1), (stage, lanes)
+    # the cell-staged build (one workgroup per cell, candidates from LDS)
+    for lanes in (None, 1, 4, 8, 32):
+        c1, r1, _ = rows(False, lanes, cells=True)
+        assert torch.equal(c0, c1) and torch.equal(r0, r1), ("cells", lanes)
