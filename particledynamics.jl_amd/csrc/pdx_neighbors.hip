@@ -321,37 +321,41 @@ __global__ __launch_bounds__(WAVE) void k_sort_cells(int ncell, const int32_t *o
                                                      const int64_t *state) {
   if (!rebuild_requested(state)) return;
   __shared__ int32_t sh[SORT_STAGE];
-  const int c = blockIdx.x;
-  const int s = offsets[c] - 1, e = offsets[c + 1] - 1, n = e - s;
-  if (n <= SORT_STAGE) {
-    for (int k = threadIdx.x; k < n; k += WAVE) sh[k] = ids_sorted[s + k];
-    __syncthreads();
-    for (int k = threadIdx.x; k < n; k += WAVE) {
-      const int32_t key = sh[k];
-      int rank = 0;
-      for (int m = 0; m < n; ++m) rank += sh[m] < key;
-      ids_sorted[s + rank] = key;
-      const float4 p = packed[key - 1];
-      sorted[s + rank] = make_float4(p.x, p.y, p.z, __int_as_float(key - 1));
-    }
-    return;
-  }
-  if (threadIdx.x == 0) {
-    for (int a = s + 1; a < e; ++a) {
-      const int32_t key = ids_sorted[a];
-      int b = a - 1;
-      while (b >= s && ids_sorted[b] > key) {
-        ids_sorted[b + 1] = ids_sorted[b];
-        --b;
+  // grid-stride over the cells (a bounded grid: on a check without rebuild the launch is cheap
+  // however many cells there are)
+  for (int c = blockIdx.x; c < ncell; c += gridDim.x) {
+    const int s = offsets[c] - 1, e = offsets[c + 1] - 1, n = e - s;
+    __syncthreads();  // the previous cell's ranks are read
+    if (n <= SORT_STAGE) {
+      for (int k = threadIdx.x; k < n; k += WAVE) sh[k] = ids_sorted[s + k];
+      __syncthreads();
+      for (int k = threadIdx.x; k < n; k += WAVE) {
+        const int32_t key = sh[k];
+        int rank = 0;
+        for (int m = 0; m < n; ++m) rank += sh[m] < key;
+        ids_sorted[s + rank] = key;
+        const float4 p = packed[key - 1];
+        sorted[s + rank] = make_float4(p.x, p.y, p.z, __int_as_float(key - 1));
       }
-      ids_sorted[b + 1] = key;
+      continue;
     }
-  }
-  __syncthreads();
-  for (int k = s + threadIdx.x; k < e; k += WAVE) {
-    const int i = ids_sorted[k] - 1;
-    const float4 p = packed[i];
-    sorted[k] = make_float4(p.x, p.y, p.z, __int_as_float(i));
+    if (threadIdx.x == 0) {
+      for (int a = s + 1; a < e; ++a) {
+        const int32_t key = ids_sorted[a];
+        int b = a - 1;
+        while (b >= s && ids_sorted[b] > key) {
+          ids_sorted[b + 1] = ids_sorted[b];
+          --b;
+        }
+        ids_sorted[b + 1] = key;
+      }
+    }
+    __syncthreads();
+    for (int k = s + threadIdx.x; k < e; k += WAVE) {
+      const int i = ids_sorted[k] - 1;
+      const float4 p = packed[i];
+      sorted[k] = make_float4(p.x, p.y, p.z, __int_as_float(i));
+    }
   }
 }
 
@@ -477,7 +481,7 @@ __global__ __launch_bounds__(TPB) void k_rows(int N, int cap, int nx, int ny, in
 // accepted candidates in stencil order), whatever G or the chunking.
 constexpr int RC_CHUNK = 2048;
 template <int G>
-__global__ __launch_bounds__(TPB) void k_rows_cells(int N, int cap, int nx, int ny, int nz,
+__global__ __launch_bounds__(TPB) void k_rows_cells(int N, int cap, int ncell, int nx, int ny, int nz,
                                                     const float4 *__restrict__ sorted,
                                                     const int32_t *__restrict__ offsets, const float *geom,
                                                     float rl2, int32_t *nbr, int32_t *counts,
@@ -486,95 +490,97 @@ __global__ __launch_bounds__(TPB) void k_rows_cells(int N, int cap, int nx, int 
   __shared__ float4 cand[RC_CHUNK];
   __shared__ int cstart[27], cpre[28];
   if (!rebuild_requested(state)) return;  // uniform over the grid
-  const int c = blockIdx.x;
-  const int p0 = offsets[c] - 1, np = offsets[c + 1] - offsets[c];
-  if (np <= 0) return;  // uniform over the workgroup
-  const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-  if (threadIdx.x < WAVE) {
-    int cnt = 0;
-    if (threadIdx.x < 27) {
-      const int o = threadIdx.x;
-      int cz2 = cz + o / 9 - 1; cz2 -= (cz2 >= nz) ? nz : 0; cz2 += (cz2 < 0) ? nz : 0;
-      int cy2 = cy + (o / 3) % 3 - 1; cy2 -= (cy2 >= ny) ? ny : 0; cy2 += (cy2 < 0) ? ny : 0;
-      int cx2 = cx + o % 3 - 1; cx2 -= (cx2 >= nx) ? nx : 0; cx2 += (cx2 < 0) ? nx : 0;
-      const int cc = (cz2 * ny + cy2) * nx + cx2;
-      cstart[o] = offsets[cc] - 1;
-      cnt = offsets[cc + 1] - offsets[cc];
-    }
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-      const int u = __shfl_up(cnt, d, WAVE);
-      if ((int)threadIdx.x >= d) cnt += u;
-    }
-    if (threadIdx.x < 27) cpre[threadIdx.x + 1] = cnt;
-    if (threadIdx.x == 0) cpre[0] = 0;
-  }
-  __syncthreads();
-  const int total = cpre[27];
-  const float Lx = geom[0], Ly = geom[1], Lz = geom[2];
-  const float hx = 0.5f * Lx, hy = 0.5f * Ly, hz = 0.5f * Lz;
-  const int lg = threadIdx.x % G, pl = threadIdx.x / G;
-  const int wl = threadIdx.x & (WAVE - 1);
-  const unsigned long long gmask = (G == WAVE) ? ~0ull : (((1ull << G) - 1ull) << (wl - lg));
-  const unsigned long long below = (1ull << wl) - 1ull;
-  for (int pb = 0; pb < np; pb += PB) {
-    const bool active = pb + pl < np;
-    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
-    int i = -1;
-    if (active) {
-      pi = sorted[p0 + pb + pl];
-      i = __float_as_int(pi.w);
-    }
-    int found = 0;
-    int64_t tested = 0;
-    for (int ch = 0; ch < total; ch += RC_CHUNK) {
-      const int nch = min(RC_CHUNK, total - ch);
-      __syncthreads();  // the previous chunk (or pass) is no longer read
-      for (int f = threadIdx.x; f < nch; f += TPB) {
-        const int g = ch + f;
-        int cc = 0;
-#pragma unroll
-        for (int h = 16; h >= 1; h >>= 1)
-          if (cc + h < 27 && cpre[cc + h] <= g) cc += h;
-        cand[f] = sorted[cstart[cc] + (g - cpre[cc])];
+  for (int c = blockIdx.x; c < ncell; c += gridDim.x) {  // grid-stride over the cells
+    const int p0 = offsets[c] - 1, np = offsets[c + 1] - offsets[c];
+    if (np <= 0) continue;  // uniform over the workgroup
+    const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+    __syncthreads();  // the previous cell's stencil and candidates are no longer read
+    if (threadIdx.x < WAVE) {
+      int cnt = 0;
+      if (threadIdx.x < 27) {
+        const int o = threadIdx.x;
+        int cz2 = cz + o / 9 - 1; cz2 -= (cz2 >= nz) ? nz : 0; cz2 += (cz2 < 0) ? nz : 0;
+        int cy2 = cy + (o / 3) % 3 - 1; cy2 -= (cy2 >= ny) ? ny : 0; cy2 += (cy2 < 0) ? ny : 0;
+        int cx2 = cx + o % 3 - 1; cx2 -= (cx2 >= nx) ? nx : 0; cx2 += (cx2 < 0) ? nx : 0;
+        const int cc = (cz2 * ny + cy2) * nx + cx2;
+        cstart[o] = offsets[cc] - 1;
+        cnt = offsets[cc + 1] - offsets[cc];
       }
-      __syncthreads();
+#pragma unroll
+      for (int d = 1; d < WAVE; d <<= 1) {
+        const int u = __shfl_up(cnt, d, WAVE);
+        if ((int)threadIdx.x >= d) cnt += u;
+      }
+      if (threadIdx.x < 27) cpre[threadIdx.x + 1] = cnt;
+      if (threadIdx.x == 0) cpre[0] = 0;
+    }
+    __syncthreads();
+    const int total = cpre[27];
+    const float Lx = geom[0], Ly = geom[1], Lz = geom[2];
+    const float hx = 0.5f * Lx, hy = 0.5f * Ly, hz = 0.5f * Lz;
+    const int lg = threadIdx.x % G, pl = threadIdx.x / G;
+    const int wl = threadIdx.x & (WAVE - 1);
+    const unsigned long long gmask = (G == WAVE) ? ~0ull : (((1ull << G) - 1ull) << (wl - lg));
+    const unsigned long long below = (1ull << wl) - 1ull;
+    for (int pb = 0; pb < np; pb += PB) {
+      const bool active = pb + pl < np;
+      float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+      int i = -1;
       if (active) {
-        for (int k0 = 0; k0 < nch; k0 += G) {
-          const int k = k0 + lg;
-          bool acc = false, cnd = false;
-          int j = 0;
-          if (k < nch) {
-            const float4 pj = cand[k];
-            j = __float_as_int(pj.w);
-            cnd = j != i;
-            const float dx = mic_fast(pj.x - pi.x, hx, Lx);
-            const float dy = mic_fast(pj.y - pi.y, hy, Ly);
-            const float dz = mic_fast(pj.z - pi.z, hz, Lz);
-            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            acc = cnd && (r2 <= rl2);
-          }
-          if (G == 1) {
-            tested += cnd;
-            if (acc) {
-              if (found < cap) nbr[(size_t)found * N + i] = j + 1;
-              ++found;
+        pi = sorted[p0 + pb + pl];
+        i = __float_as_int(pi.w);
+      }
+      int found = 0;
+      int64_t tested = 0;
+      for (int ch = 0; ch < total; ch += RC_CHUNK) {
+        const int nch = min(RC_CHUNK, total - ch);
+        __syncthreads();  // the previous chunk (or pass) is no longer read
+        for (int f = threadIdx.x; f < nch; f += TPB) {
+          const int g = ch + f;
+          int cc = 0;
+#pragma unroll
+          for (int h = 16; h >= 1; h >>= 1)
+            if (cc + h < 27 && cpre[cc + h] <= g) cc += h;
+          cand[f] = sorted[cstart[cc] + (g - cpre[cc])];
+        }
+        __syncthreads();
+        if (active) {
+          for (int k0 = 0; k0 < nch; k0 += G) {
+            const int k = k0 + lg;
+            bool acc = false, cnd = false;
+            int j = 0;
+            if (k < nch) {
+              const float4 pj = cand[k];
+              j = __float_as_int(pj.w);
+              cnd = j != i;
+              const float dx = mic_fast(pj.x - pi.x, hx, Lx);
+              const float dy = mic_fast(pj.y - pi.y, hy, Ly);
+              const float dz = mic_fast(pj.z - pi.z, hz, Lz);
+              const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+              acc = cnd && (r2 <= rl2);
             }
-          } else {
-            const unsigned long long am = __ballot(acc) & gmask;
-            tested += __popcll(__ballot(cnd) & gmask);
-            if (acc) {
-              const int slot = found + __popcll(am & below);
-              if (slot < cap) nbr[(size_t)slot * N + i] = j + 1;
+            if (G == 1) {
+              tested += cnd;
+              if (acc) {
+                if (found < cap) nbr[(size_t)found * N + i] = j + 1;
+                ++found;
+              }
+            } else {
+              const unsigned long long am = __ballot(acc) & gmask;
+              tested += __popcll(__ballot(cnd) & gmask);
+              if (acc) {
+                const int slot = found + __popcll(am & below);
+                if (slot < cap) nbr[(size_t)slot * N + i] = j + 1;
+              }
+              found += __popcll(am);
             }
-            found += __popcll(am);
           }
         }
       }
-    }
-    if (active && lg == 0) {
-      counts[i] = found;
-      if (candidates) candidates[i] = tested;
+      if (active && lg == 0) {
+        counts[i] = found;
+        if (candidates) candidates[i] = tested;
+      }
     }
   }
 }
@@ -797,7 +803,7 @@ int enqueue_rebuild(pdx_nl_t nl, hipStream_t st) {
     }
     hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(TPB), 0, st, N, nl->cell_of_particle.p, nl->cell_rank.p,
                        nl->cell_offsets.p, nl->particle_ids_sorted.p, s);
-    hipLaunchKernelGGL(k_sort_cells, dim3(nl->ncell), dim3(WAVE), 0, st, nl->ncell, nl->cell_offsets.p,
+    hipLaunchKernelGGL(k_sort_cells, dim3(std::min(nl->ncell, 8192)), dim3(WAVE), 0, st, nl->ncell, nl->cell_offsets.p,
                        nl->particle_ids_sorted.p, nl->packed.p, nl->sorted.p, s);
     // lanes per particle: the list's row lanes; with PDX_ROW_STAGE=1 the rows are staged in LDS and
     // the lanes raised until the workgroup's rows fit the 64 KiB stage. Staging cuts the build's HBM
@@ -806,16 +812,18 @@ int enqueue_rebuild(pdx_nl_t nl, hipStream_t st) {
     int G = nl->row_lanes;
     bool stage = false;
     if (const char *v = std::getenv("PDX_ROW_STAGE")) stage = std::atoi(v) != 0;
-    // the cell-staged build (k_rows_cells) once the grid has enough cells to fill the chip: at
+    // the cell-staged build (k_rows_cells) once the grid has enough cells to fill the chip and the
+    // cells hold enough particles to fill a workgroup's passes (16 on average): at
     // 486 400 ions (8 000 cells) 1.06-1.09 ms against 1.12 ms in spatial storage order and 1.33
     // against 1.55 ms in lattice order; PDX_ROW_CELLS=0/1 overrides; the LDS-staged row write
     // (PDX_ROW_STAGE=1) is a k_rows option
-    bool cells = nl->ncell >= 1024;
+    bool cells = nl->ncell >= 1024 && N >= 16 * nl->ncell;
     if (const char *v = std::getenv("PDX_ROW_CELLS")) cells = std::atoi(v) != 0;
     if (stage) cells = false;
     if (cells) {
 #define PDX_ROWC(GG)                                                                                        \
-  hipLaunchKernelGGL((k_rows_cells<GG>), dim3(nl->ncell), dim3(TPB), 0, st, N, nl->cap, nl->nx, nl->ny, nl->nz, \
+  hipLaunchKernelGGL((k_rows_cells<GG>), dim3(std::min(nl->ncell, 2048)), dim3(TPB), 0, st, N, nl->cap, nl->ncell, \
+                     nl->nx, nl->ny, nl->nz, \
                      nl->sorted.p, nl->cell_offsets.p, geom, nl->rl2, nl->neighbors_flat.p, nl->counts.p,     \
                      nl->diagnostics ? nl->candidates.p : nullptr, s);
       switch (G) {
