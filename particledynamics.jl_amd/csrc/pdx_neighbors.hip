@@ -812,17 +812,17 @@ int enqueue_rebuild(pdx_nl_t nl, hipStream_t st) {
     int G = nl->row_lanes;
     bool stage = false;
     if (const char *v = std::getenv("PDX_ROW_STAGE")) stage = std::atoi(v) != 0;
-    // the cell-staged build (k_rows_cells) once the grid has enough cells to fill the chip and the
-    // cells hold enough particles to fill a workgroup's passes (16 on average): at
-    // 486 400 ions (8 000 cells) 1.06-1.09 ms against 1.12 ms in spatial storage order and 1.33
-    // against 1.55 ms in lattice order; PDX_ROW_CELLS=0/1 overrides; the LDS-staged row write
-    // (PDX_ROW_STAGE=1) is a k_rows option
-    bool cells = nl->ncell >= 1024 && N >= 16 * nl->ncell;
+    // the cell-staged build (k_rows_cells, PDX_ROW_CELLS=1): at 486 400 ions (8 000 cells, one
+    // workgroup each) 1.06-1.09 ms against 1.12 ms for k_rows in spatial storage order and 1.33
+    // against 1.55 ms in lattice order, but on the checks without rebuild its LDS-holding
+    // workgroups cost 10 us against 5 us, so spatially ordered storage (ForceEngine.reorder)
+    // keeps k_rows; the LDS-staged row write (PDX_ROW_STAGE=1) is a k_rows option
+    bool cells = false;
     if (const char *v = std::getenv("PDX_ROW_CELLS")) cells = std::atoi(v) != 0;
     if (stage) cells = false;
     if (cells) {
 #define PDX_ROWC(GG)                                                                                        \
-  hipLaunchKernelGGL((k_rows_cells<GG>), dim3(std::min(nl->ncell, 2048)), dim3(TPB), 0, st, N, nl->cap, nl->ncell, \
+  hipLaunchKernelGGL((k_rows_cells<GG>), dim3(std::min(nl->ncell, 8192)), dim3(TPB), 0, st, N, nl->cap, nl->ncell, \
                      nl->nx, nl->ny, nl->nz, \
                      nl->sorted.p, nl->cell_offsets.p, geom, nl->rl2, nl->neighbors_flat.p, nl->counts.p,     \
                      nl->diagnostics ? nl->candidates.p : nullptr, s);
