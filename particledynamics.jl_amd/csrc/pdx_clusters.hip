@@ -1263,7 +1263,7 @@ extern "C" int pdx_cluster_forces(pdx_cluster_t c, float *fx, float *fy, float *
   }
   PDX_LAUNCH_CHECK();
   if (energy && totals) {
-    hipLaunchKernelGGL(k_sum_partials<7>, dim3(1), dim3(256), 0, st, (const double *)c->partial.p, c->sblocks, totals,
+    hipLaunchKernelGGL(k_sum_partials<7>, dim3(1), dim3(SUM_THREADS), 0, st, (const double *)c->partial.p, c->sblocks, totals,
                        s.add, (const int64_t *)nullptr);
     PDX_LAUNCH_CHECK();
   }

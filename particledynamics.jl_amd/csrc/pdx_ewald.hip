@@ -233,15 +233,15 @@ int ewald_enqueue(const EwaldArgs &a, int32_t nsample, const int32_t *sample, do
     // per-block FP64 partials, then one block sums them in block order (no device-scope tickets)
     const int rb = std::min(a.n, EBLK);
     hipLaunchKernelGGL(k_real_energy, dim3(rb), dim3(TPB), 0, st, a, part);
-    hipLaunchKernelGGL(k_sum_partials<7>, dim3(1), dim3(256), 0, st, (const double *)part, rb, acc, 0,
+    hipLaunchKernelGGL(k_sum_partials<7>, dim3(1), dim3(SUM_THREADS), 0, st, (const double *)part, rb, acc, 0,
                        (const int64_t *)nullptr);
     const int kb = std::min((a.nbox + TPB - 1) / TPB, EBLK);
     hipLaunchKernelGGL(k_recip_energy, dim3(kb), dim3(TPB), 0, st, a, (const double2 *)S, CoverV, inv4a2, part);
-    hipLaunchKernelGGL(k_sum_partials<7>, dim3(1), dim3(256), 0, st, (const double *)part, kb, acc + 8, 0,
+    hipLaunchKernelGGL(k_sum_partials<7>, dim3(1), dim3(SUM_THREADS), 0, st, (const double *)part, kb, acc + 8, 0,
                        (const int64_t *)nullptr);
     const int qb = std::min((a.n + TPB - 1) / TPB, EBLK);
     hipLaunchKernelGGL(k_charge_sums, dim3(qb), dim3(TPB), 0, st, a, part);
-    hipLaunchKernelGGL(k_sum_partials<2>, dim3(1), dim3(256), 0, st, (const double *)part, qb, acc + 16, 0,
+    hipLaunchKernelGGL(k_sum_partials<2>, dim3(1), dim3(SUM_THREADS), 0, st, (const double *)part, qb, acc + 16, 0,
                        (const int64_t *)nullptr);
   }
   PDX_LAUNCH_CHECK();

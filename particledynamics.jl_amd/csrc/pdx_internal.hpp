@@ -200,11 +200,47 @@ __device__ void sum_partials_block(const double *__restrict__ partial, int nbloc
   }
 }
 
+// The same sum as a kernel of one 1024-thread block (launch with 1024 threads): thread t adds the
+// partials of blocks t, t + 1024, ... in that order, four blocks' loads in flight at a time, then
+// a fixed tree over the threads.
+constexpr int SUM_THREADS = 1024;
 template <int NV>
-__global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ partial, int nblocks, double *out,
-                                                      int add, const int64_t *state) {
+__global__ __launch_bounds__(SUM_THREADS) void k_sum_partials(const double *__restrict__ partial, int nblocks,
+                                                              double *out, int add, const int64_t *state) {
   if (state && !(state[PDX_DN_VALID] != 0 && state[PDX_DN_ERROR] == 0)) return;
-  sum_partials_block<NV>(partial, nblocks, out, add);
+  __shared__ double sh[NV][SUM_THREADS / WAVE];
+  double acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) acc[c] = 0.0;
+  int b = threadIdx.x;
+  for (; b + 3 * SUM_THREADS < nblocks; b += 4 * SUM_THREADS) {
+    double v[4][NV];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < NV; ++c) v[u][c] = partial[(size_t)(b + u * SUM_THREADS) * NV + c];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < NV; ++c) acc[c] += v[u][c];
+  }
+  for (; b < nblocks; b += SUM_THREADS) {
+#pragma unroll
+    for (int c = 0; c < NV; ++c) acc[c] += partial[(size_t)b * NV + c];
+  }
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    double s = wave_sum(acc[c]);
+    if (lane == 0) sh[c][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < SUM_THREADS / WAVE; ++w) s += sh[threadIdx.x][w];
+    out[threadIdx.x] = add ? out[threadIdx.x] + s : s;
+  }
 }
 
 }  // namespace pdx

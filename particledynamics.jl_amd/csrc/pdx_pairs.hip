@@ -1065,7 +1065,7 @@ int pair_forces_impl(pdx_pair_t p, pdx_nl_t nl, float *fx, float *fy, float *fz,
   auto finish = [&]() -> int {
     PDX_LAUNCH_CHECK();
     if (energy && totals) {
-      hipLaunchKernelGGL(k_sum_partials<7>, dim3(1), dim3(256), 0, st, (const double *)partial, blocks,
+      hipLaunchKernelGGL(k_sum_partials<7>, dim3(1), dim3(SUM_THREADS), 0, st, (const double *)partial, blocks,
                          totals, a.add, (const int64_t *)nl->state.p);
       PDX_LAUNCH_CHECK();
     }
