@@ -745,7 +745,9 @@ extern "C" int pdx_mlp_forward_mesh(pdx_mlp_t m, pdx_mesh_t mesh, const float *x
   // one resident round: the kernel fits three 4-wave workgroups per CU (LDS), and each wave loops
   // over its tiles, so no second round of short-lived waves forms a tail
   int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;  // the part's CU count when the query fails
   const int ntile = (n + 15) / 16;
   const int blocks = std::max(1, std::min((ntile + 3) / 4, 3 * cus));
   hipLaunchKernelGGL(k_mlp_forward_mesh, dim3(blocks), dim3(TPB), 0, as_stream(stream), dev_of(m), g, n, out, fx, fy,

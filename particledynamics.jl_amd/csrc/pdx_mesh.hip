@@ -947,36 +947,109 @@ __global__ void k_zero64(unsigned long long *g, size_t n) {
 // Each block also lists, for the tile gather, the particles whose base node floor(u) lies in
 // its tile (every particle in exactly one tile): entries (u1, u2, u3, q) and the particle index,
 // up to tcap of them, and the count (which may exceed tcap) in tn[tile].
-// One particle straight into the tile's accumulator (o = floor(u) - tile origin, wrapped), in the
-// product order of k_spread; the path of particles beyond the staged list.
-template <int WP, int STILE>
-__device__ __forceinline__ void spread_direct(unsigned long long *acc, float u1, float u2, float u3,
-                                                        float q, int o1, int o2, int o3, int e1, int e2, int e3,
-                                                        int K1, int K2, int K3, float scale, const WinCoef &wc) {
+// Staged particles wait in an LDS ring of (u1, u2, u3, q). One pass of the cell scan adds at most
+// scan_cb * SBS of them, and SCHUNK of them are spread as soon as that many wait, so the ring never
+// holds more than SCHUNK - 1 + scan_cb * SBS: every particle reaching the tile is staged, at any
+// density.
+constexpr int SCHUNK = 64;  // staged particles spread together (one lane each in the line count scan)
+template <int SBS> constexpr int scan_cb() { return SBS >= 1024 ? 1 : 2; }
+template <int SBS> constexpr int ring_cap() { return SCHUNK + scan_cb<SBS>() * SBS; }
+
+// byte i of a packed 8-byte list
+__device__ __forceinline__ unsigned byte_at(const uint2 x, int i) { return ((i < 4 ? x.x : x.y) >> (8 * (i & 3))) & 0xFFu; }
+
+// Spread the staged particles h .. h + nc - 1 (ring slots mod RCAP) into the tile accumulator.
+// 1. Three threads per particle compute its window weights per dimension; for x and y they also
+//    compact the window nodes inside the tile into a byte list (entry = j | l << 3: window node j
+//    at tile-local node l) with its length, for z they keep one byte per window node (the local
+//    node, or 0xFF outside the tile).
+// 2. Wave 0 forms the exclusive prefix of the particles' in-tile (x, y) line counts n_x n_y.
+// 3. The line map: entry f of the chunk's lines = (particle, i_x, i_y), a 16-bit word.
+// 4. One line per thread: q12 = ((q w_x) w_y) 2^S, then per z node inside the tile one 64-bit LDS
+//    atomic of q12 w_z rounded to the nearest integer: the products and their order of k_spread, so
+//    the sums are bitwise those of the atomic spread.
+template <int WP, int STILE, int SBS, int RCAP>
+__device__ __forceinline__ void spread_chunk(unsigned long long *acc, const float4 *ring, int h, int nc,
+                                             float (*wsh)[3][8], uint2 *zsh, uint2 (*xyl)[2],
+                                             unsigned char (*ncnt)[2], int *lpre, unsigned short *lmap,
+                                             const int (&K)[3], const int (&A)[3], const int (&E)[3], float scale,
+                                             const WinCoef &wc) {
   constexpr int P = win_width(WP);
   constexpr int SROW = STILE + 1;
-  float wx[P], wy[P], wz[P], d[P];
-  window<WP>(u1, wx, d, wc);
-  window<WP>(u2, wy, d, wc);
-  window<WP>(u3, wz, d, wc);
-  for (int j2 = 0; j2 < P; ++j2) {
-    int l2 = o2 - (P - 1) + j2; l2 += l2 < 0 ? K2 : 0; l2 -= l2 >= K2 ? K2 : 0;
-    if (l2 >= e2) continue;
-    for (int j1 = 0; j1 < P; ++j1) {
-      int l1 = o1 - (P - 1) + j1; l1 += l1 < 0 ? K1 : 0; l1 -= l1 >= K1 ? K1 : 0;
-      if (l1 >= e1) continue;
-      const float q12 = q * wx[j1] * wy[j2] * scale;
-      for (int j3 = 0; j3 < P; ++j3) {
-        int l3 = o3 - (P - 1) + j3; l3 += l3 < 0 ? K3 : 0; l3 -= l3 >= K3 ? K3 : 0;
-        if (l3 >= e3) continue;
-        atomicAdd(&acc[(l3 * STILE + l2) * SROW + l1], (unsigned long long)fixed_round(q12 * wz[j3]));
-      }
+  __syncthreads();  // the previous chunk's lines are done with the chunk arrays
+  for (int e = threadIdx.x; e < 3 * nc; e += SBS) {
+    const int t = e / 3, d = e - 3 * t;
+    const float4 u = ring[(h + t) % RCAP];
+    const int Kd = K[d], a = A[d], ee = E[d];
+    float w[P], dw[P];
+    const int k = window<WP>(d == 0 ? u.x : (d == 1 ? u.y : u.z), w, dw, wc);
+    unsigned lo = 0, hi = 0, zlo = 0xFFFFFFFFu, zhi = 0xFFFFFFFFu;
+    int n = 0;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      wsh[t][d][j] = w[j];
+      int l = wrap_node(k + j, Kd) - a;
+      l += l < 0 ? Kd : 0;
+      const bool in = l < ee;
+      const unsigned bj = in ? (unsigned)l : 0xFFu;
+      if (j < 4) zlo = (zlo & ~(0xFFu << (8 * j))) | (bj << (8 * j));
+      else zhi = (zhi & ~(0xFFu << (8 * (j - 4)))) | (bj << (8 * (j - 4)));
+      const unsigned v = in ? ((unsigned)j | ((unsigned)l << 3)) : 0u;
+      if (n < 4) lo |= v << (8 * n);
+      else hi |= v << (8 * (n - 4));
+      n += in ? 1 : 0;
+    }
+    if (d == 2) {
+      zsh[t] = make_uint2(zlo, zhi);
+    } else {
+      xyl[t][d] = make_uint2(lo, hi);
+      ncnt[t][d] = (unsigned char)n;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {  // SCHUNK == 64: one particle per lane of wave 0
+    const int t = threadIdx.x;
+    int c = t < nc ? (int)ncnt[t][0] * (int)ncnt[t][1] : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(c, o, 64);
+      if (t >= o) c += v;
+    }
+    lpre[t + 1] = c;
+    if (t == 0) lpre[0] = 0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nc * P; e += SBS) {
+    const int t = e / P, i1 = e - t * P;
+    const int n1 = ncnt[t][0], n2 = ncnt[t][1];
+    if (i1 < n1) {
+      unsigned short *m = lmap + lpre[t] + i1 * n2;
+      for (int i2 = 0; i2 < n2; ++i2) m[i2] = (unsigned short)(t | (i1 << 8) | (i2 << 12));
+    }
+  }
+  __syncthreads();
+  const int nlines = lpre[nc];
+  for (int f = threadIdx.x; f < nlines; f += SBS) {
+    const unsigned mm = lmap[f];
+    const int t = mm & 0xFFu, i1 = (mm >> 8) & 0xFu, i2 = mm >> 12;
+    const unsigned b1 = byte_at(xyl[t][0], i1), b2 = byte_at(xyl[t][1], i2);
+    const int j1 = b1 & 7u, l1 = b1 >> 3, j2 = b2 & 7u, l2 = b2 >> 3;
+    const float qv = ring[(h + t) % RCAP].w;
+    const float q12 = qv * wsh[t][0][j1] * wsh[t][1][j2] * scale;
+    float wz[P];
+#pragma unroll
+    for (int j3 = 0; j3 < P; ++j3) wz[j3] = wsh[t][2][j3];
+    const uint2 x3 = zsh[t];
+    unsigned long long *line = acc + l2 * SROW + l1;
+#pragma unroll
+    for (int j3 = 0; j3 < P; ++j3) {
+      const unsigned l3 = byte_at(x3, j3);
+      if (l3 == 0xFFu) continue;
+      atomicAdd(line + l3 * (STILE * SROW), (unsigned long long)fixed_round(q12 * wz[j3]));
     }
   }
 }
 
-constexpr int SLIST = 768;   // particles reaching one block staged in LDS (a few hundred in practice)
-constexpr int SCHUNK = 128;  // staged particles whose weights are computed together
 template <int WP, int STILE, int SBS>
 // (1024-thread workgroups of B-spline orders up to 5 are held to 64 VGPRs, 8 waves per SIMD: two
 // of them, 32 waves, then fit on a CU; higher orders would spill)
@@ -992,13 +1065,20 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && WP <= 5) ? 8 : 1) void k_sprea
                                                       int64_t *stale_word, const WinCoef wc) {
   if (state[PDX_DN_ERROR] != 0) return;  // no usable cells / positions: the grid keeps its last value
   constexpr int P = win_width(WP);
-  constexpr int LPP = lanes_for(P);
-  // x rows padded to STILE + 1 nodes: the P x P lanes of a particle then spread over the LDS banks
+  constexpr int CB = scan_cb<SBS>();
+  constexpr int RCAP = ring_cap<SBS>();
+  // x rows padded to STILE + 1 nodes
   constexpr int SROW = STILE + 1;
   __shared__ unsigned long long acc[STILE * STILE * SROW];
-  __shared__ float4 list[SLIST];
+  __shared__ float4 ring[RCAP];
+  __shared__ float wsh[SCHUNK][3][8];
+  __shared__ uint2 zsh[SCHUNK];
+  __shared__ uint2 xyl[SCHUNK][2];
+  __shared__ unsigned char ncnt[SCHUNK][2];
+  __shared__ int lpre[SCHUNK + 1];
+  __shared__ unsigned short lmap[SCHUNK * P * P];
   __shared__ int cstart[64], cpre[65];
-  __shared__ int nlist;
+  __shared__ int tail;  // particles staged so far; particle s waits in ring slot s % RCAP
   const int tx = (K1 + STILE - 1) / STILE, ty = (K2 + STILE - 1) / STILE;
   const int b = blockIdx.x;
   const int a1 = (b % tx) * STILE, a2 = ((b / tx) % ty) * STILE, a3 = (b / (tx * ty)) * STILE;
@@ -1037,7 +1117,7 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && WP <= 5) ? 8 : 1) void k_sprea
   // remaining cells are taken in further rounds of 64)
   __shared__ int nown;
   if (threadIdx.x == 0) {
-    nlist = 0;
+    tail = 0;
     nown = 0;
   }
   float4 *own = tlist + (size_t)b * tcap;
@@ -1059,14 +1139,9 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && WP <= 5) ? 8 : 1) void k_sprea
     }
     if (p.w == 0.0f) return;
     if (o1 > e1 + P - 2 || o2 > e2 + P - 2 || o3 > e3 + P - 2) return;
-    const int slot = atomicAdd(&nlist, 1);
-    if (slot < SLIST) {
-      list[slot] = make_float4(u1, u2, u3, p.w);
-      return;
-    }
-    // list full: this thread spreads the particle itself
-    spread_direct<WP, STILE>(acc, u1, u2, u3, p.w, o1, o2, o3, e1, e2, e3, K1, K2, K3, scale, wc);
+    ring[atomicAdd(&tail, 1) % RCAP] = make_float4(u1, u2, u3, p.w);
   };
+  int head = 0;  // particles spread so far (the same in every thread)
   for (int round = 0; !stale && round * 64 < ncand; ++round) {
     const int nc = min(64, ncand - round * 64);
     if (threadIdx.x < 64) {  // wave 0: the round's cells and the inclusive scan of their sizes
@@ -1097,83 +1172,39 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && WP <= 5) ? 8 : 1) void k_sprea
     auto cell_of = [&](int f) {
       int c = 0;
 #pragma unroll
-      for (int h = 32; h >= 1; h >>= 1)
-        if (c + h < nc && cpre[c + h] <= f) c += h;
+      for (int hh = 32; hh >= 1; hh >>= 1)
+        if (c + hh < nc && cpre[c + hh] <= f) c += hh;
       return cstart[c] + (f - cpre[c]);
     };
-    constexpr int CB = 2;
-    for (int f0 = threadIdx.x; f0 < total; f0 += CB * SBS) {
+    for (int p0 = 0; p0 < total; p0 += CB * SBS) {
       int ic[CB];
       float4 pc[CB];
 #pragma unroll
       for (int u = 0; u < CB; ++u) {
-        const int f = f0 + u * SBS;
+        const int f = p0 + threadIdx.x + u * SBS;
         ic[u] = f < total ? ids[cell_of(f)] - 1 : -1;
       }
 #pragma unroll
-      for (int u = 0; u < CB; ++u) pc[u] = packed[ic[u] >= 0 ? ic[u] : ic[0]];
+      for (int u = 0; u < CB; ++u) pc[u] = packed[ic[u] >= 0 ? ic[u] : (ic[0] >= 0 ? ic[0] : 0)];
 #pragma unroll
       for (int u = 0; u < CB; ++u)
         if (ic[u] >= 0) take(ic[u], pc[u]);
+      __syncthreads();
+      const int staged = tail;
+      for (; staged - head >= SCHUNK; head += SCHUNK)
+        spread_chunk<WP, STILE, SBS, RCAP>(acc, ring, head, SCHUNK, wsh, zsh, xyl, ncnt, lpre, lmap, K, A, E, scale,
+                                           wc);
+      // every thread has read tail, and the chunks' lines are done with their ring slots, before
+      // the next pass stages more
+      __syncthreads();
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) tn[b] = stale ? 0 : nown;
-  // staged particles, SCHUNK at a time: per particle and dimension (three threads each) the
-  // B-spline weights and the tile-local index of each stencil node (a byte; 0xFF: outside the
-  // tile) go to LDS; then LPP lanes per particle, lane = one (x, y) line of the stencil walking
-  // its P z nodes. The products and their order are those of k_spread: ((q w_x) w_y) 2^S, then
-  // times w_z, rounded to the nearest integer.
-  const int nl = stale ? 0 : min(nlist, SLIST);
-  __shared__ float wsh[SCHUNK][3][8];
-  __shared__ uint2 lsh[SCHUNK][3];
-  const int grp = threadIdx.x / LPP, lane = threadIdx.x % LPP;
-  auto node_byte = [](const uint2 x, int j) { return ((j < 4 ? x.x : x.y) >> (8 * (j & 3))) & 0xFFu; };
-  for (int c0 = 0; c0 < nl; c0 += SCHUNK) {
-    const int nc = min(SCHUNK, nl - c0);
-    __syncthreads();
-    for (int e = threadIdx.x; e < 3 * nc; e += SBS) {
-      const int t = e / 3, d = e - 3 * t;
-      const float4 u = list[c0 + t];
-      const int K = d == 0 ? K1 : (d == 1 ? K2 : K3);
-      const int a = d == 0 ? a1 : (d == 1 ? a2 : a3);
-      const int ee = d == 0 ? e1 : (d == 1 ? e2 : e3);
-      float w[P], dw[P];
-      const int k = window<WP>(d == 0 ? u.x : (d == 1 ? u.y : u.z), w, dw, wc);
-      unsigned lo = 0xFFFFFFFFu, hi = 0xFFFFFFFFu;
-#pragma unroll
-      for (int j = 0; j < P; ++j) {
-        wsh[t][d][j] = w[j];
-        int l = wrap_node(k + j, K) - a;
-        l += l < 0 ? K : 0;
-        const unsigned bj = l < ee ? (unsigned)l : 0xFFu;
-        if (j < 4) lo = (lo & ~(0xFFu << (8 * j))) | (bj << (8 * j));
-        else hi = (hi & ~(0xFFu << (8 * (j - 4)))) | (bj << (8 * (j - 4)));
-      }
-      lsh[t][d] = make_uint2(lo, hi);
-    }
-    __syncthreads();
-    for (int t = grp; t < nc; t += SBS / LPP) {
-      const float qv = list[c0 + t].w;
-      for (int l = lane; l < P * P; l += LPP) {
-        const int j1 = l % P, j2 = l / P;
-        const unsigned l1 = node_byte(lsh[t][0], j1), l2 = node_byte(lsh[t][1], j2);
-        if (l1 == 0xFFu || l2 == 0xFFu) continue;
-        const float q12 = qv * wsh[t][0][j1] * wsh[t][1][j2] * scale;
-        float wz[P];
-#pragma unroll
-        for (int j3 = 0; j3 < P; ++j3) wz[j3] = wsh[t][2][j3];
-        const uint2 x3 = lsh[t][2];
-        unsigned long long *line = acc + l2 * SROW + l1;
-#pragma unroll
-        for (int j3 = 0; j3 < P; ++j3) {
-          const unsigned l3 = node_byte(x3, j3);
-          if (l3 == 0xFFu) continue;
-          atomicAdd(line + l3 * (STILE * SROW), (unsigned long long)fixed_round(q12 * wz[j3]));
-        }
-      }
-    }
-  }
+  const int staged = stale ? 0 : tail;
+  if (staged > head)
+    spread_chunk<WP, STILE, SBS, RCAP>(acc, ring, head, staged - head, wsh, zsh, xyl, ncnt, lpre, lmap, K, A, E, scale,
+                                       wc);
   __syncthreads();
   const size_t KK = (size_t)K1 * K2;
   for (int k = threadIdx.x; k < STILE * STILE * STILE; k += SBS) {

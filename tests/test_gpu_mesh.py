@@ -175,6 +175,31 @@ def test_spread_workgroup_sizes_agree(cuda, mesh):
         s.set_spread_threads(128)
 
 
+@pytest.mark.parametrize("P,mesh,N", [(6, (20, 22, 24), 20000), (5, (8, 8, 8), 4000), (4, (10, 12, 9), 3000)])
+@pytest.mark.parametrize("tile", ["8", "16"])
+def test_spread_cells_dense_equals_atomic_spread(cuda, P, mesh, N, tile, monkeypatch):
+    """Thousands of particles per tile (the staged particles pass through the ring of pending
+    chunks many times over) and meshes smaller than a tile plus its halo (a stencil wraps around
+    and meets the tile at both ends, so its in-tile nodes are not one run): the owner-computes
+    spread still gives the atomic spread's grid bit for bit, at every workgroup size."""
+    import torch
+    monkeypatch.setenv("PDX_SPREAD_TILE", tile)
+    R, q = systems.random_cloud(N, L_ES, seed=N + P)
+    s = _solver(mesh, P=P, N=N, q=q)
+    X, Y, Z, Q = dev(R[:, 0]), dev(R[:, 1]), dev(R[:, 2]), dev(q)
+    nl = pdx.DeviceNeighborWorkspace(N, L_ES, 2.0, 0.3, 512)
+    nl.update(X, Y, Z, Q)
+    nl.status()
+    s.spread(X, Y, Z, Q)
+    ga = s.fixed_grid().clone()
+    for threads in (256, 512, 1024):
+        s.set_spread_threads(threads)
+        s.spread_cells(nl)
+        gb = s.fixed_grid().clone()
+        print(f"[dense spread] P={P} mesh={mesh} N={N} tile={tile} threads={threads}", flush=True)
+        assert torch.equal(ga, gb), threads
+
+
 @pytest.mark.parametrize("mesh", [(120, 120, 120), (60, 60, 60), (48, 45, 50), (64, 128, 30)])
 def test_plane_transforms_equal_line_passes(cuda, mesh, monkeypatch):
     """The plane transforms (x and y of a z plane in one workgroup, the z solve of a y plane in
