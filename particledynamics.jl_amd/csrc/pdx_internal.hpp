@@ -360,4 +360,8 @@ struct pdx_mesh_s {
   const void *tiles_nl = nullptr;
   bool tiles_snapshot = false;  // the lists came from a snapshot spread (its word 3 may send the gather per particle)
   int tiles_stile = 0, tiles_count = 0, tiles_cap = 0;
+  // the owner-form spread's tile blocks ((STILE + P - 1)^3 fixed-point nodes per tile, folded into
+  // the grid by a second pass); empty when they would exceed the size limit (the pull form runs)
+  pdx::DevBuf<unsigned long long> tile_blocks;
+  size_t tile_block_entries = 0;
 };

@@ -937,23 +937,28 @@ __global__ void k_zero64(unsigned long long *g, size_t n) {
   if (k < n) g[k] = 0ull;
 }
 
-// ---- spread, owner computes: one workgroup per TILE^3 block of grid nodes. It scans the
-// cells of the neighbour list that can hold a particle whose P^3 stencil reaches the block
-// (a particle at u covers nodes floor(u)-P+1 .. floor(u); cells are widened by the list's
-// skin/2, the most any particle moved since it was binned), accumulates q w_x w_y w_z in
-// 64-bit fixed point with LDS integer atomics (exact, order-independent) and stores the
-// block with plain writes: no global atomics, and no clearing (every node is written).
+// ---- spread, owner computes: one workgroup per TILE^3 block of grid nodes, in two forms.
+// Pull form: the workgroup scans the cells of the neighbour list that can hold a particle whose
+// P^3 stencil reaches the block (a particle at u covers nodes floor(u)-P+1 .. floor(u); cells are
+// widened by the list's skin/2, the most any particle moved since it was binned), accumulates
+// q w_x w_y w_z of the stencil nodes inside the block in 64-bit fixed point with LDS integer
+// atomics (exact, order-independent) and stores the block with plain writes: no global atomics,
+// and no clearing (every node is written). Owner form: the workgroup takes only the particles
+// whose base node lies in the tile, spreads their whole stencils into an LDS block of the tile
+// and its P - 1 lower halo nodes, and writes the block out; k_spread_fold then sums the blocks'
+// overlapping entries per node. Each particle is then handled by one workgroup (the pull form
+// handles it in every tile its stencil reaches, about 2.3 at 16-node tiles), at the price of a
+// second pass over the blocks.
 // Tile edge: 16 nodes, or 8 on meshes too small to give 256 tiles of 16 (one per CU).
 // Each block also lists, for the tile gather, the particles whose base node floor(u) lies in
 // its tile (every particle in exactly one tile): entries (u1, u2, u3, q) and the particle index,
 // up to tcap of them, and the count (which may exceed tcap) in tn[tile].
 // Staged particles wait in an LDS ring of (u1, u2, u3, q). One pass of the cell scan adds at most
-// scan_cb * SBS of them, and SCHUNK of them are spread as soon as that many wait, so the ring never
-// holds more than SCHUNK - 1 + scan_cb * SBS: every particle reaching the tile is staged, at any
+// scan_cb * SBS of them, and a chunk of CH of them is spread as soon as that many wait, so the ring
+// never holds more than CH - 1 + scan_cb * SBS: every particle the tile needs is staged, at any
 // density.
 constexpr int SCHUNK = 64;  // staged particles spread together (one lane each in the line count scan)
 template <int SBS> constexpr int scan_cb() { return SBS >= 1024 ? 1 : 2; }
-template <int SBS> constexpr int ring_cap() { return SCHUNK + scan_cb<SBS>() * SBS; }
 
 // byte i of a packed 8-byte list
 __device__ __forceinline__ unsigned byte_at(const uint2 x, int i) { return ((i < 4 ? x.x : x.y) >> (8 * (i & 3))) & 0xFFu; }
@@ -1050,10 +1055,117 @@ __device__ __forceinline__ void spread_chunk(unsigned long long *acc, const floa
   }
 }
 
-template <int WP, int STILE, int SBS>
+// Owner form: the tile spreads only its own particles (base node in the tile), each into a block
+// of (STILE + P - 1)^3 nodes, the tile and the P - 1 nodes below it in every dimension (unwrapped:
+// block index = node - origin + P - 1), so every stencil node lands in the block. The block of
+// own particle t starts at its base offset o (node k0 + j of window node j at block index o + j).
+// 1. three threads per particle: window weights per dimension and the base offset (a byte);
+// 2. one (particle, x node j1) per thread, walking its P x P (y, z) nodes: q12 = ((q w_x) w_y) 2^S,
+//    then 64-bit LDS atomics of q12 w_z rounded to the nearest integer (the products and order of
+//    k_spread), every address the item's base plus an immediate offset.
+constexpr int OCHUNK = 160;  // own particles spread together (P items each: 960 at P = 6)
+template <int WP, int STILE, int SBS, int RCAP>
+__device__ __forceinline__ void own_chunk(unsigned long long *blk, const float4 *ring, int h, int nc,
+                                          float (*wsh)[3][8], uchar4 *ob, const int (&K)[3], const int (&A)[3],
+                                          float scale, const WinCoef &wc) {
+  constexpr int P = win_width(WP);
+  constexpr int BV = STILE + P - 1;
+  __syncthreads();  // the previous chunk's lines are done with the chunk arrays
+  for (int e = threadIdx.x; e < 3 * nc; e += SBS) {
+    const int t = e / 3, d = e - 3 * t;
+    const float4 u = ring[(h + t) % RCAP];
+    float w[P], dw[P];
+    const int k = window<WP>(d == 0 ? u.x : (d == 1 ? u.y : u.z), w, dw, wc);
+#pragma unroll
+    for (int j = 0; j < P; ++j) wsh[t][d][j] = w[j];
+    // k + P - 1 is the base node, which lies in the tile: o in [0, STILE)
+    ((unsigned char *)&ob[t])[d] = (unsigned char)wrap_node(k + P - 1 - A[d], K[d]);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nc * P; e += SBS) {
+    const int t = e / P, j1 = e - t * P;
+    const uchar4 o = ob[t];
+    const float qx = ring[(h + t) % RCAP].w * wsh[t][0][j1];
+    float wy[P], wz[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      wy[j] = wsh[t][1][j];
+      wz[j] = wsh[t][2][j];
+    }
+    unsigned long long *base = blk + ((int)o.z * BV + (int)o.y) * BV + (int)o.x + j1;
+#pragma unroll
+    for (int j2 = 0; j2 < P; ++j2) {
+      const float q12 = qx * wy[j2] * scale;
+#pragma unroll
+      for (int j3 = 0; j3 < P; ++j3)
+        atomicAdd(base + j2 * BV + j3 * (BV * BV), (unsigned long long)fixed_round(q12 * wz[j3]));
+    }
+  }
+}
+
+// Each node of the fixed-point grid = the sum of the tile blocks' entries that stand for it (the
+// owner form above). Per dimension: its own tile's (block index r + P - 1, r = node - origin), the
+// next tile's when the node lies within P - 1 below that tile's origin, and tile 0's when it lies
+// within P - 1 below K (the periodic wrap), so one to three entries; the node's sum runs over their
+// combinations. Integer sums: exact, in any order.
+// The candidates of one dimension: slot 0 its own tile's (always), slot 1 the next tile's, slot 2
+// tile 0's wrapped halo, each with its validity.
+template <int STILE>
+__device__ __forceinline__ void fold_slots(int x, int K, int nt, int P, int (&t)[3], int (&b)[3], bool (&v)[3]) {
+  const int t0 = x / STILE, t1 = (x + P - 1) / STILE;
+  t[0] = t0;
+  b[0] = x - t0 * STILE + P - 1;
+  v[0] = true;
+  t[1] = t1;
+  b[1] = x - t1 * STILE + P - 1;
+  v[1] = t1 != t0 && t1 < nt;
+  t[2] = 0;
+  b[2] = x - K + P - 1;
+  v[2] = x + P - 1 >= K;
+}
+
+// Each node of the fixed-point grid = the sum of the tile blocks' entries that stand for it (the
+// owner form above). Per dimension: its own tile's (block index r + P - 1, r = node - origin), the
+// next tile's when the node lies within P - 1 below that tile's origin, and tile 0's when it lies
+// within P - 1 below K (the periodic wrap), so one to three entries; the node's sum runs over their
+// combinations. Integer sums: exact, in any order. One wave per 64 nodes of an x row (block
+// 64 x 4: four rows of y), so the y and z candidates are the same across a wave.
+template <int STILE>
+__global__ __launch_bounds__(256) void k_spread_fold(const unsigned long long *__restrict__ blocks,
+                                                     unsigned long long *__restrict__ grid, int K1, int K2, int K3,
+                                                     int P, const int64_t *__restrict__ state) {
+  if (state[PDX_DN_ERROR] != 0) return;  // the tiles wrote nothing: the grid keeps its last value
+  const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+  if (x >= K1 || y >= K2) return;
+  const int BV = STILE + P - 1, BVOL = BV * BV * BV;
+  const int nt1 = (K1 + STILE - 1) / STILE, nt2 = (K2 + STILE - 1) / STILE, nt3 = (K3 + STILE - 1) / STILE;
+  int tx[3], bx[3], ty[3], by[3], tz[3], bz[3];
+  bool vx[3], vy[3], vz[3];
+  fold_slots<STILE>(x, K1, nt1, P, tx, bx, vx);
+  fold_slots<STILE>(y, K2, nt2, P, ty, by, vy);
+  fold_slots<STILE>(z, K3, nt3, P, tz, bz, vz);
+  unsigned long long s = 0;
+#pragma unroll
+  for (int i3 = 0; i3 < 3; ++i3) {
+    if (!vz[i3]) continue;
+#pragma unroll
+    for (int i2 = 0; i2 < 3; ++i2) {
+      if (!vy[i2]) continue;
+      const int row = tz[i3] * nt2 + ty[i2];
+      const int off = (bz[i3] * BV + by[i2]) * BV;
+#pragma unroll
+      for (int i1 = 0; i1 < 3; ++i1)
+        if (vx[i1]) s += blocks[(size_t)(row * nt1 + tx[i1]) * BVOL + off + bx[i1]];
+    }
+  }
+  grid[((size_t)z * K2 + y) * K1 + x] = s;
+}
+
+template <int WP, int STILE, int SBS, bool OWN>
 // (1024-thread workgroups of B-spline orders up to 5 are held to 64 VGPRs, 8 waves per SIMD: two
-// of them, 32 waves, then fit on a CU; higher orders would spill)
-__global__ __launch_bounds__(SBS, (SBS == 1024 && WP <= 5) ? 8 : 1) void k_spread_tiles(const float4 *__restrict__ packed,
+// of them, 32 waves, then fit on a CU; higher orders would spill. The owner form's block takes
+// most of the LDS: one workgroup per CU.)
+__global__ __launch_bounds__(SBS, (!OWN && SBS == 1024 && WP <= 5) ? 8 : 1) void k_spread_tiles(const float4 *__restrict__ packed,
                                                       const int32_t *__restrict__ offsets,
                                                       const int32_t *__restrict__ ids,
                                                       const int64_t *__restrict__ state, int nx, int ny,
@@ -1062,28 +1174,34 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && WP <= 5) ? 8 : 1) void k_sprea
                                                       float4 *__restrict__ tlist, int32_t *__restrict__ tids,
                                                       int32_t *__restrict__ tn, int tcap,
                                                       const float *__restrict__ disp, int ndisp, float stale2,
-                                                      int64_t *stale_word, const WinCoef wc) {
+                                                      int64_t *stale_word, const WinCoef wc,
+                                                      unsigned long long *__restrict__ blocks) {
   if (state[PDX_DN_ERROR] != 0) return;  // no usable cells / positions: the grid keeps its last value
   constexpr int P = win_width(WP);
-  constexpr int CB = scan_cb<SBS>();
-  constexpr int RCAP = ring_cap<SBS>();
-  // x rows padded to STILE + 1 nodes
+  constexpr int CB = OWN ? 2 : scan_cb<SBS>();
+  constexpr int CH = OWN ? OCHUNK : SCHUNK;
+  constexpr int RCAP = CH + CB * SBS;
+  // the pull form's accumulator: x rows padded to STILE + 1 nodes; the owner form's: the block
   constexpr int SROW = STILE + 1;
-  __shared__ unsigned long long acc[STILE * STILE * SROW];
+  constexpr int BV = STILE + P - 1;
+  constexpr int ACC = OWN ? BV * BV * BV : STILE * STILE * SROW;
+  __shared__ unsigned long long acc[ACC];
   __shared__ float4 ring[RCAP];
-  __shared__ float wsh[SCHUNK][3][8];
-  __shared__ uint2 zsh[SCHUNK];
-  __shared__ uint2 xyl[SCHUNK][2];
-  __shared__ unsigned char ncnt[SCHUNK][2];
-  __shared__ int lpre[SCHUNK + 1];
-  __shared__ unsigned short lmap[SCHUNK * P * P];
-  __shared__ int cstart[64], cpre[65];
+  __shared__ float wsh[CH][3][8];
+  constexpr int PS = OWN ? 1 : SCHUNK;  // the pull form's chunk arrays
+  __shared__ uint2 zsh[PS];
+  __shared__ uint2 xyl[PS][2];
+  __shared__ unsigned char ncnt[PS][2];
+  __shared__ int lpre[PS + 1];
+  __shared__ unsigned short lmap[PS * P * P];
+  __shared__ uchar4 ob[OWN ? OCHUNK : 1];  // the owner form's base offsets
+  __shared__ int cstart[SBS], cpre[SBS + 1], wsum[SBS / 64];
   __shared__ int tail;  // particles staged so far; particle s waits in ring slot s % RCAP
   const int tx = (K1 + STILE - 1) / STILE, ty = (K2 + STILE - 1) / STILE;
   const int b = blockIdx.x;
   const int a1 = (b % tx) * STILE, a2 = ((b / tx) % ty) * STILE, a3 = (b / (tx * ty)) * STILE;
   const int e1 = min(STILE, K1 - a1), e2 = min(STILE, K2 - a2), e3 = min(STILE, K3 - a3);
-  for (int k = threadIdx.x; k < STILE * STILE * SROW; k += SBS) acc[k] = 0ull;
+  for (int k = threadIdx.x; k < ACC; k += SBS) acc[k] = 0ull;
   // snapshot cells (disp != null): the check's per-block maxima of |x - x_build|^2 bound the
   // motion since the snapshot's build. A maximum beyond the margin (or -1: not measured) makes
   // the cells unusable: every tile then only clears its nodes, and the fallback spread and the
@@ -1099,22 +1217,22 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && WP <= 5) ? 8 : 1) void k_sprea
     if (b == 0 && threadIdx.x == 0) stale_word[3] = stale ? 1 : 0;
   }
   // candidate cells per dimension, in the list's frame (x + L/2) / L: a particle at u reaches the
-  // block when the base node of u + shift lies in [A, A + E + P - 2]
+  // block when the base node of u + shift lies in [A, A + E + P - 2] (owner form: is own when it
+  // lies in [A, A + E - 1])
   const int K[3] = {K1, K2, K3}, A[3] = {a1, a2, a3}, E[3] = {e1, e2, e3}, NC[3] = {nx, ny, nz};
   const float Ls[3] = {Lx, Ly, Lz};
   int c0[3], cn[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const float f0 = ((float)A[d] - win_shift<WP>()) / (float)K[d] + 0.5f - margin / Ls[d] - 1e-5f;
-    const float f1 = ((float)(A[d] + E[d] + P - 1) - win_shift<WP>()) / (float)K[d] + 0.5f + margin / Ls[d] + 1e-5f;
+    const float f1 = ((float)(A[d] + E[d] + (OWN ? 0 : P - 1)) - win_shift<WP>()) / (float)K[d] + 0.5f + margin / Ls[d] + 1e-5f;
     const int lo = (int)floorf(f0 * NC[d]);
     c0[d] = lo;
     cn[d] = min((int)floorf(f1 * NC[d]) - lo + 1, NC[d]);
   }
   const int ncand = cn[0] * cn[1] * cn[2];
-  // (the cell range is at most 64 cells: a block plus its halo spans < 4 cells per dimension
-  // whenever the cells are wider than (STILE + P + skin) / 3 mesh spacings; otherwise the
-  // remaining cells are taken in further rounds of 64)
+  // (the cell range is taken in rounds of SBS cells: one round whenever the tile's candidate cells
+  // number at most the workgroup's threads)
   __shared__ int nown;
   if (threadIdx.x == 0) {
     tail = 0;
@@ -1123,7 +1241,8 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && WP <= 5) ? 8 : 1) void k_sprea
   float4 *own = tlist + (size_t)b * tcap;
   int32_t *oid = tids + (size_t)b * tcap;
   // one candidate particle: listed for the gather when its base node lies in the tile, staged
-  // for the spread when its stencil reaches the tile (and its charge is nonzero)
+  // for the spread when its stencil reaches the tile (owner form: when its base node lies in the
+  // tile) and its charge is nonzero
   auto take = [&](int i, const float4 p) {
     const float u1 = grid_coordinate(p.x, Lx, K1), u2 = grid_coordinate(p.y, Ly, K2),
                 u3 = grid_coordinate(p.z, Lz, K3);
@@ -1138,41 +1257,54 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && WP <= 5) ? 8 : 1) void k_sprea
       }
     }
     if (p.w == 0.0f) return;
-    if (o1 > e1 + P - 2 || o2 > e2 + P - 2 || o3 > e3 + P - 2) return;
+    if (OWN ? !(o1 < e1 && o2 < e2 && o3 < e3) : (o1 > e1 + P - 2 || o2 > e2 + P - 2 || o3 > e3 + P - 2)) return;
     ring[atomicAdd(&tail, 1) % RCAP] = make_float4(u1, u2, u3, p.w);
   };
   int head = 0;  // particles spread so far (the same in every thread)
-  for (int round = 0; !stale && round * 64 < ncand; ++round) {
-    const int nc = min(64, ncand - round * 64);
-    if (threadIdx.x < 64) {  // wave 0: the round's cells and the inclusive scan of their sizes
-      int cnt = 0;
-      if (threadIdx.x < nc) {
-        const int cc = round * 64 + threadIdx.x;
-        int cx = c0[0] + cc % cn[0], cy = c0[1] + (cc / cn[0]) % cn[1], cz = c0[2] + cc / (cn[0] * cn[1]);
-        cx = ((cx % nx) + nx) % nx;
-        cy = ((cy % ny) + ny) % ny;
-        cz = ((cz % nz) + nz) % nz;
-        const int c = (cz * ny + cy) * nx + cx;
-        cstart[threadIdx.x] = offsets[c] - 1;
-        cnt = offsets[c + 1] - offsets[c];
-      }
+  for (int round = 0; !stale && round * SBS < ncand; ++round) {
+    // the round's cells, one per thread, and the block-wide inclusive scan of their sizes
+    const int nc = min(SBS, ncand - round * SBS);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int cnt = 0;
+    if (threadIdx.x < nc) {
+      const int cc = round * SBS + threadIdx.x;
+      int cx = c0[0] + cc % cn[0], cy = c0[1] + (cc / cn[0]) % cn[1], cz = c0[2] + cc / (cn[0] * cn[1]);
+      cx = ((cx % nx) + nx) % nx;
+      cy = ((cy % ny) + ny) % ny;
+      cz = ((cz % nz) + nz) % nz;
+      const int c = (cz * ny + cy) * nx + cx;
+      cstart[threadIdx.x] = offsets[c] - 1;
+      cnt = offsets[c + 1] - offsets[c];
+    }
 #pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const int u = __shfl_up(cnt, o, 64);
-        if (threadIdx.x >= o) cnt += u;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(cnt, o, 64);
+      if (lane >= o) cnt += u;
+    }
+    if (lane == 63) wsum[w] = cnt;
+    __syncthreads();
+    if (w == 0) {  // the waves' totals, scanned
+      int t = lane < SBS / 64 ? wsum[lane] : 0;
+#pragma unroll
+      for (int o = 1; o < SBS / 64; o <<= 1) {
+        const int u = __shfl_up(t, o, 64);
+        if (lane >= o) t += u;
       }
-      cpre[threadIdx.x + 1] = cnt;
-      if (threadIdx.x == 0) cpre[0] = 0;
+      if (lane < SBS / 64) wsum[lane] = t;
     }
     __syncthreads();
+    cnt += w > 0 ? wsum[w - 1] : 0;
+    cpre[threadIdx.x + 1] = cnt;
+    if (threadIdx.x == 0) cpre[0] = 0;
+    __syncthreads();
     const int total = cpre[nc];
-    // flat candidate index f -> its cell c (the last c < nc with cpre[c] <= f: six halvings)
-    // -> particle; CB candidates per thread and pass, their loads issued together (two memory
-    // round trips per pass: ids, then positions)
+    // flat candidate index f -> its cell c (the last c < nc with cpre[c] <= f, by halvings) ->
+    // particle; CB candidates per thread and pass, their loads issued together (two memory round
+    // trips per pass: ids, then positions)
     auto cell_of = [&](int f) {
       int c = 0;
 #pragma unroll
-      for (int hh = 32; hh >= 1; hh >>= 1)
+      for (int hh = SBS / 2; hh >= 1; hh >>= 1)
         if (c + hh < nc && cpre[c + hh] <= f) c += hh;
       return cstart[c] + (f - cpre[c]);
     };
@@ -1191,9 +1323,13 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && WP <= 5) ? 8 : 1) void k_sprea
         if (ic[u] >= 0) take(ic[u], pc[u]);
       __syncthreads();
       const int staged = tail;
-      for (; staged - head >= SCHUNK; head += SCHUNK)
-        spread_chunk<WP, STILE, SBS, RCAP>(acc, ring, head, SCHUNK, wsh, zsh, xyl, ncnt, lpre, lmap, K, A, E, scale,
-                                           wc);
+      for (; staged - head >= CH; head += CH) {
+        if constexpr (OWN)
+          own_chunk<WP, STILE, SBS, RCAP>(acc, ring, head, CH, wsh, ob, K, A, scale, wc);
+        else
+          spread_chunk<WP, STILE, SBS, RCAP>(acc, ring, head, CH, wsh, zsh, xyl, ncnt, lpre, lmap, K, A, E, scale,
+                                             wc);
+      }
       // every thread has read tail, and the chunks' lines are done with their ring slots, before
       // the next pass stages more
       __syncthreads();
@@ -1202,10 +1338,19 @@ __global__ __launch_bounds__(SBS, (SBS == 1024 && WP <= 5) ? 8 : 1) void k_sprea
   }
   if (threadIdx.x == 0) tn[b] = stale ? 0 : nown;
   const int staged = stale ? 0 : tail;
-  if (staged > head)
-    spread_chunk<WP, STILE, SBS, RCAP>(acc, ring, head, staged - head, wsh, zsh, xyl, ncnt, lpre, lmap, K, A, E, scale,
-                                       wc);
+  if (staged > head) {
+    if constexpr (OWN)
+      own_chunk<WP, STILE, SBS, RCAP>(acc, ring, head, staged - head, wsh, ob, K, A, scale, wc);
+    else
+      spread_chunk<WP, STILE, SBS, RCAP>(acc, ring, head, staged - head, wsh, zsh, xyl, ncnt, lpre, lmap, K, A, E,
+                                         scale, wc);
+  }
   __syncthreads();
+  if constexpr (OWN) {  // the whole block, every entry written
+    unsigned long long *out = blocks + (size_t)b * ACC;
+    for (int k = threadIdx.x; k < ACC; k += SBS) out[k] = acc[k];
+    return;
+  }
   const size_t KK = (size_t)K1 * K2;
   for (int k = threadIdx.x; k < STILE * STILE * STILE; k += SBS) {
     const int l1 = k % STILE, l2 = (k / STILE) % STILE, l3 = k / (STILE * STILE);
@@ -2311,6 +2456,19 @@ extern "C" int pdx_mesh_create(const pdx_mesh_spec *spec, int32_t n, const doubl
       delete m;
       return fail(PDX_ERR_DEVICE, "mesh initialisation failed");
     }
+    // owner-form tile blocks, for either tile edge, up to 1 GiB (a 256^3 mesh at 8-node tiles)
+    const int P = sp.order;
+    const int nt16 = ((K[0] + 15) / 16) * ((K[1] + 15) / 16) * ((K[2] + 15) / 16);
+    const size_t b16 = (size_t)nt16 * (16 + P - 1) * (16 + P - 1) * (16 + P - 1);
+    const size_t b8 = (size_t)nt8 * (8 + P - 1) * (8 + P - 1) * (8 + P - 1);
+    const size_t nb = std::max(b16, b8);
+    if (nb * sizeof(unsigned long long) <= ((size_t)1 << 30)) {
+      if ((rc = m->tile_blocks.alloc(nb))) {
+        delete m;
+        return rc;
+      }
+      m->tile_block_entries = nb;
+    }
   }
   if (hipMemset(m->grid64.p, 0, sizeof(unsigned long long) * nnodes) != hipSuccess ||
       hipMemset(m->acc.p, 0, sizeof(double) * 8) != hipSuccess ||
@@ -2499,21 +2657,31 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
   const int ndisp = nl->ndisp;
   const float stale2 = margin * margin;
   m->tiles_snapshot = snapshot;
-#define PDX_TSPREAD_L(PP, TT, BB)                                                                              \
-  hipLaunchKernelGGL((k_spread_tiles<PP, TT, BB>), dim3(tiles), dim3(BB), 0, st, nl->packed.p,                 \
+  // the owner form (own particles into tile blocks, then the fold) wherever the blocks fit; the
+  // pull form (every particle reaching a tile, straight into its nodes) otherwise, or on request
+  const int stile = small ? 8 : 16, P = win_width(m->wkey);
+  const size_t bvol = (size_t)(stile + P - 1) * (stile + P - 1) * (stile + P - 1);
+  bool own = (size_t)tiles * bvol <= m->tile_block_entries;
+  if (const char *e = std::getenv("PDX_SPREAD_OWNER")) own = own && std::atoi(e) != 0;
+#define PDX_TSPREAD_L(PP, TT, BB, OO)                                                                          \
+  hipLaunchKernelGGL((k_spread_tiles<PP, TT, BB, OO>), dim3(tiles), dim3(BB), 0, st, nl->packed.p,             \
                      offsets, ids, guard, nl->nx, nl->ny, nl->nz,                                              \
                      margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale, m->tile_list.p,   \
-                     m->tile_ids.p, m->tile_n.p, tcap, disp, ndisp, stale2, nl->snap_state.p, m->wc);
+                     m->tile_ids.p, m->tile_n.p, tcap, disp, ndisp, stale2, nl->snap_state.p, m->wc,      \
+                     m->tile_blocks.p);
 #define PDX_TSPREAD(PP)                                                                                        \
   case PP:                                                                                                     \
-    if (small) {                                                                                               \
-      if (sbs == 1024) { PDX_TSPREAD_L(PP, 8, 1024) }                                                          \
-      else if (sbs == 512) { PDX_TSPREAD_L(PP, 8, 512) }                                                       \
-      else { PDX_TSPREAD_L(PP, 8, 256) }                                                                       \
+    if (own) {                                                                                                 \
+      if (small) { PDX_TSPREAD_L(PP, 8, 1024, true) }                                                          \
+      else { PDX_TSPREAD_L(PP, 16, 1024, true) }                                                               \
+    } else if (small) {                                                                                        \
+      if (sbs == 1024) { PDX_TSPREAD_L(PP, 8, 1024, false) }                                                   \
+      else if (sbs == 512) { PDX_TSPREAD_L(PP, 8, 512, false) }                                                \
+      else { PDX_TSPREAD_L(PP, 8, 256, false) }                                                                \
     } else {                                                                                                   \
-      if (sbs == 1024) { PDX_TSPREAD_L(PP, 16, 1024) }                                                         \
-      else if (sbs == 512) { PDX_TSPREAD_L(PP, 16, 512) }                                                      \
-      else { PDX_TSPREAD_L(PP, 16, 256) }                                                                      \
+      if (sbs == 1024) { PDX_TSPREAD_L(PP, 16, 1024, false) }                                                  \
+      else if (sbs == 512) { PDX_TSPREAD_L(PP, 16, 512, false) }                                               \
+      else { PDX_TSPREAD_L(PP, 16, 256, false) }                                                               \
     }                                                                                                          \
     break;
   switch (m->wkey) {
@@ -2523,6 +2691,14 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
 #undef PDX_TSPREAD
 #undef PDX_TSPREAD_L
   PDX_LAUNCH_CHECK();
+  if (own) {
+    const dim3 fg((m->K1 + 63) / 64, (m->K2 + 3) / 4, m->K3), fb(64, 4);
+    if (small)
+      hipLaunchKernelGGL(k_spread_fold<8>, fg, fb, 0, st, m->tile_blocks.p, m->grid64.p, m->K1, m->K2, m->K3, P, guard);
+    else
+      hipLaunchKernelGGL(k_spread_fold<16>, fg, fb, 0, st, m->tile_blocks.p, m->grid64.p, m->K1, m->K2, m->K3, P, guard);
+    PDX_LAUNCH_CHECK();
+  }
   if (snapshot) {
     const int fb = (int)std::min<int64_t>(1024, ((int64_t)m->N * lanes_for(8) + TPB - 1) / TPB);
 #define PDX_STALE(PP)                                                                                          \

@@ -47,13 +47,15 @@ def _solver(mesh, P, N, q, eps=1e-4, box=L_ES):
     return pdx.ElectrostaticsSolver(_esp(1.0, 4.0, eps, mesh, P), box, q)
 
 
+@pytest.mark.parametrize("form", ["owner", "pull"])
 @pytest.mark.parametrize("P,mesh", [(4, (20, 22, 24)), (5, (20, 22, 24)), (6, (33, 16, 40)), (7, (48, 48, 54)),
                                     (8, (112, 128, 128))])
-def test_esp_spread_cells_equals_atomic_spread(cuda, P, mesh):
+def test_esp_spread_cells_equals_atomic_spread(cuda, P, mesh, form, monkeypatch):
     """The owner-computes spread with the centred prolate stencil (base node floor(u + P/2), which
     wraps past the top of the mesh) gives bitwise the atomic spread's fixed-point grid, also
-    after the particles moved within the skin."""
+    after the particles moved within the skin, in the owner and the pull form."""
     import torch
+    monkeypatch.setenv("PDX_SPREAD_OWNER", "1" if form == "owner" else "0")
     N = 3000
     R, q = systems.random_cloud(N, L_ES, seed=40 + P)
     R[:40, 0] = L_ES[0] / 2 - 1e-3 * np.arange(1, 41)  # at the top of x: stencils wrap to node 0

@@ -79,11 +79,15 @@ def test_spread_conserves_and_is_deterministic(cuda):
     assert abs(g1.sum().item() - qf.sum()) < 1e-6 * np.abs(qf).sum()
 
 
+@pytest.mark.parametrize("form", ["owner", "pull"])
 @pytest.mark.parametrize("P,mesh", [(5, (20, 22, 24)), (4, (33, 16, 40)), (6, (48, 48, 54)), (5, (112, 128, 128))])
-def test_spread_cells_equals_atomic_spread(cuda, P, mesh):
+def test_spread_cells_equals_atomic_spread(cuda, P, mesh, form, monkeypatch):
     """Owner-computes spread from the list's cells gives bitwise the atomic spread's grid,
-    also after the particles moved (within the skin) since the cells were built."""
+    also after the particles moved (within the skin) since the cells were built: in the owner
+    form (own particles into tile blocks, folded into the grid) and in the pull form (every
+    particle reaching a tile)."""
     import torch
+    monkeypatch.setenv("PDX_SPREAD_OWNER", "1" if form == "owner" else "0")
     N = 3000
     R, q = systems.random_cloud(N, L_ES, seed=P)
     s = _solver(mesh, P=P, N=N, q=q)
@@ -155,10 +159,11 @@ def test_gather_cells_equals_particle_gather(cuda, P, mesh, cap, monkeypatch):
 
 
 @pytest.mark.parametrize("mesh", [(112, 128, 128), (20, 22, 24)])
-def test_spread_workgroup_sizes_agree(cuda, mesh):
-    """The owner-computes spread gives the same fixed-point grid bit for bit with 256-, 512- and
-    1024-thread tiles (16^3 and 8^3 tiles), and refuses other sizes."""
+def test_spread_workgroup_sizes_agree(cuda, mesh, monkeypatch):
+    """The pull form of the owner-computes spread gives the same fixed-point grid bit for bit with
+    256-, 512- and 1024-thread tiles (16^3 and 8^3 tiles), and refuses other sizes."""
     import torch
+    monkeypatch.setenv("PDX_SPREAD_OWNER", "0")
     N = 3000
     R, q = systems.random_cloud(N, L_ES, seed=11)
     s = _solver(mesh, P=5, N=N, q=q)
@@ -175,15 +180,20 @@ def test_spread_workgroup_sizes_agree(cuda, mesh):
         s.set_spread_threads(128)
 
 
-@pytest.mark.parametrize("P,mesh,N", [(6, (20, 22, 24), 20000), (5, (8, 8, 8), 4000), (4, (10, 12, 9), 3000)])
+@pytest.mark.parametrize("form", ["owner", "pull"])
+@pytest.mark.parametrize("P,mesh,N", [(6, (20, 22, 24), 20000), (5, (8, 8, 8), 4000), (4, (10, 12, 9), 3000),
+                                      (5, (20, 35, 18), 6000)])
 @pytest.mark.parametrize("tile", ["8", "16"])
-def test_spread_cells_dense_equals_atomic_spread(cuda, P, mesh, N, tile, monkeypatch):
+def test_spread_cells_dense_equals_atomic_spread(cuda, P, mesh, N, tile, form, monkeypatch):
     """Thousands of particles per tile (the staged particles pass through the ring of pending
     chunks many times over) and meshes smaller than a tile plus its halo (a stencil wraps around
-    and meets the tile at both ends, so its in-tile nodes are not one run): the owner-computes
-    spread still gives the atomic spread's grid bit for bit, at every workgroup size."""
+    and meets the tile at both ends, so its in-tile nodes are not one run; in the owner form a
+    node then takes its own tile's entry, the next tile's halo and tile 0's wrapped halo): the
+    owner-computes spread still gives the atomic spread's grid bit for bit, at every workgroup
+    size."""
     import torch
     monkeypatch.setenv("PDX_SPREAD_TILE", tile)
+    monkeypatch.setenv("PDX_SPREAD_OWNER", "1" if form == "owner" else "0")
     R, q = systems.random_cloud(N, L_ES, seed=N + P)
     s = _solver(mesh, P=P, N=N, q=q)
     X, Y, Z, Q = dev(R[:, 0]), dev(R[:, 1]), dev(R[:, 2]), dev(q)
@@ -196,7 +206,7 @@ def test_spread_cells_dense_equals_atomic_spread(cuda, P, mesh, N, tile, monkeyp
         s.set_spread_threads(threads)
         s.spread_cells(nl)
         gb = s.fixed_grid().clone()
-        print(f"[dense spread] P={P} mesh={mesh} N={N} tile={tile} threads={threads}", flush=True)
+        print(f"[dense spread] {form} P={P} mesh={mesh} N={N} tile={tile} threads={threads}", flush=True)
         assert torch.equal(ga, gb), threads
 
 
@@ -255,11 +265,13 @@ def test_plane_workgroup_sizes_agree(cuda, mesh):
         s.set_transform_threads(256)
 
 
-def test_spread_snapshot_equals_atomic_spread(cuda):
+@pytest.mark.parametrize("form", ["owner", "pull"])
+def test_spread_snapshot_equals_atomic_spread(cuda, form, monkeypatch):
     """The spread over the cell snapshot of the previous build gives the atomic spread's grid bit
     for bit, also on the step whose check triggers a rebuild (the snapshot is then a build
     behind, covered by its full-skin margin), and refuses to run before a snapshot exists."""
     import torch
+    monkeypatch.setenv("PDX_SPREAD_OWNER", "1" if form == "owner" else "0")
     N = 3000
     R, q = systems.random_cloud(N, L_ES, seed=7)
     s = _solver((48, 48, 54), P=5, N=N, q=q)
