@@ -938,120 +938,257 @@ __global__ void k_zero64(unsigned long long *g, size_t n) {
 }
 
 // ---- spread, owner computes: one workgroup per TILE^3 block of grid nodes, in two forms.
-// Pull form: the workgroup scans the cells of the neighbour list that can hold a particle whose
-// P^3 stencil reaches the block (a particle at u covers nodes floor(u)-P+1 .. floor(u); cells are
-// widened by the list's skin/2, the most any particle moved since it was binned), accumulates
-// q w_x w_y w_z of the stencil nodes inside the block in 64-bit fixed point with LDS integer
-// atomics (exact, order-independent) and stores the block with plain writes: no global atomics,
-// and no clearing (every node is written). Owner form: the workgroup takes only the particles
-// whose base node lies in the tile, spreads their whole stencils into an LDS block of the tile
-// and its P - 1 lower halo nodes, and writes the block out; k_spread_fold then sums the blocks'
-// overlapping entries per node. Each particle is then handled by one workgroup (the pull form
-// handles it in every tile its stencil reaches, about 2.3 at 16-node tiles), at the price of a
-// second pass over the blocks.
+// Pull form (k_spread_tiles): the workgroup scans the cells of the neighbour list that can hold a
+// particle whose P^3 stencil reaches the block (a particle at u covers nodes floor(u)-P+1 ..
+// floor(u); cells are widened by the list's skin/2, the most any particle moved since it was
+// binned), accumulates q w_x w_y w_z of the stencil nodes inside the block in 64-bit fixed point
+// with LDS integer atomics (exact, order-independent) and stores the block with plain writes: no
+// global atomics, and no clearing (every node is written). Up to SLIST reaching particles are
+// staged in LDS; beyond that each is spread by its own thread (dense tiles, which take the owner
+// form wherever its blocks fit).
+// Owner form (k_spread_own, then k_spread_fold): the workgroup takes only the particles whose
+// base node lies in the tile, spreads their whole stencils into an LDS block of the tile and its
+// P - 1 lower halo nodes, and writes the block out; the fold then sums the blocks' overlapping
+// entries per node. Each particle is handled by one workgroup (the pull form handles it in every
+// tile its stencil reaches, about 2.3 at 16-node tiles), at the price of the blocks' writes and
+// the fold, which only dense meshes repay.
 // Tile edge: 16 nodes, or 8 on meshes too small to give 256 tiles of 16 (one per CU).
 // Each block also lists, for the tile gather, the particles whose base node floor(u) lies in
 // its tile (every particle in exactly one tile): entries (u1, u2, u3, q) and the particle index,
 // up to tcap of them, and the count (which may exceed tcap) in tn[tile].
-// Staged particles wait in an LDS ring of (u1, u2, u3, q). One pass of the cell scan adds at most
-// scan_cb * SBS of them, and a chunk of CH of them is spread as soon as that many wait, so the ring
-// never holds more than CH - 1 + scan_cb * SBS: every particle the tile needs is staged, at any
-// density.
-constexpr int SCHUNK = 64;  // staged particles spread together (one lane each in the line count scan)
-template <int SBS> constexpr int scan_cb() { return SBS >= 1024 ? 1 : 2; }
-
-// byte i of a packed 8-byte list
-__device__ __forceinline__ unsigned byte_at(const uint2 x, int i) { return ((i < 4 ? x.x : x.y) >> (8 * (i & 3))) & 0xFFu; }
-
-// Spread the staged particles h .. h + nc - 1 (ring slots mod RCAP) into the tile accumulator.
-// 1. Three threads per particle compute its window weights per dimension; for x and y they also
-//    compact the window nodes inside the tile into a byte list (entry = j | l << 3: window node j
-//    at tile-local node l) with its length, for z they keep one byte per window node (the local
-//    node, or 0xFF outside the tile).
-// 2. Wave 0 forms the exclusive prefix of the particles' in-tile (x, y) line counts n_x n_y.
-// 3. The line map: entry f of the chunk's lines = (particle, i_x, i_y), a 16-bit word.
-// 4. One line per thread: q12 = ((q w_x) w_y) 2^S, then per z node inside the tile one 64-bit LDS
-//    atomic of q12 w_z rounded to the nearest integer: the products and their order of k_spread, so
-//    the sums are bitwise those of the atomic spread.
-template <int WP, int STILE, int SBS, int RCAP>
-__device__ __forceinline__ void spread_chunk(unsigned long long *acc, const float4 *ring, int h, int nc,
-                                             float (*wsh)[3][8], uint2 *zsh, uint2 (*xyl)[2],
-                                             unsigned char (*ncnt)[2], int *lpre, unsigned short *lmap,
-                                             const int (&K)[3], const int (&A)[3], const int (&E)[3], float scale,
-                                             const WinCoef &wc) {
+// One particle straight into the tile's accumulator (o = floor(u) - tile origin, wrapped), in the
+// product order of k_spread; the path of particles beyond the staged list.
+template <int WP, int STILE>
+__device__ __forceinline__ void spread_direct(unsigned long long *acc, float u1, float u2, float u3,
+                                                        float q, int o1, int o2, int o3, int e1, int e2, int e3,
+                                                        int K1, int K2, int K3, float scale, const WinCoef &wc) {
   constexpr int P = win_width(WP);
   constexpr int SROW = STILE + 1;
-  __syncthreads();  // the previous chunk's lines are done with the chunk arrays
-  for (int e = threadIdx.x; e < 3 * nc; e += SBS) {
-    const int t = e / 3, d = e - 3 * t;
-    const float4 u = ring[(h + t) % RCAP];
-    const int Kd = K[d], a = A[d], ee = E[d];
-    float w[P], dw[P];
-    const int k = window<WP>(d == 0 ? u.x : (d == 1 ? u.y : u.z), w, dw, wc);
-    unsigned lo = 0, hi = 0, zlo = 0xFFFFFFFFu, zhi = 0xFFFFFFFFu;
-    int n = 0;
-#pragma unroll
-    for (int j = 0; j < P; ++j) {
-      wsh[t][d][j] = w[j];
-      int l = wrap_node(k + j, Kd) - a;
-      l += l < 0 ? Kd : 0;
-      const bool in = l < ee;
-      const unsigned bj = in ? (unsigned)l : 0xFFu;
-      if (j < 4) zlo = (zlo & ~(0xFFu << (8 * j))) | (bj << (8 * j));
-      else zhi = (zhi & ~(0xFFu << (8 * (j - 4)))) | (bj << (8 * (j - 4)));
-      const unsigned v = in ? ((unsigned)j | ((unsigned)l << 3)) : 0u;
-      if (n < 4) lo |= v << (8 * n);
-      else hi |= v << (8 * (n - 4));
-      n += in ? 1 : 0;
+  float wx[P], wy[P], wz[P], d[P];
+  window<WP>(u1, wx, d, wc);
+  window<WP>(u2, wy, d, wc);
+  window<WP>(u3, wz, d, wc);
+  for (int j2 = 0; j2 < P; ++j2) {
+    int l2 = o2 - (P - 1) + j2; l2 += l2 < 0 ? K2 : 0; l2 -= l2 >= K2 ? K2 : 0;
+    if (l2 >= e2) continue;
+    for (int j1 = 0; j1 < P; ++j1) {
+      int l1 = o1 - (P - 1) + j1; l1 += l1 < 0 ? K1 : 0; l1 -= l1 >= K1 ? K1 : 0;
+      if (l1 >= e1) continue;
+      const float q12 = q * wx[j1] * wy[j2] * scale;
+      for (int j3 = 0; j3 < P; ++j3) {
+        int l3 = o3 - (P - 1) + j3; l3 += l3 < 0 ? K3 : 0; l3 -= l3 >= K3 ? K3 : 0;
+        if (l3 >= e3) continue;
+        atomicAdd(&acc[(l3 * STILE + l2) * SROW + l1], (unsigned long long)fixed_round(q12 * wz[j3]));
+      }
     }
-    if (d == 2) {
-      zsh[t] = make_uint2(zlo, zhi);
-    } else {
-      xyl[t][d] = make_uint2(lo, hi);
-      ncnt[t][d] = (unsigned char)n;
+  }
+}
+
+constexpr int SLIST = 768;   // particles reaching one block staged in LDS (a few hundred in practice)
+constexpr int SCHUNK = 128;  // staged particles whose weights are computed together
+template <int WP, int STILE, int SBS>
+// (1024-thread workgroups of B-spline orders up to 5 are held to 64 VGPRs, 8 waves per SIMD: two
+// of them, 32 waves, then fit on a CU; higher orders would spill)
+__global__ __launch_bounds__(SBS, (SBS == 1024 && WP <= 5) ? 8 : 1) void k_spread_tiles(const float4 *__restrict__ packed,
+                                                      const int32_t *__restrict__ offsets,
+                                                      const int32_t *__restrict__ ids,
+                                                      const int64_t *__restrict__ state, int nx, int ny,
+                                                      int nz, float margin, unsigned long long *grid, int K1,
+                                                      int K2, int K3, float Lx, float Ly, float Lz, float scale,
+                                                      float4 *__restrict__ tlist, int32_t *__restrict__ tids,
+                                                      int32_t *__restrict__ tn, int tcap,
+                                                      const float *__restrict__ disp, int ndisp, float stale2,
+                                                      int64_t *stale_word, const WinCoef wc) {
+  if (state[PDX_DN_ERROR] != 0) return;  // no usable cells / positions: the grid keeps its last value
+  constexpr int P = win_width(WP);
+  constexpr int LPP = lanes_for(P);
+  // x rows padded to STILE + 1 nodes: the P x P lanes of a particle then spread over the LDS banks
+  constexpr int SROW = STILE + 1;
+  __shared__ unsigned long long acc[STILE * STILE * SROW];
+  __shared__ float4 list[SLIST];
+  __shared__ int cstart[64], cpre[65];
+  __shared__ int nlist;
+  const int tx = (K1 + STILE - 1) / STILE, ty = (K2 + STILE - 1) / STILE;
+  const int b = blockIdx.x;
+  const int a1 = (b % tx) * STILE, a2 = ((b / tx) % ty) * STILE, a3 = (b / (tx * ty)) * STILE;
+  const int e1 = min(STILE, K1 - a1), e2 = min(STILE, K2 - a2), e3 = min(STILE, K3 - a3);
+  for (int k = threadIdx.x; k < STILE * STILE * SROW; k += SBS) acc[k] = 0ull;
+  // snapshot cells (disp != null): the check's per-block maxima of |x - x_build|^2 bound the
+  // motion since the snapshot's build. A maximum beyond the margin (or -1: not measured) makes
+  // the cells unusable: every tile then only clears its nodes, and the fallback spread and the
+  // per-particle gather take over for this evaluation (word 3 of the snapshot state).
+  bool stale = false;
+  if (disp) {
+    int s = 0;
+    for (int k = threadIdx.x; k < ndisp; k += SBS) {
+      const float d = disp[k];
+      s |= !(d >= 0.0f && d <= stale2);
+    }
+    stale = __syncthreads_or(s) != 0;
+    if (b == 0 && threadIdx.x == 0) stale_word[3] = stale ? 1 : 0;
+  }
+  // candidate cells per dimension, in the list's frame (x + L/2) / L: a particle at u reaches the
+  // block when the base node of u + shift lies in [A, A + E + P - 2]
+  const int K[3] = {K1, K2, K3}, A[3] = {a1, a2, a3}, E[3] = {e1, e2, e3}, NC[3] = {nx, ny, nz};
+  const float Ls[3] = {Lx, Ly, Lz};
+  int c0[3], cn[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float f0 = ((float)A[d] - win_shift<WP>()) / (float)K[d] + 0.5f - margin / Ls[d] - 1e-5f;
+    const float f1 = ((float)(A[d] + E[d] + P - 1) - win_shift<WP>()) / (float)K[d] + 0.5f + margin / Ls[d] + 1e-5f;
+    const int lo = (int)floorf(f0 * NC[d]);
+    c0[d] = lo;
+    cn[d] = min((int)floorf(f1 * NC[d]) - lo + 1, NC[d]);
+  }
+  const int ncand = cn[0] * cn[1] * cn[2];
+  // (the cell range is at most 64 cells: a block plus its halo spans < 4 cells per dimension
+  // whenever the cells are wider than (STILE + P + skin) / 3 mesh spacings; otherwise the
+  // remaining cells are taken in further rounds of 64)
+  __shared__ int nown;
+  if (threadIdx.x == 0) {
+    nlist = 0;
+    nown = 0;
+  }
+  float4 *own = tlist + (size_t)b * tcap;
+  int32_t *oid = tids + (size_t)b * tcap;
+  // one candidate particle: listed for the gather when its base node lies in the tile, staged
+  // for the spread when its stencil reaches the tile (and its charge is nonzero)
+  auto take = [&](int i, const float4 p) {
+    const float u1 = grid_coordinate(p.x, Lx, K1), u2 = grid_coordinate(p.y, Ly, K2),
+                u3 = grid_coordinate(p.z, Lz, K3);
+    const int o1 = wrap_node(window_base<WP>(u1) - a1, K1);
+    const int o2 = wrap_node(window_base<WP>(u2) - a2, K2);
+    const int o3 = wrap_node(window_base<WP>(u3) - a3, K3);
+    if (o1 < e1 && o2 < e2 && o3 < e3) {
+      const int k = atomicAdd(&nown, 1);
+      if (k < tcap) {
+        own[k] = make_float4(u1, u2, u3, p.w);
+        oid[k] = i;
+      }
+    }
+    if (p.w == 0.0f) return;
+    if (o1 > e1 + P - 2 || o2 > e2 + P - 2 || o3 > e3 + P - 2) return;
+    const int slot = atomicAdd(&nlist, 1);
+    if (slot < SLIST) {
+      list[slot] = make_float4(u1, u2, u3, p.w);
+      return;
+    }
+    // list full: this thread spreads the particle itself
+    spread_direct<WP, STILE>(acc, u1, u2, u3, p.w, o1, o2, o3, e1, e2, e3, K1, K2, K3, scale, wc);
+  };
+  for (int round = 0; !stale && round * 64 < ncand; ++round) {
+    const int nc = min(64, ncand - round * 64);
+    if (threadIdx.x < 64) {  // wave 0: the round's cells and the inclusive scan of their sizes
+      int cnt = 0;
+      if (threadIdx.x < nc) {
+        const int cc = round * 64 + threadIdx.x;
+        int cx = c0[0] + cc % cn[0], cy = c0[1] + (cc / cn[0]) % cn[1], cz = c0[2] + cc / (cn[0] * cn[1]);
+        cx = ((cx % nx) + nx) % nx;
+        cy = ((cy % ny) + ny) % ny;
+        cz = ((cz % nz) + nz) % nz;
+        const int c = (cz * ny + cy) * nx + cx;
+        cstart[threadIdx.x] = offsets[c] - 1;
+        cnt = offsets[c + 1] - offsets[c];
+      }
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(cnt, o, 64);
+        if (threadIdx.x >= o) cnt += u;
+      }
+      cpre[threadIdx.x + 1] = cnt;
+      if (threadIdx.x == 0) cpre[0] = 0;
+    }
+    __syncthreads();
+    const int total = cpre[nc];
+    // flat candidate index f -> its cell c (the last c < nc with cpre[c] <= f: six halvings)
+    // -> particle; CB candidates per thread and pass, their loads issued together (two memory
+    // round trips per pass: ids, then positions)
+    auto cell_of = [&](int f) {
+      int c = 0;
+#pragma unroll
+      for (int h = 32; h >= 1; h >>= 1)
+        if (c + h < nc && cpre[c + h] <= f) c += h;
+      return cstart[c] + (f - cpre[c]);
+    };
+    constexpr int CB = 2;
+    for (int f0 = threadIdx.x; f0 < total; f0 += CB * SBS) {
+      int ic[CB];
+      float4 pc[CB];
+#pragma unroll
+      for (int u = 0; u < CB; ++u) {
+        const int f = f0 + u * SBS;
+        ic[u] = f < total ? ids[cell_of(f)] - 1 : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < CB; ++u) pc[u] = packed[ic[u] >= 0 ? ic[u] : ic[0]];
+#pragma unroll
+      for (int u = 0; u < CB; ++u)
+        if (ic[u] >= 0) take(ic[u], pc[u]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tn[b] = stale ? 0 : nown;
+  // staged particles, SCHUNK at a time: per particle and dimension (three threads each) the
+  // B-spline weights and the tile-local index of each stencil node (a byte; 0xFF: outside the
+  // tile) go to LDS; then LPP lanes per particle, lane = one (x, y) line of the stencil walking
+  // its P z nodes. The products and their order are those of k_spread: ((q w_x) w_y) 2^S, then
+  // times w_z, rounded to the nearest integer.
+  const int nl = stale ? 0 : min(nlist, SLIST);
+  __shared__ float wsh[SCHUNK][3][8];
+  __shared__ uint2 lsh[SCHUNK][3];
+  const int grp = threadIdx.x / LPP, lane = threadIdx.x % LPP;
+  auto node_byte = [](const uint2 x, int j) { return ((j < 4 ? x.x : x.y) >> (8 * (j & 3))) & 0xFFu; };
+  for (int c0 = 0; c0 < nl; c0 += SCHUNK) {
+    const int nc = min(SCHUNK, nl - c0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 3 * nc; e += SBS) {
+      const int t = e / 3, d = e - 3 * t;
+      const float4 u = list[c0 + t];
+      const int K = d == 0 ? K1 : (d == 1 ? K2 : K3);
+      const int a = d == 0 ? a1 : (d == 1 ? a2 : a3);
+      const int ee = d == 0 ? e1 : (d == 1 ? e2 : e3);
+      float w[P], dw[P];
+      const int k = window<WP>(d == 0 ? u.x : (d == 1 ? u.y : u.z), w, dw, wc);
+      unsigned lo = 0xFFFFFFFFu, hi = 0xFFFFFFFFu;
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        wsh[t][d][j] = w[j];
+        int l = wrap_node(k + j, K) - a;
+        l += l < 0 ? K : 0;
+        const unsigned bj = l < ee ? (unsigned)l : 0xFFu;
+        if (j < 4) lo = (lo & ~(0xFFu << (8 * j))) | (bj << (8 * j));
+        else hi = (hi & ~(0xFFu << (8 * (j - 4)))) | (bj << (8 * (j - 4)));
+      }
+      lsh[t][d] = make_uint2(lo, hi);
+    }
+    __syncthreads();
+    for (int t = grp; t < nc; t += SBS / LPP) {
+      const float qv = list[c0 + t].w;
+      for (int l = lane; l < P * P; l += LPP) {
+        const int j1 = l % P, j2 = l / P;
+        const unsigned l1 = node_byte(lsh[t][0], j1), l2 = node_byte(lsh[t][1], j2);
+        if (l1 == 0xFFu || l2 == 0xFFu) continue;
+        const float q12 = qv * wsh[t][0][j1] * wsh[t][1][j2] * scale;
+        float wz[P];
+#pragma unroll
+        for (int j3 = 0; j3 < P; ++j3) wz[j3] = wsh[t][2][j3];
+        const uint2 x3 = lsh[t][2];
+        unsigned long long *line = acc + l2 * SROW + l1;
+#pragma unroll
+        for (int j3 = 0; j3 < P; ++j3) {
+          const unsigned l3 = node_byte(x3, j3);
+          if (l3 == 0xFFu) continue;
+          atomicAdd(line + l3 * (STILE * SROW), (unsigned long long)fixed_round(q12 * wz[j3]));
+        }
+      }
     }
   }
   __syncthreads();
-  if (threadIdx.x < 64) {  // SCHUNK == 64: one particle per lane of wave 0
-    const int t = threadIdx.x;
-    int c = t < nc ? (int)ncnt[t][0] * (int)ncnt[t][1] : 0;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int v = __shfl_up(c, o, 64);
-      if (t >= o) c += v;
-    }
-    lpre[t + 1] = c;
-    if (t == 0) lpre[0] = 0;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < nc * P; e += SBS) {
-    const int t = e / P, i1 = e - t * P;
-    const int n1 = ncnt[t][0], n2 = ncnt[t][1];
-    if (i1 < n1) {
-      unsigned short *m = lmap + lpre[t] + i1 * n2;
-      for (int i2 = 0; i2 < n2; ++i2) m[i2] = (unsigned short)(t | (i1 << 8) | (i2 << 12));
-    }
-  }
-  __syncthreads();
-  const int nlines = lpre[nc];
-  for (int f = threadIdx.x; f < nlines; f += SBS) {
-    const unsigned mm = lmap[f];
-    const int t = mm & 0xFFu, i1 = (mm >> 8) & 0xFu, i2 = mm >> 12;
-    const unsigned b1 = byte_at(xyl[t][0], i1), b2 = byte_at(xyl[t][1], i2);
-    const int j1 = b1 & 7u, l1 = b1 >> 3, j2 = b2 & 7u, l2 = b2 >> 3;
-    const float qv = ring[(h + t) % RCAP].w;
-    const float q12 = qv * wsh[t][0][j1] * wsh[t][1][j2] * scale;
-    float wz[P];
-#pragma unroll
-    for (int j3 = 0; j3 < P; ++j3) wz[j3] = wsh[t][2][j3];
-    const uint2 x3 = zsh[t];
-    unsigned long long *line = acc + l2 * SROW + l1;
-#pragma unroll
-    for (int j3 = 0; j3 < P; ++j3) {
-      const unsigned l3 = byte_at(x3, j3);
-      if (l3 == 0xFFu) continue;
-      atomicAdd(line + l3 * (STILE * SROW), (unsigned long long)fixed_round(q12 * wz[j3]));
-    }
+  const size_t KK = (size_t)K1 * K2;
+  for (int k = threadIdx.x; k < STILE * STILE * STILE; k += SBS) {
+    const int l1 = k % STILE, l2 = (k / STILE) % STILE, l3 = k / (STILE * STILE);
+    if (l1 < e1 && l2 < e2 && l3 < e3)
+      grid[(size_t)(a3 + l3) * KK + (size_t)(a2 + l2) * K1 + (a1 + l1)] = acc[(l3 * STILE + l2) * SROW + l1];
   }
 }
 
@@ -1161,11 +1298,9 @@ __global__ __launch_bounds__(256) void k_spread_fold(const unsigned long long *_
   grid[((size_t)z * K2 + y) * K1 + x] = s;
 }
 
-template <int WP, int STILE, int SBS, bool OWN>
-// (1024-thread workgroups of B-spline orders up to 5 are held to 64 VGPRs, 8 waves per SIMD: two
-// of them, 32 waves, then fit on a CU; higher orders would spill. The owner form's block takes
-// most of the LDS: one workgroup per CU.)
-__global__ __launch_bounds__(SBS, (!OWN && SBS == 1024 && WP <= 5) ? 8 : 1) void k_spread_tiles(const float4 *__restrict__ packed,
+// The owner form's tile kernel: the block takes most of the LDS, so one workgroup per CU.
+template <int WP, int STILE, int SBS>
+__global__ __launch_bounds__(SBS) void k_spread_own(const float4 *__restrict__ packed,
                                                       const int32_t *__restrict__ offsets,
                                                       const int32_t *__restrict__ ids,
                                                       const int64_t *__restrict__ state, int nx, int ny,
@@ -1178,23 +1313,18 @@ __global__ __launch_bounds__(SBS, (!OWN && SBS == 1024 && WP <= 5) ? 8 : 1) void
                                                       unsigned long long *__restrict__ blocks) {
   if (state[PDX_DN_ERROR] != 0) return;  // no usable cells / positions: the grid keeps its last value
   constexpr int P = win_width(WP);
-  constexpr int CB = OWN ? 2 : scan_cb<SBS>();
-  constexpr int CH = OWN ? OCHUNK : SCHUNK;
+  // own particles wait in an LDS ring of (u1, u2, u3, q): one pass of the cell scan adds at most
+  // CB * SBS of them, and OCHUNK of them are spread as soon as that many wait, so the ring never
+  // holds more than OCHUNK - 1 + CB * SBS: every own particle is staged, at any density
+  constexpr int CB = 2;
+  constexpr int CH = OCHUNK;
   constexpr int RCAP = CH + CB * SBS;
-  // the pull form's accumulator: x rows padded to STILE + 1 nodes; the owner form's: the block
-  constexpr int SROW = STILE + 1;
   constexpr int BV = STILE + P - 1;
-  constexpr int ACC = OWN ? BV * BV * BV : STILE * STILE * SROW;
+  constexpr int ACC = BV * BV * BV;  // the block
   __shared__ unsigned long long acc[ACC];
   __shared__ float4 ring[RCAP];
   __shared__ float wsh[CH][3][8];
-  constexpr int PS = OWN ? 1 : SCHUNK;  // the pull form's chunk arrays
-  __shared__ uint2 zsh[PS];
-  __shared__ uint2 xyl[PS][2];
-  __shared__ unsigned char ncnt[PS][2];
-  __shared__ int lpre[PS + 1];
-  __shared__ unsigned short lmap[PS * P * P];
-  __shared__ uchar4 ob[OWN ? OCHUNK : 1];  // the owner form's base offsets
+  __shared__ uchar4 ob[OCHUNK];  // base offsets
   __shared__ int cstart[SBS], cpre[SBS + 1], wsum[SBS / 64];
   __shared__ int tail;  // particles staged so far; particle s waits in ring slot s % RCAP
   const int tx = (K1 + STILE - 1) / STILE, ty = (K2 + STILE - 1) / STILE;
@@ -1216,16 +1346,15 @@ __global__ __launch_bounds__(SBS, (!OWN && SBS == 1024 && WP <= 5) ? 8 : 1) void
     stale = __syncthreads_or(s) != 0;
     if (b == 0 && threadIdx.x == 0) stale_word[3] = stale ? 1 : 0;
   }
-  // candidate cells per dimension, in the list's frame (x + L/2) / L: a particle at u reaches the
-  // block when the base node of u + shift lies in [A, A + E + P - 2] (owner form: is own when it
-  // lies in [A, A + E - 1])
+  // candidate cells per dimension, in the list's frame (x + L/2) / L: a particle at u is own when
+  // the base node of u + shift lies in [A, A + E - 1]
   const int K[3] = {K1, K2, K3}, A[3] = {a1, a2, a3}, E[3] = {e1, e2, e3}, NC[3] = {nx, ny, nz};
   const float Ls[3] = {Lx, Ly, Lz};
   int c0[3], cn[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const float f0 = ((float)A[d] - win_shift<WP>()) / (float)K[d] + 0.5f - margin / Ls[d] - 1e-5f;
-    const float f1 = ((float)(A[d] + E[d] + (OWN ? 0 : P - 1)) - win_shift<WP>()) / (float)K[d] + 0.5f + margin / Ls[d] + 1e-5f;
+    const float f1 = ((float)(A[d] + E[d]) - win_shift<WP>()) / (float)K[d] + 0.5f + margin / Ls[d] + 1e-5f;
     const int lo = (int)floorf(f0 * NC[d]);
     c0[d] = lo;
     cn[d] = min((int)floorf(f1 * NC[d]) - lo + 1, NC[d]);
@@ -1240,25 +1369,21 @@ __global__ __launch_bounds__(SBS, (!OWN && SBS == 1024 && WP <= 5) ? 8 : 1) void
   }
   float4 *own = tlist + (size_t)b * tcap;
   int32_t *oid = tids + (size_t)b * tcap;
-  // one candidate particle: listed for the gather when its base node lies in the tile, staged
-  // for the spread when its stencil reaches the tile (owner form: when its base node lies in the
-  // tile) and its charge is nonzero
+  // one candidate particle: own when its base node lies in the tile; then listed for the gather,
+  // and staged for the spread when its charge is nonzero
   auto take = [&](int i, const float4 p) {
     const float u1 = grid_coordinate(p.x, Lx, K1), u2 = grid_coordinate(p.y, Ly, K2),
                 u3 = grid_coordinate(p.z, Lz, K3);
     const int o1 = wrap_node(window_base<WP>(u1) - a1, K1);
     const int o2 = wrap_node(window_base<WP>(u2) - a2, K2);
     const int o3 = wrap_node(window_base<WP>(u3) - a3, K3);
-    if (o1 < e1 && o2 < e2 && o3 < e3) {
-      const int k = atomicAdd(&nown, 1);
-      if (k < tcap) {
-        own[k] = make_float4(u1, u2, u3, p.w);
-        oid[k] = i;
-      }
+    if (!(o1 < e1 && o2 < e2 && o3 < e3)) return;
+    const int k = atomicAdd(&nown, 1);
+    if (k < tcap) {
+      own[k] = make_float4(u1, u2, u3, p.w);
+      oid[k] = i;
     }
-    if (p.w == 0.0f) return;
-    if (OWN ? !(o1 < e1 && o2 < e2 && o3 < e3) : (o1 > e1 + P - 2 || o2 > e2 + P - 2 || o3 > e3 + P - 2)) return;
-    ring[atomicAdd(&tail, 1) % RCAP] = make_float4(u1, u2, u3, p.w);
+    if (p.w != 0.0f) ring[atomicAdd(&tail, 1) % RCAP] = make_float4(u1, u2, u3, p.w);
   };
   int head = 0;  // particles spread so far (the same in every thread)
   for (int round = 0; !stale && round * SBS < ncand; ++round) {
@@ -1323,13 +1448,8 @@ __global__ __launch_bounds__(SBS, (!OWN && SBS == 1024 && WP <= 5) ? 8 : 1) void
         if (ic[u] >= 0) take(ic[u], pc[u]);
       __syncthreads();
       const int staged = tail;
-      for (; staged - head >= CH; head += CH) {
-        if constexpr (OWN)
-          own_chunk<WP, STILE, SBS, RCAP>(acc, ring, head, CH, wsh, ob, K, A, scale, wc);
-        else
-          spread_chunk<WP, STILE, SBS, RCAP>(acc, ring, head, CH, wsh, zsh, xyl, ncnt, lpre, lmap, K, A, E, scale,
-                                             wc);
-      }
+      for (; staged - head >= CH; head += CH)
+        own_chunk<WP, STILE, SBS, RCAP>(acc, ring, head, CH, wsh, ob, K, A, scale, wc);
       // every thread has read tail, and the chunks' lines are done with their ring slots, before
       // the next pass stages more
       __syncthreads();
@@ -1338,25 +1458,10 @@ __global__ __launch_bounds__(SBS, (!OWN && SBS == 1024 && WP <= 5) ? 8 : 1) void
   }
   if (threadIdx.x == 0) tn[b] = stale ? 0 : nown;
   const int staged = stale ? 0 : tail;
-  if (staged > head) {
-    if constexpr (OWN)
-      own_chunk<WP, STILE, SBS, RCAP>(acc, ring, head, staged - head, wsh, ob, K, A, scale, wc);
-    else
-      spread_chunk<WP, STILE, SBS, RCAP>(acc, ring, head, staged - head, wsh, zsh, xyl, ncnt, lpre, lmap, K, A, E,
-                                         scale, wc);
-  }
+  if (staged > head) own_chunk<WP, STILE, SBS, RCAP>(acc, ring, head, staged - head, wsh, ob, K, A, scale, wc);
   __syncthreads();
-  if constexpr (OWN) {  // the whole block, every entry written
-    unsigned long long *out = blocks + (size_t)b * ACC;
-    for (int k = threadIdx.x; k < ACC; k += SBS) out[k] = acc[k];
-    return;
-  }
-  const size_t KK = (size_t)K1 * K2;
-  for (int k = threadIdx.x; k < STILE * STILE * STILE; k += SBS) {
-    const int l1 = k % STILE, l2 = (k / STILE) % STILE, l3 = k / (STILE * STILE);
-    if (l1 < e1 && l2 < e2 && l3 < e3)
-      grid[(size_t)(a3 + l3) * KK + (size_t)(a2 + l2) * K1 + (a1 + l1)] = acc[(l3 * STILE + l2) * SROW + l1];
-  }
+  unsigned long long *out = blocks + (size_t)b * ACC;  // the whole block, every entry written
+  for (int k = threadIdx.x; k < ACC; k += SBS) out[k] = acc[k];
 }
 
 // Workgroups are dispatched to the 8 XCDs round robin (block b runs on XCD b mod 8), each XCD
@@ -2657,14 +2762,26 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
   const int ndisp = nl->ndisp;
   const float stale2 = margin * margin;
   m->tiles_snapshot = snapshot;
-  // the owner form (own particles into tile blocks, then the fold) wherever the blocks fit; the
-  // pull form (every particle reaching a tile, straight into its nodes) otherwise, or on request
+  // the owner form (own particles into tile blocks, then the fold) where the blocks fit and the
+  // mesh holds at least one particle per 12 nodes: its block writes and fold cost the same per
+  // node at any density, while the pull form's extra work (each particle in every tile its stencil
+  // reaches) grows with the particles. Measured on MI355X, graph replays: 0.119 particles per node
+  // (486 400 ions, 160^3): owner 128 against pull 172 us; 0.26-0.51 (ESP, 50^3-100^3): owner
+  // 20-22% faster evaluations; 0.019-0.037 (64 000 ions on 120^3-150^3, 512 000 on 300^3): pull
+  // 4-14% faster. The pull form (every particle reaching a tile, straight into its nodes)
+  // otherwise. PDX_SPREAD_OWNER=1 or 0 picks a form (the owner form still only where its blocks fit).
   const int stile = small ? 8 : 16, P = win_width(m->wkey);
   const size_t bvol = (size_t)(stile + P - 1) * (stile + P - 1) * (stile + P - 1);
-  bool own = (size_t)tiles * bvol <= m->tile_block_entries;
-  if (const char *e = std::getenv("PDX_SPREAD_OWNER")) own = own && std::atoi(e) != 0;
-#define PDX_TSPREAD_L(PP, TT, BB, OO)                                                                          \
-  hipLaunchKernelGGL((k_spread_tiles<PP, TT, BB, OO>), dim3(tiles), dim3(BB), 0, st, nl->packed.p,             \
+  const bool fits = (size_t)tiles * bvol <= m->tile_block_entries;
+  bool own = fits && (int64_t)m->N * 12 >= (int64_t)m->K1 * m->K2 * m->K3;
+  if (const char *e = std::getenv("PDX_SPREAD_OWNER")) own = fits && std::atoi(e) != 0;
+#define PDX_TSPREAD_L(PP, TT, BB)                                                                              \
+  hipLaunchKernelGGL((k_spread_tiles<PP, TT, BB>), dim3(tiles), dim3(BB), 0, st, nl->packed.p,                 \
+                     offsets, ids, guard, nl->nx, nl->ny, nl->nz,                                              \
+                     margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale, m->tile_list.p,   \
+                     m->tile_ids.p, m->tile_n.p, tcap, disp, ndisp, stale2, nl->snap_state.p, m->wc);
+#define PDX_TSPREAD_O(PP, TT)                                                                                  \
+  hipLaunchKernelGGL((k_spread_own<PP, TT, 1024>), dim3(tiles), dim3(1024), 0, st, nl->packed.p,               \
                      offsets, ids, guard, nl->nx, nl->ny, nl->nz,                                              \
                      margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale, m->tile_list.p,   \
                      m->tile_ids.p, m->tile_n.p, tcap, disp, ndisp, stale2, nl->snap_state.p, m->wc,      \
@@ -2672,16 +2789,16 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
 #define PDX_TSPREAD(PP)                                                                                        \
   case PP:                                                                                                     \
     if (own) {                                                                                                 \
-      if (small) { PDX_TSPREAD_L(PP, 8, 1024, true) }                                                          \
-      else { PDX_TSPREAD_L(PP, 16, 1024, true) }                                                               \
+      if (small) { PDX_TSPREAD_O(PP, 8) }                                                                      \
+      else { PDX_TSPREAD_O(PP, 16) }                                                                           \
     } else if (small) {                                                                                        \
-      if (sbs == 1024) { PDX_TSPREAD_L(PP, 8, 1024, false) }                                                   \
-      else if (sbs == 512) { PDX_TSPREAD_L(PP, 8, 512, false) }                                                \
-      else { PDX_TSPREAD_L(PP, 8, 256, false) }                                                                \
+      if (sbs == 1024) { PDX_TSPREAD_L(PP, 8, 1024) }                                                          \
+      else if (sbs == 512) { PDX_TSPREAD_L(PP, 8, 512) }                                                       \
+      else { PDX_TSPREAD_L(PP, 8, 256) }                                                                       \
     } else {                                                                                                   \
-      if (sbs == 1024) { PDX_TSPREAD_L(PP, 16, 1024, false) }                                                  \
-      else if (sbs == 512) { PDX_TSPREAD_L(PP, 16, 512, false) }                                               \
-      else { PDX_TSPREAD_L(PP, 16, 256, false) }                                                               \
+      if (sbs == 1024) { PDX_TSPREAD_L(PP, 16, 1024) }                                                         \
+      else if (sbs == 512) { PDX_TSPREAD_L(PP, 16, 512) }                                                      \
+      else { PDX_TSPREAD_L(PP, 16, 256) }                                                                      \
     }                                                                                                          \
     break;
   switch (m->wkey) {
@@ -2690,6 +2807,7 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
   }
 #undef PDX_TSPREAD
 #undef PDX_TSPREAD_L
+#undef PDX_TSPREAD_O
   PDX_LAUNCH_CHECK();
   if (own) {
     const dim3 fg((m->K1 + 63) / 64, (m->K2 + 3) / 4, m->K3), fb(64, 4);

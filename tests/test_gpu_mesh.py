@@ -185,12 +185,12 @@ def test_spread_workgroup_sizes_agree(cuda, mesh, monkeypatch):
                                       (5, (20, 35, 18), 6000)])
 @pytest.mark.parametrize("tile", ["8", "16"])
 def test_spread_cells_dense_equals_atomic_spread(cuda, P, mesh, N, tile, form, monkeypatch):
-    """Thousands of particles per tile (the staged particles pass through the ring of pending
-    chunks many times over) and meshes smaller than a tile plus its halo (a stencil wraps around
-    and meets the tile at both ends, so its in-tile nodes are not one run; in the owner form a
-    node then takes its own tile's entry, the next tile's halo and tile 0's wrapped halo): the
-    owner-computes spread still gives the atomic spread's grid bit for bit, at every workgroup
-    size."""
+    """Thousands of particles per tile (in the pull form most of them beyond the staged list, each
+    spread by its own thread; in the owner form through the ring of pending chunks many times
+    over) and meshes smaller than a tile plus its halo (a stencil wraps around and meets the tile
+    at both ends, so its in-tile nodes are not one run; in the owner form a node then takes its
+    own tile's entry, the next tile's halo and tile 0's wrapped halo): the owner-computes spread
+    still gives the atomic spread's grid bit for bit, at every workgroup size."""
     import torch
     monkeypatch.setenv("PDX_SPREAD_TILE", tile)
     monkeypatch.setenv("PDX_SPREAD_OWNER", "1" if form == "owner" else "0")
