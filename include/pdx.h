@@ -319,10 +319,13 @@ int pdx_mesh_set_box(pdx_mesh_t m, const double box[3]);
 int pdx_mesh_spread(pdx_mesh_t m, const float *x, const float *y, const float *z, const float *q,
                     const int32_t *order, pdx_stream_t stream);
 /* Spread from a neighbor list's cells, owner computes: each workgroup accumulates one 16^3
- * block of the grid in LDS (64-bit fixed point, exact) from the particles of the cells that
- * reach it, and stores the block; no global atomics. Positions and charges are the list's
- * packed (x, y, z, q) of the last pdx_nl_update on the same step; the list must cover the
- * mesh's particles and box. Same grid as pdx_mesh_spread (bitwise). */
+ * block of the grid in LDS (64-bit fixed point, exact) and stores it; no global atomics. On
+ * meshes holding at least one particle per 12 nodes (whose tile blocks fit the 1 GiB set aside
+ * at pdx_mesh_create) a tile takes only the particles whose base node it holds, into a block
+ * with the stencil's halo, and a second pass sums the blocks' overlapping entries; otherwise it
+ * takes every particle reaching it. Positions and charges are the list's packed (x, y, z, q) of
+ * the last pdx_nl_update on the same step; the list must cover the mesh's particles and box.
+ * Same grid as pdx_mesh_spread (bitwise). */
 int pdx_mesh_spread_cells(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t stream);
 /* The owner-computes spread from nl's cell snapshot instead of its current cells, with a full
  * skin of margin: it may run on another stream beside pdx_nl_build of the same step (the
@@ -331,9 +334,10 @@ int pdx_mesh_spread_cells(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t stream);
  * list's creation or a box change (the first builds, the second snapshots the build). Same
  * grid as pdx_mesh_spread. */
 int pdx_mesh_spread_snapshot(pdx_mesh_t m, pdx_nl_t nl, pdx_stream_t stream);
-/* Workgroup size of the owner-computes spreads: 512 (the default: 16 particles at a time per
- * tile), 1024 (32 at a time, twice the waves to launch) or 256 (more room for a kernel running
- * beside it). */
+/* Workgroup size of the owner-computes spreads that take every particle reaching a tile (sparse
+ * meshes; the dense meshes' own-particle form always runs 1024): 512 (the default: 16 particles
+ * at a time per tile), 1024 (32 at a time, twice the waves to launch) or 256 (more room for a
+ * kernel running beside it). */
 int pdx_mesh_set_spread_threads(pdx_mesh_t m, int32_t threads);
 /* Workgroup size of the plane passes of pdx_mesh_solve (meshes whose planes fit LDS): 1024 (the
  * default, the faster transform on its own) or 512 (smaller workgroups that share the CUs better
