@@ -1298,7 +1298,59 @@ __global__ __launch_bounds__(256) void k_spread_fold(const unsigned long long *_
   grid[((size_t)z * K2 + y) * K1 + x] = s;
 }
 
-// The owner form's tile kernel: the block takes most of the LDS, so one workgroup per CU.
+// Tile lists from the particles' current positions (owner form): the counters cleared and the
+// overflow word (word 3 of the list's snapshot state) reset, then k_tile_bin.
+__global__ void k_tile_prep(int32_t *__restrict__ tn, int tiles, int64_t *__restrict__ over_word) {
+  const int k = blockIdx.x * TPB + threadIdx.x;
+  if (k < tiles) tn[k] = 0;
+  if (k == 0) over_word[3] = 0;
+}
+
+// Each particle into the list of the tile that holds its base node, the lists the owner-form spread
+// and the tile gather read: (u1, u2, u3, q) and the index, its slot from the tile's counter, one
+// atomic per tile and wave (the wave's lanes grouped by tile; spatially ordered particles meet a
+// few tiles per wave). A slot beyond tcap sets the overflow word: the tiles then only clear their
+// blocks, the predicated atomic spread adds every particle and the gather walks them all.
+template <int WP, int STILE>
+__global__ __launch_bounds__(TPB) void k_tile_bin(int N, const float4 *__restrict__ packed,
+                                                  const int64_t *__restrict__ state, int K1, int K2, int K3,
+                                                  float Lx, float Ly, float Lz, float4 *__restrict__ tlist,
+                                                  int32_t *__restrict__ tids, int32_t *__restrict__ tn, int tcap,
+                                                  int64_t *__restrict__ over_word) {
+  if (state[PDX_DN_ERROR] != 0) return;  // no usable positions: the grid keeps its last value
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  const bool valid = i < N;
+  const float4 p = packed[valid ? i : 0];
+  const float u1 = grid_coordinate(p.x, Lx, K1), u2 = grid_coordinate(p.y, Ly, K2),
+              u3 = grid_coordinate(p.z, Lz, K3);
+  const int tx = (K1 + STILE - 1) / STILE, ty = (K2 + STILE - 1) / STILE;
+  const int b = ((wrap_node(window_base<WP>(u3), K3) / STILE) * ty + wrap_node(window_base<WP>(u2), K2) / STILE) * tx +
+                wrap_node(window_base<WP>(u1), K1) / STILE;
+  const int lane = threadIdx.x & (WAVE - 1);
+  unsigned long long active = __ballot(valid);
+  int slot = 0;
+  while (active) {  // the same in every lane of the wave
+    const int leader = __ffsll((unsigned long long)active) - 1;
+    const int bl = __shfl(b, leader, WAVE);
+    const unsigned long long m = __ballot(valid && b == bl) & active;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&tn[bl], (int)__popcll(m));
+    base = __shfl(base, leader, WAVE);
+    if ((m >> lane) & 1ull) slot = base + (int)__popcll(m & ((1ull << lane) - 1ull));
+    active &= ~m;
+  }
+  if (!valid) return;
+  if (slot < tcap) {
+    tlist[(size_t)b * tcap + slot] = make_float4(u1, u2, u3, p.w);
+    tids[(size_t)b * tcap + slot] = i;
+  } else {
+    over_word[3] = 1;
+  }
+}
+
+// The owner form's tile kernel: the block takes most of the LDS, so one workgroup per CU. With
+// `listed`, k_tile_bin has written the tile's own particles to its list: they are spread from
+// there, and the cells are not scanned.
 template <int WP, int STILE, int SBS>
 __global__ __launch_bounds__(SBS) void k_spread_own(const float4 *__restrict__ packed,
                                                       const int32_t *__restrict__ offsets,
@@ -1310,7 +1362,7 @@ __global__ __launch_bounds__(SBS) void k_spread_own(const float4 *__restrict__ p
                                                       int32_t *__restrict__ tn, int tcap,
                                                       const float *__restrict__ disp, int ndisp, float stale2,
                                                       int64_t *stale_word, const WinCoef wc,
-                                                      unsigned long long *__restrict__ blocks) {
+                                                      unsigned long long *__restrict__ blocks, int listed) {
   if (state[PDX_DN_ERROR] != 0) return;  // no usable cells / positions: the grid keeps its last value
   constexpr int P = win_width(WP);
   // own particles wait in an LDS ring of (u1, u2, u3, q): one pass of the cell scan adds at most
@@ -1332,6 +1384,23 @@ __global__ __launch_bounds__(SBS) void k_spread_own(const float4 *__restrict__ p
   const int a1 = (b % tx) * STILE, a2 = ((b / tx) % ty) * STILE, a3 = (b / (tx * ty)) * STILE;
   const int e1 = min(STILE, K1 - a1), e2 = min(STILE, K2 - a2), e3 = min(STILE, K3 - a3);
   for (int k = threadIdx.x; k < ACC; k += SBS) acc[k] = 0ull;
+  if (listed) {
+    // a list overflowed (word 3): the block stays clear, the predicated atomic spread adds every
+    // particle; otherwise the tile's n <= tcap own particles, OCHUNK at a time through the ring
+    const int n = stale_word[3] != 0 ? 0 : tn[b];
+    const float4 *own = tlist + (size_t)b * tcap;
+    const int Kv[3] = {K1, K2, K3}, Av[3] = {a1, a2, a3};
+    for (int c0 = 0; c0 < n; c0 += CH) {
+      const int nc = min(CH, n - c0);
+      __syncthreads();  // the previous chunk's lines are done with the ring
+      for (int t = threadIdx.x; t < nc; t += SBS) ring[t] = own[c0 + t];
+      own_chunk<WP, STILE, SBS, RCAP>(acc, ring, 0, nc, wsh, ob, Kv, Av, scale, wc);
+    }
+    __syncthreads();
+    unsigned long long *out = blocks + (size_t)b * ACC;  // the whole block, every entry written
+    for (int k = threadIdx.x; k < ACC; k += SBS) out[k] = acc[k];
+    return;
+  }
   // snapshot cells (disp != null): the check's per-block maxima of |x - x_build|^2 bound the
   // motion since the snapshot's build. A maximum beyond the margin (or -1: not measured) makes
   // the cells unusable: every tile then only clears its nodes, and the fallback spread and the
@@ -2775,17 +2844,31 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
   const bool fits = (size_t)tiles * bvol <= m->tile_block_entries;
   bool own = fits && (int64_t)m->N * 12 >= (int64_t)m->K1 * m->K2 * m->K3;
   if (const char *e = std::getenv("PDX_SPREAD_OWNER")) own = fits && std::atoi(e) != 0;
+  // the owner form's particles: from tile lists that k_tile_bin builds from the current positions
+  // (the default), or from a scan of the list's cells (PDX_SPREAD_LISTS=0). With the lists, word 3
+  // of the snapshot state is their overflow flag: the fallback spread and the per-particle gather
+  // read it as they read a stale snapshot.
+  bool lists = own;
+  if (const char *e = std::getenv("PDX_SPREAD_LISTS")) lists = own && std::atoi(e) != 0;
+  if (lists) m->tiles_snapshot = true;
 #define PDX_TSPREAD_L(PP, TT, BB)                                                                              \
   hipLaunchKernelGGL((k_spread_tiles<PP, TT, BB>), dim3(tiles), dim3(BB), 0, st, nl->packed.p,                 \
                      offsets, ids, guard, nl->nx, nl->ny, nl->nz,                                              \
                      margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale, m->tile_list.p,   \
                      m->tile_ids.p, m->tile_n.p, tcap, disp, ndisp, stale2, nl->snap_state.p, m->wc);
 #define PDX_TSPREAD_O(PP, TT)                                                                                  \
+  if (lists) {                                                                                                 \
+    hipLaunchKernelGGL(k_tile_prep, dim3((tiles + TPB - 1) / TPB), dim3(TPB), 0, st, m->tile_n.p, tiles,       \
+                       nl->snap_state.p);                                                                      \
+    hipLaunchKernelGGL((k_tile_bin<PP, TT>), dim3((m->N + TPB - 1) / TPB), dim3(TPB), 0, st, m->N,            \
+                       nl->packed.p, guard, m->K1, m->K2, m->K3, L[0], L[1], L[2], m->tile_list.p,           \
+                       m->tile_ids.p, m->tile_n.p, tcap, nl->snap_state.p);                                    \
+  }                                                                                                            \
   hipLaunchKernelGGL((k_spread_own<PP, TT, 1024>), dim3(tiles), dim3(1024), 0, st, nl->packed.p,               \
                      offsets, ids, guard, nl->nx, nl->ny, nl->nz,                                              \
                      margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale, m->tile_list.p,   \
-                     m->tile_ids.p, m->tile_n.p, tcap, disp, ndisp, stale2, nl->snap_state.p, m->wc,      \
-                     m->tile_blocks.p);
+                     m->tile_ids.p, m->tile_n.p, tcap, lists ? nullptr : disp, ndisp, stale2,              \
+                     nl->snap_state.p, m->wc, m->tile_blocks.p, lists ? 1 : 0);
 #define PDX_TSPREAD(PP)                                                                                        \
   case PP:                                                                                                     \
     if (own) {                                                                                                 \
@@ -2817,7 +2900,7 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
       hipLaunchKernelGGL(k_spread_fold<16>, fg, fb, 0, st, m->tile_blocks.p, m->grid64.p, m->K1, m->K2, m->K3, P, guard);
     PDX_LAUNCH_CHECK();
   }
-  if (snapshot) {
+  if (snapshot || lists) {
     const int fb = (int)std::min<int64_t>(1024, ((int64_t)m->N * lanes_for(8) + TPB - 1) / TPB);
 #define PDX_STALE(PP)                                                                                          \
   case PP:                                                                                                     \

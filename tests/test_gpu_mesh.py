@@ -210,6 +210,52 @@ def test_spread_cells_dense_equals_atomic_spread(cuda, P, mesh, N, tile, form, m
         assert torch.equal(ga, gb), threads
 
 
+@pytest.mark.parametrize("cap", [None, "3"])
+@pytest.mark.parametrize("lists", ["1", "0"])
+def test_owner_spread_lists_and_overflow(cuda, cap, lists, monkeypatch):
+    """The owner form takes its particles from tile lists built from the current positions (the
+    default) or from a scan of the list's cells. With a tile capacity of 3 the lists overflow: the
+    tiles only clear their blocks, the predicated atomic spread adds every particle and the tile
+    gather walks all particles. Either way the grid is the atomic spread's bit for bit and the
+    forces, energies and virial are the per-particle gather's to FP32 rounding."""
+    import torch
+    monkeypatch.setenv("PDX_SPREAD_OWNER", "1")
+    monkeypatch.setenv("PDX_SPREAD_LISTS", lists)
+    if cap is not None:
+        monkeypatch.setenv("PDX_TILE_CAP", cap)
+    N = 6000
+    R, q = systems.random_cloud(N, L_ES, seed=23)
+    q[::89] = 0.0
+    nz = q != 0
+    q[nz] -= q[nz].mean()
+    s = _solver((20, 22, 24), P=5, N=N, q=q)
+    X, Y, Z, Q = dev(R[:, 0]), dev(R[:, 1]), dev(R[:, 2]), dev(q)
+    nl = pdx.DeviceNeighborWorkspace(N, L_ES, 2.0, 0.3, 512)
+    nl.update(X, Y, Z, Q)
+    nl.status()
+    s.spread(X, Y, Z, Q)
+    ga = s.fixed_grid().clone()
+    s.spread_cells(nl)
+    gb = s.fixed_grid().clone()
+    print(f"[owner lists] lists={lists} cap={cap}", flush=True)
+    assert torch.equal(ga, gb)
+    s.solve(energy=True)
+    out = {}
+    for mode in ("particles", "tiles"):
+        f = [torch.full((N,), 7.0, device="cuda") for _ in range(4)]
+        Vg = torch.zeros(6, device="cuda")
+        if mode == "particles":
+            s.mesh_gather(X, Y, Z, Q, f[0], f[1], f[2], f[3], Vg, energy=True, set=True)
+        else:
+            s.gather_cells(nl, f[0], f[1], f[2], f[3], Vg, energy=True, set=True)
+        out[mode] = ([t.clone() for t in f], Vg.clone())
+    (a0, V0), (a1, V1) = out["particles"], out["tiles"]
+    for k in range(4):
+        scale = a0[k].abs().max().item()
+        assert (a1[k] - a0[k]).abs().max().item() <= 1e-5 * scale, k
+    assert torch.allclose(V0, V1, rtol=1e-5, atol=1e-6 * V0.abs().max().item())
+
+
 @pytest.mark.parametrize("mesh", [(120, 120, 120), (60, 60, 60), (48, 45, 50), (64, 128, 30)])
 def test_plane_transforms_equal_line_passes(cuda, mesh, monkeypatch):
     """The plane transforms (x and y of a z plane in one workgroup, the z solve of a y plane in
