@@ -1348,9 +1348,55 @@ __global__ __launch_bounds__(TPB) void k_tile_bin(int N, const float4 *__restric
   }
 }
 
-// The owner form's tile kernel: the block takes most of the LDS, so one workgroup per CU. With
-// `listed`, k_tile_bin has written the tile's own particles to its list: they are spread from
-// there, and the cells are not scanned.
+// Own particles per chunk of the list-fed owner kernel: as many as leave room for two workgroups
+// per CU (81 920 B of LDS each) beside the block, at most 160; 160 where the block alone is too
+// large for two (a 1024-thread workgroup then runs alone on its CU).
+template <int BV> constexpr int own_list_room() { return (81920 - BV * BV * BV * 8 - 512) / 116; }
+template <int BV> constexpr bool own_list_pair() { return own_list_room<BV>() >= 32; }
+template <int BV> constexpr int own_list_chunk() {
+  return own_list_pair<BV>() ? (own_list_room<BV>() < 160 ? own_list_room<BV>() : 160) : 160;
+}
+
+// The owner form's tile kernel fed by the tile lists of k_tile_bin: the block, a chunk's ring and
+// its weights in LDS, nothing else, so that two 512-thread workgroups share a CU where the block
+// leaves room (orders up to 6 at 16-node tiles); one's list loads and block write then overlap the
+// other's atomics. A list that overflowed (word 3) leaves every block clear: the predicated atomic
+// spread then adds every particle.
+template <int WP, int STILE, int SBS>
+__global__ __launch_bounds__(SBS) void k_spread_own_listed(const int64_t *__restrict__ state, int K1, int K2, int K3,
+                                                           float scale, const float4 *__restrict__ tlist,
+                                                           const int32_t *__restrict__ tn, int tcap,
+                                                           const int64_t *__restrict__ over_word, const WinCoef wc,
+                                                           unsigned long long *__restrict__ blocks) {
+  if (state[PDX_DN_ERROR] != 0) return;  // no usable positions: the grid keeps its last value
+  constexpr int P = win_width(WP);
+  constexpr int BV = STILE + P - 1;
+  constexpr int ACC = BV * BV * BV;
+  constexpr int CH = own_list_chunk<BV>();
+  __shared__ unsigned long long acc[ACC];
+  __shared__ float4 ring[CH];
+  __shared__ float wsh[CH][3][8];
+  __shared__ uchar4 ob[CH];
+  const int tx = (K1 + STILE - 1) / STILE, ty = (K2 + STILE - 1) / STILE;
+  const int b = blockIdx.x;
+  const int a1 = (b % tx) * STILE, a2 = ((b / tx) % ty) * STILE, a3 = (b / (tx * ty)) * STILE;
+  const int Kv[3] = {K1, K2, K3}, Av[3] = {a1, a2, a3};
+  for (int k = threadIdx.x; k < ACC; k += SBS) acc[k] = 0ull;
+  const int n = over_word[3] != 0 ? 0 : tn[b];  // <= tcap when no list overflowed
+  const float4 *own = tlist + (size_t)b * tcap;
+  for (int c0 = 0; c0 < n; c0 += CH) {
+    const int nc = min(CH, n - c0);
+    __syncthreads();  // the previous chunk's lines are done with the ring
+    for (int t = threadIdx.x; t < nc; t += SBS) ring[t] = own[c0 + t];
+    own_chunk<WP, STILE, SBS, CH>(acc, ring, 0, nc, wsh, ob, Kv, Av, scale, wc);
+  }
+  __syncthreads();
+  unsigned long long *out = blocks + (size_t)b * ACC;  // the whole block, every entry written
+  for (int k = threadIdx.x; k < ACC; k += SBS) out[k] = acc[k];
+}
+
+// The owner form's tile kernel over the list's cells (PDX_SPREAD_LISTS=0): the block, the ring and
+// the cell scan's arrays take most of the LDS, so one workgroup per CU.
 template <int WP, int STILE, int SBS>
 __global__ __launch_bounds__(SBS) void k_spread_own(const float4 *__restrict__ packed,
                                                       const int32_t *__restrict__ offsets,
@@ -1362,7 +1408,7 @@ __global__ __launch_bounds__(SBS) void k_spread_own(const float4 *__restrict__ p
                                                       int32_t *__restrict__ tn, int tcap,
                                                       const float *__restrict__ disp, int ndisp, float stale2,
                                                       int64_t *stale_word, const WinCoef wc,
-                                                      unsigned long long *__restrict__ blocks, int listed) {
+                                                      unsigned long long *__restrict__ blocks) {
   if (state[PDX_DN_ERROR] != 0) return;  // no usable cells / positions: the grid keeps its last value
   constexpr int P = win_width(WP);
   // own particles wait in an LDS ring of (u1, u2, u3, q): one pass of the cell scan adds at most
@@ -1384,23 +1430,6 @@ __global__ __launch_bounds__(SBS) void k_spread_own(const float4 *__restrict__ p
   const int a1 = (b % tx) * STILE, a2 = ((b / tx) % ty) * STILE, a3 = (b / (tx * ty)) * STILE;
   const int e1 = min(STILE, K1 - a1), e2 = min(STILE, K2 - a2), e3 = min(STILE, K3 - a3);
   for (int k = threadIdx.x; k < ACC; k += SBS) acc[k] = 0ull;
-  if (listed) {
-    // a list overflowed (word 3): the block stays clear, the predicated atomic spread adds every
-    // particle; otherwise the tile's n <= tcap own particles, OCHUNK at a time through the ring
-    const int n = stale_word[3] != 0 ? 0 : tn[b];
-    const float4 *own = tlist + (size_t)b * tcap;
-    const int Kv[3] = {K1, K2, K3}, Av[3] = {a1, a2, a3};
-    for (int c0 = 0; c0 < n; c0 += CH) {
-      const int nc = min(CH, n - c0);
-      __syncthreads();  // the previous chunk's lines are done with the ring
-      for (int t = threadIdx.x; t < nc; t += SBS) ring[t] = own[c0 + t];
-      own_chunk<WP, STILE, SBS, RCAP>(acc, ring, 0, nc, wsh, ob, Kv, Av, scale, wc);
-    }
-    __syncthreads();
-    unsigned long long *out = blocks + (size_t)b * ACC;  // the whole block, every entry written
-    for (int k = threadIdx.x; k < ACC; k += SBS) out[k] = acc[k];
-    return;
-  }
   // snapshot cells (disp != null): the check's per-block maxima of |x - x_build|^2 bound the
   // motion since the snapshot's build. A maximum beyond the margin (or -1: not measured) makes
   // the cells unusable: every tile then only clears its nodes, and the fallback spread and the
@@ -2864,11 +2893,20 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
                        nl->packed.p, guard, m->K1, m->K2, m->K3, L[0], L[1], L[2], m->tile_list.p,           \
                        m->tile_ids.p, m->tile_n.p, tcap, nl->snap_state.p);                                    \
   }                                                                                                            \
-  hipLaunchKernelGGL((k_spread_own<PP, TT, 1024>), dim3(tiles), dim3(1024), 0, st, nl->packed.p,               \
-                     offsets, ids, guard, nl->nx, nl->ny, nl->nz,                                              \
-                     margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale, m->tile_list.p,   \
-                     m->tile_ids.p, m->tile_n.p, tcap, lists ? nullptr : disp, ndisp, stale2,              \
-                     nl->snap_state.p, m->wc, m->tile_blocks.p, lists ? 1 : 0);
+  if (lists && own_list_pair<TT + win_width(PP) - 1>())                                                        \
+    hipLaunchKernelGGL((k_spread_own_listed<PP, TT, 512>), dim3(tiles), dim3(512), 0, st, guard, m->K1, m->K2, \
+                       m->K3, scale, m->tile_list.p, m->tile_n.p, tcap, nl->snap_state.p, m->wc,                \
+                       m->tile_blocks.p);                                                                      \
+  else if (lists)                                                                                              \
+    hipLaunchKernelGGL((k_spread_own_listed<PP, TT, 1024>), dim3(tiles), dim3(1024), 0, st, guard, m->K1,     \
+                       m->K2, m->K3, scale, m->tile_list.p, m->tile_n.p, tcap, nl->snap_state.p, m->wc,         \
+                       m->tile_blocks.p);                                                                      \
+  else                                                                                                         \
+    hipLaunchKernelGGL((k_spread_own<PP, TT, 1024>), dim3(tiles), dim3(1024), 0, st, nl->packed.p,             \
+                       offsets, ids, guard, nl->nx, nl->ny, nl->nz,                                            \
+                       margin, m->grid64.p, m->K1, m->K2, m->K3, L[0], L[1], L[2], scale, m->tile_list.p, \
+                       m->tile_ids.p, m->tile_n.p, tcap, disp, ndisp, stale2, nl->snap_state.p, m->wc,    \
+                       m->tile_blocks.p);
 #define PDX_TSPREAD(PP)                                                                                        \
   case PP:                                                                                                     \
     if (own) {                                                                                                 \
