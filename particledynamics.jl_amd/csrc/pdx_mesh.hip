@@ -1308,19 +1308,22 @@ __global__ void k_tile_prep(int32_t *__restrict__ tn, int tiles, int64_t *__rest
 
 // Each particle into the list of the tile that holds its base node, the lists the owner-form spread
 // and the tile gather read: (u1, u2, u3, q) and the index, its slot from the tile's counter, one
-// atomic per tile and wave (the wave's lanes grouped by tile; spatially ordered particles meet a
-// few tiles per wave). A slot beyond tcap sets the overflow word: the tiles then only clear their
-// blocks, the predicated atomic spread adds every particle and the gather walks them all.
+// atomic per tile and wave (the wave's lanes grouped by tile). The particles are taken in the
+// list's cell order (`order`: 1-based, a permutation of all particles), so that a wave meets a few
+// tiles whatever the storage order. A slot beyond tcap sets the overflow word: the tiles then only
+// clear their blocks, the predicated atomic spread adds every particle and the gather walks them all.
 template <int WP, int STILE>
 __global__ __launch_bounds__(TPB) void k_tile_bin(int N, const float4 *__restrict__ packed,
+                                                  const int32_t *__restrict__ order,
                                                   const int64_t *__restrict__ state, int K1, int K2, int K3,
                                                   float Lx, float Ly, float Lz, float4 *__restrict__ tlist,
                                                   int32_t *__restrict__ tids, int32_t *__restrict__ tn, int tcap,
                                                   int64_t *__restrict__ over_word) {
   if (state[PDX_DN_ERROR] != 0) return;  // no usable positions: the grid keeps its last value
-  const int i = blockIdx.x * TPB + threadIdx.x;
-  const bool valid = i < N;
-  const float4 p = packed[valid ? i : 0];
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  const bool valid = t < N;
+  const int i = valid ? order[t] - 1 : 0;
+  const float4 p = packed[i];
   const float u1 = grid_coordinate(p.x, Lx, K1), u2 = grid_coordinate(p.y, Ly, K2),
               u3 = grid_coordinate(p.z, Lz, K3);
   const int tx = (K1 + STILE - 1) / STILE, ty = (K2 + STILE - 1) / STILE;
@@ -2873,11 +2876,21 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
   const bool fits = (size_t)tiles * bvol <= m->tile_block_entries;
   bool own = fits && (int64_t)m->N * 12 >= (int64_t)m->K1 * m->K2 * m->K3;
   if (const char *e = std::getenv("PDX_SPREAD_OWNER")) own = fits && std::atoi(e) != 0;
-  // the owner form's particles: from tile lists that k_tile_bin builds from the current positions
-  // (the default), or from a scan of the list's cells (PDX_SPREAD_LISTS=0). With the lists, word 3
-  // of the snapshot state is their overflow flag: the fallback spread and the per-particle gather
-  // read it as they read a stale snapshot.
-  bool lists = own;
+  // the owner form's particles: from tile lists that k_tile_bin builds from the current positions,
+  // or from a scan of the list's cells. The lists' clearing, binning and fallback launches cost
+  // ~15 us whatever the tiles, and repay it only where the tiles fill every CU twice (two 512-thread
+  // tiles per CU): measured, 1 000 tiles (486 400 ions, 160^3) 25 us faster, 512 tiles 3 us faster,
+  // 343 tiles 6-21 us slower. PDX_SPREAD_LISTS=1 or 0 picks. With the lists, word 3 of the snapshot
+  // state is their overflow flag: the fallback spread and the per-particle gather read it as they
+  // read a stale snapshot.
+  static const int device_cus = [] {
+    int dev = 0, c = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0)
+      c = 256;
+    return c;
+  }();
+  const bool pairs_fill = tiles >= 2 * device_cus;
+  bool lists = own && pairs_fill;
   if (const char *e = std::getenv("PDX_SPREAD_LISTS")) lists = own && std::atoi(e) != 0;
   if (lists) m->tiles_snapshot = true;
 #define PDX_TSPREAD_L(PP, TT, BB)                                                                              \
@@ -2890,10 +2903,10 @@ int spread_cells_impl(pdx_mesh_t m, pdx_nl_t nl, bool snapshot, pdx_stream_t str
     hipLaunchKernelGGL(k_tile_prep, dim3((tiles + TPB - 1) / TPB), dim3(TPB), 0, st, m->tile_n.p, tiles,       \
                        nl->snap_state.p);                                                                      \
     hipLaunchKernelGGL((k_tile_bin<PP, TT>), dim3((m->N + TPB - 1) / TPB), dim3(TPB), 0, st, m->N,            \
-                       nl->packed.p, guard, m->K1, m->K2, m->K3, L[0], L[1], L[2], m->tile_list.p,           \
+                       nl->packed.p, ids, guard, m->K1, m->K2, m->K3, L[0], L[1], L[2], m->tile_list.p,      \
                        m->tile_ids.p, m->tile_n.p, tcap, nl->snap_state.p);                                    \
   }                                                                                                            \
-  if (lists && own_list_pair<TT + win_width(PP) - 1>())                                                        \
+  if (lists && own_list_pair<TT + win_width(PP) - 1>() && pairs_fill)                                          \
     hipLaunchKernelGGL((k_spread_own_listed<PP, TT, 512>), dim3(tiles), dim3(512), 0, st, guard, m->K1, m->K2, \
                        m->K3, scale, m->tile_list.p, m->tile_n.p, tcap, nl->snap_state.p, m->wc,                \
                        m->tile_blocks.p);                                                                      \

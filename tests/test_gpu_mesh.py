@@ -210,11 +210,13 @@ def test_spread_cells_dense_equals_atomic_spread(cuda, P, mesh, N, tile, form, m
         assert torch.equal(ga, gb), threads
 
 
+@pytest.mark.parametrize("mesh,N", [((20, 22, 24), 6000), ((128, 128, 128), 30000)])
 @pytest.mark.parametrize("cap", [None, "3"])
 @pytest.mark.parametrize("lists", ["1", "0"])
-def test_owner_spread_lists_and_overflow(cuda, cap, lists, monkeypatch):
+def test_owner_spread_lists_and_overflow(cuda, mesh, N, cap, lists, monkeypatch):
     """The owner form takes its particles from tile lists built from the current positions (the
-    default) or from a scan of the list's cells. With a tile capacity of 3 the lists overflow: the
+    default where the tiles fill every CU twice: 512 tiles of 16^3 at 128^3, two 512-thread tiles
+    per CU) or from a scan of the list's cells. With a tile capacity of 3 the lists overflow: the
     tiles only clear their blocks, the predicated atomic spread adds every particle and the tile
     gather walks all particles. Either way the grid is the atomic spread's bit for bit and the
     forces, energies and virial are the per-particle gather's to FP32 rounding."""
@@ -223,21 +225,20 @@ def test_owner_spread_lists_and_overflow(cuda, cap, lists, monkeypatch):
     monkeypatch.setenv("PDX_SPREAD_LISTS", lists)
     if cap is not None:
         monkeypatch.setenv("PDX_TILE_CAP", cap)
-    N = 6000
     R, q = systems.random_cloud(N, L_ES, seed=23)
     q[::89] = 0.0
     nz = q != 0
     q[nz] -= q[nz].mean()
-    s = _solver((20, 22, 24), P=5, N=N, q=q)
+    s = _solver(mesh, P=5, N=N, q=q)
     X, Y, Z, Q = dev(R[:, 0]), dev(R[:, 1]), dev(R[:, 2]), dev(q)
-    nl = pdx.DeviceNeighborWorkspace(N, L_ES, 2.0, 0.3, 512)
+    nl = pdx.DeviceNeighborWorkspace(N, L_ES, 2.0, 0.3, 1024)
     nl.update(X, Y, Z, Q)
     nl.status()
     s.spread(X, Y, Z, Q)
     ga = s.fixed_grid().clone()
     s.spread_cells(nl)
     gb = s.fixed_grid().clone()
-    print(f"[owner lists] lists={lists} cap={cap}", flush=True)
+    print(f"[owner lists] mesh={mesh} N={N} lists={lists} cap={cap}", flush=True)
     assert torch.equal(ga, gb)
     s.solve(energy=True)
     out = {}
