@@ -348,7 +348,7 @@ struct pdx_mesh_s {
   pdx::DevBuf<double> espq;
   int T1 = 0, T2 = 0, T3 = 0;  // FFT tile widths, fixed at creation
   int plane_tp = 0;            // plane passes: lines per plane padded to this power of two (0: line passes)
-  int spread_threads = 512;    // workgroup size of the owner-computes spread (256, 512 or 1024)
+  int spread_threads = 512;    // workgroup size of the pull-form spread (256, 512 or 1024)
   int transform_threads = 1024;  // workgroup size of the plane passes (512 or 1024)
   // per-tile particle lists of the last owner-computes spread (read by the tile gather):
   // tile_entries entries shared by the tiles, tile_n counts; valid for the positions of

@@ -1655,7 +1655,8 @@ __global__ __launch_bounds__(TPB) void k_gather(int N, const float *__restrict__
 }
 
 // ---- tile gather: the forces at the positions of the last owner-computes spread, from the
-// particle lists it wrote (k_spread_tiles). One workgroup per tile, in three phases with one
+// particle lists it wrote (the tile kernels' own lists, or k_tile_bin's before the list-fed owner
+// form). One workgroup per tile, in three phases with one
 // memory round trip each: (1) the potential of the nodes the tile's particles reach (the tile
 // and the P-1 nodes below it in each dimension, (STILE+P-1)^3 values) and the tile's list are
 // loaded, every load issued before the first LDS store; (2) each listed particle is gathered

@@ -7,8 +7,8 @@ Here a setting holds the knobs of one ``ForceEngine`` evaluation:
   * ``build_lanes``: lanes per particle of the row build;
   * ``overlap``: how the pair kernel shares the device with the mesh branch ("fft": it waits for
     the spread and runs beside the transforms; "spread": it runs beside the whole mesh branch);
-  * ``spread_threads``, ``transform_threads``: workgroup sizes of the owner-computes spread and of
-    the plane passes.
+  * ``spread_threads``, ``transform_threads``: workgroup sizes of the owner-computes spread that
+    takes every particle reaching a tile (sparse meshes) and of the plane passes.
 ``tune`` measures candidate settings on the caller's own system (graph replays of the captured
 evaluation, as the bench times it) and applies the fastest; ``classify`` gives the reference's
 classes (size: N <= 30 000, <= 300 000, larger; neighbours rho 4 pi/3 rc^3 below 9.8, 36, 72, or

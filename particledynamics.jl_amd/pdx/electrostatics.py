@@ -244,7 +244,8 @@ class ElectrostaticsSolver:
         L.check(L.lib.pdx_mesh_spread_cells(self._h, nl.handle, L.stream_handle(stream)))
 
     def set_spread_threads(self, threads):
-        """Workgroup size of the owner-computes spreads (512, 1024 or 256)."""
+        """Workgroup size of the owner-computes spreads that take every particle reaching a tile
+        (sparse meshes; the dense meshes' own-particle form picks its own): 512, 1024 or 256."""
         L.check(L.lib.pdx_mesh_set_spread_threads(self._h, int(threads)))
         self.spread_threads = int(threads)
 
